@@ -1,0 +1,11 @@
+"""classicaldft_amd: MI355X-native FMT / DFT hot path of classicalDFT.
+
+Python front end over the C ABI in ``include/cdft.h`` (``libcdft.so``).  The
+library is required: importing the bindings raises if it is missing, there is
+no CPU fallback.
+"""
+from .cdft import (  # noqa: F401
+    CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT, FP64, MIXED, MODEL_NONE, ROSENFELD, RSLT, ESFMT,
+    WHITEBEAR_I, WHITEBEAR_II, DEVICE_PTRS, ACCUMULATE, CdftError, EtaTooLarge, System, fft_r2c, fft_c2r,
+    device_count, library_path,
+)

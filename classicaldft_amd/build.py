@@ -1,0 +1,67 @@
+"""Builds the native libraries of classicaldft_amd in-tree.
+
+* ``libcdft.so``   - HIP kernels + the C ABI of ``include/cdft.h`` (hipcc, gfx950)
+* ``libclassicaldft.so`` - the C++ drop-in API of ``include/classicaldft/`` on top of it
+
+Run ``python -m classicaldft_amd.build`` (or ``__graft_entry__.build()``).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+ARCH = os.environ.get("CDFT_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    # keep the compiler from emitting packed-FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_cdft():
+    out = os.path.join(HERE, "libcdft.so")
+    tmp = out + ".tmp"
+    run([hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp,
+         os.path.join(CSRC, "cdft_api.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host():
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp")) if os.path.isdir(HOST) else []
+    if not srcs:
+        return None
+    out = os.path.join(HERE, "libclassicaldft.so")
+    tmp = out + ".tmp"
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs,
+         "-L", HERE, "-lcdft", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    build_cdft()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,210 @@
+"""ctypes bindings of ``libcdft.so`` (C ABI declared in ``include/cdft.h``)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT = 0, 1, 2
+CDFT_BAD_ARG, CDFT_HIP_ERROR, CDFT_NOT_SUPPORTED = -1, -2, -3
+MODEL_NONE, ROSENFELD, RSLT, ESFMT, WHITEBEAR_I, WHITEBEAR_II = -1, 0, 1, 2, 3, 4
+FP64, MIXED = 0, 1
+DEVICE_PTRS, ACCUMULATE = 1, 2
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+
+
+class CdftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cdft error {code}: {msg}")
+        self.code = code
+
+
+class EtaTooLarge(CdftError):
+    """Mirror of the reference's Eta_Too_Large_Exception (include/FMT.h:649-663)."""
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [
+        ("Nx", _i), ("Ny", _i), ("Nz", _i),
+        ("dx", _d), ("dy", _d), ("dz", _d),
+        ("model", _i), ("A", _d), ("B", _d), ("hsd", _d),
+        ("precision", _i),
+        ("fmt_weights", _p),
+        ("ideal", _i), ("mu", _d), ("vext", _p),
+        ("mean_field", _i), ("lj_sigma", _d), ("lj_eps", _d), ("lj_rcut", _d), ("kT", _d),
+        ("mf_kernel", _p),
+    ]
+
+
+def library_path():
+    return os.path.join(_HERE, "libcdft.so")
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python -m classicaldft_amd.build`")
+        L = ctypes.CDLL(path)
+        L.cdft_last_error.restype = ctypes.c_char_p
+        L.cdft_device_count.argtypes = [ctypes.POINTER(_i)]
+        L.cdft_system_create.argtypes = [ctypes.POINTER(_Desc), ctypes.POINTER(_p)]
+        L.cdft_system_destroy.argtypes = [_p]
+        L.cdft_system_eval.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_fmt_eval.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_system_stream.argtypes = [_p, ctypes.POINTER(_p)]
+        L.cdft_system_a_vdw.argtypes = [_p, ctypes.POINTER(_d)]
+        L.cdft_system_get_weights.argtypes = [_p, _p]
+        L.cdft_system_get_mf_kernel.argtypes = [_p, _p]
+        L.cdft_fft_r2c.argtypes = [_i, _i, _i, _i, _p, _p]
+        L.cdft_fft_c2r.argtypes = [_i, _i, _i, _i, _p, _p]
+        L.cdft_system_set_timing.argtypes = [_p, _i]
+        L.cdft_system_get_timing.argtypes = [_p, ctypes.POINTER(_i), _p, ctypes.POINTER(ctypes.c_char_p)]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc == CDFT_OK:
+        return
+    msg = lib().cdft_last_error().decode(errors="replace")
+    if rc == CDFT_ETA_TOO_LARGE:
+        raise EtaTooLarge(rc, msg)
+    raise CdftError(rc, msg)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+def device_count():
+    n = _i(0)
+    _check(lib().cdft_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class System:
+    """One grid + FMT (+ ideal gas, LJ mean field, V_ext, mu) on the GPU."""
+
+    def __init__(self, shape, dx, hsd=1.0, model=WHITEBEAR_I, precision=FP64, weights=None, ideal=False, mu=0.0,
+                 vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0):
+        self.shape = tuple(int(n) for n in shape)
+        dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+        self.dx = dxs
+        self.dV = dxs[0] * dxs[1] * dxs[2]
+        self.N = int(np.prod(self.shape))
+        keep = []
+
+        def arr(a, n):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            if a.size != n:
+                raise ValueError(f"expected {n} values, got {a.size}")
+            keep.append(a)
+            return a.ctypes.data
+
+        d = _Desc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A),
+                  float(B), float(hsd), int(precision), arr(weights, 11 * self.N), int(bool(ideal)), float(mu),
+                  arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
+                  arr(mf_kernel, self.N))
+        h = _p()
+        _check(lib().cdft_system_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.model = model
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- evaluations on host arrays --
+    def eval(self, rho, dF=None, accumulate=False):
+        """DFT::calculateFreeEnergyAndDerivatives: returns (parts[5], dF)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        out = np.zeros(self.shape) if dF is None else dF
+        parts = np.zeros(5)
+        _check(lib().cdft_system_eval(self.h, _ptr(rho), _ptr(out), _ptr(parts), ACCUMULATE if accumulate else 0))
+        return parts, out
+
+    def fmt_eval(self, rho, dF=None, accumulate=False):
+        """FMT::calculateFreeEnergyAndDerivatives: returns (F_hs, dF)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        out = np.zeros(self.shape) if dF is None else dF
+        F = _d()
+        _check(lib().cdft_fmt_eval(self.h, _ptr(rho), _ptr(out), ctypes.byref(F), ACCUMULATE if accumulate else 0))
+        return F.value, out
+
+    # -- evaluations on device pointers (ints) --
+    def eval_device(self, rho_ptr, dF_ptr, parts=None, fmt_only=False, accumulate=False):
+        flags = DEVICE_PTRS | (ACCUMULATE if accumulate else 0)
+        if fmt_only:
+            F = _d()
+            _check(lib().cdft_fmt_eval(self.h, _p(rho_ptr), _p(dF_ptr), ctypes.byref(F), flags))
+            return F.value
+        p = np.zeros(5) if parts is None else parts
+        _check(lib().cdft_system_eval(self.h, _p(rho_ptr), _p(dF_ptr), _ptr(p), flags))
+        return p
+
+    @property
+    def stream(self):
+        s = _p()
+        _check(lib().cdft_system_stream(self.h, ctypes.byref(s)))
+        return s.value or 0
+
+    @property
+    def a_vdw(self):
+        a = _d()
+        _check(lib().cdft_system_a_vdw(self.h, ctypes.byref(a)))
+        return a.value
+
+    def device_weights(self):
+        w = np.zeros((11,) + self.shape)
+        _check(lib().cdft_system_get_weights(self.h, _ptr(w)))
+        return w
+
+    def device_mf_kernel(self):
+        w = np.zeros(self.shape)
+        _check(lib().cdft_system_get_mf_kernel(self.h, _ptr(w)))
+        return w
+
+    def set_timing(self, on=True):
+        _check(lib().cdft_system_set_timing(self.h, int(bool(on))))
+
+    def timing(self):
+        n = _i(0)
+        ms = np.zeros(16)
+        names = (ctypes.c_char_p * 16)()
+        _check(lib().cdft_system_get_timing(self.h, ctypes.byref(n), _ptr(ms), names))
+        return [(names[i].decode(), float(ms[i])) for i in range(n.value)]
+
+
+def fft_r2c(x, precision=FP64):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    Nx, Ny, Nz = x.shape
+    out = np.zeros((Nx, Ny, Nz // 2 + 1), dtype=np.complex128)
+    _check(lib().cdft_fft_r2c(Nx, Ny, Nz, precision, _ptr(x), _ptr(out)))
+    return out
+
+
+def fft_c2r(s, Nz, precision=FP64):
+    s = np.ascontiguousarray(s, dtype=np.complex128)
+    Nx, Ny, _ = s.shape
+    out = np.zeros((Nx, Ny, Nz))
+    _check(lib().cdft_fft_c2r(Nx, Ny, Nz, precision, _ptr(s), _ptr(out)))
+    return out

@@ -1,0 +1,728 @@
+// C ABI of the MI355X hot path (declared in include/cdft.h).
+//
+// A cdft_system owns: the resident weight spectra (one real number per k for
+// each of the m FMT weights and the mean-field kernel), the spectral work
+// arrays, the twiddle tables and a HIP stream.  cdft_system_eval runs the
+// fused pipeline documented in kernels.hpp; no C++ exception crosses the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cdft.h"
+#include "kernels.hpp"
+#include "weights.hpp"
+
+using namespace cdft;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct CdftError : std::runtime_error {
+  int code;
+  CdftError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                      \
+  do {                                                                                                    \
+    hipError_t e_ = (expr);                                                                               \
+    if (e_ != hipSuccess)                                                                                 \
+      throw CdftError(CDFT_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_) + " (line " +    \
+                                          std::to_string(__LINE__) + ")");                                \
+  } while (0)
+
+#define REQUIRE(cond, msg) \
+  do {                     \
+    if (!(cond)) throw CdftError(CDFT_BAD_ARG, msg); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t b) {
+    release();
+    if (b) HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename U>
+  U* as() const { return reinterpret_cast<U*>(p); }
+};
+
+bool smooth23(int n) {
+  if (n < 1) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  return n == 1;
+}
+
+Radices make_radices(int L) {
+  Radices r{};
+  r.L = L;
+  int m = L;
+  int k = 0;
+  while (m % 8 == 0) { r.r[k++] = 8; m /= 8; }
+  if (m % 4 == 0) { r.r[k++] = 4; m /= 4; }
+  if (m % 2 == 0) { r.r[k++] = 2; m /= 2; }
+  while (m % 3 == 0) { r.r[k++] = 3; m /= 3; }
+  if (m != 1) throw CdftError(CDFT_BAD_ARG, "transform length " + std::to_string(L) + " is not a product of 2 and 3");
+  r.npass = k;
+  return r;
+}
+
+// W_L^j = exp(-2 pi i j / L), j in [0, L), in extended precision.
+template <typename T>
+void upload_twiddles(DevBuf& buf, int L) {
+  std::vector<cplx<T>> h(L);
+  const long double tp = 6.283185307179586476925286766559005768L;
+  for (int j = 0; j < L; ++j) {
+    const long double a = -tp * (long double)j / (long double)L;
+    h[j] = {T(cosl(a)), T(sinl(a))};
+  }
+  buf.alloc(sizeof(cplx<T>) * L);
+  HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cplx<T>) * L, hipMemcpyHostToDevice));
+}
+
+template <typename T>
+struct Tables {
+  DevBuf x, y, m, n;  // Nx, Ny, Nz/2, Nz
+  void build(const Geo& g) {
+    upload_twiddles<T>(x, g.Nx);
+    upload_twiddles<T>(y, g.Ny);
+    upload_twiddles<T>(m, std::max(1, g.Nz / 2));
+    upload_twiddles<T>(n, g.Nz);
+  }
+};
+
+int div_up(long a, long b) { return int((a + b - 1) / b); }
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+}  // namespace
+
+struct cdft_system_s {
+  cdft_system_desc d{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geo g{};
+  long N = 0, Nkp = 0, nrows = 0;
+  int m = 0, nconv = 0;
+  unsigned odd_mask = 0x1Cu;  // arrays 2, 3, 4 (vector weights) have imaginary spectra
+  bool mixed = false;
+  Radices radX{}, radY{}, radM{};
+  Tables<double> twd;
+  Tables<float> twf;
+  DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
+  double a_vdw = 0.0;
+  int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
+  bool has_vext = false;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<const char*> ev_names;
+  int n_marks = 0;
+
+  ~cdft_system_s() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void mark(const char* name) {
+    if (!timing) return;
+    if (n_marks < (int)ev.size()) {
+      HIPCHK(hipEventRecord(ev[n_marks], stream));
+      ev_names[n_marks] = name;
+      ++n_marks;
+    }
+  }
+};
+
+namespace {
+
+template <typename T>
+const Tables<T>& tables(const cdft_system_s& s);
+template <>
+const Tables<double>& tables<double>(const cdft_system_s& s) { return s.twd; }
+template <>
+const Tables<float>& tables<float>(const cdft_system_s& s) { return s.twf; }
+
+// ------------------------------- transforms --------------------------------
+template <typename T, typename TI>
+void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride, cplx<T>* out, long out_stride, int narr) {
+  const Tables<T>& tw = tables<T>(s);
+  const int M = s.g.Nz / 2;
+  const int RB = s.RBrow;
+  dim3 grid(div_up(s.nrows, RB), narr);
+  hipLaunchKernelGGL((k_rows_r2c<T, TI, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
+                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T, typename TO>
+void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_stride, TO* out, long out_stride, int narr,
+                     double scale) {
+  const Tables<T>& tw = tables<T>(s);
+  const int M = s.g.Nz / 2;
+  const int RB = s.RBrow;
+  dim3 grid(div_up(s.nrows, RB), narr);
+  hipLaunchKernelGGL((k_rows_c2r<T, TO, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
+                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB,
+                     scale);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd) {
+  const Tables<T>& tw = tables<T>(s);
+  const Radices& rad = axis == 0 ? s.radX : s.radY;
+  const cplx<T>* t = (axis == 0 ? tw.x : tw.y).template as<cplx<T>>();
+  const int TW = s.TWcol;
+  dim3 grid(div_up(s.g.Zh, TW), axis == 0 ? s.g.Ny : s.g.Nx, narr);
+  const size_t smem = sizeof(cplx<T>) * rad.L * TW;
+  if (fwd)
+    hipLaunchKernelGGL((k_cols<T, 16, true>), grid, dim3(kThreads), smem, s.stream, data, stride, s.g, axis, rad, t, TW);
+  else
+    hipLaunchKernelGGL((k_cols<T, 16, false>), grid, dim3(kThreads), smem, s.stream, data, stride, s.g, axis, rad, t,
+                       TW);
+  HIPCHK(hipGetLastError());
+}
+
+// ------------------------------- set-up ------------------------------------
+void validate(const cdft_system_desc& d) {
+  REQUIRE(d.Nx > 0 && d.Ny > 0 && d.Nz > 0, "grid dimensions must be positive");
+  REQUIRE(d.Nz % 2 == 0, "Nz must be even");
+  REQUIRE(smooth23(d.Nx) && smooth23(d.Ny) && smooth23(d.Nz / 2), "Nx, Ny and Nz/2 must be products of 2 and 3");
+  REQUIRE(d.Nx <= 512 && d.Ny <= 512 && d.Nz <= 512, "grid dimensions are limited to 512");
+  REQUIRE(d.dx > 0 && d.dy > 0 && d.dz > 0, "lattice spacings must be positive");
+  REQUIRE(d.precision == CDFT_FP64 || d.precision == CDFT_MIXED, "unknown precision");
+  REQUIRE(d.model >= CDFT_MODEL_NONE && d.model <= CDFT_WHITEBEAR_II, "unknown FMT model");
+  if (d.model != CDFT_MODEL_NONE) {
+    REQUIRE(d.hsd > 0, "hsd must be positive");
+    // src/FMT_Species.cpp:321-322
+    REQUIRE(d.fmt_weights || d.hsd >= std::min(std::min(d.dx, d.dy), d.dz), "hsd is less than the lattice spacing");
+  }
+  if (d.mean_field) REQUIRE(d.mf_kernel || (d.kT > 0 && d.lj_sigma > 0 && d.lj_rcut > 0), "bad mean-field parameters");
+}
+
+template <typename T>
+void build_weight_spectra(cdft_system_s& s) {
+  const long N = s.N, Nkp = s.Nkp;
+  DevBuf S;  // one double-precision spectrum
+  S.alloc(sizeof(cplx<double>) * Nkp);
+  const int nb = div_up(Nkp, kThreads);
+  if (s.m > 0) {
+    DevBuf wreal;
+    wreal.alloc(sizeof(double) * 11 * N);
+    if (s.d.fmt_weights) {
+      HIPCHK(hipMemcpyAsync(wreal.p, s.d.fmt_weights, sizeof(double) * 11 * N, hipMemcpyHostToDevice, s.stream));
+    } else {
+      const double R = s.d.hsd / 2;
+      const int Sxm = 2 + int(R / s.d.dx), Sym = 2 + int(R / s.d.dy), Szm = 2 + int(R / s.d.dz);
+      REQUIRE(Sxm < 4 * s.g.Nx && Sym < 4 * s.g.Ny && Szm < 4 * s.g.Nz, "sphere is too large for the periodic box");
+      DevBuf cells;
+      const long ncell = (long)(Sxm + 1) * (Sym + 1) * (Szm + 1);
+      cells.alloc(sizeof(double) * 11 * ncell);
+      hipLaunchKernelGGL(k_fmt_cells, dim3(div_up(ncell, 64)), dim3(64), 0, s.stream, Sxm, Sym, Szm, s.d.dx, s.d.dy,
+                         s.d.dz, R, cells.as<double>());
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(k_fmt_gather, dim3(div_up(N, kThreads)), dim3(kThreads), 0, s.stream, s.g.Nx, s.g.Ny, s.g.Nz,
+                         Sxm, Sym, Szm, cells.as<double>(), wreal.as<double>());
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(s.stream));
+    }
+    for (int a = 0; a < s.m; ++a) {
+      launch_rows_r2c<double, double>(s, wreal.as<double>() + a * N, 0, S.as<cplx<double>>(), 0, 1);
+      launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 1, true);
+      launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 0, true);
+      hipLaunchKernelGGL((k_extract_weight<T>), dim3(nb), dim3(kThreads), 0, s.stream, S.as<cplx<double>>(),
+                         s.W.as<T>() + a * Nkp, Nkp, (int)((s.odd_mask >> a) & 1u), 1.0 / (double)N);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s.stream));
+  }
+  if (s.d.mean_field) {
+    DevBuf w;
+    w.alloc(sizeof(double) * N);
+    if (s.d.mf_kernel) {
+      HIPCHK(hipMemcpyAsync(w.p, s.d.mf_kernel, sizeof(double) * N, hipMemcpyHostToDevice, s.stream));
+    } else {
+      LJParams v;
+      v.sigma = s.d.lj_sigma;
+      v.eps = s.d.lj_eps;
+      v.rcut = s.d.lj_rcut;
+      auto vr = [&](double r) {
+        const double y = v.sigma / r, y3 = y * y * y, y6 = y3 * y3;
+        return 4 * v.eps * (y6 * y6 - y6);
+      };
+      v.shift = v.rcut < 0 ? 0.0 : vr(v.rcut);  // include/Potential1.h:142-145
+      v.rmin = std::pow(2.0, 1.0 / 6.0) * v.sigma;
+      v.vmin = vr(v.rmin) - v.shift;
+      const int lx = 2 + int(v.rcut / s.d.dx), ly = 2 + int(v.rcut / s.d.dy), lz = 2 + int(v.rcut / s.d.dz);
+      hipLaunchKernelGGL(k_mf_kernel, dim3(div_up(N, kThreads)), dim3(kThreads), 0, s.stream, s.g.Nx, s.g.Ny, s.g.Nz,
+                         s.d.dx, s.d.dy, s.d.dz, lx, ly, lz, v, s.d.kT, w.as<double>());
+      HIPCHK(hipGetLastError());
+    }
+    // a_vdw = sum_R w_att(R) (src/Interaction.cpp:290-296, 42-46)
+    const int nblk = 256;
+    hipLaunchKernelGGL(k_sum_real, dim3(nblk), dim3(kThreads), 0, s.stream, w.as<double>(), N,
+                       s.partials.as<double2>());
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials.as<double2>(), (long)nblk, 1,
+                       s.results.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(&s.a_vdw, s.results.p, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    launch_rows_r2c<double, double>(s, w.as<double>(), 0, S.as<cplx<double>>(), 0, 1);
+    launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 1, true);
+    launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 0, true);
+    // v = (dV/N) c2r(rho_k w_k): fold dV/N into the stored spectrum
+    const double dV = s.d.dx * s.d.dy * s.d.dz;
+    hipLaunchKernelGGL((k_extract_weight<T>), dim3(nb), dim3(kThreads), 0, s.stream, S.as<cplx<double>>(),
+                       s.W.as<T>() + s.m * Nkp, Nkp, 0, dV / (double)N);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s.stream));
+  }
+}
+
+void create_system(const cdft_system_desc* desc, cdft_system* out) {
+  REQUIRE(desc && out, "null argument");
+  validate(*desc);
+  auto s = std::make_unique<cdft_system_s>();
+  s->d = *desc;
+  HIPCHK(hipGetDevice(&s->device));
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  Geo& g = s->g;
+  g.Nx = desc->Nx;
+  g.Ny = desc->Ny;
+  g.Nz = desc->Nz;
+  g.Zh = desc->Nz / 2 + 1;
+  g.Zp = (g.Zh + 7) / 8 * 8;
+  s->N = (long)g.Nx * g.Ny * g.Nz;
+  s->Nkp = (long)g.Nx * g.Ny * g.Zp;
+  s->nrows = (long)g.Nx * g.Ny;
+  s->mixed = desc->precision == CDFT_MIXED;
+  s->m = desc->model == CDFT_MODEL_NONE ? 0 : (desc->model >= CDFT_ESFMT ? 11 : 5);
+  s->nconv = s->m + (desc->mean_field ? 1 : 0);
+  s->radX = make_radices(g.Nx);
+  s->radY = make_radices(g.Ny);
+  s->radM = make_radices(std::max(1, g.Nz / 2));
+  const int M = g.Nz / 2;
+  s->RBrow = std::max(1, 1024 / std::max(1, M));
+  s->RBpw = std::max(1, 256 / std::max(1, M));
+  const int Lmax = std::max(g.Nx, g.Ny);
+  s->TWcol = std::max(1, std::min(16, 4096 / Lmax));
+  const int esz = s->mixed ? 8 : 16;
+  s->TWconv = std::max(1, std::min(16, (65536 / (2 * esz)) / Lmax));
+  s->twd.build(g);
+  if (s->mixed) s->twf.build(g);
+  const size_t cs = s->mixed ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const size_t rs = s->mixed ? sizeof(float) : sizeof(double);
+  s->A.alloc(cs * s->Nkp);
+  if (s->nconv) {
+    s->B.alloc(cs * s->Nkp * s->nconv);
+    s->W.alloc(rs * s->Nkp * s->nconv);
+  }
+  if (desc->mean_field) s->vmf.alloc(sizeof(double) * s->N);
+  s->rho.alloc(sizeof(double) * s->N);
+  s->dF.alloc(sizeof(double) * s->N);
+  const long maxblk = std::max<long>(std::max<long>(div_up(s->nrows, s->RBpw), div_up(s->nrows, s->RBrow)), 256);
+  s->partials.alloc(sizeof(double2) * maxblk);
+  s->partials2.alloc(sizeof(double2) * maxblk * 4);
+  s->flag.alloc(sizeof(int));
+  s->results.alloc(sizeof(double) * 8);
+  if (desc->vext) {
+    s->vext.alloc(sizeof(double) * s->N);
+    HIPCHK(hipMemcpy(s->vext.p, desc->vext, sizeof(double) * s->N, hipMemcpyHostToDevice));
+    s->has_vext = true;
+  }
+  if (s->mixed) build_weight_spectra<float>(*s);
+  else build_weight_spectra<double>(*s);
+  // host pointers of the descriptor are not retained
+  s->d.fmt_weights = nullptr;
+  s->d.vext = nullptr;
+  s->d.mf_kernel = nullptr;
+  s->ev.resize(16);
+  s->ev_names.resize(16);
+  for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+  *out = s.release();
+}
+
+template <typename T, int MODEL>
+void launch_pointwise(cdft_system_s& s, int nconv) {
+  const Tables<T>& tw = tables<T>(s);
+  const int M = s.g.Nz / 2;
+  const int RB = s.RBpw;
+  PhiParams pp{s.d.hsd, s.d.A, s.d.B};
+  if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
+  const size_t smem = sizeof(cplx<T>) * nconv * RB * M;
+  hipLaunchKernelGGL((k_z_pointwise<T, MODEL, 16>), dim3(div_up(s.nrows, RB)), dim3(kThreads), smem, s.stream,
+                     s.B.as<cplx<T>>(), s.Nkp, s.m, nconv, s.vmf.as<double>(), s.g, s.radM, tw.m.template as<cplx<T>>(),
+                     tw.n.template as<cplx<T>>(), RB, pp, s.partials.as<double2>(), s.flag.as<int>());
+  HIPCHK(hipGetLastError());
+}
+
+// The full evaluation on device pointers.  with_dft = false: FMT only.
+template <typename T>
+void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft, bool accumulate) {
+  s.n_marks = 0;
+  s.mark("start");
+  const long Nkp = s.Nkp;
+  const int nconv = with_dft ? s.nconv : s.m;
+  const bool mf = with_dft && s.d.mean_field;
+  cplx<T>* A = s.A.as<cplx<T>>();
+  cplx<T>* B = s.B.as<cplx<T>>();
+  const Tables<T>& tw = tables<T>(s);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s.stream, s.flag.as<int>());
+  if (nconv > 0) {
+    launch_rows_r2c<T, double>(s, rho, 0, A, 0, 1);
+    s.mark("z_r2c");
+    launch_cols<T>(s, A, 0, 1, 1, true);
+    s.mark("y_fwd");
+    {
+      const int TW = s.TWconv;
+      dim3 grid(div_up(s.g.Zh, TW), s.g.Ny);
+      hipLaunchKernelGGL((k_conv_fwd<T, 16>), grid, dim3(kThreads), 2 * sizeof(cplx<T>) * s.g.Nx * TW, s.stream, A, B,
+                         Nkp, s.W.as<T>(), nconv, s.odd_mask, s.g, s.radX, tw.x.template as<cplx<T>>(), TW);
+      HIPCHK(hipGetLastError());
+    }
+    s.mark("x_conv_fwd");
+    launch_cols<T>(s, B, Nkp, nconv, 1, false);
+    s.mark("y_inv");
+    if (s.m > 0) {
+      switch (s.d.model) {
+        case CDFT_ROSENFELD: launch_pointwise<T, M_ROSENFELD>(s, nconv); break;
+        case CDFT_RSLT: launch_pointwise<T, M_RSLT>(s, nconv); break;
+        case CDFT_ESFMT: launch_pointwise<T, M_ESFMT>(s, nconv); break;
+        case CDFT_WHITEBEAR_I: launch_pointwise<T, M_WBI>(s, nconv); break;
+        default: launch_pointwise<T, M_WBII>(s, nconv); break;
+      }
+      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials.as<double2>(),
+                         (long)div_up(s.nrows, s.RBpw), 1, s.results.as<double>());
+      s.mark("z_pointwise");
+      launch_cols<T>(s, B, Nkp, s.m, 1, true);
+      s.mark("y_fwd2");
+      {
+        const int TW = s.TWconv;
+        dim3 grid(div_up(s.g.Zh, TW), s.g.Ny);
+        hipLaunchKernelGGL((k_conv_bwd<T, 16>), grid, dim3(kThreads), 2 * sizeof(cplx<T>) * s.g.Nx * TW, s.stream, B,
+                           A, Nkp, s.W.as<T>(), s.m, s.odd_mask, s.g, s.radX, tw.x.template as<cplx<T>>(), TW);
+        HIPCHK(hipGetLastError());
+      }
+      s.mark("x_conv_bwd");
+      launch_cols<T>(s, A, 0, 1, 1, false);
+      s.mark("y_inv2");
+    } else {
+      // mean field only: inverse z of the single convolved array -> vmf
+      launch_rows_c2r<T, double>(s, B, 0, s.vmf.as<double>(), 0, 1, 1.0);
+      s.mark("z_c2r_mf");
+    }
+  }
+  FinalParams fp;
+  fp.dV = s.d.dx * s.d.dy * s.d.dz;
+  fp.mu = with_dft ? s.d.mu : 0.0;
+  fp.ideal = with_dft ? s.d.ideal : 0;
+  fp.fmt = s.m > 0;
+  fp.mf = mf;
+  fp.ext = with_dft && s.has_vext;
+  fp.accumulate = accumulate;
+  {
+    const int M = s.g.Nz / 2;
+    const int RB = s.RBrow;
+    hipLaunchKernelGGL((k_z_final<T, 4>), dim3(div_up(s.nrows, RB)), dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream,
+                       A, rho, s.vext.as<double>(), s.vmf.as<double>(), dF, s.g, s.radM, tw.m.template as<cplx<T>>(),
+                       tw.n.template as<cplx<T>>(), RB, fp, s.partials2.as<double2>());
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials2.as<double2>(),
+                       (long)div_up(s.nrows, RB), 4, s.results.as<double>() + 1);
+    HIPCHK(hipGetLastError());
+  }
+  s.mark("z_final");
+}
+
+// Two-sum based accurate addition of a few terms.
+double csum(std::initializer_list<double> v) {
+  double s = 0.0, c = 0.0;
+  for (double x : v) {
+    const double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  return s + c;
+}
+
+int run_eval(cdft_system s, const double* rho, double* dF, double* parts, int flags, bool with_dft) {
+  REQUIRE(s && rho && dF, "null argument");
+  const bool dev = flags & CDFT_DEVICE_PTRS;
+  const bool acc = flags & CDFT_ACCUMULATE;
+  const size_t bytes = sizeof(double) * s->N;
+  const double* rho_d = rho;
+  double* dF_d = dF;
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(s->rho.p, rho, bytes, hipMemcpyHostToDevice, s->stream));
+    if (acc) HIPCHK(hipMemcpyAsync(s->dF.p, dF, bytes, hipMemcpyHostToDevice, s->stream));
+    rho_d = s->rho.as<double>();
+    dF_d = s->dF.as<double>();
+  }
+  if (s->mixed) eval_device<float>(*s, rho_d, dF_d, with_dft, acc);
+  else eval_device<double>(*s, rho_d, dF_d, with_dft, acc);
+  double res[5];
+  int fl = 0;
+  HIPCHK(hipMemcpyAsync(res, s->results.p, sizeof(double) * 5, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(&fl, s->flag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  if (!dev) HIPCHK(hipMemcpyAsync(dF, s->dF.p, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (fl) {
+    g_err = "Eta too large in FMT::dPHI";
+    return CDFT_ETA_TOO_LARGE;
+  }
+  const double dV = s->d.dx * s->d.dy * s->d.dz;
+  const double F_hs = s->m > 0 ? res[0] * dV : 0.0;
+  const double F_id = (with_dft && s->d.ideal) ? res[1] : 0.0;
+  const double F_mf = (with_dft && s->d.mean_field) ? 0.5 * res[2] : 0.0;
+  const double mu = with_dft ? s->d.mu : 0.0;
+  const double F_ext = (with_dft && s->has_vext ? res[3] * dV : 0.0) - res[4] * dV * mu;
+  if (parts) {
+    parts[0] = csum({F_id, F_hs, F_mf, F_ext});
+    parts[1] = F_id;
+    parts[2] = F_hs;
+    parts[3] = F_mf;
+    parts[4] = F_ext;
+  }
+  return CDFT_OK;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const CdftError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return CDFT_HIP_ERROR;
+  }
+}
+
+// Standalone transform helper (a system with no functional terms).
+int standalone_fft(int Nx, int Ny, int Nz, int precision, const double* in, double* out, bool r2c) {
+  cdft_system_desc d{};
+  d.Nx = Nx;
+  d.Ny = Ny;
+  d.Nz = Nz;
+  d.dx = d.dy = d.dz = 1.0;
+  d.model = CDFT_MODEL_NONE;
+  d.precision = precision;
+  cdft_system s = nullptr;
+  create_system(&d, &s);
+  std::unique_ptr<cdft_system_s> guard(s);
+  const long N = s->N;
+  const int Zh = s->g.Zh;
+  const long Nk = (long)Nx * Ny * Zh;
+  if (r2c) {
+    HIPCHK(hipMemcpy(s->rho.p, in, sizeof(double) * N, hipMemcpyHostToDevice));
+    std::vector<double> host(2 * s->Nkp);
+    if (s->mixed) {
+      launch_rows_r2c<float, double>(*s, s->rho.as<double>(), 0, s->A.as<cplx<float>>(), 0, 1);
+      launch_cols<float>(*s, s->A.as<cplx<float>>(), 0, 1, 1, true);
+      launch_cols<float>(*s, s->A.as<cplx<float>>(), 0, 1, 0, true);
+      std::vector<float> hf(2 * s->Nkp);
+      HIPCHK(hipMemcpyAsync(hf.data(), s->A.p, sizeof(float) * 2 * s->Nkp, hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+      for (size_t i = 0; i < hf.size(); ++i) host[i] = hf[i];
+    } else {
+      launch_rows_r2c<double, double>(*s, s->rho.as<double>(), 0, s->A.as<cplx<double>>(), 0, 1);
+      launch_cols<double>(*s, s->A.as<cplx<double>>(), 0, 1, 1, true);
+      launch_cols<double>(*s, s->A.as<cplx<double>>(), 0, 1, 0, true);
+      HIPCHK(hipMemcpyAsync(host.data(), s->A.p, sizeof(double) * 2 * s->Nkp, hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+    }
+    for (long r = 0; r < (long)Nx * Ny; ++r)
+      std::memcpy(out + 2 * r * Zh, host.data() + 2 * r * s->g.Zp, sizeof(double) * 2 * Zh);
+    (void)Nk;
+  } else {
+    std::vector<double> host(2 * s->Nkp, 0.0);
+    for (long r = 0; r < (long)Nx * Ny; ++r)
+      std::memcpy(host.data() + 2 * r * s->g.Zp, in + 2 * r * Zh, sizeof(double) * 2 * Zh);
+    if (s->mixed) {
+      std::vector<float> hf(host.begin(), host.end());
+      HIPCHK(hipMemcpy(s->A.p, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice));
+      launch_cols<float>(*s, s->A.as<cplx<float>>(), 0, 1, 0, false);
+      launch_cols<float>(*s, s->A.as<cplx<float>>(), 0, 1, 1, false);
+      launch_rows_c2r<float, double>(*s, s->A.as<cplx<float>>(), 0, s->dF.as<double>(), 0, 1, 1.0);
+    } else {
+      HIPCHK(hipMemcpy(s->A.p, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice));
+      launch_cols<double>(*s, s->A.as<cplx<double>>(), 0, 1, 0, false);
+      launch_cols<double>(*s, s->A.as<cplx<double>>(), 0, 1, 1, false);
+      launch_rows_c2r<double, double>(*s, s->A.as<cplx<double>>(), 0, s->dF.as<double>(), 0, 1, 1.0);
+    }
+    HIPCHK(hipMemcpyAsync(out, s->dF.p, sizeof(double) * N, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return CDFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cdft_last_error(void) { return g_err.c_str(); }
+
+int cdft_device_count(int* n) {
+  return guarded([&] {
+    REQUIRE(n, "null argument");
+    HIPCHK(hipGetDeviceCount(n));
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_create(const cdft_system_desc* desc, cdft_system* out) {
+  return guarded([&] {
+    create_system(desc, out);
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_destroy(cdft_system sys) {
+  return guarded([&] {
+    if (sys) {
+      (void)hipStreamSynchronize(sys->stream);
+      delete sys;
+    }
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_eval(cdft_system sys, const double* rho, double* dF, double* parts, int flags) {
+  return guarded([&] { return run_eval(sys, rho, dF, parts, flags, true); });
+}
+
+int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, int flags) {
+  return guarded([&] {
+    REQUIRE(sys && sys->m > 0, "system has no FMT functional");
+    double parts[5];
+    const int rc = run_eval(sys, rho, dF, parts, flags, false);
+    if (rc == CDFT_OK && F_hs) *F_hs = parts[2];
+    return rc;
+  });
+}
+
+int cdft_system_stream(cdft_system sys, void** hip_stream) {
+  return guarded([&] {
+    REQUIRE(sys && hip_stream, "null argument");
+    *hip_stream = (void*)sys->stream;
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_a_vdw(cdft_system sys, double* a) {
+  return guarded([&] {
+    REQUIRE(sys && a, "null argument");
+    *a = sys->a_vdw;
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_get_weights(cdft_system sys, double* w11) {
+  return guarded([&] {
+    REQUIRE(sys && w11, "null argument");
+    REQUIRE(sys->m > 0, "system has no FMT functional");
+    const cdft_system_desc& d = sys->d;
+    const double R = d.hsd / 2;
+    const int Sxm = 2 + int(R / d.dx), Sym = 2 + int(R / d.dy), Szm = 2 + int(R / d.dz);
+    REQUIRE(Sxm < 4 * sys->g.Nx && Sym < 4 * sys->g.Ny && Szm < 4 * sys->g.Nz, "sphere is too large for the box");
+    DevBuf cells, wreal;
+    const long ncell = (long)(Sxm + 1) * (Sym + 1) * (Szm + 1);
+    cells.alloc(sizeof(double) * 11 * ncell);
+    wreal.alloc(sizeof(double) * 11 * sys->N);
+    hipLaunchKernelGGL(k_fmt_cells, dim3(div_up(ncell, 64)), dim3(64), 0, sys->stream, Sxm, Sym, Szm, d.dx, d.dy, d.dz,
+                       R, cells.as<double>());
+    hipLaunchKernelGGL(k_fmt_gather, dim3(div_up(sys->N, kThreads)), dim3(kThreads), 0, sys->stream, sys->g.Nx,
+                       sys->g.Ny, sys->g.Nz, Sxm, Sym, Szm, cells.as<double>(), wreal.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(w11, wreal.p, sizeof(double) * 11 * sys->N, hipMemcpyDeviceToHost, sys->stream));
+    HIPCHK(hipStreamSynchronize(sys->stream));
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_get_mf_kernel(cdft_system sys, double* w) {
+  return guarded([&] {
+    REQUIRE(sys && w, "null argument");
+    REQUIRE(sys->d.mean_field, "system has no mean field");
+    const cdft_system_desc& d = sys->d;
+    LJParams v;
+    v.sigma = d.lj_sigma;
+    v.eps = d.lj_eps;
+    v.rcut = d.lj_rcut;
+    auto vr = [&](double r) {
+      const double y = v.sigma / r, y3 = y * y * y, y6 = y3 * y3;
+      return 4 * v.eps * (y6 * y6 - y6);
+    };
+    v.shift = v.rcut < 0 ? 0.0 : vr(v.rcut);
+    v.rmin = std::pow(2.0, 1.0 / 6.0) * v.sigma;
+    v.vmin = vr(v.rmin) - v.shift;
+    const int lx = 2 + int(v.rcut / d.dx), ly = 2 + int(v.rcut / d.dy), lz = 2 + int(v.rcut / d.dz);
+    DevBuf buf;
+    buf.alloc(sizeof(double) * sys->N);
+    hipLaunchKernelGGL(k_mf_kernel, dim3(div_up(sys->N, kThreads)), dim3(kThreads), 0, sys->stream, sys->g.Nx,
+                       sys->g.Ny, sys->g.Nz, d.dx, d.dy, d.dz, lx, ly, lz, v, d.kT, buf.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(w, buf.p, sizeof(double) * sys->N, hipMemcpyDeviceToHost, sys->stream));
+    HIPCHK(hipStreamSynchronize(sys->stream));
+    return CDFT_OK;
+  });
+}
+
+int cdft_fft_r2c(int Nx, int Ny, int Nz, int precision, const double* in, double* out) {
+  return guarded([&] {
+    REQUIRE(in && out, "null argument");
+    return standalone_fft(Nx, Ny, Nz, precision, in, out, true);
+  });
+}
+
+int cdft_fft_c2r(int Nx, int Ny, int Nz, int precision, const double* in, double* out) {
+  return guarded([&] {
+    REQUIRE(in && out, "null argument");
+    return standalone_fft(Nx, Ny, Nz, precision, in, out, false);
+  });
+}
+
+int cdft_system_set_timing(cdft_system sys, int enable) {
+  return guarded([&] {
+    REQUIRE(sys, "null argument");
+    sys->timing = enable != 0;
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_get_timing(cdft_system sys, int* n, double* ms, const char** names) {
+  return guarded([&] {
+    REQUIRE(sys && n, "null argument");
+    HIPCHK(hipStreamSynchronize(sys->stream));
+    const int k = std::max(0, sys->n_marks - 1);
+    for (int i = 0; i < k; ++i) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, sys->ev[i], sys->ev[i + 1]));
+      if (ms) ms[i] = t;
+      if (names) names[i] = sys->ev_names[i + 1];
+    }
+    *n = k;
+    return CDFT_OK;
+  });
+}
+
+}  // extern "C"

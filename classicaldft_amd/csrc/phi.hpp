@@ -1,0 +1,160 @@
+// Pointwise FMT free-energy density Phi(n_alpha) and its derivatives, in FP64,
+// for the five reference models, collapsed to the 11 stored dPhi arrays.
+//
+// Reference: FMT::calculate_Phi / calculate_dPhi_wrt_fundamental_measures
+// (src/FMT.cpp:114-176), the model hooks in include/FMT.h:139-643, the measure
+// construction FMT::getWeightedDensities (src/FMT.cpp:78-111) and the collapse
+// FMT_Species::set_fundamental_measure_derivatives (include/Species.h:310-329).
+//
+// Array order (include/Species.h:351-360): 0 eta, 1 s, 2..4 v, 5 Txx, 6 Txy,
+// 7 Txz, 8 Tyy, 9 Tyz, 10 Tzz.  The tensor is symmetric by construction, so
+// T.T, v.T and T.v are evaluated from the six independent entries.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cdft {
+
+enum { M_ROSENFELD = 0, M_RSLT = 1, M_ESFMT = 2, M_WBI = 3, M_WBII = 4 };
+
+struct PhiParams {
+  double hsd;
+  double A, B;  // esFMT coefficients (WB I/II: 1, -1)
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+// eta-dependent cofactors f1, f2, f3 and first derivatives.
+template <int MODEL>
+__device__ __forceinline__ void fcoef(double x, double& f1, double& f1p, double& f2, double& f2p, double& f3,
+                                      double& f3p) {
+  const double om = 1.0 - x;
+  f1 = log(om);  // FMT.h:153-156
+  f1p = -1.0 / om;
+  if constexpr (MODEL == M_WBII) {  // FMT.h:588-632
+    if (x < 1e-12) {
+      f2 = 1 + x * (1 + x * ((10.0 / 9) + (7.0 / 6) * x));
+      f2p = 1 + x * ((20.0 / 9) + 3.5 * x);
+      f3 = 1.5 + (7.0 / 3) * x + 3.25 * x * x + 4.2 * x * x * x;
+      f3p = (7.0 / 3) + 7.5 * x + 12.6 * x * x;
+    } else {
+      const double L = f1;
+      f2 = (1.0 / 3.0) + (4.0 / 3.0) * (1.0 / om) + (2.0 / (3.0 * x)) * L;
+      f2p = (2 * (3 * x - 1) / (3 * x * om * om)) - (2.0 / (3.0 * x * x)) * L;
+      f3 = -((1.0 - 3 * x + x * x) / (x * om * om)) - (1.0 / (x * x)) * L;
+      f3p = ((2 - 5 * x + 6 * x * x - x * x * x) / (x * x * om * om * om)) + (2.0 / (x * x * x)) * L;
+    }
+  } else {
+    const double f = 1.0 / om;  // FMT.h:168-183
+    f2 = f;
+    f2p = f * f;
+    if constexpr (MODEL == M_RSLT || MODEL == M_WBI) {  // FMT.h:267-281, 536-550
+      if (x < 1e-12) {
+        f3 = 1.5 + (8.0 / 3) * x + 3.75 * x * x + 4.8 * x * x * x;
+        f3p = (8.0 / 3) + 7.5 * x + 14.4 * x * x;
+      } else {
+        f3 = (1.0 / (x * om * om)) + (f1 / (x * x));
+        f3p = ((-2 + 5 * x - x * x) / (x * x * om * om * om)) - (2 / (x * x * x)) * f1;
+      }
+    } else {  // Rosenfeld, esFMT: FMT.h:185-195
+      f3 = f * f;
+      f3p = 2 * f * f * f;
+    }
+  }
+}
+
+// Returns Phi at one site; writes the 11 collapsed derivatives to out[].
+// n[] holds the 11 weighted densities of the site (tensor entries ignored for
+// Rosenfeld / RSLT, whose tensor arrays are never convolved).
+template <int MODEL>
+__device__ __forceinline__ double phi_site(const double* n, const PhiParams& pp, double* out) {
+  constexpr bool TENSOR = MODEL >= M_ESFMT;
+  const double d = pp.hsd;
+  const double eta = n[0];
+  const double s0 = n[1] / (d * d), s1 = n[1] / d, s2 = n[1];
+  const double v2[3] = {n[2], n[3], n[4]};
+  const double v1[3] = {n[2] / d, n[3] / d, n[4] / d};
+  const double v1v2 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+  const double v2v2 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+
+  double f1, f1p, f2, f2p, f3, f3p;
+  fcoef<MODEL>(eta, f1, f1p, f2, f2p, f3, f3p);
+
+  double P3, dS2, dV2[3];
+  double dT[6] = {0, 0, 0, 0, 0, 0};  // xx, xy, xz, yy, yz, zz
+  if constexpr (MODEL == M_ROSENFELD) {  // FMT.h:202-240
+    P3 = (1.0 / (24 * kPi)) * (s2 * s2 * s2 - 3 * s2 * v2v2);
+    dS2 = (1.0 / (24 * kPi)) * (3 * s2 * s2 - 3 * v2v2);
+    for (int k = 0; k < 3; ++k) dV2[k] = (1.0 / (24 * kPi)) * (-6 * s2 * v2[k]);
+  } else if constexpr (MODEL == M_RSLT) {  // FMT.h:293-358
+    if (s2 < 1e-20) {
+      P3 = 0.0;
+      dS2 = 0.0;
+      dV2[0] = dV2[1] = dV2[2] = 0.0;
+    } else {
+      const double psi = v2v2 / (s2 * s2);
+      const double q = 1 - psi;
+      P3 = (1.0 / (36 * kPi)) * s2 * s2 * s2 * q * q * q;
+      dS2 = (1.0 / (12 * kPi)) * s2 * s2 * (1 + psi) * q * q;
+      for (int k = 0; k < 3; ++k) dV2[k] = -(1.0 / (6 * kPi)) * s2 * v2[k] * q * q;
+    }
+  } else {  // esFMT family: FMT.h:396-455
+    const double A = pp.A, B = pp.B;
+    const double Txx = n[5], Txy = n[6], Txz = n[7], Tyy = n[8], Tyz = n[9], Tzz = n[10];
+    // TT = T.T (symmetric)
+    const double Qxx = Txx * Txx + Txy * Txy + Txz * Txz;
+    const double Qxy = Txx * Txy + Txy * Tyy + Txz * Tyz;
+    const double Qxz = Txx * Txz + Txy * Tyz + Txz * Tzz;
+    const double Qyy = Txy * Txy + Tyy * Tyy + Tyz * Tyz;
+    const double Qyz = Txy * Txz + Tyy * Tyz + Tyz * Tzz;
+    const double Qzz = Txz * Txz + Tyz * Tyz + Tzz * Tzz;
+    const double vT0 = v2[0] * Txx + v2[1] * Txy + v2[2] * Txz;
+    const double vT1 = v2[0] * Txy + v2[1] * Tyy + v2[2] * Tyz;
+    const double vT2 = v2[0] * Txz + v2[1] * Tyz + v2[2] * Tzz;
+    const double vTv = v2[0] * vT0 + v2[1] * vT1 + v2[2] * vT2;
+    const double T2 = Qxx + Qyy + Qzz;
+    const double T3 = Qxx * Txx + Qyy * Tyy + Qzz * Tzz + 2 * (Qxy * Txy + Qxz * Txz + Qyz * Tyz);
+    const double ca = A / (24 * kPi), cb = B / (24 * kPi);
+    P3 = ca * (s2 * s2 * s2 - 3 * s2 * v2v2 + 3 * vTv - T3) + cb * (s2 * s2 * s2 - 3 * s2 * T2 + 2 * T3);
+    dS2 = ca * (3 * s2 * s2 - 3 * v2v2) + cb * (3 * s2 * s2 - 3 * T2);
+    dV2[0] = ca * (-6 * s2 * v2[0] + 6 * vT0);
+    dV2[1] = ca * (-6 * s2 * v2[1] + 6 * vT1);
+    dV2[2] = ca * (-6 * s2 * v2[2] + 6 * vT2);
+    const double ea = A / (8 * kPi), eb = B / (4 * kPi);
+    dT[0] = ea * (v2[0] * v2[0] - Qxx) + eb * (-s2 * Txx + Qxx);
+    dT[1] = ea * (v2[0] * v2[1] - Qxy) + eb * (-s2 * Txy + Qxy);
+    dT[2] = ea * (v2[0] * v2[2] - Qxz) + eb * (-s2 * Txz + Qxz);
+    dT[3] = ea * (v2[1] * v2[1] - Qyy) + eb * (-s2 * Tyy + Qyy);
+    dT[4] = ea * (v2[1] * v2[2] - Qyz) + eb * (-s2 * Tyz + Qyz);
+    dT[5] = ea * (v2[2] * v2[2] - Qzz) + eb * (-s2 * Tzz + Qzz);
+  }
+
+  const double c1 = 1.0 / kPi, c2 = 1.0 / (2 * kPi);
+  const double phi = -c1 * s0 * f1 + c2 * (s1 * s2 - v1v2) * f2 + P3 * f3;
+
+  const double dEta = -c1 * s0 * f1p + c2 * (s1 * s2 - v1v2) * f2p + P3 * f3p;
+  const double dS0 = -c1 * f1;
+  const double dS1 = c2 * s2 * f2;
+  const double dS2t = c2 * s1 * f2 + dS2 * f3;
+  out[0] = dEta;
+  out[1] = dS0 / (d * d) + dS1 / d + dS2t;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double dv1 = -c2 * v2[k] * f2;
+    const double dv2 = -c2 * v1[k] * f2 + dV2[k] * f3;
+    out[2 + k] = -(dv2 + dv1 / d);  // parity sign, Species.h:324
+  }
+  if constexpr (TENSOR) {
+    out[5] = dT[0] * f3;
+    out[6] = 2 * dT[1] * f3;  // off-diagonals counted twice, Species.h:327
+    out[7] = 2 * dT[2] * f3;
+    out[8] = dT[3] * f3;
+    out[9] = 2 * dT[4] * f3;
+    out[10] = dT[5] * f3;
+  } else {
+#pragma unroll
+    for (int a = 5; a < 11; ++a) out[a] = 0.0;
+  }
+  return phi;
+}
+
+}  // namespace cdft

@@ -1,0 +1,116 @@
+/*
+ * cdft.h - C ABI of the MI355X-native classicalDFT hot path.
+ *
+ * One "system" is one periodic grid with one species: the FMT hard-sphere
+ * functional plus the optional ideal-gas, LJ mean-field (quadratic-force
+ * interpolation) and external-field / chemical-potential terms.  A system
+ * owns its device memory, its HIP stream and its resident weight spectra.
+ *
+ * All entry points return CDFT_OK (0) or a negative/positive status; the last
+ * error message of the calling thread is available from cdft_last_error().
+ * No C++ exception crosses this boundary.  Arrays are row-major with z
+ * fastest: pos = z + Nz*(y + Ny*x), as in the reference Lattice::pos
+ * (include/Lattice.h:120-126).
+ *
+ * Thread safety: one system per host thread at a time; distinct systems are
+ * independent.
+ */
+#ifndef CDFT_H
+#define CDFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cdft_status {
+  CDFT_OK = 0,
+  CDFT_ETA_TOO_LARGE = 1, /* eta > 1 somewhere: Eta_Too_Large_Exception, include/FMT.h:649-663 */
+  CDFT_UPHILL_LIMIT = 2,  /* FIRE2: too many uphill steps, src/Minimizer.cpp:207 */
+  CDFT_BAD_ARG = -1,
+  CDFT_HIP_ERROR = -2,
+  CDFT_NOT_SUPPORTED = -3
+};
+
+/* FMT models: include/FMT.h:139 (Rosenfeld), :262 (RSLT), :376 (esFMT A,B),
+   :505 (WhiteBearI = esFMT(1,-1) + CS f3), :576 (WhiteBearII). */
+enum cdft_model {
+  CDFT_MODEL_NONE = -1,
+  CDFT_ROSENFELD = 0,
+  CDFT_RSLT = 1,
+  CDFT_ESFMT = 2,
+  CDFT_WHITEBEAR_I = 3,
+  CDFT_WHITEBEAR_II = 4
+};
+
+enum cdft_precision {
+  CDFT_FP64 = 0,  /* drop-in path: everything in double */
+  CDFT_MIXED = 1  /* FFTs, spectra and convolutions in float; pointwise Phi,
+                     ideal gas, reductions and solver state in double */
+};
+
+enum cdft_flags {
+  CDFT_DEVICE_PTRS = 1, /* rho / dF / vext are device pointers */
+  CDFT_ACCUMULATE = 2   /* add into dF instead of overwriting it */
+};
+
+typedef struct cdft_system_s* cdft_system;
+
+typedef struct cdft_system_desc {
+  int Nx, Ny, Nz;         /* grid; Nx, Ny, Nz/2 must be products of 2 and 3, <= 512; Nz even */
+  double dx, dy, dz;      /* lattice spacings */
+  int model;              /* enum cdft_model */
+  double A, B;            /* esFMT coefficients (ignored for other models) */
+  double hsd;             /* hard-sphere diameter */
+  int precision;          /* enum cdft_precision */
+  const double* fmt_weights; /* host, 11*N real-space cell weights in the
+                                FMT_Species index order (include/Species.h:351-360),
+                                or NULL to generate them on the device */
+  int ideal;              /* include the ideal-gas term (src/DFT.cpp:388-394) */
+  double mu;              /* beta*mu (src/Species.cpp:51-62) */
+  const double* vext;     /* host, N values of beta*V_ext, or NULL */
+  int mean_field;         /* include the LJ mean field (src/Interaction.cpp:76-115) */
+  double lj_sigma, lj_eps, lj_rcut, kT; /* LJ with WCA split, include/Potential1.h:136-189 */
+  const double* mf_kernel; /* host, N values of w_att (already / kT), or NULL to generate */
+} cdft_system_desc;
+
+const char* cdft_last_error(void);
+int cdft_device_count(int* n);
+
+/* Replaces the FMT_Species + FMT + Interaction + DFT construction
+   (src/FMT_Species.cpp:95-108 weights; src/Interaction.cpp:32-52 kernel). */
+int cdft_system_create(const cdft_system_desc* desc, cdft_system* out);
+int cdft_system_destroy(cdft_system sys);
+
+/* Replaces DFT::calculateFreeEnergyAndDerivatives(false) (src/DFT.cpp:360-425).
+   parts (host, may be NULL) receives {F, F_id, F_hs, F_mf, F_ext}.  dF is
+   dF/drho_i including dV (include/Species.h:55 convention). */
+int cdft_system_eval(cdft_system sys, const double* rho, double* dF, double* parts, int flags);
+
+/* Replaces FMT::calculateFreeEnergyAndDerivatives (src/FMT.cpp:363-427): only
+   the hard-sphere part; returns F_hs (already x dV) in *F_hs and adds (or
+   writes) the FMT dF. */
+int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, int flags);
+
+/* Introspection / plumbing. */
+int cdft_system_stream(cdft_system sys, void** hip_stream);
+int cdft_system_a_vdw(cdft_system sys, double* a_vdw); /* Interaction_Base::getVDWParameter */
+/* Copies the device-generated real-space FMT weights (11*N) or MF kernel (N). */
+int cdft_system_get_weights(cdft_system sys, double* w11);
+int cdft_system_get_mf_kernel(cdft_system sys, double* w);
+
+/* Standalone 3D FFTs with the FFTW conventions of DFT_FFT (include/DFT_LinAlg.h:210-211):
+   unnormalised r2c (N real -> Nx*Ny*(Nz/2+1) complex interleaved) and c2r.
+   Host pointers; used to check the transform against numpy / the oracle. */
+int cdft_fft_r2c(int Nx, int Ny, int Nz, int precision, const double* in, double* out);
+int cdft_fft_c2r(int Nx, int Ny, int Nz, int precision, const double* in, double* out);
+
+/* Per-stage timing of the last cdft_system_eval with timing enabled: up to
+   16 entries of {kernel id, milliseconds}.  enable = 1 records hipEvents
+   between stages (adds no synchronisation inside the pipeline). */
+int cdft_system_set_timing(cdft_system sys, int enable);
+int cdft_system_get_timing(cdft_system sys, int* n, double* ms, const char** names);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CDFT_H */

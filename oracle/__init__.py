@@ -1,0 +1,267 @@
+"""ctypes front end of the CPU oracle (``oracle/oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, by ``__graft_entry__.smoke()``
+and by the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product
+package ``classicaldft_amd`` never imports it.
+
+Model ids follow the C ABI of the product: 0 Rosenfeld, 1 RSLT, 2 esFMT(A, B),
+3 White Bear I, 4 White Bear II (reference ``include/FMT.h:139-643``).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ROSENFELD, RSLT, ESFMT, WBI, WBII = 0, 1, 2, 3, 4
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_set_threads.argtypes = [_i, _i]
+        L.orc_fft1d.argtypes = [_i, _p, _i]
+        L.orc_r2c3d.argtypes = [_i, _i, _i, _p, _p]
+        L.orc_c2r3d.argtypes = [_i, _i, _i, _p, _p]
+        L.orc_fmt_weights.argtypes = [_i, _i, _i, _d, _d, _d, _d, _p, _i]
+        L.orc_fmt_weights.restype = _i
+        L.orc_phi.argtypes = [_i, _d, _d, _p, _p]
+        L.orc_phi.restype = _d
+        L.orc_bulk.argtypes = [_i, _d, _d, _d, _d, _p, _p]
+        L.orc_fcoef.argtypes = [_i, _d, _p]
+        L.orc_fmt_eval.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _p, _p, _p, _p, _p]
+        L.orc_fmt_eval.restype = _i
+        L.orc_mf_kernel.argtypes = [_i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _p]
+        L.orc_mf_kernel.restype = _d
+        L.orc_dft_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _i, _i, _i, _d, _d, _d, _d, _d, _p]
+        L.orc_dft_create.restype = _p
+        L.orc_dft_destroy.argtypes = [_p]
+        L.orc_dft_avdw.argtypes = [_p]
+        L.orc_dft_avdw.restype = _d
+        L.orc_dft_eval.argtypes = [_p, _p, _p, _p]
+        L.orc_dft_eval.restype = _i
+        L.orc_fire_create.argtypes = [_p, _p, _d, _d]
+        L.orc_fire_create.restype = _p
+        L.orc_fire_destroy.argtypes = [_p]
+        L.orc_fire_step.argtypes = [_p, _p]
+        L.orc_fire_step.restype = _i
+        L.orc_fire_get.argtypes = [_p, _p, _p, _p, _p]
+        L.orc_ddft_create.argtypes = [_p, _p, _d, _d, _d]
+        L.orc_ddft_create.restype = _p
+        L.orc_ddft_destroy.argtypes = [_p]
+        L.orc_ddft_step.argtypes = [_p, _p]
+        L.orc_ddft_step.restype = _i
+        L.orc_ddft_get.argtypes = [_p, _p]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def set_threads(pointwise=1, fft=1):
+    lib().orc_set_threads(int(pointwise), int(fft))
+
+
+def fft1d(x, sign=-1):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().orc_fft1d(a.size, _ptr(a), int(sign))
+    return a
+
+
+def rfftn(rho):
+    """FFTW-convention 3D r2c: [Nx,Ny,Nz] real -> [Nx,Ny,Nz//2+1] complex."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    Nx, Ny, Nz = rho.shape
+    out = np.empty((Nx, Ny, Nz // 2 + 1), dtype=np.complex128)
+    lib().orc_r2c3d(Nx, Ny, Nz, _ptr(rho), _ptr(out))
+    return out
+
+
+def irfftn(spec, Nz):
+    """FFTW-convention unnormalised 3D c2r."""
+    spec = np.ascontiguousarray(spec, dtype=np.complex128)
+    Nx, Ny, _ = spec.shape
+    out = np.empty((Nx, Ny, Nz), dtype=np.float64)
+    lib().orc_c2r3d(Nx, Ny, Nz, _ptr(spec), _ptr(out))
+    return out
+
+
+WEIGHTS_FAITHFUL, WEIGHTS_ACCURATE, WEIGHTS_WC_DOUBLE = 0, 1, 2
+
+
+def fmt_weights(shape, dx, hsd, mode=WEIGHTS_FAITHFUL):
+    """Real-space cell-integrated FMT weights, [11, Nx, Ny, Nz].
+
+    mode 0 restates the reference's double-precision expressions; mode 1 is the
+    well-conditioned (atan2) evaluation in long double; mode 2 the same in double.
+    """
+    Nx, Ny, Nz = shape
+    dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+    w = np.zeros((11, Nx, Ny, Nz), dtype=np.float64)
+    rc = lib().orc_fmt_weights(Nx, Ny, Nz, dxs[0], dxs[1], dxs[2], hsd, _ptr(w), int(mode))
+    if rc != 0:
+        raise ValueError("hsd is less than the lattice spacing")
+    return w
+
+
+def phi(model, meas19, A=1.0, B=0.0):
+    m = np.ascontiguousarray(meas19, dtype=np.float64)
+    d = np.zeros(19)
+    p = lib().orc_phi(model, A, B, _ptr(m), _ptr(d))
+    return p, d
+
+
+def fcoef(model, eta):
+    """(f1, f1', f2, f2', f3, f3') of a model at eta."""
+    out = np.zeros(6)
+    lib().orc_fcoef(model, eta, _ptr(out))
+    return out
+
+
+def bulk(model, rho, hsd, A=1.0, B=0.0):
+    fex, mu = _d(), _d()
+    lib().orc_bulk(model, A, B, rho, hsd, ctypes.byref(fex), ctypes.byref(mu))
+    return fex.value, mu.value
+
+
+class EtaTooLarge(RuntimeError):
+    pass
+
+
+def fmt_eval(model, rho, dx, hsd, weights=None, A=1.0, B=0.0, want_phi=False):
+    """One FMT F + dF/drho evaluation.  Returns (F, dF[, phi])."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    Nx, Ny, Nz = rho.shape
+    dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+    dF = np.zeros_like(rho)
+    F = _d()
+    ph = np.zeros_like(rho) if want_phi else None
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    rc = lib().orc_fmt_eval(model, A, B, Nx, Ny, Nz, dxs[0], dxs[1], dxs[2], hsd, _ptr(w), _ptr(rho), _ptr(dF),
+                            ctypes.byref(F), _ptr(ph))
+    if rc == 1:
+        raise EtaTooLarge("eta too large")
+    if rc != 0:
+        raise ValueError("bad arguments")
+    return (F.value, dF, ph) if want_phi else (F.value, dF)
+
+
+def mf_kernel(shape, dx, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0):
+    Nx, Ny, Nz = shape
+    dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+    w = np.zeros(shape, dtype=np.float64)
+    a = lib().orc_mf_kernel(Nx, Ny, Nz, dxs[0], dxs[1], dxs[2], sigma, eps, rcut, kT, _ptr(w))
+    return w, a
+
+
+class DFT:
+    """Whole-DFT oracle (ideal + FMT + LJ/QF mean field + V_ext - mu)."""
+
+    def __init__(self, shape, dx, hsd, model=WBI, A=1.0, B=0.0, fmt=True, ideal=True, mf=False, sigma=1.0,
+                 eps=1.0, rcut=2.5, kT=1.0, mu=0.0, vext=None):
+        self.shape = tuple(shape)
+        dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+        self.dV = dxs[0] * dxs[1] * dxs[2]
+        self._vext = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64)
+        self.h = lib().orc_dft_create(model, A, B, *self.shape, *dxs, hsd, int(fmt), int(ideal), int(mf), sigma, eps,
+                                      rcut, kT, mu, _ptr(self._vext))
+
+    @property
+    def a_vdw(self):
+        return lib().orc_dft_avdw(self.h)
+
+    def eval(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        dF = np.zeros_like(rho)
+        parts = np.zeros(5)
+        rc = lib().orc_dft_eval(self.h, _ptr(rho), _ptr(dF), _ptr(parts))
+        if rc == 1:
+            raise EtaTooLarge("eta too large")
+        return parts, dF
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_dft_destroy(self.h)
+            self.h = None
+
+
+class Fire:
+    def __init__(self, dft, rho0, dt=1e-3, dt_max=1e-2):
+        self.dft = dft
+        self.shape = dft.shape
+        r = np.ascontiguousarray(rho0, dtype=np.float64)
+        self.h = lib().orc_fire_create(dft.h, _ptr(r), dt, dt_max)
+
+    def step(self):
+        info = np.zeros(8)
+        rc = lib().orc_fire_step(self.h, _ptr(info))
+        return rc, dict(F=info[0], fmax=info[1], fmax_raw=info[2], dt=info[3], alpha=info[4], vv=info[5],
+                        rms_force=info[6], calls=int(info[7]))
+
+    def state(self):
+        rho, x, v, df = (np.zeros(self.shape) for _ in range(4))
+        lib().orc_fire_get(self.h, _ptr(rho), _ptr(x), _ptr(v), _ptr(df))
+        return rho, x, v, df
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_fire_destroy(self.h)
+            self.h = None
+
+
+class DDFT:
+    def __init__(self, dft, rho0, dt=0.0, dt_max=1.0, tol=1e-4):
+        self.dft = dft
+        self.shape = dft.shape
+        r = np.ascontiguousarray(rho0, dtype=np.float64)
+        self.h = lib().orc_ddft_create(dft.h, _ptr(r), dt, dt_max, tol)
+
+    def step(self):
+        info = np.zeros(6)
+        rc = lib().orc_ddft_step(self.h, _ptr(info))
+        if rc == 1:
+            raise EtaTooLarge("eta too large")
+        return dict(F=info[0], dt=info[1], time=info[2], rhs_max=info[3], calls=int(info[4]), restarts=int(info[5]))
+
+    def density(self):
+        rho = np.zeros(self.shape)
+        lib().orc_ddft_get(self.h, _ptr(rho))
+        return rho
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_ddft_destroy(self.h)
+            self.h = None
+
+
+def splitmix64_uniform(n, seed):
+    """u in [0,1) from splitmix64(seed), (x >> 11) * 2^-53, site order (z fastest)."""
+    out = np.empty(n, dtype=np.float64)
+    state = np.uint64(seed)
+    mask = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        # vectorised splitmix64: state_k = seed + (k+1) * golden
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = (state + k * np.uint64(0x9E3779B97F4A7C15)) & mask
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & mask
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & mask
+        z = z ^ (z >> np.uint64(31))
+    out[:] = (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return out

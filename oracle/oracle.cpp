@@ -1,0 +1,1355 @@
+// CPU oracle for the classicalDFT FMT / DFT / FIRE2 / DDFT hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is a plain FP64 C++ restatement of the
+// reference algorithms, used by tests/, by __graft_entry__.smoke() and by the
+// cpu_baseline leg of bench.py as the checker.  Nothing in the product
+// (classicaldft_amd/, include/) links, loads or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the classicalDFT source tree).  Parity pins for each part live in
+// tests/test_oracle_*.py:
+//   FFT          -> numpy.fft golden vectors + naive DFT + Parseval/round trip
+//   weights      -> sum rules (partition of unity of the trilinear hat),
+//                   parity, independent Monte-Carlo-free quadrature of one cell
+//   Phi / dPhi   -> Carnahan-Starling / Percus-Yevick bulk closed forms,
+//                   finite differences over all 19 measures, series-branch
+//                   continuity at eta = 1e-12
+//   F + dF       -> uniform-density closed forms, brute-force direct-space
+//                   convolution at 8^3, central finite differences of F
+//   mean field   -> brute-force convolution (Interaction.cpp:536-563 recipe),
+//                   uniform F_mf/V = a_vdw rho^2 / 2
+//   FIRE2, DDFT  -> invariants (mass conservation, exact diffusion solution)
+//                   and trajectory replay against the GPU path.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using cd = std::complex<double>;
+
+// ---------------------------------------------------------------------------
+// Neumaier compensated sum: include/Summation.h:15-56
+// ---------------------------------------------------------------------------
+struct NSum {
+  double s = 0.0, c = 0.0;
+  void add(double v) {
+    double t = s + v;
+    if (std::fabs(s) >= std::fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+  }
+  double sum() const { return s + c; }
+};
+
+int g_threads = 1;      // threads for pointwise loops
+int g_fft_threads = 1;  // threads for FFT rows (1 = reference-faithful, DFT_Factory.h:86-90)
+
+// ---------------------------------------------------------------------------
+// Mixed-radix Stockham FFT (stands in for FFTW3 r2c/c2r, DFT_LinAlg.h:163-288).
+// Unnormalised, forward sign -1, like FFTW.
+// ---------------------------------------------------------------------------
+struct Plan1D {
+  int n = 0;
+  std::vector<int> radix;
+  std::vector<cd> w;  // w[j] = exp(-2 pi i j / n)
+};
+
+Plan1D make_plan(int n) {
+  Plan1D p;
+  p.n = n;
+  int m = n;
+  const int order[] = {4, 2, 3, 5, 7};
+  for (int r : order)
+    while (m % r == 0) { p.radix.push_back(r); m /= r; }
+  if (m != 1) p.radix.push_back(m);  // generic prime radix (direct DFT)
+  p.w.resize(n);
+  for (int j = 0; j < n; ++j) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+    p.w[j] = cd((double)cosl(a), (double)sinl(a));
+  }
+  return p;
+}
+
+// y <- DFT_R(u) with roots exp(sign*2 pi i rq / R), taken from the plan's table.
+inline void small_dft(int R, const cd* u, cd* y, int sign, const Plan1D& p) {
+  const int step = p.n / R;
+  for (int q = 0; q < R; ++q) {
+    cd acc = 0.0;
+    for (int r = 0; r < R; ++r) {
+      cd wr = p.w[((long)r * q % R) * step];
+      if (sign > 0) wr = std::conj(wr);
+      acc += u[r] * wr;
+    }
+    y[q] = acc;
+  }
+}
+
+// In-place on x (uses scratch of length n).
+void fft1d(const Plan1D& p, cd* x, cd* scratch, int sign) {
+  const int n = p.n;
+  cd* src = x;
+  cd* dst = scratch;
+  int pacc = 1;
+  cd u[64], y[64];
+  for (int R : p.radix) {
+    const int m = n / R;
+    const int tstep = n / (pacc * R);
+    for (int i = 0; i < m; ++i) {
+      const int k = i % pacc;
+      for (int r = 0; r < R; ++r) {
+        cd v = src[i + r * m];
+        if (r && k) {
+          cd w = p.w[(long)r * k * tstep];
+          if (sign > 0) w = std::conj(w);
+          v *= w;
+        }
+        u[r] = v;
+      }
+      small_dft(R, u, y, sign, p);
+      const int j = (i - k) * R + k;
+      for (int r = 0; r < R; ++r) dst[j + r * pacc] = y[r];
+    }
+    pacc *= R;
+    std::swap(src, dst);
+  }
+  if (src != x) std::memcpy(x, src, sizeof(cd) * n);
+}
+
+struct Grid {
+  int Nx, Ny, Nz;
+  long N() const { return (long)Nx * Ny * Nz; }
+  int Zh() const { return Nz / 2 + 1; }
+  long Nk() const { return (long)Nx * Ny * Zh(); }
+};
+
+// Strided 1D transforms over one axis of a complex [Nx][Ny][Zh] array.
+void fft_axis(const Grid& g, cd* a, int axis, int sign) {
+  const int Zh = g.Zh();
+  const int n = axis == 0 ? g.Nx : g.Ny;
+  const long stride = axis == 0 ? (long)g.Ny * Zh : Zh;
+  const int outer = axis == 0 ? g.Ny : g.Nx;
+  Plan1D p = make_plan(n);
+#pragma omp parallel num_threads(g_fft_threads)
+  {
+    std::vector<cd> buf(n), scr(n);
+#pragma omp for schedule(static)
+    for (long t = 0; t < (long)outer * Zh; ++t) {
+      const long o = t / Zh, kz = t % Zh;
+      const long base = axis == 0 ? o * Zh + kz : o * (long)g.Ny * Zh + kz;
+      for (int j = 0; j < n; ++j) buf[j] = a[base + j * stride];
+      fft1d(p, buf.data(), scr.data(), sign);
+      for (int j = 0; j < n; ++j) a[base + j * stride] = buf[j];
+    }
+  }
+}
+
+// FFTW-convention r2c of a row-major (z fastest) real array: Lattice.h:120-134
+void r2c3d(const Grid& g, const double* in, cd* out) {
+  const int Nz = g.Nz, Zh = g.Zh();
+  Plan1D pz = make_plan(Nz);
+#pragma omp parallel num_threads(g_fft_threads)
+  {
+    std::vector<cd> buf(Nz), scr(Nz);
+#pragma omp for schedule(static)
+    for (long row = 0; row < (long)g.Nx * g.Ny; ++row) {
+      for (int z = 0; z < Nz; ++z) buf[z] = cd(in[row * Nz + z], 0.0);
+      fft1d(pz, buf.data(), scr.data(), -1);
+      for (int k = 0; k < Zh; ++k) out[row * Zh + k] = buf[k];
+    }
+  }
+  fft_axis(g, out, 1, -1);
+  fft_axis(g, out, 0, -1);
+}
+
+// FFTW-convention c2r (unnormalised); input is not modified.
+void c2r3d(const Grid& g, const cd* in, double* out) {
+  const int Nz = g.Nz, Zh = g.Zh();
+  std::vector<cd> work(in, in + g.Nk());
+  fft_axis(g, work.data(), 0, +1);
+  fft_axis(g, work.data(), 1, +1);
+  Plan1D pz = make_plan(Nz);
+#pragma omp parallel num_threads(g_fft_threads)
+  {
+    std::vector<cd> buf(Nz), scr(Nz);
+#pragma omp for schedule(static)
+    for (long row = 0; row < (long)g.Nx * g.Ny; ++row) {
+      const cd* r = work.data() + row * Zh;
+      for (int k = 0; k < Zh; ++k) buf[k] = r[k];
+      for (int k = Zh; k < Nz; ++k) buf[k] = std::conj(r[Nz - k]);
+      fft1d(pz, buf.data(), scr.data(), +1);
+      for (int z = 0; z < Nz; ++z) out[row * Nz + z] = buf[z].real();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cell-integrated FMT weights: src/FMT_Species.cpp:112-308 (I, J, G functions)
+// and :310-496 (generateWeights).  Array order is the reference's index map
+// include/Species.h:351-360: 0 eta, 1 s, 2-4 v_{x,y,z}, 5 Txx, 6 Txy, 7 Txz,
+// 8 Tyy, 9 Tyz, 10 Tzz.
+//
+// Two evaluations of the same integrals are provided:
+//  * "faithful" (T = double): the reference's expressions, including its
+//    asin(p/q) forms, the 1e-15 guards and the R += 1e-15 shift.  Where the
+//    exact argument of an asin is +-1 (a corner coordinate equal to zero),
+//    rounding leaves it at 1 - O(1e-16) and asin's square-root singularity
+//    turns that into an O(1e-8) error, so the reference's own weights carry
+//    per-cell roundoff of up to ~1e-6 (dx = 0.1) .. ~1e-4 (dx = 0.04)
+//    relative.  The oracle reproduces that path for reference-faithful runs.
+//  * "well-conditioned" (any T): every asin(p/q) whose complementary leg is
+//    known exactly is evaluated as atan2(p, leg).  With
+//    E = R^2 - X^2 - V^2:  asin(X/sqrt(R^2-V^2)) = atan2(X, sqrt E),
+//    asin(V/sqrt(R^2-X^2)) = atan2(V, sqrt E),
+//    asin(XV/sqrt((R^2-V^2)(R^2-X^2))) = atan2(XV, R sqrt E), and at the
+//    upper limit X = sqrt(R^2-Va^2-Vb^2) one has E_a = Vb^2, E_b = Va^2
+//    exactly.  In long double this is the oracle's accurate weight set.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Moments {
+  T Iy[5], Iz[5], Jy[5], Jz[5];
+};
+
+template <typename T>
+void I_faithful(T X, T A, T I[5]) {  // FMT_Species.cpp:112-122
+  const T a = std::asin(X / A);
+  const T b = std::sqrt(std::fabs(A * A - X * X));
+  const T A2 = A * A, X2 = X * X;
+  I[0] = T(0.5) * X * b + T(0.5) * A2 * a;
+  I[1] = -(T(1) / 3) * (A2 - X2) * b;
+  I[2] = T(0.125) * X * (2 * X2 - A2) * b + T(0.125) * A2 * A2 * a;
+  I[3] = -(T(1) / 15) * (A2 - X2) * (3 * X2 + 2 * A2) * b;
+  I[4] = (T(1) / 48) * (8 * X2 * X2 - 2 * A2 * X2 - 3 * A2 * A2) * X * b + T(0.0625) * A2 * A2 * A2 * a;
+}
+
+template <typename T>
+T clamp1(T v) { return std::max(T(-1), std::min(T(1), v)); }
+
+template <typename T>
+void J_from(T X, T V, T R, T aV, T aX, T aVX, T b, T J[5]) {  // FMT_Species.cpp:133-137
+  const T R2 = R * R, V2 = V * V, X2 = X * X;
+  const T pi = T(3.14159265358979323846264338327950288L);
+  J[0] = X * aV + V * aX - R * aVX;
+  J[1] = T(-0.5) * V * b + T(0.5) * (X2 - R2) * aV;
+  J[2] = -(T(1) / 6) * V * (V2 - 3 * R2) * aX + (T(1) / 3) * X2 * X * aV - (T(1) / 3) * R2 * R * aVX - (T(1) / 6) * V * X * b;
+  J[3] = T(0.25) * (X2 * X2 - R2 * R2) * aV + (pi / 8) * R2 * R2 - (V / T(12)) * (5 * R2 + X2 - 2 * V2) * b;
+  J[4] = T(0.025) * X * V * (3 * V2 - 2 * X2 - 7 * R2) * b + T(0.025) * V * (3 * V2 * V2 - 10 * V2 * R2 + 15 * R2 * R2) * aX +
+         T(0.2) * X2 * X2 * X * aV - T(0.2) * R2 * R2 * R * aVX;
+}
+
+template <typename T>
+void J_faithful(T X, T V, T R, T J[5]) {  // FMT_Species.cpp:126-132
+  const T tiny = T(1e-15);
+  const T R2 = R * R, V2 = V * V, X2 = X * X;
+  const T aV = std::asin(clamp1(V / std::max(tiny, std::sqrt(R2 - X2))));
+  const T aX = std::asin(clamp1(X / std::sqrt(R2 - V2)));
+  const T aVX = std::asin(clamp1(X * V / std::max(tiny, std::sqrt((R2 - V2) * (R2 - X2)))));
+  const T b = std::sqrt(std::fabs(R2 - V2 - X2));
+  J_from(X, V, R, aV, aX, aVX, b, J);
+}
+
+template <typename T>
+void IJ_wc(T X, T V, T R, T E, T I[5], T J[5]) {
+  if (E < 0) E = 0;
+  const T sE = std::sqrt(E);
+  const T aX = std::atan2(X, sE);
+  const T aV = std::atan2(V, sE);
+  const T aVX = std::atan2(X * V, R * sE);
+  const T A2 = X * X + E;  // = R^2 - V^2
+  const T X2 = X * X;
+  I[0] = T(0.5) * X * sE + T(0.5) * A2 * aX;
+  I[1] = -(T(1) / 3) * E * sE;
+  I[2] = T(0.125) * X * (2 * X2 - A2) * sE + T(0.125) * A2 * A2 * aX;
+  I[3] = -(T(1) / 15) * E * (3 * X2 + 2 * A2) * sE;
+  I[4] = (T(1) / 48) * (8 * X2 * X2 - 2 * A2 * X2 - 3 * A2 * A2) * X * sE + T(0.0625) * A2 * A2 * A2 * aX;
+  J_from(X, V, R, aV, aX, aVX, sE, J);
+}
+
+// Fill the moments for G(R, X, Vy, Vz, ...).  Ey = R^2-X^2-Vy^2, Ez = R^2-X^2-Vz^2
+// (used only by the well-conditioned path).  The faithful path applies the
+// reference's R += 1e-15 (FMT_Species.cpp:156,203,231,260,288).
+template <typename T, bool WC>
+void moments(T& R, T X, T Vy, T Vz, T Ey, T Ez, Moments<T>& q) {
+  if (WC) {
+    IJ_wc(X, Vy, R, Ey, q.Iy, q.Jy);
+    IJ_wc(X, Vz, R, Ez, q.Iz, q.Jz);
+  } else {
+    R += T(1e-15);
+    I_faithful(X, std::sqrt(R * R - Vy * Vy), q.Iy);
+    I_faithful(X, std::sqrt(R * R - Vz * Vz), q.Iz);
+    J_faithful(X, Vy, R, q.Jy);
+    J_faithful(X, Vz, R, q.Jz);
+  }
+}
+
+// G_eta, FMT_Species.cpp:148-193
+template <typename T>
+T G_eta(T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
+  const T pi = T(3.14159265358979323846264338327950288L);
+  const T R2 = R * R;
+  const T A = Ty * Tz * Vy * Vz + T(0.25) * Vy * Vy * Vz * Vz + T(0.125) * R2 * R2 -
+              (T(1) / 6) * (Ty * Vy * Vy * Vy + Tz * Vz * Vz * Vz) - T(0.5) * Tz * Vz * Vy * Vy - T(0.5) * Ty * Vy * Vz * Vz +
+              T(0.125) * Vy * Vy * Vy * Vy + T(0.125) * Vz * Vz * Vz * Vz +
+              T(0.5) * R2 * (Ty * Vy + Tz * Vz - T(0.5) * Vy * Vy - T(0.5) * Vz * Vz + T(0.5) * pi * Ty * Tz);
+  const T B = T(0.5) * Ty * Vy + T(0.5) * Tz * Vz - T(0.25) * Vy * Vy - T(0.25) * Vz * Vz + (pi / 4) * Ty * Tz + T(0.25) * R2;
+  const T C = T(0.5) * Ty * Vy + (T(1) / 3) * (R2 - Vy * Vy);
+  const T D = T(0.5) * Tz * Vz + (T(1) / 3) * (R2 - Vz * Vz);
+  const T X2 = X * X;
+  T g = Tx * A * X - T(0.5) * A * X2 - (T(1) / 3) * B * Tx * X2 * X + T(0.25) * X2 * X2 * B + T(0.025) * Tx * X2 * X2 * X -
+        (T(1) / 48) * X2 * X2 * X2;
+  g -= T(0.5) * Ty * Tz * (Tx * R2 * q.Jy[0] - R2 * q.Jy[1] - Tx * q.Jy[2] + q.Jy[3]);
+  g -= T(0.5) * Ty * Tz * (Tx * R2 * q.Jz[0] - R2 * q.Jz[1] - Tx * q.Jz[2] + q.Jz[3]);
+  g -= (Tx * Tz * C * q.Iy[0] - Tz * C * q.Iy[1] - (T(1) / 3) * Tx * Tz * q.Iy[2] + (T(1) / 3) * Tz * q.Iy[3]);
+  g -= (Tx * Ty * D * q.Iz[0] - Ty * D * q.Iz[1] - (T(1) / 3) * Tx * Ty * q.Iz[2] + (T(1) / 3) * Ty * q.Iz[3]);
+  return g;
+}
+
+// G_s, G_vx, G_txx share one structure: a polynomial in X plus I/J moments of
+// orders (o, o+1); s -> 0, vx -> 1, txx -> 2 (FMT_Species.cpp:195-278).
+template <typename T>
+T G_moment(int o, T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
+  const T pi = T(3.14159265358979323846264338327950288L);
+  const T A = Ty * Vy + Tz * Vz - T(0.5) * Vy * Vy - T(0.5) * Vz * Vz + T(0.5) * R * R + (pi / 2) * Ty * Tz;
+  const T X2 = X * X;
+  T g;
+  if (o == 0)
+    g = T(0.125) * R * X2 * X2 - (T(1) / 6) * R * Tx * X2 * X - T(0.5) * R * A * X2 + R * A * Tx * X;
+  else if (o == 1)
+    g = T(0.1) * R * X2 * X2 * X - T(0.125) * R * Tx * X2 * X2 - (T(1) / 3) * R * A * X2 * X + T(0.5) * R * A * Tx * X2;
+  else
+    g = (T(1) / 12) * R * X2 * X2 * X2 - T(0.1) * R * Tx * X2 * X2 * X - T(0.25) * R * A * X2 * X2 + (T(1) / 3) * R * A * Tx * X2 * X;
+  g -= R * Tz * (Tx * q.Iy[o] - q.Iy[o + 1]);
+  g -= R * Ty * (Tx * q.Iz[o] - q.Iz[o + 1]);
+  g -= R * Ty * Tz * (Tx * q.Jy[o] - q.Jy[o + 1]);
+  g -= R * Ty * Tz * (Tx * q.Jz[o] - q.Jz[o + 1]);
+  return g;
+}
+
+// G_txy, FMT_Species.cpp:280-308
+template <typename T>
+T G_txy(T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
+  const T pi = T(3.14159265358979323846264338327950288L);
+  const T R2 = R * R;
+  const T A = (T(1) / 6) * (3 * Ty * R2 + 2 * Vy * Vy * Vy - 3 * Ty * Vy * Vy - 3 * Ty * Vz * Vz + 6 * Ty * Tz * Vz + T(1.5) * pi * R2 * Tz);
+  const T B = T(0.25) * pi * Tz + T(0.5) * Ty;
+  const T C = -(T(1) / 6) * (2 * R2 - 2 * Vz * Vz + 3 * Tz * Vz);
+  const T X2 = X * X;
+  T g = T(-0.5) * R * A * Tx * X2 + (T(1) / 3) * R * A * X2 * X + T(0.25) * R * B * Tx * X2 * X2 - T(0.2) * R * B * X2 * X2 * X;
+  g += T(0.5) * R * Tz * (Tx * R2 * q.Jy[1] - R2 * q.Jy[2] - Tx * q.Jy[3] + q.Jy[4]);
+  g += T(0.5) * R * Tz * (Tx * R2 * q.Jz[1] - R2 * q.Jz[2] - Tx * q.Jz[3] + q.Jz[4]);
+  g += T(0.5) * R * Tz * (2 * Ty - Vy) * (Tx * q.Iy[1] - q.Iy[2]);
+  g += R * (-Tx * C * q.Iz[1] + C * q.Iz[2] - (T(1) / 3) * Tx * q.Iz[3] + (T(1) / 3) * q.Iz[4]);
+  return g;
+}
+
+enum { GK_ETA, GK_S, GK_V, GK_TXX, GK_TXY };
+
+// One difference G(lower) - G(upper) for integrand kind k, with the axis roles
+// already permuted by the caller: X-axis corner coordinate Xc, others Va, Vb.
+template <typename T, bool WC>
+T G_diff(int k, T R, T Xc, T Va, T Vb, T Tx, T Ta, T Tb) {
+  const T h2 = R * R - Va * Va - Vb * Vb;
+  const T h = std::sqrt(h2);
+  auto eval = [&](T X, T Ea, T Eb) {
+    T r = R;
+    Moments<T> q;
+    moments<T, WC>(r, X, Va, Vb, Ea, Eb, q);
+    switch (k) {
+      case GK_ETA: return G_eta(r, X, Va, Vb, Tx, Ta, Tb, q);
+      case GK_S: return G_moment(0, r, X, Va, Vb, Tx, Ta, Tb, q);
+      case GK_V: return G_moment(1, r, X, Va, Vb, Tx, Ta, Tb, q);
+      case GK_TXX: return G_moment(2, r, X, Va, Vb, Tx, Ta, Tb, q);
+      default: return G_txy(r, X, Va, Vb, Tx, Ta, Tb, q);
+    }
+  };
+  const T lo = eval(Xc, R * R - Xc * Xc - Va * Va, R * R - Xc * Xc - Vb * Vb);
+  const T hi = eval(h, Vb * Vb, Va * Va);
+  return lo - hi;
+}
+
+inline long pbc(int i, int n) {
+  i %= n;
+  return i < 0 ? i + n : i;
+}
+
+// FMT_Species::generateWeights, src/FMT_Species.cpp:310-496.  w must hold 11*N
+// zero-initialised doubles.  Returns 0, or -1 if hsd < lattice spacing (:321-322).
+template <typename T, bool WC>
+int gen_weights(const Grid& g, double dx_, double dy_, double dz_, double hsd, double* w) {
+  const T dx = dx_, dy = dy_, dz = dz_;
+  const T dV = dx * dy * dz;
+  const T R = T(hsd) / 2;
+  if (hsd < std::min(std::min(dx_, dy_), dz_)) return -1;
+  const int Sxm = 2 + int(R / dx), Sym = 2 + int(R / dy), Szm = 2 + int(R / dz);
+  const long N = g.N();
+  const int I2[2] = {-1, 1};
+  for (int Sx = 0; Sx <= Sxm; ++Sx)
+    for (int Sy = 0; Sy <= Sym; ++Sy)
+      for (int Sz = 0; Sz <= Szm; ++Sz) {
+        auto lo = [](int S) { return T(S == 0 ? 0 : S - 1); };
+        const T R2min = dx * dx * lo(Sx) * lo(Sx) + dy * dy * lo(Sy) * lo(Sy) + dz * dz * lo(Sz) * lo(Sz);
+        const T R2max = dx * dx * T(Sx + 1) * T(Sx + 1) + dy * dy * T(Sy + 1) * T(Sy + 1) + dz * dz * T(Sz + 1) * T(Sz + 1);
+        T we = 0, ws = 0, wv[3] = {0, 0, 0}, wt[3][3] = {{0}};
+        if (R * R > R2max) {
+          we = dV;
+        } else if (R * R > R2min) {
+          for (int ax : I2) {
+            int ix = ax;
+            T Tx = dx * T(Sx + ix);
+            const int px = Tx < 0 ? -1 : 1;
+            if (Tx < 0) { ix = 1; Tx = dx; }
+            for (int ay : I2) {
+              int iy = ay;
+              T Ty = dy * T(Sy + iy);
+              const int py = Ty < 0 ? -1 : 1;
+              if (Ty < 0) { iy = 1; Ty = dy; }
+              for (int az : I2) {
+                int iz = az;
+                T Tz = dz * T(Sz + iz);
+                const int pz = Tz < 0 ? -1 : 1;
+                if (Tz < 0) { iz = 1; Tz = dz; }
+                const int vx[2] = {0, ix}, vy[2] = {0, iy}, vz[2] = {0, iz};
+                for (int jx = 0; jx < 2; ++jx)
+                  for (int jy = 0; jy < 2; ++jy)
+                    for (int jz = 0; jz < 2; ++jz) {
+                      const T Vx = Tx - dx * vx[jx], Vy = Ty - dy * vy[jy], Vz = Tz - dz * vz[jz];
+                      const int sgn = ((jx + jy + jz) & 1) ? -1 : 1;
+                      if ((R * R - (Vx * Vx + Vy * Vy + Vz * Vz)) < std::nextafter(T(0), T(1))) continue;
+                      const T c = T(sgn) / dV;
+                      const T cr = c / (R * R);
+                      we += c * G_diff<T, WC>(GK_ETA, R, Vx, Vy, Vz, Tx, Ty, Tz);
+                      ws += c * G_diff<T, WC>(GK_S, R, Vx, Vy, Vz, Tx, Ty, Tz);
+                      wv[0] += c * T(px) / R * G_diff<T, WC>(GK_V, R, Vx, Vy, Vz, Tx, Ty, Tz);
+                      wv[1] += c * T(py) / R * G_diff<T, WC>(GK_V, R, Vy, Vx, Vz, Ty, Tx, Tz);
+                      wv[2] += c * T(pz) / R * G_diff<T, WC>(GK_V, R, Vz, Vy, Vx, Tz, Ty, Tx);
+                      wt[0][0] += cr * G_diff<T, WC>(GK_TXX, R, Vx, Vy, Vz, Tx, Ty, Tz);
+                      wt[1][1] += cr * G_diff<T, WC>(GK_TXX, R, Vy, Vx, Vz, Ty, Tx, Tz);
+                      wt[2][2] += cr * G_diff<T, WC>(GK_TXX, R, Vz, Vy, Vx, Tz, Ty, Tx);
+                      wt[0][1] += cr * T(px * py) * G_diff<T, WC>(GK_TXY, R, Vx, Vy, Vz, Tx, Ty, Tz);
+                      wt[0][2] += cr * T(px * pz) * G_diff<T, WC>(GK_TXY, R, Vx, Vz, Vy, Tx, Tz, Ty);
+                      wt[1][2] += cr * T(py * pz) * G_diff<T, WC>(GK_TXY, R, Vy, Vz, Vx, Ty, Tz, Tx);
+                    }
+              }
+            }
+          }
+        }
+        // octant scatter with vector/tensor parity: FMT_Species.cpp:471-493
+        static const int TIdx[3][3] = {{5, 6, 7}, {6, 8, 9}, {7, 9, 10}};
+        for (int ox = 0; ox < (Sx == 0 ? 1 : 2); ++ox)
+          for (int oy = 0; oy < (Sy == 0 ? 1 : 2); ++oy)
+            for (int oz = 0; oz < (Sz == 0 ? 1 : 2); ++oz) {
+              const int sx = 1 - 2 * ox, sy = 1 - 2 * oy, sz = 1 - 2 * oz;
+              const long pos = pbc(Sz * sz, g.Nz) + g.Nz * (pbc(Sy * sy, g.Ny) + g.Ny * pbc(Sx * sx, g.Nx));
+              const int sg[3] = {sx, sy, sz};
+              w[0 * N + pos] += double(we);
+              w[1 * N + pos] += double(ws);
+              for (int a = 0; a < 3; ++a) {
+                w[(2 + a) * N + pos] += double(T(sg[a]) * wv[a]);
+                for (int b = a; b < 3; ++b) w[TIdx[a][b] * N + pos] += double(T(sg[a] * sg[b]) * wt[a][b]);
+              }
+            }
+      }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Fundamental measures and the Phi models: include/Fundamental_Measures.h:5-242,
+// include/FMT.h:139-643, src/FMT.cpp:114-176.
+// ---------------------------------------------------------------------------
+struct FM {
+  double eta = 0, s0 = 0, s1 = 0, s2 = 0, v1[3] = {0, 0, 0}, v2[3] = {0, 0, 0}, T[3][3] = {{0}};
+  double v1_v2 = 0, v2_v2 = 0, vTv = 0, T2 = 0, T3 = 0, vT[3] = {0, 0, 0}, Tv[3] = {0, 0, 0}, TT[3][3] = {{0}};
+  void derived() {  // Fundamental_Measures.h:76-106
+    for (int k = 0; k < 3; ++k) {
+      vT[k] = v2[0] * T[0][k] + v2[1] * T[1][k] + v2[2] * T[2][k];
+      Tv[k] = T[k][0] * v2[0] + T[k][1] * v2[1] + T[k][2] * v2[2];
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) TT[i][k] = T[i][0] * T[0][k] + T[i][1] * T[1][k] + T[i][2] * T[2][k];
+    v1_v2 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+    v2_v2 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+    vTv = v2[0] * vT[0] + v2[1] * vT[1] + v2[2] * vT[2];
+    T2 = TT[0][0] + TT[1][1] + TT[2][2];
+    T3 = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) T3 += TT[i][k] * T[k][i];
+  }
+  void uniform(double rho, double d) {  // Fundamental_Measures.h:20-31
+    *this = FM();
+    eta = M_PI * rho * d * d * d / 6;
+    s0 = M_PI * rho;
+    s1 = s0 * d;
+    s2 = s1 * d;
+    T[0][0] = T[1][1] = T[2][2] = s2 / 3;
+    derived();
+  }
+  void to19(double f[19]) const {  // get_as_vector order, Fundamental_Measures.h:136-163
+    f[0] = eta; f[1] = s0; f[2] = s1; f[3] = s2;
+    for (int k = 0; k < 3; ++k) { f[4 + k] = v1[k]; f[7 + k] = v2[k]; }
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) f[10 + 3 * i + k] = T[i][k];
+  }
+  void from19(const double f[19]) {
+    eta = f[0]; s0 = f[1]; s1 = f[2]; s2 = f[3];
+    for (int k = 0; k < 3; ++k) { v1[k] = f[4 + k]; v2[k] = f[7 + k]; }
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) T[i][k] = f[10 + 3 * i + k];
+    derived();
+  }
+};
+
+enum { ROSENFELD = 0, RSLT = 1, ESFMT = 2, WBI = 3, WBII = 4 };
+
+struct Model {
+  int kind;
+  double A, B;
+  bool tensor() const { return kind >= ESFMT; }
+  double f1(double e) const { return std::log(1.0 - e); }
+  double f1p(double e) const { return -1.0 / (1.0 - e); }
+  double f2(double x) const {
+    if (kind == WBII) {  // FMT.h:588-594
+      if (x < 1e-12) return 1 + x * (1 + x * ((10.0 / 9) + (7.0 / 6) * x));
+      return (1.0 / 3.0) + (4.0 / 3.0) * (1.0 / (1.0 - x)) + (2.0 / (3.0 * x)) * std::log(1 - x);
+    }
+    return 1.0 / (1.0 - x);  // FMT.h:168-171
+  }
+  double f2p(double x) const {
+    if (kind == WBII) {  // FMT.h:596-602
+      if (x < 1e-12) return 1 + x * ((20.0 / 9) + 3.5 * x);
+      return (2 * (3 * x - 1) / (3 * x * (1 - x) * (1 - x))) - (2.0 / (3.0 * x * x)) * std::log(1 - x);
+    }
+    const double f = 1.0 / (1.0 - x);
+    return f * f;
+  }
+  double f3(double x) const {
+    if (kind == RSLT || kind == WBI) {  // FMT.h:267-273, 536-542
+      if (x < 1e-12) return 1.5 + (8.0 / 3) * x + 3.75 * x * x + 4.8 * x * x * x;
+      return (1.0 / (x * (1 - x) * (1 - x))) + (std::log(1 - x) / (x * x));
+    }
+    if (kind == WBII) {  // FMT.h:611-617
+      if (x < 1e-12) return 1.5 + (7.0 / 3) * x + 3.25 * x * x + 4.2 * x * x * x;
+      return -((1.0 - 3 * x + x * x) / (x * (1 - x) * (1 - x))) - (1.0 / (x * x)) * std::log(1 - x);
+    }
+    const double f = 1.0 / (1.0 - x);  // FMT.h:185-189
+    return f * f;
+  }
+  double f3p(double x) const {
+    if (kind == RSLT || kind == WBI) {  // FMT.h:275-281, 544-550
+      if (x < 1e-12) return (8.0 / 3) + 7.5 * x + 14.4 * x * x;
+      return ((-2 + 5 * x - x * x) / (x * x * (1 - x) * (1 - x) * (1 - x))) - (2 / (x * x * x)) * std::log(1 - x);
+    }
+    if (kind == WBII) {  // FMT.h:619-625
+      if (x < 1e-12) return (7.0 / 3) + 7.5 * x + 12.6 * x * x;
+      return ((2 - 5 * x + 6 * x * x - x * x * x) / (x * x * (1 - x) * (1 - x) * (1 - x))) + (2.0 / (x * x * x)) * std::log(1 - x);
+    }
+    const double f = 1.0 / (1.0 - x);  // FMT.h:191-195
+    return 2 * f * f * f;
+  }
+  double Phi3(const FM& m) const {
+    if (kind == ROSENFELD) return (1.0 / (24 * M_PI)) * (m.s2 * m.s2 * m.s2 - 3 * m.s2 * m.v2_v2);  // FMT.h:202-208
+    if (kind == RSLT) {  // FMT.h:293-301
+      if (m.s2 < 1e-20) return 0.0;
+      const double psi = m.v2_v2 / (m.s2 * m.s2);
+      return (1.0 / (36 * M_PI)) * m.s2 * m.s2 * m.s2 * (1 - psi) * (1 - psi) * (1 - psi);
+    }
+    const double s2 = m.s2;  // esFMT family, FMT.h:396-406
+    return (A / (24 * M_PI)) * (s2 * s2 * s2 - 3 * s2 * m.v2_v2 + 3 * m.vTv - m.T3) +
+           (B / (24 * M_PI)) * (s2 * s2 * s2 - 3 * s2 * m.T2 + 2 * m.T3);
+  }
+  double dPhi3_dS2(const FM& m) const {
+    const double s2 = m.s2;
+    if (kind == ROSENFELD) return (1.0 / (24 * M_PI)) * (3 * s2 * s2 - 3 * m.v2_v2);  // FMT.h:210-216
+    if (kind == RSLT) {  // FMT.h:303-311
+      if (s2 < 1e-20) return 0.0;
+      const double psi = m.v2_v2 / (s2 * s2);
+      return (1.0 / (12 * M_PI)) * s2 * s2 * (1 + psi) * (1 - psi) * (1 - psi);
+    }
+    return (A / (24 * M_PI)) * (3 * s2 * s2 - 3 * m.v2_v2) + (B / (24 * M_PI)) * (3 * s2 * s2 - 3 * m.T2);  // FMT.h:408-415
+  }
+  double dPhi3_dV2(int k, const FM& m) const {
+    const double s2 = m.s2;
+    if (kind == ROSENFELD) return (1.0 / (24 * M_PI)) * (-6 * s2 * m.v2[k]);  // FMT.h:233-240
+    if (kind == RSLT) {  // FMT.h:349-358
+      if (s2 < 1e-20) return 0.0;
+      const double psi = m.v2_v2 / (s2 * s2);
+      return -(1.0 / (6 * M_PI)) * s2 * m.v2[k] * (1 - psi) * (1 - psi);
+    }
+    return (A / (24 * M_PI)) * (-6 * s2 * m.v2[k] + 3 * m.vT[k] + 3 * m.Tv[k]);  // FMT.h:436-444
+  }
+  double dPhi3_dT(int j, int k, const FM& m) const {  // FMT.h:446-455
+    if (!tensor()) return 0.0;
+    return (A / (8 * M_PI)) * (m.v2[j] * m.v2[k] - m.TT[k][j]) + (B / (4 * M_PI)) * (-m.s2 * m.T[k][j] + m.TT[k][j]);
+  }
+  double phi(const FM& m) const {  // FMT.cpp:114-135
+    const double e = m.eta;
+    double p = 0;
+    p -= (1 / M_PI) * m.s0 * f1(e);
+    p += (1 / (2 * M_PI)) * (m.s1 * m.s2 - m.v1_v2) * f2(e);
+    p += Phi3(m) * f3(e);
+    return p;
+  }
+  void dphi(const FM& m, FM& d) const {  // FMT.cpp:137-176
+    const double e = m.eta;
+    const double F1 = f1(e), F2 = f2(e), F3 = f3(e), F1p = f1p(e), F2p = f2p(e), F3p = f3p(e);
+    d = FM();
+    d.eta -= (1 / M_PI) * m.s0 * F1p;
+    d.eta += (1 / (2 * M_PI)) * (m.s1 * m.s2 - m.v1_v2) * F2p;
+    d.eta += Phi3(m) * F3p;
+    d.s0 += -(1 / M_PI) * F1;
+    d.s1 += (1 / (2 * M_PI)) * m.s2 * F2;
+    d.s2 += (1 / (2 * M_PI)) * m.s1 * F2;
+    d.s2 += dPhi3_dS2(m) * F3;
+    for (int k = 0; k < 3; ++k) {
+      d.v1[k] += (1 / (2 * M_PI)) * (-m.v2[k]) * F2;
+      d.v2[k] += (1 / (2 * M_PI)) * (-m.v1[k]) * F2;
+      d.v2[k] += dPhi3_dV2(k, m) * F3;
+    }
+    if (tensor())
+      for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j) d.T[j][k] = dPhi3_dT(j, k, m) * F3;
+  }
+};
+
+Model make_model(int kind, double A, double B) {
+  Model m{kind, A, B};
+  if (kind == ROSENFELD || kind == RSLT) { m.A = 1; m.B = 0; }
+  if (kind == WBI || kind == WBII) { m.A = 1; m.B = -1; }  // FMT.h:510, 581
+  return m;
+}
+
+const int kTI[3][3] = {{5, 6, 7}, {6, 8, 9}, {7, 9, 10}};  // Species.h:354-360
+
+// Weighted densities at one site -> measures: FMT.cpp:78-111
+inline void measures_from_arrays(const double* const* n, long i, double d, bool tensor, FM& m) {
+  m = FM();
+  m.eta = n[0][i];
+  m.s0 = n[1][i] / (d * d);
+  m.s1 = n[1][i] / d;
+  m.s2 = n[1][i];
+  for (int j = 0; j < 3; ++j) {
+    m.v1[j] = n[2 + j][i] / d;
+    m.v2[j] = n[2 + j][i];
+    for (int k = 0; k < 3; ++k) m.T[j][k] = tensor ? n[kTI[j][k]][i] : 0.0;
+  }
+  m.derived();
+}
+
+// dPhi collapse to the 11 stored arrays: Species.h:310-329
+inline void collapse(const FM& D, double d, bool tensor, double out[11]) {
+  out[0] = D.eta;
+  out[1] = (D.s0 / (d * d)) + (D.s1 / d) + D.s2;
+  for (int j = 0; j < 3; ++j) out[2 + j] = -(D.v2[j] + D.v1[j] / d);
+  for (int j = 0; j < 3; ++j)
+    for (int k = j; k < 3; ++k) out[kTI[j][k]] = tensor ? (j == k ? 1 : 2) * D.T[j][k] : 0.0;
+}
+
+struct FmtState {
+  Grid g;
+  double dx, dy, dz, hsd;
+  Model model;
+  std::vector<cd> wk;  // 11 full complex spectra (x 1/N), FMT_Weighted_Density.h:35-39
+};
+
+void fmt_setup(FmtState& s, const double* wreal) {
+  const Grid& g = s.g;
+  const long N = g.N(), Nk = g.Nk();
+  std::vector<double> w;
+  if (!wreal) {
+    w.assign(11 * N, 0.0);
+    gen_weights<double, false>(g, s.dx, s.dy, s.dz, s.hsd, w.data());
+    wreal = w.data();
+  }
+  s.wk.assign(11 * Nk, cd(0, 0));
+  for (int a = 0; a < 11; ++a) {
+    r2c3d(g, wreal + a * N, s.wk.data() + a * Nk);
+    for (long k = 0; k < Nk; ++k) s.wk[a * Nk + k] *= 1.0 / (double)N;
+  }
+}
+
+// FMT::calculateFreeEnergyAndDerivatives, src/FMT.cpp:363-427 with
+// calculateFreeEnergy :310-352 and dPHI :283-305.  dF is ACCUMULATED
+// (Species::addToForce, Species.h:51).  Returns 1 on eta > 1.
+int fmt_eval(const FmtState& s, const double* rho, double* dF, double* Fout, double* phi_out = nullptr) {
+  const Grid& g = s.g;
+  const long N = g.N(), Nk = g.Nk();
+  const int m = s.model.tensor() ? 11 : 5;  // Species.h:140
+  const double dV = s.dx * s.dy * s.dz;
+  std::vector<cd> rk(Nk), tmp(Nk);
+  r2c3d(g, rho, rk.data());
+  std::vector<double> n(11 * N, 0.0);
+  for (int a = 0; a < m; ++a) {
+    for (long k = 0; k < Nk; ++k) tmp[k] = rk[k] * s.wk[a * Nk + k];
+    c2r3d(g, tmp.data(), n.data() + a * N);
+  }
+  const double* np[11];
+  for (int a = 0; a < 11; ++a) np[a] = n.data() + a * N;
+  std::vector<double> dphi(11 * N, 0.0);
+  int bad = 0;
+  std::vector<NSum> part(g_threads);
+#pragma omp parallel num_threads(g_threads)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    NSum loc;
+    FM meas, D;
+    double out[11];
+#pragma omp for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      measures_from_arrays(np, i, s.hsd, s.model.tensor(), meas);
+      const double phi = s.model.phi(meas);
+      if (meas.eta > 0.5 && 1 - meas.eta < 0.0) {  // FMT.cpp:289-290
+#pragma omp atomic write
+        bad = 1;
+      }
+      if (phi_out) phi_out[i] = phi;
+      s.model.dphi(meas, D);
+      collapse(D, s.hsd, s.model.tensor(), out);
+      for (int a = 0; a < 11; ++a) dphi[a * N + i] = out[a];
+      loc.add(phi);
+    }
+    part[t] = loc;
+  }
+  if (bad) return 1;
+  NSum F;
+  for (auto& p : part) { F.s += p.s; F.c += p.c; }  // Summation::operator+=(Summation), Summation.h:39-44
+  std::vector<cd> acc(Nk, cd(0, 0));
+  for (int a = 0; a < m; ++a) {
+    r2c3d(g, dphi.data() + a * N, tmp.data());
+    for (long k = 0; k < Nk; ++k) acc[k] += tmp[k] * s.wk[a * Nk + k];
+  }
+  std::vector<double> out(N);
+  c2r3d(g, acc.data(), out.data());
+  for (long i = 0; i < N; ++i) dF[i] += out[i] * dV;
+  *Fout = F.sum() * dV;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Mean field: LJ with WCA split (include/Potential1.h:23-189) and the
+// quadratic-force interpolation kernel Interaction_Interpolation_QF
+// (include/Interaction.h:416-431, src/Interaction.cpp:32-52, 178-307, 522-534).
+// ---------------------------------------------------------------------------
+struct LJ {
+  double sigma, eps, rcut, shift, rmin, Vmin, r_att_min;
+  double vr2(double r2) const {
+    const double y2 = sigma * sigma / r2;
+    const double y6 = y2 * y2 * y2;
+    return 4 * eps * (y6 * y6 - y6);
+  }
+  double vr(double r) const {
+    const double y = sigma / r, y3 = y * y * y, y6 = y3 * y3;
+    return 4 * eps * (y6 * y6 - y6);
+  }
+  void init(double s, double e, double rc) {  // Potential1.h:140-146
+    sigma = s; eps = e; rcut = rc; r_att_min = 0.0;
+    shift = rc < 0 ? 0.0 : vr(rc);
+    rmin = std::pow(2.0, 1.0 / 6.0) * sigma;
+    Vmin = vr(rmin) - shift;
+  }
+  double Watt2(double r2) const {  // Potential1.h:56-65 (WCA branch)
+    double ret = 0.0;
+    if (r2 < rcut * rcut) {
+      if (r2 < r_att_min * r_att_min) ret = 0.0;
+      else if (r2 < rmin * rmin) ret = Vmin;
+      else ret = vr2(r2) - shift;
+    }
+    return ret;
+  }
+};
+
+// Fills w_att (real space, x 1/kT, N doubles, zeroed) and returns a_vdw/kT.
+double mf_kernel(const Grid& g, double dx, double dy, double dz, const LJ& v, double kT, double* w_att) {
+  const double pt[3] = {-0.5, 0.0, 0.5};
+  const double vv[3] = {1.0 / 3, 1.0 / 3, 1.0 / 3};
+  const int lx = 2 + int(v.rcut / dx), ly = 2 + int(v.rcut / dy), lz = 2 + int(v.rcut / dz);
+  const double gf = dx * dx * dy * dy * dz * dz;
+  const double dV = dx * dy * dz;
+  double a_vdw = 0.0;
+  for (int ix = -lx; ix <= lx; ++ix)
+    for (int iy = -ly; iy <= ly; ++iy)
+      for (int iz = -lz; iz <= lz; ++iz) {
+        const int Sx = std::abs(ix), Sy = std::abs(iy), Sz = std::abs(iz);
+        double sum = 0.0;
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+              const double r2 = (Sx + pt[i]) * (Sx + pt[i]) * dx * dx + (Sy + pt[j]) * (Sy + pt[j]) * dy * dy +
+                                (Sz + pt[k]) * (Sz + pt[k]) * dz * dz;
+              sum += vv[i] * vv[j] * vv[k] * v.Watt2(r2);
+            }
+        const double w = gf * sum;
+        const long pos = pbc(iz, g.Nz) + g.Nz * (pbc(iy, g.Ny) + g.Ny * pbc(ix, g.Nx));
+        w_att[pos] += w / dV;
+        a_vdw += w;
+      }
+  a_vdw /= dV;
+  for (long i = 0; i < g.N(); ++i) w_att[i] /= kT;
+  return a_vdw / kT;
+}
+
+// ---------------------------------------------------------------------------
+// Whole DFT evaluation: DFT::calculateFreeEnergyAndDerivatives, src/DFT.cpp:360-425,
+// Species::externalField src/Species.cpp:51-62, Interaction_Base::getInteractionEnergyAndForces
+// src/Interaction.cpp:76-115.
+// ---------------------------------------------------------------------------
+struct DftState {
+  FmtState fmt;
+  bool has_fmt = true, ideal = true, has_mf = false;
+  std::vector<cd> wmf;  // r2c(w_att), no 1/N
+  double a_vdw = 0.0;
+  double mu = 0.0;
+  std::vector<double> vext;  // may be empty
+  double parts[5] = {0, 0, 0, 0, 0};  // F, F_id, F_hs, F_mf, F_ext
+};
+
+int dft_eval(DftState& s, const double* rho, double* dF) {
+  const Grid& g = s.fmt.g;
+  const long N = g.N(), Nk = g.Nk();
+  const double dV = s.fmt.dx * s.fmt.dy * s.fmt.dz;
+  std::fill(dF, dF + N, 0.0);
+  NSum Fid;
+  if (s.ideal)
+    for (long i = 0; i < N; ++i) {
+      const double d0 = rho[i];
+      Fid.add((d0 * std::log(d0) - d0) * dV);
+      dF[i] += std::log(d0) * dV;
+    }
+  double Fhs = 0.0;
+  if (s.has_fmt)
+    if (fmt_eval(s.fmt, rho, dF, &Fhs)) return 1;
+  double Fmf = 0.0;
+  if (s.has_mf) {
+    std::vector<cd> rk(Nk);
+    r2c3d(g, rho, rk.data());
+    for (long k = 0; k < Nk; ++k) rk[k] *= s.wmf[k];
+    std::vector<double> v(N);
+    c2r3d(g, rk.data(), v.data());
+    NSum E;
+    for (long i = 0; i < N; ++i) {
+      v[i] *= dV / (double)N;
+      dF[i] += v[i];
+      E.add(rho[i] * v[i]);
+    }
+    Fmf = 0.5 * E.sum();
+  }
+  // external field + chemical potential
+  NSum Nat;
+  double fdot = 0.0;
+  for (long i = 0; i < N; ++i) {
+    Nat.add(rho[i]);
+    if (!s.vext.empty()) fdot += s.vext[i] * rho[i];
+  }
+  const double Fext = fdot * dV - Nat.sum() * dV * s.mu;
+  for (long i = 0; i < N; ++i) {
+    if (!s.vext.empty()) dF[i] += s.vext[i] * dV;
+    dF[i] += -s.mu * dV;
+  }
+  NSum F;
+  F.add(Fid.sum());
+  F.add(Fhs);
+  F.add(Fmf);
+  F.add(Fext);
+  s.parts[0] = F.sum();
+  s.parts[1] = Fid.sum();
+  s.parts[2] = Fhs;
+  s.parts[3] = Fmf;
+  s.parts[4] = Fext;
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI used by oracle/oracle.py (ctypes).
+// ===========================================================================
+extern "C" {
+
+void orc_set_threads(int pointwise, int fft) {
+  g_threads = std::max(1, pointwise);
+  g_fft_threads = std::max(1, fft);
+}
+
+void orc_fft1d(int n, double* data, int sign) {
+  Plan1D p = make_plan(n);
+  std::vector<cd> scr(n);
+  fft1d(p, reinterpret_cast<cd*>(data), scr.data(), sign);
+}
+
+void orc_r2c3d(int Nx, int Ny, int Nz, const double* in, double* out) {
+  Grid g{Nx, Ny, Nz};
+  r2c3d(g, in, reinterpret_cast<cd*>(out));
+}
+
+void orc_c2r3d(int Nx, int Ny, int Nz, const double* in, double* out) {
+  Grid g{Nx, Ny, Nz};
+  c2r3d(g, reinterpret_cast<const cd*>(in), out);
+}
+
+// mode 0: reference-faithful (double); 1: well-conditioned, long double;
+// 2: well-conditioned, double.
+int orc_fmt_weights(int Nx, int Ny, int Nz, double dx, double dy, double dz, double hsd, double* w, int mode) {
+  Grid g{Nx, Ny, Nz};
+  std::fill(w, w + 11 * g.N(), 0.0);
+  if (mode == 1) return gen_weights<long double, true>(g, dx, dy, dz, hsd, w);
+  if (mode == 2) return gen_weights<double, true>(g, dx, dy, dz, hsd, w);
+  return gen_weights<double, false>(g, dx, dy, dz, hsd, w);
+}
+
+// Phi and the 19 derivatives at one set of measures (input in get_as_vector order).
+double orc_phi(int kind, double A, double B, const double* meas19, double* dphi19) {
+  Model m = make_model(kind, A, B);
+  FM f;
+  f.from19(meas19);
+  FM d;
+  m.dphi(f, d);
+  if (dphi19) d.to19(dphi19);
+  return m.phi(f);
+}
+
+// f1, f1', f2, f2', f3, f3' of a model at eta (include/FMT.h:139-643).
+void orc_fcoef(int kind, double eta, double* out6) {
+  Model m = make_model(kind, 1.0, 0.0);
+  out6[0] = m.f1(eta); out6[1] = m.f1p(eta); out6[2] = m.f2(eta);
+  out6[3] = m.f2p(eta); out6[4] = m.f3(eta); out6[5] = m.f3p(eta);
+}
+
+// FMT::BulkFex / BulkMuex for one species (src/FMT.cpp:616-709).
+void orc_bulk(int kind, double A, double B, double rho, double hsd, double* fex, double* muex) {
+  Model m = make_model(kind, A, B);
+  FM f;
+  f.uniform(rho, hsd);
+  *fex = m.phi(f);
+  FM d;
+  m.dphi(f, d);
+  FM wts;
+  wts.uniform(1.0, hsd);
+  double dv[19], wv[19];
+  d.to19(dv);
+  wts.to19(wv);
+  double mu = 0.0;
+  for (int i = 0; i < 19; ++i) mu += dv[i] * wv[i];
+  *muex = mu;
+}
+
+// One FMT F + dF evaluation.  weights: 11*N real-space weights or NULL.
+// dF is overwritten.  Returns 0, 1 (eta too large) or -1 (bad hsd).
+int orc_fmt_eval(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, double hsd,
+                 const double* weights, const double* rho, double* dF, double* F, double* phi_out) {
+  FmtState s;
+  s.g = Grid{Nx, Ny, Nz};
+  s.dx = dx; s.dy = dy; s.dz = dz; s.hsd = hsd;
+  s.model = make_model(kind, A, B);
+  if (!weights && hsd < std::min(std::min(dx, dy), dz)) return -1;
+  fmt_setup(s, weights);
+  std::fill(dF, dF + s.g.N(), 0.0);
+  return fmt_eval(s, rho, dF, F, phi_out);
+}
+
+// Real-space LJ/QF mean-field kernel (x 1/kT); returns a_vdw / kT.
+double orc_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy, double dz, double sigma, double eps, double rcut,
+                     double kT, double* w_att) {
+  Grid g{Nx, Ny, Nz};
+  std::fill(w_att, w_att + g.N(), 0.0);
+  LJ v;
+  v.init(sigma, eps, rcut);
+  return mf_kernel(g, dx, dy, dz, v, kT, w_att);
+}
+
+// ---- stateful DFT / FIRE2 / DDFT objects (opaque handles) ----
+struct orc_dft;
+
+orc_dft* orc_dft_create(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, double hsd,
+                        int has_fmt, int ideal, int has_mf, double sigma, double eps, double rcut, double kT, double mu,
+                        const double* vext) {
+  auto* s = new DftState();
+  s->fmt.g = Grid{Nx, Ny, Nz};
+  s->fmt.dx = dx; s->fmt.dy = dy; s->fmt.dz = dz; s->fmt.hsd = hsd;
+  s->fmt.model = make_model(kind, A, B);
+  s->has_fmt = has_fmt != 0;
+  s->ideal = ideal != 0;
+  s->has_mf = has_mf != 0;
+  s->mu = mu;
+  if (s->has_fmt) fmt_setup(s->fmt, nullptr);
+  const long N = s->fmt.g.N();
+  if (vext) s->vext.assign(vext, vext + N);
+  if (s->has_mf) {
+    std::vector<double> w(N, 0.0);
+    LJ v;
+    v.init(sigma, eps, rcut);
+    s->a_vdw = mf_kernel(s->fmt.g, dx, dy, dz, v, kT, w.data());
+    s->wmf.resize(s->fmt.g.Nk());
+    r2c3d(s->fmt.g, w.data(), s->wmf.data());
+  }
+  return reinterpret_cast<orc_dft*>(s);
+}
+
+void orc_dft_destroy(orc_dft* h) { delete reinterpret_cast<DftState*>(h); }
+
+double orc_dft_avdw(orc_dft* h) { return reinterpret_cast<DftState*>(h)->a_vdw; }
+
+// parts[5] = F, F_id, F_hs, F_mf, F_ext.  Returns 1 on eta too large.
+int orc_dft_eval(orc_dft* h, const double* rho, double* dF, double* parts) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  const int rc = dft_eval(*s, rho, dF);
+  if (parts) std::memcpy(parts, s->parts, sizeof(s->parts));
+  return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FIRE2 minimizer: include/Minimizer.h:13-182, src/Minimizer.cpp:18-327, alias
+// maps src/FMT_Species.cpp:29-93 (SMALL_VALUE = 1e-18, include/Density.h:24).
+// Single species; one object holds the full reference state so a trajectory
+// can be replayed step by step.
+// ---------------------------------------------------------------------------
+namespace {
+struct Fire {
+  DftState* dft;
+  long N;
+  double dV, hsd, c;
+  std::vector<double> x, v, rho, dF, dF_raw;
+  double dt = 1e-3, dt_max = 1e-2, dt_min = 0.0, dt_best = 0.0;
+  int N_delay = 5, N_P_positive = 0, N_P_negative = 0, N_P_negative_max = 20, backtracks = 0;
+  double alpha_start = 0.1, f_dec = 0.5, f_inc = 1.1, f_alf = 0.99, alpha = 0.1;
+  bool initial_delay = true;
+  unsigned it = 0;
+  double vv = 1.0, F = 0.0, fold = 0.0, fmax = -1.0, fmax_raw = 0.0, rms_force = 0.0, vnorm = 0.0;
+  bool first_step = true;
+  int calls = 0;
+  int last_status = 0;  // 0 ok, 1 backtracked (eta too large), 2 uphill limit
+
+  double etamin() const { return 1e-18 * (4 * M_PI * hsd * hsd * hsd / 3); }
+  void set_rho_from_alias() {  // FMT_Species.cpp:29-48
+    for (long i = 0; i < N; ++i) {
+      const double y = x[i];
+      rho[i] = 1e-18 + c * y * y / (1 + y * y);
+    }
+  }
+  void alias_from_rho() {  // FMT_Species.cpp:51-72
+    for (long i = 0; i < N; ++i) {
+      const double z = (rho[i] - 1e-18) / c;
+      x[i] = std::sqrt(std::fabs(z / (1 - z)));
+    }
+  }
+  // Minimizer::getDF_DX, Minimizer.cpp:90-115 (+ DFT.cpp:348-358). Returns 1 on eta too large.
+  int getDF_DX() {
+    set_rho_from_alias();
+    if (dft_eval(*dft, rho.data(), dF.data())) return 1;
+    calls++;
+    F = dft->parts[0];
+    dF_raw = dF;
+    for (long i = 0; i < N; ++i) {  // FMT_Species.cpp:74-93
+      const double y = x[i];
+      dF[i] = dF[i] * (2 * c * y / ((1 + y * y) * (1 + y * y)));
+    }
+    return 0;
+  }
+  static double norm2(const std::vector<double>& a) {
+    double s = 0;
+    for (double q : a) s += q * q;
+    return std::sqrt(s);
+  }
+  static double infn(const std::vector<double>& a) {
+    double s = 0;
+    for (double q : a) s = std::max(s, std::fabs(q));
+    return s;
+  }
+  // fireMinimizer2::SemiImplicitEuler, Minimizer.cpp:273-327
+  int semi_implicit_euler() {
+    for (long i = 0; i < N; ++i) v[i] += -dt * dF[i];
+    double vn = norm2(v), fn = norm2(dF);
+    double vnorm2 = vn * vn, fnorm2 = fn * fn;
+    rms_force = std::sqrt(fnorm2 / N);
+    vnorm = std::sqrt(vnorm2 / N);
+    const double mix = -alpha * std::sqrt(vnorm2 / fnorm2);
+    for (long i = 0; i < N; ++i) {
+      v[i] *= (1 - alpha);
+      v[i] += dF[i] * mix;
+      x[i] += v[i] * dt;
+    }
+    if (getDF_DX()) return 1;
+    fn = norm2(dF);
+    rms_force = std::sqrt(fn * fn / N);
+    fmax = infn(dF) / dV;
+    fmax_raw = infn(dF_raw) / dV;
+    return 0;
+  }
+  // fireMinimizer2::step, Minimizer.cpp:160-271 (snapshot I/O omitted)
+  int step() {
+    it++;
+    if (first_step) { fold = F; first_step = false; }  // static double fold = F_
+    NSum P;
+    {
+      double dot = 0.0;
+      for (long i = 0; i < N; ++i) dot += v[i] * dF[i];
+      P.add(-dot);
+    }
+    fold = F;
+    std::vector<double> x_rem = x, v_rem = v, dF_rem = dF, raw_rem = dF_raw;
+    last_status = 0;
+    if (P.sum() > 0) {
+      N_P_positive++;
+      N_P_negative = 0;
+      if (N_P_positive > N_delay) {
+        dt = std::min(dt * f_inc, dt_max);
+        alpha = alpha * f_alf;
+      }
+    } else {
+      N_P_positive = 0;
+      N_P_negative++;
+      if (N_P_negative > N_P_negative_max) { last_status = 2; return 2; }
+      if (!(initial_delay && it < (unsigned)N_delay)) {
+        if (dt * f_dec >= dt_min) dt *= f_dec;
+        alpha = alpha_start;
+      }
+      for (long i = 0; i < N; ++i) {
+        x[i] += v[i] * (-0.5 * dt);
+        v[i] *= 0.1;
+      }
+      backtracks++;
+    }
+    if (semi_implicit_euler() == 0) {
+      vv = vv * 0.9 + 0.1 * (std::fabs(F - fold) / dt);
+      dt_best = std::max(dt, dt_best);
+    } else {
+      x = x_rem;
+      v = v_rem;
+      for (long i = 0; i < N; ++i) v[i] *= 0.5;
+      dF = dF_rem;
+      dF_raw = raw_rem;
+      dt /= 2;
+      fmax = 1000;
+      backtracks++;
+      last_status = 1;
+    }
+    if (backtracks >= 10) {
+      dt_max = std::min(dt_best, 0.9 * dt_max);
+      dt_best = 0;
+      backtracks = 0;
+    }
+    return last_status;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// DDFT integrating-factor step: src/DDFT.cpp:42-351 (single species).
+// ---------------------------------------------------------------------------
+struct Ddft {
+  DftState* dft;
+  Grid g;
+  double dx, dy, dz;
+  double dt, time = 0.0, tol = 1e-4, dtMax = 1.0, RHS_max = 1.0, F = 0.0;
+  int successes = 0;
+  std::vector<double> Lx, Ly, Lz, fx, fy, fz;
+  std::vector<double> rho, dF;
+  std::vector<cd> R0, R1;
+  int calls = 0, restarts = 0;
+
+  long P(int ix, int iy, int iz) const { return pbc(iz, g.Nz) + g.Nz * (pbc(iy, g.Ny) + (long)g.Ny * pbc(ix, g.Nx)); }
+
+  int eval() {
+    calls++;
+    if (dft_eval(*dft, rho.data(), dF.data())) return 1;
+    F = dft->parts[0];
+    return 0;
+  }
+  void change_timestep(double ndt) {  // DDFT.cpp:184-207
+    if (std::fabs(dt - ndt) < 1e-10) return;
+    dt = ndt;
+    fx.resize(Lx.size()); fy.resize(Ly.size()); fz.resize(Lz.size());
+    for (size_t i = 0; i < Lx.size(); ++i) fx[i] = std::exp(Lx[i] * dt);
+    for (size_t i = 0; i < Ly.size(); ++i) fy[i] = std::exp(Ly[i] * dt);
+    for (size_t i = 0; i < Lz.size(); ++i) fz[i] = std::exp(Lz[i] * dt);
+  }
+  // g_dot_x (DDFT.cpp:320-351) / dV, minus the discrete Laplacian of rho
+  // (subtract_ideal_gas :223-235), then r2c (calculate_excess_RHS :212-220).
+  double excess_rhs(const std::vector<double>& d, std::vector<cd>& out) {
+    const long N = g.N();
+    const double D[3] = {1 / (2 * dx * dx), 1 / (2 * dy * dy), 1 / (2 * dz * dz)};
+    const double L[3] = {1 / (dx * dx), 1 / (dy * dy), 1 / (dz * dz)};
+    const double dV = dx * dy * dz;
+    std::vector<double> r(N);
+    double rmax = 0.0;
+    for (int ix = 0; ix < g.Nx; ++ix)
+      for (int iy = 0; iy < g.Ny; ++iy)
+        for (int iz = 0; iz < g.Nz; ++iz) {
+          const long p = P(ix, iy, iz);
+          const long px = P(ix + 1, iy, iz), mx = P(ix - 1, iy, iz), py = P(ix, iy + 1, iz), my = P(ix, iy - 1, iz),
+                     pz = P(ix, iy, iz + 1), mz = P(ix, iy, iz - 1);
+          const double x0 = dF[p], d0 = d[p];
+          double gx = D[0] * ((d[px] + d0) * (dF[px] - x0) - (d0 + d[mx]) * (x0 - dF[mx])) +
+                      D[1] * ((d[py] + d0) * (dF[py] - x0) - (d0 + d[my]) * (x0 - dF[my])) +
+                      D[2] * ((d[pz] + d0) * (dF[pz] - x0) - (d0 + d[mz]) * (x0 - dF[mz]));
+          gx *= 1.0 / dV;
+          rmax = std::max(rmax, std::fabs(gx));
+          r[p] = gx;
+        }
+    for (int ix = 0; ix < g.Nx; ++ix)
+      for (int iy = 0; iy < g.Ny; ++iy)
+        for (int iz = 0; iz < g.Nz; ++iz) {
+          const long p = P(ix, iy, iz);
+          const double d0 = d[p];
+          r[p] += -L[0] * (d[P(ix + 1, iy, iz)] + d[P(ix - 1, iy, iz)] - 2 * d0) -
+                  L[1] * (d[P(ix, iy + 1, iz)] + d[P(ix, iy - 1, iz)] - 2 * d0) -
+                  L[2] * (d[P(ix, iy, iz + 1)] + d[P(ix, iy, iz - 1)] - 2 * d0);
+        }
+    out.resize(g.Nk());
+    r2c3d(g, r.data(), out.data());
+    return rmax / dV;
+  }
+  // apply_diffusion_propogator, DDFT.cpp:239-292: d1 <- F^-1[U0 rho0_k + U1 R0 + U2 (R1 - R0)]/N
+  double propagate(const std::vector<double>& d0, std::vector<double>& d1) {
+    const long Nk = g.Nk(), N = g.N();
+    std::vector<cd> x(Nk), w(Nk);
+    r2c3d(g, d0.data(), x.data());
+    long pos = 0;
+    for (int ix = 0; ix < g.Nx; ++ix)
+      for (int iy = 0; iy < g.Ny; ++iy)
+        for (int iz = 0; iz < g.Zh(); ++iz, ++pos) {
+          const double Lam = Lx[ix] + Ly[iy] + Lz[iz];
+          const double e = fx[ix] * fy[iy] * fz[iz];
+          const double U0 = e;
+          const double U1 = pos == 0 ? dt : (e - 1) / Lam;
+          const double U2 = pos == 0 ? dt / 2 : (e - 1 - dt * Lam) / (Lam * Lam * dt);
+          w[pos] = U0 * x[pos] + U1 * R0[pos] + U2 * (R1[pos] - R0[pos]);
+        }
+    std::vector<double> out(N);
+    c2r3d(g, w.data(), out.data());
+    double maxdev = 0.0;
+    for (long i = 0; i < N; ++i) {
+      const double o = out[i] / (double)N;
+      maxdev = std::max(maxdev, std::fabs(d1[i] - o));
+      d1[i] = o;
+    }
+    return maxdev;
+  }
+  // DDFT::step, DDFT.cpp:113-182.  Returns 1 on eta too large.
+  int step() {
+    if (fx.size() != Lx.size()) { double t = dt; dt = 0; change_timestep(t); }
+    if (eval()) return 1;
+    std::vector<double> d0 = rho, d1 = rho;
+    RHS_max = excess_rhs(rho, R0);
+    bool restart;
+    bool decreased = false;
+    do {
+      double dev = 1;
+      restart = false;
+      for (int i = 0; i < 100 && dev > tol && !restart; ++i) {
+        rho = d1;
+        if (eval()) return 1;
+        excess_rhs(rho, R1);
+        rho = d0;
+        const double old = dev;
+        dev = propagate(d0, d1);
+        double mn = d1[0];
+        for (double q : d1) mn = std::min(mn, q);
+        if (mn < 0 || (i > 0 && old < dev)) restart = true;
+      }
+      if (restart || dev > tol) {
+        change_timestep(dt / 10);
+        d1 = d0;
+        decreased = true;
+        restart = true;
+        restarts++;
+      }
+    } while (restart);
+    time += dt;
+    if (decreased) successes = 0;
+    else successes++;
+    if (successes >= 5 && dt < dtMax) { change_timestep(std::min(2 * dt, dtMax)); successes = 0; }
+    rho = d1;
+    return eval();
+  }
+};
+}  // namespace
+
+extern "C" {
+struct orc_fire;
+struct orc_ddft;
+
+// FIRE2 on alias variables.  rho0: initial density (N).  Runs the constructor
+// (Minimizer.cpp:18-28, 117-147): alias from rho, first getDF_DX, v = 0.
+orc_fire* orc_fire_create(orc_dft* h, const double* rho0, double dt, double dt_max) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  auto* f = new Fire();
+  f->dft = s;
+  f->N = s->fmt.g.N();
+  f->dV = s->fmt.dx * s->fmt.dy * s->fmt.dz;
+  f->hsd = s->fmt.hsd;
+  f->c = (1.0 - f->etamin()) / f->dV;
+  f->rho.assign(rho0, rho0 + f->N);
+  f->x.assign(f->N, 0.0);
+  f->v.assign(f->N, 0.0);
+  f->dF.assign(f->N, 0.0);
+  f->dt = dt;
+  f->dt_max = dt_max;
+  f->dt_best = dt;
+  f->alpha = f->alpha_start;
+  f->alias_from_rho();
+  f->last_status = f->getDF_DX();
+  return reinterpret_cast<orc_fire*>(f);
+}
+void orc_fire_destroy(orc_fire* h) { delete reinterpret_cast<Fire*>(h); }
+// One fireMinimizer2::step.  info[8] = F, fmax(alias), fmax_raw, dt, alpha, vv, rms_force, calls
+int orc_fire_step(orc_fire* h, double* info) {
+  auto* f = reinterpret_cast<Fire*>(h);
+  const int rc = f->step();
+  if (info) {
+    info[0] = f->F; info[1] = f->fmax; info[2] = f->fmax_raw; info[3] = f->dt;
+    info[4] = f->alpha; info[5] = f->vv; info[6] = f->rms_force; info[7] = f->calls;
+  }
+  return rc;
+}
+void orc_fire_get(orc_fire* h, double* rho, double* x, double* v, double* dF_raw) {
+  auto* f = reinterpret_cast<Fire*>(h);
+  f->set_rho_from_alias();
+  if (rho) std::memcpy(rho, f->rho.data(), sizeof(double) * f->N);
+  if (x) std::memcpy(x, f->x.data(), sizeof(double) * f->N);
+  if (v) std::memcpy(v, f->v.data(), sizeof(double) * f->N);
+  if (dF_raw) std::memcpy(dF_raw, f->dF_raw.data(), sizeof(double) * f->N);
+}
+
+// DDFT constructor (DDFT.cpp:42-93): dt = 1e-4 dx^2 unless dt > 0 is given.
+orc_ddft* orc_ddft_create(orc_dft* h, const double* rho0, double dt, double dtMax, double tol) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  auto* d = new Ddft();
+  d->dft = s;
+  d->g = s->fmt.g;
+  d->dx = s->fmt.dx; d->dy = s->fmt.dy; d->dz = s->fmt.dz;
+  d->dt = dt > 0 ? dt : 0.0001 * d->dx * d->dx;
+  d->dtMax = dtMax;
+  d->tol = tol;
+  d->rho.assign(rho0, rho0 + d->g.N());
+  d->dF.assign(d->g.N(), 0.0);
+  d->eval();
+  const double Dx = 1 / (d->dx * d->dx), Dy = 1 / (d->dy * d->dy), Dz = 1 / (d->dz * d->dz);
+  for (int i = 0; i < d->g.Nx; ++i) d->Lx.push_back(2 * Dx * (std::cos(2 * M_PI * i / d->g.Nx) - 1));
+  for (int i = 0; i < d->g.Ny; ++i) d->Ly.push_back(2 * Dy * (std::cos(2 * M_PI * i / d->g.Ny) - 1));
+  for (int i = 0; i < d->g.Nz / 2 + 1; ++i) d->Lz.push_back(2 * Dz * (std::cos(2 * M_PI * i / d->g.Nz) - 1));
+  return reinterpret_cast<orc_ddft*>(d);
+}
+void orc_ddft_destroy(orc_ddft* h) { delete reinterpret_cast<Ddft*>(h); }
+// info[6] = F, dt, time, RHS_max, calls, restarts
+int orc_ddft_step(orc_ddft* h, double* info) {
+  auto* d = reinterpret_cast<Ddft*>(h);
+  const int rc = d->step();
+  if (info) {
+    info[0] = d->F; info[1] = d->dt; info[2] = d->time; info[3] = d->RHS_max; info[4] = d->calls; info[5] = d->restarts;
+  }
+  return rc;
+}
+void orc_ddft_get(orc_ddft* h, double* rho) {
+  auto* d = reinterpret_cast<Ddft*>(h);
+  std::memcpy(rho, d->rho.data(), sizeof(double) * d->g.N());
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+"""GPU parity of the FMT / DFT hot path against the CPU oracle (through the C ABI).
+
+Tolerances: the FP64 path must match the oracle to 1e-11 relative in F and in
+max-norm dF.  The weights are an input of the
+comparison: the reference's double-precision weight generator carries
+roundoff that alone moves F by ~1e-10 at dx = 0.1 (see
+tests/test_oracle_weights.py), so the FP64 parity cases hand both sides the
+same weights, and a separate case checks the device generator against the
+oracle's accurate (long double, well-conditioned) weights.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-11
+
+
+def seeded_density(shape, rho_b, seed, amp=0.05):
+    u = oracle.splitmix64_uniform(int(np.prod(shape)), seed).reshape(shape)
+    return rho_b * (1.0 + amp * (2.0 * u - 1.0))
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def maxrel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 18), (32, 32, 32), (48, 48, 48), (24, 16, 96), (64, 64, 64)])
+def test_fft_matches_numpy(cdft, shape):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(shape)
+    ref = np.fft.rfftn(x)
+    got = cdft.fft_r2c(x)
+    assert maxrel(got, ref) < 1e-13
+    back = cdft.fft_c2r(ref, shape[2])
+    assert maxrel(back, x * x.size) < 1e-13
+    gotm = cdft.fft_r2c(x, precision=cdft.MIXED)
+    assert maxrel(gotm, ref) < 2e-6
+
+
+def test_config1_fp64_parity_shared_weights(cdft):
+    """Config 1: 32^3, dx = 0.1, d = 1, White Bear I, rho = 0.6 (1 + 0.05 U[-1,1])."""
+    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
+    rho = seeded_density(shape, 0.6, 0x5EED0001)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    F0, dF0 = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=w)
+    sysm = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, weights=w)
+    F1, dF1 = sysm.fmt_eval(rho)
+    print("config1 F", F0, F1, rel(F1, F0), "dF", maxrel(dF1, dF0))
+    assert rel(F1, F0) <= TOL
+    assert maxrel(dF1, dF0) <= TOL
+
+
+@pytest.mark.parametrize("model,A,B", [(0, 1.0, 0.0), (1, 1.0, 0.0), (2, 1.3, -0.4), (3, 1.0, -1.0), (4, 1.0, -1.0)])
+def test_all_models_fp64_parity(cdft, model, A, B):
+    shape, dx, hsd = (16, 16, 16), 0.1, 1.0
+    rho = seeded_density(shape, 0.5, 0x5EED0100 + model, amp=0.2)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    F0, dF0 = oracle.fmt_eval(model, rho, dx, hsd, weights=w, A=A, B=B)
+    s = cdft.System(shape, dx, hsd, model=model, weights=w, A=A, B=B)
+    F1, dF1 = s.fmt_eval(rho)
+    assert rel(F1, F0) <= TOL
+    assert maxrel(dF1, dF0) <= TOL
+
+
+def test_device_weights_match_accurate_oracle(cdft):
+    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
+    s = cdft.System(shape, dx, hsd)
+    wg = s.device_weights()
+    wa = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_ACCURATE)
+    dV = dx ** 3
+    err = np.abs(wg - wa).max() / dV
+    print("max |w_gpu - w_accurate| / dV =", err)
+    # each cell value is a difference of O(R^6) G terms divided by dV, so
+    # double precision leaves ~1e-11 dV; the long-double oracle is ~1e-15 dV
+    assert err < 1e-10
+    # sum rules of the trilinear-hat partition of unity
+    assert abs(wg[0].sum() / (np.pi * hsd ** 3 / 6) - 1) < 1e-12
+    assert abs(wg[1].sum() / (np.pi * hsd ** 2) - 1) < 1e-12
+
+
+def test_config1_fp64_parity_own_weights(cdft):
+    """End to end: device-generated weights vs the oracle on its accurate weights."""
+    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
+    rho = seeded_density(shape, 0.6, 0x5EED0001)
+    wa = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_ACCURATE)
+    F0, dF0 = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=wa)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I)
+    F1, dF1 = s.fmt_eval(rho)
+    print("own weights F", rel(F1, F0), "dF", maxrel(dF1, dF0))
+    assert rel(F1, F0) <= TOL
+    assert maxrel(dF1, dF0) <= TOL
+
+
+def test_uniform_density_closed_form(cdft):
+    """Uniform rho: F/V = f_ex(rho) (Carnahan-Starling for WB I) and dF = dV mu_ex."""
+    shape, dx, hsd, rb = (32, 32, 32), 0.1, 1.0, 0.7
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I)
+    F, dF = s.fmt_eval(np.full(shape, rb))
+    eta = np.pi * rb / 6
+    cs = rb * eta * (4 - 3 * eta) / (1 - eta) ** 2
+    V = np.prod(shape) * dx ** 3
+    fex, muex = oracle.bulk(oracle.WBI, rb, hsd)
+    assert abs(F / V - cs) / cs < 1e-11
+    assert abs(fex - cs) / cs < 1e-13
+    assert np.abs(dF / dx ** 3 - muex).max() / muex < 1e-11
+
+
+def test_full_dft_parity(cdft):
+    """Ideal + WB I + LJ/QF mean field + V_ext - mu against the oracle DFT."""
+    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
+    rho = seeded_density(shape, 0.5, 0x5EED0200, amp=0.1)
+    z = np.arange(shape[2]) * dx
+    vext = np.broadcast_to(np.exp(-z / 0.2) + np.exp(-(shape[2] * dx - z) / 0.2), shape).copy()
+    kT, mu = 0.8, -1.3
+    w = oracle.fmt_weights(shape, dx, hsd)
+    o = oracle.DFT(shape, dx, hsd, model=oracle.WBI, mf=True, kT=kT, mu=mu, vext=vext)
+    p0, dF0 = o.eval(rho)
+    wk, a0 = oracle.mf_kernel(shape, dx, kT=kT)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mu=mu, vext=vext, mean_field=True,
+                    kT=kT)
+    assert maxrel(s.device_mf_kernel(), wk) < 1e-13
+    assert rel(s.a_vdw, a0) < 1e-13
+    assert rel(s.a_vdw, o.a_vdw) < 1e-13
+    p1, dF1 = s.eval(rho)
+    print("parts oracle", p0, "gpu", p1)
+    for k in range(5):
+        assert abs(p1[k] - p0[k]) <= TOL * abs(p0[0]) + 1e-300
+    assert maxrel(dF1, dF0) <= TOL
+
+
+def test_eta_too_large_is_reported(cdft):
+    shape, dx = (16, 16, 16), 0.1
+    rho = np.full(shape, 0.3)
+    rho[3, 4, 5] = 2.0 / dx ** 3 * 10
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I)
+    with pytest.raises(cdft.EtaTooLarge):
+        s.fmt_eval(rho)
+    with pytest.raises(oracle.EtaTooLarge):
+        oracle.fmt_eval(oracle.WBI, rho, dx, 1.0)
+    # the system stays usable afterwards
+    F, _ = s.fmt_eval(np.full(shape, 0.3))
+    assert np.isfinite(F)
+
+
+def test_accumulate_flag(cdft):
+    shape, dx = (16, 16, 16), 0.1
+    rho = seeded_density(shape, 0.4, 7)
+    s = cdft.System(shape, dx, 1.0)
+    _, d1 = s.fmt_eval(rho)
+    base = np.full(shape, 3.0)
+    _, d2 = s.fmt_eval(rho, dF=base.copy(), accumulate=True)
+    assert np.abs(d2 - 3.0 - d1).max() < 1e-14 * np.abs(d1).max() + 1e-15
+
+
+def test_mixed_precision_error(cdft):
+    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
+    rho = seeded_density(shape, 0.6, 0x5EED0001)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    F0, dF0 = cdft.System(shape, dx, hsd, weights=w).fmt_eval(rho)
+    F1, dF1 = cdft.System(shape, dx, hsd, weights=w, precision=cdft.MIXED).fmt_eval(rho)
+    print("mixed F rel err", rel(F1, F0), "dF", maxrel(dF1, dF0))
+    assert rel(F1, F0) < 1e-6
+    assert maxrel(dF1, dF0) < 1e-5
