@@ -10,7 +10,7 @@ CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT = 0, 1, 2
 CDFT_BAD_ARG, CDFT_HIP_ERROR, CDFT_NOT_SUPPORTED = -1, -2, -3
 MODEL_NONE, ROSENFELD, RSLT, ESFMT, WHITEBEAR_I, WHITEBEAR_II = -1, 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
-DEVICE_PTRS, ACCUMULATE = 1, 2
+DEVICE_PTRS, ACCUMULATE, FMT_ONLY = 1, 2, 4
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -66,6 +66,13 @@ def lib():
         L.cdft_system_get_mf_kernel.argtypes = [_p, _p]
         L.cdft_fft_r2c.argtypes = [_i, _i, _i, _i, _p, _p]
         L.cdft_fft_c2r.argtypes = [_i, _i, _i, _i, _p, _p]
+        L.cdft_system_eval_async.argtypes = [_p, _p, _p, _i]
+        L.cdft_system_fetch.argtypes = [_p, _p]
+        L.cdft_system_time.argtypes = [_p, _p, _p, _i, _i, ctypes.POINTER(_d)]
+        L.cdft_malloc.argtypes = [ctypes.POINTER(_p), ctypes.c_ulonglong]
+        L.cdft_free.argtypes = [_p]
+        L.cdft_memcpy_h2d.argtypes = [_p, _p, ctypes.c_ulonglong]
+        L.cdft_memcpy_d2h.argtypes = [_p, _p, ctypes.c_ulonglong]
         L.cdft_system_set_timing.argtypes = [_p, _i]
         L.cdft_system_get_timing.argtypes = [_p, ctypes.POINTER(_i), _p, ctypes.POINTER(ctypes.c_char_p)]
         _LIB = L
@@ -161,6 +168,22 @@ class System:
         _check(lib().cdft_system_eval(self.h, _p(rho_ptr), _p(dF_ptr), _ptr(p), flags))
         return p
 
+    def eval_async(self, rho_ptr, dF_ptr, fmt_only=False, accumulate=False):
+        flags = (FMT_ONLY if fmt_only else 0) | (ACCUMULATE if accumulate else 0)
+        _check(lib().cdft_system_eval_async(self.h, _p(rho_ptr), _p(dF_ptr), flags))
+
+    def fetch(self):
+        p = np.zeros(5)
+        _check(lib().cdft_system_fetch(self.h, _ptr(p)))
+        return p
+
+    def time(self, rho_ptr, dF_ptr, iters, fmt_only=False):
+        """Milliseconds for `iters` back-to-back evaluations (hipEvents on the system stream)."""
+        ms = _d()
+        flags = FMT_ONLY if fmt_only else 0
+        _check(lib().cdft_system_time(self.h, _p(rho_ptr), _p(dF_ptr), flags, int(iters), ctypes.byref(ms)))
+        return ms.value
+
     @property
     def stream(self):
         s = _p()
@@ -192,6 +215,45 @@ class System:
         names = (ctypes.c_char_p * 16)()
         _check(lib().cdft_system_get_timing(self.h, ctypes.byref(n), _ptr(ms), names))
         return [(names[i].decode(), float(ms[i])) for i in range(n.value)]
+
+
+class DeviceArray:
+    """A float64 device buffer allocated through the C ABI (no second HIP runtime)."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = _p()
+        _check(lib().cdft_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.nbytes == self.nbytes
+        _check(lib().cdft_memcpy_h2d(_p(self.ptr), _ptr(a), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape)
+        _check(lib().cdft_memcpy_d2h(_ptr(out), _p(self.ptr), self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().cdft_free(_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def fft_r2c(x, precision=FP64):

@@ -130,6 +130,7 @@ struct cdft_system_s {
   double a_vdw = 0.0;
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
   bool has_vext = false;
+  bool last_with_dft = true;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -341,6 +342,7 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
   s->partials.alloc(sizeof(double2) * maxblk);
   s->partials2.alloc(sizeof(double2) * maxblk * 4);
   s->flag.alloc(sizeof(int));
+  HIPCHK(hipMemset(s->flag.p, 0, sizeof(int)));
   s->results.alloc(sizeof(double) * 8);
   if (desc->vext) {
     s->vext.alloc(sizeof(double) * s->N);
@@ -384,7 +386,6 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
   cplx<T>* A = s.A.as<cplx<T>>();
   cplx<T>* B = s.B.as<cplx<T>>();
   const Tables<T>& tw = tables<T>(s);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s.stream, s.flag.as<int>());
   if (nconv > 0) {
     launch_rows_r2c<T, double>(s, rho, 0, A, 0, 1);
     s.mark("z_r2c");
@@ -463,31 +464,27 @@ double csum(std::initializer_list<double> v) {
   return s + c;
 }
 
-int run_eval(cdft_system s, const double* rho, double* dF, double* parts, int flags, bool with_dft) {
-  REQUIRE(s && rho && dF, "null argument");
-  const bool dev = flags & CDFT_DEVICE_PTRS;
-  const bool acc = flags & CDFT_ACCUMULATE;
-  const size_t bytes = sizeof(double) * s->N;
-  const double* rho_d = rho;
-  double* dF_d = dF;
-  if (!dev) {
-    HIPCHK(hipMemcpyAsync(s->rho.p, rho, bytes, hipMemcpyHostToDevice, s->stream));
-    if (acc) HIPCHK(hipMemcpyAsync(s->dF.p, dF, bytes, hipMemcpyHostToDevice, s->stream));
-    rho_d = s->rho.as<double>();
-    dF_d = s->dF.as<double>();
-  }
+// Enqueue one evaluation on device pointers (no synchronisation).
+void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc) {
   if (s->mixed) eval_device<float>(*s, rho_d, dF_d, with_dft, acc);
   else eval_device<double>(*s, rho_d, dF_d, with_dft, acc);
+  s->last_with_dft = with_dft;
+}
+
+// Wait for the stream and turn the device results into the F parts.
+int fetch_parts(cdft_system s, double* parts) {
   double res[5];
   int fl = 0;
   HIPCHK(hipMemcpyAsync(res, s->results.p, sizeof(double) * 5, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipMemcpyAsync(&fl, s->flag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-  if (!dev) HIPCHK(hipMemcpyAsync(dF, s->dF.p, bytes, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   if (fl) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s->stream, s->flag.as<int>());
+    HIPCHK(hipStreamSynchronize(s->stream));
     g_err = "Eta too large in FMT::dPHI";
     return CDFT_ETA_TOO_LARGE;
   }
+  const bool with_dft = s->last_with_dft;
   const double dV = s->d.dx * s->d.dy * s->d.dz;
   const double F_hs = s->m > 0 ? res[0] * dV : 0.0;
   const double F_id = (with_dft && s->d.ideal) ? res[1] : 0.0;
@@ -502,6 +499,24 @@ int run_eval(cdft_system s, const double* rho, double* dF, double* parts, int fl
     parts[4] = F_ext;
   }
   return CDFT_OK;
+}
+
+int run_eval(cdft_system s, const double* rho, double* dF, double* parts, int flags, bool with_dft) {
+  REQUIRE(s && rho && dF, "null argument");
+  const bool dev = flags & CDFT_DEVICE_PTRS;
+  const bool acc = flags & CDFT_ACCUMULATE;
+  const size_t bytes = sizeof(double) * s->N;
+  const double* rho_d = rho;
+  double* dF_d = dF;
+  if (!dev) {
+    HIPCHK(hipMemcpyAsync(s->rho.p, rho, bytes, hipMemcpyHostToDevice, s->stream));
+    if (acc) HIPCHK(hipMemcpyAsync(s->dF.p, dF, bytes, hipMemcpyHostToDevice, s->stream));
+    rho_d = s->rho.as<double>();
+    dF_d = s->dF.as<double>();
+  }
+  enqueue_eval(s, rho_d, dF_d, with_dft, acc);
+  if (!dev) HIPCHK(hipMemcpyAsync(dF, s->dF.p, bytes, hipMemcpyDeviceToHost, s->stream));
+  return fetch_parts(s, parts);
 }
 
 template <typename F>
@@ -617,6 +632,73 @@ int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, 
     const int rc = run_eval(sys, rho, dF, parts, flags, false);
     if (rc == CDFT_OK && F_hs) *F_hs = parts[2];
     return rc;
+  });
+}
+
+int cdft_system_eval_async(cdft_system sys, const double* rho_d, double* dF_d, int flags) {
+  return guarded([&] {
+    REQUIRE(sys && rho_d && dF_d, "null argument");
+    const bool fmt_only = flags & CDFT_FMT_ONLY;
+    REQUIRE(!fmt_only || sys->m > 0, "system has no FMT functional");
+    enqueue_eval(sys, rho_d, dF_d, !fmt_only, (flags & CDFT_ACCUMULATE) != 0);
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_fetch(cdft_system sys, double* parts) {
+  return guarded([&] {
+    REQUIRE(sys, "null argument");
+    return fetch_parts(sys, parts);
+  });
+}
+
+int cdft_system_time(cdft_system sys, const double* rho_d, double* dF_d, int flags, int iters, double* ms) {
+  return guarded([&] {
+    REQUIRE(sys && rho_d && dF_d && ms && iters > 0, "bad argument");
+    const bool fmt_only = flags & CDFT_FMT_ONLY;
+    REQUIRE(!fmt_only || sys->m > 0, "system has no FMT functional");
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, sys->stream));
+    for (int i = 0; i < iters; ++i) enqueue_eval(sys, rho_d, dF_d, !fmt_only, (flags & CDFT_ACCUMULATE) != 0);
+    HIPCHK(hipEventRecord(e1, sys->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t;
+    return CDFT_OK;
+  });
+}
+
+int cdft_malloc(void** p, unsigned long long bytes) {
+  return guarded([&] {
+    REQUIRE(p, "null argument");
+    HIPCHK(hipMalloc(p, bytes));
+    return CDFT_OK;
+  });
+}
+
+int cdft_free(void* p) {
+  return guarded([&] {
+    if (p) HIPCHK(hipFree(p));
+    return CDFT_OK;
+  });
+}
+
+int cdft_memcpy_h2d(void* dst, const void* src, unsigned long long bytes) {
+  return guarded([&] {
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return CDFT_OK;
+  });
+}
+
+int cdft_memcpy_d2h(void* dst, const void* src, unsigned long long bytes) {
+  return guarded([&] {
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return CDFT_OK;
   });
 }
 

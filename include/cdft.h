@@ -50,7 +50,8 @@ enum cdft_precision {
 
 enum cdft_flags {
   CDFT_DEVICE_PTRS = 1, /* rho / dF / vext are device pointers */
-  CDFT_ACCUMULATE = 2   /* add into dF instead of overwriting it */
+  CDFT_ACCUMULATE = 2,  /* add into dF instead of overwriting it */
+  CDFT_FMT_ONLY = 4     /* hard-sphere part only (cdft_system_eval_async / _time) */
 };
 
 typedef struct cdft_system_s* cdft_system;
@@ -90,6 +91,22 @@ int cdft_system_eval(cdft_system sys, const double* rho, double* dF, double* par
    the hard-sphere part; returns F_hs (already x dV) in *F_hs and adds (or
    writes) the FMT dF. */
 int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, int flags);
+
+/* Asynchronous form on device pointers: enqueues one evaluation on the
+   system's stream and returns without synchronising.  cdft_system_fetch
+   waits for the stream and returns the parts of the last evaluation
+   (CDFT_ETA_TOO_LARGE if any evaluation since the previous fetch saw eta > 1). */
+int cdft_system_eval_async(cdft_system sys, const double* rho_d, double* dF_d, int flags);
+int cdft_system_fetch(cdft_system sys, double* parts);
+/* Enqueues `iters` asynchronous evaluations between two hipEvents recorded on
+   the system's stream; *ms = elapsed time of the whole batch. */
+int cdft_system_time(cdft_system sys, const double* rho_d, double* dF_d, int flags, int iters, double* ms);
+
+/* Device memory plumbing for callers without their own HIP runtime. */
+int cdft_malloc(void** p, unsigned long long bytes);
+int cdft_free(void* p);
+int cdft_memcpy_h2d(void* dst, const void* src, unsigned long long bytes);
+int cdft_memcpy_d2h(void* dst, const void* src, unsigned long long bytes);
 
 /* Introspection / plumbing. */
 int cdft_system_stream(cdft_system sys, void** hip_stream);

@@ -46,6 +46,11 @@ def lib():
         L.orc_fcoef.argtypes = [_i, _d, _p]
         L.orc_fmt_eval.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _p, _p, _p, _p, _p]
         L.orc_fmt_eval.restype = _i
+        L.orc_fmt_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _p]
+        L.orc_fmt_create.restype = _p
+        L.orc_fmt_run.argtypes = [_p, _p, _p, _p]
+        L.orc_fmt_run.restype = _i
+        L.orc_fmt_destroy.argtypes = [_p]
         L.orc_mf_kernel.argtypes = [_i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _p]
         L.orc_mf_kernel.restype = _d
         L.orc_dft_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _i, _i, _i, _d, _d, _d, _d, _d, _p]
@@ -161,6 +166,29 @@ def fmt_eval(model, rho, dx, hsd, weights=None, A=1.0, B=0.0, want_phi=False):
     if rc != 0:
         raise ValueError("bad arguments")
     return (F.value, dF, ph) if want_phi else (F.value, dF)
+
+
+class FMT:
+    """Handle form of the FMT oracle: spectra built once, evaluations timed alone."""
+
+    def __init__(self, model, shape, dx, hsd, weights=None, A=1.0, B=0.0):
+        self.shape = tuple(shape)
+        dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        self.h = lib().orc_fmt_create(model, A, B, *self.shape, *dxs, hsd, _ptr(w))
+
+    def eval(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        dF = np.zeros_like(rho)
+        F = _d()
+        if lib().orc_fmt_run(self.h, _ptr(rho), _ptr(dF), ctypes.byref(F)) == 1:
+            raise EtaTooLarge("eta too large")
+        return F.value, dF
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_fmt_destroy(self.h)
+            self.h = None
 
 
 def mf_kernel(shape, dx, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0):

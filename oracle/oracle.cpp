@@ -953,6 +953,24 @@ int orc_fmt_eval(int kind, double A, double B, int Nx, int Ny, int Nz, double dx
   return fmt_eval(s, rho, dF, F, phi_out);
 }
 
+// Handle form: weight spectra built once (create), evaluations timed alone (run).
+struct orc_fmt;
+orc_fmt* orc_fmt_create(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz,
+                        double hsd, const double* weights) {
+  auto* s = new FmtState();
+  s->g = Grid{Nx, Ny, Nz};
+  s->dx = dx; s->dy = dy; s->dz = dz; s->hsd = hsd;
+  s->model = make_model(kind, A, B);
+  fmt_setup(*s, weights);
+  return reinterpret_cast<orc_fmt*>(s);
+}
+int orc_fmt_run(orc_fmt* h, const double* rho, double* dF, double* F) {
+  auto* s = reinterpret_cast<FmtState*>(h);
+  std::fill(dF, dF + s->g.N(), 0.0);
+  return fmt_eval(*s, rho, dF, F);
+}
+void orc_fmt_destroy(orc_fmt* h) { delete reinterpret_cast<FmtState*>(h); }
+
 // Real-space LJ/QF mean-field kernel (x 1/kT); returns a_vdw / kT.
 double orc_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy, double dz, double sigma, double eps, double rcut,
                      double kT, double* w_att) {
