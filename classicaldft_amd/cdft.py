@@ -40,6 +40,26 @@ class _Desc(ctypes.Structure):
     ]
 
 
+class FireParams(ctypes.Structure):
+    _fields_ = [("dt", _d), ("dt_max", _d), ("dt_min", _d), ("alpha_start", _d), ("f_dec", _d), ("f_inc", _d),
+                ("f_alf", _d), ("N_delay", _i), ("N_P_negative_max", _i), ("initial_delay", _i)]
+
+
+class FireReport(ctypes.Structure):
+    _fields_ = [("F", _d), ("fmax_alias", _d), ("fmax_raw", _d), ("dt", _d), ("alpha", _d), ("vv", _d),
+                ("rms_force", _d), ("monitor", _d), ("steps", _i), ("calls", _i)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DdftReport(ctypes.Structure):
+    _fields_ = [("F", _d), ("dt", _d), ("time", _d), ("rhs_max", _d), ("mass", _d), ("calls", _i), ("restarts", _i)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def library_path():
     return os.path.join(_HERE, "libcdft.so")
 
@@ -73,6 +93,18 @@ def lib():
         L.cdft_free.argtypes = [_p]
         L.cdft_memcpy_h2d.argtypes = [_p, _p, ctypes.c_ulonglong]
         L.cdft_memcpy_d2h.argtypes = [_p, _p, ctypes.c_ulonglong]
+        L.cdft_system_bulk.argtypes = [_p, _d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
+        L.cdft_fire_default_params.argtypes = [ctypes.POINTER(FireParams)]
+        L.cdft_fire_create.argtypes = [_p, _p, _i, ctypes.POINTER(FireParams), ctypes.POINTER(_p)]
+        L.cdft_fire_step.argtypes = [_p, ctypes.POINTER(FireReport)]
+        L.cdft_fire_run.argtypes = [_p, _i, _d, ctypes.POINTER(FireReport)]
+        L.cdft_fire_get.argtypes = [_p, _p, _p, _p, _p]
+        L.cdft_fire_destroy.argtypes = [_p]
+        L.cdft_ddft_create.argtypes = [_p, _p, _d, _d, _d, ctypes.POINTER(_p)]
+        L.cdft_ddft_step.argtypes = [_p, ctypes.POINTER(DdftReport)]
+        L.cdft_ddft_set_max_time_step.argtypes = [_p, _d]
+        L.cdft_ddft_get_density.argtypes = [_p, _p]
+        L.cdft_ddft_destroy.argtypes = [_p]
         L.cdft_system_set_timing.argtypes = [_p, _i]
         L.cdft_system_get_timing.argtypes = [_p, ctypes.POINTER(_i), _p, ctypes.POINTER(ctypes.c_char_p)]
         _LIB = L
@@ -190,6 +222,12 @@ class System:
         _check(lib().cdft_system_stream(self.h, ctypes.byref(s)))
         return s.value or 0
 
+    def bulk(self, rho):
+        """(beta f, beta mu) at uniform density: DFT::fhelmholtz_times_beta_over_volume, mu_times_beta."""
+        f, mu = _d(), _d()
+        _check(lib().cdft_system_bulk(self.h, float(rho), ctypes.byref(f), ctypes.byref(mu)))
+        return f.value, mu.value
+
     @property
     def a_vdw(self):
         a = _d()
@@ -215,6 +253,93 @@ class System:
         names = (ctypes.c_char_p * 16)()
         _check(lib().cdft_system_get_timing(self.h, ctypes.byref(n), _ptr(ms), names))
         return [(names[i].decode(), float(ms[i])) for i in range(n.value)]
+
+
+class Fire:
+    """fireMinimizer2 on the device (src/Minimizer.cpp:117-327)."""
+
+    def __init__(self, system, rho0, dt=None, dt_max=None, **params):
+        self.system = system
+        p = FireParams()
+        _check(lib().cdft_fire_default_params(ctypes.byref(p)))
+        if dt is not None:
+            p.dt = dt
+            p.dt_max = 10 * dt if dt_max is None else dt_max
+        elif dt_max is not None:
+            p.dt_max = dt_max
+        for k, v in params.items():
+            setattr(p, k, v)
+        rho0 = np.ascontiguousarray(rho0, dtype=np.float64)
+        h = _p()
+        _check(lib().cdft_fire_create(system.h, _ptr(rho0), 0, ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+
+    def step(self):
+        """Returns (status, report): status 0 ok, 1 back-tracked (eta > 1), 2 uphill limit."""
+        r = FireReport()
+        rc = lib().cdft_fire_step(self.h, ctypes.byref(r))
+        if rc < 0:
+            _check(rc)
+        return rc, r.as_dict()
+
+    def run(self, max_steps, tol_raw):
+        r = FireReport()
+        rc = lib().cdft_fire_run(self.h, int(max_steps), float(tol_raw), ctypes.byref(r))
+        if rc < 0:
+            _check(rc)
+        return rc, r.as_dict()
+
+    def state(self):
+        shp = self.system.shape
+        rho, df, x, v = (np.zeros(shp) for _ in range(4))
+        _check(lib().cdft_fire_get(self.h, _ptr(rho), _ptr(df), _ptr(x), _ptr(v)))
+        return rho, x, v, df
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_fire_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DDFT:
+    """Integrating-factor DDFT on the device (src/DDFT.cpp:42-351)."""
+
+    def __init__(self, system, rho0, dt=0.0, dt_max=0.0, tol=0.0):
+        self.system = system
+        rho0 = np.ascontiguousarray(rho0, dtype=np.float64)
+        h = _p()
+        _check(lib().cdft_ddft_create(system.h, _ptr(rho0), float(dt), float(dt_max), float(tol), ctypes.byref(h)))
+        self.h = h
+
+    def step(self):
+        r = DdftReport()
+        _check(lib().cdft_ddft_step(self.h, ctypes.byref(r)))
+        return r.as_dict()
+
+    def set_max_time_step(self, dt_max):
+        _check(lib().cdft_ddft_set_max_time_step(self.h, float(dt_max)))
+
+    def density(self):
+        out = np.zeros(self.system.shape)
+        _check(lib().cdft_ddft_get_density(self.h, _ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_ddft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceArray:

@@ -4,203 +4,12 @@
 // each of the m FMT weights and the mean-field kernel), the spectral work
 // arrays, the twiddle tables and a HIP stream.  cdft_system_eval runs the
 // fused pipeline documented in kernels.hpp; no C++ exception crosses the ABI.
-#include <hip/hip_runtime.h>
+#include "system.hpp"
 
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <memory>
-#include <stdexcept>
-#include <string>
-#include <vector>
-
-#include "../../include/cdft.h"
-#include "kernels.hpp"
-#include "weights.hpp"
-
-using namespace cdft;
-
-namespace {
+namespace cdft_impl {
 
 thread_local std::string g_err;
-
-struct CdftError : std::runtime_error {
-  int code;
-  CdftError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(expr)                                                                                      \
-  do {                                                                                                    \
-    hipError_t e_ = (expr);                                                                               \
-    if (e_ != hipSuccess)                                                                                 \
-      throw CdftError(CDFT_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_) + " (line " +    \
-                                          std::to_string(__LINE__) + ")");                                \
-  } while (0)
-
-#define REQUIRE(cond, msg) \
-  do {                     \
-    if (!(cond)) throw CdftError(CDFT_BAD_ARG, msg); \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  ~DevBuf() { release(); }
-  void alloc(size_t b) {
-    release();
-    if (b) HIPCHK(hipMalloc(&p, b));
-    bytes = b;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <typename U>
-  U* as() const { return reinterpret_cast<U*>(p); }
-};
-
-bool smooth23(int n) {
-  if (n < 1) return false;
-  while (n % 2 == 0) n /= 2;
-  while (n % 3 == 0) n /= 3;
-  return n == 1;
-}
-
-Radices make_radices(int L) {
-  Radices r{};
-  r.L = L;
-  int m = L;
-  int k = 0;
-  while (m % 8 == 0) { r.r[k++] = 8; m /= 8; }
-  if (m % 4 == 0) { r.r[k++] = 4; m /= 4; }
-  if (m % 2 == 0) { r.r[k++] = 2; m /= 2; }
-  while (m % 3 == 0) { r.r[k++] = 3; m /= 3; }
-  if (m != 1) throw CdftError(CDFT_BAD_ARG, "transform length " + std::to_string(L) + " is not a product of 2 and 3");
-  r.npass = k;
-  return r;
-}
-
-// W_L^j = exp(-2 pi i j / L), j in [0, L), in extended precision.
-template <typename T>
-void upload_twiddles(DevBuf& buf, int L) {
-  std::vector<cplx<T>> h(L);
-  const long double tp = 6.283185307179586476925286766559005768L;
-  for (int j = 0; j < L; ++j) {
-    const long double a = -tp * (long double)j / (long double)L;
-    h[j] = {T(cosl(a)), T(sinl(a))};
-  }
-  buf.alloc(sizeof(cplx<T>) * L);
-  HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cplx<T>) * L, hipMemcpyHostToDevice));
-}
-
-template <typename T>
-struct Tables {
-  DevBuf x, y, m, n;  // Nx, Ny, Nz/2, Nz
-  void build(const Geo& g) {
-    upload_twiddles<T>(x, g.Nx);
-    upload_twiddles<T>(y, g.Ny);
-    upload_twiddles<T>(m, std::max(1, g.Nz / 2));
-    upload_twiddles<T>(n, g.Nz);
-  }
-};
-
-int div_up(long a, long b) { return int((a + b - 1) / b); }
-
 __global__ void k_zero_int(int* p) { *p = 0; }
-
-}  // namespace
-
-struct cdft_system_s {
-  cdft_system_desc d{};
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Geo g{};
-  long N = 0, Nkp = 0, nrows = 0;
-  int m = 0, nconv = 0;
-  unsigned odd_mask = 0x1Cu;  // arrays 2, 3, 4 (vector weights) have imaginary spectra
-  bool mixed = false;
-  Radices radX{}, radY{}, radM{};
-  Tables<double> twd;
-  Tables<float> twf;
-  DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
-  double a_vdw = 0.0;
-  int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
-  bool has_vext = false;
-  bool last_with_dft = true;
-  // timing
-  bool timing = false;
-  std::vector<hipEvent_t> ev;
-  std::vector<const char*> ev_names;
-  int n_marks = 0;
-
-  ~cdft_system_s() {
-    for (auto e : ev) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-
-  void mark(const char* name) {
-    if (!timing) return;
-    if (n_marks < (int)ev.size()) {
-      HIPCHK(hipEventRecord(ev[n_marks], stream));
-      ev_names[n_marks] = name;
-      ++n_marks;
-    }
-  }
-};
-
-namespace {
-
-template <typename T>
-const Tables<T>& tables(const cdft_system_s& s);
-template <>
-const Tables<double>& tables<double>(const cdft_system_s& s) { return s.twd; }
-template <>
-const Tables<float>& tables<float>(const cdft_system_s& s) { return s.twf; }
-
-// ------------------------------- transforms --------------------------------
-template <typename T, typename TI>
-void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride, cplx<T>* out, long out_stride, int narr) {
-  const Tables<T>& tw = tables<T>(s);
-  const int M = s.g.Nz / 2;
-  const int RB = s.RBrow;
-  dim3 grid(div_up(s.nrows, RB), narr);
-  hipLaunchKernelGGL((k_rows_r2c<T, TI, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
-                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
-  HIPCHK(hipGetLastError());
-}
-
-template <typename T, typename TO>
-void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_stride, TO* out, long out_stride, int narr,
-                     double scale) {
-  const Tables<T>& tw = tables<T>(s);
-  const int M = s.g.Nz / 2;
-  const int RB = s.RBrow;
-  dim3 grid(div_up(s.nrows, RB), narr);
-  hipLaunchKernelGGL((k_rows_c2r<T, TO, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
-                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB,
-                     scale);
-  HIPCHK(hipGetLastError());
-}
-
-template <typename T>
-void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd) {
-  const Tables<T>& tw = tables<T>(s);
-  const Radices& rad = axis == 0 ? s.radX : s.radY;
-  const cplx<T>* t = (axis == 0 ? tw.x : tw.y).template as<cplx<T>>();
-  const int TW = s.TWcol;
-  dim3 grid(div_up(s.g.Zh, TW), axis == 0 ? s.g.Ny : s.g.Nx, narr);
-  const size_t smem = sizeof(cplx<T>) * rad.L * TW;
-  if (fwd)
-    hipLaunchKernelGGL((k_cols<T, 16, true>), grid, dim3(kThreads), smem, s.stream, data, stride, s.g, axis, rad, t, TW);
-  else
-    hipLaunchKernelGGL((k_cols<T, 16, false>), grid, dim3(kThreads), smem, s.stream, data, stride, s.g, axis, rad, t,
-                       TW);
-  HIPCHK(hipGetLastError());
-}
 
 // ------------------------------- set-up ------------------------------------
 void validate(const cdft_system_desc& d) {
@@ -519,19 +328,6 @@ int run_eval(cdft_system s, const double* rho, double* dF, double* parts, int fl
   return fetch_parts(s, parts);
 }
 
-template <typename F>
-int guarded(F&& f) {
-  try {
-    return f();
-  } catch (const CdftError& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return CDFT_HIP_ERROR;
-  }
-}
-
 // Standalone transform helper (a system with no functional terms).
 int standalone_fft(int Nx, int Ny, int Nz, int precision, const double* in, double* out, bool r2c) {
   cdft_system_desc d{};
@@ -590,7 +386,9 @@ int standalone_fft(int Nx, int Ny, int Nz, int precision, const double* in, doub
   return CDFT_OK;
 }
 
-}  // namespace
+}  // namespace cdft_impl
+
+using namespace cdft_impl;
 
 extern "C" {
 
@@ -698,6 +496,36 @@ int cdft_memcpy_h2d(void* dst, const void* src, unsigned long long bytes) {
 int cdft_memcpy_d2h(void* dst, const void* src, unsigned long long bytes) {
   return guarded([&] {
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
+  return guarded([&] {
+    REQUIRE(sys && rho > 0, "bad argument");
+    // uniform weighted densities n_a = rho * sum_R w_a(R) (fillUniform,
+    // include/Fundamental_Measures.h:20-31); derivatives collapse onto them.
+    const double d = sys->d.hsd;
+    double fex = 0.0, muex = 0.0;
+    if (sys->m > 0) {
+      const double om[11] = {M_PI * d * d * d / 6, M_PI * d * d, 0, 0, 0, M_PI * d * d / 3, 0, 0, M_PI * d * d / 3, 0,
+                             M_PI * d * d / 3};
+      double n[11], out[11];
+      for (int a = 0; a < 11; ++a) n[a] = rho * om[a];
+      PhiParams pp{d, sys->d.A, sys->d.B};
+      if (sys->d.model == CDFT_WHITEBEAR_I || sys->d.model == CDFT_WHITEBEAR_II) { pp.A = 1.0; pp.B = -1.0; }
+      switch (sys->d.model) {
+        case CDFT_ROSENFELD: fex = phi_site<M_ROSENFELD>(n, pp, out); break;
+        case CDFT_RSLT: fex = phi_site<M_RSLT>(n, pp, out); break;
+        case CDFT_ESFMT: fex = phi_site<M_ESFMT>(n, pp, out); break;
+        case CDFT_WHITEBEAR_I: fex = phi_site<M_WBI>(n, pp, out); break;
+        default: fex = phi_site<M_WBII>(n, pp, out); break;
+      }
+      for (int a = 0; a < 11; ++a) muex += out[a] * om[a];
+    }
+    const double a = sys->d.mean_field ? sys->a_vdw : 0.0;
+    if (f) *f = csum({rho * std::log(rho) - rho, fex, 0.5 * a * rho * rho});
+    if (mu) *mu = csum({std::log(rho), muex, a * rho});
     return CDFT_OK;
   });
 }

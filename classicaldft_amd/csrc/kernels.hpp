@@ -84,7 +84,7 @@ __device__ void block_reduce_store(CSum* v, double2* __restrict__ partials, long
 }
 
 // Final deterministic reduction of nblk x nq partials -> out[q] (one block).
-__global__ void k_reduce_partials(const double2* __restrict__ partials, long nblk, int nq, double* __restrict__ out) {
+static __global__ void k_reduce_partials(const double2* __restrict__ partials, long nblk, int nq, double* __restrict__ out) {
   __shared__ double rs[kThreads], rc[kThreads];
   const int t = threadIdx.x;
   for (int q = 0; q < nq; ++q) {
@@ -508,14 +508,14 @@ __global__ __launch_bounds__(kThreads) void k_z_final(const cplx<T>* __restrict_
 // Weight spectra: real part (even weights) or imaginary part (odd weights) of
 // r2c(w_a), times `scale`, converted to the pipeline precision.
 template <typename T>
-__global__ void k_extract_weight(const cplx<double>* __restrict__ S, T* __restrict__ W, long n, int odd, double scale) {
+static __global__ void k_extract_weight(const cplx<double>* __restrict__ S, T* __restrict__ W, long n, int odd, double scale) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const cplx<double> v = S[i];
   W[i] = T((odd ? v.y : v.x) * scale);
 }
 
-__global__ void k_sum_real(const double* __restrict__ x, long n, double2* __restrict__ partials) {
+static __global__ void k_sum_real(const double* __restrict__ x, long n, double2* __restrict__ partials) {
   CSum a[1] = {{0.0, 0.0}};
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) a[0].add(x[i]);
   block_reduce_store<1>(a, partials, blockIdx.x);

@@ -25,7 +25,7 @@ constexpr double kPi = 3.14159265358979323846;
 
 // eta-dependent cofactors f1, f2, f3 and first derivatives.
 template <int MODEL>
-__device__ __forceinline__ void fcoef(double x, double& f1, double& f1p, double& f2, double& f2p, double& f3,
+__host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double& f2, double& f2p, double& f3,
                                       double& f3p) {
   const double om = 1.0 - x;
   f1 = log(om);  // FMT.h:153-156
@@ -66,7 +66,7 @@ __device__ __forceinline__ void fcoef(double x, double& f1, double& f1p, double&
 // n[] holds the 11 weighted densities of the site (tensor entries ignored for
 // Rosenfeld / RSLT, whose tensor arrays are never convolved).
 template <int MODEL>
-__device__ __forceinline__ double phi_site(const double* n, const PhiParams& pp, double* out) {
+__host__ __device__ inline double phi_site(const double* n, const PhiParams& pp, double* out) {
   constexpr bool TENSOR = MODEL >= M_ESFMT;
   const double d = pp.hsd;
   const double eta = n[0];

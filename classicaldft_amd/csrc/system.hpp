@@ -1,0 +1,222 @@
+// Internal state shared by the translation units of libcdft: device buffers,
+// twiddle tables, the system object and the launch helpers of the transforms.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cdft.h"
+#include "kernels.hpp"
+#include "weights.hpp"
+
+using namespace cdft;
+
+namespace cdft_impl {
+
+extern thread_local std::string g_err;
+
+struct CdftError : std::runtime_error {
+  int code;
+  CdftError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                      \
+  do {                                                                                                    \
+    hipError_t e_ = (expr);                                                                               \
+    if (e_ != hipSuccess)                                                                                 \
+      throw CdftError(CDFT_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_) + " (line " +    \
+                                          std::to_string(__LINE__) + ")");                                \
+  } while (0)
+
+#define REQUIRE(cond, msg) \
+  do {                     \
+    if (!(cond)) throw CdftError(CDFT_BAD_ARG, msg); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t b) {
+    release();
+    if (b) HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename U>
+  U* as() const { return reinterpret_cast<U*>(p); }
+};
+
+inline bool smooth23(int n) {
+  if (n < 1) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  return n == 1;
+}
+
+inline Radices make_radices(int L) {
+  Radices r{};
+  r.L = L;
+  int m = L;
+  int k = 0;
+  while (m % 8 == 0) { r.r[k++] = 8; m /= 8; }
+  if (m % 4 == 0) { r.r[k++] = 4; m /= 4; }
+  if (m % 2 == 0) { r.r[k++] = 2; m /= 2; }
+  while (m % 3 == 0) { r.r[k++] = 3; m /= 3; }
+  if (m != 1) throw CdftError(CDFT_BAD_ARG, "transform length " + std::to_string(L) + " is not a product of 2 and 3");
+  r.npass = k;
+  return r;
+}
+
+// W_L^j = exp(-2 pi i j / L), j in [0, L), in extended precision.
+template <typename T>
+inline void upload_twiddles(DevBuf& buf, int L) {
+  std::vector<cplx<T>> h(L);
+  const long double tp = 6.283185307179586476925286766559005768L;
+  for (int j = 0; j < L; ++j) {
+    const long double a = -tp * (long double)j / (long double)L;
+    h[j] = {T(cosl(a)), T(sinl(a))};
+  }
+  buf.alloc(sizeof(cplx<T>) * L);
+  HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cplx<T>) * L, hipMemcpyHostToDevice));
+}
+
+template <typename T>
+struct Tables {
+  DevBuf x, y, m, n;  // Nx, Ny, Nz/2, Nz
+  void build(const Geo& g) {
+    upload_twiddles<T>(x, g.Nx);
+    upload_twiddles<T>(y, g.Ny);
+    upload_twiddles<T>(m, std::max(1, g.Nz / 2));
+    upload_twiddles<T>(n, g.Nz);
+  }
+};
+
+inline int div_up(long a, long b) { return int((a + b - 1) / b); }
+
+
+}  // namespace cdft_impl
+
+using namespace cdft_impl;
+
+struct cdft_system_s {
+  cdft_system_desc d{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Geo g{};
+  long N = 0, Nkp = 0, nrows = 0;
+  int m = 0, nconv = 0;
+  unsigned odd_mask = 0x1Cu;  // arrays 2, 3, 4 (vector weights) have imaginary spectra
+  bool mixed = false;
+  Radices radX{}, radY{}, radM{};
+  Tables<double> twd;
+  Tables<float> twf;
+  DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
+  double a_vdw = 0.0;
+  int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
+  bool has_vext = false;
+  bool last_with_dft = true;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<const char*> ev_names;
+  int n_marks = 0;
+
+  ~cdft_system_s() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void mark(const char* name) {
+    if (!timing) return;
+    if (n_marks < (int)ev.size()) {
+      HIPCHK(hipEventRecord(ev[n_marks], stream));
+      ev_names[n_marks] = name;
+      ++n_marks;
+    }
+  }
+};
+
+namespace cdft_impl {
+
+template <typename T>
+const Tables<T>& tables(const cdft_system_s& s);
+template <>
+inline const Tables<double>& tables<double>(const cdft_system_s& s) { return s.twd; }
+template <>
+inline const Tables<float>& tables<float>(const cdft_system_s& s) { return s.twf; }
+
+// ------------------------------- transforms --------------------------------
+template <typename T, typename TI>
+inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride, cplx<T>* out, long out_stride, int narr) {
+  const Tables<T>& tw = tables<T>(s);
+  const int M = s.g.Nz / 2;
+  const int RB = s.RBrow;
+  dim3 grid(div_up(s.nrows, RB), narr);
+  hipLaunchKernelGGL((k_rows_r2c<T, TI, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
+                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T, typename TO>
+inline void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_stride, TO* out, long out_stride, int narr,
+                     double scale) {
+  const Tables<T>& tw = tables<T>(s);
+  const int M = s.g.Nz / 2;
+  const int RB = s.RBrow;
+  dim3 grid(div_up(s.nrows, RB), narr);
+  hipLaunchKernelGGL((k_rows_c2r<T, TO, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
+                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB,
+                     scale);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+inline void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd) {
+  const Tables<T>& tw = tables<T>(s);
+  const Radices& rad = axis == 0 ? s.radX : s.radY;
+  const cplx<T>* t = (axis == 0 ? tw.x : tw.y).template as<cplx<T>>();
+  const int TW = s.TWcol;
+  dim3 grid(div_up(s.g.Zh, TW), axis == 0 ? s.g.Ny : s.g.Nx, narr);
+  const size_t smem = sizeof(cplx<T>) * rad.L * TW;
+  if (fwd)
+    hipLaunchKernelGGL((k_cols<T, 16, true>), grid, dim3(kThreads), smem, s.stream, data, stride, s.g, axis, rad, t, TW);
+  else
+    hipLaunchKernelGGL((k_cols<T, 16, false>), grid, dim3(kThreads), smem, s.stream, data, stride, s.g, axis, rad, t,
+                       TW);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace cdft_impl
+
+namespace cdft_impl {
+// Defined in cdft_api.hip.
+void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc);
+int fetch_parts(cdft_system s, double* parts);
+double csum(std::initializer_list<double> v);
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const CdftError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return CDFT_HIP_ERROR;
+  }
+}
+}  // namespace cdft_impl

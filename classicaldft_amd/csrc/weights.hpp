@@ -109,7 +109,7 @@ __device__ __forceinline__ double wc_Gdiff(int kind, double R, double Xc, double
 
 // One thread per cell (Sx,Sy,Sz) in [0,Smax]^3: writes the 11 cell values
 // (eta, s, vx, vy, vz, Txx, Txy, Txz, Tyy, Tyz, Tzz) for the positive octant.
-__global__ void k_fmt_cells(int Sxm, int Sym, int Szm, double dx, double dy, double dz, double R, double* __restrict__ cells) {
+static __global__ void k_fmt_cells(int Sxm, int Sym, int Szm, double dx, double dy, double dz, double R, double* __restrict__ cells) {
   const int ncell = (Sxm + 1) * (Sym + 1) * (Szm + 1);
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
@@ -168,7 +168,7 @@ __global__ void k_fmt_cells(int Sxm, int Sym, int Szm, double dx, double dy, dou
 // Gather the signed octant images of the cells onto the periodic grid
 // (octant scatter of FMT_Species.cpp:471-493, written as a gather so every
 // grid point is owned by one thread).  out: 11 arrays of N doubles.
-__global__ void k_fmt_gather(int Nx, int Ny, int Nz, int Sxm, int Sym, int Szm, const double* __restrict__ cells,
+static __global__ void k_fmt_gather(int Nx, int Ny, int Nz, int Sxm, int Sym, int Szm, const double* __restrict__ cells,
                              double* __restrict__ out) {
   const long N = (long)Nx * Ny * Nz;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,7 +230,7 @@ __device__ __forceinline__ double lj_watt2(const LJParams& v, double r2) {
 // src/Interaction.cpp:522-534): w(S) = dx^2 dy^2 dz^2 sum_{ijk} (1/27)
 // Watt2(|S + pt|^2), pt in {-1/2, 0, 1/2}.  Gathered onto the periodic grid
 // over offsets |i| <= lim (src/Interaction.cpp:178-307), divided by dV and kT.
-__global__ void k_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy, double dz, int lx, int ly, int lz, LJParams v,
+static __global__ void k_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy, double dz, int lx, int ly, int lz, LJParams v,
                             double kT, double* __restrict__ out) {
   const long N = (long)Nx * Ny * Nz;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -102,6 +102,57 @@ int cdft_system_fetch(cdft_system sys, double* parts);
    the system's stream; *ms = elapsed time of the whole batch. */
 int cdft_system_time(cdft_system sys, const double* rho_d, double* dF_d, int flags, int iters, double* ms);
 
+/* Bulk thermodynamics of the system's functional at uniform density rho:
+   beta f = rho ln rho - rho + Phi(uniform) + a_vdw rho^2 / 2 (per volume) and
+   beta mu = ln rho + mu_ex + a_vdw rho.  Replaces DFT::mu_times_beta and
+   fhelmholtz_times_beta_over_volume (src/DFT.cpp:58-113; FMT::BulkMuex /
+   BulkFex src/FMT.cpp:616-709; Interaction_Base::Mu src/Interaction.cpp:61-72). */
+int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu);
+
+/* ---------------------------------------------------------------- FIRE2 ---
+   fireMinimizer2 (include/Minimizer.h:86-182, src/Minimizer.cpp:117-327) on
+   the alias variables of FMT_Species (src/FMT_Species.cpp:29-93).  The
+   iteration state lives on the device; the control logic is the reference's. */
+typedef struct cdft_fire_s* cdft_fire;
+typedef struct cdft_fire_params {
+  double dt, dt_max, dt_min, alpha_start, f_dec, f_inc, f_alf;
+  int N_delay, N_P_negative_max, initial_delay;
+} cdft_fire_params;
+typedef struct cdft_fire_report {
+  double F;          /* free energy after the step */
+  double fmax_alias; /* max |dF/dx| / dV on the alias variables (the reference's fmax_) */
+  double fmax_raw;   /* max |dF/drho| / dV (Species.h:55 convention) */
+  double dt, alpha, vv, rms_force;
+  double monitor;    /* |vv| / V: fireMinimizer2::get_convergence_monitor */
+  int steps, calls;
+} cdft_fire_report;
+int cdft_fire_default_params(cdft_fire_params* p);
+/* Constructor: alias from rho0 (host unless flags has CDFT_DEVICE_PTRS), first
+   getDF_DX, v = 0.  params may be NULL (reference defaults). */
+int cdft_fire_create(cdft_system sys, const double* rho0, int flags, const cdft_fire_params* params, cdft_fire* out);
+/* One fireMinimizer2::step.  Returns CDFT_OK, CDFT_ETA_TOO_LARGE (the step
+   back-tracked: x, v/2 and dF restored, dt halved) or CDFT_UPHILL_LIMIT. */
+int cdft_fire_step(cdft_fire f, cdft_fire_report* rep);
+/* Steps until fmax_raw <= tol_raw or max_steps (back-tracks included). */
+int cdft_fire_run(cdft_fire f, int max_steps, double tol_raw, cdft_fire_report* rep);
+int cdft_fire_get(cdft_fire f, double* rho, double* dF_raw, double* x, double* v);
+int cdft_fire_destroy(cdft_fire f);
+
+/* ----------------------------------------------------------------- DDFT ---
+   Integrating-factor DDFT (src/DDFT.cpp:42-351), single species. */
+typedef struct cdft_ddft_s* cdft_ddft;
+typedef struct cdft_ddft_report {
+  double F, dt, time, rhs_max, mass;
+  int calls, restarts;
+} cdft_ddft_report;
+/* dt <= 0 selects the reference default 1e-4 dx^2 (src/DDFT.cpp:46); dt_max
+   <= 0 selects 1; tol <= 0 selects 1e-4 (include/Minimizer.h:239). */
+int cdft_ddft_create(cdft_system sys, const double* rho0, double dt, double dt_max, double tol, cdft_ddft* out);
+int cdft_ddft_step(cdft_ddft d, cdft_ddft_report* rep);
+int cdft_ddft_set_max_time_step(cdft_ddft d, double dt_max);
+int cdft_ddft_get_density(cdft_ddft d, double* rho);
+int cdft_ddft_destroy(cdft_ddft d);
+
 /* Device memory plumbing for callers without their own HIP runtime. */
 int cdft_malloc(void** p, unsigned long long bytes);
 int cdft_free(void* p);
