@@ -54,6 +54,13 @@ def build_host():
     run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs,
          "-L", HERE, "-lcdft", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
+    ex = os.path.join(HOST, "examples")
+    for f in sorted(os.listdir(ex)) if os.path.isdir(ex) else []:
+        if f.endswith(".cpp"):
+            exe = os.path.join(HERE, f[:-4])
+            run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", exe + ".tmp",
+                 os.path.join(ex, f), "-L", HERE, "-lclassicaldft", "-lcdft", "-Wl,-rpath,$ORIGIN"])
+            os.replace(exe + ".tmp", exe)
     return out
 
 

@@ -1,0 +1,468 @@
+// Host side of the source-level drop-in (include/classicaldft/classicaldft.h).
+// Every evaluation and solver step goes through the C ABI of <cdft.h>.
+#include "classicaldft/classicaldft.h"
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+
+namespace {
+
+void check(int rc) {
+  if (rc == CDFT_OK) return;
+  if (rc == CDFT_ETA_TOO_LARGE) throw Eta_Too_Large_Exception();
+  throw std::runtime_error(std::string("cdft: ") + cdft_last_error());
+}
+
+// Neumaier accumulator (include/Summation.h:15-56)
+struct NSum {
+  double s = 0, c = 0;
+  void add(double v) {
+    const double t = s + v;
+    if (std::fabs(s) >= std::fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+  }
+  double sum() const { return s + c; }
+};
+
+// Adaptive Gauss-Kronrod (7/15) quadrature.
+double gk15(const std::function<double(double)>& f, double a, double b, double tol, int depth = 0) {
+  static const double xk[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                               0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                               0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                               0.207784955007898467600689403773245, 0.0};
+  static const double wk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                               0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                               0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                               0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+  static const double wg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                               0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+  const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+  double k = wk[7] * f(c), g = wg[3] * f(c);
+  for (int i = 0; i < 7; ++i) {
+    const double fp = f(c + h * xk[i]), fm = f(c - h * xk[i]);
+    k += wk[i] * (fp + fm);
+    if (i % 2 == 1) g += wg[i / 2] * (fp + fm);
+  }
+  k *= h;
+  g *= h;
+  if (std::fabs(k - g) <= tol || depth > 40) return k;
+  return gk15(f, a, c, 0.5 * tol, depth + 1) + gk15(f, c, b, 0.5 * tol, depth + 1);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ DFT_Vec
+void DFT_Vec::IncrementBy(const DFT_Vec& o) {
+  for (size_t i = 0; i < d_.size(); ++i) d_[i] += o.d_[i];
+}
+void DFT_Vec::IncrementBy_Scaled_Vector(const DFT_Vec& o, double s) {
+  for (size_t i = 0; i < d_.size(); ++i) d_[i] += o.d_[i] * s;
+}
+void DFT_Vec::MultBy(double s) {
+  for (double& v : d_) v *= s;
+}
+void DFT_Vec::add(double s) {
+  for (double& v : d_) v += s;
+}
+double DFT_Vec::inf_norm() const {
+  double m = 0;
+  for (double v : d_) m = std::max(m, std::fabs(v));
+  return m;
+}
+double DFT_Vec::euclidean_norm() const {
+  double s = 0;
+  for (double v : d_) s += v * v;
+  return std::sqrt(s);
+}
+double DFT_Vec::dotWith(const DFT_Vec& o) const {
+  double s = 0;
+  for (size_t i = 0; i < d_.size(); ++i) s += d_[i] * o.d_[i];
+  return s;
+}
+double DFT_Vec::accu() const {
+  double s = 0;
+  for (double v : d_) s += v;
+  return s;
+}
+double DFT_Vec::min() const { return *std::min_element(d_.begin(), d_.end()); }
+double DFT_Vec::max() const { return *std::max_element(d_.begin(), d_.end()); }
+
+// ------------------------------------------------------------------ Lattice
+// Lattice::init, include/Lattice.h:57-85
+Lattice::Lattice(double dx, double dy, double dz, double L[]) : dx_(dx), dy_(dy), dz_(dz) {
+  Nx_ = long((L[0] + 1e-8 * dx_) / dx_);
+  Ny_ = long((L[1] + 1e-8 * dy_) / dy_);
+  Nz_ = long((L[2] + 1e-8 * dz_) / dz_);
+  L_[0] = Nx_ * dx_;
+  L_[1] = Ny_ * dy_;
+  L_[2] = Nz_ * dz_;
+  if (std::fabs(L[0] - Nx_ * dx_) > 1e-5 * dx_) throw std::runtime_error("Box length incommensurate with lattice in x direction");
+  if (std::fabs(L[1] - Ny_ * dy_) > 1e-5 * dy_) throw std::runtime_error("Box length incommensurate with lattice in y direction");
+  if (std::fabs(L[2] - Nz_ * dz_) > 1e-5 * dz_) throw std::runtime_error("Box length incommensurate with lattice in z direction");
+}
+
+long Lattice::pos(int i, int j, int k) const {
+  if (i < 0 || i >= Nx_) throw std::runtime_error("ix out of bounds");
+  if (j < 0 || j >= Ny_) throw std::runtime_error("iy out of bounds");
+  if (k < 0 || k >= Nz_) throw std::runtime_error("iz out of bounds");
+  return k + Nz_ * (j + Ny_ * i);
+}
+
+long Lattice::get_PBC_Pos(int ix, int iy, int iz) const {
+  auto w = [](long v, long n) { v %= n; return v < 0 ? v + n : v; };
+  return w(iz, Nz_) + Nz_ * (w(iy, Ny_) + Ny_ * w(ix, Nx_));
+}
+
+// ------------------------------------------------------------------ Density
+Density::Density(double dx, double L[]) : Lattice(dx, L), rho_(Ntot()), vWall_(Ntot()) {}
+Density::Density(double dx, double dy, double dz, double L[]) : Lattice(dx, dy, dz, L), rho_(Ntot()), vWall_(Ntot()) {}
+void Density::set(double val) {
+  for (long i = 0; i < rho_.size(); ++i) rho_.set(i, val);
+}
+double Density::getNumberAtoms() const {
+  NSum s;
+  for (long i = 0; i < rho_.size(); ++i) s.add(rho_.get(i));
+  return s.sum() * dV();
+}
+bool Density::has_field_() const { return vWall_.inf_norm() > 0.0; }
+
+// ------------------------------------------------------------------ Species
+Species::Species(Density& density, double mu, int) : density_(density), dF_(density.Ntot()), mu_(mu) {}
+
+void Species::beginForceCalculation() {
+  if (fixedMass_ > 0.0) {
+    density_.scale_to(fixedMass_ / density_.getNumberAtoms());
+    mu_ = 0.0;
+  }
+}
+
+double Species::endForceCalculation() {
+  if (fixedMass_ > 0.0) {
+    mu_ = 0.0;
+    for (long p = 0; p < density_.Ntot(); ++p) mu_ += dF_.get(p) * density_.get(p);
+    mu_ /= fixedMass_;
+    for (long p = 0; p < density_.Ntot(); ++p) dF_.set(p, dF_.get(p) - mu_ * density_.dV());
+  }
+  return 0;
+}
+
+FMT_Species::FMT_Species(Density& density, double hsd, double mu, int seq) : Species(density, mu, seq), hsd_(hsd) {
+  // src/FMT_Species.cpp:321-322
+  if (hsd < std::min(std::min(density.getDX(), density.getDY()), density.getDZ()))
+    throw std::runtime_error("hsd is less than the lattice spacing ... aborting");
+}
+
+// ---------------------------------------------------------------------- FMT
+FMT::~FMT() {
+  if (sys_) cdft_system_destroy(sys_);
+}
+
+std::string esFMT::Name() const {
+  return "esFMT with A = " + std::to_string(A_) + " and B = " + std::to_string(B_);
+}
+
+cdft_system FMT::system_for(const Species& s) const {
+  const Lattice& L = s.getLattice();
+  if (sys_ && sys_lattice_ == &L && sys_hsd_ == s.getHSD()) return sys_;
+  if (sys_) cdft_system_destroy(sys_);
+  cdft_system_desc d{};
+  d.Nx = (int)L.Nx();
+  d.Ny = (int)L.Ny();
+  d.Nz = (int)L.Nz();
+  d.dx = L.getDX();
+  d.dy = L.getDY();
+  d.dz = L.getDZ();
+  d.model = model_;
+  d.A = A_;
+  d.B = B_;
+  d.hsd = s.getHSD();
+  d.precision = precision_;
+  check(cdft_system_create(&d, &sys_));
+  sys_lattice_ = &L;
+  sys_hsd_ = s.getHSD();
+  return sys_;
+}
+
+double FMT::calculateFreeEnergyAndDerivatives(std::vector<Species*>& allSpecies) {
+  double F = 0.0;
+  int n = 0;
+  for (Species* s : allSpecies) {
+    auto* fs = dynamic_cast<FMT_Species*>(s);
+    if (!fs) continue;
+    if (++n > 1) throw std::runtime_error("FMT on the GPU is implemented for a single species");
+    cdft_system sys = system_for(*fs);
+    double Fs = 0.0;
+    check(cdft_fmt_eval(sys, fs->getDensity().get_density_pointer(), fs->getDF().memptr(), &Fs, CDFT_ACCUMULATE));
+    F += Fs;
+  }
+  return F;
+}
+
+double FMT::BulkFex(const std::vector<double>& x, const std::vector<Species*>& allSpecies) const {
+  for (size_t i = 0; i < allSpecies.size(); ++i)
+    if (auto* fs = dynamic_cast<FMT_Species*>(allSpecies[i])) {
+      double f, mu;
+      check(cdft_system_bulk(system_for(*fs), x[i], &f, &mu));
+      return f - (x[i] * std::log(x[i]) - x[i]);
+    }
+  return 0.0;
+}
+
+double FMT::BulkMuex(const std::vector<double>& x, const std::vector<Species*>& allSpecies, int species) const {
+  auto* fs = dynamic_cast<FMT_Species*>(allSpecies[species]);
+  if (!fs) return 0.0;
+  double f, mu;
+  check(cdft_system_bulk(system_for(*fs), x[species], &f, &mu));
+  return mu - std::log(x[species]);
+}
+
+// --------------------------------------------------------------- Potential
+double Potential1::Watt(double r) const {  // include/Potential1.h:56-65, WCA branch
+  if (!(r < rcut_)) return 0.0;
+  if (r < rmin_) return Vmin_;
+  return V(r);
+}
+
+double Potential1::getHSD(double kT) const {
+  auto f = [&](double r) { return r <= 0 ? 1.0 : 1.0 - std::exp(-V0(r) / kT); };
+  return gk15(f, 0.0, rmin_, 1e-13);
+}
+
+LJ::LJ(double sigma, double eps, double rcut) : Potential1(sigma, eps, rcut) {  // include/Potential1.h:140-146
+  shift_ = rcut_ < 0 ? 0.0 : vr(rcut_);
+  rmin_ = getRmin();
+  Vmin_ = V(rmin_);
+}
+
+// ------------------------------------------------------------- Interaction
+Interaction_Interpolation_QF::Interaction_Interpolation_QF(Species* s1, Species* s2, Potential1* v, double kT)
+    : Interaction_Base(s1, s2, v, kT) {
+  if (s1 != s2) throw std::runtime_error("only the single-species (s1 == s2) mean field is implemented");
+}
+
+double Interaction_Base::getVDWParameter() const {
+  if (std::isnan(a_vdw_)) {
+    const Lattice& L = s1_->getLattice();
+    cdft_system_desc d{};
+    d.Nx = (int)L.Nx();
+    d.Ny = (int)L.Ny();
+    d.Nz = (int)L.Nz();
+    d.dx = L.getDX();
+    d.dy = L.getDY();
+    d.dz = L.getDZ();
+    d.model = CDFT_MODEL_NONE;
+    d.mean_field = 1;
+    d.lj_sigma = v_->getSigma();
+    d.lj_eps = v_->getEps();
+    d.lj_rcut = v_->getRcut();
+    d.kT = kT_;
+    cdft_system s = nullptr;
+    check(cdft_system_create(&d, &s));
+    check(cdft_system_a_vdw(s, &a_vdw_));
+    cdft_system_destroy(s);
+  }
+  return a_vdw_;
+}
+
+// src/Interaction.cpp:61-72
+double Interaction_Base::Mu(const std::vector<double>& x, int) const { return getVDWParameter() * x[0]; }
+double Interaction_Base::Fhelmholtz(const std::vector<double>& x) const { return 0.5 * getVDWParameter() * x[0] * x[0]; }
+
+// ---------------------------------------------------------------------- DFT
+DFT::DFT(Species* s) {
+  if (s) addSpecies(s);
+}
+DFT::~DFT() { invalidate(); }
+void DFT::invalidate() const {
+  if (sys_) cdft_system_destroy(sys_);
+  sys_ = nullptr;
+}
+void DFT::addSpecies(Species* s) {
+  if (!species_.empty()) throw std::runtime_error("the GPU DFT is implemented for a single species");
+  species_.push_back(s);
+  invalidate();
+}
+void DFT::addInteraction(Interaction_Base* i) {
+  if (!interactions_.empty()) throw std::runtime_error("one mean-field interaction is supported");
+  if (!dynamic_cast<LJ*>(i->potential())) throw std::runtime_error("the GPU mean field implements the LJ potential");
+  interactions_.push_back(i);
+  invalidate();
+}
+void DFT::addHardCoreContribution(FMT* fmt) {
+  fmt_ = fmt;
+  invalidate();
+}
+void DFT::set_precision(int p) {
+  precision_ = p;
+  invalidate();
+}
+
+cdft_system DFT::system() const {
+  Species* s = species_.front();
+  const Density& D = s->getDensity();
+  const double mu = s->getChemPotential();
+  if (sys_ && sys_mu_ == mu && sys_field_ == D.field_version()) return sys_;
+  invalidate();
+  cdft_system_desc d{};
+  d.Nx = (int)D.Nx();
+  d.Ny = (int)D.Ny();
+  d.Nz = (int)D.Nz();
+  d.dx = D.getDX();
+  d.dy = D.getDY();
+  d.dz = D.getDZ();
+  d.model = fmt_ ? fmt_->model_id() : CDFT_MODEL_NONE;
+  d.A = fmt_ ? fmt_->A() : 1.0;
+  d.B = fmt_ ? fmt_->B() : 0.0;
+  d.hsd = s->getHSD();
+  d.precision = precision_;
+  d.ideal = 1;
+  d.mu = mu;
+  d.vext = D.has_external_field() ? D.get_external_field().memptr() : nullptr;
+  if (!interactions_.empty()) {
+    const Interaction_Base* i = interactions_.front();
+    d.mean_field = 1;
+    d.lj_sigma = i->potential()->getSigma();
+    d.lj_eps = i->potential()->getEps();
+    d.lj_rcut = i->potential()->getRcut();
+    d.kT = i->kT();
+  }
+  check(cdft_system_create(&d, &sys_));
+  sys_mu_ = mu;
+  sys_field_ = D.field_version();
+  return sys_;
+}
+
+// src/DFT.cpp:360-425
+double DFT::calculateFreeEnergyAndDerivatives(bool onlyFex) {
+  Species* s = species_.front();
+  s->zeroForce();
+  s->beginForceCalculation();
+  cdft_system sys = system();
+  double parts[5];
+  check(cdft_system_eval(sys, s->getDensity().get_density_pointer(), s->getDF().memptr(), parts, 0));
+  F_id_ = parts[1];
+  F_hs_ = parts[2];
+  F_mf_ = parts[3];
+  F_ext_ = parts[4];
+  double F = parts[0];
+  if (onlyFex) {
+    // drop the ideal and external parts: dF_id = dV ln rho, dF_ext = dV (V_ext - mu)
+    const Density& D = s->getDensity();
+    const double dV = D.dV();
+    DFT_Vec& df = s->getDF();
+    const bool fld = D.has_external_field();
+    for (long i = 0; i < D.Ntot(); ++i) {
+      double v = std::log(D.get(i)) * dV - s->getChemPotential() * dV;
+      if (fld) v += D.get_external_field().get(i) * dV;
+      df.set(i, df.get(i) - v);
+    }
+    F = F_hs_ + F_mf_;
+    F_id_ = F_ext_ = 0.0;
+  }
+  s->endForceCalculation();
+  return F;
+}
+
+double DFT::mu_times_beta(double density) const {
+  double f, mu;
+  check(cdft_system_bulk(system(), density, &f, &mu));
+  return mu;
+}
+double DFT::fhelmholtz_times_beta_over_volume(double density) const {
+  double f, mu;
+  check(cdft_system_bulk(system(), density, &f, &mu));
+  return f;
+}
+double DFT::omega_times_beta_over_volume(double density) const {
+  return fhelmholtz_times_beta_over_volume(density) - density * mu_times_beta(density);
+}
+
+// ---------------------------------------------------------------- Minimizer
+// Minimizer::run / resume, src/Minimizer.cpp:36-87 (no wall-clock or snapshot I/O)
+void Minimizer::run(long maxSteps) {
+  step_counter_ = 0;
+  calls_ = 0;
+  resume(maxSteps);
+}
+
+void Minimizer::resume(long maxSteps) {
+  do {
+    F_ = step();
+    step_counter_++;
+    if (get_convergence_monitor() < forceLimit_) break;
+    if (maxSteps > 0 && step_counter_ == maxSteps) break;
+  } while (true);
+}
+
+fireMinimizer2::fireMinimizer2(DFT* dft) : Minimizer(dft) {
+  cdft_fire_default_params(&params_);
+  forceLimit_ = 0.1;
+}
+fireMinimizer2::~fireMinimizer2() {
+  if (fire_) cdft_fire_destroy(fire_);
+}
+void fireMinimizer2::ensure() {
+  if (fire_) return;
+  Species* s = dft_->getSpecies(0);
+  check(cdft_fire_create(dft_->system(), s->getDensity().get_density_pointer(), 0, &params_, &fire_));
+}
+double fireMinimizer2::step() {
+  ensure();
+  const int rc = cdft_fire_step(fire_, &rep_);
+  if (rc == CDFT_UPHILL_LIMIT) throw std::runtime_error("Cannot stop going uphill in Fire2");
+  if (rc < 0) check(rc);
+  calls_ = rep_.calls;
+  F_ = rep_.F;
+  DFT_Vec rho(dft_->get_lattice().Ntot());
+  density(rho);
+  dft_->getSpecies(0)->set_density(rho);
+  return F_;
+}
+int fireMinimizer2::run_to_raw_tolerance(long maxSteps, double tol) {
+  ensure();
+  const int rc = cdft_fire_run(fire_, (int)maxSteps, tol, &rep_);
+  if (rc < 0) check(rc);
+  F_ = rep_.F;
+  calls_ = rep_.calls;
+  DFT_Vec rho(dft_->get_lattice().Ntot());
+  density(rho);
+  dft_->getSpecies(0)->set_density(rho);
+  return rep_.steps;
+}
+void fireMinimizer2::density(DFT_Vec& out) const {
+  out.resize(dft_->get_lattice().Ntot());
+  check(cdft_fire_get(fire_, out.memptr(), nullptr, nullptr, nullptr));
+}
+
+DDFT::DDFT(DFT* dft, bool) : Minimizer(dft) {
+  const double dx = dft->get_lattice().getDX();
+  dt_ = 0.0001 * dx * dx;  // src/DDFT.cpp:46
+}
+DDFT::~DDFT() {
+  if (ddft_) cdft_ddft_destroy(ddft_);
+}
+void DDFT::ensure() {
+  if (ddft_) return;
+  check(cdft_ddft_create(dft_->system(), dft_->getSpecies(0)->getDensity().get_density_pointer(), dt_, dtMax_, tol_,
+                         &ddft_));
+}
+void DDFT::set_max_time_step(double t) {
+  dtMax_ = t;
+  if (ddft_) check(cdft_ddft_set_max_time_step(ddft_, t));
+}
+double DDFT::step() {
+  ensure();
+  check(cdft_ddft_step(ddft_, &rep_));
+  F_ = rep_.F;
+  calls_ = rep_.calls;
+  DFT_Vec rho(dft_->get_lattice().Ntot());
+  density(rho);
+  dft_->getSpecies(0)->set_density(rho);
+  return F_;
+}
+void DDFT::density(DFT_Vec& out) const {
+  out.resize(dft_->get_lattice().Ntot());
+  check(cdft_ddft_get_density(ddft_, out.memptr()));
+}

@@ -1,0 +1,64 @@
+// The hot path driven through the reference-named C++ API (config 1 and a
+// short FIRE / DDFT run), printing machine-readable lines for the tests:
+//   F_hs <value>            FMT::calculateFreeEnergyAndDerivatives on config 1
+//   dF_sum <value>          sum of the FMT dF
+//   mu <value>              DFT::mu_times_beta(0.5) with LJ mean field, kT = 1
+//   fire <F> <fmax_raw>     after 20 fireMinimizer2 steps
+//   ddft <F> <mass>         after 3 DDFT steps
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "classicaldft/classicaldft.h"
+
+static double splitmix_u(uint64_t seed, uint64_t k) {  // (k+1)-th draw of splitmix64(seed)
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+int main() {
+  // config 1: 32^3, dx = 0.1, hsd = 1, White Bear I, rho = 0.6 (1 + 0.05 U[-1,1])
+  double L[3] = {3.2, 3.2, 3.2};
+  Density density(0.1, L);
+  for (long i = 0; i < density.Ntot(); ++i) density.set(i, 0.6 * (1.0 + 0.05 * (2.0 * splitmix_u(0x5EED0001ULL, i) - 1.0)));
+  FMT_Species species(density, 1.0, 0.0);
+  WhiteBearI fmt;
+  std::vector<Species*> all{&species};
+  species.zeroForce();
+  const double F = fmt.calculateFreeEnergyAndDerivatives(all);
+  std::printf("F_hs %.17g\n", F);
+  std::printf("dF_sum %.17g\n", species.getDF().accu());
+
+  // 16^3 DFT with ideal gas + WB I + LJ/QF mean field at kT = 1
+  double L2[3] = {1.6, 1.6, 1.6};
+  Density d2(0.1, L2);
+  for (long i = 0; i < d2.Ntot(); ++i) d2.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(11, i) - 1.0)));
+  FMT_Species s2(d2, 1.0, 0.5);
+  LJ lj(1.0, 1.0, 2.5);
+  Interaction_Interpolation_QF qf(&s2, &s2, &lj, 1.0);
+  WhiteBearI fmt2;
+  DFT dft(&s2);
+  dft.addHardCoreContribution(&fmt2);
+  dft.addInteraction(&qf);
+  std::printf("mu %.17g\n", dft.mu_times_beta(0.5));
+  fireMinimizer2 fire(&dft);
+  for (int i = 0; i < 20; ++i) fire.step();
+  std::printf("fire %.17g %.17g\n", fire.getF(), fire.get_raw_force_max());
+
+  Density d3(0.1, L2);
+  for (long i = 0; i < d3.Ntot(); ++i) d3.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(21, i) - 1.0)));
+  FMT_Species s3(d3, 1.0, 0.0);
+  Interaction_Interpolation_QF qf3(&s3, &s3, &lj, 1.0);
+  WhiteBearI fmt3;
+  DFT dft3(&s3);
+  dft3.addHardCoreContribution(&fmt3);
+  dft3.addInteraction(&qf3);
+  DDFT ddft(&dft3);
+  for (int i = 0; i < 3; ++i) ddft.step();
+  std::printf("ddft %.17g %.17g\n", ddft.getF(), ddft.get_mass());
+  std::printf("hsd_lj_kT0.8 %.17g\n", lj.getHSD(0.8));
+  return 0;
+}

@@ -1,0 +1,380 @@
+// classicaldft.h - source-level drop-in for the hot path of classicalDFT.
+//
+// Classes carry the reference's names and the signatures a caller of the hot
+// path uses (Lattice, Density, Species, FMT_Species, FMT and its five models,
+// Potential1/LJ, Interaction_Interpolation_QF, DFT, Minimizer/fireMinimizer2,
+// DDFT, Eta_Too_Large_Exception).  Every evaluation and every solver step runs
+// on the GPU through the C ABI of <cdft.h>; these classes only hold host
+// copies of the fields the reference exposes and translate status codes into
+// the reference's exceptions.  References are to the classicalDFT sources.
+#ifndef CLASSICALDFT_AMD_H
+#define CLASSICALDFT_AMD_H
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../cdft.h"
+
+// --------------------------------------------------------------- vectors ---
+// DFT_Vec (include/DFT_LinAlg.h:14-97): the operations the hot path's callers use.
+class DFT_Vec {
+ public:
+  DFT_Vec() = default;
+  explicit DFT_Vec(long n) : d_(n, 0.0) {}
+  long size() const { return (long)d_.size(); }
+  void resize(long n) { d_.resize(n); }
+  void zeros() { std::fill(d_.begin(), d_.end(), 0.0); }
+  void zeros(long n) { d_.assign(n, 0.0); }
+  double get(long i) const { return d_[i]; }
+  void set(long i, double v) { d_[i] = v; }
+  void set(const DFT_Vec& o) { d_ = o.d_; }
+  void IncrementBy(long i, double v) { d_[i] += v; }
+  void IncrementBy(const DFT_Vec& o);
+  void IncrementBy_Scaled_Vector(const DFT_Vec& o, double s);
+  void MultBy(double s);
+  void add(double s);
+  double inf_norm() const;
+  double euclidean_norm() const;
+  double dotWith(const DFT_Vec& o) const;
+  double accu() const;
+  double min() const;
+  double max() const;
+  double* memptr() { return d_.data(); }
+  const double* memptr() const { return d_.data(); }
+
+ private:
+  std::vector<double> d_;
+};
+
+// ------------------------------------------------------------ exceptions ---
+// include/FMT.h:649-663
+class Eta_Too_Large_Exception : public std::runtime_error {
+ public:
+  Eta_Too_Large_Exception() : std::runtime_error("Eta too large in FMT::dPHI") {}
+};
+
+// --------------------------------------------------------------- lattice ---
+// include/Lattice.h:11-134
+class Lattice {
+ public:
+  Lattice(double dx, double L[]) : Lattice(dx, dx, dx, L) {}
+  Lattice(double dx, double dy, double dz, double L[]);
+  double getDX() const { return dx_; }
+  double getDY() const { return dy_; }
+  double getDZ() const { return dz_; }
+  long Nx() const { return Nx_; }
+  long Ny() const { return Ny_; }
+  long Nz() const { return Nz_; }
+  long Ntot() const { return Nx_ * Ny_ * Nz_; }
+  long size() const { return Ntot(); }
+  long Nout() const { return Nx_ * Ny_ * (Nz_ / 2 + 1); }
+  double dV() const { return dx_ * dy_ * dz_; }
+  double getVolume() const { return L_[0] * L_[1] * L_[2]; }
+  double Lx() const { return L_[0]; }
+  double Ly() const { return L_[1]; }
+  double Lz() const { return L_[2]; }
+  long pos(int i, int j, int k) const;
+  long get_PBC_Pos(int ix, int iy, int iz) const;
+
+ protected:
+  long Nx_ = 0, Ny_ = 0, Nz_ = 0;
+  double dx_ = 0, dy_ = 0, dz_ = 0;
+  double L_[3] = {0, 0, 0};
+};
+
+// --------------------------------------------------------------- density ---
+// include/Density.h:35-118; the external field gets a public setter.
+class Density : public Lattice {
+ public:
+  Density(double dx, double L[]);
+  Density(double dx, double dy, double dz, double L[]);
+  void set(long pos, double val) { rho_.set(pos, val); }
+  void set(int i, int j, int k, double val) { rho_.set(pos(i, j, k), val); }
+  void set(const DFT_Vec& x) { rho_.set(x); }
+  void set(double val);
+  double get(long pos) const { return rho_.get(pos); }
+  double getNumberAtoms() const;  // compensated, src/Density.cpp:136-142
+  void scale_to(double a) { rho_.MultBy(a); }
+  const DFT_Vec& get_density_real() const { return rho_; }
+  const double* get_density_pointer() const { return rho_.memptr(); }
+  const DFT_Vec& get_external_field() const { return vWall_; }
+  void set_external_field(const DFT_Vec& v) { vWall_ = v; ++field_version_; }
+  bool has_external_field() const { return has_field_(); }
+  unsigned field_version() const { return field_version_; }
+
+ private:
+  bool has_field_() const;
+  DFT_Vec rho_;
+  DFT_Vec vWall_;
+  unsigned field_version_ = 0;
+};
+
+// --------------------------------------------------------------- species ---
+// include/Species.h:13-97, src/Species.cpp
+class Species {
+ public:
+  Species(Density& density, double mu = 0, int seq = -1);
+  virtual ~Species() = default;
+  const Lattice& getLattice() const { return density_; }
+  const Density& getDensity() const { return density_; }
+  Density& density() { return density_; }
+  void set_density(DFT_Vec& x) { density_.set(x); }
+  void set_density(long j, double x) { density_.set(j, x); }
+  double getChemPotential() const { return mu_; }
+  void setChemPotential(double m) { mu_ = m; fixedMass_ = -1; }
+  void setFixedMass(double m) { fixedMass_ = m; if (m > 0.0) mu_ = 0.0; }
+  bool is_mass_fixed() const { return fixedMass_ > 0; }
+  void zeroForce() { dF_.zeros(); }
+  void addToForce(const DFT_Vec& f) { dF_.IncrementBy(f); }
+  void addToForce(long i, double v) { dF_.IncrementBy(i, v); }
+  DFT_Vec& getDF() { return dF_; }
+  const DFT_Vec& get_const_DF() const { return dF_; }
+  void setDF(DFT_Vec& df) { dF_.set(df); }
+  double get_convergence_monitor() const { return dF_.inf_norm() / density_.dV(); }
+  virtual double getHSD() const { return 0.0; }
+  void beginForceCalculation();   // src/Species.cpp:64-71
+  double endForceCalculation();   // src/Species.cpp:73-112 (fixed mass)
+
+ protected:
+  Density& density_;
+  DFT_Vec dF_;
+  double mu_;
+  double fixedMass_ = -1;
+};
+
+// include/Species.h:109-373: the weights live on the GPU (resident spectra).
+class FMT_Species : public Species {
+ public:
+  FMT_Species(Density& density, double hsd, double mu = 0, int seq = -1);
+  double getHSD() const override { return hsd_; }
+
+ private:
+  double hsd_;
+};
+
+// ------------------------------------------------------------------- FMT ---
+// include/FMT.h:28-643.  calculateFreeEnergyAndDerivatives accumulates into
+// each FMT species' dF and returns F_hs (x dV), like src/FMT.cpp:363-427.
+class FMT {
+ public:
+  virtual ~FMT();
+  double calculateFreeEnergyAndDerivatives(std::vector<Species*>& allSpecies);
+  double BulkMuex(const std::vector<double>& x, const std::vector<Species*>& allSpecies, int species) const;
+  double BulkFex(const std::vector<double>& x, const std::vector<Species*>& allSpecies) const;
+  virtual std::string Name() const = 0;
+  std::string get_name() const { return Name(); }
+  virtual bool needsTensor() const { return model_ >= CDFT_ESFMT; }
+  int model_id() const { return model_; }
+  double A() const { return A_; }
+  double B() const { return B_; }
+  void set_precision(int p) { precision_ = p; }  // CDFT_FP64 (default) or CDFT_MIXED
+  int precision() const { return precision_; }
+
+ protected:
+  FMT(int model, double A = 1, double B = 0) : model_(model), A_(A), B_(B) {}
+  int model_;
+  double A_, B_;
+  int precision_ = CDFT_FP64;
+  mutable cdft_system sys_ = nullptr;
+  mutable const Lattice* sys_lattice_ = nullptr;
+  mutable double sys_hsd_ = -1;
+  cdft_system system_for(const Species& s) const;
+};
+
+class Rosenfeld : public FMT {
+ public:
+  Rosenfeld() : FMT(CDFT_ROSENFELD) {}
+  std::string Name() const override { return "Rosenfeld"; }
+};
+class RSLT : public FMT {
+ public:
+  RSLT() : FMT(CDFT_RSLT) {}
+  std::string Name() const override { return "RSLT"; }
+};
+class esFMT : public FMT {
+ public:
+  esFMT(double A = 1, double B = 0) : FMT(CDFT_ESFMT, A, B) {}
+  std::string Name() const override;
+};
+class WhiteBearI : public FMT {
+ public:
+  WhiteBearI() : FMT(CDFT_WHITEBEAR_I, 1, -1) {}
+  std::string Name() const override { return "WhiteBear I"; }
+};
+class WhiteBearII : public FMT {
+ public:
+  WhiteBearII() : FMT(CDFT_WHITEBEAR_II, 1, -1) {}
+  std::string Name() const override { return "WhiteBear II"; }
+};
+
+// ------------------------------------------------------------ potentials ---
+// include/Potential1.h:23-189 (LJ with the WCA split; BH split not provided)
+class Potential1 {
+ public:
+  Potential1(double sigma, double eps, double rcut) : sigma_(sigma), eps_(eps), rcut_(rcut) {}
+  virtual ~Potential1() = default;
+  double getRcut() const { return rcut_; }
+  double getSigma() const { return sigma_; }
+  double getEps() const { return eps_; }
+  virtual double getRmin() const = 0;
+  double V(double r) const { return vr(r) - shift_; }
+  double Watt(double r) const;
+  double V0(double r) const { return r < rmin_ ? V(r) - Vmin_ : 0.0; }
+  // hard-sphere diameter int_0^rmin (1 - exp(-V0/kT)) dr (Potential1.h:73-83),
+  // adaptive Gauss-Kronrod to 1e-13 instead of GSL qags at 1e-4.
+  double getHSD(double kT) const;
+
+ protected:
+  virtual double vr(double r) const = 0;
+  double sigma_, eps_, rcut_;
+  double shift_ = 0, rmin_ = 1, Vmin_ = -1;
+};
+
+class LJ : public Potential1 {
+ public:
+  LJ(double sigma, double eps, double rcut);
+  double getRmin() const override { return std::pow(2.0, 1.0 / 6.0) * sigma_; }
+
+ protected:
+  double vr(double r) const override {
+    const double y = sigma_ / r, y3 = y * y * y, y6 = y3 * y3;
+    return 4 * eps_ * (y6 * y6 - y6);
+  }
+};
+
+// ----------------------------------------------------------- interaction ---
+// include/Interaction.h:33-431: the quadratic-force interpolation kernel.
+class Interaction_Base {
+ public:
+  Interaction_Base(Species* s1, Species* s2, Potential1* v, double kT) : s1_(s1), s2_(s2), v_(v), kT_(kT) {}
+  virtual ~Interaction_Base() = default;
+  double getVDWParameter() const;  // a_vdw / kT of the lattice kernel
+  double Mu(const std::vector<double>& x, int species) const;
+  double Fhelmholtz(const std::vector<double>& x) const;
+  Species* species1() const { return s1_; }
+  Potential1* potential() const { return v_; }
+  double kT() const { return kT_; }
+
+ protected:
+  Species *s1_, *s2_;
+  Potential1* v_;
+  double kT_;
+  mutable double a_vdw_ = NAN;
+};
+
+class Interaction_Interpolation_QF : public Interaction_Base {
+ public:
+  Interaction_Interpolation_QF(Species* s1, Species* s2, Potential1* v, double kT);
+};
+
+// ------------------------------------------------------------------- DFT ---
+// include/DFT.h:35-160, src/DFT.cpp:58-425 (single species).
+class DFT {
+ public:
+  explicit DFT(Species* s = nullptr);
+  ~DFT();
+  void addSpecies(Species* s);
+  void addInteraction(Interaction_Base* i);
+  void addHardCoreContribution(FMT* fmt);
+  const Lattice& get_lattice() const { return species_.front()->getLattice(); }
+  const Density& getDensity(int s) const { return species_[s]->getDensity(); }
+  Species* getSpecies(int s) const { return species_[s]; }
+  int getNumberOfSpecies() const { return (int)species_.size(); }
+  double getNumberAtoms(int s) const { return species_[s]->getDensity().getNumberAtoms(); }
+  DFT_Vec& getDF(int s) { return species_[s]->getDF(); }
+  double calculateFreeEnergyAndDerivatives(bool onlyFex);
+  double mu_times_beta(double density) const;
+  double fhelmholtz_times_beta_over_volume(double density) const;
+  double omega_times_beta_over_volume(double density) const;
+  double get_f_id() const { return F_id_; }
+  double get_f_ext() const { return F_ext_; }
+  double get_f_hs() const { return F_hs_; }
+  double get_f_mf() const { return F_mf_; }
+  // The GPU system holding this functional (built on first use).
+  cdft_system system() const;
+  void set_precision(int p);
+
+ private:
+  std::vector<Species*> species_;
+  std::vector<Interaction_Base*> interactions_;
+  FMT* fmt_ = nullptr;
+  int precision_ = CDFT_FP64;
+  mutable cdft_system sys_ = nullptr;
+  mutable double sys_mu_ = NAN;
+  mutable unsigned sys_field_ = ~0u;
+  double F_id_ = 0, F_ext_ = 0, F_hs_ = 0, F_mf_ = 0;
+  void invalidate() const;
+};
+
+// ------------------------------------------------------------- minimizer ---
+// include/Minimizer.h:13-182, src/Minimizer.cpp.
+class Minimizer {
+ public:
+  explicit Minimizer(DFT* dft) : dft_(dft) {}
+  virtual ~Minimizer() = default;
+  void run(long maxSteps = -1);
+  void resume(long maxSteps = -1);
+  double getF() const { return F_; }
+  int getCalls() const { return calls_; }
+  void setForceTerminationCriterion(double v) { forceLimit_ = v; }
+  double getForceTerminationCriterion() const { return forceLimit_; }
+  virtual double get_convergence_monitor() const = 0;
+  virtual double step() = 0;
+
+ protected:
+  DFT* dft_;
+  double F_ = 0.0;
+  int calls_ = 0;
+  int step_counter_ = 0;
+  double forceLimit_ = 0.1;
+};
+
+class fireMinimizer2 : public Minimizer {
+ public:
+  explicit fireMinimizer2(DFT* dft);
+  ~fireMinimizer2() override;
+  double step() override;
+  // |vv| / V, src/Minimizer.h:105
+  double get_convergence_monitor() const override { return rep_.monitor; }
+  // max |dF/drho| / dV on the raw gradient (the criterion ||dF/drho||_inf of the configs)
+  double get_raw_force_max() const { return rep_.fmax_raw; }
+  double getRMS_Force() const { return rep_.rms_force; }
+  void setTimeStep(double dt) { params_.dt = dt; }
+  void setTimeStepMax(double dt) { params_.dt_max = dt; }
+  void setAlphaStart(double a) { params_.alpha_start = a; }
+  void setAlphaFac(double a) { params_.f_alf = a; }
+  // Runs FIRE steps on the GPU until max |dF/drho|/dV <= tol.
+  int run_to_raw_tolerance(long maxSteps, double tol);
+  void density(DFT_Vec& out) const;  // current density (host copy)
+
+ private:
+  void ensure();
+  cdft_fire_params params_{};
+  cdft_fire fire_ = nullptr;
+  cdft_fire_report rep_{};
+};
+
+class DDFT : public Minimizer {
+ public:
+  explicit DDFT(DFT* dft, bool showGraphics = false);
+  ~DDFT() override;
+  double step() override;
+  void set_tolerence_fixed_point(double e) { tol_ = e; }
+  void set_max_time_step(double t);
+  void setTimeStep(double dt) { dt_ = dt; }
+  double getTimeStep() const { return rep_.dt > 0 ? rep_.dt : dt_; }
+  double get_time() const { return rep_.time; }
+  double get_convergence_monitor() const override { return rep_.rhs_max; }
+  double get_mass() const { return rep_.mass; }
+  void density(DFT_Vec& out) const;
+
+ private:
+  void ensure();
+  double dt_ = 0.0, dtMax_ = 1.0, tol_ = 1e-4;
+  cdft_ddft ddft_ = nullptr;
+  cdft_ddft_report rep_{};
+};
+
+#endif  // CLASSICALDFT_AMD_H

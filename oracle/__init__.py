@@ -203,12 +203,13 @@ class DFT:
     """Whole-DFT oracle (ideal + FMT + LJ/QF mean field + V_ext - mu)."""
 
     def __init__(self, shape, dx, hsd, model=WBI, A=1.0, B=0.0, fmt=True, ideal=True, mf=False, sigma=1.0,
-                 eps=1.0, rcut=2.5, kT=1.0, mu=0.0, vext=None):
+                 eps=1.0, rcut=2.5, kT=1.0, mu=0.0, vext=None, accurate_weights=False):
         self.shape = tuple(shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dV = dxs[0] * dxs[1] * dxs[2]
         self._vext = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64)
-        self.h = lib().orc_dft_create(model, A, B, *self.shape, *dxs, hsd, int(fmt), int(ideal), int(mf), sigma, eps,
+        fmode = (2 if accurate_weights else 1) if fmt else 0
+        self.h = lib().orc_dft_create(model, A, B, *self.shape, *dxs, hsd, fmode, int(ideal), int(mf), sigma, eps,
                                       rcut, kT, mu, _ptr(self._vext))
 
     @property

@@ -995,8 +995,14 @@ orc_dft* orc_dft_create(int kind, double A, double B, int Nx, int Ny, int Nz, do
   s->ideal = ideal != 0;
   s->has_mf = has_mf != 0;
   s->mu = mu;
-  if (s->has_fmt) fmt_setup(s->fmt, nullptr);
   const long N = s->fmt.g.N();
+  if (has_fmt == 2) {  // accurate (well-conditioned, long double) weights
+    std::vector<double> w(11 * N, 0.0);
+    gen_weights<long double, true>(s->fmt.g, dx, dy, dz, hsd, w.data());
+    fmt_setup(s->fmt, w.data());
+  } else if (s->has_fmt) {
+    fmt_setup(s->fmt, nullptr);
+  }
   if (vext) s->vext.assign(vext, vext + N);
   if (s->has_mf) {
     std::vector<double> w(N, 0.0);

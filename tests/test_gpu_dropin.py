@@ -1,0 +1,85 @@
+"""The reference-named C++ API (include/classicaldft/classicaldft.h) end to end on the GPU.
+
+Runs classicaldft_amd/dropin_demo (built by build()) and checks its printed
+numbers against the oracle: config-1 F_hs with device weights vs the oracle
+on accurate weights (1e-11), bulk mu, 20 FIRE2 steps and 3 DDFT steps of a
+16^3 ideal + White Bear I + LJ/QF system (1e-10), and the LJ hard-sphere
+diameter against an independent scipy quadrature.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def seeded(shape, rho_b, seed, amp):
+    u = oracle.splitmix64_uniform(int(np.prod(shape)), seed).reshape(shape)
+    return rho_b * (1 + amp * (2 * u - 1))
+
+
+@pytest.fixture(scope="module")
+def demo(cdft):
+    exe = os.path.join(ROOT, "classicaldft_amd", "dropin_demo")
+    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout
+    vals = {}
+    for line in out.splitlines():
+        k, *v = line.split()
+        vals[k] = [float(x) for x in v]
+    return vals
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+def test_config1_through_cpp_api(demo):
+    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
+    rho = seeded(shape, 0.6, 0x5EED0001, 0.05)
+    w = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_ACCURATE)
+    F, dF = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=w)
+    assert rel(demo["F_hs"][0], F) <= 1e-11
+    assert abs(demo["dF_sum"][0] - dF.sum()) <= 1e-11 * np.abs(dF).sum()
+
+
+def test_bulk_mu_through_cpp_api(demo):
+    o = oracle.DFT((16, 16, 16), 0.1, 1.0, mf=True, kT=1.0)
+    _, muex = oracle.bulk(oracle.WBI, 0.5, 1.0)
+    assert rel(demo["mu"][0], np.log(0.5) + muex + o.a_vdw * 0.5) <= 1e-12
+
+
+def test_fire_and_ddft_through_cpp_api(demo):
+    shape, dx = (16, 16, 16), 0.1
+    o = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, mu=0.5, accurate_weights=True)
+    f = oracle.Fire(o, seeded(shape, 0.5, 11, 0.1))
+    for _ in range(20):
+        rc, info = f.step()
+    assert rel(demo["fire"][0], info["F"]) <= 1e-10
+    assert rel(demo["fire"][1], info["fmax_raw"]) <= 1e-8
+    o3 = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, accurate_weights=True)
+    rho3 = seeded(shape, 0.5, 21, 0.1)
+    d = oracle.DDFT(o3, rho3)
+    for _ in range(3):
+        r = d.step()
+    assert rel(demo["ddft"][0], r["F"]) <= 1e-10
+    assert rel(demo["ddft"][1], rho3.sum() * dx ** 3) <= 1e-12
+
+
+def test_lj_hard_sphere_diameter(demo):
+    kT, rc = 0.8, 2.5
+    rmin = 2 ** (1 / 6)
+    shift = 4 * (rc ** -12 - rc ** -6)
+
+    def V(r):
+        return 4 * (r ** -12 - r ** -6) - shift
+
+    vmin = V(rmin)
+    val, _ = integrate.quad(lambda r: 1 - np.exp(-(V(r) - vmin) / kT) if r > 0 else 1.0, 0, rmin, epsabs=1e-14,
+                            epsrel=1e-13, limit=200)
+    assert abs(demo["hsd_lj_kT0.8"][0] - val) < 1e-11
