@@ -57,6 +57,31 @@ struct CSum {
   }
 };
 
+// Runs use(i, load(i)) for i in [0, n) across the workgroup with U independent
+// global loads in flight per thread before any of them is consumed: the
+// global -> LDS phases are latency-bound unless many loads overlap.
+template <int U, typename V, typename Load, typename Use>
+__device__ __forceinline__ void batched(int n, Load load, Use use) {
+  for (int base = 0; base < n; base += U * kThreads) {
+    V v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * kThreads + (int)threadIdx.x;
+      if (i < n) v[u] = load(i);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * kThreads + (int)threadIdx.x;
+      if (i < n) use(i, v[u]);
+    }
+  }
+}
+
+template <typename T>
+struct cpair {
+  cplx<T> a, b;
+};
+
 // Block reduction of NQ compensated sums; thread 0 writes partials[blk*NQ+q].
 template <int NQ>
 __device__ void block_reduce_store(CSum* v, double2* __restrict__ partials, long blk) {
@@ -89,10 +114,8 @@ static __global__ void k_reduce_partials(const double2* __restrict__ partials, l
   const int t = threadIdx.x;
   for (int q = 0; q < nq; ++q) {
     CSum a{0.0, 0.0};
-    for (long b = t; b < nblk; b += kThreads) {
-      const double2 p = partials[b * nq + q];
-      a.merge(p.x, p.y);
-    }
+    batched<16, double2>(
+        (int)nblk, [&](int b) { return partials[(long)b * nq + q]; }, [&](int, double2 p) { a.merge(p.x, p.y); });
     rs[t] = a.s;
     rc[t] = a.c;
     __syncthreads();
@@ -148,6 +171,35 @@ __device__ __forceinline__ void load_row_c2r(const cplx<T>* __restrict__ src, cp
   if (k != 0 && 2 * k != M) row[M - k] = c2r_pre(Xmk, Xk, cconj(twN[M - k]));
 }
 
+// Batched form of load_row_c2r: the load phase fetches X_k, X_{M-k} and both
+// twiddles, the use phase writes z_k, z_{M-k} into the LDS row.
+template <typename T>
+struct c2r_item {
+  cplx<T> xk, xmk, wk, wmk;
+};
+
+template <typename T>
+__device__ __forceinline__ c2r_item<T> fetch_c2r(const cplx<T>* __restrict__ src, int M,
+                                                 const cplx<T>* __restrict__ twN, int k) {
+  return {src[k], src[M - k], twN[k], twN[M - k]};
+}
+
+template <typename T>
+__device__ __forceinline__ void put_c2r(cplx<T>* row, int M, int k, c2r_item<T> it) {
+  if (k == 0) {
+    it.xk.y = T(0);
+    it.xmk.y = T(0);
+  }
+  row[k] = c2r_pre(it.xk, it.xmk, cconj(it.wk));
+  if (k != 0 && 2 * k != M) row[M - k] = c2r_pre(it.xmk, it.xk, cconj(it.wmk));
+}
+
+template <typename T>
+__device__ __forceinline__ void zero_c2r(cplx<T>* row, int M, int k) {
+  row[k] = {T(0), T(0)};
+  if (k != 0 && 2 * k != M) row[M - k] = {T(0), T(0)};
+}
+
 template <typename T>
 __device__ __forceinline__ void store_row_r2c(const cplx<T>* row, cplx<T>* __restrict__ dst, int M,
                                               const cplx<T>* __restrict__ twN, int k) {
@@ -184,16 +236,19 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(const TI* __restrict__ in
   const TI* src = in + blockIdx.y * in_arr_stride;
   cplx<T>* dst = out + blockIdx.y * out_arr_stride;
   const int tid = threadIdx.x;
-  for (int i = tid; i < RB * M; i += kThreads) {
-    const int r = i / M, m = i % M;
-    const long row = row0 + r;
-    cplx<T> v = {T(0), T(0)};
-    if (row < nrows) {
-      const TI* p = src + row * g.Nz + 2 * m;
-      v = {T(p[0]), T(p[1])};
-    }
-    buf[i] = v;
-  }
+  batched<4, cplx<T>>(
+      RB * M,
+      [&](int i) {
+        const int r = i / M, m = i % M;
+        const long row = row0 + r;
+        cplx<T> v = {T(0), T(0)};
+        if (row < nrows) {
+          const TI* p = src + row * g.Nz + 2 * m;
+          v = {T(p[0]), T(p[1])};
+        }
+        return v;
+      },
+      [&](int i, cplx<T> v) { buf[i] = v; });
   __syncthreads();
   lds_fft<T, MAXPTS, true, false>(buf, radM, RB, 1, M, twM, tid, kThreads);
   const int H = M / 2 + 1;
@@ -219,15 +274,18 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const cplx<T>* __restrict
   TO* dst = out + blockIdx.y * out_arr_stride;
   const int tid = threadIdx.x;
   const int H = M / 2 + 1;
-  for (int i = tid; i < RB * H; i += kThreads) {
-    const int r = i / H, k = i % H;
-    const long row = row0 + r;
-    if (row < nrows) load_row_c2r(src + row * g.Zp, buf + r * M, M, twN, k);
-    else {
-      buf[r * M + k] = {T(0), T(0)};
-      if (k != 0 && 2 * k != M) buf[r * M + M - k] = {T(0), T(0)};
-    }
-  }
+  batched<4, c2r_item<T>>(
+      RB * H,
+      [&](int i) {
+        const int r = i / H, k = i % H;
+        const long row = row0 + r;
+        return row < nrows ? fetch_c2r(src + row * g.Zp, M, twN, k) : c2r_item<T>{};
+      },
+      [&](int i, c2r_item<T> it) {
+        const int r = i / H, k = i % H;
+        if (row0 + r < nrows) put_c2r(buf + r * M, M, k, it);
+        else zero_c2r(buf + r * M, M, k);
+      });
   __syncthreads();
   lds_fft<T, MAXPTS, false, false>(buf, radM, RB, 1, M, twM, tid, kThreads);
   for (int i = tid; i < RB * M; i += kThreads) {
@@ -258,11 +316,13 @@ __global__ __launch_bounds__(kThreads) void k_cols(cplx<T>* __restrict__ data, l
   const long es = axis == 0 ? (long)g.Ny * g.Zp : g.Zp;
   const long base = axis == 0 ? (long)o * g.Zp : (long)o * g.Ny * g.Zp;
   const int tid = threadIdx.x;
-  for (int i = tid; i < L * TW; i += kThreads) {
-    const int j = i / TW, t = i % TW;
-    const int kz = kz0 + t;
-    buf[i] = kz < g.Zh ? d[base + j * es + kz] : cplx<T>{T(0), T(0)};
-  }
+  batched<16, cplx<T>>(
+      L * TW,
+      [&](int i) {
+        const int j = i / TW, kz = kz0 + i % TW;
+        return kz < g.Zh ? d[base + j * es + kz] : cplx<T>{T(0), T(0)};
+      },
+      [&](int i, cplx<T> v) { buf[i] = v; });
   __syncthreads();
   lds_fft<T, MAXPTS, FWD, true>(buf, rad, TW, TW, 1, tw, tid, kThreads);
   for (int i = tid; i < L * TW; i += kThreads) {
@@ -295,22 +355,25 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(const cplx<T>* __restrict
   const long es = (long)g.Ny * g.Zp;
   const long base = (long)y * g.Zp;
   const int tid = threadIdx.x;
-  for (int i = tid; i < L * TW; i += kThreads) {
-    const int j = i / TW, t = i % TW;
-    const int kz = kz0 + t;
-    bufR[i] = kz < g.Zh ? A[base + j * es + kz] : cplx<T>{T(0), T(0)};
-  }
+  batched<8, cplx<T>>(
+      L * TW,
+      [&](int i) {
+        const int j = i / TW, kz = kz0 + i % TW;
+        return kz < g.Zh ? A[base + j * es + kz] : cplx<T>{T(0), T(0)};
+      },
+      [&](int i, cplx<T> v) { bufR[i] = v; });
   __syncthreads();
   lds_fft<T, MAXPTS, true, true>(bufR, radX, TW, TW, 1, twX, tid, kThreads);
   for (int a = 0; a < nconv; ++a) {
     const T* Wa = W + a * arr_stride;
     const bool odd = (odd_mask >> a) & 1u;
-    for (int i = tid; i < L * TW; i += kThreads) {
-      const int j = i / TW, t = i % TW;
-      const int kz = kz0 + t;
-      const T w = kz < g.Zh ? Wa[base + j * es + kz] : T(0);
-      bufW[i] = wmul(bufR[i], w, odd);
-    }
+    batched<8, T>(
+        L * TW,
+        [&](int i) {
+          const int j = i / TW, kz = kz0 + i % TW;
+          return kz < g.Zh ? Wa[base + j * es + kz] : T(0);
+        },
+        [&](int i, T w) { bufW[i] = wmul(bufR[i], w, odd); });
     __syncthreads();
     lds_fft<T, MAXPTS, false, true>(bufW, radX, TW, TW, 1, twX, tid, kThreads);
     cplx<T>* Ba = B + a * arr_stride;
@@ -342,22 +405,27 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd(const cplx<T>* __restrict
   for (int i = tid; i < L * TW; i += kThreads) bufA[i] = {T(0), T(0)};
   for (int a = 0; a < m; ++a) {
     const cplx<T>* Da = D + a * arr_stride;
-    for (int i = tid; i < L * TW; i += kThreads) {
-      const int j = i / TW, t = i % TW;
-      const int kz = kz0 + t;
-      bufW[i] = kz < g.Zh ? Da[base + j * es + kz] : cplx<T>{T(0), T(0)};
-    }
+    batched<8, cplx<T>>(
+        L * TW,
+        [&](int i) {
+          const int j = i / TW, kz = kz0 + i % TW;
+          return kz < g.Zh ? Da[base + j * es + kz] : cplx<T>{T(0), T(0)};
+        },
+        [&](int i, cplx<T> v) { bufW[i] = v; });
     __syncthreads();
     lds_fft<T, MAXPTS, true, true>(bufW, radX, TW, TW, 1, twX, tid, kThreads);
     const T* Wa = W + a * arr_stride;
     const bool odd = (odd_mask >> a) & 1u;
-    for (int i = tid; i < L * TW; i += kThreads) {
-      const int j = i / TW, t = i % TW;
-      const int kz = kz0 + t;
-      const T w = kz < g.Zh ? Wa[base + j * es + kz] : T(0);
-      const cplx<T> p = wmul(bufW[i], w, odd);
-      bufA[i] = {bufA[i].x + p.x, bufA[i].y + p.y};
-    }
+    batched<8, T>(
+        L * TW,
+        [&](int i) {
+          const int j = i / TW, kz = kz0 + i % TW;
+          return kz < g.Zh ? Wa[base + j * es + kz] : T(0);
+        },
+        [&](int i, T w) {
+          const cplx<T> p = wmul(bufW[i], w, odd);
+          bufA[i] = {bufA[i].x + p.x, bufA[i].y + p.y};
+        });
     __syncthreads();
   }
   lds_fft<T, MAXPTS, false, true>(bufA, radX, TW, TW, 1, twX, tid, kThreads);
@@ -388,18 +456,19 @@ __global__ __launch_bounds__(kThreads) void k_z_pointwise(cplx<T>* __restrict__ 
   const int tid = threadIdx.x;
   const int H = M / 2 + 1;
   // 1. load + c2r pre-processing for all arrays and rows
-  for (int i = tid; i < nconv * RB * H; i += kThreads) {
-    const int k = i % H;
-    const int r = (i / H) % RB;
-    const int a = i / (H * RB);
-    const long row = row0 + r;
-    cplx<T>* lrow = buf + (a * RB + r) * M;
-    if (row < nrows) load_row_c2r(B + a * arr_stride + row * g.Zp, lrow, M, twN, k);
-    else {
-      lrow[k] = {T(0), T(0)};
-      if (k != 0 && 2 * k != M) lrow[M - k] = {T(0), T(0)};
-    }
-  }
+  batched<8, c2r_item<T>>(
+      nconv * RB * H,
+      [&](int i) {
+        const int k = i % H, r = (i / H) % RB, a = i / (H * RB);
+        const long row = row0 + r;
+        return row < nrows ? fetch_c2r(B + a * arr_stride + row * g.Zp, M, twN, k) : c2r_item<T>{};
+      },
+      [&](int i, c2r_item<T> it) {
+        const int k = i % H, r = (i / H) % RB, a = i / (H * RB);
+        cplx<T>* lrow = buf + (a * RB + r) * M;
+        if (row0 + r < nrows) put_c2r(lrow, M, k, it);
+        else zero_c2r(lrow, M, k);
+      });
   __syncthreads();
   lds_fft<T, MAXPTS, false, false>(buf, radM, nconv * RB, 1, M, twM, tid, kThreads);
   // 2. pointwise
@@ -461,47 +530,66 @@ __global__ __launch_bounds__(kThreads) void k_z_final(const cplx<T>* __restrict_
   const int tid = threadIdx.x;
   const int H = M / 2 + 1;
   if (fp.fmt) {
-    for (int i = tid; i < RB * H; i += kThreads) {
-      const int r = i / H, k = i % H;
-      const long row = row0 + r;
-      if (row < nrows) load_row_c2r(Acc + row * g.Zp, buf + r * M, M, twN, k);
-      else {
-        buf[r * M + k] = {T(0), T(0)};
-        if (k != 0 && 2 * k != M) buf[r * M + M - k] = {T(0), T(0)};
-      }
-    }
+    batched<4, c2r_item<T>>(
+        RB * H,
+        [&](int i) {
+          const int r = i / H, k = i % H;
+          const long row = row0 + r;
+          return row < nrows ? fetch_c2r(Acc + row * g.Zp, M, twN, k) : c2r_item<T>{};
+        },
+        [&](int i, c2r_item<T> it) {
+          const int r = i / H, k = i % H;
+          if (row0 + r < nrows) put_c2r(buf + r * M, M, k, it);
+          else zero_c2r(buf + r * M, M, k);
+        });
     __syncthreads();
     lds_fft<T, MAXPTS, false, false>(buf, radM, RB, 1, M, twM, tid, kThreads);
   }
   const T* bufT = reinterpret_cast<const T*>(buf);
   CSum q[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  for (int s = tid; s < RB * 2 * M; s += kThreads) {
-    const int r = s / (2 * M), z = s % (2 * M);
-    const long row = row0 + r;
-    if (row >= nrows) continue;
-    const long p = row * g.Nz + z;
-    const double d0 = rho[p];
-    double d = fp.accumulate ? dF[p] : 0.0;
-    if (fp.ideal) {
-      const double l = log(d0);
-      q[0].add((d0 * l - d0) * fp.dV);
-      d += l * fp.dV;
-    }
-    if (fp.fmt) d += double(bufT[(long)r * 2 * M + z]) * fp.dV;
-    if (fp.mf) {
-      const double v = vmf[p];
-      d += v;
-      q[1].add(d0 * v);
-    }
-    if (fp.ext) {
-      const double ve = vext[p];
-      q[2].add(ve * d0);
-      d += ve * fp.dV;
-    }
-    q[3].add(d0);
-    d += -fp.mu * fp.dV;
-    dF[p] = d;
-  }
+  struct Site {
+    double rho, df, vm, ve;
+  };
+  const long nsite = (long)RB * 2 * M;
+  batched<4, Site>(
+      (int)nsite,
+      [&](int s) {
+        const long row = row0 + s / (2 * M);
+        Site v{1.0, 0.0, 0.0, 0.0};
+        if (row < nrows) {
+          const long p = row * g.Nz + s % (2 * M);
+          v.rho = rho[p];
+          if (fp.accumulate) v.df = dF[p];
+          if (fp.mf) v.vm = vmf[p];
+          if (fp.ext) v.ve = vext[p];
+        }
+        return v;
+      },
+      [&](int s, Site v) {
+        const int r = s / (2 * M), z = s % (2 * M);
+        const long row = row0 + r;
+        if (row >= nrows) return;
+        const long p = row * g.Nz + z;
+        const double d0 = v.rho;
+        double d = v.df;
+        if (fp.ideal) {
+          const double l = log(d0);
+          q[0].add((d0 * l - d0) * fp.dV);
+          d += l * fp.dV;
+        }
+        if (fp.fmt) d += double(bufT[(long)r * 2 * M + z]) * fp.dV;
+        if (fp.mf) {
+          d += v.vm;
+          q[1].add(d0 * v.vm);
+        }
+        if (fp.ext) {
+          q[2].add(v.ve * d0);
+          d += v.ve * fp.dV;
+        }
+        q[3].add(d0);
+        d += -fp.mu * fp.dV;
+        dF[p] = d;
+      });
   block_reduce_store<4>(q, partials, blockIdx.x);
 }
 
