@@ -135,6 +135,8 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
   s->TWcol = std::max(1, std::min(16, 4096 / Lmax));
   const int esz = s->mixed ? 8 : 16;
   s->TWconv = std::max(1, std::min(16, (65536 / (2 * esz)) / Lmax));
+  if (const char* e = std::getenv("CDFT_LDS_FFT")) s->reg_fft = !(e[0] == '1');
+  if (const char* e = std::getenv("CDFT_NB_HALF")) s->nb_half = e[0] == '1';
   s->twd.build(g);
   if (s->mixed) s->twf.build(g);
   const size_t cs = s->mixed ? sizeof(cplx<float>) : sizeof(cplx<double>);
@@ -200,7 +202,18 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
     s.mark("z_r2c");
     launch_cols<T>(s, A, 0, 1, 1, true);
     s.mark("y_fwd");
-    {
+    if (reg_fft_ok<T>(s, s.g.Nx)) {
+      const ColGeo cg = col_geo<T>(s.g, 0);
+      with_reg_len(s.g.Nx, [&](auto lc) {
+        constexpr int Lc = decltype(lc)::value;
+        with_nb<T, Lc>(s.nb_half, [&](auto nbc) {
+          constexpr int NB = decltype(nbc)::value;
+          hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0, s.stream,
+                             A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+          HIPCHK(hipGetLastError());
+        });
+      });
+    } else {
       const int TW = s.TWconv;
       dim3 grid(div_up(s.g.Zh, TW), s.g.Ny);
       hipLaunchKernelGGL((k_conv_fwd<T, 16>), grid, dim3(kThreads), 2 * sizeof(cplx<T>) * s.g.Nx * TW, s.stream, A, B,
@@ -223,7 +236,18 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
       s.mark("z_pointwise");
       launch_cols<T>(s, B, Nkp, s.m, 1, true);
       s.mark("y_fwd2");
-      {
+      if (reg_fft_ok<T>(s, s.g.Nx)) {
+        const ColGeo cg = col_geo<T>(s.g, 0);
+        with_reg_len(s.g.Nx, [&](auto lc) {
+          constexpr int Lc = decltype(lc)::value;
+          with_nb<T, Lc>(s.nb_half, [&](auto nbc) {
+            constexpr int NB = decltype(nbc)::value;
+            hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
+                               s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+            HIPCHK(hipGetLastError());
+          });
+        });
+      } else {
         const int TW = s.TWconv;
         dim3 grid(div_up(s.g.Zh, TW), s.g.Ny);
         hipLaunchKernelGGL((k_conv_bwd<T, 16>), grid, dim3(kThreads), 2 * sizeof(cplx<T>) * s.g.Nx * TW, s.stream, B,

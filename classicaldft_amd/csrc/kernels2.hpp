@@ -1,0 +1,206 @@
+// Column-pass kernels (x and y transforms) built on the register FFT of
+// fft2.hpp.  Same data layout and results as k_cols / k_conv_fwd / k_conv_bwd
+// in kernels.hpp.
+//
+// Columns are numbered flat: column c lives at element (c / cz) * os + c % cz,
+// its j-th element es further on.  For the x pass cz = Ny * Zp (every (y, kz)
+// pair, padding included), for the y pass cz = Zp, os = Ny * Zp.  A block
+// handles NB consecutive columns with P = gcd(R1, R2) threads each (lane
+// layout: column fastest), so small grids still fill whole wavefronts and the
+// loads of one wavefront cover NB contiguous elements per row.
+//
+// Addressing: every array is read and written through a buffer resource;
+// the part of an offset common to the block (row of the transform) goes in the
+// scalar offset, the per-lane part in one 32-bit vector offset per sequence,
+// so a thread keeps one offset register per sequence instead of a 64-bit
+// address per element.  Arrays are below 2 GiB (checked by the host).
+#pragma once
+#include "fft2.hpp"
+#include "kernels.hpp"
+
+namespace cdft {
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename U>
+__device__ __forceinline__ U buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(U) == 16)
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+  else if constexpr (sizeof(U) == 8)
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+  else
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+template <typename U>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, U v) {
+  if constexpr (sizeof(U) == 16) {
+    using V = __attribute__((ext_vector_type(4))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, v), r, (int)voff, (int)soff, 0);
+  } else if constexpr (sizeof(U) == 8) {
+    using V = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, (int)voff, (int)soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+  }
+}
+
+struct ColGeo {
+  unsigned bytes;  // bytes of one complex array (Nx * Ny * Zp elements)
+  unsigned ncols;  // number of columns
+  unsigned cz;     // columns per contiguous group
+  unsigned os;     // element stride between groups
+  unsigned es;     // element stride along the transform
+};
+
+// In-place transform of ncols columns per array.  grid = (ceil(ncols/NB), narr), block = NB * P.
+template <typename T, int L, int NB, bool FWD>
+__global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long arr_stride, ColGeo cg,
+                                               const cplx<T>* __restrict__ tw) {
+  using F = F2<L>;
+  constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
+  __shared__ cplx<T> buf[L * NB];
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  const unsigned c0 = blockIdx.x * NB + b;
+  const bool ok = c0 < cg.ncols;
+  const unsigned c = ok ? c0 : cg.ncols - 1;  // tail lanes recompute the last column, store nothing
+  const auto rd = buf_rsrc(data + blockIdx.y * arr_stride, cg.bytes);
+  const unsigned col = (c / cg.cz) * cg.os + c % cg.cz;
+  constexpr unsigned E = sizeof(cplx<T>);
+  cplx<T> v1[Q1][R1], v2[Q2][R2];
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) {
+    const unsigned vo = (col + (t + P * q) * cg.es) * E;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1)
+      v1[q][n1] = buf_ld<cplx<T>>(rd, vo, R2 * n1 * cg.es * E);
+  }
+  reg_fft_seq<T, L, R1, R2, FWD>(v1, v2, t, buf + b, NB, tw);
+  if (!ok) return;
+#pragma unroll
+  for (int q = 0; q < Q2; ++q) {
+    const unsigned vo = (col + (t + P * q) * cg.es) * E;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) buf_st(rd, vo, R1 * k2 * cg.es * E, v2[q][k2]);
+  }
+}
+
+// x pass of rho_k, then per weight a: n_a(k) = rho_k w_a(k) and its inverse x
+// pass, straight from registers.  grid = ceil(ncols/NB), block = NB * P.
+template <typename T, int L, int NB>
+__global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A, cplx<T>* __restrict__ B,
+                                                   long arr_stride, const T* __restrict__ W, int nconv,
+                                                   unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
+  using F = F2<L>;
+  constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
+  __shared__ cplx<T> buf[L * NB];
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  const unsigned c0 = blockIdx.x * NB + b;
+  const bool ok = c0 < cg.ncols;
+  const unsigned c = ok ? c0 : cg.ncols - 1;  // tail lanes recompute the last column, store nothing
+  constexpr unsigned E = sizeof(cplx<T>), EW = sizeof(T);
+  const auto ra = buf_rsrc(A, cg.bytes);
+  cplx<T> v1[Q1][R1], rk[Q2][R2];
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) {
+    const unsigned vo = (c + (t + P * q) * cg.es) * E;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1)
+      v1[q][n1] = buf_ld<cplx<T>>(ra, vo, R2 * n1 * cg.es * E);
+  }
+  reg_fft_seq<T, L, R1, R2, true>(v1, rk, t, buf + b, NB, tw);
+  for (int a = 0; a < nconv; ++a) {
+    const auto rw = buf_rsrc(W + a * arr_stride, cg.bytes / 2);
+    const auto rb = buf_rsrc(B + a * arr_stride, cg.bytes);
+    const bool odd = (odd_mask >> a) & 1u;
+    cplx<T> u1[Q2][R2], u2[Q1][R1];
+#pragma unroll
+    for (int q = 0; q < Q2; ++q) {
+      const unsigned vo = (c + (t + P * q) * cg.es) * EW;
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        const T w = buf_ld<T>(rw, vo, R1 * k2 * cg.es * EW);
+        u1[q][k2] = wmul(rk[q][k2], w, odd);
+      }
+    }
+    // Re-read the twiddles in every iteration: kept live across the loop they
+    // would cost 4 (L/P - 1) registers and halve the occupancy.
+    int zero = 0;
+    asm volatile("" : "+s"(zero));
+    const cplx<T>* twa = tw + zero;
+    __syncthreads();  // the exchange slot is reused
+    reg_fft_seq<T, L, R2, R1, false>(u1, u2, t, buf + b, NB, twa);
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < Q1; ++q) {
+        const unsigned vo = (c + (t + P * q) * cg.es) * E;
+#pragma unroll
+        for (int k2 = 0; k2 < R1; ++k2) buf_st(rb, vo, R2 * k2 * cg.es * E, u2[q][k2]);
+      }
+    }
+  }
+}
+
+// x pass of each dPhi_a, acc = sum_a w_a(k) dPhi_a(k) in registers, inverse x
+// pass of acc.  grid = ceil(ncols/NB), block = NB * P.
+template <typename T, int L, int NB>
+__global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D, cplx<T>* __restrict__ Acc,
+                                                   long arr_stride, const T* __restrict__ W, int m,
+                                                   unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
+  using F = F2<L>;
+  constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
+  __shared__ cplx<T> buf[L * NB];
+  const int b = threadIdx.x % NB, t = threadIdx.x / NB;
+  const unsigned c0 = blockIdx.x * NB + b;
+  const bool ok = c0 < cg.ncols;
+  const unsigned c = ok ? c0 : cg.ncols - 1;  // tail lanes recompute the last column, store nothing
+  constexpr unsigned E = sizeof(cplx<T>), EW = sizeof(T);
+  cplx<T> acc[Q2][R2];
+#pragma unroll
+  for (int q = 0; q < Q2; ++q)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) acc[q][k2] = {T(0), T(0)};
+  for (int a = 0; a < m; ++a) {
+    const auto rd = buf_rsrc(D + a * arr_stride, cg.bytes);
+    const auto rw = buf_rsrc(W + a * arr_stride, cg.bytes / 2);
+    const bool odd = (odd_mask >> a) & 1u;
+    cplx<T> v1[Q1][R1], v2[Q2][R2];
+#pragma unroll
+    for (int q = 0; q < Q1; ++q) {
+      const unsigned vo = (c + (t + P * q) * cg.es) * E;
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1)
+        v1[q][n1] = buf_ld<cplx<T>>(rd, vo, R2 * n1 * cg.es * E);
+    }
+    int zero = 0;  // re-read the twiddles per iteration (register pressure)
+    asm volatile("" : "+s"(zero));
+    const cplx<T>* twa = tw + zero;
+    if (a) __syncthreads();  // the exchange slot is reused
+    reg_fft_seq<T, L, R1, R2, true>(v1, v2, t, buf + b, NB, twa);
+#pragma unroll
+    for (int q = 0; q < Q2; ++q) {
+      const unsigned vo = (c + (t + P * q) * cg.es) * EW;
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        const T w = buf_ld<T>(rw, vo, R1 * k2 * cg.es * EW);
+        const cplx<T> p = wmul(v2[q][k2], w, odd);
+        acc[q][k2] = {acc[q][k2].x + p.x, acc[q][k2].y + p.y};
+      }
+    }
+  }
+  cplx<T> u2[Q1][R1];
+  __syncthreads();
+  reg_fft_seq<T, L, R2, R1, false>(acc, u2, t, buf + b, NB, tw);
+  if (!ok) return;
+  const auto racc = buf_rsrc(Acc, cg.bytes);
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) {
+    const unsigned vo = (c + (t + P * q) * cg.es) * E;
+#pragma unroll
+    for (int k2 = 0; k2 < R1; ++k2) buf_st(racc, vo, R2 * k2 * cg.es * E, u2[q][k2]);
+  }
+}
+
+}  // namespace cdft

@@ -11,8 +11,12 @@
 #include <string>
 #include <vector>
 
+#include <cstdlib>
+#include <type_traits>
+
 #include "../../include/cdft.h"
 #include "kernels.hpp"
+#include "kernels2.hpp"
 #include "weights.hpp"
 
 using namespace cdft;
@@ -127,6 +131,8 @@ struct cdft_system_s {
   DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
   double a_vdw = 0.0;
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
+  bool reg_fft = true;  // register two-pass FFT for the x / y passes where the length allows
+  bool nb_half = false;  // half the columns per block in those kernels
   bool has_vext = false;
   bool last_with_dft = true;
   // timing
@@ -184,8 +190,90 @@ inline void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_s
   HIPCHK(hipGetLastError());
 }
 
+// Calls f(std::integral_constant<int, L>{}) when the register FFT handles
+// length L; returns false otherwise.
+template <typename Fn>
+inline bool with_reg_len(int L, Fn&& f) {
+  switch (L) {
+#define CDFT_REG_LEN(n)                       \
+  case n:                                     \
+    f(std::integral_constant<int, n>{});      \
+    return true;
+    CDFT_REG_LEN(256) CDFT_REG_LEN(128) CDFT_REG_LEN(96) CDFT_REG_LEN(64) CDFT_REG_LEN(48) CDFT_REG_LEN(32)
+    CDFT_REG_LEN(24) CDFT_REG_LEN(16) CDFT_REG_LEN(12) CDFT_REG_LEN(8) CDFT_REG_LEN(6) CDFT_REG_LEN(4)
+#undef CDFT_REG_LEN
+    default: return false;
+  }
+}
+
+// Columns per block of the register FFT kernels: NB * P <= 256 threads and an
+// exchange buffer of L * NB elements of at most 64 KiB; `half` halves NB.
+template <typename T, int L>
+constexpr int reg_nb_max() {
+  int nb = kThreads / F2<L>::P;
+  while (nb > 1 && nb * L * int(sizeof(cplx<T>)) > 65536) nb /= 2;
+  return nb;
+}
+
+template <typename T, int L, typename Fn>
+inline void with_nb(bool half, Fn&& f) {
+  constexpr int nb = reg_nb_max<T, L>();
+  if (half && nb > 1)
+    f(std::integral_constant<int, nb / 2>{});
+  else
+    f(std::integral_constant<int, nb>{});
+}
+
+// Flat column numbering of the x (axis 0) or y (axis 1) pass.
+template <typename T>
+inline ColGeo col_geo(const Geo& g, int axis) {
+  ColGeo c{};
+  c.bytes = unsigned((long)g.Nx * g.Ny * g.Zp * (long)sizeof(cplx<T>));
+  if (axis == 0) {
+    c.ncols = unsigned(g.Ny * g.Zp);
+    c.cz = c.ncols;
+    c.os = 0;
+    c.es = unsigned(g.Ny * g.Zp);
+  } else {
+    c.ncols = unsigned(g.Nx * g.Zp);
+    c.cz = unsigned(g.Zp);
+    c.os = unsigned(g.Ny * g.Zp);
+    c.es = unsigned(g.Zp);
+  }
+  return c;
+}
+
+// The register kernels address one spectral array through a buffer resource
+// (byte offsets below 2^31).
+template <typename T>
+inline bool reg_fft_ok(const cdft_system_s& s, int L) {
+  return s.reg_fft && s.Nkp * (long)sizeof(cplx<T>) < (1L << 31) && with_reg_len(L, [](auto) {});
+}
+
+template <typename T>
+inline bool launch_cols_reg(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd) {
+  const int L = axis == 0 ? s.g.Nx : s.g.Ny;
+  if (!reg_fft_ok<T>(s, L)) return false;
+  const Tables<T>& tw = tables<T>(s);
+  const cplx<T>* t = (axis == 0 ? tw.x : tw.y).template as<cplx<T>>();
+  const ColGeo cg = col_geo<T>(s.g, axis);
+  return with_reg_len(L, [&](auto lc) {
+    constexpr int Lc = decltype(lc)::value;
+    with_nb<T, Lc>(s.nb_half, [&](auto nbc) {
+      constexpr int NB = decltype(nbc)::value;
+      const dim3 grid(div_up(cg.ncols, NB), narr), block(NB * F2<Lc>::P);
+      if (fwd)
+        hipLaunchKernelGGL((k_cols2<T, Lc, NB, true>), grid, block, 0, s.stream, data, stride, cg, t);
+      else
+        hipLaunchKernelGGL((k_cols2<T, Lc, NB, false>), grid, block, 0, s.stream, data, stride, cg, t);
+      HIPCHK(hipGetLastError());
+    });
+  });
+}
+
 template <typename T>
 inline void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd) {
+  if (launch_cols_reg<T>(s, data, stride, narr, axis, fwd)) return;
   const Tables<T>& tw = tables<T>(s);
   const Radices& rad = axis == 0 ? s.radX : s.radY;
   const cplx<T>* t = (axis == 0 ? tw.x : tw.y).template as<cplx<T>>();

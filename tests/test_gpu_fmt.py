@@ -31,7 +31,8 @@ def maxrel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 18), (32, 32, 32), (48, 48, 48), (24, 16, 96), (64, 64, 64)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 18), (32, 32, 32), (48, 48, 48), (24, 16, 96), (64, 64, 64),
+                                   (256, 6, 8), (4, 256, 12), (128, 96, 4), (96, 128, 6), (12, 24, 12), (6, 4, 16)])
 def test_fft_matches_numpy(cdft, shape):
     rng = np.random.default_rng(1)
     x = rng.standard_normal(shape)
@@ -65,6 +66,20 @@ def test_all_models_fp64_parity(cdft, model, A, B):
     F0, dF0 = oracle.fmt_eval(model, rho, dx, hsd, weights=w, A=A, B=B)
     s = cdft.System(shape, dx, hsd, model=model, weights=w, A=A, B=B)
     F1, dF1 = s.fmt_eval(rho)
+    assert rel(F1, F0) <= TOL
+    assert maxrel(dF1, dF0) <= TOL
+
+
+@pytest.mark.parametrize("shape", [(128, 24, 16), (12, 256, 24), (96, 48, 12)])
+def test_long_axes_fp64_parity(cdft, shape):
+    """x / y transforms of lengths 256, 128, 96, 48, 24, 12 inside the full F + dF pipeline."""
+    dx, hsd = 0.1, 1.0
+    rho = seeded_density(shape, 0.55, 0x5EED0300 + shape[0], amp=0.1)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    F0, dF0 = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=w)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, weights=w)
+    F1, dF1 = s.fmt_eval(rho)
+    print(shape, "F", rel(F1, F0), "dF", maxrel(dF1, dF0))
     assert rel(F1, F0) <= TOL
     assert maxrel(dF1, dF0) <= TOL
 
@@ -168,3 +183,17 @@ def test_mixed_precision_error(cdft):
     print("mixed F rel err", rel(F1, F0), "dF", maxrel(dF1, dF0))
     assert rel(F1, F0) < 1e-6
     assert maxrel(dF1, dF0) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_register_fft_matches_lds_fft(cdft, monkeypatch, precision):
+    """The register two-pass FFT kernels and the LDS Stockham kernels give the same F and dF."""
+    shape, dx, hsd = (64, 48, 32), 0.1, 1.0
+    rho = seeded_density(shape, 0.6, 0x5EED0400)
+    F1, dF1 = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, precision=precision).fmt_eval(rho)
+    monkeypatch.setenv("CDFT_LDS_FFT", "1")
+    F0, dF0 = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, precision=precision).fmt_eval(rho)
+    tol = 1e-13 if precision == 0 else 2e-6
+    print("reg vs lds", rel(F1, F0), maxrel(dF1, dF0))
+    assert rel(F1, F0) <= tol
+    assert maxrel(dF1, dF0) <= tol
