@@ -137,6 +137,7 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
   s->TWconv = std::max(1, std::min(16, (65536 / (2 * esz)) / Lmax));
   if (const char* e = std::getenv("CDFT_LDS_FFT")) s->reg_fft = !(e[0] == '1');
   if (const char* e = std::getenv("CDFT_NB_HALF")) s->nb_half = e[0] == '1';
+  if (const char* e = std::getenv("CDFT_PW_ROWS")) s->pw_rows = std::atoi(e);
   s->twd.build(g);
   if (s->mixed) s->twf.build(g);
   const size_t cs = s->mixed ? sizeof(cplx<float>) : sizeof(cplx<double>);
@@ -149,7 +150,7 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
   if (desc->mean_field) s->vmf.alloc(sizeof(double) * s->N);
   s->rho.alloc(sizeof(double) * s->N);
   s->dF.alloc(sizeof(double) * s->N);
-  const long maxblk = std::max<long>(std::max<long>(div_up(s->nrows, s->RBpw), div_up(s->nrows, s->RBrow)), 256);
+  const long maxblk = std::max<long>(s->nrows, 256);  // at least one row per block in every z kernel
   s->partials.alloc(sizeof(double2) * maxblk);
   s->partials2.alloc(sizeof(double2) * maxblk * 4);
   s->flag.alloc(sizeof(int));
@@ -173,17 +174,36 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
 }
 
 template <typename T, int MODEL>
-void launch_pointwise(cdft_system_s& s, int nconv) {
+long launch_pointwise(cdft_system_s& s, int nconv) {
   const Tables<T>& tw = tables<T>(s);
   const int M = s.g.Nz / 2;
-  const int RB = s.RBpw;
-  PhiParams pp{s.d.hsd, s.d.A, s.d.B};
+  PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
   if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
+  if (rows_reg_ok<T>(s)) {
+    long nblk = -1;
+    with_reg_len(M, [&](auto mc) {
+      constexpr int Mc = decltype(mc)::value;
+      using RF = RowF<T, Mc>;
+      const int per = nconv * RF::SLOT * int(sizeof(cplx<T>));
+      int RB = std::min(kThreads / (nconv * RF::P), (65536 - 4096) / per);
+      if (s.pw_rows > 0) RB = std::min(RB, s.pw_rows);
+      if (RB < 1) return;
+      nblk = div_up(s.nrows, RB);
+      hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk), dim3(nconv * RB * RF::P), size_t(RB) * per,
+                         s.stream, s.B.as<cplx<T>>(), s.Nkp, unsigned(s.Nkp * sizeof(cplx<T>)), s.m, nconv,
+                         s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<T>>(),
+                         tw.n.template as<cplx<T>>(), RB, pp, s.partials.as<double2>(), s.flag.as<int>());
+      HIPCHK(hipGetLastError());
+    });
+    if (nblk > 0) return nblk;
+  }
+  const int RB = s.RBpw;
   const size_t smem = sizeof(cplx<T>) * nconv * RB * M;
   hipLaunchKernelGGL((k_z_pointwise<T, MODEL, 16>), dim3(div_up(s.nrows, RB)), dim3(kThreads), smem, s.stream,
                      s.B.as<cplx<T>>(), s.Nkp, s.m, nconv, s.vmf.as<double>(), s.g, s.radM, tw.m.template as<cplx<T>>(),
                      tw.n.template as<cplx<T>>(), RB, pp, s.partials.as<double2>(), s.flag.as<int>());
   HIPCHK(hipGetLastError());
+  return div_up(s.nrows, RB);
 }
 
 // The full evaluation on device pointers.  with_dft = false: FMT only.
@@ -224,15 +244,16 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
     launch_cols<T>(s, B, Nkp, nconv, 1, false);
     s.mark("y_inv");
     if (s.m > 0) {
+      long nblk = 0;
       switch (s.d.model) {
-        case CDFT_ROSENFELD: launch_pointwise<T, M_ROSENFELD>(s, nconv); break;
-        case CDFT_RSLT: launch_pointwise<T, M_RSLT>(s, nconv); break;
-        case CDFT_ESFMT: launch_pointwise<T, M_ESFMT>(s, nconv); break;
-        case CDFT_WHITEBEAR_I: launch_pointwise<T, M_WBI>(s, nconv); break;
-        default: launch_pointwise<T, M_WBII>(s, nconv); break;
+        case CDFT_ROSENFELD: nblk = launch_pointwise<T, M_ROSENFELD>(s, nconv); break;
+        case CDFT_RSLT: nblk = launch_pointwise<T, M_RSLT>(s, nconv); break;
+        case CDFT_ESFMT: nblk = launch_pointwise<T, M_ESFMT>(s, nconv); break;
+        case CDFT_WHITEBEAR_I: nblk = launch_pointwise<T, M_WBI>(s, nconv); break;
+        default: nblk = launch_pointwise<T, M_WBII>(s, nconv); break;
       }
-      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials.as<double2>(),
-                         (long)div_up(s.nrows, s.RBpw), 1, s.results.as<double>());
+      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+                         s.results.as<double>());
       s.mark("z_pointwise");
       launch_cols<T>(s, B, Nkp, s.m, 1, true);
       s.mark("y_fwd2");
@@ -273,13 +294,27 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
   fp.accumulate = accumulate;
   {
     const int M = s.g.Nz / 2;
-    const int RB = s.RBrow;
-    hipLaunchKernelGGL((k_z_final<T, 4>), dim3(div_up(s.nrows, RB)), dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream,
-                       A, rho, s.vext.as<double>(), s.vmf.as<double>(), dF, s.g, s.radM, tw.m.template as<cplx<T>>(),
-                       tw.n.template as<cplx<T>>(), RB, fp, s.partials2.as<double2>());
+    long nblk = 0;
+    if (rows_reg_ok<T>(s)) {
+      with_reg_len(M, [&](auto mc) {
+        constexpr int Mc = decltype(mc)::value;
+        constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
+        nblk = div_up(s.nrows, RB);
+        hipLaunchKernelGGL((k_z_final2<T, Mc>), dim3(nblk), dim3(RB * P), RB * SLOT * sizeof(cplx<T>), s.stream, A, unsigned(s.Nkp * sizeof(cplx<T>)),
+                           rho, s.vext.as<double>(), s.vmf.as<double>(), dF, unsigned(s.N * 8), s.nrows, s.g.Zp,
+                           tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, fp,
+                           s.partials2.as<double2>());
+      });
+    } else {
+      const int RB = s.RBrow;
+      nblk = div_up(s.nrows, RB);
+      hipLaunchKernelGGL((k_z_final<T, 4>), dim3(nblk), dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, A, rho,
+                         s.vext.as<double>(), s.vmf.as<double>(), dF, s.g, s.radM, tw.m.template as<cplx<T>>(),
+                         tw.n.template as<cplx<T>>(), RB, fp, s.partials2.as<double2>());
+    }
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials2.as<double2>(),
-                       (long)div_up(s.nrows, RB), 4, s.results.as<double>() + 1);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials2.as<double2>(), nblk, 4,
+                       s.results.as<double>() + 1);
     HIPCHK(hipGetLastError());
   }
   s.mark("z_final");
@@ -536,7 +571,7 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
                              M_PI * d * d / 3};
       double n[11], out[11];
       for (int a = 0; a < 11; ++a) n[a] = rho * om[a];
-      PhiParams pp{d, sys->d.A, sys->d.B};
+      PhiParams pp = make_phi_params(d, sys->d.A, sys->d.B);
       if (sys->d.model == CDFT_WHITEBEAR_I || sys->d.model == CDFT_WHITEBEAR_II) { pp.A = 1.0; pp.B = -1.0; }
       switch (sys->d.model) {
         case CDFT_ROSENFELD: fex = phi_site<M_ROSENFELD>(n, pp, out); break;

@@ -107,13 +107,14 @@ constexpr bool reg_fft_supported() {
 // One transform of a sequence shared by P threads (lane t in [0, P)).
 // Input  v1[q][n1] = x[R2 n1 + n2], n2 = t + P q   (q < Q1)
 // Output v2[q][k2] = X[k1 + R1 k2], k1 = t + P q   (q < Q2)
-// lds: this sequence's L-element exchange slot, element e at lds[e * ls];
+// lds: this sequence's exchange slot, element (k1, n2) at lds[(k1 (R2 + PAD) + n2) ls]
+// (PAD = 1 breaks the power-of-two stride of the pass-2 reads when ls = 1);
 // tw: W_L^j table (j < L).  The caller must __syncthreads() before reusing lds.
-template <typename T, int L, int R1, int R2, bool FWD>
+template <typename T, int L, int R1, int R2, bool FWD, int PAD = 0>
 __device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1],
                                             cplx<T> (&v2)[R1 / cgcd(R1, R2)][R2], int t, cplx<T>* lds, int ls,
                                             const cplx<T>* __restrict__ tw) {
-  constexpr int P = cgcd(R1, R2), Q1 = R2 / P, Q2 = R1 / P;
+  constexpr int P = cgcd(R1, R2), Q1 = R2 / P, Q2 = R1 / P, S = R2 + PAD;
 #pragma unroll
   for (int q = 0; q < Q1; ++q) {
     const int n2 = t + P * q;
@@ -122,7 +123,7 @@ __device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1]
     for (int k1 = 0; k1 < R1; ++k1) {
       cplx<T> x = v1[q][k1];
       if (k1) x = cmul(x, twiddle<T, FWD>(tw, n2 * k1));
-      lds[(k1 * R2 + n2) * ls] = x;
+      lds[(k1 * S + n2) * ls] = x;
     }
   }
   __syncthreads();
@@ -130,7 +131,7 @@ __device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1]
   for (int q = 0; q < Q2; ++q) {
     const int k1 = t + P * q;
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) v2[q][n2] = lds[(k1 * R2 + n2) * ls];
+    for (int n2 = 0; n2 < R2; ++n2) v2[q][n2] = lds[(k1 * S + n2) * ls];
     dft_small<T, R2, FWD>(v2[q]);
   }
 }

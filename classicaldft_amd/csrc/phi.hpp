@@ -19,17 +19,25 @@ enum { M_ROSENFELD = 0, M_RSLT = 1, M_ESFMT = 2, M_WBI = 3, M_WBII = 4 };
 struct PhiParams {
   double hsd;
   double A, B;  // esFMT coefficients (WB I/II: 1, -1)
+  double inv_d, inv_d2;  // 1/hsd, 1/hsd^2
 };
+
+inline PhiParams make_phi_params(double hsd, double A, double B) {
+  return PhiParams{hsd, A, B, 1.0 / hsd, 1.0 / (hsd * hsd)};
+}
 
 constexpr double kPi = 3.14159265358979323846;
 
-// eta-dependent cofactors f1, f2, f3 and first derivatives.
+// eta-dependent cofactors f1, f2, f3 and first derivatives.  Written with the
+// two reciprocals 1/(1-x) and 1/x instead of one division per term (a double
+// division is ~10 instructions on the GPU); same formulas as the reference.
 template <int MODEL>
 __host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double& f2, double& f2p, double& f3,
                                       double& f3p) {
   const double om = 1.0 - x;
+  const double f = 1.0 / om;
   f1 = log(om);  // FMT.h:153-156
-  f1p = -1.0 / om;
+  f1p = -f;
   if constexpr (MODEL == M_WBII) {  // FMT.h:588-632
     if (x < 1e-12) {
       f2 = 1 + x * (1 + x * ((10.0 / 9) + (7.0 / 6) * x));
@@ -37,23 +45,23 @@ __host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double&
       f3 = 1.5 + (7.0 / 3) * x + 3.25 * x * x + 4.2 * x * x * x;
       f3p = (7.0 / 3) + 7.5 * x + 12.6 * x * x;
     } else {
-      const double L = f1;
-      f2 = (1.0 / 3.0) + (4.0 / 3.0) * (1.0 / om) + (2.0 / (3.0 * x)) * L;
-      f2p = (2 * (3 * x - 1) / (3 * x * om * om)) - (2.0 / (3.0 * x * x)) * L;
-      f3 = -((1.0 - 3 * x + x * x) / (x * om * om)) - (1.0 / (x * x)) * L;
-      f3p = ((2 - 5 * x + 6 * x * x - x * x * x) / (x * x * om * om * om)) + (2.0 / (x * x * x)) * L;
+      const double L = f1, ix = 1.0 / x, ix2 = ix * ix, f2_ = f * f;
+      f2 = (1.0 / 3.0) + (4.0 / 3.0) * f + (2.0 / 3.0) * ix * L;
+      f2p = (2.0 / 3.0) * (3 * x - 1) * ix * f2_ - (2.0 / 3.0) * ix2 * L;
+      f3 = -(1.0 - 3 * x + x * x) * ix * f2_ - ix2 * L;
+      f3p = (2 - 5 * x + 6 * x * x - x * x * x) * ix2 * f2_ * f + 2.0 * ix2 * ix * L;
     }
   } else {
-    const double f = 1.0 / om;  // FMT.h:168-183
-    f2 = f;
+    f2 = f;  // FMT.h:168-183
     f2p = f * f;
     if constexpr (MODEL == M_RSLT || MODEL == M_WBI) {  // FMT.h:267-281, 536-550
       if (x < 1e-12) {
         f3 = 1.5 + (8.0 / 3) * x + 3.75 * x * x + 4.8 * x * x * x;
         f3p = (8.0 / 3) + 7.5 * x + 14.4 * x * x;
       } else {
-        f3 = (1.0 / (x * om * om)) + (f1 / (x * x));
-        f3p = ((-2 + 5 * x - x * x) / (x * x * om * om * om)) - (2 / (x * x * x)) * f1;
+        const double ix = 1.0 / x, ix2 = ix * ix, f2_ = f * f;
+        f3 = ix * f2_ + f1 * ix2;
+        f3p = (-2 + 5 * x - x * x) * ix2 * f2_ * f - 2 * ix2 * ix * f1;
       }
     } else {  // Rosenfeld, esFMT: FMT.h:185-195
       f3 = f * f;
@@ -68,11 +76,11 @@ __host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double&
 template <int MODEL>
 __host__ __device__ inline double phi_site(const double* n, const PhiParams& pp, double* out) {
   constexpr bool TENSOR = MODEL >= M_ESFMT;
-  const double d = pp.hsd;
+  const double id = pp.inv_d, id2 = pp.inv_d2;
   const double eta = n[0];
-  const double s0 = n[1] / (d * d), s1 = n[1] / d, s2 = n[1];
+  const double s0 = n[1] * id2, s1 = n[1] * id, s2 = n[1];
   const double v2[3] = {n[2], n[3], n[4]};
-  const double v1[3] = {n[2] / d, n[3] / d, n[4] / d};
+  const double v1[3] = {n[2] * id, n[3] * id, n[4] * id};
   const double v1v2 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
   const double v2v2 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
 
@@ -136,12 +144,12 @@ __host__ __device__ inline double phi_site(const double* n, const PhiParams& pp,
   const double dS1 = c2 * s2 * f2;
   const double dS2t = c2 * s1 * f2 + dS2 * f3;
   out[0] = dEta;
-  out[1] = dS0 / (d * d) + dS1 / d + dS2t;
+  out[1] = dS0 * id2 + dS1 * id + dS2t;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const double dv1 = -c2 * v2[k] * f2;
     const double dv2 = -c2 * v1[k] * f2 + dV2[k] * f3;
-    out[2 + k] = -(dv2 + dv1 / d);  // parity sign, Species.h:324
+    out[2 + k] = -(dv2 + dv1 * id);  // parity sign, Species.h:324
   }
   if constexpr (TENSOR) {
     out[5] = dT[0] * f3;

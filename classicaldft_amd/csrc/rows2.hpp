@@ -1,0 +1,318 @@
+// z-pass kernels (real rows <-> half spectra along z) built on the register
+// FFT of fft2.hpp.  Same data layout and results as k_rows_r2c, k_rows_c2r,
+// k_z_pointwise and k_z_final in kernels.hpp.
+//
+// A real row of length Nz = 2M is handled as one complex sequence of length M
+// (z_m = x_2m + i x_2m+1) by P = gcd(R1, R2) threads (lane layout: thread
+// fastest, then sequence).  The c2r pre-processing reads X_j and X_{M-j}
+// straight from HBM into the pass-1 registers; the r2c post-processing pairs
+// Z_k with Z_{M-k} through the sequence's LDS slot.  Each sequence owns a slot
+// of R1 (R2 + 1) elements: the padded exchange of the transform, or M values in
+// natural order.
+#pragma once
+#include "kernels2.hpp"
+#include "phi.hpp"
+
+namespace cdft {
+
+template <typename T, int M>
+struct RowF {
+  using F = F2<M>;
+  static constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
+  static constexpr int SLOT = R1 * (R2 + 1);
+};
+
+// Deterministic block reduction of NQ compensated sums for blockDim.x <= kThreads.
+template <int NQ>
+__device__ void block_reduce_store_n(CSum* v, double2* __restrict__ partials, long blk) {
+  __shared__ double rs[NQ][kThreads], rc[NQ][kThreads];
+  const int t = threadIdx.x, nt = blockDim.x;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    rs[q][t] = v[q].s;
+    rc[q][t] = v[q].c;
+  }
+  for (int i = nt + t; i < kThreads; i += nt)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) rs[q][i] = rc[q][i] = 0.0;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    for (int i = t; i < off; i += nt) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        CSum a{rs[q][i], rc[q][i]};
+        a.merge(rs[q][i + off], rc[q][i + off]);
+        rs[q][i] = a.s;
+        rc[q][i] = a.c;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) partials[blk * NQ + q] = make_double2(rs[q][0], rc[q][0]);
+  }
+}
+
+// Pass-1 registers of the inverse length-M transform of one half-spectrum row
+// X_0..X_M (src points at X_0):
+// z_j = (X_j + X*_{M-j}) + i W_N^{-j} (X_j - X*_{M-j}), j = R2 n1 + t + P q;
+// the imaginary parts of X_0 and X_M are dropped (Hermitian input).  The lane
+// part t is folded into two base pointers so that every load carries a
+// constant immediate offset.
+template <typename T, int M>
+__device__ __forceinline__ void c2r_to_regs(const cplx<T>* __restrict__ src, const cplx<T>* __restrict__ twN,
+                                            int t, cplx<T> (&v)[RowF<T, M>::Q1][RowF<T, M>::R1]) {
+  using RF = RowF<T, M>;
+  const cplx<T>* fw = src + t;        // X_{c + t}
+  const cplx<T>* bw = src + (M - t);  // X_{M - t - c}
+  const cplx<T>* tw = twN + t;
+#pragma unroll
+  for (int q = 0; q < RF::Q1; ++q)
+#pragma unroll
+    for (int n1 = 0; n1 < RF::R1; ++n1) {
+      const int c = RF::R2 * n1 + RF::P * q;
+      cplx<T> xk = fw[c];
+      cplx<T> xm = bw[-c];
+      if (n1 == 0 && q == 0 && t == 0) {
+        xk.y = T(0);
+        xm.y = T(0);
+      }
+      v[q][n1] = c2r_pre(xk, xm, cconj(tw[c]));
+    }
+}
+
+// Half spectrum X_0..X_M of a row (written at dst) from the registers
+// Z[k1 + R1 k2] (k1 = t + P q) of its forward length-M transform:
+// X_k = (Z_k + Z*_{M-k})/2 - i W_N^k (Z_k - Z*_{M-k})/2,
+// X_0 = Re Z_0 + Im Z_0, X_M = Re Z_0 - Im Z_0.  The caller has synchronised so
+// that slot is free; every thread of the block must call this.
+template <typename T, int M>
+__device__ __forceinline__ void r2c_from_regs(cplx<T>* slot, int t, const cplx<T>* __restrict__ twN,
+                                              cplx<T>* __restrict__ dst, bool store,
+                                              cplx<T> (&v)[RowF<T, M>::Q2][RowF<T, M>::R2]) {
+  using RF = RowF<T, M>;
+#pragma unroll
+  for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+    for (int k2 = 0; k2 < RF::R2; ++k2) slot[t + RF::P * q + RF::R1 * k2] = v[q][k2];
+  __syncthreads();
+  if (!store) return;
+  cplx<T>* d = dst + t;
+  const cplx<T>* tw = twN + t;
+#pragma unroll
+  for (int i = 0; i < (M + RF::P) / RF::P; ++i) {
+    const int k = t + RF::P * i;
+    if (k > M) break;
+    cplx<T> X;
+    if (k == 0 || k == M) {
+      const cplx<T> z0 = slot[0];
+      X = {k == 0 ? z0.x + z0.y : z0.x - z0.y, T(0)};
+    } else {
+      r2c_post(slot[k], slot[M - k], tw[RF::P * i], X);
+    }
+    d[RF::P * i] = X;
+  }
+}
+
+// Real rows -> half spectra.  grid = (ceil(nrows/RB), narr), block = RB * P,
+// dynamic LDS = RB * SLOT elements.
+template <typename T, typename TI, int M>
+__global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, long in_arr_stride, unsigned in_bytes,
+                                                   cplx<T>* __restrict__ out, long out_arr_stride, unsigned out_bytes,
+                                                   long nrows, int Zp, const cplx<T>* __restrict__ twM,
+                                                   const cplx<T>* __restrict__ twN, int RB) {
+  using RF = RowF<T, M>;
+  extern __shared__ unsigned char smem_raw[];
+  const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
+  cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
+  const long row0 = (long)blockIdx.x * RB + seq;
+  const bool ok = row0 < nrows;
+  const long row = ok ? row0 : nrows - 1;
+  const cplx<TI>* src = reinterpret_cast<const cplx<TI>*>(in + blockIdx.y * in_arr_stride + row * 2 * M) + t;
+  cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+#pragma unroll
+  for (int q = 0; q < RF::Q1; ++q)
+#pragma unroll
+    for (int n1 = 0; n1 < RF::R1; ++n1) {
+      const cplx<TI> x = src[RF::R2 * n1 + RF::P * q];
+      v1[q][n1] = {T(x.x), T(x.y)};
+    }
+  reg_fft_seq<T, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+  __syncthreads();
+  r2c_from_regs<T, M>(slot, t, twN, out + blockIdx.y * out_arr_stride + row * Zp, ok, v2);
+}
+
+// Half spectra -> real rows (unnormalised, times `scale`).
+template <typename T, typename TO, int M>
+__global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ in, long in_arr_stride,
+                                                   unsigned in_bytes, TO* __restrict__ out, long out_arr_stride,
+                                                   unsigned out_bytes, long nrows, int Zp,
+                                                   const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
+                                                   int RB, double scale) {
+  using RF = RowF<T, M>;
+  extern __shared__ unsigned char smem_raw[];
+  const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
+  cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
+  const long row0 = (long)blockIdx.x * RB + seq;
+  const bool ok = row0 < nrows;
+  const long row = ok ? row0 : nrows - 1;
+  cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+  c2r_to_regs<T, M>(in + blockIdx.y * in_arr_stride + row * Zp, twN, t, v1);
+  reg_fft_seq<T, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+  if (!ok) return;
+  cplx<TO>* dst = reinterpret_cast<cplx<TO>*>(out + blockIdx.y * out_arr_stride + row * 2 * M) + t;
+#pragma unroll
+  for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+    for (int k2 = 0; k2 < RF::R2; ++k2)
+      dst[RF::P * q + RF::R1 * k2] = {TO(double(v2[q][k2].x) * scale), TO(double(v2[q][k2].y) * scale)};
+}
+
+// Pointwise z pass: inverse z of the nconv arrays of RB rows, Phi / dPhi at
+// every site, forward z of the m dPhi rows back into B, v_mf = the (m+1)-th
+// array in real space, F partial sums.  Sequence s = a * RB + r.
+// grid = ceil(nrows/RB), block = nconv * RB * P, dynamic LDS = nconv * RB * SLOT elements.
+template <typename T, int MODEL, int M>
+__global__ __launch_bounds__(256) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, unsigned arr_bytes,
+                                                      int m, int nconv, double* __restrict__ vmf, long nrows, int Zp,
+                                                      const cplx<T>* __restrict__ twM,
+                                                      const cplx<T>* __restrict__ twN, int RB, PhiParams pp,
+                                                      double2* __restrict__ partials, int* __restrict__ flag) {
+  using RF = RowF<T, M>;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
+  const int a = seq / RB, r = seq % RB;
+  cplx<T>* slot = lds + seq * RF::SLOT;
+  const long row0 = (long)blockIdx.x * RB;
+  const long rrow = row0 + r < nrows ? row0 + r : nrows - 1;
+  const bool ok = row0 + r < nrows;
+  cplx<T>* brow = B + a * arr_stride + rrow * Zp;
+  // 1. inverse z of every array into the slots (natural order: complex m = sites 2m, 2m+1)
+  {
+    cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+    c2r_to_regs<T, M>(brow, twN, t, v1);
+    reg_fft_seq<T, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+      for (int k2 = 0; k2 < RF::R2; ++k2) slot[t + RF::P * q + RF::R1 * k2] = v2[q][k2];
+  }
+  __syncthreads();
+  // 2. Phi and dPhi per site
+  CSum F{0.0, 0.0};
+  int bad = 0;
+  const bool has_mf = nconv > m;
+  for (int s = threadIdx.x; s < RB * 2 * M; s += blockDim.x) {
+    const int rr = s / (2 * M), z = s % (2 * M);
+    const long row = row0 + rr;
+    if (row >= nrows) continue;
+    double n[11];
+#pragma unroll
+    for (int b = 0; b < 11; ++b)
+      n[b] = b < m ? double(reinterpret_cast<const T*>(lds + (b * RB + rr) * RF::SLOT)[z]) : 0.0;
+    double out[11];
+    const double phi = phi_site<MODEL>(n, pp, out);
+    if (n[0] > 0.5 && 1 - n[0] < 0.0) bad = 1;
+    F.add(phi);
+#pragma unroll
+    for (int b = 0; b < 11; ++b)
+      if (b < m) reinterpret_cast<T*>(lds + (b * RB + rr) * RF::SLOT)[z] = T(out[b]);
+    if (has_mf) vmf[row * 2 * M + z] = double(reinterpret_cast<const T*>(lds + (m * RB + rr) * RF::SLOT)[z]);
+  }
+  if (bad) atomicOr(flag, 1);
+  __syncthreads();
+  // 3. forward z of the dPhi rows (the mean-field sequences run along, storing nothing)
+  {
+    cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+#pragma unroll
+    for (int q = 0; q < RF::Q1; ++q)
+#pragma unroll
+      for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = slot[RF::R2 * n1 + RF::P * q + t];
+    __syncthreads();
+    reg_fft_seq<T, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+    __syncthreads();
+    r2c_from_regs<T, M>(slot, t, twN, brow, ok && a < m, v2);
+  }
+  CSum v[1] = {F};
+  block_reduce_store_n<1>(v, partials, blockIdx.x);
+}
+
+// Final z pass: dF = [dF_in] + dV ln rho + dV c2r(acc) + v_mf + dV V_ext - dV mu
+// (src/DFT.cpp:377-425), sites 2m, 2m+1 of each row straight from the
+// registers of the inverse transform.  partials per block:
+// {sum (rho ln rho - rho) dV, sum rho v_mf, sum V_ext rho, sum rho}.
+template <typename T, int M>
+__global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Acc, unsigned acc_bytes,
+                                                  const double* __restrict__ rho, const double* __restrict__ vext,
+                                                  const double* __restrict__ vmf, double* __restrict__ dF,
+                                                  unsigned real_bytes, long nrows, int Zp,
+                                                  const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
+                                                  int RB, FinalParams fp, double2* __restrict__ partials) {
+  using RF = RowF<T, M>;
+  extern __shared__ unsigned char smem_raw[];
+  const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
+  cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
+  const long row0 = (long)blockIdx.x * RB + seq;
+  const bool ok = row0 < nrows;
+  const long row = ok ? row0 : nrows - 1;
+  cplx<T> y[RF::Q2][RF::R2];
+  if (fp.fmt) {
+    cplx<T> v1[RF::Q1][RF::R1];
+    c2r_to_regs<T, M>(Acc + row * Zp, twN, t, v1);
+    reg_fft_seq<T, M, RF::R1, RF::R2, false, 1>(v1, y, t, slot, 1, twM);
+  } else {
+#pragma unroll
+    for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+      for (int k2 = 0; k2 < RF::R2; ++k2) y[q][k2] = {T(0), T(0)};
+  }
+  CSum qs[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  if (ok) {
+    const auto rr = buf_rsrc(rho, real_bytes);
+    const auto rf = buf_rsrc(dF, real_bytes);
+    const auto rm = buf_rsrc(vmf, real_bytes);
+    const auto re = buf_rsrc(vext, real_bytes);
+    const unsigned row_byte = unsigned(row * 2 * M) * 8u;
+#pragma unroll
+    for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+      for (int k2 = 0; k2 < RF::R2; ++k2) {
+        const unsigned off = row_byte + unsigned(t + RF::P * q + RF::R1 * k2) * 16u;
+        const cplx<double> r2 = buf_ld<cplx<double>>(rr, off, 0);
+        const cplx<double> f2 = fp.accumulate ? buf_ld<cplx<double>>(rf, off, 0) : cplx<double>{0.0, 0.0};
+        const cplx<double> m2 = fp.mf ? buf_ld<cplx<double>>(rm, off, 0) : cplx<double>{0.0, 0.0};
+        const cplx<double> e2 = fp.ext ? buf_ld<cplx<double>>(re, off, 0) : cplx<double>{0.0, 0.0};
+        double out[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const double d0 = h ? r2.y : r2.x;
+          const double vm = h ? m2.y : m2.x, ve = h ? e2.y : e2.x;
+          double d = h ? f2.y : f2.x;
+          if (fp.ideal) {
+            const double l = log(d0);
+            qs[0].add((d0 * l - d0) * fp.dV);
+            d += l * fp.dV;
+          }
+          if (fp.fmt) d += double(h ? y[q][k2].y : y[q][k2].x) * fp.dV;
+          if (fp.mf) {
+            d += vm;
+            qs[1].add(d0 * vm);
+          }
+          if (fp.ext) {
+            qs[2].add(ve * d0);
+            d += ve * fp.dV;
+          }
+          qs[3].add(d0);
+          d += -fp.mu * fp.dV;
+          out[h] = d;
+        }
+        buf_st(rf, off, 0, cplx<double>{out[0], out[1]});
+      }
+  }
+  block_reduce_store_n<4>(qs, partials, blockIdx.x);
+}
+
+}  // namespace cdft

@@ -17,6 +17,7 @@
 #include "../../include/cdft.h"
 #include "kernels.hpp"
 #include "kernels2.hpp"
+#include "rows2.hpp"
 #include "weights.hpp"
 
 using namespace cdft;
@@ -133,6 +134,7 @@ struct cdft_system_s {
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
   bool reg_fft = true;  // register two-pass FFT for the x / y passes where the length allows
   bool nb_half = false;  // half the columns per block in those kernels
+  int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
   bool has_vext = false;
   bool last_with_dft = true;
   // timing
@@ -165,31 +167,6 @@ inline const Tables<double>& tables<double>(const cdft_system_s& s) { return s.t
 template <>
 inline const Tables<float>& tables<float>(const cdft_system_s& s) { return s.twf; }
 
-// ------------------------------- transforms --------------------------------
-template <typename T, typename TI>
-inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride, cplx<T>* out, long out_stride, int narr) {
-  const Tables<T>& tw = tables<T>(s);
-  const int M = s.g.Nz / 2;
-  const int RB = s.RBrow;
-  dim3 grid(div_up(s.nrows, RB), narr);
-  hipLaunchKernelGGL((k_rows_r2c<T, TI, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
-                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
-  HIPCHK(hipGetLastError());
-}
-
-template <typename T, typename TO>
-inline void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_stride, TO* out, long out_stride, int narr,
-                     double scale) {
-  const Tables<T>& tw = tables<T>(s);
-  const int M = s.g.Nz / 2;
-  const int RB = s.RBrow;
-  dim3 grid(div_up(s.nrows, RB), narr);
-  hipLaunchKernelGGL((k_rows_c2r<T, TO, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
-                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB,
-                     scale);
-  HIPCHK(hipGetLastError());
-}
-
 // Calls f(std::integral_constant<int, L>{}) when the register FFT handles
 // length L; returns false otherwise.
 template <typename Fn>
@@ -204,6 +181,74 @@ inline bool with_reg_len(int L, Fn&& f) {
 #undef CDFT_REG_LEN
     default: return false;
   }
+}
+
+// Rows per block of the register z kernels: RB * P <= 256 threads and
+// RB * SLOT elements of LDS within cap bytes (0 if one row does not fit).
+template <typename T, int M>
+constexpr int row_rb(int cap) {
+  constexpr int per = RowF<T, M>::SLOT * int(sizeof(cplx<T>)), P = RowF<T, M>::P;
+  int rb = kThreads / P;
+  const int wave = 64 / P;  // rows per full wavefront
+  while (rb > wave && (rb * per > cap || rb % wave)) --rb;
+  while (rb > 0 && rb * per > cap) --rb;
+  return rb;
+}
+
+// The register z kernels address arrays through buffer resources (< 2^31 bytes).
+template <typename T>
+inline bool rows_reg_ok(const cdft_system_s& s) {
+  return s.reg_fft && s.Nkp * (long)sizeof(cplx<T>) < (1L << 31) && s.N * 8L < (1L << 31) &&
+         with_reg_len(s.g.Nz / 2, [](auto) {});
+}
+
+// ------------------------------- transforms --------------------------------
+template <typename T, typename TI>
+inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride, cplx<T>* out, long out_stride, int narr) {
+  const Tables<T>& tw = tables<T>(s);
+  if (rows_reg_ok<T>(s)) {
+    with_reg_len(s.g.Nz / 2, [&](auto mc) {
+      constexpr int Mc = decltype(mc)::value;
+      constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
+      hipLaunchKernelGGL((k_rows_r2c2<T, TI, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
+                         RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride,
+                         unsigned(s.N * sizeof(TI)), out, out_stride, unsigned(s.Nkp * sizeof(cplx<T>)), s.nrows,
+                         s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
+    });
+    HIPCHK(hipGetLastError());
+    return;
+  }
+  const int M = s.g.Nz / 2;
+  const int RB = s.RBrow;
+  dim3 grid(div_up(s.nrows, RB), narr);
+  hipLaunchKernelGGL((k_rows_r2c<T, TI, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
+                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T, typename TO>
+inline void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_stride, TO* out, long out_stride, int narr,
+                     double scale) {
+  const Tables<T>& tw = tables<T>(s);
+  if (rows_reg_ok<T>(s)) {
+    with_reg_len(s.g.Nz / 2, [&](auto mc) {
+      constexpr int Mc = decltype(mc)::value;
+      constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
+      hipLaunchKernelGGL((k_rows_c2r2<T, TO, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
+                         RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride,
+                         unsigned(s.Nkp * sizeof(cplx<T>)), out, out_stride, unsigned(s.N * sizeof(TO)), s.nrows,
+                         s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, scale);
+    });
+    HIPCHK(hipGetLastError());
+    return;
+  }
+  const int M = s.g.Nz / 2;
+  const int RB = s.RBrow;
+  dim3 grid(div_up(s.nrows, RB), narr);
+  hipLaunchKernelGGL((k_rows_c2r<T, TO, 4>), grid, dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, in, in_stride,
+                     out, out_stride, s.g, s.radM, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB,
+                     scale);
+  HIPCHK(hipGetLastError());
 }
 
 // Columns per block of the register FFT kernels: NB * P <= 256 threads and an

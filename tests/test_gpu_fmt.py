@@ -32,7 +32,8 @@ def maxrel(a, b):
 
 
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 18), (32, 32, 32), (48, 48, 48), (24, 16, 96), (64, 64, 64),
-                                   (256, 6, 8), (4, 256, 12), (128, 96, 4), (96, 128, 6), (12, 24, 12), (6, 4, 16)])
+                                   (256, 6, 8), (4, 256, 12), (128, 96, 4), (96, 128, 6), (12, 24, 12), (6, 4, 16),
+                                   (4, 6, 512), (6, 4, 256), (4, 4, 192), (4, 8, 128), (3, 6, 36)])
 def test_fft_matches_numpy(cdft, shape):
     rng = np.random.default_rng(1)
     x = rng.standard_normal(shape)
@@ -70,9 +71,10 @@ def test_all_models_fp64_parity(cdft, model, A, B):
     assert maxrel(dF1, dF0) <= TOL
 
 
-@pytest.mark.parametrize("shape", [(128, 24, 16), (12, 256, 24), (96, 48, 12)])
+@pytest.mark.parametrize("shape", [(128, 24, 16), (12, 256, 24), (96, 48, 12), (16, 12, 256), (12, 16, 192),
+                                   (12, 12, 512)])
 def test_long_axes_fp64_parity(cdft, shape):
-    """x / y transforms of lengths 256, 128, 96, 48, 24, 12 inside the full F + dF pipeline."""
+    """Transforms of lengths 512 (z), 256, 192 (z), 128, 96, 48, 24, 12 inside the full F + dF pipeline."""
     dx, hsd = 0.1, 1.0
     rho = seeded_density(shape, 0.55, 0x5EED0300 + shape[0], amp=0.1)
     w = oracle.fmt_weights(shape, dx, hsd)
