@@ -136,7 +136,20 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
   const int esz = s->mixed ? 8 : 16;
   s->TWconv = std::max(1, std::min(16, (65536 / (2 * esz)) / Lmax));
   if (const char* e = std::getenv("CDFT_LDS_FFT")) s->reg_fft = !(e[0] == '1');
-  if (const char* e = std::getenv("CDFT_NB_HALF")) s->nb_half = e[0] == '1';
+  // measured on MI355X at 256^3: FP64 x/y passes and backward convolution run
+  // faster with 8 columns per block than with 16 (occupancy vs coalescing)
+  s->nb_half_cols = s->nb_half_bwd = !s->mixed;
+  if (const char* e = std::getenv("CDFT_PF")) {  // two digits: forward, backward convolution prefetch
+    const size_t n = std::strlen(e);
+    if (n > 0) s->pf_fwd = e[0] == '1';
+    if (n > 1) s->pf_bwd = e[1] == '1';
+  }
+  if (const char* e = std::getenv("CDFT_NB_HALF")) {  // three digits: x/y passes, forward, backward
+    const size_t n = std::strlen(e);
+    if (n > 0) s->nb_half_cols = e[0] == '1';
+    if (n > 1) s->nb_half_fwd = e[1] == '1';
+    if (n > 2) s->nb_half_bwd = e[2] == '1';
+  }
   if (const char* e = std::getenv("CDFT_PW_ROWS")) s->pw_rows = std::atoi(e);
   s->twd.build(g);
   if (s->mixed) s->twf.build(g);
@@ -226,10 +239,14 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
       const ColGeo cg = col_geo<T>(s.g, 0);
       with_reg_len(s.g.Nx, [&](auto lc) {
         constexpr int Lc = decltype(lc)::value;
-        with_nb<T, Lc>(s.nb_half, [&](auto nbc) {
+        with_nb<T, Lc>(s.nb_half_fwd, [&](auto nbc) {
           constexpr int NB = decltype(nbc)::value;
-          hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0, s.stream,
-                             A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+          if (s.pf_fwd)
+            hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB, true>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
+                               s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+          else
+            hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB, false>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
+                               s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
           HIPCHK(hipGetLastError());
         });
       });
@@ -261,10 +278,14 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         const ColGeo cg = col_geo<T>(s.g, 0);
         with_reg_len(s.g.Nx, [&](auto lc) {
           constexpr int Lc = decltype(lc)::value;
-          with_nb<T, Lc>(s.nb_half, [&](auto nbc) {
+          with_nb<T, Lc>(s.nb_half_bwd, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
-            hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
-                               s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+            if (s.pf_bwd)
+              hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB, true>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
+                                 s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+            else
+              hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB, false>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
+                                 s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
             HIPCHK(hipGetLastError());
           });
         });

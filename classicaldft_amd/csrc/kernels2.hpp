@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long 
 
 // x pass of rho_k, then per weight a: n_a(k) = rho_k w_a(k) and its inverse x
 // pass, straight from registers.  grid = ceil(ncols/NB), block = NB * P.
-template <typename T, int L, int NB>
+// PF: the weights of array a+1 are loaded before the transform of array a.
+template <typename T, int L, int NB, bool PF>
 __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A, cplx<T>* __restrict__ B,
                                                    long arr_stride, const T* __restrict__ W, int nconv,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
@@ -110,21 +111,36 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
     for (int n1 = 0; n1 < R1; ++n1)
       v1[q][n1] = buf_ld<cplx<T>>(ra, vo, R2 * n1 * cg.es * E);
   }
-  reg_fft_seq<T, L, R1, R2, true>(v1, rk, t, buf + b, NB, tw);
-  for (int a = 0; a < nconv; ++a) {
+  auto load_w = [&](int a, T (&w)[Q2][R2]) {
     const auto rw = buf_rsrc(W + a * arr_stride, cg.bytes / 2);
-    const auto rb = buf_rsrc(B + a * arr_stride, cg.bytes);
-    const bool odd = (odd_mask >> a) & 1u;
-    cplx<T> u1[Q2][R2], u2[Q1][R1];
 #pragma unroll
     for (int q = 0; q < Q2; ++q) {
       const unsigned vo = (c + (t + P * q) * cg.es) * EW;
 #pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) {
-        const T w = buf_ld<T>(rw, vo, R1 * k2 * cg.es * EW);
-        u1[q][k2] = wmul(rk[q][k2], w, odd);
-      }
+      for (int k2 = 0; k2 < R2; ++k2) w[q][k2] = buf_ld<T>(rw, vo, R1 * k2 * cg.es * EW);
     }
+  };
+  T nw[Q2][R2];
+  if (PF) load_w(0, nw);
+  reg_fft_seq<T, L, R1, R2, true>(v1, rk, t, buf + b, NB, tw);
+  for (int a = 0; a < nconv; ++a) {
+    const auto rb = buf_rsrc(B + a * arr_stride, cg.bytes);
+    const bool odd = (odd_mask >> a) & 1u;
+    cplx<T> u1[Q2][R2], u2[Q1][R1];
+    T w[Q2][R2];
+    if (PF) {
+#pragma unroll
+      for (int q = 0; q < Q2; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) w[q][k2] = nw[q][k2];
+      if (a + 1 < nconv) load_w(a + 1, nw);
+    } else {
+      load_w(a, w);
+    }
+#pragma unroll
+    for (int q = 0; q < Q2; ++q)
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) u1[q][k2] = wmul(rk[q][k2], w[q][k2], odd);
     // Re-read the twiddles in every iteration: kept live across the loop they
     // would cost 4 (L/P - 1) registers and halve the occupancy.
     int zero = 0;
@@ -144,8 +160,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
 }
 
 // x pass of each dPhi_a, acc = sum_a w_a(k) dPhi_a(k) in registers, inverse x
-// pass of acc.  grid = ceil(ncols/NB), block = NB * P.
-template <typename T, int L, int NB>
+// pass of acc.  grid = ceil(ncols/NB), block = NB * P.  PF: the loads of array
+// a+1 are issued before the transform of array a (more registers, more bytes
+// in flight).
+template <typename T, int L, int NB, bool PF>
 __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D, cplx<T>* __restrict__ Acc,
                                                    long arr_stride, const T* __restrict__ W, int m,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
@@ -157,38 +175,68 @@ __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D
   const bool ok = c0 < cg.ncols;
   const unsigned c = ok ? c0 : cg.ncols - 1;  // tail lanes recompute the last column, store nothing
   constexpr unsigned E = sizeof(cplx<T>), EW = sizeof(T);
+  auto load_d = [&](int a, cplx<T> (&v)[Q1][R1]) {
+    const auto rd = buf_rsrc(D + a * arr_stride, cg.bytes);
+#pragma unroll
+    for (int q = 0; q < Q1; ++q) {
+      const unsigned vo = (c + (t + P * q) * cg.es) * E;
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) v[q][n1] = buf_ld<cplx<T>>(rd, vo, R2 * n1 * cg.es * E);
+    }
+  };
+  auto load_w = [&](int a, T (&w)[Q2][R2]) {
+    const auto rw = buf_rsrc(W + a * arr_stride, cg.bytes / 2);
+#pragma unroll
+    for (int q = 0; q < Q2; ++q) {
+      const unsigned vo = (c + (t + P * q) * cg.es) * EW;
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) w[q][k2] = buf_ld<T>(rw, vo, R1 * k2 * cg.es * EW);
+    }
+  };
   cplx<T> acc[Q2][R2];
 #pragma unroll
   for (int q = 0; q < Q2; ++q)
 #pragma unroll
     for (int k2 = 0; k2 < R2; ++k2) acc[q][k2] = {T(0), T(0)};
+  cplx<T> nv[Q1][R1];
+  T nw[Q2][R2];
+  if (PF) {
+    load_d(0, nv);
+    load_w(0, nw);
+  }
   for (int a = 0; a < m; ++a) {
-    const auto rd = buf_rsrc(D + a * arr_stride, cg.bytes);
-    const auto rw = buf_rsrc(W + a * arr_stride, cg.bytes / 2);
     const bool odd = (odd_mask >> a) & 1u;
     cplx<T> v1[Q1][R1], v2[Q2][R2];
+    T w[Q2][R2];
+    if (PF) {
 #pragma unroll
-    for (int q = 0; q < Q1; ++q) {
-      const unsigned vo = (c + (t + P * q) * cg.es) * E;
+      for (int q = 0; q < Q1; ++q)
 #pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1)
-        v1[q][n1] = buf_ld<cplx<T>>(rd, vo, R2 * n1 * cg.es * E);
+        for (int n1 = 0; n1 < R1; ++n1) v1[q][n1] = nv[q][n1];
+#pragma unroll
+      for (int q = 0; q < Q2; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) w[q][k2] = nw[q][k2];
+      if (a + 1 < m) {
+        load_d(a + 1, nv);
+        load_w(a + 1, nw);
+      }
+    } else {
+      load_d(a, v1);
     }
     int zero = 0;  // re-read the twiddles per iteration (register pressure)
     asm volatile("" : "+s"(zero));
     const cplx<T>* twa = tw + zero;
     if (a) __syncthreads();  // the exchange slot is reused
     reg_fft_seq<T, L, R1, R2, true>(v1, v2, t, buf + b, NB, twa);
+    if (!PF) load_w(a, w);
 #pragma unroll
-    for (int q = 0; q < Q2; ++q) {
-      const unsigned vo = (c + (t + P * q) * cg.es) * EW;
+    for (int q = 0; q < Q2; ++q)
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) {
-        const T w = buf_ld<T>(rw, vo, R1 * k2 * cg.es * EW);
-        const cplx<T> p = wmul(v2[q][k2], w, odd);
+        const cplx<T> p = wmul(v2[q][k2], w[q][k2], odd);
         acc[q][k2] = {acc[q][k2].x + p.x, acc[q][k2].y + p.y};
       }
-    }
   }
   cplx<T> u2[Q1][R1];
   __syncthreads();

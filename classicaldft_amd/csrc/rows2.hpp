@@ -15,42 +15,57 @@
 
 namespace cdft {
 
+// Slot stride: the padded exchange R1 (R2 + 1), rounded up so that
+// consecutive slots start 64 B apart modulo the 256 B bank window (the
+// sequences of one wavefront then spread over the LDS banks).
+constexpr int slot_stride(int base, int esz) {
+  for (int p = 0; p < 32; ++p)
+    if ((base + p) * esz % 256 == 64 || (base + p) * esz % 256 == 192) return base + p;
+  return base;
+}
+
 template <typename T, int M>
 struct RowF {
   using F = F2<M>;
   static constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
-  static constexpr int SLOT = R1 * (R2 + 1);
+  static constexpr int SLOT = slot_stride(R1 * (R2 + 1), int(sizeof(cplx<T>)));
 };
 
-// Deterministic block reduction of NQ compensated sums for blockDim.x <= kThreads.
+// Deterministic block reduction of NQ compensated sums for blockDim.x <= kThreads:
+// a fixed xor butterfly inside each wavefront (lanes beyond the block's
+// thread count contribute zero), then thread 0 merges the wavefront results
+// in order and writes partials[blk*NQ+q].
 template <int NQ>
 __device__ void block_reduce_store_n(CSum* v, double2* __restrict__ partials, long blk) {
-  __shared__ double rs[NQ][kThreads], rc[NQ][kThreads];
-  const int t = threadIdx.x, nt = blockDim.x;
+  __shared__ double ws[NQ][kThreads / 64][2];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nact = min(64, (int)blockDim.x - wid * 64);
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    rs[q][t] = v[q].s;
-    rc[q][t] = v[q].c;
-  }
-  for (int i = nt + t; i < kThreads; i += nt)
+  for (int off = 32; off > 0; off >>= 1) {
+    const bool have = (lane ^ off) < nact;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) rs[q][i] = rc[q][i] = 0.0;
-  __syncthreads();
-  for (int off = kThreads / 2; off > 0; off >>= 1) {
-    for (int i = t; i < off; i += nt) {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        CSum a{rs[q][i], rc[q][i]};
-        a.merge(rs[q][i + off], rc[q][i + off]);
-        rs[q][i] = a.s;
-        rc[q][i] = a.c;
-      }
+    for (int q = 0; q < NQ; ++q) {
+      const double s2 = __shfl_xor(v[q].s, off);
+      const double c2 = __shfl_xor(v[q].c, off);
+      if (have) v[q].merge(s2, c2);
     }
-    __syncthreads();
   }
-  if (t == 0) {
+  if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) partials[blk * NQ + q] = make_double2(rs[q][0], rc[q][0]);
+    for (int q = 0; q < NQ; ++q) {
+      ws[q][wid][0] = v[q].s;
+      ws[q][wid][1] = v[q].c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = ((int)blockDim.x + 63) / 64;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      CSum a{ws[q][0][0], ws[q][0][1]};
+      for (int w = 1; w < nw; ++w) a.merge(ws[q][w][0], ws[q][w][1]);
+      partials[blk * NQ + q] = make_double2(a.s, a.c);
+    }
   }
 }
 

@@ -133,7 +133,10 @@ struct cdft_system_s {
   double a_vdw = 0.0;
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
   bool reg_fft = true;  // register two-pass FFT for the x / y passes where the length allows
-  bool nb_half = false;  // half the columns per block in those kernels
+  // half the columns per block in the register column kernels: x/y passes,
+  // forward convolution, backward convolution (tuned per precision at create)
+  bool nb_half_cols = false, nb_half_fwd = false, nb_half_bwd = false;
+  bool pf_fwd = false, pf_bwd = false;  // software prefetch of the next array in the convolutions
   int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
   bool has_vext = false;
   bool last_with_dft = true;
@@ -304,7 +307,7 @@ inline bool launch_cols_reg(const cdft_system_s& s, cplx<T>* data, long stride, 
   const ColGeo cg = col_geo<T>(s.g, axis);
   return with_reg_len(L, [&](auto lc) {
     constexpr int Lc = decltype(lc)::value;
-    with_nb<T, Lc>(s.nb_half, [&](auto nbc) {
+    with_nb<T, Lc>(s.nb_half_cols, [&](auto nbc) {
       constexpr int NB = decltype(nbc)::value;
       const dim3 grid(div_up(cg.ncols, NB), narr), block(NB * F2<Lc>::P);
       if (fwd)
