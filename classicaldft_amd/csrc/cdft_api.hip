@@ -90,7 +90,7 @@ void build_weight_spectra(cdft_system_s& s) {
     const int nblk = 256;
     hipLaunchKernelGGL(k_sum_real, dim3(nblk), dim3(kThreads), 0, s.stream, w.as<double>(), N,
                        s.partials.as<double2>());
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials.as<double2>(), (long)nblk, 1,
+    hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials.as<double2>(), (long)nblk, 1,
                        s.results.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&s.a_vdw, s.results.p, sizeof(double), hipMemcpyDeviceToHost, s.stream));
@@ -139,11 +139,6 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
   // measured on MI355X at 256^3: FP64 x/y passes and backward convolution run
   // faster with 8 columns per block than with 16 (occupancy vs coalescing)
   s->nb_half_cols = s->nb_half_bwd = !s->mixed;
-  if (const char* e = std::getenv("CDFT_PF")) {  // two digits: forward, backward convolution prefetch
-    const size_t n = std::strlen(e);
-    if (n > 0) s->pf_fwd = e[0] == '1';
-    if (n > 1) s->pf_bwd = e[1] == '1';
-  }
   if (const char* e = std::getenv("CDFT_NB_HALF")) {  // three digits: x/y passes, forward, backward
     const size_t n = std::strlen(e);
     if (n > 0) s->nb_half_cols = e[0] == '1';
@@ -241,12 +236,8 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         constexpr int Lc = decltype(lc)::value;
         with_nb<T, Lc>(s.nb_half_fwd, [&](auto nbc) {
           constexpr int NB = decltype(nbc)::value;
-          if (s.pf_fwd)
-            hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB, true>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
-                               s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
-          else
-            hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB, false>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
-                               s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+          hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
+                             s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
           HIPCHK(hipGetLastError());
         });
       });
@@ -269,7 +260,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         case CDFT_WHITEBEAR_I: nblk = launch_pointwise<T, M_WBI>(s, nconv); break;
         default: nblk = launch_pointwise<T, M_WBII>(s, nconv); break;
       }
-      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
                          s.results.as<double>());
       s.mark("z_pointwise");
       launch_cols<T>(s, B, Nkp, s.m, 1, true);
@@ -280,12 +271,8 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
           constexpr int Lc = decltype(lc)::value;
           with_nb<T, Lc>(s.nb_half_bwd, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
-            if (s.pf_bwd)
-              hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB, true>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
-                                 s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
-            else
-              hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB, false>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
-                                 s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+            hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
+                               s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
             HIPCHK(hipGetLastError());
           });
         });
@@ -334,7 +321,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
                          tw.n.template as<cplx<T>>(), RB, fp, s.partials2.as<double2>());
     }
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kThreads), 0, s.stream, s.partials2.as<double2>(), nblk, 4,
+    hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials2.as<double2>(), nblk, 4,
                        s.results.as<double>() + 1);
     HIPCHK(hipGetLastError());
   }

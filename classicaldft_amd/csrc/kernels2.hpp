@@ -89,8 +89,7 @@ __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long 
 
 // x pass of rho_k, then per weight a: n_a(k) = rho_k w_a(k) and its inverse x
 // pass, straight from registers.  grid = ceil(ncols/NB), block = NB * P.
-// PF: the weights of array a+1 are loaded before the transform of array a.
-template <typename T, int L, int NB, bool PF>
+template <typename T, int L, int NB>
 __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A, cplx<T>* __restrict__ B,
                                                    long arr_stride, const T* __restrict__ W, int nconv,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
@@ -120,23 +119,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
       for (int k2 = 0; k2 < R2; ++k2) w[q][k2] = buf_ld<T>(rw, vo, R1 * k2 * cg.es * EW);
     }
   };
-  T nw[Q2][R2];
-  if (PF) load_w(0, nw);
   reg_fft_seq<T, L, R1, R2, true>(v1, rk, t, buf + b, NB, tw);
   for (int a = 0; a < nconv; ++a) {
     const auto rb = buf_rsrc(B + a * arr_stride, cg.bytes);
     const bool odd = (odd_mask >> a) & 1u;
     cplx<T> u1[Q2][R2], u2[Q1][R1];
     T w[Q2][R2];
-    if (PF) {
-#pragma unroll
-      for (int q = 0; q < Q2; ++q)
-#pragma unroll
-        for (int k2 = 0; k2 < R2; ++k2) w[q][k2] = nw[q][k2];
-      if (a + 1 < nconv) load_w(a + 1, nw);
-    } else {
-      load_w(a, w);
-    }
+    load_w(a, w);
 #pragma unroll
     for (int q = 0; q < Q2; ++q)
 #pragma unroll
@@ -160,10 +149,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
 }
 
 // x pass of each dPhi_a, acc = sum_a w_a(k) dPhi_a(k) in registers, inverse x
-// pass of acc.  grid = ceil(ncols/NB), block = NB * P.  PF: the loads of array
-// a+1 are issued before the transform of array a (more registers, more bytes
-// in flight).
-template <typename T, int L, int NB, bool PF>
+// pass of acc.  grid = ceil(ncols/NB), block = NB * P.  The weights of array a
+// are loaded as one batch after its transform (all loads in flight at once).
+template <typename T, int L, int NB>
 __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D, cplx<T>* __restrict__ Acc,
                                                    long arr_stride, const T* __restrict__ W, int m,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
@@ -198,38 +186,17 @@ __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D
   for (int q = 0; q < Q2; ++q)
 #pragma unroll
     for (int k2 = 0; k2 < R2; ++k2) acc[q][k2] = {T(0), T(0)};
-  cplx<T> nv[Q1][R1];
-  T nw[Q2][R2];
-  if (PF) {
-    load_d(0, nv);
-    load_w(0, nw);
-  }
   for (int a = 0; a < m; ++a) {
     const bool odd = (odd_mask >> a) & 1u;
     cplx<T> v1[Q1][R1], v2[Q2][R2];
     T w[Q2][R2];
-    if (PF) {
-#pragma unroll
-      for (int q = 0; q < Q1; ++q)
-#pragma unroll
-        for (int n1 = 0; n1 < R1; ++n1) v1[q][n1] = nv[q][n1];
-#pragma unroll
-      for (int q = 0; q < Q2; ++q)
-#pragma unroll
-        for (int k2 = 0; k2 < R2; ++k2) w[q][k2] = nw[q][k2];
-      if (a + 1 < m) {
-        load_d(a + 1, nv);
-        load_w(a + 1, nw);
-      }
-    } else {
-      load_d(a, v1);
-    }
+    load_d(a, v1);
     int zero = 0;  // re-read the twiddles per iteration (register pressure)
     asm volatile("" : "+s"(zero));
     const cplx<T>* twa = tw + zero;
     if (a) __syncthreads();  // the exchange slot is reused
     reg_fft_seq<T, L, R1, R2, true>(v1, v2, t, buf + b, NB, twa);
-    if (!PF) load_w(a, w);
+    load_w(a, w);
 #pragma unroll
     for (int q = 0; q < Q2; ++q)
 #pragma unroll

@@ -69,6 +69,44 @@ __device__ void block_reduce_store_n(CSum* v, double2* __restrict__ partials, lo
   }
 }
 
+// Final deterministic reduction of nblk x nq partials -> out[q] = s + c, one
+// block of 1024 threads: fixed strided assignment, 8 loads in flight per
+// thread, xor butterflies in the wavefronts, wavefront results merged in order.
+static __global__ __launch_bounds__(1024) void k_reduce_partials_wide(const double2* __restrict__ partials, long nblk,
+                                                                      int nq, double* __restrict__ out) {
+  __shared__ double ws[16][2];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  for (int q = 0; q < nq; ++q) {
+    CSum a{0.0, 0.0};
+    for (long base = 0; base < nblk; base += 8 * 1024) {
+      double2 v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const long i = base + u * 1024 + t;
+        v[u] = i < nblk ? partials[i * nq + q] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a.merge(v[u].x, v[u].y);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double s2 = __shfl_xor(a.s, off), c2 = __shfl_xor(a.c, off);
+      a.merge(s2, c2);
+    }
+    if (lane == 0) {
+      ws[wid][0] = a.s;
+      ws[wid][1] = a.c;
+    }
+    __syncthreads();
+    if (t == 0) {
+      CSum r{ws[0][0], ws[0][1]};
+      for (int w = 1; w < 16; ++w) r.merge(ws[w][0], ws[w][1]);
+      out[q] = r.s + r.c;
+    }
+    __syncthreads();
+  }
+}
+
 // Pass-1 registers of the inverse length-M transform of one half-spectrum row
 // X_0..X_M (src points at X_0):
 // z_j = (X_j + X*_{M-j}) + i W_N^{-j} (X_j - X*_{M-j}), j = R2 n1 + t + P q;
@@ -189,7 +227,7 @@ __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ i
 // array in real space, F partial sums.  Sequence s = a * RB + r.
 // grid = ceil(nrows/RB), block = nconv * RB * P, dynamic LDS = nconv * RB * SLOT elements.
 template <typename T, int MODEL, int M>
-__global__ __launch_bounds__(256) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, unsigned arr_bytes,
+__global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, unsigned arr_bytes,
                                                       int m, int nconv, double* __restrict__ vmf, long nrows, int Zp,
                                                       const cplx<T>* __restrict__ twM,
                                                       const cplx<T>* __restrict__ twN, int RB, PhiParams pp,

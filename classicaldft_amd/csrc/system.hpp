@@ -136,7 +136,6 @@ struct cdft_system_s {
   // half the columns per block in the register column kernels: x/y passes,
   // forward convolution, backward convolution (tuned per precision at create)
   bool nb_half_cols = false, nb_half_fwd = false, nb_half_bwd = false;
-  bool pf_fwd = false, pf_bwd = false;  // software prefetch of the next array in the convolutions
   int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
   bool has_vext = false;
   bool last_with_dft = true;
