@@ -100,6 +100,76 @@ def time_system(cdft, precision, rho_d, dF_d, steps, warmup):
     return ms / steps, parts[2], dF, acc
 
 
+def slit_pore(cdft, n=128, dx=0.05, kT=0.8, rho_b=0.7, precision=0):
+    """Slit pore: White Bear I + LJ (sigma = eps = 1, rc = 2.5, WCA, QF kernel) at kT, hard walls
+    V_ext(z) = 10 [exp(-z/0.2) + exp(-(L-z)/0.2)], mu = mu_times_beta(rho_b), start rho_b exp(-V_ext)."""
+    shape = (n, n, n)
+    L = n * dx
+    hsd = cdft.lj_hsd(kT)
+    z = np.arange(n) * dx
+    vz = 10.0 * (np.exp(-z / 0.2) + np.exp(-(L - z) / 0.2))
+    vext = np.ascontiguousarray(np.broadcast_to(vz, shape))
+    probe = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=kT)
+    _, mu = probe.bulk(rho_b)
+    probe.close()
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mu=mu, vext=vext, mean_field=True, kT=kT,
+                    precision=precision)
+    return s, rho_b * np.exp(-vext), {"grid": list(shape), "dx": dx, "kT": kT, "hsd": hsd, "mu": mu,
+                                       "rho_bulk": rho_b}
+
+
+def fire_time_to_tol(cdft, n=128, tol=1e-8, max_steps=20000, precision=0):
+    """FIRE2 from rho_b exp(-V_ext) until max_i |dF/drho_i| / dV <= tol; wall time of the whole run."""
+    s, rho0, info = slit_pore(cdft, n=n, precision=precision)
+    fire = cdft.Fire(s, rho0)
+    t0 = time.perf_counter()
+    rc, rep = fire.run(max_steps, tol)
+    dt = time.perf_counter() - t0
+    info.update({"seconds": dt, "steps": rep["steps"], "evaluations": rep["calls"], "F": rep["F"],
+                 "fmax_raw": rep["fmax_raw"], "converged": bool(rep["fmax_raw"] <= tol), "status": rc,
+                 "ms_per_step": 1e3 * dt / max(1, rep["steps"])})
+    fire.close()
+    s.close()
+    return info
+
+
+def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha=50.0, dt=1e-4):
+    """DDFT from a liquid with a 4x4x4 fcc seed of Gaussians (alpha/pi)^{3/2} exp(-alpha |r - R|^2) centred in the
+    box, dt pinned at 1e-4; ms per step and the relative mass drift."""
+    shape = (n, n, n)
+    L = n * dx
+    hsd = cdft.lj_hsd(kT)
+    x = np.arange(n) * dx
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.array([[i, j, k] for i in range(4) for j in range(4) for k in range(4)], dtype=float)
+    sites = ((cells[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a) + (L - 4 * a) / 2
+    rho = np.full(shape, rho_l)
+    lo, hi = (L - 4 * a) / 2, (L + 4 * a) / 2
+    inside = ((x >= lo) & (x < hi))
+    sub = np.zeros((inside.sum(),) * 3)
+    xi = x[inside]
+    gx = np.exp(-alpha * (xi[None, :] - sites[:, 0:1]) ** 2)
+    gy = np.exp(-alpha * (xi[None, :] - sites[:, 1:2]) ** 2)
+    gz = np.exp(-alpha * (xi[None, :] - sites[:, 2:3]) ** 2)
+    sub = np.einsum("si,sj,sk->ijk", gx, gy, gz) * (alpha / np.pi) ** 1.5
+    idx = np.where(inside)[0]
+    rho[np.ix_(idx, idx, idx)] = np.maximum(sub, 1e-12)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=kT)
+    d = cdft.DDFT(s, rho, dt=dt, dt_max=dt)
+    d.set_max_time_step(dt)
+    m0 = rho.sum() * dx ** 3
+    t0 = time.perf_counter()
+    r = None
+    for _ in range(steps):
+        r = d.step()
+    sec = time.perf_counter() - t0
+    out = {"grid": list(shape), "dx": dx, "kT": kT, "steps": steps, "seconds": sec, "ms_per_step": 1e3 * sec / steps,
+           "F": r["F"], "evaluations": r["calls"], "mass_rel_drift": abs(r["mass"] - m0) / m0, "dt": r["dt"]}
+    d.close()
+    s.close()
+    return out
+
+
 def cpu_baseline(rho, threads):
     import oracle
     oracle.set_threads(threads, threads)
@@ -120,6 +190,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--fire-grid", type=int, default=128, help="slit-pore FIRE time-to-tolerance grid (0: skip)")
+    ap.add_argument("--fire-max-steps", type=int, default=20000)
+    ap.add_argument("--ddft-steps", type=int, default=0, help="DDFT fcc-seed steps at 256^3 (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -191,6 +264,14 @@ def main():
         "stages_ms_mixed": stmx,
         "F_fp64": F64,
     }
+    if args.fire_grid:
+        print("# FIRE time to tolerance", flush=True)
+        result["fire_time_to_tol"] = {"cold": fire_time_to_tol(cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
+                                      "start": "rho_bulk exp(-V_ext)", "tol": 1e-8,
+                                      "criterion": "max_i |dF/drho_i| / dV (raw gradient)", "warm": None}
+    if args.ddft_steps:
+        print("# DDFT fcc seed", flush=True)
+        result["ddft_fcc_seed"] = ddft_fcc_seed(cdft, args.ddft_steps)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"# cpu baseline ({threads} threads)", flush=True)

@@ -84,6 +84,7 @@ def lib():
         L.cdft_system_a_vdw.argtypes = [_p, ctypes.POINTER(_d)]
         L.cdft_system_get_weights.argtypes = [_p, _p]
         L.cdft_system_get_mf_kernel.argtypes = [_p, _p]
+        L.cdft_lj_hsd.argtypes = [_d, _d, _d, _d, ctypes.POINTER(_d)]
         L.cdft_fft_r2c.argtypes = [_i, _i, _i, _i, _p, _p]
         L.cdft_fft_c2r.argtypes = [_i, _i, _i, _i, _p, _p]
         L.cdft_system_eval_async.argtypes = [_p, _p, _p, _i]
@@ -379,6 +380,13 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def lj_hsd(kT, sigma=1.0, eps=1.0, rcut=2.5):
+    """Hard-sphere diameter of the WCA-split LJ potential (Potential1::getHSD, include/Potential1.h:73-83)."""
+    d = _d()
+    _check(lib().cdft_lj_hsd(float(sigma), float(eps), float(rcut), float(kT), ctypes.byref(d)))
+    return d.value
 
 
 def fft_r2c(x, precision=FP64):

@@ -457,6 +457,34 @@ int standalone_fft(int Nx, int Ny, int Nz, int precision, const double* in, doub
 
 using namespace cdft_impl;
 
+namespace {
+// Adaptive Gauss-Kronrod (7/15) quadrature, absolute tolerance tol.
+template <typename Fn>
+double gk15(const Fn& f, double a, double b, double tol, int depth = 0) {
+  static const double xk[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                               0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                               0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                               0.207784955007898467600689403773245, 0.0};
+  static const double wk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                               0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                               0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                               0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+  static const double wg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                               0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+  const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+  double k = wk[7] * f(c), g = wg[3] * f(c);
+  for (int i = 0; i < 7; ++i) {
+    const double fp = f(c + h * xk[i]), fm = f(c - h * xk[i]);
+    k += wk[i] * (fp + fm);
+    if (i % 2 == 1) g += wg[i / 2] * (fp + fm);
+  }
+  k *= h;
+  g *= h;
+  if (std::fabs(k - g) <= tol || depth > 40) return k;
+  return gk15(f, a, c, 0.5 * tol, depth + 1) + gk15(f, c, b, 0.5 * tol, depth + 1);
+}
+}  // namespace
+
 extern "C" {
 
 const char* cdft_last_error(void) { return g_err.c_str(); }
@@ -564,6 +592,23 @@ int cdft_memcpy_d2h(void* dst, const void* src, unsigned long long bytes) {
   return guarded([&] {
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return CDFT_OK;
+  });
+}
+
+
+int cdft_lj_hsd(double sigma, double eps, double rcut, double kT, double* hsd) {
+  return guarded([&] {
+    REQUIRE(hsd && sigma > 0 && kT > 0, "bad arguments");
+    auto vr = [&](double r) {
+      const double y = sigma / r, y3 = y * y * y, y6 = y3 * y3;
+      return 4 * eps * (y6 * y6 - y6);
+    };
+    const double shift = rcut < 0 ? 0.0 : vr(rcut);  // include/Potential1.h:142-145
+    const double rmin = std::pow(2.0, 1.0 / 6.0) * sigma;
+    const double vmin = vr(rmin) - shift;
+    auto f = [&](double r) { return r <= 0 ? 1.0 : 1.0 - std::exp(-((vr(r) - shift) - vmin) / kT); };
+    *hsd = gk15(f, 0.0, rmin, 1e-13);
+    return (int)CDFT_OK;
   });
 }
 

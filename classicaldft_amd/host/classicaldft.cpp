@@ -26,31 +26,6 @@ struct NSum {
   double sum() const { return s + c; }
 };
 
-// Adaptive Gauss-Kronrod (7/15) quadrature.
-double gk15(const std::function<double(double)>& f, double a, double b, double tol, int depth = 0) {
-  static const double xk[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
-                               0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
-                               0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
-                               0.207784955007898467600689403773245, 0.0};
-  static const double wk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
-                               0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
-                               0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
-                               0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
-  static const double wg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
-                               0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
-  const double c = 0.5 * (a + b), h = 0.5 * (b - a);
-  double k = wk[7] * f(c), g = wg[3] * f(c);
-  for (int i = 0; i < 7; ++i) {
-    const double fp = f(c + h * xk[i]), fm = f(c - h * xk[i]);
-    k += wk[i] * (fp + fm);
-    if (i % 2 == 1) g += wg[i / 2] * (fp + fm);
-  }
-  k *= h;
-  g *= h;
-  if (std::fabs(k - g) <= tol || depth > 40) return k;
-  return gk15(f, a, c, 0.5 * tol, depth + 1) + gk15(f, c, b, 0.5 * tol, depth + 1);
-}
-
 }  // namespace
 
 // ------------------------------------------------------------------ DFT_Vec
@@ -226,8 +201,9 @@ double Potential1::Watt(double r) const {  // include/Potential1.h:56-65, WCA br
 }
 
 double Potential1::getHSD(double kT) const {
-  auto f = [&](double r) { return r <= 0 ? 1.0 : 1.0 - std::exp(-V0(r) / kT); };
-  return gk15(f, 0.0, rmin_, 1e-13);
+  double d = 0.0;
+  check(cdft_lj_hsd(sigma_, eps_, rcut_, kT, &d));
+  return d;
 }
 
 LJ::LJ(double sigma, double eps, double rcut) : Potential1(sigma, eps, rcut) {  // include/Potential1.h:140-146

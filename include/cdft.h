@@ -109,6 +109,13 @@ int cdft_system_time(cdft_system sys, const double* rho_d, double* dF_d, int fla
    BulkFex src/FMT.cpp:616-709; Interaction_Base::Mu src/Interaction.cpp:61-72). */
 int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu);
 
+/* Hard-sphere diameter of the WCA-split Lennard-Jones potential (cut and
+   shifted at rcut), d = int_0^rmin (1 - exp(-V0(r)/kT)) dr with
+   V0 = V - V(rmin) for r < rmin, rmin = 2^(1/6) sigma.  Replaces
+   Potential1::getHSD (include/Potential1.h:73-83; GSL qags there, adaptive
+   Gauss-Kronrod 7/15 to 1e-13 absolute here).  Host-only. */
+int cdft_lj_hsd(double sigma, double eps, double rcut, double kT, double* hsd);
+
 /* ---------------------------------------------------------------- FIRE2 ---
    fireMinimizer2 (include/Minimizer.h:86-182, src/Minimizer.cpp:117-327) on
    the alias variables of FMT_Species (src/FMT_Species.cpp:29-93).  The
