@@ -85,6 +85,16 @@ def lib():
         L.cdft_system_get_weights.argtypes = [_p, _p]
         L.cdft_system_get_mf_kernel.argtypes = [_p, _p]
         L.cdft_lj_hsd.argtypes = [_d, _d, _d, _d, ctypes.POINTER(_d)]
+        L.cdft_system_batch_create.argtypes = [ctypes.POINTER(_Desc), _i, _p, _p, ctypes.POINTER(_p)]
+        L.cdft_system_autotune.argtypes = [_p, _i, ctypes.POINTER(_d)]
+        L.cdft_fire_batch_create.argtypes = [ctypes.POINTER(_Desc), _i, _p, _p, _p, ctypes.POINTER(FireParams),
+                                             ctypes.POINTER(_p)]
+        L.cdft_fire_batch_run.argtypes = [_p, _i, _d, _i]
+        L.cdft_fire_batch_status.argtypes = [_p, _p, _p, ctypes.POINTER(_i)]
+        L.cdft_fire_batch_get.argtypes = [_p, _i, _p]
+        L.cdft_fire_batch_system.argtypes = [_p, ctypes.POINTER(_p)]
+        L.cdft_fire_batch_destroy.argtypes = [_p]
+        L.cdft_fire_batch_destroy.restype = None
         L.cdft_fft_r2c.argtypes = [_i, _i, _i, _i, _p, _p]
         L.cdft_fft_c2r.argtypes = [_i, _i, _i, _i, _p, _p]
         L.cdft_system_eval_async.argtypes = [_p, _p, _p, _i]
@@ -135,7 +145,9 @@ class System:
     """One grid + FMT (+ ideal gas, LJ mean field, V_ext, mu) on the GPU."""
 
     def __init__(self, shape, dx, hsd=1.0, model=WHITEBEAR_I, precision=FP64, weights=None, ideal=False, mu=0.0,
-                 vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0):
+                 vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0,
+                 mus=None, vexts=None):
+        """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create)."""
         self.shape = tuple(int(n) for n in shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
@@ -157,7 +169,15 @@ class System:
                   arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
                   arr(mf_kernel, self.N))
         h = _p()
-        _check(lib().cdft_system_create(ctypes.byref(d), ctypes.byref(h)))
+        if mus is None:
+            self.nb = 1
+            _check(lib().cdft_system_create(ctypes.byref(d), ctypes.byref(h)))
+        else:
+            m = np.ascontiguousarray(mus, dtype=np.float64).reshape(-1)
+            self.nb = m.size
+            vx = arr(vexts, self.nb * self.N)
+            _check(lib().cdft_system_batch_create(ctypes.byref(d), self.nb, m.ctypes.data_as(_p), _p(vx),
+                                                  ctypes.byref(h)))
         self.h = h
         self.model = model
         self.precision = precision
@@ -174,21 +194,34 @@ class System:
             pass
 
     # -- evaluations on host arrays --
+    def _ashape(self):
+        return self.shape if self.nb == 1 else (self.nb,) + self.shape
+
     def eval(self, rho, dF=None, accumulate=False):
-        """DFT::calculateFreeEnergyAndDerivatives: returns (parts[5], dF)."""
+        """DFT::calculateFreeEnergyAndDerivatives: returns (parts[5], dF); batches: parts (nb, 5)."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
-        out = np.zeros(self.shape) if dF is None else dF
-        parts = np.zeros(5)
+        if rho.size != self.nb * self.N:
+            raise ValueError(f"expected {self.nb * self.N} densities, got {rho.size}")
+        out = np.zeros(self._ashape()) if dF is None else dF
+        parts = np.zeros(5 * self.nb)
         _check(lib().cdft_system_eval(self.h, _ptr(rho), _ptr(out), _ptr(parts), ACCUMULATE if accumulate else 0))
-        return parts, out
+        return (parts if self.nb == 1 else parts.reshape(self.nb, 5)), out
 
     def fmt_eval(self, rho, dF=None, accumulate=False):
-        """FMT::calculateFreeEnergyAndDerivatives: returns (F_hs, dF)."""
+        """FMT::calculateFreeEnergyAndDerivatives: returns (F_hs, dF); batches: F_hs (nb,)."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
-        out = np.zeros(self.shape) if dF is None else dF
-        F = _d()
-        _check(lib().cdft_fmt_eval(self.h, _ptr(rho), _ptr(out), ctypes.byref(F), ACCUMULATE if accumulate else 0))
-        return F.value, out
+        if rho.size != self.nb * self.N:
+            raise ValueError(f"expected {self.nb * self.N} densities, got {rho.size}")
+        out = np.zeros(self._ashape()) if dF is None else dF
+        F = np.zeros(self.nb)
+        _check(lib().cdft_fmt_eval(self.h, _ptr(rho), _ptr(out), _ptr(F), ACCUMULATE if accumulate else 0))
+        return (F[0] if self.nb == 1 else F), out
+
+    def autotune(self, reps=3):
+        """cdft_system_autotune: picks the fastest kernel tiles for this shape; returns ms per evaluation."""
+        ms = _d()
+        _check(lib().cdft_system_autotune(self.h, int(reps), ctypes.byref(ms)))
+        return ms.value
 
     # -- evaluations on device pointers (ints) --
     def eval_device(self, rho_ptr, dF_ptr, parts=None, fmt_only=False, accumulate=False):
@@ -206,9 +239,9 @@ class System:
         _check(lib().cdft_system_eval_async(self.h, _p(rho_ptr), _p(dF_ptr), flags))
 
     def fetch(self):
-        p = np.zeros(5)
+        p = np.zeros(5 * self.nb)
         _check(lib().cdft_system_fetch(self.h, _ptr(p)))
-        return p
+        return p if self.nb == 1 else p.reshape(self.nb, 5)
 
     def time(self, rho_ptr, dF_ptr, iters, fmt_only=False):
         """Milliseconds for `iters` back-to-back evaluations (hipEvents on the system stream)."""
@@ -334,6 +367,67 @@ class DDFT:
     def close(self):
         if getattr(self, "h", None):
             lib().cdft_ddft_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FireBatch:
+    """FIRE2 on nb same-shape systems with device-side control (cdft_fire_batch_*)."""
+
+    def __init__(self, shape, dx, mus, rho0, vexts=None, hsd=1.0, model=WHITEBEAR_I, precision=FP64, ideal=True,
+                 mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, A=1.0, B=0.0, dt=None, dt_max=None,
+                 **params):
+        self.shape = tuple(int(n) for n in shape)
+        dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+        self.N = int(np.prod(self.shape))
+        m = np.ascontiguousarray(mus, dtype=np.float64).reshape(-1)
+        self.nb = m.size
+        r0 = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
+        if r0.size != self.nb * self.N:
+            raise ValueError("rho0 must hold nb x N values")
+        vx = None if vexts is None else np.ascontiguousarray(vexts, dtype=np.float64).reshape(-1)
+        d = _Desc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A), float(B),
+                  float(hsd), int(precision), None, int(bool(ideal)), 0.0, None, int(bool(mean_field)), float(sigma),
+                  float(eps), float(rcut), float(kT), None)
+        p = FireParams()
+        _check(lib().cdft_fire_default_params(ctypes.byref(p)))
+        if dt is not None:
+            p.dt = dt
+            p.dt_max = 10 * dt if dt_max is None else dt_max
+        elif dt_max is not None:
+            p.dt_max = dt_max
+        for k, v in params.items():
+            setattr(p, k, v)
+        h = _p()
+        _check(lib().cdft_fire_batch_create(ctypes.byref(d), self.nb, m.ctypes.data_as(_p), _ptr(vx), _ptr(r0),
+                                            ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+
+    def run(self, steps, tol_raw, graph=True):
+        """Enqueue `steps` steps (no synchronisation)."""
+        _check(lib().cdft_fire_batch_run(self.h, int(steps), float(tol_raw), int(bool(graph))))
+
+    def status(self):
+        """(reports, status array, number still running); synchronises."""
+        reps = (FireReport * self.nb)()
+        st = np.zeros(self.nb, dtype=np.int32)
+        run = _i()
+        _check(lib().cdft_fire_batch_status(self.h, ctypes.cast(reps, _p), st.ctypes.data_as(_p), ctypes.byref(run)))
+        return [r.as_dict() for r in reps], st, run.value
+
+    def density(self, member):
+        out = np.zeros(self.shape)
+        _check(lib().cdft_fire_batch_get(self.h, int(member), _ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_fire_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -9,7 +9,9 @@
 namespace cdft_impl {
 
 thread_local std::string g_err;
-__global__ void k_zero_int(int* p) { *p = 0; }
+__global__ void k_zero_int(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
 
 // ------------------------------- set-up ------------------------------------
 void validate(const cdft_system_desc& d) {
@@ -91,7 +93,7 @@ void build_weight_spectra(cdft_system_s& s) {
     hipLaunchKernelGGL(k_sum_real, dim3(nblk), dim3(kThreads), 0, s.stream, w.as<double>(), N,
                        s.partials.as<double2>());
     hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials.as<double2>(), (long)nblk, 1,
-                       s.results.as<double>());
+                       s.results.as<double>(), 0L, 0L);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&s.a_vdw, s.results.p, sizeof(double), hipMemcpyDeviceToHost, s.stream));
     launch_rows_r2c<double, double>(s, w.as<double>(), 0, S.as<cplx<double>>(), 0, 1);
@@ -106,11 +108,13 @@ void build_weight_spectra(cdft_system_s& s) {
   }
 }
 
-void create_system(const cdft_system_desc* desc, cdft_system* out) {
+void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const double* mus, const double* vexts) {
   REQUIRE(desc && out, "null argument");
+  REQUIRE(nb >= 1, "member count must be positive");
   validate(*desc);
   auto s = std::make_unique<cdft_system_s>();
   s->d = *desc;
+  s->nb = nb;
   HIPCHK(hipGetDevice(&s->device));
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   Geo& g = s->g;
@@ -150,24 +154,34 @@ void create_system(const cdft_system_desc* desc, cdft_system* out) {
   if (s->mixed) s->twf.build(g);
   const size_t cs = s->mixed ? sizeof(cplx<float>) : sizeof(cplx<double>);
   const size_t rs = s->mixed ? sizeof(float) : sizeof(double);
-  s->A.alloc(cs * s->Nkp);
+  REQUIRE(nb == 1 || (rows_reg_ok<double>(*s) && reg_fft_ok<double>(*s, g.Nx) && reg_fft_ok<double>(*s, g.Ny) &&
+                      s->Nkp * (long)cs * s->nconv * nb < (1L << 40)),
+          "batched systems need Nx, Ny and Nz/2 in {4,6,8,12,16,24,32,48,64,96,128,256}");
+  s->A.alloc(cs * s->Nkp * nb);
   if (s->nconv) {
-    s->B.alloc(cs * s->Nkp * s->nconv);
+    s->B.alloc(cs * s->Nkp * s->nconv * nb);
     s->W.alloc(rs * s->Nkp * s->nconv);
   }
-  if (desc->mean_field) s->vmf.alloc(sizeof(double) * s->N);
-  s->rho.alloc(sizeof(double) * s->N);
-  s->dF.alloc(sizeof(double) * s->N);
-  const long maxblk = std::max<long>(s->nrows, 256);  // at least one row per block in every z kernel
-  s->partials.alloc(sizeof(double2) * maxblk);
-  s->partials2.alloc(sizeof(double2) * maxblk * 4);
-  s->flag.alloc(sizeof(int));
-  HIPCHK(hipMemset(s->flag.p, 0, sizeof(int)));
-  s->results.alloc(sizeof(double) * 8);
-  if (desc->vext) {
-    s->vext.alloc(sizeof(double) * s->N);
-    HIPCHK(hipMemcpy(s->vext.p, desc->vext, sizeof(double) * s->N, hipMemcpyHostToDevice));
+  if (desc->mean_field) s->vmf.alloc(sizeof(double) * s->N * nb);
+  s->rho.alloc(sizeof(double) * s->N * nb);
+  s->dF.alloc(sizeof(double) * s->N * nb);
+  s->pslots = std::max<long>(s->nrows, 256);  // at least one row per block in every z kernel
+  s->partials.alloc(sizeof(double2) * s->pslots * nb);
+  s->partials2.alloc(sizeof(double2) * s->pslots * 4 * nb);
+  s->flag.alloc(sizeof(int) * nb);
+  HIPCHK(hipMemset(s->flag.p, 0, sizeof(int) * nb));
+  s->results.alloc(sizeof(double) * 8 * nb);
+  const double* vx = vexts ? vexts : desc->vext;
+  if (vx) {
+    s->vext.alloc(sizeof(double) * s->N * nb);
+    HIPCHK(hipMemcpy(s->vext.p, vx, sizeof(double) * s->N * nb, hipMemcpyHostToDevice));
     s->has_vext = true;
+  }
+  REQUIRE(nb == 1 || mus, "batched systems need one mu per member");
+  if (mus) {
+    s->mus.alloc(sizeof(double) * nb);
+    HIPCHK(hipMemcpy(s->mus.p, mus, sizeof(double) * nb, hipMemcpyHostToDevice));
+    if (nb == 1) s->d.mu = mus[0];
   }
   if (s->mixed) build_weight_spectra<float>(*s);
   else build_weight_spectra<double>(*s);
@@ -197,14 +211,16 @@ long launch_pointwise(cdft_system_s& s, int nconv) {
       if (s.pw_rows > 0) RB = std::min(RB, s.pw_rows);
       if (RB < 1) return;
       nblk = div_up(s.nrows, RB);
-      hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk), dim3(nconv * RB * RF::P), size_t(RB) * per,
-                         s.stream, s.B.as<cplx<T>>(), s.Nkp, unsigned(s.Nkp * sizeof(cplx<T>)), s.m, nconv,
-                         s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<T>>(),
-                         tw.n.template as<cplx<T>>(), RB, pp, s.partials.as<double2>(), s.flag.as<int>());
+      const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr};
+      hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk, s.nb), dim3(nconv * RB * RF::P),
+                         size_t(RB) * per, s.stream, s.B.as<cplx<T>>(), s.Nkp, s.m, nconv, s.vmf.as<double>(), s.nrows,
+                         s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, pp,
+                         s.partials.as<double2>(), s.flag.as<int>(), bt);
       HIPCHK(hipGetLastError());
     });
     if (nblk > 0) return nblk;
   }
+  REQUIRE(s.nb == 1, "batched systems need transform lengths the register FFT supports");
   const int RB = s.RBpw;
   const size_t smem = sizeof(cplx<T>) * nconv * RB * M;
   hipLaunchKernelGGL((k_z_pointwise<T, MODEL, 16>), dim3(div_up(s.nrows, RB)), dim3(kThreads), smem, s.stream,
@@ -225,10 +241,12 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
   cplx<T>* A = s.A.as<cplx<T>>();
   cplx<T>* B = s.B.as<cplx<T>>();
   const Tables<T>& tw = tables<T>(s);
+  const int nb = s.nb;
+  const long sA = Nkp, sB = s.nconv * Nkp;  // member strides of A and of the weight-array block B
   if (nconv > 0) {
-    launch_rows_r2c<T, double>(s, rho, 0, A, 0, 1);
+    launch_rows_r2c<T, double>(s, rho, s.N, A, sA, nb);
     s.mark("z_r2c");
-    launch_cols<T>(s, A, 0, 1, 1, true);
+    launch_cols<T>(s, A, 0, 1, 1, true, nb, sA);
     s.mark("y_fwd");
     if (reg_fft_ok<T>(s, s.g.Nx)) {
       const ColGeo cg = col_geo<T>(s.g, 0);
@@ -236,12 +254,14 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         constexpr int Lc = decltype(lc)::value;
         with_nb<T, Lc>(s.nb_half_fwd, [&](auto nbc) {
           constexpr int NB = decltype(nbc)::value;
-          hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
-                             s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+          const Batch bt{sA, sB, s.N, 0, nullptr};
+          hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB), nb), dim3(NB * F2<Lc>::P), 0,
+                             s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>(), bt);
           HIPCHK(hipGetLastError());
         });
       });
     } else {
+      REQUIRE(nb == 1, "batched systems need transform lengths the register FFT supports");
       const int TW = s.TWconv;
       dim3 grid(div_up(s.g.Zh, TW), s.g.Ny);
       hipLaunchKernelGGL((k_conv_fwd<T, 16>), grid, dim3(kThreads), 2 * sizeof(cplx<T>) * s.g.Nx * TW, s.stream, A, B,
@@ -249,7 +269,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
       HIPCHK(hipGetLastError());
     }
     s.mark("x_conv_fwd");
-    launch_cols<T>(s, B, Nkp, nconv, 1, false);
+    launch_cols<T>(s, B, Nkp, nconv, 1, false, nb, sB);
     s.mark("y_inv");
     if (s.m > 0) {
       long nblk = 0;
@@ -260,10 +280,10 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         case CDFT_WHITEBEAR_I: nblk = launch_pointwise<T, M_WBI>(s, nconv); break;
         default: nblk = launch_pointwise<T, M_WBII>(s, nconv); break;
       }
-      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
-                         s.results.as<double>());
+      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+                         s.results.as<double>(), s.pslots, 8L);
       s.mark("z_pointwise");
-      launch_cols<T>(s, B, Nkp, s.m, 1, true);
+      launch_cols<T>(s, B, Nkp, s.m, 1, true, nb, sB);
       s.mark("y_fwd2");
       if (reg_fft_ok<T>(s, s.g.Nx)) {
         const ColGeo cg = col_geo<T>(s.g, 0);
@@ -271,8 +291,10 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
           constexpr int Lc = decltype(lc)::value;
           with_nb<T, Lc>(s.nb_half_bwd, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
-            hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB)), dim3(NB * F2<Lc>::P), 0,
-                               s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>());
+            const Batch bt{sA, sB, s.N, 0, nullptr};
+            hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB), nb), dim3(NB * F2<Lc>::P), 0,
+                               s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>(),
+                               bt);
             HIPCHK(hipGetLastError());
           });
         });
@@ -284,11 +306,11 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         HIPCHK(hipGetLastError());
       }
       s.mark("x_conv_bwd");
-      launch_cols<T>(s, A, 0, 1, 1, false);
+      launch_cols<T>(s, A, 0, 1, 1, false, nb, sA);
       s.mark("y_inv2");
     } else {
       // mean field only: inverse z of the single convolved array -> vmf
-      launch_rows_c2r<T, double>(s, B, 0, s.vmf.as<double>(), 0, 1, 1.0);
+      launch_rows_c2r<T, double>(s, B, sB, s.vmf.as<double>(), s.N, nb, 1.0);
       s.mark("z_c2r_mf");
     }
   }
@@ -308,12 +330,14 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         constexpr int Mc = decltype(mc)::value;
         constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
         nblk = div_up(s.nrows, RB);
-        hipLaunchKernelGGL((k_z_final2<T, Mc>), dim3(nblk), dim3(RB * P), RB * SLOT * sizeof(cplx<T>), s.stream, A, unsigned(s.Nkp * sizeof(cplx<T>)),
+        const Batch bt{sA, sB, s.N, 4 * s.pslots, with_dft && s.nb > 1 ? s.mus.as<double>() : nullptr};
+        hipLaunchKernelGGL((k_z_final2<T, Mc>), dim3(nblk, nb), dim3(RB * P), RB * SLOT * sizeof(cplx<T>), s.stream, A,
                            rho, s.vext.as<double>(), s.vmf.as<double>(), dF, unsigned(s.N * 8), s.nrows, s.g.Zp,
                            tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, fp,
-                           s.partials2.as<double2>());
+                           s.partials2.as<double2>(), bt);
       });
     } else {
+      REQUIRE(nb == 1, "batched systems need transform lengths the register FFT supports");
       const int RB = s.RBrow;
       nblk = div_up(s.nrows, RB);
       hipLaunchKernelGGL((k_z_final<T, 4>), dim3(nblk), dim3(kThreads), sizeof(cplx<T>) * RB * M, s.stream, A, rho,
@@ -321,8 +345,8 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
                          tw.n.template as<cplx<T>>(), RB, fp, s.partials2.as<double2>());
     }
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials2.as<double2>(), nblk, 4,
-                       s.results.as<double>() + 1);
+    hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials2.as<double2>(), nblk, 4,
+                       s.results.as<double>() + 1, 4 * s.pslots, 8L);
     HIPCHK(hipGetLastError());
   }
   s.mark("z_final");
@@ -347,33 +371,42 @@ void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_df
   s->last_with_dft = with_dft;
 }
 
-// Wait for the stream and turn the device results into the F parts.
+// F parts of one member from its five device results (sum Phi, sum rho ln rho - rho, sum rho v_mf, sum V rho, sum rho).
+void parts_from_results(const cdft_system_s& s, const double* res, double mu, double* parts) {
+  const bool with_dft = s.last_with_dft;
+  const double dV = s.d.dx * s.d.dy * s.d.dz;
+  const double F_hs = s.m > 0 ? res[0] * dV : 0.0;
+  const double F_id = (with_dft && s.d.ideal) ? res[1] : 0.0;
+  const double F_mf = (with_dft && s.d.mean_field) ? 0.5 * res[2] : 0.0;
+  const double mu_ = with_dft ? mu : 0.0;
+  const double F_ext = (with_dft && s.has_vext ? res[3] * dV : 0.0) - res[4] * dV * mu_;
+  parts[0] = csum({F_id, F_hs, F_mf, F_ext});
+  parts[1] = F_id;
+  parts[2] = F_hs;
+  parts[3] = F_mf;
+  parts[4] = F_ext;
+}
+
+// Wait for the stream and turn the device results into the F parts (5 per
+// member).  CDFT_ETA_TOO_LARGE if any member's flag is set (flags cleared).
 int fetch_parts(cdft_system s, double* parts) {
-  double res[5];
-  int fl = 0;
-  HIPCHK(hipMemcpyAsync(res, s->results.p, sizeof(double) * 5, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipMemcpyAsync(&fl, s->flag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  const int nb = s->nb;
+  std::vector<double> res(8 * (size_t)nb), mus(nb, s->d.mu);
+  std::vector<int> fl(nb, 0);
+  HIPCHK(hipMemcpyAsync(res.data(), s->results.p, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(fl.data(), s->flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s->stream));
+  if (nb > 1) HIPCHK(hipMemcpyAsync(mus.data(), s->mus.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  if (fl) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s->stream, s->flag.as<int>());
+  bool bad = false;
+  for (int b = 0; b < nb; ++b) bad = bad || fl[b];
+  if (bad) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(kThreads), 0, s->stream, s->flag.as<int>(), nb);
     HIPCHK(hipStreamSynchronize(s->stream));
     g_err = "Eta too large in FMT::dPHI";
     return CDFT_ETA_TOO_LARGE;
   }
-  const bool with_dft = s->last_with_dft;
-  const double dV = s->d.dx * s->d.dy * s->d.dz;
-  const double F_hs = s->m > 0 ? res[0] * dV : 0.0;
-  const double F_id = (with_dft && s->d.ideal) ? res[1] : 0.0;
-  const double F_mf = (with_dft && s->d.mean_field) ? 0.5 * res[2] : 0.0;
-  const double mu = with_dft ? s->d.mu : 0.0;
-  const double F_ext = (with_dft && s->has_vext ? res[3] * dV : 0.0) - res[4] * dV * mu;
-  if (parts) {
-    parts[0] = csum({F_id, F_hs, F_mf, F_ext});
-    parts[1] = F_id;
-    parts[2] = F_hs;
-    parts[3] = F_mf;
-    parts[4] = F_ext;
-  }
+  if (parts)
+    for (int b = 0; b < nb; ++b) parts_from_results(*s, res.data() + 8 * b, mus[b], parts + 5 * b);
   return CDFT_OK;
 }
 
@@ -381,7 +414,7 @@ int run_eval(cdft_system s, const double* rho, double* dF, double* parts, int fl
   REQUIRE(s && rho && dF, "null argument");
   const bool dev = flags & CDFT_DEVICE_PTRS;
   const bool acc = flags & CDFT_ACCUMULATE;
-  const size_t bytes = sizeof(double) * s->N;
+  const size_t bytes = sizeof(double) * s->N * s->nb;
   const double* rho_d = rho;
   double* dF_d = dF;
   if (!dev) {
@@ -521,9 +554,10 @@ int cdft_system_eval(cdft_system sys, const double* rho, double* dF, double* par
 int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, int flags) {
   return guarded([&] {
     REQUIRE(sys && sys->m > 0, "system has no FMT functional");
-    double parts[5];
-    const int rc = run_eval(sys, rho, dF, parts, flags, false);
-    if (rc == CDFT_OK && F_hs) *F_hs = parts[2];
+    std::vector<double> parts(5 * (size_t)sys->nb);
+    const int rc = run_eval(sys, rho, dF, parts.data(), flags, false);
+    if (rc == CDFT_OK && F_hs)
+      for (int b = 0; b < sys->nb; ++b) F_hs[b] = parts[5 * b + 2];
     return rc;
   });
 }

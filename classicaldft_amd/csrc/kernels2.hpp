@@ -47,6 +47,14 @@ __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned voff, 
   }
 }
 
+// Member strides of a batch of same-shape systems (member = blockIdx.y of the
+// kernels below): one complex array, the block of per-weight complex arrays, a
+// real array, the partial-sum slots; per-member chemical potentials.
+struct Batch {
+  long sa, sb, sr, sp;
+  const double* mu;  // null: use the scalar of the call
+};
+
 struct ColGeo {
   unsigned bytes;  // bytes of one complex array (Nx * Ny * Zp elements)
   unsigned ncols;  // number of columns
@@ -55,10 +63,11 @@ struct ColGeo {
   unsigned es;     // element stride along the transform
 };
 
-// In-place transform of ncols columns per array.  grid = (ceil(ncols/NB), narr), block = NB * P.
+// In-place transform of ncols columns per array.  grid = (ceil(ncols/NB), narr, members), block = NB * P.
 template <typename T, int L, int NB, bool FWD>
 __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long arr_stride, ColGeo cg,
-                                               const cplx<T>* __restrict__ tw) {
+                                               const cplx<T>* __restrict__ tw, long mstride) {
+  data += blockIdx.z * mstride;  // batch member
   using F = F2<L>;
   constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
   __shared__ cplx<T> buf[L * NB];
@@ -88,11 +97,14 @@ __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long 
 }
 
 // x pass of rho_k, then per weight a: n_a(k) = rho_k w_a(k) and its inverse x
-// pass, straight from registers.  grid = ceil(ncols/NB), block = NB * P.
+// pass, straight from registers.  grid = (ceil(ncols/NB), members), block = NB * P.
 template <typename T, int L, int NB>
 __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A, cplx<T>* __restrict__ B,
                                                    long arr_stride, const T* __restrict__ W, int nconv,
-                                                   unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
+                                                   unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw,
+                                                   Batch bt) {
+  A += blockIdx.y * bt.sa;
+  B += blockIdx.y * bt.sb;
   using F = F2<L>;
   constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
   __shared__ cplx<T> buf[L * NB];
@@ -149,12 +161,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
 }
 
 // x pass of each dPhi_a, acc = sum_a w_a(k) dPhi_a(k) in registers, inverse x
-// pass of acc.  grid = ceil(ncols/NB), block = NB * P.  The weights of array a
+// pass of acc.  grid = (ceil(ncols/NB), members), block = NB * P.  The weights of array a
 // are loaded as one batch after its transform (all loads in flight at once).
 template <typename T, int L, int NB>
 __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D, cplx<T>* __restrict__ Acc,
                                                    long arr_stride, const T* __restrict__ W, int m,
-                                                   unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw) {
+                                                   unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw,
+                                                   Batch bt) {
+  D += blockIdx.y * bt.sb;
+  Acc += blockIdx.y * bt.sa;
   using F = F2<L>;
   constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
   __shared__ cplx<T> buf[L * NB];

@@ -70,10 +70,13 @@ __device__ void block_reduce_store_n(CSum* v, double2* __restrict__ partials, lo
 }
 
 // Final deterministic reduction of nblk x nq partials -> out[q] = s + c, one
-// block of 1024 threads: fixed strided assignment, 8 loads in flight per
+// block of 1024 threads per batch member (partials sp, outputs so apart): fixed strided assignment, 8 loads in flight per
 // thread, xor butterflies in the wavefronts, wavefront results merged in order.
 static __global__ __launch_bounds__(1024) void k_reduce_partials_wide(const double2* __restrict__ partials, long nblk,
-                                                                      int nq, double* __restrict__ out) {
+                                                                      int nq, double* __restrict__ out, long sp,
+                                                                      long so) {
+  partials += blockIdx.x * sp;  // one block per batch member
+  out += blockIdx.x * so;
   __shared__ double ws[16][2];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   for (int q = 0; q < nq; ++q) {
@@ -171,9 +174,8 @@ __device__ __forceinline__ void r2c_from_regs(cplx<T>* slot, int t, const cplx<T
 // Real rows -> half spectra.  grid = (ceil(nrows/RB), narr), block = RB * P,
 // dynamic LDS = RB * SLOT elements.
 template <typename T, typename TI, int M>
-__global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, long in_arr_stride, unsigned in_bytes,
-                                                   cplx<T>* __restrict__ out, long out_arr_stride, unsigned out_bytes,
-                                                   long nrows, int Zp, const cplx<T>* __restrict__ twM,
+__global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, long in_arr_stride,
+                                                   cplx<T>* __restrict__ out, long out_arr_stride, long nrows, int Zp, const cplx<T>* __restrict__ twM,
                                                    const cplx<T>* __restrict__ twN, int RB) {
   using RF = RowF<T, M>;
   extern __shared__ unsigned char smem_raw[];
@@ -199,8 +201,7 @@ __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, lo
 // Half spectra -> real rows (unnormalised, times `scale`).
 template <typename T, typename TO, int M>
 __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ in, long in_arr_stride,
-                                                   unsigned in_bytes, TO* __restrict__ out, long out_arr_stride,
-                                                   unsigned out_bytes, long nrows, int Zp,
+                                                   TO* __restrict__ out, long out_arr_stride, long nrows, int Zp,
                                                    const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
                                                    int RB, double scale) {
   using RF = RowF<T, M>;
@@ -225,13 +226,18 @@ __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ i
 // Pointwise z pass: inverse z of the nconv arrays of RB rows, Phi / dPhi at
 // every site, forward z of the m dPhi rows back into B, v_mf = the (m+1)-th
 // array in real space, F partial sums.  Sequence s = a * RB + r.
-// grid = ceil(nrows/RB), block = nconv * RB * P, dynamic LDS = nconv * RB * SLOT elements.
+// grid = (ceil(nrows/RB), members), block = nconv * RB * P, dynamic LDS = nconv * RB * SLOT elements.
 template <typename T, int MODEL, int M>
-__global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, unsigned arr_bytes,
-                                                      int m, int nconv, double* __restrict__ vmf, long nrows, int Zp,
+__global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, int m, int nconv,
+                                                      double* __restrict__ vmf, long nrows, int Zp,
                                                       const cplx<T>* __restrict__ twM,
                                                       const cplx<T>* __restrict__ twN, int RB, PhiParams pp,
-                                                      double2* __restrict__ partials, int* __restrict__ flag) {
+                                                      double2* __restrict__ partials, int* __restrict__ flag,
+                                                      Batch bt) {
+  B += blockIdx.y * bt.sb;
+  if (vmf) vmf += blockIdx.y * bt.sr;
+  partials += blockIdx.y * bt.sp;
+  flag += blockIdx.y;
   using RF = RowF<T, M>;
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -298,12 +304,19 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
 // registers of the inverse transform.  partials per block:
 // {sum (rho ln rho - rho) dV, sum rho v_mf, sum V_ext rho, sum rho}.
 template <typename T, int M>
-__global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Acc, unsigned acc_bytes,
-                                                  const double* __restrict__ rho, const double* __restrict__ vext,
-                                                  const double* __restrict__ vmf, double* __restrict__ dF,
-                                                  unsigned real_bytes, long nrows, int Zp,
+__global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Acc, const double* __restrict__ rho,
+                                                  const double* __restrict__ vext, const double* __restrict__ vmf,
+                                                  double* __restrict__ dF, unsigned real_bytes, long nrows, int Zp,
                                                   const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
-                                                  int RB, FinalParams fp, double2* __restrict__ partials) {
+                                                  int RB, FinalParams fp, double2* __restrict__ partials, Batch bt) {
+  const long mb = blockIdx.y;  // batch member
+  Acc += mb * bt.sa;
+  rho += mb * bt.sr;
+  dF += mb * bt.sr;
+  if (vext) vext += mb * bt.sr;
+  if (vmf) vmf += mb * bt.sr;
+  partials += mb * bt.sp;
+  if (bt.mu) fp.mu = bt.mu[mb];
   using RF = RowF<T, M>;
   extern __shared__ unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;

@@ -79,26 +79,6 @@ __global__ void k_finish(const double2* __restrict__ partials, int nblk, int NS,
 __global__ void k_zero_flag(int* p) { *p = 0; }
 
 // ----------------------------- FIRE2 kernels -------------------------------
-// alias <-> density: FMT_Species (c = (1 - 1e-18 * 4 pi d^3/3)/dV) or plain
-// Species (rho = 1e-18 + x^2, src/Species.cpp:24-48).
-struct Alias {
-  double c;
-  int fmt;
-  __device__ __forceinline__ double rho(double y) const {
-    return fmt ? 1e-18 + c * y * y / (1 + y * y) : 1e-18 + y * y;
-  }
-  __device__ __forceinline__ double x_of(double r) const {
-    if (fmt) {
-      const double z = (r - 1e-18) / c;
-      return sqrt(fabs(z / (1 - z)));
-    }
-    return sqrt(fmax(0.0, r - 1e-20));
-  }
-  __device__ __forceinline__ double dfac(double y) const {
-    return fmt ? 2 * c * y / ((1 + y * y) * (1 + y * y)) : 2 * y;
-  }
-};
-
 __global__ void k_alias_from_rho(const double* __restrict__ rho, double* __restrict__ x, long N, Alias al) {
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) x[i] = al.x_of(rho[i]);
 }
@@ -539,6 +519,7 @@ int cdft_fire_default_params(cdft_fire_params* p) {
 int cdft_fire_create(cdft_system sys, const double* rho0, int flags, const cdft_fire_params* params, cdft_fire* out) {
   return guarded([&] {
     REQUIRE(sys && rho0 && out, "null argument");
+    REQUIRE(sys->nb == 1, "use cdft_fire_batch_create for batched systems");
     auto f = std::make_unique<cdft_fire_s>();
     f->sys = sys;
     f->N = sys->N;
@@ -620,6 +601,7 @@ int cdft_fire_destroy(cdft_fire f) {
 int cdft_ddft_create(cdft_system sys, const double* rho0, double dt, double dt_max, double tol, cdft_ddft* out) {
   return guarded([&] {
     REQUIRE(sys && rho0 && out, "null argument");
+    REQUIRE(sys->nb == 1, "DDFT runs on single systems");
     auto d = std::make_unique<cdft_ddft_s>();
     d->sys = sys;
     d->N = sys->N;

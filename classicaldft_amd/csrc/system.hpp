@@ -112,6 +112,25 @@ struct Tables {
 
 inline int div_up(long a, long b) { return int((a + b - 1) / b); }
 
+// alias <-> density: FMT_Species (c = (1 - 1e-18 * 4 pi d^3/3)/dV) or plain
+// Species (rho = 1e-18 + x^2, src/Species.cpp:24-48).
+struct Alias {
+  double c;
+  int fmt;
+  __device__ __forceinline__ double rho(double y) const {
+    return fmt ? 1e-18 + c * y * y / (1 + y * y) : 1e-18 + y * y;
+  }
+  __device__ __forceinline__ double x_of(double r) const {
+    if (fmt) {
+      const double z = (r - 1e-18) / c;
+      return sqrt(fabs(z / (1 - z)));
+    }
+    return sqrt(fmax(0.0, r - 1e-20));
+  }
+  __device__ __forceinline__ double dfac(double y) const {
+    return fmt ? 2 * c * y / ((1 + y * y) * (1 + y * y)) : 2 * y;
+  }
+};
 
 }  // namespace cdft_impl
 
@@ -132,6 +151,11 @@ struct cdft_system_s {
   DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
   double a_vdw = 0.0;
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
+  // batch of nb same-shape members (weights shared; rho, dF, V_ext, v_mf, mu,
+  // spectra, partial sums, flags and results per member); nb = 1 for one system
+  int nb = 1;
+  long pslots = 0;  // partial-sum slots per member
+  DevBuf mus;       // per-member mu (batches only)
   bool reg_fft = true;  // register two-pass FFT for the x / y passes where the length allows
   // half the columns per block in the register column kernels: x/y passes,
   // forward convolution, backward convolution (tuned per precision at create)
@@ -213,9 +237,8 @@ inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride
       constexpr int Mc = decltype(mc)::value;
       constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
       hipLaunchKernelGGL((k_rows_r2c2<T, TI, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
-                         RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride,
-                         unsigned(s.N * sizeof(TI)), out, out_stride, unsigned(s.Nkp * sizeof(cplx<T>)), s.nrows,
-                         s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
+                         RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride, out, out_stride, s.nrows, s.g.Zp,
+                         tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
     });
     HIPCHK(hipGetLastError());
     return;
@@ -237,9 +260,8 @@ inline void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_s
       constexpr int Mc = decltype(mc)::value;
       constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
       hipLaunchKernelGGL((k_rows_c2r2<T, TO, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
-                         RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride,
-                         unsigned(s.Nkp * sizeof(cplx<T>)), out, out_stride, unsigned(s.N * sizeof(TO)), s.nrows,
-                         s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, scale);
+                         RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride, out, out_stride, s.nrows, s.g.Zp,
+                         tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, scale);
     });
     HIPCHK(hipGetLastError());
     return;
@@ -298,7 +320,8 @@ inline bool reg_fft_ok(const cdft_system_s& s, int L) {
 }
 
 template <typename T>
-inline bool launch_cols_reg(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd) {
+inline bool launch_cols_reg(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd,
+                            int members, long mstride) {
   const int L = axis == 0 ? s.g.Nx : s.g.Ny;
   if (!reg_fft_ok<T>(s, L)) return false;
   const Tables<T>& tw = tables<T>(s);
@@ -308,19 +331,23 @@ inline bool launch_cols_reg(const cdft_system_s& s, cplx<T>* data, long stride, 
     constexpr int Lc = decltype(lc)::value;
     with_nb<T, Lc>(s.nb_half_cols, [&](auto nbc) {
       constexpr int NB = decltype(nbc)::value;
-      const dim3 grid(div_up(cg.ncols, NB), narr), block(NB * F2<Lc>::P);
+      const dim3 grid(div_up(cg.ncols, NB), narr, members), block(NB * F2<Lc>::P);
       if (fwd)
-        hipLaunchKernelGGL((k_cols2<T, Lc, NB, true>), grid, block, 0, s.stream, data, stride, cg, t);
+        hipLaunchKernelGGL((k_cols2<T, Lc, NB, true>), grid, block, 0, s.stream, data, stride, cg, t, mstride);
       else
-        hipLaunchKernelGGL((k_cols2<T, Lc, NB, false>), grid, block, 0, s.stream, data, stride, cg, t);
+        hipLaunchKernelGGL((k_cols2<T, Lc, NB, false>), grid, block, 0, s.stream, data, stride, cg, t, mstride);
       HIPCHK(hipGetLastError());
     });
   });
 }
 
+// x or y pass of narr arrays (stride apart) of each of `members` batch members
+// (mstride apart).  Batches need the register kernels.
 template <typename T>
-inline void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd) {
-  if (launch_cols_reg<T>(s, data, stride, narr, axis, fwd)) return;
+inline void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int narr, int axis, bool fwd,
+                        int members = 1, long mstride = 0) {
+  if (launch_cols_reg<T>(s, data, stride, narr, axis, fwd, members, mstride)) return;
+  REQUIRE(members == 1, "batched systems need transform lengths the register FFT supports");
   const Tables<T>& tw = tables<T>(s);
   const Radices& rad = axis == 0 ? s.radX : s.radY;
   const cplx<T>* t = (axis == 0 ? tw.x : tw.y).template as<cplx<T>>();
@@ -340,6 +367,10 @@ inline void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int 
 namespace cdft_impl {
 // Defined in cdft_api.hip.
 void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc);
+// nb > 1: a batch of nb same-shape members with their own mus[nb] and V_ext
+// (nb x N or none); desc->mu / desc->vext are then ignored.
+void create_system(const cdft_system_desc* desc, cdft_system* out, int nb = 1, const double* mus = nullptr,
+                   const double* vexts = nullptr);
 int fetch_parts(cdft_system s, double* parts);
 double csum(std::initializer_list<double> v);
 template <typename F>

@@ -185,6 +185,46 @@ int cdft_fft_c2r(int Nx, int Ny, int Nz, int precision, const double* in, double
 int cdft_system_set_timing(cdft_system sys, int enable);
 int cdft_system_get_timing(cdft_system sys, int* n, double* ms, const char** names);
 
+/* ------------------------------------------------------------- batches ---
+   No counterpart in the reference, which holds one system per DFT object;
+   for callers that step many independent systems at once (parameter scans,
+   training-data generation).
+   A batch is one cdft_system with nb members of the same grid and functional
+   (desc: shape, spacing, model, hsd, precision, ideal / mean-field / LJ
+   fields; desc->mu and desc->vext are ignored) with their own mu[nb] and
+   V_ext (nb x N, member-major, or NULL).  The weight spectra are shared and
+   every kernel of the evaluation runs all members in one launch.
+   cdft_system_eval / _eval_async / _fetch / _time and cdft_fmt_eval on a
+   batch take nb x N densities and forces and return nb x 5 parts (nb F_hs).
+   Needs Nx, Ny and Nz/2 in {4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 256}. */
+int cdft_system_batch_create(const cdft_system_desc* desc, int nb, const double* mu, const double* vext,
+                             cdft_system* out);
+
+/* Tile autotuner: times `reps` evaluations of a uniform density for each
+   kernel variant (register or LDS Stockham transforms, columns per block of
+   the x / y passes and of both convolutions, rows per block of the pointwise
+   kernel), keeps the fastest on `sys` and caches it per (shape, precision,
+   functional, member count) for the process.  Overwrites the system's own
+   density / force buffers (not the caller's). */
+int cdft_system_autotune(cdft_system sys, int reps, double* best_ms);
+
+/* FIRE2 on every member of a batch with the control logic on the device
+   (same step as cdft_fire_step, src/Minimizer.cpp:160-327, per member).
+   create: evaluates the start densities rho0 (nb x N, host).  run: enqueues
+   `steps` steps of every member that has not converged (max_i |dF/drho_i| /
+   dV <= tol_raw) or failed; use_graph = 1 captures the `steps` steps once
+   into a hipGraph and launches it (recaptured when steps or tol change).
+   run does not synchronise; status does, and reports per member (status 0
+   running, 1 converged, 2 uphill limit) and the number still running. */
+typedef struct cdft_fire_batch_s* cdft_fire_batch;
+int cdft_fire_batch_create(const cdft_system_desc* desc, int nb, const double* mu, const double* vext,
+                           const double* rho0, const cdft_fire_params* params, cdft_fire_batch* out);
+int cdft_fire_batch_run(cdft_fire_batch b, int steps, double tol_raw, int use_graph);
+int cdft_fire_batch_status(cdft_fire_batch b, cdft_fire_report* reports, int* status, int* running);
+int cdft_fire_batch_get(cdft_fire_batch b, int member, double* rho);
+int cdft_fire_batch_system(cdft_fire_batch b, cdft_system* sys);
+void cdft_fire_batch_destroy(cdft_fire_batch b);
+
 #ifdef __cplusplus
 }
 #endif
