@@ -170,6 +170,111 @@ def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha
     return out
 
 
+def well_systems(n, count, first_seed, dx=0.1):
+    """Per system s (seed first_seed + s): bulk density U[0.3, 0.8] and V_ext = sum of 3 Gaussian wells,
+    amplitude U[-3, 3] kT, width U[0.3, 1.0], centre U[0, L)^3, minimum-image distances."""
+    shape = (n, n, n)
+    L = n * dx
+    x = np.arange(n) * dx
+    rho_b, vexts = [], []
+    for s in range(count):
+        u = splitmix64_uniform(16, first_seed + s)
+        v = np.zeros(shape)
+        for j in range(3):
+            amp, w = -3 + 6 * u[1 + j], 0.3 + 0.7 * u[4 + j]
+            c = u[7 + 3 * j: 10 + 3 * j] * L
+            d = [np.minimum(np.abs(x - c[k]), L - np.abs(x - c[k])) ** 2 for k in range(3)]
+            v += amp * np.exp(-(d[0][:, None, None] + d[1][None, :, None] + d[2][None, None, :]) / (2 * w * w))
+        rho_b.append(0.3 + 0.5 * u[0])
+        vexts.append(v)
+    return np.array(rho_b), np.array(vexts)
+
+
+def batch_fire(cdft, classes=((48, 32), (64, 32), (96, 32)), dx=0.1, kT=0.8, tol=1e-8, ladder_tol=1e-5,
+               ladder_rounds=24, check=25, max_rounds=2000, sequential=True):
+    """Batched FIRE2 to max |dF/drho| <= tol on systems of several grid sizes: one device-controlled,
+    hipGraph-captured FIRE loop per grid class (all members in every launch), tiles autotuned per class,
+    mixed precision until ladder_tol (at most ladder_rounds x check steps), then FP64 from a FIRE restart."""
+    hsd = cdft.lj_hsd(kT)
+    kw = dict(hsd=hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=kT)
+    seed = 0x5EED4000
+    batches, info = [], {"classes": [], "tol": tol, "ladder_tol": ladder_tol, "hsd": hsd, "dx": dx, "kT": kT}
+    for n, count in classes:
+        rho_b, vexts = well_systems(n, count, seed, dx)
+        seed += count
+        probe = cdft.System((n, n, n), dx, **kw)
+        mus = np.array([probe.bulk(r)[1] for r in rho_b])
+        probe.close()
+        rho0 = np.repeat(rho_b, n ** 3).reshape(count, n, n, n)
+        fb = cdft.FireBatch((n, n, n), dx, mus, rho0, vexts=vexts, precision=cdft.MIXED, **kw)
+        t_mixed = fb.autotune(reps=3)
+        batches.append(fb)
+        info["classes"].append({"grid": n, "systems": count, "ms_per_batched_eval_mixed": t_mixed})
+    t0 = time.perf_counter()
+    phase_t = []
+    for phase, (prec, ptol) in enumerate(((cdft.MIXED, ladder_tol), (cdft.FP64, tol))):
+        if phase == 1:
+            for b, fb in enumerate(batches):
+                fb.set_precision(prec)
+                info["classes"][b]["ms_per_batched_eval_fp64"] = fb.autotune(reps=3)
+        tp = time.perf_counter()
+        live = list(range(len(batches)))
+        last = tp
+        for rnd in range(ladder_rounds if phase == 0 else max_rounds):
+            for b in live:
+                batches[b].run(check, ptol)
+            still, worst, nrun = [], 0.0, 0
+            for b in live:
+                reps, st, running = batches[b].status()
+                if running:
+                    still.append(b)
+                    nrun += running
+                    worst = max(worst, max(r["fmax_raw"] for r, q in zip(reps, st) if q == 0))
+            live = still
+            if not live:
+                break
+            if time.perf_counter() - last > 10:
+                last = time.perf_counter()
+                print(f"#   phase {phase} round {rnd}: {nrun} running, worst max|dF/drho| {worst:.3e}, "
+                      f"{last - tp:.1f} s", flush=True)
+        phase_t.append(time.perf_counter() - tp)
+        print(f"#   phase {phase} done in {phase_t[-1]:.2f} s", flush=True)
+    total = time.perf_counter() - t0
+    steps = conv = fails = 0
+    for b, fb in enumerate(batches):
+        reps, st, _ = fb.status()
+        cs = [r["steps"] for r in reps]
+        info["classes"][b].update({"converged": int((st == 1).sum()), "max_steps": int(max(cs)),
+                                   "mean_steps": float(np.mean(cs)), "max_fmax_raw": float(max(r["fmax_raw"] for r in reps))})
+        steps += sum(cs)
+        conv += int((st == 1).sum())
+        fails += int((st == 2).sum())
+        fb.close()
+    info.update({"seconds": total, "seconds_mixed_phase": phase_t[0], "seconds_fp64_phase": phase_t[1],
+                 "systems": sum(c for _, c in classes), "converged": conv, "failed": fails, "system_steps": steps,
+                 "systems_per_s": conv / total, "system_steps_per_s": steps / total, "check_every": check})
+    if sequential:
+        # one system per class with the host-controlled single-system FIRE2 (FP64), scaled by the class size
+        est = 0.0
+        seed = 0x5EED4000
+        for n, count in classes:
+            rho_b, vexts = well_systems(n, 1, seed, dx)
+            seed += count
+            s = cdft.System((n, n, n), dx, **kw)
+            mu = s.bulk(rho_b[0])[1]
+            s.close()
+            s = cdft.System((n, n, n), dx, mu=mu, vext=vexts[0], **kw)
+            f = cdft.Fire(s, np.full((n, n, n), rho_b[0]))
+            t1 = time.perf_counter()
+            f.run(100000, tol)
+            est += (time.perf_counter() - t1) * count
+            f.close()
+            s.close()
+        info["sequential_single_system_estimate_s"] = est
+        info["batched_speedup_vs_sequential"] = est / total
+    return info
+
+
 def cpu_baseline(rho, threads):
     import oracle
     oracle.set_threads(threads, threads)
@@ -193,6 +298,7 @@ def main():
     ap.add_argument("--fire-grid", type=int, default=128, help="slit-pore FIRE time-to-tolerance grid (0: skip)")
     ap.add_argument("--fire-max-steps", type=int, default=20000)
     ap.add_argument("--ddft-steps", type=int, default=0, help="DDFT fcc-seed steps at 256^3 (0: skip)")
+    ap.add_argument("--batch", action="store_true", help="batched FIRE on 96 systems of 48^3 / 64^3 / 96^3")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -269,6 +375,9 @@ def main():
         result["fire_time_to_tol"] = {"cold": fire_time_to_tol(cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
                                       "start": "rho_bulk exp(-V_ext)", "tol": 1e-8,
                                       "criterion": "max_i |dF/drho_i| / dV (raw gradient)", "warm": None}
+    if args.batch:
+        print("# batched FIRE", flush=True)
+        result["batched_fire"] = batch_fire(cdft)
     if args.ddft_steps:
         print("# DDFT fcc seed", flush=True)
         result["ddft_fcc_seed"] = ddft_fcc_seed(cdft, args.ddft_steps)

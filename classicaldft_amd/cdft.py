@@ -90,6 +90,7 @@ def lib():
         L.cdft_fire_batch_create.argtypes = [ctypes.POINTER(_Desc), _i, _p, _p, _p, ctypes.POINTER(FireParams),
                                              ctypes.POINTER(_p)]
         L.cdft_fire_batch_run.argtypes = [_p, _i, _d, _i]
+        L.cdft_fire_batch_set_precision.argtypes = [_p, _i]
         L.cdft_fire_batch_status.argtypes = [_p, _p, _p, ctypes.POINTER(_i)]
         L.cdft_fire_batch_get.argtypes = [_p, _i, _p]
         L.cdft_fire_batch_system.argtypes = [_p, ctypes.POINTER(_p)]
@@ -411,6 +412,18 @@ class FireBatch:
     def run(self, steps, tol_raw, graph=True):
         """Enqueue `steps` steps (no synchronisation)."""
         _check(lib().cdft_fire_batch_run(self.h, int(steps), float(tol_raw), int(bool(graph))))
+
+    def autotune(self, reps=3):
+        """cdft_system_autotune on the batch's current system; ms per evaluation of all members."""
+        sysh = _p()
+        _check(lib().cdft_fire_batch_system(self.h, ctypes.byref(sysh)))
+        ms = _d()
+        _check(lib().cdft_system_autotune(sysh, int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def set_precision(self, precision):
+        """Evaluate with FP64 or MIXED from now on (re-evaluates the current densities)."""
+        _check(lib().cdft_fire_batch_set_precision(self.h, int(precision)))
 
     def status(self):
         """(reports, status array, number still running); synchronises."""

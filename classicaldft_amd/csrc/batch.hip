@@ -324,6 +324,27 @@ __global__ void k_fb_alias_init(const double* __restrict__ rho0, double* __restr
   }
 }
 
+// Restart of the FIRE2 control for a precision change: the members that met the
+// old tolerance run again, time step / mixing / counters start afresh (the
+// velocities are zeroed separately); step and call counts are kept.
+__global__ void k_fb_reopen(FireCtl* __restrict__ st, int nb, cdft_fire_params p) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= nb) return;
+  FireCtl c = st[y];
+  if (c.status == 1) c.status = 0;
+  c.active = c.status == 0;
+  c.restore = 0;
+  c.uphill = 0;
+  c.dt = p.dt;
+  c.dt_max = p.dt_max;
+  c.dt_best = p.dt;
+  c.alpha = p.alpha_start;
+  c.N_P_positive = c.N_P_negative = c.backtracks = 0;
+  c.it = 0;
+  c.vv = 1.0;
+  st[y] = c;
+}
+
 __global__ void k_fill(double* __restrict__ p, long n, double v) {
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long)gridDim.x * kThreads) p[i] = v;
 }
@@ -340,7 +361,10 @@ __global__ void k_zero_ints(int* __restrict__ p, int n) {
 using namespace cdft_impl;
 
 struct cdft_fire_batch_s {
-  cdft_system sys = nullptr;  // owned
+  cdft_system sys = nullptr;        // the system evaluating now (one of sysp)
+  cdft_system sysp[2] = {nullptr, nullptr};  // owned, by precision (FP64, MIXED), created on demand
+  cdft_system_desc desc{};
+  std::vector<double> mus, vexts;  // host copies for creating the other precision
   int nb = 0;
   long N = 0;
   CtlParams cp{};
@@ -351,9 +375,16 @@ struct cdft_fire_batch_s {
   double graph_tol = -1;
 
   ~cdft_fire_batch_s() {
+    drop_graph();
+    for (cdft_system q : sysp)
+      if (q) cdft_system_destroy(q);
+  }
+
+  void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (sys) cdft_system_destroy(sys);
+    exec = nullptr;
+    graph = nullptr;
   }
 
   dim3 vgrid() const { return dim3(kBB, nb); }
@@ -411,6 +442,11 @@ int cdft_fire_batch_create(const cdft_system_desc* desc, int nb, const double* m
     cdft_system s = nullptr;
     create_system(desc, &s, nb, mu, vext);
     b->sys = s;
+    b->sysp[desc->precision == CDFT_MIXED ? 1 : 0] = s;
+    b->desc = *desc;
+    b->desc.vext = nullptr;
+    b->mus.assign(mu, mu + nb);
+    if (vext) b->vexts.assign(vext, vext + (size_t)s->N * nb);
     b->nb = nb;
     b->N = s->N;
     const long n = s->N * nb;
@@ -466,10 +502,7 @@ int cdft_fire_batch_run(cdft_fire_batch b, int steps, double tol_raw, int use_gr
       return (int)CDFT_OK;
     }
     if (!b->exec || b->graph_steps != steps || b->graph_tol != tol_raw) {
-      if (b->exec) HIPCHK(hipGraphExecDestroy(b->exec));
-      if (b->graph) HIPCHK(hipGraphDestroy(b->graph));
-      b->exec = nullptr;
-      b->graph = nullptr;
+      b->drop_graph();
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       try {
         for (int k = 0; k < steps; ++k) b->enqueue_step();
@@ -485,6 +518,32 @@ int cdft_fire_batch_run(cdft_fire_batch b, int steps, double tol_raw, int use_gr
       b->graph_tol = tol_raw;
     }
     HIPCHK(hipGraphLaunch(b->exec, st));
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_fire_batch_set_precision(cdft_fire_batch b, int precision) {
+  return guarded([&] {
+    REQUIRE(b && (precision == CDFT_FP64 || precision == CDFT_MIXED), "bad argument");
+    const int k = precision == CDFT_MIXED ? 1 : 0;
+    if (b->sys == b->sysp[k]) return (int)CDFT_OK;
+    HIPCHK(hipStreamSynchronize(b->sys->stream));
+    if (!b->sysp[k]) {
+      cdft_system_desc d = b->desc;
+      d.precision = precision;
+      create_system(&d, &b->sysp[k], b->nb, b->mus.data(), b->vexts.empty() ? nullptr : b->vexts.data());
+    }
+    b->sys = b->sysp[k];
+    b->drop_graph();
+    // members that met the tolerance at the old precision run again; the
+    // current densities are re-evaluated at the new one
+    hipLaunchKernelGGL(k_fb_reopen, dim3(div_up(b->nb, 64)), dim3(64), 0, b->sys->stream, b->ctl.as<FireCtl>(), b->nb,
+                       b->cp.p);
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(kThreads), 0, b->sys->stream, b->v.as<double>(), b->N * b->nb, 0.0);
+    HIPCHK(hipGetLastError());
+    b->cp.tol = -1.0;  // no convergence test on the re-evaluation; the next run sets the tolerance
+    b->eval_and_post(1);
+    HIPCHK(hipStreamSynchronize(b->sys->stream));
     return (int)CDFT_OK;
   });
 }

@@ -220,6 +220,14 @@ typedef struct cdft_fire_batch_s* cdft_fire_batch;
 int cdft_fire_batch_create(const cdft_system_desc* desc, int nb, const double* mu, const double* vext,
                            const double* rho0, const cdft_fire_params* params, cdft_fire_batch* out);
 int cdft_fire_batch_run(cdft_fire_batch b, int steps, double tol_raw, int use_graph);
+/* Switches the evaluations to FP64 or MIXED (the other system is created on
+   first use): a precision ladder, mixed-precision steps until the FP32
+   gradient noise (~1e-6 in max |dF/drho|) then FP64 to the final tolerance.
+   Re-evaluates the current densities and restarts the FIRE2 control of every
+   running member (v = 0, dt, dt_max, alpha and counters from the parameters):
+   a noisy mixed-precision end game can shrink dt_max by back-tracking.
+   Members marked converged run again. */
+int cdft_fire_batch_set_precision(cdft_fire_batch b, int precision);
 int cdft_fire_batch_status(cdft_fire_batch b, cdft_fire_report* reports, int* status, int* running);
 int cdft_fire_batch_get(cdft_fire_batch b, int member, double* rho);
 int cdft_fire_batch_system(cdft_fire_batch b, cdft_system* sys);

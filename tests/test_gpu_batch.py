@@ -144,3 +144,30 @@ def test_autotune_keeps_results(cdft):
     q1, e1 = s.eval(rhos[0])
     assert np.max(np.abs(q1 - q0)) <= 1e-13 * np.max(np.abs(q0))
     assert np.max(np.abs(e1 - e0)) <= 1e-13 * np.max(np.abs(e0))
+
+
+def test_precision_ladder_reaches_fp64_equilibrium(cdft):
+    """Mixed precision until its gradient noise floor, then FP64: the same equilibrium as FP64 throughout."""
+    nb = 2
+    rho_b, vexts, rhos = member_inputs(nb, seed=0x5EED4600)
+    kw = dict(hsd=1.0, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=KT)
+    mus = np.array([cdft.System(SHAPE, DX, hsd=1.0, ideal=True, mean_field=True, kT=KT).bulk(r)[1] for r in rho_b])
+    ref = cdft.FireBatch(SHAPE, DX, mus, rhos, vexts=vexts, **kw)
+    lad = cdft.FireBatch(SHAPE, DX, mus, rhos, vexts=vexts, precision=cdft.MIXED, **kw)
+    for fb, phases in ((ref, [(cdft.FP64, 1e-9)]), (lad, [(cdft.MIXED, 3e-6), (cdft.FP64, 1e-9)])):
+        for prec, tol in phases:
+            fb.set_precision(prec)
+            for _ in range(400):
+                fb.run(25, tol)
+                _, st, running = fb.status()
+                if running == 0:
+                    break
+            assert (st == 1).all(), (prec, st)
+    # both end at max |dF/drho| <= 1e-9 of the FP64 functional; the grand potentials agree to second order in
+    # the density difference, which a soft mode of this near-critical LJ state leaves at ~1e-6
+    r_ref, _, _ = ref.status()
+    r_lad, _, _ = lad.status()
+    for b in range(nb):
+        assert abs(r_lad[b]["F"] - r_ref[b]["F"]) <= 1e-10 * abs(r_ref[b]["F"])
+        r0, r1 = ref.density(b), lad.density(b)
+        assert np.max(np.abs(r1 - r0)) <= 1e-5 * np.max(r0)
