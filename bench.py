@@ -26,22 +26,13 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+from classicaldft_amd.workloads import splitmix64_uniform, well_systems  # noqa: E402  (no library load)
 
 SEED = 0x5EED0256
 SHAPE = (256, 256, 256)
 DX = 0.04
 HSD = 1.0
 PEAK_HBM_GBS = 8000.0
-
-
-def splitmix64_uniform(n, seed):
-    k = np.arange(1, n + 1, dtype=np.uint64)
-    with np.errstate(over="ignore"):
-        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
 
 
 def workload_density(shape=SHAPE):
@@ -168,26 +159,6 @@ def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha
     d.close()
     s.close()
     return out
-
-
-def well_systems(n, count, first_seed, dx=0.1):
-    """Per system s (seed first_seed + s): bulk density U[0.3, 0.8] and V_ext = sum of 3 Gaussian wells,
-    amplitude U[-3, 3] kT, width U[0.3, 1.0], centre U[0, L)^3, minimum-image distances."""
-    shape = (n, n, n)
-    L = n * dx
-    x = np.arange(n) * dx
-    rho_b, vexts = [], []
-    for s in range(count):
-        u = splitmix64_uniform(16, first_seed + s)
-        v = np.zeros(shape)
-        for j in range(3):
-            amp, w = -3 + 6 * u[1 + j], 0.3 + 0.7 * u[4 + j]
-            c = u[7 + 3 * j: 10 + 3 * j] * L
-            d = [np.minimum(np.abs(x - c[k]), L - np.abs(x - c[k])) ** 2 for k in range(3)]
-            v += amp * np.exp(-(d[0][:, None, None] + d[1][None, :, None] + d[2][None, None, :]) / (2 * w * w))
-        rho_b.append(0.3 + 0.5 * u[0])
-        vexts.append(v)
-    return np.array(rho_b), np.array(vexts)
 
 
 def batch_fire(cdft, classes=((48, 32), (64, 32), (96, 32)), dx=0.1, kT=0.8, tol=1e-8, ladder_tol=1e-5,

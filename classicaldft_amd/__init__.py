@@ -6,7 +6,7 @@ no CPU fallback.
 """
 from .cdft import (  # noqa: F401
     CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT, FP64, MIXED, MODEL_NONE, ROSENFELD, RSLT, ESFMT,
-    WHITEBEAR_I, WHITEBEAR_II, DEVICE_PTRS, ACCUMULATE, FMT_ONLY, CdftError, EtaTooLarge, System, DeviceArray, Fire,
-    DDFT, FireBatch, fft_r2c, fft_c2r, lj_hsd,
+    WHITEBEAR_I, WHITEBEAR_II, DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, CdftError, EtaTooLarge, System, DeviceArray, Fire,
+    DDFT, FireBatch, UNet, fft_r2c, fft_c2r, lj_hsd,
     device_count, library_path,
 )

@@ -40,7 +40,8 @@ def build_cdft():
     out = os.path.join(HERE, "libcdft.so")
     tmp = out + ".tmp"
     run([hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp,
-         os.path.join(CSRC, "cdft_api.hip"), os.path.join(CSRC, "solvers.hip"), os.path.join(CSRC, "batch.hip")])
+         os.path.join(CSRC, "cdft_api.hip"), os.path.join(CSRC, "solvers.hip"), os.path.join(CSRC, "batch.hip"),
+         os.path.join(CSRC, "unet.hip")])
     os.replace(tmp, out)
     return out
 

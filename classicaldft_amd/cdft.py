@@ -10,7 +10,7 @@ CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT = 0, 1, 2
 CDFT_BAD_ARG, CDFT_HIP_ERROR, CDFT_NOT_SUPPORTED = -1, -2, -3
 MODEL_NONE, ROSENFELD, RSLT, ESFMT, WHITEBEAR_I, WHITEBEAR_II = -1, 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
-DEVICE_PTRS, ACCUMULATE, FMT_ONLY = 1, 2, 4
+DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL = 1, 2, 4, 8
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -96,6 +96,17 @@ def lib():
         L.cdft_fire_batch_system.argtypes = [_p, ctypes.POINTER(_p)]
         L.cdft_fire_batch_destroy.argtypes = [_p]
         L.cdft_fire_batch_destroy.restype = None
+        L.cdft_system_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_fmt_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_unet_create.argtypes = [_i, ctypes.c_ulonglong, ctypes.POINTER(_p)]
+        L.cdft_unet_destroy.argtypes = [_p]
+        L.cdft_unet_destroy.restype = None
+        L.cdft_unet_num_params.argtypes = [_p, ctypes.POINTER(ctypes.c_longlong)]
+        L.cdft_unet_get_params.argtypes = [_p, _p]
+        L.cdft_unet_set_params.argtypes = [_p, _p]
+        L.cdft_unet_forward.argtypes = [_p, _p, _p, _p, _i, _p]
+        L.cdft_unet_grad.argtypes = [_p, _p, _p, _p, _p, _i, ctypes.POINTER(ctypes.c_float), _p]
+        L.cdft_unet_train_step.argtypes = [_p, _p, _p, _p, _p, _i, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
         L.cdft_fft_r2c.argtypes = [_i, _i, _i, _i, _p, _p]
         L.cdft_fft_c2r.argtypes = [_i, _i, _i, _i, _p, _p]
         L.cdft_system_eval_async.argtypes = [_p, _p, _p, _i]
@@ -217,6 +228,27 @@ class System:
         F = np.zeros(self.nb)
         _check(lib().cdft_fmt_eval(self.h, _ptr(rho), _ptr(out), _ptr(F), ACCUMULATE if accumulate else 0))
         return (F[0] if self.nb == 1 else F), out
+
+    def hessian_dot(self, rho, v, ideal=True, out=None, accumulate=False):
+        """DFT::matrix_dot_v_intern: (d2 beta Omega / drho drho) v (ideal = full_hessian_)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if rho.size != self.nb * self.N or v.size != rho.size:
+            raise ValueError(f"expected {self.nb * self.N} densities and directions")
+        res = np.zeros(self._ashape()) if out is None else out
+        fl = (0 if ideal else NO_IDEAL) | (ACCUMULATE if accumulate else 0)
+        _check(lib().cdft_system_hessian_dot(self.h, _ptr(rho), _ptr(v), _ptr(res), fl))
+        return res
+
+    def fmt_hessian_dot(self, rho, v, out=None, accumulate=False):
+        """FMT::add_second_derivative: the hard-sphere part of the Hessian applied to v."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if rho.size != self.nb * self.N or v.size != rho.size:
+            raise ValueError(f"expected {self.nb * self.N} densities and directions")
+        res = np.zeros(self._ashape()) if out is None else out
+        _check(lib().cdft_fmt_hessian_dot(self.h, _ptr(rho), _ptr(v), _ptr(res), ACCUMULATE if accumulate else 0))
+        return res
 
     def autotune(self, reps=3):
         """cdft_system_autotune: picks the fastest kernel tiles for this shape; returns ms per evaluation."""
@@ -451,29 +483,31 @@ class FireBatch:
 
 
 class DeviceArray:
-    """A float64 device buffer allocated through the C ABI (no second HIP runtime)."""
+    """A device buffer (float64 by default) allocated through the C ABI (no second HIP runtime)."""
 
-    def __init__(self, shape):
+    def __init__(self, shape, dtype=np.float64):
         self.shape = tuple(shape)
-        self.nbytes = int(np.prod(self.shape)) * 8
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         p = _p()
-        _check(lib().cdft_malloc(ctypes.byref(p), self.nbytes))
+        _check(lib().cdft_malloc(ctypes.byref(p), max(1, self.nbytes)))
         self.ptr = p.value
 
     @classmethod
-    def from_host(cls, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        d = cls(a.shape)
+    def from_host(cls, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.shape, dtype)
         d.upload(a)
         return d
 
-    def upload(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        assert a.nbytes == self.nbytes
-        _check(lib().cdft_memcpy_h2d(_p(self.ptr), _ptr(a), self.nbytes))
+    def upload(self, a, offset=0):
+        """Copy host array a into the buffer, `offset` elements in."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert offset * self.dtype.itemsize + a.nbytes <= self.nbytes
+        _check(lib().cdft_memcpy_h2d(_p(self.ptr + offset * self.dtype.itemsize), _ptr(a), a.nbytes))
 
     def download(self):
-        out = np.empty(self.shape)
+        out = np.empty(self.shape, dtype=self.dtype)
         _check(lib().cdft_memcpy_d2h(_ptr(out), _p(self.ptr), self.nbytes))
         return out
 
@@ -485,6 +519,76 @@ class DeviceArray:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class UNet:
+    """The warm-start network (cdft_unet_*): (beta V_ext, mu) on D^3 grids -> log rho."""
+
+    def __init__(self, D, seed=1):
+        h = _p()
+        _check(lib().cdft_unet_create(int(D), ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        self.h = h
+        self.D = int(D)
+        n = ctypes.c_longlong()
+        _check(lib().cdft_unet_num_params(h, ctypes.byref(n)))
+        self.num_params = n.value
+
+    def params(self):
+        out = np.empty(self.num_params, dtype=np.float32)
+        _check(lib().cdft_unet_get_params(self.h, _ptr(out)))
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.num_params
+        _check(lib().cdft_unet_set_params(self.h, _ptr(w)))
+
+    def _idx(self, idx, *arrays):
+        """Sample numbers as int32, checked against the sample count of every data array."""
+        i = np.ascontiguousarray(idx, dtype=np.int32).reshape(-1)
+        if i.size == 0:
+            raise ValueError("empty batch")
+        for a in arrays:
+            n = a.shape[0]
+            if a.dtype != np.float32 or i.min() < 0 or i.max() >= n:
+                raise ValueError("sample index out of range or data not float32")
+        for a in arrays[:1] + arrays[2:]:
+            if a.shape[1:] != (self.D,) * 3:
+                raise ValueError("grids must be D^3")
+        return i
+
+    def forward(self, V, mu, idx, out):
+        """V, mu, out: DeviceArray (float32); idx: sample numbers of the batch."""
+        i = self._idx(idx, V, mu)
+        if out.shape != (i.size,) + (self.D,) * 3 or out.dtype != np.float32:
+            raise ValueError("out must be float32 (nb, D, D, D)")
+        _check(lib().cdft_unet_forward(self.h, _p(V.ptr), _p(mu.ptr), _ptr(i), i.size, _p(out.ptr)))
+
+    def grad(self, V, mu, T, idx):
+        i = self._idx(idx, V, mu, T)
+        loss = ctypes.c_float()
+        g = np.empty(self.num_params, dtype=np.float32)
+        _check(lib().cdft_unet_grad(self.h, _p(V.ptr), _p(mu.ptr), _p(T.ptr), _ptr(i), i.size, ctypes.byref(loss),
+                                    _ptr(g)))
+        return loss.value, g
+
+    def train_step(self, V, mu, T, idx, lr):
+        i = self._idx(idx, V, mu, T)
+        loss = ctypes.c_float()
+        _check(lib().cdft_unet_train_step(self.h, _p(V.ptr), _p(mu.ptr), _p(T.ptr), _ptr(i), i.size, float(lr),
+                                          ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

@@ -314,7 +314,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
       s.mark("z_c2r_mf");
     }
   }
-  FinalParams fp;
+  FinalParams fp{};
   fp.dV = s.d.dx * s.d.dy * s.d.dz;
   fp.mu = with_dft ? s.d.mu : 0.0;
   fp.ideal = with_dft ? s.d.ideal : 0;
@@ -365,6 +365,115 @@ double csum(std::initializer_list<double> v) {
 }
 
 // Enqueue one evaluation on device pointers (no synchronisation).
+// Forward half of the convolution pipeline on FP64 device data: in (nb members of N reals) -> A (spectra) ->
+// B (nconv weighted-density arrays, x and y inverse-transformed, z still spectral).  Register-FFT lengths only.
+void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<double>* B, int nconv) {
+  const Tables<double>& tw = tables<double>(s);
+  const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
+  launch_rows_r2c<double, double>(s, in, s.N, A, sA, s.nb);
+  launch_cols<double>(s, A, 0, 1, 1, true, s.nb, sA);
+  const ColGeo cg = col_geo<double>(s.g, 0);
+  with_reg_len(s.g.Nx, [&](auto lc) {
+    constexpr int Lc = decltype(lc)::value;
+    with_nb<double, Lc>(s.nb_half_fwd, [&](auto nbc) {
+      constexpr int NB = decltype(nbc)::value;
+      const Batch bt{sA, sB, s.N, 0, nullptr};
+      hipLaunchKernelGGL((k_conv_fwd2<double, Lc, NB>), dim3(div_up(cg.ncols, NB), s.nb), dim3(NB * F2<Lc>::P), 0,
+                         s.stream, A, B, Nkp, s.W.as<double>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<double>>(),
+                         bt);
+      HIPCHK(hipGetLastError());
+    });
+  });
+  launch_cols<double>(s, B, Nkp, nconv, 1, false, s.nb, sB);
+}
+
+template <int MODEL>
+void launch_hess(cdft_system_s& s, bool mf) {
+  const Tables<double>& tw = tables<double>(s);
+  const int M = s.g.Nz / 2;
+  PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
+  if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
+  bool done = false;
+  with_reg_len(M, [&](auto mc) {
+    constexpr int Mc = decltype(mc)::value;
+    using RF = RowF<double, Mc>;
+    const int nseq = 2 * s.m + (mf ? 1 : 0);
+    const int per = nseq * RF::SLOT * int(sizeof(cplx<double>));
+    const int RB = std::min(512 / (nseq * RF::P), (65536 - 4096) / per);
+    if (RB < 1) return;
+    const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, 0, nullptr};
+    hipLaunchKernelGGL((k_z_hess2<MODEL, Mc>), dim3(div_up(s.nrows, RB), s.nb), dim3(nseq * RB * RF::P),
+                       size_t(RB) * per, s.stream, s.B.as<cplx<double>>(), s.B2.as<cplx<double>>(), s.Nkp, s.m,
+                       mf ? 1 : 0, s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
+                       tw.n.template as<cplx<double>>(), RB, pp, s.flag.as<int>(), bt);
+    HIPCHK(hipGetLastError());
+    done = true;
+  });
+  REQUIRE(done, "grid too long in z for the Hessian-vector product (needs 2m+1 rows of Nz in 64 KB of LDS)");
+}
+
+// out = [out] + (d2 beta Omega / drho drho) v on FP64 device data (with_dft = false: FMT part only).
+void hess_device(cdft_system_s& s, const double* rho, const double* v, double* out, bool with_dft, bool ideal,
+                 bool accumulate) {
+  REQUIRE(!s.mixed, "the Hessian-vector product needs an FP64 system");
+  REQUIRE(rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) && reg_fft_ok<double>(s, s.g.Ny),
+          "the Hessian-vector product needs register-FFT transform lengths");
+  const bool mf = with_dft && s.d.mean_field;
+  const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
+  const size_t cs = sizeof(cplx<double>);
+  if (s.A2.bytes < cs * Nkp * s.nb) s.A2.alloc(cs * Nkp * s.nb);
+  if (s.B2.bytes < cs * Nkp * s.nconv * s.nb) s.B2.alloc(cs * Nkp * s.nconv * s.nb);
+  cplx<double>* A = s.A.as<cplx<double>>();
+  cplx<double>* B = s.B.as<cplx<double>>();
+  const Tables<double>& tw = tables<double>(s);
+  if (s.m > 0) forward_weighted(s, rho, A, B, s.m);
+  if (s.m > 0 || mf) forward_weighted(s, v, s.A2.as<cplx<double>>(), s.B2.as<cplx<double>>(), s.m + (mf ? 1 : 0));
+  if (s.m > 0) {
+    switch (s.d.model) {
+      case CDFT_ROSENFELD: launch_hess<M_ROSENFELD>(s, mf); break;
+      case CDFT_RSLT: launch_hess<M_RSLT>(s, mf); break;
+      case CDFT_ESFMT: launch_hess<M_ESFMT>(s, mf); break;
+      case CDFT_WHITEBEAR_I: launch_hess<M_WBI>(s, mf); break;
+      default: launch_hess<M_WBII>(s, mf); break;
+    }
+    launch_cols<double>(s, B, Nkp, s.m, 1, true, s.nb, sB);
+    const ColGeo cg = col_geo<double>(s.g, 0);
+    with_reg_len(s.g.Nx, [&](auto lc) {
+      constexpr int Lc = decltype(lc)::value;
+      with_nb<double, Lc>(s.nb_half_bwd, [&](auto nbc) {
+        constexpr int NB = decltype(nbc)::value;
+        const Batch bt{sA, sB, s.N, 0, nullptr};
+        hipLaunchKernelGGL((k_conv_bwd2<double, Lc, NB>), dim3(div_up(cg.ncols, NB), s.nb), dim3(NB * F2<Lc>::P), 0,
+                           s.stream, B, A, Nkp, s.W.as<double>(), s.m, s.odd_mask, cg,
+                           tw.x.template as<cplx<double>>(), bt);
+        HIPCHK(hipGetLastError());
+      });
+    });
+    launch_cols<double>(s, A, 0, 1, 1, false, s.nb, sA);
+  } else if (mf) {
+    launch_rows_c2r<double, double>(s, s.B2.as<cplx<double>>(), sB, s.vmf.as<double>(), s.N, s.nb, 1.0);
+  }
+  FinalParams fp{};
+  fp.dV = s.d.dx * s.d.dy * s.d.dz;
+  fp.ideal = with_dft && ideal && s.d.ideal;
+  fp.fmt = s.m > 0;
+  fp.mf = mf;
+  fp.accumulate = accumulate;
+  fp.hess = 1;
+  const int M = s.g.Nz / 2;
+  with_reg_len(M, [&](auto mc) {
+    constexpr int Mc = decltype(mc)::value;
+    constexpr int RB = row_rb<double, Mc>(32768), P = RowF<double, Mc>::P, SLOT = RowF<double, Mc>::SLOT;
+    const long nblk = div_up(s.nrows, RB);
+    const Batch bt{sA, sB, s.N, 4 * s.pslots, nullptr};
+    hipLaunchKernelGGL((k_z_final2<double, Mc>), dim3(nblk, s.nb), dim3(RB * P), RB * SLOT * sizeof(cplx<double>),
+                       s.stream, A, rho, v, s.vmf.as<double>(), out, unsigned(s.N * 8), s.nrows, s.g.Zp,
+                       tw.m.template as<cplx<double>>(), tw.n.template as<cplx<double>>(), RB, fp,
+                       s.partials2.as<double2>(), bt);
+  });
+  HIPCHK(hipGetLastError());
+}
+
 void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc) {
   if (s->mixed) eval_device<float>(*s, rho_d, dF_d, with_dft, acc);
   else eval_device<double>(*s, rho_d, dF_d, with_dft, acc);
@@ -559,6 +668,44 @@ int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, 
     if (rc == CDFT_OK && F_hs)
       for (int b = 0; b < sys->nb; ++b) F_hs[b] = parts[5 * b + 2];
     return rc;
+  });
+}
+
+namespace {
+int run_hess(cdft_system s, const double* rho, const double* v, double* out, int flags, bool with_dft) {
+  REQUIRE(s && rho && v && out, "null argument");
+  const bool dev = flags & CDFT_DEVICE_PTRS;
+  const bool acc = flags & CDFT_ACCUMULATE;
+  const size_t bytes = sizeof(double) * s->N * s->nb;
+  const double *rho_d = rho, *v_d = v;
+  double* out_d = out;
+  if (!dev) {
+    if (s->vdir.bytes < bytes) s->vdir.alloc(bytes);
+    HIPCHK(hipMemcpyAsync(s->rho.p, rho, bytes, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->vdir.p, v, bytes, hipMemcpyHostToDevice, s->stream));
+    if (acc) HIPCHK(hipMemcpyAsync(s->dF.p, out, bytes, hipMemcpyHostToDevice, s->stream));
+    rho_d = s->rho.as<double>();
+    v_d = s->vdir.as<double>();
+    out_d = s->dF.as<double>();
+  }
+  hess_device(*s, rho_d, v_d, out_d, with_dft, !(flags & CDFT_NO_IDEAL), acc);
+  if (!dev) HIPCHK(hipMemcpyAsync(out, s->dF.p, bytes, hipMemcpyDeviceToHost, s->stream));
+  std::vector<double> parts(5 * (size_t)s->nb);
+  const bool last = s->last_with_dft;
+  const int rc = fetch_parts(s, parts.data());  // waits; reports eta > 1
+  s->last_with_dft = last;
+  return rc;
+}
+}  // namespace
+
+int cdft_system_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags) {
+  return guarded([&] { return run_hess(sys, rho, v, out, flags, true); });
+}
+
+int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags) {
+  return guarded([&] {
+    REQUIRE(sys && sys->m > 0, "system has no FMT functional");
+    return run_hess(sys, rho, v, out, flags, false);
   });
 }
 

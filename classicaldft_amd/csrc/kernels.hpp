@@ -514,6 +514,7 @@ __global__ __launch_bounds__(kThreads) void k_z_pointwise(cplx<T>* __restrict__ 
 struct FinalParams {
   double dV, mu;
   int ideal, fmt, mf, ext, accumulate;
+  int hess;  // Hessian-vector mode of k_z_final2 (the V_ext pointer carries v)
 };
 
 template <typename T, int MAXPTS>

@@ -28,15 +28,44 @@ inline PhiParams make_phi_params(double hsd, double A, double B) {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Forward-mode dual number v + d e (e^2 = 0): evaluating the derivative code on
+// n + psi e gives, in the e part, the second derivative of Phi applied to psi
+// (the Hessian-vector product of FMT::calculate_d2Phi_dot_V, src/FMT.cpp:181-280).
+struct Dual {
+  double v, d;
+  __host__ __device__ constexpr Dual(double a = 0.0, double b = 0.0) : v(a), d(b) {}
+};
+__host__ __device__ inline Dual operator+(const Dual& a, const Dual& b) { return Dual(a.v + b.v, a.d + b.d); }
+__host__ __device__ inline Dual operator+(const Dual& a, double b) { return Dual(a.v + b, a.d); }
+__host__ __device__ inline Dual operator+(double a, const Dual& b) { return Dual(a + b.v, b.d); }
+__host__ __device__ inline Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
+__host__ __device__ inline Dual operator-(const Dual& a, const Dual& b) { return Dual(a.v - b.v, a.d - b.d); }
+__host__ __device__ inline Dual operator-(const Dual& a, double b) { return Dual(a.v - b, a.d); }
+__host__ __device__ inline Dual operator-(double a, const Dual& b) { return Dual(a - b.v, -b.d); }
+__host__ __device__ inline Dual operator*(const Dual& a, const Dual& b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__host__ __device__ inline Dual operator*(const Dual& a, double b) { return Dual(a.v * b, a.d * b); }
+__host__ __device__ inline Dual operator*(double a, const Dual& b) { return Dual(a * b.v, a * b.d); }
+__host__ __device__ inline Dual operator/(const Dual& a, double b) { return Dual(a.v / b, a.d / b); }
+__host__ __device__ inline Dual operator/(double a, const Dual& b) {
+  const double q = a / b.v;
+  return Dual(q, -q * b.d / b.v);
+}
+__host__ __device__ inline Dual operator/(const Dual& a, const Dual& b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+__host__ __device__ inline bool operator<(const Dual& a, double b) { return a.v < b; }
+__host__ __device__ inline double flog(double x) { return log(x); }
+__host__ __device__ inline Dual flog(const Dual& x) { return Dual(log(x.v), x.d / x.v); }
+
 // eta-dependent cofactors f1, f2, f3 and first derivatives.  Written with the
 // two reciprocals 1/(1-x) and 1/x instead of one division per term (a double
 // division is ~10 instructions on the GPU); same formulas as the reference.
-template <int MODEL>
-__host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double& f2, double& f2p, double& f3,
-                                      double& f3p) {
-  const double om = 1.0 - x;
-  const double f = 1.0 / om;
-  f1 = log(om);  // FMT.h:153-156
+template <int MODEL, typename S = double>
+__host__ __device__ inline void fcoef(S x, S& f1, S& f1p, S& f2, S& f2p, S& f3, S& f3p) {
+  const S om = 1.0 - x;
+  const S f = 1.0 / om;
+  f1 = flog(om);  // FMT.h:153-156
   f1p = -f;
   if constexpr (MODEL == M_WBII) {  // FMT.h:588-632
     if (x < 1e-12) {
@@ -45,7 +74,7 @@ __host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double&
       f3 = 1.5 + (7.0 / 3) * x + 3.25 * x * x + 4.2 * x * x * x;
       f3p = (7.0 / 3) + 7.5 * x + 12.6 * x * x;
     } else {
-      const double L = f1, ix = 1.0 / x, ix2 = ix * ix, f2_ = f * f;
+      const S L = f1, ix = 1.0 / x, ix2 = ix * ix, f2_ = f * f;
       f2 = (1.0 / 3.0) + (4.0 / 3.0) * f + (2.0 / 3.0) * ix * L;
       f2p = (2.0 / 3.0) * (3 * x - 1) * ix * f2_ - (2.0 / 3.0) * ix2 * L;
       f3 = -(1.0 - 3 * x + x * x) * ix * f2_ - ix2 * L;
@@ -59,7 +88,7 @@ __host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double&
         f3 = 1.5 + (8.0 / 3) * x + 3.75 * x * x + 4.8 * x * x * x;
         f3p = (8.0 / 3) + 7.5 * x + 14.4 * x * x;
       } else {
-        const double ix = 1.0 / x, ix2 = ix * ix, f2_ = f * f;
+        const S ix = 1.0 / x, ix2 = ix * ix, f2_ = f * f;
         f3 = ix * f2_ + f1 * ix2;
         f3p = (-2 + 5 * x - x * x) * ix2 * f2_ * f - 2 * ix2 * ix * f1;
       }
@@ -73,22 +102,24 @@ __host__ __device__ inline void fcoef(double x, double& f1, double& f1p, double&
 // Returns Phi at one site; writes the 11 collapsed derivatives to out[].
 // n[] holds the 11 weighted densities of the site (tensor entries ignored for
 // Rosenfeld / RSLT, whose tensor arrays are never convolved).
-template <int MODEL>
-__host__ __device__ inline double phi_site(const double* n, const PhiParams& pp, double* out) {
+// S = double: Phi and its collapsed first derivatives.  S = Dual with n = n_rho + psi e: the e parts of out[]
+// are the collapsed second derivatives applied to psi (same collapse and parity sign).
+template <int MODEL, typename S = double>
+__host__ __device__ inline S phi_site(const S* n, const PhiParams& pp, S* out) {
   constexpr bool TENSOR = MODEL >= M_ESFMT;
   const double id = pp.inv_d, id2 = pp.inv_d2;
-  const double eta = n[0];
-  const double s0 = n[1] * id2, s1 = n[1] * id, s2 = n[1];
-  const double v2[3] = {n[2], n[3], n[4]};
-  const double v1[3] = {n[2] * id, n[3] * id, n[4] * id};
-  const double v1v2 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
-  const double v2v2 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
+  const S eta = n[0];
+  const S s0 = n[1] * id2, s1 = n[1] * id, s2 = n[1];
+  const S v2[3] = {n[2], n[3], n[4]};
+  const S v1[3] = {n[2] * id, n[3] * id, n[4] * id};
+  const S v1v2 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+  const S v2v2 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
 
-  double f1, f1p, f2, f2p, f3, f3p;
-  fcoef<MODEL>(eta, f1, f1p, f2, f2p, f3, f3p);
+  S f1, f1p, f2, f2p, f3, f3p;
+  fcoef<MODEL, S>(eta, f1, f1p, f2, f2p, f3, f3p);
 
-  double P3, dS2, dV2[3];
-  double dT[6] = {0, 0, 0, 0, 0, 0};  // xx, xy, xz, yy, yz, zz
+  S P3, dS2, dV2[3];
+  S dT[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // xx, xy, xz, yy, yz, zz
   if constexpr (MODEL == M_ROSENFELD) {  // FMT.h:202-240
     P3 = (1.0 / (24 * kPi)) * (s2 * s2 * s2 - 3 * s2 * v2v2);
     dS2 = (1.0 / (24 * kPi)) * (3 * s2 * s2 - 3 * v2v2);
@@ -99,28 +130,28 @@ __host__ __device__ inline double phi_site(const double* n, const PhiParams& pp,
       dS2 = 0.0;
       dV2[0] = dV2[1] = dV2[2] = 0.0;
     } else {
-      const double psi = v2v2 / (s2 * s2);
-      const double q = 1 - psi;
+      const S psi = v2v2 / (s2 * s2);
+      const S q = 1.0 - psi;
       P3 = (1.0 / (36 * kPi)) * s2 * s2 * s2 * q * q * q;
       dS2 = (1.0 / (12 * kPi)) * s2 * s2 * (1 + psi) * q * q;
       for (int k = 0; k < 3; ++k) dV2[k] = -(1.0 / (6 * kPi)) * s2 * v2[k] * q * q;
     }
   } else {  // esFMT family: FMT.h:396-455
     const double A = pp.A, B = pp.B;
-    const double Txx = n[5], Txy = n[6], Txz = n[7], Tyy = n[8], Tyz = n[9], Tzz = n[10];
+    const S Txx = n[5], Txy = n[6], Txz = n[7], Tyy = n[8], Tyz = n[9], Tzz = n[10];
     // TT = T.T (symmetric)
-    const double Qxx = Txx * Txx + Txy * Txy + Txz * Txz;
-    const double Qxy = Txx * Txy + Txy * Tyy + Txz * Tyz;
-    const double Qxz = Txx * Txz + Txy * Tyz + Txz * Tzz;
-    const double Qyy = Txy * Txy + Tyy * Tyy + Tyz * Tyz;
-    const double Qyz = Txy * Txz + Tyy * Tyz + Tyz * Tzz;
-    const double Qzz = Txz * Txz + Tyz * Tyz + Tzz * Tzz;
-    const double vT0 = v2[0] * Txx + v2[1] * Txy + v2[2] * Txz;
-    const double vT1 = v2[0] * Txy + v2[1] * Tyy + v2[2] * Tyz;
-    const double vT2 = v2[0] * Txz + v2[1] * Tyz + v2[2] * Tzz;
-    const double vTv = v2[0] * vT0 + v2[1] * vT1 + v2[2] * vT2;
-    const double T2 = Qxx + Qyy + Qzz;
-    const double T3 = Qxx * Txx + Qyy * Tyy + Qzz * Tzz + 2 * (Qxy * Txy + Qxz * Txz + Qyz * Tyz);
+    const S Qxx = Txx * Txx + Txy * Txy + Txz * Txz;
+    const S Qxy = Txx * Txy + Txy * Tyy + Txz * Tyz;
+    const S Qxz = Txx * Txz + Txy * Tyz + Txz * Tzz;
+    const S Qyy = Txy * Txy + Tyy * Tyy + Tyz * Tyz;
+    const S Qyz = Txy * Txz + Tyy * Tyz + Tyz * Tzz;
+    const S Qzz = Txz * Txz + Tyz * Tyz + Tzz * Tzz;
+    const S vT0 = v2[0] * Txx + v2[1] * Txy + v2[2] * Txz;
+    const S vT1 = v2[0] * Txy + v2[1] * Tyy + v2[2] * Tyz;
+    const S vT2 = v2[0] * Txz + v2[1] * Tyz + v2[2] * Tzz;
+    const S vTv = v2[0] * vT0 + v2[1] * vT1 + v2[2] * vT2;
+    const S T2 = Qxx + Qyy + Qzz;
+    const S T3 = Qxx * Txx + Qyy * Tyy + Qzz * Tzz + 2 * (Qxy * Txy + Qxz * Txz + Qyz * Tyz);
     const double ca = A / (24 * kPi), cb = B / (24 * kPi);
     P3 = ca * (s2 * s2 * s2 - 3 * s2 * v2v2 + 3 * vTv - T3) + cb * (s2 * s2 * s2 - 3 * s2 * T2 + 2 * T3);
     dS2 = ca * (3 * s2 * s2 - 3 * v2v2) + cb * (3 * s2 * s2 - 3 * T2);
@@ -137,18 +168,18 @@ __host__ __device__ inline double phi_site(const double* n, const PhiParams& pp,
   }
 
   const double c1 = 1.0 / kPi, c2 = 1.0 / (2 * kPi);
-  const double phi = -c1 * s0 * f1 + c2 * (s1 * s2 - v1v2) * f2 + P3 * f3;
+  const S phi = -c1 * s0 * f1 + c2 * (s1 * s2 - v1v2) * f2 + P3 * f3;
 
-  const double dEta = -c1 * s0 * f1p + c2 * (s1 * s2 - v1v2) * f2p + P3 * f3p;
-  const double dS0 = -c1 * f1;
-  const double dS1 = c2 * s2 * f2;
-  const double dS2t = c2 * s1 * f2 + dS2 * f3;
+  const S dEta = -c1 * s0 * f1p + c2 * (s1 * s2 - v1v2) * f2p + P3 * f3p;
+  const S dS0 = -c1 * f1;
+  const S dS1 = c2 * s2 * f2;
+  const S dS2t = c2 * s1 * f2 + dS2 * f3;
   out[0] = dEta;
   out[1] = dS0 * id2 + dS1 * id + dS2t;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double dv1 = -c2 * v2[k] * f2;
-    const double dv2 = -c2 * v1[k] * f2 + dV2[k] * f3;
+    const S dv1 = -c2 * v2[k] * f2;
+    const S dv2 = -c2 * v1[k] * f2 + dV2[k] * f3;
     out[2 + k] = -(dv2 + dv1 * id);  // parity sign, Species.h:324
   }
   if constexpr (TENSOR) {
@@ -160,7 +191,7 @@ __host__ __device__ inline double phi_site(const double* n, const PhiParams& pp,
     out[10] = dT[5] * f3;
   } else {
 #pragma unroll
-    for (int a = 5; a < 11; ++a) out[a] = 0.0;
+    for (int a = 5; a < 11; ++a) out[a] = S(0.0);
   }
   return phi;
 }

@@ -299,9 +299,84 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
   block_reduce_store_n<1>(v, partials, blockIdx.x);
 }
 
+// Pointwise z pass of the Hessian-vector product (FP64): sequences a < m are the
+// weighted densities n_a of rho (array a of B), m <= a < 2m the weighted
+// densities psi_a of the direction v (array a - m of B2), a = 2m (mf) the
+// mean-field convolution of v.  Per site: Lambda = e part of the collapsed dPhi
+// at n + psi e (phi_site on dual numbers), written over n_a and transformed back
+// into B; v_mf <- the mean-field row of v.  Reference: FMT::add_second_derivative
+// (src/FMT.cpp:437-567), whose back-convolution with conjugated weights equals
+// the parity-signed collapse used for dF.
+template <int MODEL, int M>
+__global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, const cplx<double>* __restrict__ B2,
+                                                 long arr_stride, int m, int mf, double* __restrict__ vmf, long nrows,
+                                                 int Zp, const cplx<double>* __restrict__ twM,
+                                                 const cplx<double>* __restrict__ twN, int RB, PhiParams pp,
+                                                 int* __restrict__ flag, Batch bt) {
+  B += blockIdx.y * bt.sb;
+  B2 += blockIdx.y * bt.sb;
+  if (vmf) vmf += blockIdx.y * bt.sr;
+  flag += blockIdx.y;
+  using RF = RowF<double, M>;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+  const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
+  const int a = seq / RB, r = seq % RB;
+  cplx<double>* slot = lds + seq * RF::SLOT;
+  const long row0 = (long)blockIdx.x * RB;
+  const long rrow = row0 + r < nrows ? row0 + r : nrows - 1;
+  const bool ok = row0 + r < nrows;
+  cplx<double>* brow = B + (a < m ? a : 0) * arr_stride + rrow * Zp;
+  const cplx<double>* src = a < m ? brow : B2 + (a - m) * arr_stride + rrow * Zp;
+  {
+    cplx<double> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+    c2r_to_regs<double, M>(src, twN, t, v1);
+    reg_fft_seq<double, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+      for (int k2 = 0; k2 < RF::R2; ++k2) slot[t + RF::P * q + RF::R1 * k2] = v2[q][k2];
+  }
+  __syncthreads();
+  int bad = 0;
+  for (int s = threadIdx.x; s < RB * 2 * M; s += blockDim.x) {
+    const int rr = s / (2 * M), z = s % (2 * M);
+    const long row = row0 + rr;
+    if (row >= nrows) continue;
+    Dual n[11];
+#pragma unroll
+    for (int b = 0; b < 11; ++b)
+      n[b] = b < m ? Dual(reinterpret_cast<const double*>(lds + (b * RB + rr) * RF::SLOT)[z],
+                          reinterpret_cast<const double*>(lds + ((m + b) * RB + rr) * RF::SLOT)[z])
+                   : Dual(0.0, 0.0);
+    Dual out[11];
+    phi_site<MODEL, Dual>(n, pp, out);
+    if (n[0].v > 0.5 && 1 - n[0].v < 0.0) bad = 1;
+#pragma unroll
+    for (int b = 0; b < 11; ++b)
+      if (b < m) reinterpret_cast<double*>(lds + (b * RB + rr) * RF::SLOT)[z] = out[b].d;
+    if (mf) vmf[row * 2 * M + z] = reinterpret_cast<const double*>(lds + (2 * m * RB + rr) * RF::SLOT)[z];
+  }
+  if (bad) atomicOr(flag, 1);
+  __syncthreads();
+  {
+    cplx<double> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+#pragma unroll
+    for (int q = 0; q < RF::Q1; ++q)
+#pragma unroll
+      for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = slot[RF::R2 * n1 + RF::P * q + t];
+    __syncthreads();
+    reg_fft_seq<double, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+    __syncthreads();
+    r2c_from_regs<double, M>(slot, t, twN, brow, ok && a < m, v2);
+  }
+}
+
 // Final z pass: dF = [dF_in] + dV ln rho + dV c2r(acc) + v_mf + dV V_ext - dV mu
 // (src/DFT.cpp:377-425), sites 2m, 2m+1 of each row straight from the
-// registers of the inverse transform.  partials per block:
+// registers of the inverse transform.  Hessian mode (fp.hess): out = [out_in] +
+// dV v / rho (ideal) + dV c2r(acc) + v_mf, with v passed in the V_ext slot.  partials per block:
 // {sum (rho ln rho - rho) dV, sum rho v_mf, sum V_ext rho, sum rho}.
 template <typename T, int M>
 __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Acc, const double* __restrict__ rho,
@@ -350,14 +425,16 @@ __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Ac
         const cplx<double> r2 = buf_ld<cplx<double>>(rr, off, 0);
         const cplx<double> f2 = fp.accumulate ? buf_ld<cplx<double>>(rf, off, 0) : cplx<double>{0.0, 0.0};
         const cplx<double> m2 = fp.mf ? buf_ld<cplx<double>>(rm, off, 0) : cplx<double>{0.0, 0.0};
-        const cplx<double> e2 = fp.ext ? buf_ld<cplx<double>>(re, off, 0) : cplx<double>{0.0, 0.0};
+        const cplx<double> e2 = fp.ext || fp.hess ? buf_ld<cplx<double>>(re, off, 0) : cplx<double>{0.0, 0.0};
         double out[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const double d0 = h ? r2.y : r2.x;
           const double vm = h ? m2.y : m2.x, ve = h ? e2.y : e2.x;
           double d = h ? f2.y : f2.x;
-          if (fp.ideal) {
+          if (fp.ideal && fp.hess) {
+            d += ve / d0 * fp.dV;  // ideal-gas Hessian: V_ext slot carries the direction v
+          } else if (fp.ideal) {
             const double l = log(d0);
             qs[0].add((d0 * l - d0) * fp.dV);
             d += l * fp.dV;

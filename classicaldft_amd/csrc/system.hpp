@@ -149,6 +149,7 @@ struct cdft_system_s {
   Tables<double> twd;
   Tables<float> twf;
   DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
+  DevBuf A2, B2, vdir;  // Hessian-vector product: spectra of the direction, its host copy (allocated on first use)
   double a_vdw = 0.0;
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
   // batch of nb same-shape members (weights shared; rho, dF, V_ext, v_mf, mu,
@@ -366,6 +367,8 @@ inline void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int 
 
 namespace cdft_impl {
 // Defined in cdft_api.hip.
+void hess_device(cdft_system_s& s, const double* rho, const double* v, double* out, bool with_dft, bool ideal,
+                 bool accumulate);
 void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc);
 // nb > 1: a batch of nb same-shape members with their own mus[nb] and V_ext
 // (nb x N or none); desc->mu / desc->vext are then ignored.

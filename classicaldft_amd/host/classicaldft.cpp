@@ -175,6 +175,18 @@ double FMT::calculateFreeEnergyAndDerivatives(std::vector<Species*>& allSpecies)
   return F;
 }
 
+void FMT::add_second_derivative(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& d2F,
+                                const std::vector<Species*>& allSpecies) {
+  if (allSpecies.empty()) throw std::runtime_error("No species for FMT::add_second_derivative to work with");
+  for (size_t i = 0; i < allSpecies.size(); ++i) {
+    auto* fs = dynamic_cast<FMT_Species*>(allSpecies[i]);
+    if (!fs) continue;
+    if (i > 0) throw std::runtime_error("FMT on the GPU is implemented for a single species");
+    check(cdft_fmt_hessian_dot(system_for(*fs), fs->getDensity().get_density_pointer(), v[i].memptr(),
+                               d2F[i].memptr(), CDFT_ACCUMULATE));
+  }
+}
+
 double FMT::BulkFex(const std::vector<double>& x, const std::vector<Species*>& allSpecies) const {
   for (size_t i = 0; i < allSpecies.size(); ++i)
     if (auto* fs = dynamic_cast<FMT_Species*>(allSpecies[i])) {
@@ -339,6 +351,32 @@ double DFT::calculateFreeEnergyAndDerivatives(bool onlyFex) {
   }
   s->endForceCalculation();
   return F;
+}
+
+void DFT::matrix_dot_v_intern(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& result, void*) const {
+  Species* s = species_.front();
+  const long N = s->getDensity().Ntot();
+  if (v.size() < 1 || v[0].size() != N) throw std::runtime_error("DFT::matrix_dot_v: one direction of Ntot values");
+  if (result.size() < 1) result.resize(1);
+  if (result[0].size() != N) result[0].resize(N);
+  check(cdft_system_hessian_dot(system(), s->getDensity().get_density_pointer(), v[0].memptr(), result[0].memptr(),
+                                full_hessian_ ? 0 : CDFT_NO_IDEAL));
+}
+
+void DFT::matrix_dot_v(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& result, void* param) const {
+  matrix_dot_v_intern(v, result, param);
+  if (use_squared_matrix_) {  // src/Dynamical_Matrix.cpp:38-49 (a DFT is not dynamic: no sign change)
+    std::vector<DFT_Vec> v1(1);
+    v1[0].set(result[0]);
+    matrix_dot_v_intern(v1, result, param);
+  }
+}
+
+void DFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param) const {
+  std::vector<DFT_Vec> vw(1), rw(1);
+  vw[0].set(v);
+  matrix_dot_v(vw, rw, param);
+  result.set(rw[0]);
 }
 
 double DFT::mu_times_beta(double density) const {

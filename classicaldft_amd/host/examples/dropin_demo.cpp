@@ -5,6 +5,8 @@
 //   mu <value>              DFT::mu_times_beta(0.5) with LJ mean field, kT = 1
 //   fire <F> <fmax_raw>     after 20 fireMinimizer2 steps
 //   ddft <F> <mass>         after 3 DDFT steps
+//   hess <sum Hv> <v.Hv>    DFT::matrix_dot_v1 of a seeded direction (16^3, ideal + WB I + LJ/QF)
+//   hess_fmt <sum> <v.Hv>   FMT::add_second_derivative of the same direction
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -60,5 +62,25 @@ int main() {
   for (int i = 0; i < 3; ++i) ddft.step();
   std::printf("ddft %.17g %.17g\n", ddft.getF(), ddft.get_mass());
   std::printf("hsd_lj_kT0.8 %.17g\n", lj.getHSD(0.8));
+
+  // Hessian-vector products (DFT::matrix_dot_v1 and FMT::add_second_derivative) on a 16^3 system
+  Density d4(0.1, L2);
+  for (long i = 0; i < d4.Ntot(); ++i) d4.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(41, i) - 1.0)));
+  FMT_Species s4(d4, 1.0, 0.0);
+  Interaction_Interpolation_QF qf4(&s4, &s4, &lj, 1.0);
+  WhiteBearI fmt4;
+  DFT dft4(&s4);
+  dft4.addHardCoreContribution(&fmt4);
+  dft4.addInteraction(&qf4);
+  DFT_Vec v(d4.Ntot()), Hv;
+  for (long i = 0; i < d4.Ntot(); ++i) v.set(i, 2.0 * splitmix_u(43, i) - 1.0);
+  dft4.matrix_dot_v1(v, Hv);
+  std::printf("hess %.17g %.17g\n", Hv.accu(), Hv.dotWith(v));
+  std::vector<DFT_Vec> vv(1), d2F(1);
+  vv[0].set(v);
+  d2F[0].zeros(d4.Ntot());
+  std::vector<Species*> sp{&s4};
+  fmt4.add_second_derivative(vv, d2F, sp);
+  std::printf("hess_fmt %.17g %.17g\n", d2F[0].accu(), d2F[0].dotWith(v));
   return 0;
 }

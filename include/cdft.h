@@ -51,7 +51,8 @@ enum cdft_precision {
 enum cdft_flags {
   CDFT_DEVICE_PTRS = 1, /* rho / dF / vext are device pointers */
   CDFT_ACCUMULATE = 2,  /* add into dF instead of overwriting it */
-  CDFT_FMT_ONLY = 4     /* hard-sphere part only (cdft_system_eval_async / _time) */
+  CDFT_FMT_ONLY = 4,    /* hard-sphere part only (cdft_system_eval_async / _time) */
+  CDFT_NO_IDEAL = 8     /* Hessian-vector product without the ideal-gas term (DFT::full_hessian_ false) */
 };
 
 typedef struct cdft_system_s* cdft_system;
@@ -91,6 +92,19 @@ int cdft_system_eval(cdft_system sys, const double* rho, double* dF, double* par
    the hard-sphere part; returns F_hs (already x dV) in *F_hs and adds (or
    writes) the FMT dF. */
 int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, int flags);
+
+/* Replaces DFT::matrix_dot_v_intern (src/DFT.cpp:435-488) for periodic systems:
+   out = (d2 beta Omega / drho_i drho_j) v_j = dV v/rho (ideal gas; left out with
+   CDFT_NO_IDEAL, i.e. full_hessian_ false) + the FMT part
+   (FMT::add_second_derivative, src/FMT.cpp:437-567) + the mean field
+   (Interaction_Base::add_second_derivative, src/Interaction.cpp:119-154).
+   FP64 systems; batches take nb members of rho, v and out.  Flags:
+   CDFT_DEVICE_PTRS, CDFT_ACCUMULATE, CDFT_NO_IDEAL.  CDFT_ETA_TOO_LARGE if eta > 1. */
+int cdft_system_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
+
+/* Replaces FMT::add_second_derivative (src/FMT.cpp:437-567): the hard-sphere
+   part only; add into out with CDFT_ACCUMULATE as the reference does. */
+int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
 
 /* Asynchronous form on device pointers: enqueues one evaluation on the
    system's stream and returns without synchronising.  cdft_system_fetch
@@ -232,6 +246,29 @@ int cdft_fire_batch_status(cdft_fire_batch b, cdft_fire_report* reports, int* st
 int cdft_fire_batch_get(cdft_fire_batch b, int member, double* rho);
 int cdft_fire_batch_system(cdft_fire_batch b, cdft_system* sys);
 void cdft_fire_batch_destroy(cdft_fire_batch b);
+
+/* ------------------------------------------------------- learned warm start ---
+   No counterpart in the reference.  A 3-level 3D U-Net on periodic D^3 grids
+   (D a multiple of 32): 3x3x3 convolutions with 16 / 32 / 64 channels, ReLU,
+   2x average pooling, nearest upsampling with skip concatenation, a 1x1 head;
+   input channels beta V_ext / 3 and (mu + 3) / 2, output log rho.  BF16
+   weights and activations, FP32 accumulation (MFMA), FP32 master weights and
+   Adam.  Data are device arrays: V (n_samples x D^3 floats), mu (n_samples),
+   targets log rho (n_samples x D^3); idx (host, nb ints) picks the batch.
+   out (device, nb x D^3 floats).  grad returns the loss (mean squared error of
+   log rho) and the FP32 gradient of all parameters (host, num_params floats)
+   without updating them; train_step also applies one Adam step. */
+typedef struct cdft_unet_s* cdft_unet;
+int cdft_unet_create(int D, unsigned long long seed, cdft_unet* out);
+void cdft_unet_destroy(cdft_unet u);
+int cdft_unet_num_params(cdft_unet u, long long* n);
+int cdft_unet_get_params(cdft_unet u, float* host);
+int cdft_unet_set_params(cdft_unet u, const float* host);
+int cdft_unet_forward(cdft_unet u, const float* V, const float* mu, const int* idx, int nb, float* out);
+int cdft_unet_grad(cdft_unet u, const float* V, const float* mu, const float* target, const int* idx, int nb,
+                   float* loss, float* grad);
+int cdft_unet_train_step(cdft_unet u, const float* V, const float* mu, const float* target, const int* idx, int nb,
+                         float lr, float* loss);
 
 #ifdef __cplusplus
 }

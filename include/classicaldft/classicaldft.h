@@ -161,6 +161,10 @@ class FMT {
  public:
   virtual ~FMT();
   double calculateFreeEnergyAndDerivatives(std::vector<Species*>& allSpecies);
+  // src/FMT.cpp:437-567: d2F[s] += (d2 F_hs / drho drho) v[s] (directions in real space; the reference takes
+  // their DFT_FFT with the spectrum already computed).
+  void add_second_derivative(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& d2F,
+                             const std::vector<Species*>& allSpecies);
   double BulkMuex(const std::vector<double>& x, const std::vector<Species*>& allSpecies, int species) const;
   double BulkFex(const std::vector<double>& x, const std::vector<Species*>& allSpecies) const;
   virtual std::string Name() const = 0;
@@ -285,6 +289,14 @@ class DFT {
   double getNumberAtoms(int s) const { return species_[s]->getDensity().getNumberAtoms(); }
   DFT_Vec& getDF(int s) { return species_[s]->getDF(); }
   double calculateFreeEnergyAndDerivatives(bool onlyFex);
+  // Dynamical_Matrix interface (include/Dynamical_Matrix.h:22-25, src/Dynamical_Matrix.cpp:34-70,
+  // src/DFT.cpp:435-488): result = (d2 beta Omega / drho drho) v, the ideal-gas term only with
+  // full_hessian_; with use_squared_matrix_ the matrix is applied twice.  Periodic systems.
+  void matrix_dot_v_intern(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& result, void* param = nullptr) const;
+  void matrix_dot_v(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& result, void* param = nullptr) const;
+  void matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param = nullptr) const;
+  void set_full_hessian(bool full) { full_hessian_ = full; }
+  void set_use_squared_matrix_(bool v) { use_squared_matrix_ = v; }
   double mu_times_beta(double density) const;
   double fhelmholtz_times_beta_over_volume(double density) const;
   double omega_times_beta_over_volume(double density) const;
@@ -305,6 +317,7 @@ class DFT {
   mutable double sys_mu_ = NAN;
   mutable unsigned sys_field_ = ~0u;
   double F_id_ = 0, F_ext_ = 0, F_hs_ = 0, F_mf_ = 0;
+  bool full_hessian_ = true, use_squared_matrix_ = false;
   void invalidate() const;
 };
 

@@ -60,6 +60,8 @@ def lib():
         L.orc_dft_avdw.restype = _d
         L.orc_dft_eval.argtypes = [_p, _p, _p, _p]
         L.orc_dft_eval.restype = _i
+        L.orc_dft_hess.argtypes = [_p, _p, _p, _p, _i]
+        L.orc_dft_hess.restype = None
         L.orc_fire_create.argtypes = [_p, _p, _d, _d]
         L.orc_fire_create.restype = _p
         L.orc_fire_destroy.argtypes = [_p]
@@ -224,6 +226,14 @@ class DFT:
         if rc == 1:
             raise EtaTooLarge("eta too large")
         return parts, dF
+
+    def hess(self, rho, v, full=True):
+        """(d2F / drho drho) v: DFT::matrix_dot_v_intern (full = full_hessian_, the ideal-gas term)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros_like(rho)
+        lib().orc_dft_hess(self.h, _ptr(rho), _ptr(v), _ptr(out), int(bool(full)))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

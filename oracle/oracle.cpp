@@ -586,6 +586,129 @@ struct Model {
     if (!tensor()) return 0.0;
     return (A / (8 * M_PI)) * (m.v2[j] * m.v2[k] - m.TT[k][j]) + (B / (4 * M_PI)) * (-m.s2 * m.T[k][j] + m.TT[k][j]);
   }
+  // ---- second derivatives (FMT::calculate_d2Phi_dot_V and the model hooks) ----
+  double f1pp(double e) const { return -1.0 / (1.0 - 2 * e + e * e); }  // FMT.h:163-166
+  double f2pp(double x) const {
+    if (kind == WBII) {  // FMT.h:604-609
+      if (x < 1e-12) return (20.0 / 9) + 7 * x;
+      return (2 * (7 * x * x - 5 * x + 2) / (3 * x * x * (1 - x) * (1 - x) * (1 - x))) + (4.0 / (3.0 * x * x * x)) * std::log(1 - x);
+    }
+    const double f = 1.0 / (1.0 - x);  // FMT.h:179-183
+    return 2 * f * f * f;
+  }
+  double f3pp(double x) const {
+    if (kind == RSLT) {  // FMT.h:284-290
+      if (x < 1e-12) return 7.5 + 28.8 * x;
+      return -(1.0 / (x * x * x)) * (1.0 / ((1.0 - x) * (1.0 - x) * (1.0 - x) * (1 - x))) *
+                 (5 * x * x * x - 26 * x * x + 21 * x - 6) + (6.0 / (x * x * x * x)) * std::log(1 - x);
+    }
+    if (kind == WBI) {  // FMT.h:551-557
+      if (x < 1e-12) return 7.5 + 28.8 * x;
+      return ((6 - 21 * x + 26 * x * x - 5 * x * x * x) / (x * x * x * (1 - x) * (1 - x) * (1 - x) * (1 - x))) +
+             (6 * std::log(1 - x)) / (x * x * x * x);
+    }
+    if (kind == WBII) {  // FMT.h:626-632
+      if (x < 1e-12) return 7.5 + 25.2 * x;
+      return (-(6 - 21 * x + 26 * x * x - 19 * x * x * x + 2 * x * x * x * x) /
+              (x * x * x * (1 - x) * (1 - x) * (1 - x) * (1 - x))) - (6.0 / (x * x * x * x)) * std::log(1 - x);
+    }
+    const double f = 1.0 / (1.0 - x);  // FMT.h:196-200
+    return 6 * f * f * f * f;
+  }
+  double dPhi3_dS2_dS2(const FM& m) const {
+    const double s2 = m.s2;
+    if (kind == ROSENFELD) return (1.0 / (24 * M_PI)) * (6 * s2);  // FMT.h:218-223
+    if (kind == RSLT) {  // FMT.h:313-321
+      if (s2 < 1e-20) return 0.0;
+      const double psi = m.v2_v2 / (s2 * s2);
+      return (1.0 / (6 * M_PI)) * s2 * (1 - psi) * (1 + psi + 2 * psi * psi);
+    }
+    return (A / (24 * M_PI)) * (6 * s2) + (B / (24 * M_PI)) * (6 * s2);  // FMT.h:417-422
+  }
+  double dPhi3_dV2_dS2(int i, const FM& m) const {
+    const double s2 = m.s2;
+    if (kind == ROSENFELD) return (1.0 / (24 * M_PI)) * (-6 * m.v2[i]);  // FMT.h:224-227
+    if (kind == RSLT) {  // FMT.h:323-331
+      if (s2 < 1e-20) return 0.0;
+      const double psi = m.v2_v2 / (s2 * s2);
+      return (1.0 / (36 * M_PI)) * (-6 * m.v2[i]) * (1 - psi) * (1 + 3 * psi);
+    }
+    return (A / (24 * M_PI)) * (-6 * m.v2[i]);  // FMT.h:423-426
+  }
+  double dPhi3_dV2_dV2(int i, int j, const FM& m) const {
+    const double s2 = m.s2;
+    if (kind == ROSENFELD) return i == j ? (1.0 / (24 * M_PI)) * (-6 * s2) : 0.0;  // FMT.h:228-231
+    if (kind == RSLT) {  // FMT.h:333-346
+      if (s2 < 1e-20) return 0.0;
+      const double psi = m.v2_v2 / (s2 * s2);
+      if (i == j) return (1.0 / (36 * M_PI)) * (-6 * s2) * (1 - psi) * (1 - psi - 4 * m.v2[i] * m.v2[i] / (s2 * s2));
+      return (1.0 / (36 * M_PI)) * (24 / s2) * m.v2[i] * m.v2[j] * (1 - psi);
+    }
+    return (A / (24 * M_PI)) * (3 * m.T[i][j] + 3 * m.T[j][i] - 6 * (i == j ? s2 : 0.0));  // FMT.h:427-434
+  }
+  double dPhi3_dS2_dT(int j, int k, const FM& m) const {  // FMT.h:456-461 (0 for the non-tensor models)
+    return tensor() ? (B / (4 * M_PI)) * (-m.T[k][j]) : 0.0;
+  }
+  double dPhi3_dV2_dT(int i, int j, int k, const FM& m) const {  // FMT.h:462-471
+    if (!tensor()) return 0.0;
+    double val = 0;
+    if (i == j) val += A * m.v2[k];
+    if (i == k) val += A * m.v2[j];
+    return val / (8 * M_PI);
+  }
+  double dPhi3_dT_dT(int j, int k, int l, int n, const FM& m) const {  // FMT.h:473-480
+    if (!tensor()) return 0.0;
+    double val = 0;
+    if (l == k) val += -A * m.T[n][j] + 2 * B * m.T[n][j];
+    if (n == j) val += -A * m.T[k][l] + 2 * B * m.T[k][l];
+    if (l == k && n == j) val += -2 * B * m.s2;
+    return val / (8 * M_PI);
+  }
+  // r = (d2 Phi / dn dn) . v over the 19 measures: FMT::calculate_d2Phi_dot_V, src/FMT.cpp:181-280
+  void d2phi_dot(const FM& n, const FM& v, FM& r) const {
+    const double e = n.eta;
+    const double F1p = f1p(e), F2 = f2(e), F3 = f3(e), F2p = f2p(e), F3p = f3p(e);
+    const double F1pp = f1pp(e), F2pp = f2pp(e), F3pp = f3pp(e);
+    const double c1 = 1 / M_PI, c2 = 1 / (2 * M_PI);
+    r = FM();
+    r.eta += (-c1 * n.s0 * F1pp + c2 * (n.s1 * n.s2 - n.v1_v2) * F2pp + Phi3(n) * F3pp) * v.eta;  // eta-eta
+    r.eta += -c1 * F1p * v.s0;  // eta-s0
+    r.s0 += -c1 * F1p * v.eta;
+    r.eta += c2 * n.s2 * F2p * v.s1;  // eta-s1
+    r.s1 += c2 * n.s2 * F2p * v.eta;
+    const double es2 = c2 * n.s1 * F2p + dPhi3_dS2(n) * F3p;  // eta-s2
+    r.eta += es2 * v.s2;
+    r.s2 += es2 * v.eta;
+    for (int k = 0; k < 3; ++k) {
+      const double ev1 = -c2 * n.v2[k] * F2p;  // eta-v1
+      r.eta += ev1 * v.v1[k];
+      r.v1[k] += ev1 * v.eta;
+      const double ev2 = -c2 * n.v1[k] * F2p + dPhi3_dV2(k, n) * F3p;  // eta-v2
+      r.eta += ev2 * v.v2[k];
+      r.v2[k] += ev2 * v.eta;
+    }
+    r.s1 += c2 * F2 * v.s2;  // s1-s2
+    r.s2 += c2 * F2 * v.s1;
+    r.s2 += dPhi3_dS2_dS2(n) * F3 * v.s2;  // s2-s2
+    for (int i = 0; i < 3; ++i) {
+      r.s2 += dPhi3_dV2_dS2(i, n) * F3 * v.v2[i];  // s2-v2
+      r.v2[i] += dPhi3_dV2_dS2(i, n) * F3 * v.s2;
+      r.v1[i] += -c2 * F2 * v.v2[i];  // v1-v2
+      r.v2[i] += -c2 * F2 * v.v1[i];
+      for (int j = 0; j < 3; ++j) {
+        r.v2[i] += dPhi3_dV2_dV2(i, j, n) * F3 * v.v2[j];  // v2-v2
+        r.eta += dPhi3_dT(i, j, n) * F3p * v.T[i][j];  // eta-T
+        r.T[i][j] += dPhi3_dT(i, j, n) * F3p * v.eta;
+        r.s2 += dPhi3_dS2_dT(i, j, n) * F3 * v.T[i][j];  // s2-T
+        r.T[i][j] += dPhi3_dS2_dT(i, j, n) * F3 * v.s2;
+        for (int k = 0; k < 3; ++k) {
+          r.v2[i] += dPhi3_dV2_dT(i, j, k, n) * F3 * v.T[j][k];  // v2-T
+          r.T[j][k] += dPhi3_dV2_dT(i, j, k, n) * F3 * v.v2[i];
+          for (int l = 0; l < 3; ++l) r.T[i][j] += dPhi3_dT_dT(i, j, k, l, n) * F3 * v.T[k][l];  // T-T
+        }
+      }
+    }
+  }
   double phi(const FM& m) const {  // FMT.cpp:114-135
     const double e = m.eta;
     double p = 0;
@@ -732,6 +855,55 @@ int fmt_eval(const FmtState& s, const double* rho, double* dF, double* Fout, dou
   return 0;
 }
 
+// Hessian of the FMT part applied to v, ACCUMULATED into out: FMT::add_second_derivative, src/FMT.cpp:437-567.
+// psi_b = conv(w_b, v) as measures; Lambda = d2Phi . psi per site; out += dV sum_a conv(conj(w_a), Lambda_a),
+// with the scale factors of the measures (s0: 1/d^2, s1: 1/d, v1: 1/d) and all nine tensor entries.
+void fmt_hess(const FmtState& s, const double* rho, const double* v, double* out) {
+  const Grid& g = s.g;
+  const long N = g.N(), Nk = g.Nk();
+  const bool tensor = s.model.tensor();
+  const int m = tensor ? 11 : 5;
+  const double dV = s.dx * s.dy * s.dz, d = s.hsd;
+  std::vector<cd> rk(Nk), vk(Nk), tmp(Nk);
+  r2c3d(g, rho, rk.data());
+  r2c3d(g, v, vk.data());
+  std::vector<double> n(11 * N, 0.0), p(11 * N, 0.0);
+  for (int a = 0; a < m; ++a) {
+    for (long k = 0; k < Nk; ++k) tmp[k] = rk[k] * s.wk[a * Nk + k];
+    c2r3d(g, tmp.data(), n.data() + a * N);
+    for (long k = 0; k < Nk; ++k) tmp[k] = vk[k] * s.wk[a * Nk + k];
+    c2r3d(g, tmp.data(), p.data() + a * N);
+  }
+  const double *np[11], *pp[11];
+  for (int a = 0; a < 11; ++a) {
+    np[a] = n.data() + a * N;
+    pp[a] = p.data() + a * N;
+  }
+  // Lambda per array (species-level back-convolution inputs, conjugated weights)
+  std::vector<double> lam(11 * N, 0.0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (long i = 0; i < N; ++i) {
+    FM mn, mp, L;
+    measures_from_arrays(np, i, d, tensor, mn);
+    measures_from_arrays(pp, i, d, tensor, mp);
+    s.model.d2phi_dot(mn, mp, L);
+    lam[0 * N + i] = L.eta;
+    lam[1 * N + i] = L.s0 / (d * d) + L.s1 / d + L.s2;
+    for (int j = 0; j < 3; ++j) lam[(2 + j) * N + i] = L.v1[j] / d + L.v2[j];
+    if (tensor)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) lam[kTI[j][k] * N + i] += L.T[j][k];
+  }
+  std::vector<cd> acc(Nk, cd(0, 0));
+  for (int a = 0; a < m; ++a) {
+    r2c3d(g, lam.data() + a * N, tmp.data());
+    for (long k = 0; k < Nk; ++k) acc[k] += tmp[k] * std::conj(s.wk[a * Nk + k]);
+  }
+  std::vector<double> r(N);
+  c2r3d(g, acc.data(), r.data());
+  for (long i = 0; i < N; ++i) out[i] += r[i] * dV;
+}
+
 // ---------------------------------------------------------------------------
 // Mean field: LJ with WCA split (include/Potential1.h:23-189) and the
 // quadratic-force interpolation kernel Interaction_Interpolation_QF
@@ -863,6 +1035,25 @@ int dft_eval(DftState& s, const double* rho, double* dF) {
   s.parts[3] = Fmf;
   s.parts[4] = Fext;
   return 0;
+}
+
+// (d2F / drho_i drho_j) v_j: DFT::matrix_dot_v_intern, src/DFT.cpp:435-488 (periodic, no fixed boundary):
+// ideal gas dV v / rho when full (full_hessian_), FMT::add_second_derivative, and the mean field
+// Interaction_Base::add_second_derivative (src/Interaction.cpp:119-154): (dV/N) c2r(v_k w_att,k).
+void dft_hess(const DftState& s, const double* rho, const double* v, double* out, bool full) {
+  const Grid& g = s.fmt.g;
+  const long N = g.N(), Nk = g.Nk();
+  const double dV = s.fmt.dx * s.fmt.dy * s.fmt.dz;
+  for (long i = 0; i < N; ++i) out[i] = full && s.ideal ? dV * v[i] / rho[i] : 0.0;
+  if (s.has_fmt) fmt_hess(s.fmt, rho, v, out);
+  if (s.has_mf) {
+    std::vector<cd> vk(Nk);
+    r2c3d(g, v, vk.data());
+    for (long k = 0; k < Nk; ++k) vk[k] *= s.wmf[k];
+    std::vector<double> r(N);
+    c2r3d(g, vk.data(), r.data());
+    for (long i = 0; i < N; ++i) out[i] += r[i] * dV / (double)N;
+  }
 }
 
 }  // namespace
@@ -1018,6 +1209,11 @@ orc_dft* orc_dft_create(int kind, double A, double B, int Nx, int Ny, int Nz, do
 void orc_dft_destroy(orc_dft* h) { delete reinterpret_cast<DftState*>(h); }
 
 double orc_dft_avdw(orc_dft* h) { return reinterpret_cast<DftState*>(h)->a_vdw; }
+
+// out = (d2F/drho drho) v; full = 0 leaves out the ideal-gas term (DFT::full_hessian_ false).
+void orc_dft_hess(orc_dft* h, const double* rho, const double* v, double* out, int full) {
+  dft_hess(*reinterpret_cast<DftState*>(h), rho, v, out, full != 0);
+}
 
 // parts[5] = F, F_id, F_hs, F_mf, F_ext.  Returns 1 on eta too large.
 int orc_dft_eval(orc_dft* h, const double* rho, double* dF, double* parts) {

@@ -83,3 +83,17 @@ def test_lj_hard_sphere_diameter(demo):
     val, _ = integrate.quad(lambda r: 1 - np.exp(-(V(r) - vmin) / kT) if r > 0 else 1.0, 0, rmin, epsabs=1e-14,
                             epsrel=1e-13, limit=200)
     assert abs(demo["hsd_lj_kT0.8"][0] - val) < 1e-11
+
+
+def test_hessian_through_cpp_api(demo):
+    shape, dx = (16, 16, 16), 0.1
+    rho = seeded(shape, 0.5, 41, 0.1)
+    v = 2 * oracle.splitmix64_uniform(int(np.prod(shape)), 43).reshape(shape) - 1
+    o = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, accurate_weights=True)
+    Hv = o.hess(rho, v)
+    assert abs(demo["hess"][0] - Hv.sum()) <= 1e-10 * np.abs(Hv).sum()
+    assert rel(demo["hess"][1], np.sum(v * Hv)) <= 1e-10
+    of = oracle.DFT(shape, dx, 1.0, ideal=False, accurate_weights=True)
+    Hf = of.hess(rho, v)
+    assert abs(demo["hess_fmt"][0] - Hf.sum()) <= 1e-10 * np.abs(Hf).sum()
+    assert rel(demo["hess_fmt"][1], np.sum(v * Hf)) <= 1e-10
