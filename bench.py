@@ -63,6 +63,7 @@ def stage_bytes(shape, e, m):
         "y_inv": 2 * m * c * Nk,
         "z_pointwise": 2 * m * c * Nk,
         "y_fwd2": 2 * m * c * Nk,
+        "y_z_y_slabs": 6 * m * c * Nk,  # the three passes above, fused by x-slab (counted without cache reuse)
         "x_conv_bwd": m * c * Nk + m * e * Nk + c * Nk,
         "y_inv2": 2 * c * Nk,
         "z_final": c * Nk + 8 * N + 8 * N,  # acc in, rho in, dF out
@@ -259,6 +260,39 @@ def cpu_baseline(rho, threads):
     return dt, setup, F, dF
 
 
+def warm_start_leg(cdft, count, check_unet=True):
+    """Learned warm start vs uniform start on held-out 64^3 Gaussian-well systems (FP64 FIRE to 1e-8), with the
+    committed network; plus the network's own kernel throughput at 64^3 (forward batch 16, train step batch 8)."""
+    from classicaldft_amd import warmstart as W
+    wl = W.Workload(64)
+    ws = W.WarmStart.load()
+    out = W.compare(wl, ws, 0x5EED5000 + 4096, count, schedule="fp64")
+    if check_unet:
+        D = 64
+        rng = np.random.default_rng(0)
+        V = cdft.DeviceArray.from_host(rng.standard_normal((16, D, D, D)).astype(np.float32), np.float32)
+        M = cdft.DeviceArray.from_host(np.zeros(16, np.float32), np.float32)
+        T = cdft.DeviceArray.from_host(0.1 * rng.standard_normal((16, D, D, D)).astype(np.float32), np.float32)
+        o = cdft.DeviceArray((16, D, D, D), np.float32)
+        net = cdft.UNet(D, seed=3)
+        idx16, idx8 = np.arange(16), np.arange(8)
+        net.forward(V, M, idx16, o)
+        t = time.perf_counter()
+        for _ in range(10):
+            net.forward(V, M, idx16, o)
+        tf = (time.perf_counter() - t) / 10
+        net.train_step(V, M, T, idx8, 1e-5)
+        t = time.perf_counter()
+        for _ in range(10):
+            net.train_step(V, M, T, idx8, 1e-5)
+        tt = (time.perf_counter() - t) / 10
+        fwd = W.unet_flops(D)
+        out["unet"] = {"grid": D, "forward_ms_batch16": tf * 1e3, "train_step_ms_batch8": tt * 1e3,
+                       "forward_tflops": 16 * fwd / tf * 1e-12, "train_tflops": 8 * 3 * fwd / tt * 1e-12,
+                       "flops_forward_per_sample": fwd, "peak_bf16_dense_tflops": 2500.0}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +304,8 @@ def main():
     ap.add_argument("--fire-max-steps", type=int, default=20000)
     ap.add_argument("--ddft-steps", type=int, default=0, help="DDFT fcc-seed steps at 256^3 (0: skip)")
     ap.add_argument("--batch", action="store_true", help="batched FIRE on 96 systems of 48^3 / 64^3 / 96^3")
+    ap.add_argument("--warm-systems", type=int, default=64,
+                    help="held-out 64^3 systems for the learned warm start vs uniform start (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -343,9 +379,12 @@ def main():
     }
     if args.fire_grid:
         print("# FIRE time to tolerance", flush=True)
-        result["fire_time_to_tol"] = {"cold": fire_time_to_tol(cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
+        result["fire_time_to_tol"] = {"slit_pore": fire_time_to_tol(cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
                                       "start": "rho_bulk exp(-V_ext)", "tol": 1e-8,
-                                      "criterion": "max_i |dF/drho_i| / dV (raw gradient)", "warm": None}
+                                      "criterion": "max_i |dF/drho_i| / dV (raw gradient)"}
+    if args.warm_systems:
+        print("# learned warm start vs uniform start", flush=True)
+        result.setdefault("fire_time_to_tol", {})["warm_vs_cold"] = warm_start_leg(cdft, args.warm_systems)
     if args.batch:
         print("# batched FIRE", flush=True)
         result["batched_fire"] = batch_fire(cdft)

@@ -17,11 +17,12 @@ HOST = os.path.join(HERE, "host")
 ARCH = os.environ.get("CDFT_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # keep the compiler from emitting packed-FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
+HIP_SOURCES = ("cdft_api.hip", "solvers.hip", "batch.hip", "unet.hip")
 
 
 def hipcc():
@@ -37,11 +38,21 @@ def run(cmd):
 
 
 def build_cdft():
+    """Each translation unit compiled in its own hipcc process (in parallel), then one link."""
     out = os.path.join(HERE, "libcdft.so")
     tmp = out + ".tmp"
-    run([hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp,
-         os.path.join(CSRC, "cdft_api.hip"), os.path.join(CSRC, "solvers.hip"), os.path.join(CSRC, "batch.hip"),
-         os.path.join(CSRC, "unet.hip")])
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, f[:-4] + ".o") for f in HIP_SOURCES]
+    cmds = [[hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", "-o", o, os.path.join(CSRC, f)]
+            for f, o in zip(HIP_SOURCES, objs)]
+    for c in cmds:
+        print(" ".join(c), flush=True)
+    procs = [subprocess.Popen(c) for c in cmds]
+    codes = [p.wait() for p in procs]
+    if any(codes):
+        raise subprocess.CalledProcessError(max(codes), cmds[[i for i, c in enumerate(codes) if c][0]])
+    run([hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, out)
     return out
 

@@ -603,6 +603,7 @@ int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
     struct Cfg {
       bool reg, cols, fwd, bwd;
       int pw;
+      long slab;
     };
     struct Key {
       int Nx, Ny, Nz, mixed, model, nconv, nb;
@@ -619,6 +620,7 @@ int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
       s->nb_half_fwd = c.fwd;
       s->nb_half_bwd = c.bwd;
       s->pw_rows = c.pw;
+      s->slab_bytes = c.slab;
     };
     {
       std::lock_guard<std::mutex> lk(mtx);
@@ -637,7 +639,7 @@ int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
           for (int bwd = 0; bwd < 2; ++bwd)
             for (int pw : {0, 1}) {
               if (!r && (cols || fwd || bwd || pw)) continue;  // the LDS kernels have no such choices
-              cands.push_back({r == 1, cols == 1, fwd == 1, bwd == 1, pw});
+              cands.push_back({r == 1, cols == 1, fwd == 1, bwd == 1, pw, s->slab_bytes});
             }
     // a uniform density in the member buffers (no eta flags, realistic work)
     const long n = s->N * s->nb;
@@ -648,7 +650,7 @@ int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
     HIPCHK(hipEventCreate(&e1));
     double best = 1e300;
     Cfg bestc = cands.front();
-    for (const Cfg& c : cands) {
+    auto trial = [&](const Cfg& c) {
       apply(c);
       enqueue_eval(s, s->rho.as<double>(), s->dF.as<double>(), true, false);  // warm-up
       HIPCHK(hipEventRecord(e0, s->stream));
@@ -661,7 +663,15 @@ int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
         best = t / reps;
         bestc = c;
       }
-    }
+    };
+    for (const Cfg& c : cands) trial(c);
+    // then the x-slab size of the y / z / y section on the best tiles (0: three whole-array passes)
+    if (bestc.reg)
+      for (long mb : {0L, 96L, 192L, 288L}) {
+        Cfg c = bestc;
+        c.slab = mb << 20;
+        if (c.slab != bestc.slab) trial(c);
+      }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(kThreads), 0, s->stream, s->flag.as<int>(), s->nb);

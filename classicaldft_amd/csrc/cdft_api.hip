@@ -150,6 +150,10 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
     if (n > 2) s->nb_half_bwd = e[2] == '1';
   }
   if (const char* e = std::getenv("CDFT_PW_ROWS")) s->pw_rows = std::atoi(e);
+  // y / z / y section in x-slabs of ~192 MB (all arrays of all members): each slab stays in the 256 MB
+  // last-level cache between its three kernels (measured optimum at 256^3, FP64 and mixed)
+  s->slab_bytes = 192L << 20;
+  if (const char* e = std::getenv("CDFT_SLAB_MB")) s->slab_bytes = std::atol(e) << 20;
   s->twd.build(g);
   if (s->mixed) s->twf.build(g);
   const size_t cs = s->mixed ? sizeof(cplx<float>) : sizeof(cplx<double>);
@@ -195,10 +199,14 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
   *out = s.release();
 }
 
+// Pointwise z pass over the rows of x-planes [x0, x1) (all planes by default); its partial sums go to block
+// slots blk_off, blk_off + 1, ...  Returns the number of blocks.
 template <typename T, int MODEL>
-long launch_pointwise(cdft_system_s& s, int nconv) {
+long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long blk_off = 0) {
   const Tables<T>& tw = tables<T>(s);
   const int M = s.g.Nz / 2;
+  if (x1 < 0) x1 = s.g.Nx;
+  const long nrows = (long)(x1 - x0) * s.g.Ny;
   PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
   if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
   if (rows_reg_ok<T>(s)) {
@@ -210,16 +218,18 @@ long launch_pointwise(cdft_system_s& s, int nconv) {
       int RB = std::min(kThreads / (nconv * RF::P), (65536 - 4096) / per);
       if (s.pw_rows > 0) RB = std::min(RB, s.pw_rows);
       if (RB < 1) return;
-      nblk = div_up(s.nrows, RB);
+      nblk = div_up(nrows, RB);
       const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr};
+      double* vmf = s.vmf.p ? s.vmf.as<double>() + (long)x0 * s.g.Ny * s.g.Nz : nullptr;
       hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk, s.nb), dim3(nconv * RB * RF::P),
-                         size_t(RB) * per, s.stream, s.B.as<cplx<T>>(), s.Nkp, s.m, nconv, s.vmf.as<double>(), s.nrows,
-                         s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, pp,
-                         s.partials.as<double2>(), s.flag.as<int>(), bt);
+                         size_t(RB) * per, s.stream, s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
+                         vmf, nrows, s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, pp,
+                         s.partials.as<double2>() + blk_off, s.flag.as<int>(), bt);
       HIPCHK(hipGetLastError());
     });
     if (nblk > 0) return nblk;
   }
+  REQUIRE(x0 == 0 && x1 == s.g.Nx, "x-slab passes need the register z kernels");
   REQUIRE(s.nb == 1, "batched systems need transform lengths the register FFT supports");
   const int RB = s.RBpw;
   const size_t smem = sizeof(cplx<T>) * nconv * RB * M;
@@ -269,22 +279,42 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
       HIPCHK(hipGetLastError());
     }
     s.mark("x_conv_fwd");
-    launch_cols<T>(s, B, Nkp, nconv, 1, false, nb, sB);
-    s.mark("y_inv");
-    if (s.m > 0) {
-      long nblk = 0;
+    const int planes = s.m > 0 ? slab_planes<T>(s, nconv) : 0;
+    auto pointwise = [&](int x0, int x1, long off) -> long {
       switch (s.d.model) {
-        case CDFT_ROSENFELD: nblk = launch_pointwise<T, M_ROSENFELD>(s, nconv); break;
-        case CDFT_RSLT: nblk = launch_pointwise<T, M_RSLT>(s, nconv); break;
-        case CDFT_ESFMT: nblk = launch_pointwise<T, M_ESFMT>(s, nconv); break;
-        case CDFT_WHITEBEAR_I: nblk = launch_pointwise<T, M_WBI>(s, nconv); break;
-        default: nblk = launch_pointwise<T, M_WBII>(s, nconv); break;
+        case CDFT_ROSENFELD: return launch_pointwise<T, M_ROSENFELD>(s, nconv, x0, x1, off);
+        case CDFT_RSLT: return launch_pointwise<T, M_RSLT>(s, nconv, x0, x1, off);
+        case CDFT_ESFMT: return launch_pointwise<T, M_ESFMT>(s, nconv, x0, x1, off);
+        case CDFT_WHITEBEAR_I: return launch_pointwise<T, M_WBI>(s, nconv, x0, x1, off);
+        default: return launch_pointwise<T, M_WBII>(s, nconv, x0, x1, off);
       }
+    };
+    if (planes > 0) {
+      // y inverse, pointwise z and y forward x-slab by x-slab: a slab's arrays stay in the last-level cache
+      // between the three kernels instead of making three round trips through HBM
+      long nblk = 0;
+      for (int x0 = 0; x0 < s.g.Nx; x0 += planes) {
+        const int x1 = std::min(s.g.Nx, x0 + planes);
+        launch_cols_slab<T>(s, B, Nkp, nconv, false, nb, sB, x0, x1);
+        nblk += pointwise(x0, x1, nblk);
+        launch_cols_slab<T>(s, B, Nkp, s.m, true, nb, sB, x0, x1);
+      }
+      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+                         s.results.as<double>(), s.pslots, 8L);
+      s.mark("y_z_y_slabs");
+    } else {
+      launch_cols<T>(s, B, Nkp, nconv, 1, false, nb, sB);
+      s.mark("y_inv");
+    }
+    if (s.m > 0 && planes == 0) {
+      const long nblk = pointwise(0, s.g.Nx, 0);
       hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
                          s.results.as<double>(), s.pslots, 8L);
       s.mark("z_pointwise");
       launch_cols<T>(s, B, Nkp, s.m, 1, true, nb, sB);
       s.mark("y_fwd2");
+    }
+    if (s.m > 0) {
       if (reg_fft_ok<T>(s, s.g.Nx)) {
         const ColGeo cg = col_geo<T>(s.g, 0);
         with_reg_len(s.g.Nx, [&](auto lc) {

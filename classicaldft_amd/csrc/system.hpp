@@ -162,6 +162,7 @@ struct cdft_system_s {
   // forward convolution, backward convolution (tuned per precision at create)
   bool nb_half_cols = false, nb_half_fwd = false, nb_half_bwd = false;
   int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
+  long slab_bytes = 0;   // cache budget of one x-slab of the y / z / y section (0: no slabs)
   bool has_vext = false;
   bool last_with_dft = true;
   // timing
@@ -340,6 +341,43 @@ inline bool launch_cols_reg(const cdft_system_s& s, cplx<T>* data, long stride, 
       HIPCHK(hipGetLastError());
     });
   });
+}
+
+// y pass over the x-planes [x0, x1) only (register kernels): the columns of those planes are one contiguous range
+// of the flat column numbering, so the pass is the full-array pass on an offset base with fewer columns.
+template <typename T>
+inline void launch_cols_slab(const cdft_system_s& s, cplx<T>* data, long stride, int narr, bool fwd, int members,
+                             long mstride, int x0, int x1) {
+  const Tables<T>& tw = tables<T>(s);
+  const cplx<T>* t = tw.y.template as<cplx<T>>();
+  ColGeo cg = col_geo<T>(s.g, 1);
+  cg.ncols = unsigned((x1 - x0) * s.g.Zp);
+  cg.bytes = unsigned((long)(x1 - x0) * s.g.Ny * s.g.Zp * (long)sizeof(cplx<T>));
+  cplx<T>* base = data + (long)x0 * s.g.Ny * s.g.Zp;
+  const bool ok = with_reg_len(s.g.Ny, [&](auto lc) {
+    constexpr int Lc = decltype(lc)::value;
+    with_nb<T, Lc>(s.nb_half_cols, [&](auto nbc) {
+      constexpr int NB = decltype(nbc)::value;
+      const dim3 grid(div_up(cg.ncols, NB), narr, members), block(NB * F2<Lc>::P);
+      if (fwd)
+        hipLaunchKernelGGL((k_cols2<T, Lc, NB, true>), grid, block, 0, s.stream, base, stride, cg, t, mstride);
+      else
+        hipLaunchKernelGGL((k_cols2<T, Lc, NB, false>), grid, block, 0, s.stream, base, stride, cg, t, mstride);
+      HIPCHK(hipGetLastError());
+    });
+  });
+  REQUIRE(ok, "x-slab passes need register-FFT lengths");
+}
+
+// x-planes per slab of the fused y / z / y section (0: whole arrays, three separate passes): the planes of
+// nconv arrays of all members that fit slab_bytes, when the register kernels handle y and z.
+template <typename T>
+inline int slab_planes(const cdft_system_s& s, int nconv) {
+  if (s.slab_bytes <= 0 || !rows_reg_ok<T>(s) || !reg_fft_ok<T>(s, s.g.Ny)) return 0;
+  const long per_plane = (long)nconv * s.nb * s.g.Ny * s.g.Zp * (long)sizeof(cplx<T>);
+  const long p = s.slab_bytes / per_plane;
+  if (p < 1 || p >= s.g.Nx) return 0;
+  return (int)p;
 }
 
 // x or y pass of narr arrays (stride apart) of each of `members` batch members

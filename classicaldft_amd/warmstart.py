@@ -31,6 +31,17 @@ from .workloads import well_systems
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "warmstart_64.npz")
 
 
+# (input channels, output channels, level) of the ten 3x3x3 convolutions, as in unet.hip
+UNET_LAYERS = ((8, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (96, 32, 1),
+               (32, 32, 1), (48, 16, 0), (16, 16, 0))
+
+
+def unet_flops(D):
+    """Multiply-add flops of one sample's forward convolutions (2 x 27 x cin x cout per output site; the
+    backward data and weight products cost the same again each)."""
+    return sum(2 * 27 * ci * co * (D >> lv) ** 3 for ci, co, lv in UNET_LAYERS)
+
+
 def log(msg):
     print(msg, flush=True)
 

@@ -199,3 +199,23 @@ def test_register_fft_matches_lds_fft(cdft, monkeypatch, precision):
     print("reg vs lds", rel(F1, F0), maxrel(dF1, dF0))
     assert rel(F1, F0) <= tol
     assert maxrel(dF1, dF0) <= tol
+
+
+def test_x_slab_section_matches_whole_array_passes(cdft, monkeypatch):
+    """The y / z / y section run x-slab by x-slab (a small cache budget forces several slabs) gives the same
+    F and dF as three whole-array passes, FP64 and mixed, single systems and batches."""
+    shape, dx = (32, 24, 48), 0.1
+    rho = seeded_density(shape, 0.5, 0x5EED0300, amp=0.1)
+    kw = dict(model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=1.0, mu=0.2)
+    for prec in (cdft.FP64, cdft.MIXED):
+        monkeypatch.setenv("CDFT_SLAB_MB", "0")
+        p0, d0 = cdft.System(shape, dx, 1.0, precision=prec, **kw).eval(rho)
+        monkeypatch.setenv("CDFT_SLAB_MB", "1")  # 1 MiB: a few x-planes per slab
+        p1, d1 = cdft.System(shape, dx, 1.0, precision=prec, **kw).eval(rho)
+        assert np.max(np.abs(p1 - p0)) <= 1e-14 * abs(p0[0])
+        assert maxrel(d1, d0) <= 1e-14
+        b = cdft.System(shape, dx, 1.0, precision=prec, mus=[0.2, 0.3], model=cdft.WHITEBEAR_I, ideal=True,
+                        mean_field=True, kT=1.0)
+        pb, db = b.eval(np.array([rho, rho]))
+        assert np.max(np.abs(pb[0] - p0)) <= 1e-14 * abs(p0[0])
+        assert maxrel(db[0], d0) <= 1e-14
