@@ -10,3 +10,4 @@ from .cdft import (  # noqa: F401
     DDFT, FireBatch, UNet, fft_r2c, fft_c2r, lj_hsd,
     device_count, library_path,
 )
+from .formats import read_density, write_density  # noqa: F401,E402

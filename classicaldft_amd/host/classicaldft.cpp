@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <functional>
 
 namespace {
@@ -353,6 +355,34 @@ double DFT::calculateFreeEnergyAndDerivatives(bool onlyFex) {
   return F;
 }
 
+void Density::writeDensity(std::string filename) const {
+  std::ofstream of(filename.c_str(), std::ios::binary);
+  if (!of.good()) throw std::runtime_error("Could not open output file " + filename);
+  of << rho_.size() << std::endl;
+  for (long i = 0; i < rho_.size(); ++i) of << rho_.get(i) << " ";
+  of << std::endl;
+}
+
+void Density::readDensity(const char* filename) {
+  std::ifstream in(filename, std::ios::binary);
+  if (!in.good()) {
+    std::stringstream s;
+    s << "Could not open input file " << filename << " ... aborting ... " << std::endl;
+    throw std::runtime_error(s.str());
+  }
+  long n = -1;
+  in >> n;
+  if (n != rho_.size())
+    throw std::runtime_error(std::string("Density file ") + filename + " holds " + std::to_string(n) +
+                             " values, the lattice has " + std::to_string(rho_.size()));
+  for (long i = 0; i < n; ++i) {
+    double v;
+    in >> v;
+    rho_.set(i, v);
+  }
+  if (in.fail()) throw std::runtime_error(std::string("Could not read the values of ") + filename);
+}
+
 void DFT::matrix_dot_v_intern(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& result, void*) const {
   Species* s = species_.front();
   const long N = s->getDensity().Ntot();
@@ -432,6 +462,11 @@ double fireMinimizer2::step() {
   DFT_Vec rho(dft_->get_lattice().Ntot());
   density(rho);
   dft_->getSpecies(0)->set_density(rho);
+  if (snapshot_every_ > 0 && snapshot_count_ % snapshot_every_ == 0) {
+    std::string of = "snapshot_s0.dat";
+    dft_->writeDensity(0, of);
+  }
+  ++snapshot_count_;
   return F_;
 }
 int fireMinimizer2::run_to_raw_tolerance(long maxSteps, double tol) {

@@ -103,6 +103,11 @@ class Density : public Lattice {
   void set_external_field(const DFT_Vec& v) { vWall_ = v; ++field_version_; }
   bool has_external_field() const { return has_field_(); }
   unsigned field_version() const { return field_version_; }
+  // src/Density.cpp:227-244 with DFT_Vec::save / load (additionalCode/DFT_Eigen.cpp:71-85): text, "N" on the first
+  // line, then the N values separated by blanks at the stream's default precision (6 significant digits).
+  // readDensity throws std::runtime_error if the file cannot be opened or holds another number of values.
+  void writeDensity(std::string filename) const;
+  void readDensity(const char* filename);
 
  private:
   bool has_field_() const;
@@ -289,6 +294,7 @@ class DFT {
   double getNumberAtoms(int s) const { return species_[s]->getDensity().getNumberAtoms(); }
   DFT_Vec& getDF(int s) { return species_[s]->getDF(); }
   double calculateFreeEnergyAndDerivatives(bool onlyFex);
+  void writeDensity(int i, std::string& of) const { species_[i]->getDensity().writeDensity(of); }  // DFT.h:76
   // Dynamical_Matrix interface (include/Dynamical_Matrix.h:22-25, src/Dynamical_Matrix.cpp:34-70,
   // src/DFT.cpp:435-488): result = (d2 beta Omega / drho drho) v, the ideal-gas term only with
   // full_hessian_; with use_squared_matrix_ the matrix is applied twice.  Periodic systems.
@@ -358,6 +364,9 @@ class fireMinimizer2 : public Minimizer {
   void setTimeStepMax(double dt) { params_.dt_max = dt; }
   void setAlphaStart(double a) { params_.alpha_start = a; }
   void setAlphaFac(double a) { params_.f_alf = a; }
+  // src/Minimizer.cpp:259-268: every 10th step writes snapshot_s0.dat (the density) to the working directory;
+  // n = 0 turns the snapshots off.  The step counter is per minimizer (a function-static in the reference).
+  void set_snapshot_interval(int n) { snapshot_every_ = n; }
   // Runs FIRE steps on the GPU until max |dF/drho|/dV <= tol.
   int run_to_raw_tolerance(long maxSteps, double tol);
   void density(DFT_Vec& out) const;  // current density (host copy)
@@ -367,6 +376,8 @@ class fireMinimizer2 : public Minimizer {
   cdft_fire_params params_{};
   cdft_fire fire_ = nullptr;
   cdft_fire_report rep_{};
+  int snapshot_every_ = 10;
+  long snapshot_count_ = 0;
 };
 
 class DDFT : public Minimizer {

@@ -25,10 +25,11 @@ def seeded(shape, rho_b, seed, amp):
 
 
 @pytest.fixture(scope="module")
-def demo(cdft):
+def demo(cdft, tmp_path_factory):
     exe = os.path.join(ROOT, "classicaldft_amd", "dropin_demo")
-    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout
-    vals = {}
+    wd = tmp_path_factory.mktemp("demo")  # fireMinimizer2 writes snapshot_s0.dat to the working directory
+    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120, cwd=wd).stdout
+    vals = {"_dir": wd}
     for line in out.splitlines():
         k, *v = line.split()
         vals[k] = [float(x) for x in v]
@@ -97,3 +98,10 @@ def test_hessian_through_cpp_api(demo):
     Hf = of.hess(rho, v)
     assert abs(demo["hess_fmt"][0] - Hf.sum()) <= 1e-10 * np.abs(Hf).sum()
     assert rel(demo["hess_fmt"][1], np.sum(v * Hf)) <= 1e-10
+
+
+def test_fire_snapshot_file(demo):
+    """fireMinimizer2 wrote snapshot_s0.dat (src/Minimizer.cpp:259-268) in the reference's density format."""
+    from classicaldft_amd import formats
+    rho = formats.read_density(os.path.join(demo["_dir"], "snapshot_s0.dat"), (16, 16, 16))
+    assert np.all(np.isfinite(rho)) and np.all(rho > 0)
