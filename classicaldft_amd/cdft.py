@@ -37,7 +37,19 @@ class _Desc(ctypes.Structure):
         ("ideal", _i), ("mu", _d), ("vext", _p),
         ("mean_field", _i), ("lj_sigma", _d), ("lj_eps", _d), ("lj_rcut", _d), ("kT", _d),
         ("mf_kernel", _p),
+        ("mf_potential", _i), ("mf_alpha", _d), ("mf_bh", _i), ("mf_r_att_min", _d), ("mf_scheme", _i),
+        ("mf_ngauss", _i),
     ]
+
+
+class _PotDesc(ctypes.Structure):
+    _fields_ = [("kind", _i), ("sigma", _d), ("eps", _d), ("rcut", _d), ("alpha", _d), ("bh", _i),
+                ("r_att_min", _d)]
+
+
+# mean-field potentials and kernel rules (cdft_potential, cdft_mf_scheme)
+POT_LJ, POT_TWF, POT_WHDF = 0, 1, 2
+MF_QF, MF_ZERO, MF_LE, MF_QE, MF_LF, MF_GAUSS_E, MF_GAUSS_F = range(7)
 
 
 class FireParams(ctypes.Structure):
@@ -97,6 +109,9 @@ def lib():
         L.cdft_fire_batch_destroy.argtypes = [_p]
         L.cdft_fire_batch_destroy.restype = None
         L.cdft_system_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_potential_hsd.argtypes = [ctypes.POINTER(_PotDesc), _d, ctypes.POINTER(_d)]
+        L.cdft_potential_vdw.argtypes = [ctypes.POINTER(_PotDesc), _d, ctypes.POINTER(_d)]
+        L.cdft_potential_watt.argtypes = [ctypes.POINTER(_PotDesc), _i, _p, _p]
         L.cdft_fmt_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_unet_create.argtypes = [_i, ctypes.c_ulonglong, ctypes.POINTER(_p)]
         L.cdft_unet_destroy.argtypes = [_p]
@@ -158,8 +173,10 @@ class System:
 
     def __init__(self, shape, dx, hsd=1.0, model=WHITEBEAR_I, precision=FP64, weights=None, ideal=False, mu=0.0,
                  vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0,
-                 mus=None, vexts=None):
-        """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create)."""
+                 mus=None, vexts=None, potential=POT_LJ, alpha=0.0, bh=False, r_att_min=0.0, mf_scheme=MF_QF,
+                 ngauss=0):
+        """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create).
+        potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule."""
         self.shape = tuple(int(n) for n in shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
@@ -179,7 +196,8 @@ class System:
         d = _Desc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A),
                   float(B), float(hsd), int(precision), arr(weights, 11 * self.N), int(bool(ideal)), float(mu),
                   arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
-                  arr(mf_kernel, self.N))
+                  arr(mf_kernel, self.N), int(potential), float(alpha), int(bool(bh)), float(r_att_min),
+                  int(mf_scheme), int(ngauss))
         h = _p()
         if mus is None:
             self.nb = 1
@@ -426,7 +444,7 @@ class FireBatch:
         vx = None if vexts is None else np.ascontiguousarray(vexts, dtype=np.float64).reshape(-1)
         d = _Desc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A), float(B),
                   float(hsd), int(precision), None, int(bool(ideal)), 0.0, None, int(bool(mean_field)), float(sigma),
-                  float(eps), float(rcut), float(kT), None)
+                  float(eps), float(rcut), float(kT), None, 0, 0.0, 0, 0.0, 0, 0)
         p = FireParams()
         _check(lib().cdft_fire_default_params(ctypes.byref(p)))
         if dt is not None:
@@ -591,6 +609,35 @@ class UNet:
             self.close()
         except Exception:
             pass
+
+
+def _pot(kind, sigma, eps, rcut, alpha, bh, r_att_min):
+    return _PotDesc(int(kind), float(sigma), float(eps), float(rcut), float(alpha), int(bool(bh)), float(r_att_min))
+
+
+def potential_hsd(kT, kind=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, bh=False, r_att_min=0.0):
+    """Potential1::getHSD for any potential and split (cdft_potential_hsd)."""
+    out = _d()
+    _check(lib().cdft_potential_hsd(ctypes.byref(_pot(kind, sigma, eps, rcut, alpha, bh, r_att_min)), float(kT),
+                                    ctypes.byref(out)))
+    return out.value
+
+
+def potential_vdw(kT, kind=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, bh=False, r_att_min=0.0):
+    """Potential1::getVDW_Parameter (cdft_potential_vdw)."""
+    out = _d()
+    _check(lib().cdft_potential_vdw(ctypes.byref(_pot(kind, sigma, eps, rcut, alpha, bh, r_att_min)), float(kT),
+                                    ctypes.byref(out)))
+    return out.value
+
+
+def potential_watt(r, kind=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, bh=False, r_att_min=0.0):
+    """Potential1::Watt at the distances r (cdft_potential_watt)."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    w = np.empty_like(r)
+    _check(lib().cdft_potential_watt(ctypes.byref(_pot(kind, sigma, eps, rcut, alpha, bh, r_att_min)), r.size, _ptr(r),
+                                     _ptr(w)))
+    return w
 
 
 def lj_hsd(kT, sigma=1.0, eps=1.0, rcut=2.5):

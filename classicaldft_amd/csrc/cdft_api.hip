@@ -28,6 +28,31 @@ void validate(const cdft_system_desc& d) {
     REQUIRE(d.fmt_weights || d.hsd >= std::min(std::min(d.dx, d.dy), d.dz), "hsd is less than the lattice spacing");
   }
   if (d.mean_field) REQUIRE(d.mf_kernel || (d.kT > 0 && d.lj_sigma > 0 && d.lj_rcut > 0), "bad mean-field parameters");
+  if (d.mean_field && !d.mf_kernel) {
+    REQUIRE(d.mf_potential >= CDFT_POT_LJ && d.mf_potential <= CDFT_POT_WHDF, "unknown mean-field potential");
+    REQUIRE(d.mf_scheme >= CDFT_MF_QF && d.mf_scheme <= CDFT_MF_GAUSS_F, "unknown mean-field kernel rule");
+    REQUIRE((d.mf_scheme != CDFT_MF_GAUSS_E && d.mf_scheme != CDFT_MF_GAUSS_F) ||
+                (d.mf_ngauss >= 1 && d.mf_ngauss <= kMaxRule),
+            "Gauss kernel rules need 1..32 points per axis");
+    REQUIRE(d.mf_potential != CDFT_POT_WHDF || d.lj_rcut > d.lj_sigma, "WHDF needs rcut > sigma");
+  }
+}
+
+Pot pot_from(const cdft_potential_desc& p) {
+  REQUIRE(p.kind >= CDFT_POT_LJ && p.kind <= CDFT_POT_WHDF, "unknown potential");
+  return make_pot(p.kind, p.sigma, p.eps, p.rcut, p.alpha, p.bh, p.r_att_min);
+}
+
+// The system's mean-field kernel w_att (N reals, / kT) into out on the system's stream.
+void launch_mf_kernel(cdft_system_s& s, double* out) {
+  const cdft_system_desc& d = s.d;
+  const Pot v = make_pot(d.mf_potential, d.lj_sigma, d.lj_eps, d.lj_rcut, d.mf_alpha, d.mf_bh, d.mf_r_att_min);
+  const MFRule rule = make_rule(d.mf_scheme, d.mf_ngauss);
+  // offsets |i| <= 1 + int(Rc / dx) + 1 per axis (src/Interaction.cpp:194-196, 272-274)
+  const int lx = 2 + int(v.rcut / d.dx), ly = 2 + int(v.rcut / d.dy), lz = 2 + int(v.rcut / d.dz);
+  hipLaunchKernelGGL(k_mf_kernel, dim3(div_up(s.N, kThreads)), dim3(kThreads), 0, s.stream, s.g.Nx, s.g.Ny, s.g.Nz,
+                     d.dx, d.dy, d.dz, lx, ly, lz, v, rule, d.kT, out);
+  HIPCHK(hipGetLastError());
 }
 
 template <typename T>
@@ -72,20 +97,7 @@ void build_weight_spectra(cdft_system_s& s) {
     if (s.d.mf_kernel) {
       HIPCHK(hipMemcpyAsync(w.p, s.d.mf_kernel, sizeof(double) * N, hipMemcpyHostToDevice, s.stream));
     } else {
-      LJParams v;
-      v.sigma = s.d.lj_sigma;
-      v.eps = s.d.lj_eps;
-      v.rcut = s.d.lj_rcut;
-      auto vr = [&](double r) {
-        const double y = v.sigma / r, y3 = y * y * y, y6 = y3 * y3;
-        return 4 * v.eps * (y6 * y6 - y6);
-      };
-      v.shift = v.rcut < 0 ? 0.0 : vr(v.rcut);  // include/Potential1.h:142-145
-      v.rmin = std::pow(2.0, 1.0 / 6.0) * v.sigma;
-      v.vmin = vr(v.rmin) - v.shift;
-      const int lx = 2 + int(v.rcut / s.d.dx), ly = 2 + int(v.rcut / s.d.dy), lz = 2 + int(v.rcut / s.d.dz);
-      hipLaunchKernelGGL(k_mf_kernel, dim3(div_up(N, kThreads)), dim3(kThreads), 0, s.stream, s.g.Nx, s.g.Ny, s.g.Nz,
-                         s.d.dx, s.d.dy, s.d.dz, lx, ly, lz, v, s.d.kT, w.as<double>());
+      launch_mf_kernel(s, w.as<double>());
       HIPCHK(hipGetLastError());
     }
     // a_vdw = sum_R w_att(R) (src/Interaction.cpp:290-296, 42-46)
@@ -807,20 +819,42 @@ int cdft_memcpy_d2h(void* dst, const void* src, unsigned long long bytes) {
 }
 
 
-int cdft_lj_hsd(double sigma, double eps, double rcut, double kT, double* hsd) {
+int cdft_potential_hsd(const cdft_potential_desc* p, double kT, double* hsd) {
   return guarded([&] {
-    REQUIRE(hsd && sigma > 0 && kT > 0, "bad arguments");
-    auto vr = [&](double r) {
-      const double y = sigma / r, y3 = y * y * y, y6 = y3 * y3;
-      return 4 * eps * (y6 * y6 - y6);
-    };
-    const double shift = rcut < 0 ? 0.0 : vr(rcut);  // include/Potential1.h:142-145
-    const double rmin = std::pow(2.0, 1.0 / 6.0) * sigma;
-    const double vmin = vr(rmin) - shift;
-    auto f = [&](double r) { return r <= 0 ? 1.0 : 1.0 - std::exp(-((vr(r) - shift) - vmin) / kT); };
-    *hsd = gk15(f, 0.0, rmin, 1e-13);
+    REQUIRE(p && hsd && p->sigma > 0 && kT > 0, "bad arguments");
+    const Pot v = pot_from(*p);
+    const double hc = v.hard_core(), rlimit = v.bh ? v.r0 : v.rmin;
+    auto f = [&](double r) { return r <= 0 ? 1.0 : 1.0 - std::exp(-v.V0(r) / kT); };
+    *hsd = hc + (rlimit > hc ? gk15(f, hc, rlimit, 1e-13) : 0.0);
     return (int)CDFT_OK;
   });
+}
+
+int cdft_potential_vdw(const cdft_potential_desc* p, double kT, double* a) {
+  return guarded([&] {
+    REQUIRE(p && a && p->sigma > 0 && kT > 0, "bad arguments");
+    const Pot v = pot_from(*p);
+    auto f = [&](double r) { return r * r * v.watt2(r * r); };
+    double s = 0.0;
+    if (v.bh) s = gk15(f, v.r0, v.rcut, 1e-14);
+    else s = gk15(f, v.r_att_min, v.rmin, 1e-14) + gk15(f, v.rmin, v.rcut, 1e-14);
+    *a = (2 * M_PI / kT) * s;
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_potential_watt(const cdft_potential_desc* p, int n, const double* r, double* w) {
+  return guarded([&] {
+    REQUIRE(p && r && w && n >= 0, "bad arguments");
+    const Pot v = pot_from(*p);
+    for (int i = 0; i < n; ++i) w[i] = v.watt2(r[i] * r[i]);
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_lj_hsd(double sigma, double eps, double rcut, double kT, double* hsd) {
+  const cdft_potential_desc p{CDFT_POT_LJ, sigma, eps, rcut, 0.0, 0, 0.0};
+  return cdft_potential_hsd(&p, kT, hsd);
 }
 
 int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
@@ -896,23 +930,9 @@ int cdft_system_get_mf_kernel(cdft_system sys, double* w) {
   return guarded([&] {
     REQUIRE(sys && w, "null argument");
     REQUIRE(sys->d.mean_field, "system has no mean field");
-    const cdft_system_desc& d = sys->d;
-    LJParams v;
-    v.sigma = d.lj_sigma;
-    v.eps = d.lj_eps;
-    v.rcut = d.lj_rcut;
-    auto vr = [&](double r) {
-      const double y = v.sigma / r, y3 = y * y * y, y6 = y3 * y3;
-      return 4 * v.eps * (y6 * y6 - y6);
-    };
-    v.shift = v.rcut < 0 ? 0.0 : vr(v.rcut);
-    v.rmin = std::pow(2.0, 1.0 / 6.0) * v.sigma;
-    v.vmin = vr(v.rmin) - v.shift;
-    const int lx = 2 + int(v.rcut / d.dx), ly = 2 + int(v.rcut / d.dy), lz = 2 + int(v.rcut / d.dz);
     DevBuf buf;
     buf.alloc(sizeof(double) * sys->N);
-    hipLaunchKernelGGL(k_mf_kernel, dim3(div_up(sys->N, kThreads)), dim3(kThreads), 0, sys->stream, sys->g.Nx,
-                       sys->g.Ny, sys->g.Nz, d.dx, d.dy, d.dz, lx, ly, lz, v, d.kT, buf.as<double>());
+    launch_mf_kernel(*sys, buf.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(w, buf.p, sizeof(double) * sys->N, hipMemcpyDeviceToHost, sys->stream));
     HIPCHK(hipStreamSynchronize(sys->stream));

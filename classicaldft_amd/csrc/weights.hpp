@@ -17,6 +17,7 @@
 // One thread evaluates one cell (Sx,Sy,Sz) of the positive octant; a second
 // kernel gathers the signed octant images onto the periodic grid.
 #pragma once
+#include "potential.hpp"
 #include <hip/hip_runtime.h>
 
 namespace cdft {
@@ -213,30 +214,15 @@ static __global__ void k_fmt_gather(int Nx, int Ny, int Nz, int Sxm, int Sym, in
 
 // LJ with WCA split: Potential1::Watt2 (include/Potential1.h:56-65) for LJ
 // (:136-189), bhFlag = false, r_att_min = 0.
-struct LJParams {
-  double sigma, eps, rcut, shift, rmin, vmin;
-};
-
-__device__ __forceinline__ double lj_watt2(const LJParams& v, double r2) {
-  if (!(r2 < v.rcut * v.rcut)) return 0.0;
-  if (r2 < v.rmin * v.rmin) return v.vmin;
-  const double y2 = v.sigma * v.sigma / r2;
-  const double y6 = y2 * y2 * y2;
-  return 4 * v.eps * (y6 * y6 - y6) - v.shift;
-}
-
-// Quadratic-force interpolation kernel (Interaction_Interpolation_QF,
-// include/Interaction.h:416-431; Interaction_Interpolation::generateWeight,
-// src/Interaction.cpp:522-534): w(S) = dx^2 dy^2 dz^2 sum_{ijk} (1/27)
-// Watt2(|S + pt|^2), pt in {-1/2, 0, 1/2}.  Gathered onto the periodic grid
-// over offsets |i| <= lim (src/Interaction.cpp:178-307), divided by dV and kT.
-static __global__ void k_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy, double dz, int lx, int ly, int lz, LJParams v,
-                            double kT, double* __restrict__ out) {
+// Lattice mean-field kernel (Interaction_Base::generateWeightsXYZSym, src/Interaction.cpp:178-307, and
+// initialize :32-52): w(S) = dx^2 dy^2 dz^2 generateWeight(S) with the offsets sorted |Sx| >= |Sy| >= |Sz| as the
+// reference tabulates them, gathered onto the periodic grid over |offset| <= lim per axis, divided by dV and kT.
+static __global__ void k_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy, double dz, int lx, int ly, int lz, Pot v,
+                                   MFRule rule, double kT, double* __restrict__ out) {
   const long N = (long)Nx * Ny * Nz;
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= N) return;
   const int z = p % Nz, y = (p / Nz) % Ny, x = p / ((long)Nz * Ny);
-  const double pt[3] = {-0.5, 0.0, 0.5};
   const double gf = dx * dx * dy * dy * dz * dz;
   const double dV = dx * dy * dz;
   double acc = 0.0;
@@ -247,16 +233,11 @@ static __global__ void k_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy,
   for (int ix = ix0; ix <= lx; ix += Nx)
     for (int iy = iy0; iy <= ly; iy += Ny)
       for (int iz = iz0; iz <= lz; iz += Nz) {
-        const int Sx = ix < 0 ? -ix : ix, Sy = iy < 0 ? -iy : iy, Sz = iz < 0 ? -iz : iz;
-        double sum = 0.0;
-        for (int i = 0; i < 3; ++i)
-          for (int j = 0; j < 3; ++j)
-            for (int k = 0; k < 3; ++k) {
-              const double r2 = (Sx + pt[i]) * (Sx + pt[i]) * dx * dx + (Sy + pt[j]) * (Sy + pt[j]) * dy * dy +
-                                (Sz + pt[k]) * (Sz + pt[k]) * dz * dz;
-              sum += (1.0 / 27.0) * lj_watt2(v, r2);
-            }
-        acc += gf * sum / dV;
+        int a = ix < 0 ? -ix : ix, b = iy < 0 ? -iy : iy, c = iz < 0 ? -iz : iz, t;
+        if (b > a) { t = a; a = b; b = t; }
+        if (c > a) { t = a; a = c; c = t; }
+        if (c > b) { t = b; b = c; c = t; }
+        acc += gf * mf_cell_weight(rule, v, a, b, c, dx, dy, dz) / dV;
       }
   out[p] = acc / kT;
 }

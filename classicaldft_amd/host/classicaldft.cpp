@@ -208,28 +208,95 @@ double FMT::BulkMuex(const std::vector<double>& x, const std::vector<Species*>& 
 }
 
 // --------------------------------------------------------------- Potential
-double Potential1::Watt(double r) const {  // include/Potential1.h:56-65, WCA branch
-  if (!(r < rcut_)) return 0.0;
-  if (r < rmin_) return Vmin_;
-  return V(r);
+cdft_potential_desc Potential1::desc() const {
+  cdft_potential_desc p{};
+  p.kind = kind();
+  p.sigma = sigma_;
+  p.eps = eps_;
+  p.rcut = rcut_;
+  p.alpha = alpha();
+  p.bh = bh_ ? 1 : 0;
+  p.r_att_min = r_att_min_;
+  return p;
+}
+
+double Potential1::Watt(double r) const {
+  const cdft_potential_desc p = desc();
+  double w = 0.0;
+  check(cdft_potential_watt(&p, 1, &r, &w));
+  return w;
+}
+
+double Potential1::V0(double r) const {  // include/Potential1.h:77-81
+  if (bh_) return r < r0_ ? V(r) : 0.0;
+  return r < rmin_ ? V(r) - Vmin_ : 0.0;
 }
 
 double Potential1::getHSD(double kT) const {
+  const cdft_potential_desc p = desc();
   double d = 0.0;
-  check(cdft_lj_hsd(sigma_, eps_, rcut_, kT, &d));
+  check(cdft_potential_hsd(&p, kT, &d));
   return d;
 }
 
-LJ::LJ(double sigma, double eps, double rcut) : Potential1(sigma, eps, rcut) {  // include/Potential1.h:140-146
+double Potential1::getVDW_Parameter(double kT) const {
+  const cdft_potential_desc p = desc();
+  double a = 0.0;
+  check(cdft_potential_vdw(&p, kT, &a));
+  return a;
+}
+
+LJ::LJ(double sigma, double eps, double rcut) : Potential1(sigma, eps, rcut) {  // include/Potential1.h:140-145
   shift_ = rcut_ < 0 ? 0.0 : vr(rcut_);
   rmin_ = getRmin();
   Vmin_ = V(rmin_);
+  r0_ = std::pow(0.5 * std::sqrt(1 + shift_) + 0.5, -1.0 / 6.0);
+}
+
+tWF::tWF(double sigma, double eps, double rcut, double alpha) : Potential1(sigma, eps, rcut), alpha_(alpha) {
+  shift_ = rcut_ <= 0.0 ? 0.0 : vr(rcut_);  // include/Potential1.h:199-206
+  rmin_ = getRmin();
+  Vmin_ = V(rmin_);
+  r0_ = std::sqrt(1 + std::pow(25 * std::sqrt(1 + shift_) + 25, -1.0 / 3.0));
+}
+
+double tWF::vr(double r) const {  // include/Potential1.h:234-243
+  if (r < sigma_) return 1e50;
+  const double s = r / sigma_, y = 1.0 / (s * s - 1), y3 = y * y * y;
+  return (4 * eps_ / (alpha_ * alpha_)) * (y3 * y3 - alpha_ * y3);
+}
+
+WHDF::WHDF(double sigma, double eps, double rcut) : Potential1(sigma, eps, rcut) {  // include/Potential1.h:264-271
+  eps_eff_ = eps * 2 * std::pow(rcut / sigma, 2) * std::pow(2 * ((rcut / sigma) * (rcut / sigma) - 1) / 3.0, -3.0);
+  shift_ = 0.0;
+  rmin_ = getRmin();
+  Vmin_ = V(rmin_);
+  r0_ = 1.0;
+}
+
+double WHDF::vr(double r) const {  // include/Potential1.h:298-303
+  const double y = sigma_ / r, z = rcut_ / r;
+  return eps_eff_ * (y * y - 1) * (z * z - 1) * (z * z - 1);
 }
 
 // ------------------------------------------------------------- Interaction
-Interaction_Interpolation_QF::Interaction_Interpolation_QF(Species* s1, Species* s2, Potential1* v, double kT)
-    : Interaction_Base(s1, s2, v, kT) {
+Interaction_Base::Interaction_Base(Species* s1, Species* s2, Potential1* v, double kT, int scheme, int ngauss)
+    : s1_(s1), s2_(s2), v_(v), kT_(kT), scheme_(scheme), ngauss_(ngauss) {
   if (s1 != s2) throw std::runtime_error("only the single-species (s1 == s2) mean field is implemented");
+}
+
+void Interaction_Base::describe(cdft_system_desc& d) const {
+  d.mean_field = 1;
+  d.lj_sigma = v_->getSigma();
+  d.lj_eps = v_->getEps();
+  d.lj_rcut = v_->getRcut();
+  d.kT = kT_;
+  d.mf_potential = v_->kind();
+  d.mf_alpha = v_->alpha();
+  d.mf_bh = v_->isBH() ? 1 : 0;
+  d.mf_r_att_min = v_->isBH() ? 0.0 : v_->getWCALimit();
+  d.mf_scheme = scheme_;
+  d.mf_ngauss = ngauss_;
 }
 
 double Interaction_Base::getVDWParameter() const {
@@ -243,11 +310,7 @@ double Interaction_Base::getVDWParameter() const {
     d.dy = L.getDY();
     d.dz = L.getDZ();
     d.model = CDFT_MODEL_NONE;
-    d.mean_field = 1;
-    d.lj_sigma = v_->getSigma();
-    d.lj_eps = v_->getEps();
-    d.lj_rcut = v_->getRcut();
-    d.kT = kT_;
+    describe(d);
     cdft_system s = nullptr;
     check(cdft_system_create(&d, &s));
     check(cdft_system_a_vdw(s, &a_vdw_));
@@ -276,7 +339,6 @@ void DFT::addSpecies(Species* s) {
 }
 void DFT::addInteraction(Interaction_Base* i) {
   if (!interactions_.empty()) throw std::runtime_error("one mean-field interaction is supported");
-  if (!dynamic_cast<LJ*>(i->potential())) throw std::runtime_error("the GPU mean field implements the LJ potential");
   interactions_.push_back(i);
   invalidate();
 }
@@ -310,14 +372,7 @@ cdft_system DFT::system() const {
   d.ideal = 1;
   d.mu = mu;
   d.vext = D.has_external_field() ? D.get_external_field().memptr() : nullptr;
-  if (!interactions_.empty()) {
-    const Interaction_Base* i = interactions_.front();
-    d.mean_field = 1;
-    d.lj_sigma = i->potential()->getSigma();
-    d.lj_eps = i->potential()->getEps();
-    d.lj_rcut = i->potential()->getRcut();
-    d.kT = i->kT();
-  }
+  if (!interactions_.empty()) interactions_.front()->describe(d);
   check(cdft_system_create(&d, &sys_));
   sys_mu_ = mu;
   sys_field_ = D.field_version();

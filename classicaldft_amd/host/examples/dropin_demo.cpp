@@ -5,6 +5,8 @@
 //   mu <value>              DFT::mu_times_beta(0.5) with LJ mean field, kT = 1
 //   fire <F> <fmax_raw>     after 20 fireMinimizer2 steps
 //   ddft <F> <mass>         after 3 DDFT steps
+//   mu_twf_bh_gauss_f <mu> <hsd> <vdw>  tWF (BH split) on a Gauss_F kernel, kT = 1.2
+//   avdw_whdf_le <a>        lattice vdW parameter of WHDF on the LE kernel, kT = 0.9
 //   hess <sum Hv> <v.Hv>    DFT::matrix_dot_v1 of a seeded direction (16^3, ideal + WB I + LJ/QF)
 //   hess_fmt <sum> <v.Hv>   FMT::add_second_derivative of the same direction
 #include <cstdint>
@@ -62,6 +64,23 @@ int main() {
   for (int i = 0; i < 3; ++i) ddft.step();
   std::printf("ddft %.17g %.17g\n", ddft.getF(), ddft.get_mass());
   std::printf("hsd_lj_kT0.8 %.17g\n", lj.getHSD(0.8));
+
+  // other potentials and kernel rules: tWF with the BH split on a 3-point Gauss_F kernel; WHDF on LE
+  Density d5(0.1, L2);
+  d5.set(0.5);
+  FMT_Species s5(d5, 1.0, 0.0);
+  tWF twf(1.0, 1.0, 2.0, 50.0);
+  twf.setBH();
+  Interaction_Gauss_F gf(&s5, &s5, &twf, 1.2, 3);
+  WhiteBearI fmt5;
+  DFT dft5(&s5);
+  dft5.addHardCoreContribution(&fmt5);
+  dft5.addInteraction(&gf);
+  std::printf("mu_twf_bh_gauss_f %.17g %.17g %.17g\n", dft5.mu_times_beta(0.5), twf.getHSD(1.2),
+              twf.getVDW_Parameter(1.2));
+  WHDF whdf(1.0, 1.0, 2.0);
+  Interaction_Interpolation_LE le(&s5, &s5, &whdf, 0.9);
+  std::printf("avdw_whdf_le %.17g\n", le.getVDWParameter());
 
   // Hessian-vector products (DFT::matrix_dot_v1 and FMT::add_second_derivative) on a 16^3 system
   Density d4(0.1, L2);

@@ -48,6 +48,24 @@ enum cdft_precision {
                      ideal gas, reductions and solver state in double */
 };
 
+/* Pair potential of the mean field (include/Potential1.h) */
+enum cdft_potential {
+  CDFT_POT_LJ = 0,   /* Lennard-Jones, cut and shifted (:136-189) */
+  CDFT_POT_TWF = 1,  /* ten Wolde-Frenkel (:195-257), range parameter alpha */
+  CDFT_POT_WHDF = 2  /* Wang-Ramirez-Dobnikar-Frenkel (:262-324) */
+};
+
+/* Lattice kernel rule of the mean field (include/Interaction.h:130-431) */
+enum cdft_mf_scheme {
+  CDFT_MF_QF = 0,       /* Interaction_Interpolation_QF (default) */
+  CDFT_MF_ZERO = 1,     /* Interaction_Interpolation_Zero */
+  CDFT_MF_LE = 2,       /* Interaction_Interpolation_LE */
+  CDFT_MF_QE = 3,       /* Interaction_Interpolation_QE */
+  CDFT_MF_LF = 4,       /* Interaction_Interpolation_LF */
+  CDFT_MF_GAUSS_E = 5,  /* Interaction_Gauss_E (mf_ngauss points) */
+  CDFT_MF_GAUSS_F = 6   /* Interaction_Gauss_F (mf_ngauss points) */
+};
+
 enum cdft_flags {
   CDFT_DEVICE_PTRS = 1, /* rho / dF / vext are device pointers */
   CDFT_ACCUMULATE = 2,  /* add into dF instead of overwriting it */
@@ -73,7 +91,22 @@ typedef struct cdft_system_desc {
   int mean_field;         /* include the LJ mean field (src/Interaction.cpp:76-115) */
   double lj_sigma, lj_eps, lj_rcut, kT; /* LJ with WCA split, include/Potential1.h:136-189 */
   const double* mf_kernel; /* host, N values of w_att (already / kT), or NULL to generate */
+  /* mean-field potential and kernel rule; all zero = LJ with the WCA split, quadratic-force interpolation */
+  int mf_potential;       /* enum cdft_potential (lj_sigma / lj_eps / lj_rcut are its sigma, eps, rcut) */
+  double mf_alpha;        /* tWF range parameter (0: 50, the reference's default) */
+  int mf_bh;              /* 1: Barker-Henderson split (Potential1::setBH) */
+  double mf_r_att_min;    /* WCA continuation limit (Potential1::set_WCA_limit), 0 by default */
+  int mf_scheme;          /* enum cdft_mf_scheme */
+  int mf_ngauss;          /* Gauss-Legendre points per axis of CDFT_MF_GAUSS_E / _F (1..32) */
 } cdft_system_desc;
+
+/* A pair potential for the host-side helpers below (fields as in cdft_system_desc). */
+typedef struct cdft_potential_desc {
+  int kind;               /* enum cdft_potential */
+  double sigma, eps, rcut, alpha;
+  int bh;
+  double r_att_min;
+} cdft_potential_desc;
 
 const char* cdft_last_error(void);
 int cdft_device_count(int* n);
@@ -129,6 +162,16 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu);
    Potential1::getHSD (include/Potential1.h:73-83; GSL qags there, adaptive
    Gauss-Kronrod 7/15 to 1e-13 absolute here).  Host-only. */
 int cdft_lj_hsd(double sigma, double eps, double rcut, double kT, double* hsd);
+
+/* Potential1::getHSD (include/Potential1.h:73-83) for any potential and split:
+   hard core + int_hc^rlimit (1 - exp(-V0(r)/kT)) dr, rlimit = rmin (WCA) or
+   r0 (BH).  Adaptive Gauss-Kronrod 7/15 to 1e-13.  Host-only. */
+int cdft_potential_hsd(const cdft_potential_desc* p, double kT, double* hsd);
+/* Potential1::getVDW_Parameter (include/Potential1.h:85-91): (2 pi / kT) int r^2
+   Watt(r) dr over [r_att_min, rmin] + [rmin, rcut] (WCA) or [r0, rcut] (BH). */
+int cdft_potential_vdw(const cdft_potential_desc* p, double kT, double* a);
+/* Potential1::Watt at n distances (host). */
+int cdft_potential_watt(const cdft_potential_desc* p, int n, const double* r, double* w);
 
 /* ---------------------------------------------------------------- FIRE2 ---
    fireMinimizer2 (include/Minimizer.h:86-182, src/Minimizer.cpp:117-327) on

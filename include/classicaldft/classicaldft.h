@@ -226,25 +226,38 @@ class Potential1 {
   virtual ~Potential1() = default;
   double getRcut() const { return rcut_; }
   double getSigma() const { return sigma_; }
-  double getEps() const { return eps_; }
+  double getEps() const { return eps_; }  // the constructor's eps (WHDF rescales it internally)
+  double getR0() const { return r0_; }
   virtual double getRmin() const = 0;
+  virtual double getHardCore() const { return 0.0; }
+  void set_WCA_limit(double r) { r_att_min_ = r; }           // Potential1.h:46
+  void setBH() { bh_ = true; r_att_min_ = getR0(); }         // Potential1.h:47
   double V(double r) const { return vr(r) - shift_; }
-  double Watt(double r) const;
-  double V0(double r) const { return r < rmin_ ? V(r) - Vmin_ : 0.0; }
-  // hard-sphere diameter int_0^rmin (1 - exp(-V0/kT)) dr (Potential1.h:73-83),
-  // adaptive Gauss-Kronrod to 1e-13 instead of GSL qags at 1e-4.
+  double Watt(double r) const;                               // Potential1.h:49, 67-75
+  double V0(double r) const;                                 // Potential1.h:77-81
+  // Potential1.h:73-91: adaptive Gauss-Kronrod to 1e-13 instead of GSL qags at 1e-4 / 1e-6
   double getHSD(double kT) const;
+  double getVDW_Parameter(double kT) const;
+  // the C ABI description of this potential and split
+  cdft_potential_desc desc() const;
+  virtual int kind() const = 0;
+  virtual double alpha() const { return 0.0; }
+  bool isBH() const { return bh_; }
+  double getWCALimit() const { return r_att_min_; }
 
  protected:
   virtual double vr(double r) const = 0;
   double sigma_, eps_, rcut_;
-  double shift_ = 0, rmin_ = 1, Vmin_ = -1;
+  double shift_ = 0, rmin_ = 1, Vmin_ = -1, r0_ = 1, r_att_min_ = 0;
+  bool bh_ = false;
 };
 
+// include/Potential1.h:136-189
 class LJ : public Potential1 {
  public:
   LJ(double sigma, double eps, double rcut);
   double getRmin() const override { return std::pow(2.0, 1.0 / 6.0) * sigma_; }
+  int kind() const override { return CDFT_POT_LJ; }
 
  protected:
   double vr(double r) const override {
@@ -253,11 +266,37 @@ class LJ : public Potential1 {
   }
 };
 
+// include/Potential1.h:195-257
+class tWF : public Potential1 {
+ public:
+  tWF(double sigma, double eps, double rcut, double alpha = 50.0);
+  double getRmin() const override { return sigma_ * std::sqrt(1 + std::pow(2.0 / alpha_, 1.0 / 3.0)); }
+  double getHardCore() const override { return sigma_; }
+  int kind() const override { return CDFT_POT_TWF; }
+  double alpha() const override { return alpha_; }
+
+ protected:
+  double vr(double r) const override;
+  double alpha_;
+};
+
+// include/Potential1.h:262-324
+class WHDF : public Potential1 {
+ public:
+  WHDF(double sigma, double eps, double rcut);
+  double getRmin() const override { return rcut_ * std::pow((1.0 + 2.0 * (rcut_ / sigma_) * (rcut_ / sigma_)) / 3.0, -0.5); }
+  int kind() const override { return CDFT_POT_WHDF; }
+
+ protected:
+  double vr(double r) const override;
+  double eps_eff_;
+};
+
 // ----------------------------------------------------------- interaction ---
-// include/Interaction.h:33-431: the quadratic-force interpolation kernel.
+// include/Interaction.h:33-431: lattice mean-field kernels (single species, s1 == s2).
 class Interaction_Base {
  public:
-  Interaction_Base(Species* s1, Species* s2, Potential1* v, double kT) : s1_(s1), s2_(s2), v_(v), kT_(kT) {}
+  Interaction_Base(Species* s1, Species* s2, Potential1* v, double kT, int scheme = CDFT_MF_QF, int ngauss = 0);
   virtual ~Interaction_Base() = default;
   double getVDWParameter() const;  // a_vdw / kT of the lattice kernel
   double Mu(const std::vector<double>& x, int species) const;
@@ -265,17 +304,51 @@ class Interaction_Base {
   Species* species1() const { return s1_; }
   Potential1* potential() const { return v_; }
   double kT() const { return kT_; }
+  // the mean-field fields of a system description (potential, split, rule, kT)
+  void describe(cdft_system_desc& d) const;
 
  protected:
   Species *s1_, *s2_;
   Potential1* v_;
   double kT_;
+  int scheme_, ngauss_;
   mutable double a_vdw_ = NAN;
 };
 
-class Interaction_Interpolation_QF : public Interaction_Base {
+class Interaction_Interpolation_QF : public Interaction_Base {  // Interaction.h:416-431
  public:
-  Interaction_Interpolation_QF(Species* s1, Species* s2, Potential1* v, double kT);
+  Interaction_Interpolation_QF(Species* s1, Species* s2, Potential1* v, double kT)
+      : Interaction_Base(s1, s2, v, kT, CDFT_MF_QF) {}
+};
+class Interaction_Interpolation_Zero : public Interaction_Base {  // Interaction.h:293-313
+ public:
+  Interaction_Interpolation_Zero(Species* s1, Species* s2, Potential1* v, double kT)
+      : Interaction_Base(s1, s2, v, kT, CDFT_MF_ZERO) {}
+};
+class Interaction_Interpolation_LE : public Interaction_Base {  // Interaction.h:318-343
+ public:
+  Interaction_Interpolation_LE(Species* s1, Species* s2, Potential1* v, double kT)
+      : Interaction_Base(s1, s2, v, kT, CDFT_MF_LE) {}
+};
+class Interaction_Interpolation_QE : public Interaction_Base {  // Interaction.h:348-373
+ public:
+  Interaction_Interpolation_QE(Species* s1, Species* s2, Potential1* v, double kT)
+      : Interaction_Base(s1, s2, v, kT, CDFT_MF_QE) {}
+};
+class Interaction_Interpolation_LF : public Interaction_Base {  // Interaction.h:378-403
+ public:
+  Interaction_Interpolation_LF(Species* s1, Species* s2, Potential1* v, double kT)
+      : Interaction_Base(s1, s2, v, kT, CDFT_MF_LF) {}
+};
+class Interaction_Gauss_E : public Interaction_Base {  // Interaction.h:168-201, src/Interaction.cpp:482-520
+ public:
+  Interaction_Gauss_E(Species* s1, Species* s2, Potential1* v, double kT, int Ngauss)
+      : Interaction_Base(s1, s2, v, kT, CDFT_MF_GAUSS_E, Ngauss) {}
+};
+class Interaction_Gauss_F : public Interaction_Base {  // Interaction.h:203-236, src/Interaction.cpp:465-480
+ public:
+  Interaction_Gauss_F(Species* s1, Species* s2, Potential1* v, double kT, int Ngauss)
+      : Interaction_Base(s1, s2, v, kT, CDFT_MF_GAUSS_F, Ngauss) {}
 };
 
 // ------------------------------------------------------------------- DFT ---

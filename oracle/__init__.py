@@ -53,6 +53,10 @@ def lib():
         L.orc_fmt_destroy.argtypes = [_p]
         L.orc_mf_kernel.argtypes = [_i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _p]
         L.orc_mf_kernel.restype = _d
+        L.orc_mf_kernel2.argtypes = [_i, _i, _i, _d, _d, _d, _i, _d, _d, _d, _d, _i, _d, _i, _i, _p, _p, _d, _p]
+        L.orc_mf_kernel2.restype = _d
+        L.orc_dft_set_mf.argtypes = [_p, _p, _d]
+        L.orc_dft_set_mf.restype = None
         L.orc_dft_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _i, _i, _i, _d, _d, _d, _d, _d, _p]
         L.orc_dft_create.restype = _p
         L.orc_dft_destroy.argtypes = [_p]
@@ -201,8 +205,37 @@ def mf_kernel(shape, dx, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0):
     return w, a
 
 
+POT_LJ, POT_TWF, POT_WHDF = 0, 1, 2
+MF_QF, MF_ZERO, MF_LE, MF_QE, MF_LF, MF_GAUSS_E, MF_GAUSS_F = range(7)
+
+
+def gauss_legendre01(n):
+    """Gauss-Legendre points and weights on [0, 1] from numpy (the reference takes them from GSL's
+    gsl_integration_glfixed_table, src/Interaction.h:133-143)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def mf_kernel2(shape, dx, kind=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, bh=False, r_att_min=0.0,
+               scheme=MF_QF, ngauss=0, kT=1.0):
+    """Mean-field kernel w_att (/ kT) and a_vdw / kT for any potential, split and kernel rule."""
+    Nx, Ny, Nz = shape
+    dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+    gp, gw = gauss_legendre01(ngauss) if ngauss > 0 else (np.zeros(1), np.zeros(1))
+    gp, gw = np.ascontiguousarray(gp), np.ascontiguousarray(gw)
+    w = np.zeros(shape, dtype=np.float64)
+    a = lib().orc_mf_kernel2(Nx, Ny, Nz, dxs[0], dxs[1], dxs[2], int(kind), sigma, eps, rcut, alpha, int(bool(bh)),
+                             r_att_min, int(scheme), int(ngauss), _ptr(gp), _ptr(gw), kT, _ptr(w))
+    return w, a
+
+
 class DFT:
     """Whole-DFT oracle (ideal + FMT + LJ/QF mean field + V_ext - mu)."""
+
+    def set_mean_field(self, w_att, a_vdw):
+        """Use another mean-field kernel (e.g. from mf_kernel2)."""
+        w = np.ascontiguousarray(w_att, dtype=np.float64)
+        lib().orc_dft_set_mf(self.h, _ptr(w), float(a_vdw))
 
     def __init__(self, shape, dx, hsd, model=WBI, A=1.0, B=0.0, fmt=True, ideal=True, mf=False, sigma=1.0,
                  eps=1.0, rcut=2.5, kT=1.0, mu=0.0, vext=None, accurate_weights=False):

@@ -967,6 +967,165 @@ double mf_kernel(const Grid& g, double dx, double dy, double dz, const LJ& v, do
   return a_vdw / kT;
 }
 
+// All three potentials of include/Potential1.h with the WCA or BH split, and every kernel rule of
+// include/Interaction.h / src/Interaction.cpp:443-534, tabulated at sorted offsets and scattered as
+// Interaction_Base::generateWeightsXYZSym does (src/Interaction.cpp:178-307).
+struct Pot1 {
+  int kind = 0;  // 0 LJ, 1 tWF, 2 WHDF
+  double sigma = 1, eps = 1, rcut = -1, alpha = 50, shift = 0, rmin = 1, Vmin = -1, r_att_min = 0, r0 = 1;
+  bool bh = false;
+  double vr(double r) const {
+    if (kind == 1) {  // tWF, Potential1.h:234-243
+      if (r < sigma) return 1e50;
+      const double q = r / sigma, y = 1.0 / (q * q - 1), y3 = y * y * y;
+      return (4 * eps / (alpha * alpha)) * (y3 * y3 - alpha * y3);
+    }
+    if (kind == 2) {  // WHDF, Potential1.h:298-303
+      const double y = sigma / r, z = rcut / r;
+      return eps * (y * y - 1) * (z * z - 1) * (z * z - 1);
+    }
+    const double y = sigma / r, y3 = y * y * y, y6 = y3 * y3;  // LJ, Potential1.h:170-175
+    return 4 * eps * (y6 * y6 - y6);
+  }
+  double vr2(double r2) const {
+    if (kind == 1) {  // Potential1.h:245-254
+      if (r2 < sigma * sigma) return 1e50;
+      const double s2 = r2 / (sigma * sigma), y = 1.0 / (s2 - 1), y3 = y * y * y;
+      return (4 * eps / (alpha * alpha)) * (y3 * y3 - alpha * y3);
+    }
+    if (kind == 2) {  // Potential1.h:304-309
+      const double y2 = sigma * sigma / r2, z2 = rcut * rcut / r2;
+      return eps * (y2 - 1) * (z2 - 1) * (z2 - 1);
+    }
+    const double y2 = sigma * sigma / r2, y6 = y2 * y2 * y2;  // Potential1.h:176-181
+    return 4 * eps * (y6 * y6 - y6);
+  }
+  double V(double r) const { return vr(r) - shift; }
+  double V2(double r2) const { return vr2(r2) - shift; }
+  double Watt2(double r2) const {  // Potential1.h:67-75
+    double ret = 0.0;
+    if (r2 < rcut * rcut) {
+      if (bh) ret = (r2 < r0 * r0 ? 0.0 : V2(r2));
+      else if (r2 < r_att_min * r_att_min) ret = 0.0;
+      else if (r2 < rmin * rmin) ret = Vmin;
+      else ret = V2(r2);
+    }
+    return ret;
+  }
+  void init(int k, double s, double e, double rc, double a, bool bhf, double ratt) {
+    kind = k; sigma = s; eps = e; rcut = rc; alpha = a > 0 ? a : 50.0; r_att_min = ratt; bh = false;
+    if (k == 1) {  // tWF constructor, Potential1.h:199-206
+      shift = (rcut <= 0.0 ? 0.0 : vr(rcut));
+      rmin = sigma * std::sqrt(1 + std::pow(2.0 / alpha, 1.0 / 3.0));
+      Vmin = V(rmin);
+      r0 = std::sqrt(1 + std::pow(25 * std::sqrt(1 + shift) + 25, -1.0 / 3.0));
+    } else if (k == 2) {  // WHDF constructor, Potential1.h:264-271
+      eps *= 2 * std::pow(rc / s, 2) * std::pow(2 * ((rc / s) * (rc / s) - 1) / 3.0, -3.0);
+      shift = 0.0;
+      rmin = rcut * std::pow((1.0 + 2.0 * (rcut / sigma) * (rcut / sigma)) / 3.0, -0.5);
+      Vmin = V(rmin);
+      r0 = 1.0;
+    } else {  // LJ constructor, Potential1.h:140-145
+      shift = (rcut < 0 ? 0.0 : vr(rcut));
+      rmin = std::pow(2.0, 1.0 / 6.0) * sigma;
+      Vmin = V(rmin);
+      r0 = std::pow(0.5 * std::sqrt(1 + shift) + 0.5, -1.0 / 6.0);
+    }
+    if (bhf) { bh = true; r_att_min = r0; }  // setBH, Potential1.h:47
+  }
+};
+
+// generateWeight of one rule at offset S (src/Interaction.cpp:443-534); scheme 0 QF, 1 Zero, 2 LE, 3 QE, 4 LF,
+// 5 Gauss_E, 6 Gauss_F (gp, gw: the Gauss-Legendre points and weights on [0, 1]).
+double gen_weight(int scheme, const Pot1& v, int Sx, int Sy, int Sz, double dx, double dy, double dz, int ng,
+                  const double* gp, const double* gw) {
+  std::vector<double> vv, pt;
+  switch (scheme) {
+    case 1: vv = {1.0}; pt = {0.0}; break;  // Interaction.h:301
+    case 2: vv = {1.0, 26.0, 66.0, 26.0, 1.0}; pt = {-2.0, -1.0, 0.0, 1.0, 2.0}; for (auto& x : vv) x /= 120.0; break;
+    case 3:
+      vv = {-1.0, 8.0, 18.0, 112.0, 86.0, 112.0, 18.0, 8.0, -1.0};
+      pt = {-2.0, -1.5, -1.0, -0.5, 0.0, 0.5, 1.0, 1.5, 2.0};
+      for (auto& x : vv) x /= 360.0;
+      break;
+    case 4: vv = {1.0, 4.0, 1.0}; pt = {-1.0, 0.0, 1.0}; for (auto& x : vv) x /= 6; break;
+    case 5:
+    case 6: break;
+    default: vv = {1.0 / 3, 1.0 / 3, 1.0 / 3}; pt = {-0.5, 0.0, 0.5}; break;
+  }
+  double sum = 0.0;
+  if (scheme == 5 || scheme == 6) {
+    for (int Gx = 0; Gx < ng; Gx++)
+      for (int Gy = 0; Gy < ng; Gy++)
+        for (int Gz = 0; Gz < ng; Gz++) {
+          const double x = gp[Gx], y = gp[Gy], z = gp[Gz];
+          double k = 0.0;
+          if (scheme == 6) {  // Interaction_Gauss_F::getKernel, src/Interaction.cpp:465-480
+            for (int a = 0; a <= 1; a++)
+              for (int b = 0; b <= 1; b++)
+                for (int c = 0; c <= 1; c++) {
+                  const double r2 = (Sx + a - 1 + x) * (Sx + a - 1 + x) * dx * dx +
+                                    (Sy + b - 1 + y) * (Sy + b - 1 + y) * dy * dy +
+                                    (Sz + c - 1 + z) * (Sz + c - 1 + z) * dz * dz;
+                  k += (a == 0 ? x : 1 - x) * (b == 0 ? y : 1 - y) * (c == 0 ? z : 1 - z) * v.Watt2(r2);
+                }
+          } else {  // Interaction_Gauss_E::getKernel, src/Interaction.cpp:482-520
+            for (int a = -2; a <= 1; a++) {
+              const double fx = a == -2 ? x * x * x : a == -1 ? 1 + 3 * x * (1 + x * (1 - x))
+                                : a == 0 ? 4 + 3 * x * x * (x - 2) : (1 - x) * (1 - x) * (1 - x);
+              for (int b = -2; b <= 1; b++) {
+                const double fy = b == -2 ? y * y * y : b == -1 ? 1 + 3 * y * (1 + y * (1 - y))
+                                  : b == 0 ? 4 + 3 * y * y * (y - 2) : (1 - y) * (1 - y) * (1 - y);
+                for (int c = -2; c <= 1; c++) {
+                  const double fz = c == -2 ? z * z * z : c == -1 ? 1 + 3 * z * (1 + z * (1 - z))
+                                    : c == 0 ? 4 + 3 * z * z * (z - 2) : (1 - z) * (1 - z) * (1 - z);
+                  const double r2 = (Sx + a + x) * (Sx + a + x) * dx * dx + (Sy + b + y) * (Sy + b + y) * dy * dy +
+                                    (Sz + c + z) * (Sz + c + z) * dz * dz;
+                  k += fx * fy * fz * v.Watt2(r2);
+                }
+              }
+            }
+            k /= 216;
+          }
+          sum += gw[Gx] * gw[Gy] * gw[Gz] * k;
+        }
+    return sum;
+  }
+  for (size_t i = 0; i < vv.size(); i++)
+    for (size_t j = 0; j < vv.size(); j++)
+      for (size_t k = 0; k < vv.size(); k++) {
+        const double r2 = (Sx + pt[i]) * (Sx + pt[i]) * dx * dx + (Sy + pt[j]) * (Sy + pt[j]) * dy * dy +
+                          (Sz + pt[k]) * (Sz + pt[k]) * dz * dz;
+        sum += vv[i] * vv[j] * vv[k] * v.Watt2(r2);
+      }
+  return sum;
+}
+
+// Fills w_att (x 1/kT, zeroed first) and returns a_vdw / kT: generateWeightsXYZSym + initialize.
+double mf_kernel_general(const Grid& g, double dx, double dy, double dz, const Pot1& v, int scheme, int ng,
+                         const double* gp, const double* gw, double kT, double* w_att) {
+  std::fill(w_att, w_att + g.N(), 0.0);
+  const int Nx_lim = 1 + int(v.rcut / dx), Ny_lim = 1 + int(v.rcut / dy), Nz_lim = 1 + int(v.rcut / dz);
+  const double gf = dx * dx * dy * dy * dz * dz;
+  const double dV = dx * dy * dz;
+  double a_vdw = 0.0;
+  for (int ix = -Nx_lim - 1; ix <= Nx_lim + 1; ix++)
+    for (int iy = -Ny_lim - 1; iy <= Ny_lim + 1; iy++)
+      for (int iz = -Nz_lim - 1; iz <= Nz_lim + 1; iz++) {
+        int nx = std::abs(ix), ny = std::abs(iy), nz = std::abs(iz);
+        if (ny > nx) std::swap(nx, ny);
+        if (nz > nx) std::swap(nx, nz);
+        if (nz > ny) std::swap(ny, nz);
+        const double w = gf * gen_weight(scheme, v, nx, ny, nz, dx, dy, dz, ng, gp, gw);
+        const long pos = pbc(iz, g.Nz) + g.Nz * (pbc(iy, g.Ny) + g.Ny * pbc(ix, g.Nx));
+        w_att[pos] += w / dV;
+        a_vdw += w;
+      }
+  a_vdw /= dV;
+  for (long i = 0; i < g.N(); ++i) w_att[i] /= kT;
+  return a_vdw / kT;
+}
+
 // ---------------------------------------------------------------------------
 // Whole DFT evaluation: DFT::calculateFreeEnergyAndDerivatives, src/DFT.cpp:360-425,
 // Species::externalField src/Species.cpp:51-62, Interaction_Base::getInteractionEnergyAndForces
@@ -1172,6 +1331,16 @@ double orc_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy, double dz, do
   return mf_kernel(g, dx, dy, dz, v, kT, w_att);
 }
 
+// Any potential / split / kernel rule (gp, gw: ng Gauss-Legendre points and weights on [0, 1] for rules 5, 6).
+double orc_mf_kernel2(int Nx, int Ny, int Nz, double dx, double dy, double dz, int kind, double sigma, double eps,
+                      double rcut, double alpha, int bh, double r_att_min, int scheme, int ng, const double* gp,
+                      const double* gw, double kT, double* w_att) {
+  Grid g{Nx, Ny, Nz};
+  Pot1 v;
+  v.init(kind, sigma, eps, rcut, alpha, bh != 0, r_att_min);
+  return mf_kernel_general(g, dx, dy, dz, v, scheme, ng, gp, gw, kT, w_att);
+}
+
 // ---- stateful DFT / FIRE2 / DDFT objects (opaque handles) ----
 struct orc_dft;
 
@@ -1209,6 +1378,15 @@ orc_dft* orc_dft_create(int kind, double A, double B, int Nx, int Ny, int Nz, do
 void orc_dft_destroy(orc_dft* h) { delete reinterpret_cast<DftState*>(h); }
 
 double orc_dft_avdw(orc_dft* h) { return reinterpret_cast<DftState*>(h)->a_vdw; }
+
+// Replaces the DFT's mean-field kernel (real space, / kT) and a_vdw / kT, e.g. by one of orc_mf_kernel2.
+void orc_dft_set_mf(orc_dft* h, const double* w_att, double a_vdw) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  s->has_mf = true;
+  s->a_vdw = a_vdw;
+  s->wmf.resize(s->fmt.g.Nk());
+  r2c3d(s->fmt.g, w_att, s->wmf.data());
+}
 
 // out = (d2F/drho drho) v; full = 0 leaves out the ideal-gas term (DFT::full_hessian_ false).
 void orc_dft_hess(orc_dft* h, const double* rho, const double* v, double* out, int full) {

@@ -105,3 +105,13 @@ def test_fire_snapshot_file(demo):
     from classicaldft_amd import formats
     rho = formats.read_density(os.path.join(demo["_dir"], "snapshot_s0.dat"), (16, 16, 16))
     assert np.all(np.isfinite(rho)) and np.all(rho > 0)
+
+
+def test_other_potentials_through_cpp_api(demo):
+    """tWF / WHDF potentials, the BH split and the Gauss_F / LE kernel rules through the reference-named classes."""
+    w, a = oracle.mf_kernel2((16, 16, 16), 0.1, kind=oracle.POT_TWF, rcut=2.0, alpha=50.0, bh=True,
+                             scheme=oracle.MF_GAUSS_F, ngauss=3, kT=1.2)
+    _, muex = oracle.bulk(oracle.WBI, 0.5, 1.0)
+    assert rel(demo["mu_twf_bh_gauss_f"][0], np.log(0.5) + muex + a * 0.5) <= 1e-12
+    _, a_le = oracle.mf_kernel2((16, 16, 16), 0.1, kind=oracle.POT_WHDF, rcut=2.0, scheme=oracle.MF_LE, kT=0.9)
+    assert rel(demo["avdw_whdf_le"][0], a_le) <= 1e-12
