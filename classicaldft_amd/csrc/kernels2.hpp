@@ -55,6 +55,23 @@ struct Batch {
   const double* mu;  // null: use the scalar of the call
 };
 
+// Logical tile of this block such that consecutive tiles run on one XCD.  The hardware hands the blocks of a
+// launch to the 8 XCDs round-robin (block b to XCD b % 8), each XCD with its own L2; tiles that share cache lines
+// (neighbouring column groups, rows that straddle a line) then fetch every line once per XCD.  Giving XCD k the
+// contiguous tile range [k per + min(k, rem), ...) keeps neighbours on one L2.  A bijection of [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_tile() {
+  const unsigned n = gridDim.x, b = blockIdx.x;
+  const unsigned per = n >> 3, rem = n & 7u, k = b & 7u;
+  return k * per + (k < rem ? k : rem) + (b >> 3);
+}
+
+// Row kernels: FP32 half-spectrum rows (Zp x 8 bytes) straddle cache lines, so neighbouring rows go to one XCD;
+// FP64 rows are whole lines and keep the hardware order (measured faster for the pointwise pass).
+template <typename T>
+__device__ __forceinline__ unsigned row_tile() {
+  return sizeof(T) == 4 ? xcd_tile() : blockIdx.x;
+}
+
 struct ColGeo {
   unsigned bytes;  // bytes of one complex array (Nx * Ny * Zp elements)
   unsigned ncols;  // number of columns
@@ -72,7 +89,7 @@ __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long 
   constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
   __shared__ cplx<T> buf[L * NB];
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
-  const unsigned c0 = blockIdx.x * NB + b;
+  const unsigned c0 = xcd_tile() * NB + b;
   const bool ok = c0 < cg.ncols;
   const unsigned c = ok ? c0 : cg.ncols - 1;  // tail lanes recompute the last column, store nothing
   const auto rd = buf_rsrc(data + blockIdx.y * arr_stride, cg.bytes);
@@ -109,7 +126,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
   constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
   __shared__ cplx<T> buf[L * NB];
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
-  const unsigned c0 = blockIdx.x * NB + b;
+  const unsigned c0 = xcd_tile() * NB + b;
   const bool ok = c0 < cg.ncols;
   const unsigned c = ok ? c0 : cg.ncols - 1;  // tail lanes recompute the last column, store nothing
   constexpr unsigned E = sizeof(cplx<T>), EW = sizeof(T);
@@ -174,7 +191,7 @@ __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D
   constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
   __shared__ cplx<T> buf[L * NB];
   const int b = threadIdx.x % NB, t = threadIdx.x / NB;
-  const unsigned c0 = blockIdx.x * NB + b;
+  const unsigned c0 = xcd_tile() * NB + b;
   const bool ok = c0 < cg.ncols;
   const unsigned c = ok ? c0 : cg.ncols - 1;  // tail lanes recompute the last column, store nothing
   constexpr unsigned E = sizeof(cplx<T>), EW = sizeof(T);

@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, lo
   extern __shared__ unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
-  const long row0 = (long)blockIdx.x * RB + seq;
+  const long row0 = (long)row_tile<T>() * RB + seq;
   const bool ok = row0 < nrows;
   const long row = ok ? row0 : nrows - 1;
   const cplx<TI>* src = reinterpret_cast<const cplx<TI>*>(in + blockIdx.y * in_arr_stride + row * 2 * M) + t;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ i
   extern __shared__ unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
-  const long row0 = (long)blockIdx.x * RB + seq;
+  const long row0 = (long)row_tile<T>() * RB + seq;
   const bool ok = row0 < nrows;
   const long row = ok ? row0 : nrows - 1;
   cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   const int a = seq / RB, r = seq % RB;
   cplx<T>* slot = lds + seq * RF::SLOT;
-  const long row0 = (long)blockIdx.x * RB;
+  const long row0 = (long)row_tile<T>() * RB;
   const long rrow = row0 + r < nrows ? row0 + r : nrows - 1;
   const bool ok = row0 + r < nrows;
   cplx<T>* brow = B + a * arr_stride + rrow * Zp;
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
     r2c_from_regs<T, M>(slot, t, twN, brow, ok && a < m, v2);
   }
   CSum v[1] = {F};
-  block_reduce_store_n<1>(v, partials, blockIdx.x);
+  block_reduce_store_n<1>(v, partials, row_tile<T>());
 }
 
 // Pointwise z pass of the Hessian-vector product (FP64): sequences a < m are the
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Ac
   extern __shared__ unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
-  const long row0 = (long)blockIdx.x * RB + seq;
+  const long row0 = (long)row_tile<T>() * RB + seq;
   const bool ok = row0 < nrows;
   const long row = ok ? row0 : nrows - 1;
   cplx<T> y[RF::Q2][RF::R2];
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Ac
         buf_st(rf, off, 0, cplx<double>{out[0], out[1]});
       }
   }
-  block_reduce_store_n<4>(qs, partials, blockIdx.x);
+  block_reduce_store_n<4>(qs, partials, row_tile<T>());
 }
 
 }  // namespace cdft
