@@ -92,6 +92,17 @@ def time_system(cdft, precision, rho_d, dF_d, steps, warmup):
     return ms / steps, parts[2], dF, acc
 
 
+def time_ao(cdft, rho_d, dF_d, steps, Rp=0.25, rho_p=0.5):
+    """The same 256^3 evaluation with an Asakura-Oosawa polymer (FMT_AO_Species) of radius Rp."""
+    s = cdft.System(SHAPE, DX, HSD, model=cdft.WHITEBEAR_I, ao_Rp=Rp, ao_reservoir=rho_p)
+    s.eval_async(rho_d.ptr, dF_d.ptr, fmt_only=True)
+    s.fetch()
+    ms = s.time(rho_d.ptr, dF_d.ptr, steps, fmt_only=True)
+    s.fetch()
+    s.close()
+    return {"Rp": Rp, "reservoir_density": rho_p, "ms_per_step": ms / steps, "evals_per_s": 1e3 * steps / ms}
+
+
 def slit_pore(cdft, n=128, dx=0.05, kT=0.8, rho_b=0.7, precision=0):
     """Slit pore: White Bear I + LJ (sigma = eps = 1, rc = 2.5, WCA, QF kernel) at kT, hard walls
     V_ext(z) = 10 [exp(-z/0.2) + exp(-(L-z)/0.2)], mu = mu_times_beta(rho_b), start rho_b exp(-V_ext)."""
@@ -334,6 +345,10 @@ def main():
     print("# mixed timing", flush=True)
     tmx, Fmx, dFmx, stmx = time_system(cdft, cdft.MIXED, rho_d, dF_d, args.steps, args.warmup)
 
+    print("# AO polymer timing", flush=True)
+    ao = time_ao(cdft, rho_d, dF_d, max(3, args.steps // 5))
+    ao["vs_fp64_plain"] = ao["ms_per_step"] / t64
+
     m = 11
     B64 = model_bytes(8, m)
     Bmx = model_bytes(4, m, fp64_io=True)
@@ -375,6 +390,7 @@ def main():
                   "roofline_frac": Bmx / (tmx * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "stages_ms_fp64": st64,
         "stages_ms_mixed": stmx,
+        "ao_polymer_fp64": ao,
         "F_fp64": F64,
     }
     if args.fire_grid:

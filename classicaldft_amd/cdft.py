@@ -39,6 +39,7 @@ class _Desc(ctypes.Structure):
         ("mf_kernel", _p),
         ("mf_potential", _i), ("mf_alpha", _d), ("mf_bh", _i), ("mf_r_att_min", _d), ("mf_scheme", _i),
         ("mf_ngauss", _i),
+        ("ao_Rp", _d), ("ao_reservoir", _d), ("ao_weights", _p),
     ]
 
 
@@ -174,9 +175,10 @@ class System:
     def __init__(self, shape, dx, hsd=1.0, model=WHITEBEAR_I, precision=FP64, weights=None, ideal=False, mu=0.0,
                  vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0,
                  mus=None, vexts=None, potential=POT_LJ, alpha=0.0, bh=False, r_att_min=0.0, mf_scheme=MF_QF,
-                 ngauss=0):
+                 ngauss=0, ao_Rp=0.0, ao_reservoir=0.0, ao_weights=None):
         """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create).
-        potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule."""
+        potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule;
+        ao_Rp > 0 adds an Asakura-Oosawa polymer of that radius at reservoir density ao_reservoir."""
         self.shape = tuple(int(n) for n in shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
@@ -197,7 +199,7 @@ class System:
                   float(B), float(hsd), int(precision), arr(weights, 11 * self.N), int(bool(ideal)), float(mu),
                   arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
                   arr(mf_kernel, self.N), int(potential), float(alpha), int(bool(bh)), float(r_att_min),
-                  int(mf_scheme), int(ngauss))
+                  int(mf_scheme), int(ngauss), float(ao_Rp), float(ao_reservoir), arr(ao_weights, 11 * self.N))
         h = _p()
         if mus is None:
             self.nb = 1
@@ -444,7 +446,7 @@ class FireBatch:
         vx = None if vexts is None else np.ascontiguousarray(vexts, dtype=np.float64).reshape(-1)
         d = _Desc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A), float(B),
                   float(hsd), int(precision), None, int(bool(ideal)), 0.0, None, int(bool(mean_field)), float(sigma),
-                  float(eps), float(rcut), float(kT), None, 0, 0.0, 0, 0.0, 0, 0)
+                  float(eps), float(rcut), float(kT), None, 0, 0.0, 0, 0.0, 0, 0, 0.0, 0.0, None)
         p = FireParams()
         _check(lib().cdft_fire_default_params(ctypes.byref(p)))
         if dt is not None:

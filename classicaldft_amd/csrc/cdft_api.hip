@@ -38,6 +38,15 @@ void validate(const cdft_system_desc& d) {
   }
 }
 
+void validate_ao(const cdft_system_desc& d) {
+  if (!(d.ao_Rp > 0)) return;
+  REQUIRE(d.model != CDFT_MODEL_NONE, "the AO polymer needs an FMT functional");
+  REQUIRE(d.precision == CDFT_FP64, "the AO polymer is implemented in FP64");
+  REQUIRE(d.ao_reservoir >= 0, "the AO reservoir density must be non-negative");
+  REQUIRE(d.ao_weights || 2 * d.ao_Rp >= std::min(std::min(d.dx, d.dy), d.dz),
+          "polymer diameter is less than the lattice spacing");
+}
+
 Pot pot_from(const cdft_potential_desc& p) {
   REQUIRE(p.kind >= CDFT_POT_LJ && p.kind <= CDFT_POT_WHDF, "unknown potential");
   return make_pot(p.kind, p.sigma, p.eps, p.rcut, p.alpha, p.bh, p.r_att_min);
@@ -55,19 +64,20 @@ void launch_mf_kernel(cdft_system_s& s, double* out) {
   HIPCHK(hipGetLastError());
 }
 
+// The m weight spectra (x 1/N) of spheres of diameter hsd into W: from host cell weights or generated on the device.
 template <typename T>
-void build_weight_spectra(cdft_system_s& s) {
+void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights, T* W) {
   const long N = s.N, Nkp = s.Nkp;
   DevBuf S;  // one double-precision spectrum
   S.alloc(sizeof(cplx<double>) * Nkp);
   const int nb = div_up(Nkp, kThreads);
-  if (s.m > 0) {
+  {
     DevBuf wreal;
     wreal.alloc(sizeof(double) * 11 * N);
-    if (s.d.fmt_weights) {
-      HIPCHK(hipMemcpyAsync(wreal.p, s.d.fmt_weights, sizeof(double) * 11 * N, hipMemcpyHostToDevice, s.stream));
+    if (host_weights) {
+      HIPCHK(hipMemcpyAsync(wreal.p, host_weights, sizeof(double) * 11 * N, hipMemcpyHostToDevice, s.stream));
     } else {
-      const double R = s.d.hsd / 2;
+      const double R = hsd / 2;
       const int Sxm = 2 + int(R / s.d.dx), Sym = 2 + int(R / s.d.dy), Szm = 2 + int(R / s.d.dz);
       REQUIRE(Sxm < 4 * s.g.Nx && Sym < 4 * s.g.Ny && Szm < 4 * s.g.Nz, "sphere is too large for the periodic box");
       DevBuf cells;
@@ -86,11 +96,21 @@ void build_weight_spectra(cdft_system_s& s) {
       launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 1, true);
       launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 0, true);
       hipLaunchKernelGGL((k_extract_weight<T>), dim3(nb), dim3(kThreads), 0, s.stream, S.as<cplx<double>>(),
-                         s.W.as<T>() + a * Nkp, Nkp, (int)((s.odd_mask >> a) & 1u), 1.0 / (double)N);
+                         W + a * Nkp, Nkp, (int)((s.odd_mask >> a) & 1u), 1.0 / (double)N);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s.stream));
   }
+}
+
+template <typename T>
+void build_weight_spectra(cdft_system_s& s) {
+  const long N = s.N, Nkp = s.Nkp;
+  DevBuf S;  // one double-precision spectrum
+  S.alloc(sizeof(cplx<double>) * Nkp);
+  const int nb = div_up(Nkp, kThreads);
+  if (s.m > 0) fmt_weight_spectra<T>(s, s.d.hsd, s.d.fmt_weights, s.W.as<T>());
+  if (s.has_ao) fmt_weight_spectra<T>(s, 2 * s.d.ao_Rp, s.d.ao_weights, s.Wao.as<T>());
   if (s.d.mean_field) {
     DevBuf w;
     w.alloc(sizeof(double) * N);
@@ -179,6 +199,14 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
     s->W.alloc(rs * s->Nkp * s->nconv);
   }
   if (desc->mean_field) s->vmf.alloc(sizeof(double) * s->N * nb);
+  validate_ao(*desc);
+  if (desc->ao_Rp > 0) {
+    s->has_ao = true;
+    s->Wao.alloc(rs * s->Nkp * s->m);
+    s->psi.alloc(sizeof(double) * s->N * nb);
+    s->A2.alloc(cs * s->Nkp * nb);
+    s->B2.alloc(cs * s->Nkp * s->nconv * nb);
+  }
   s->rho.alloc(sizeof(double) * s->N * nb);
   s->dF.alloc(sizeof(double) * s->N * nb);
   s->pslots = std::max<long>(s->nrows, 256);  // at least one row per block in every z kernel
@@ -205,6 +233,7 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
   s->d.fmt_weights = nullptr;
   s->d.vext = nullptr;
   s->d.mf_kernel = nullptr;
+  s->d.ao_weights = nullptr;
   s->ev.resize(16);
   s->ev_names.resize(16);
   for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
@@ -409,7 +438,9 @@ double csum(std::initializer_list<double> v) {
 // Enqueue one evaluation on device pointers (no synchronisation).
 // Forward half of the convolution pipeline on FP64 device data: in (nb members of N reals) -> A (spectra) ->
 // B (nconv weighted-density arrays, x and y inverse-transformed, z still spectral).  Register-FFT lengths only.
-void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<double>* B, int nconv) {
+void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<double>* B, int nconv,
+                      const double* W = nullptr) {
+  if (!W) W = s.W.as<double>();
   const Tables<double>& tw = tables<double>(s);
   const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
   launch_rows_r2c<double, double>(s, in, s.N, A, sA, s.nb);
@@ -421,8 +452,7 @@ void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<
       constexpr int NB = decltype(nbc)::value;
       const Batch bt{sA, sB, s.N, 0, nullptr};
       hipLaunchKernelGGL((k_conv_fwd2<double, Lc, NB>), dim3(div_up(cg.ncols, NB), s.nb), dim3(NB * F2<Lc>::P), 0,
-                         s.stream, A, B, Nkp, s.W.as<double>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<double>>(),
-                         bt);
+                         s.stream, A, B, Nkp, W, nconv, s.odd_mask, cg, tw.x.template as<cplx<double>>(), bt);
       HIPCHK(hipGetLastError());
     });
   });
@@ -430,7 +460,7 @@ void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<
 }
 
 template <int MODEL>
-void launch_hess(cdft_system_s& s, bool mf) {
+void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 1.0) {
   const Tables<double>& tw = tables<double>(s);
   const int M = s.g.Nz / 2;
   PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
@@ -447,17 +477,160 @@ void launch_hess(cdft_system_s& s, bool mf) {
     hipLaunchKernelGGL((k_z_hess2<MODEL, Mc>), dim3(div_up(s.nrows, RB), s.nb), dim3(nseq * RB * RF::P),
                        size_t(RB) * per, s.stream, s.B.as<cplx<double>>(), s.B2.as<cplx<double>>(), s.Nkp, s.m,
                        mf ? 1 : 0, s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
-                       tw.n.template as<cplx<double>>(), RB, pp, s.flag.as<int>(), bt);
+                       tw.n.template as<cplx<double>>(), RB, pp, inv_dp > 0 ? inv_dp : pp.inv_d, scale,
+                       s.flag.as<int>(), bt);
     HIPCHK(hipGetLastError());
     done = true;
   });
   REQUIRE(done, "grid too long in z for the Hessian-vector product (needs 2m+1 rows of Nz in 64 KB of LDS)");
 }
 
+// Backward convolution (x pass, sum_a w_a D_a, inverse x) of m FP64 arrays D into Acc with weight spectra W.
+void conv_bwd_f64(cdft_system_s& s, const cplx<double>* D, cplx<double>* Acc, const double* W, int m) {
+  const Tables<double>& tw = tables<double>(s);
+  const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
+  const ColGeo cg = col_geo<double>(s.g, 0);
+  with_reg_len(s.g.Nx, [&](auto lc) {
+    constexpr int Lc = decltype(lc)::value;
+    with_nb<double, Lc>(s.nb_half_bwd, [&](auto nbc) {
+      constexpr int NB = decltype(nbc)::value;
+      const Batch bt{sA, sB, s.N, 0, nullptr};
+      hipLaunchKernelGGL((k_conv_bwd2<double, Lc, NB>), dim3(div_up(cg.ncols, NB), s.nb), dim3(NB * F2<Lc>::P), 0,
+                         s.stream, D, Acc, Nkp, W, m, s.odd_mask, cg, tw.x.template as<cplx<double>>(), bt);
+      HIPCHK(hipGetLastError());
+    });
+  });
+}
+
+template <int MODEL>
+long launch_pw_ao(cdft_system_s& s, int nconv) {
+  const Tables<double>& tw = tables<double>(s);
+  const int M = s.g.Nz / 2;
+  PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
+  if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
+  long nblk = -1;
+  with_reg_len(M, [&](auto mc) {
+    constexpr int Mc = decltype(mc)::value;
+    using RF = RowF<double, Mc>;
+    const int nseq = nconv + s.m;
+    const int per = nseq * RF::SLOT * int(sizeof(cplx<double>));
+    const int RB = std::min(512 / (nseq * RF::P), (65536 - 4096) / per);
+    if (RB < 1) return;
+    nblk = div_up(s.nrows, RB);
+    const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr};
+    hipLaunchKernelGGL((k_z_pw_ao<MODEL, Mc>), dim3(nblk, s.nb), dim3(nseq * RB * RF::P), size_t(RB) * per, s.stream,
+                       s.B.as<cplx<double>>(), s.B2.as<cplx<double>>(), s.Nkp, s.m, nconv, s.vmf.as<double>(), s.nrows,
+                       s.g.Zp, tw.m.template as<cplx<double>>(), tw.n.template as<cplx<double>>(), RB, pp,
+                       1.0 / (2 * s.d.ao_Rp), s.partials.as<double2>(), s.flag.as<int>(), bt);
+    HIPCHK(hipGetLastError());
+  });
+  REQUIRE(nblk > 0, "grid too long in z for the AO functional (needs 2m+1 rows of Nz in 64 KB of LDS)");
+  return nblk;
+}
+
+// FMT::calculateFreeEnergyAndDerivatives with an FMT_AO_Species (src/FMT.cpp:363-427, src/FMT_Species.cpp:647-738)
+// plus the rest of the DFT, FP64: (1) n of rho; (2) Phi, dPhi collapsed for the colloid and the polymer; (3) the
+// colloid back-convolution; (4) psi = sum_a w^p_a * dPhi^p_a; (5) exp(-psi) and F_AO = rho_p sum (1 - exp(-psi));
+// (6) Upsilon = w^p * exp(-psi); (7) n again; (8) rho_p d2Phi(n) . Upsilon (polymer measures) collapsed for the
+// colloid; (9) its back-convolution; (10) the final pass adds the colloid and AO forces.  The tensor off-diagonals
+// of (8) are counted twice, as the derivative of F_AO requires.
+void eval_device_ao(cdft_system_s& s, const double* rho, double* dF, bool with_dft, bool accumulate) {
+  REQUIRE(rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) && reg_fft_ok<double>(s, s.g.Ny),
+          "the AO polymer needs register-FFT transform lengths");
+  s.n_marks = 0;
+  s.mark("start");
+  const int nconv = with_dft ? s.nconv : s.m;
+  const bool mf = with_dft && s.d.mean_field;
+  const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
+  const int nb = s.nb;
+  cplx<double>* A = s.A.as<cplx<double>>();
+  cplx<double>* B = s.B.as<cplx<double>>();
+  cplx<double>* A2 = s.A2.as<cplx<double>>();
+  cplx<double>* B2 = s.B2.as<cplx<double>>();
+  const Tables<double>& tw = tables<double>(s);
+  forward_weighted(s, rho, A, B, nconv);
+  s.mark("forward");
+  long nblk = 0;
+  switch (s.d.model) {
+    case CDFT_ROSENFELD: nblk = launch_pw_ao<M_ROSENFELD>(s, nconv); break;
+    case CDFT_RSLT: nblk = launch_pw_ao<M_RSLT>(s, nconv); break;
+    case CDFT_ESFMT: nblk = launch_pw_ao<M_ESFMT>(s, nconv); break;
+    case CDFT_WHITEBEAR_I: nblk = launch_pw_ao<M_WBI>(s, nconv); break;
+    default: nblk = launch_pw_ao<M_WBII>(s, nconv); break;
+  }
+  hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+                     s.results.as<double>(), s.pslots, 8L);
+  s.mark("z_pointwise_ao");
+  launch_cols<double>(s, B, Nkp, s.m, 1, true, nb, sB);
+  conv_bwd_f64(s, B, A, s.W.as<double>(), s.m);
+  launch_cols<double>(s, A, 0, 1, 1, false, nb, sA);
+  s.mark("colloid_back");
+  launch_cols<double>(s, B2, Nkp, s.m, 1, true, nb, sB);
+  conv_bwd_f64(s, B2, A2, s.Wao.as<double>(), s.m);
+  launch_cols<double>(s, A2, 0, 1, 1, false, nb, sA);
+  launch_rows_c2r<double, double>(s, A2, sA, s.psi.as<double>(), s.N, nb, 1.0);
+  {
+    const int eb = 256;
+    hipLaunchKernelGGL(k_ao_exp, dim3(eb, nb), dim3(kThreads), 0, s.stream, s.psi.as<double>(), s.N, s.N,
+                       s.partials.as<double2>(), s.pslots);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), (long)eb, 1,
+                       s.results.as<double>() + 5, s.pslots, 8L);
+  }
+  s.mark("psi");
+  forward_weighted(s, s.psi.as<double>(), A2, B2, s.m, s.Wao.as<double>());
+  forward_weighted(s, rho, A2, B, s.m);
+  s.mark("upsilon");
+  switch (s.d.model) {
+    case CDFT_ROSENFELD: launch_hess<M_ROSENFELD>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
+    case CDFT_RSLT: launch_hess<M_RSLT>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
+    case CDFT_ESFMT: launch_hess<M_ESFMT>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
+    case CDFT_WHITEBEAR_I: launch_hess<M_WBI>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
+    default: launch_hess<M_WBII>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
+  }
+  launch_cols<double>(s, B, Nkp, s.m, 1, true, nb, sB);
+  conv_bwd_f64(s, B, A2, s.W.as<double>(), s.m);
+  launch_cols<double>(s, A2, 0, 1, 1, false, nb, sA);
+  s.mark("ao_force");
+  const int M = s.g.Nz / 2;
+  for (int pass = 0; pass < 2; ++pass) {
+    FinalParams fp{};
+    fp.dV = s.d.dx * s.d.dy * s.d.dz;
+    fp.fmt = 1;
+    if (pass == 0) {
+      fp.mu = with_dft ? s.d.mu : 0.0;
+      fp.ideal = with_dft ? s.d.ideal : 0;
+      fp.mf = mf;
+      fp.ext = with_dft && s.has_vext;
+      fp.accumulate = accumulate;
+    } else {
+      fp.accumulate = 1;  // add dV c2r(AO force spectrum)
+    }
+    long fblk = 0;
+    with_reg_len(M, [&](auto mc) {
+      constexpr int Mc = decltype(mc)::value;
+      constexpr int RB = row_rb<double, Mc>(32768), P = RowF<double, Mc>::P, SLOT = RowF<double, Mc>::SLOT;
+      fblk = div_up(s.nrows, RB);
+      const Batch bt{sA, sB, s.N, 4 * s.pslots, pass == 0 && with_dft && s.nb > 1 ? s.mus.as<double>() : nullptr};
+      hipLaunchKernelGGL((k_z_final2<double, Mc>), dim3(fblk, nb), dim3(RB * P), RB * SLOT * sizeof(cplx<double>),
+                         s.stream, pass == 0 ? A : A2, rho, s.vext.as<double>(), s.vmf.as<double>(), dF,
+                         unsigned(s.N * 8), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
+                         tw.n.template as<cplx<double>>(), RB, fp, s.partials2.as<double2>(), bt);
+    });
+    HIPCHK(hipGetLastError());
+    if (pass == 0)
+      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials2.as<double2>(), fblk, 4,
+                         s.results.as<double>() + 1, 4 * s.pslots, 8L);
+  }
+  HIPCHK(hipGetLastError());
+  s.mark("z_final");
+}
+
 // out = [out] + (d2 beta Omega / drho drho) v on FP64 device data (with_dft = false: FMT part only).
 void hess_device(cdft_system_s& s, const double* rho, const double* v, double* out, bool with_dft, bool ideal,
                  bool accumulate) {
   REQUIRE(!s.mixed, "the Hessian-vector product needs an FP64 system");
+  REQUIRE(!s.has_ao, "the Hessian-vector product is not implemented with an AO polymer");
   REQUIRE(rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) && reg_fft_ok<double>(s, s.g.Ny),
           "the Hessian-vector product needs register-FFT transform lengths");
   const bool mf = with_dft && s.d.mean_field;
@@ -516,8 +689,29 @@ void hess_device(cdft_system_s& s, const double* rho, const double* v, double* o
   HIPCHK(hipGetLastError());
 }
 
+// Bulk AO polymer terms (FMT::BulkFex / BulkMuex, src/FMT.cpp:616-716): arg = sum_a dPhi_a(n) w^p_a is the
+// polymer's insertion free energy, pref = d arg / d rho (the second derivative of Phi along the colloid weights,
+// from the dual part); f_AO = rho_p (1 - e^-arg), mu_AO = rho_p pref e^-arg.
+template <int MODEL>
+void bulk_ao(const double* om, const double* omp, double rho, const PhiParams& pp, double dp, double* arg,
+             double* pref) {
+  constexpr bool TENSOR = MODEL >= M_ESFMT;
+  Dual n[11], out[11];
+  for (int a = 0; a < 11; ++a) n[a] = Dual(rho * om[a], om[a]);
+  const double id = pp.inv_d, id2 = pp.inv_d2;
+  const Dual v2[3] = {n[2], n[3], n[4]}, v1[3] = {n[2] * id, n[3] * id, n[4] * id};
+  PhiDeriv<Dual> dd;
+  phi_meas<MODEL, Dual>(n[0], n[1] * id2, n[1] * id, n[1], v1, v2, n + 5, pp, dd);
+  phi_collapse<TENSOR, Dual>(dd, 1.0 / dp, 1.0 / (dp * dp), out);
+  Dual acc(0.0, 0.0);
+  for (int a = 0; a < 11; ++a) acc = acc + out[a] * omp[a];
+  *arg = acc.v;
+  *pref = acc.d;
+}
+
 void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc) {
-  if (s->mixed) eval_device<float>(*s, rho_d, dF_d, with_dft, acc);
+  if (s->has_ao) eval_device_ao(*s, rho_d, dF_d, with_dft, acc);
+  else if (s->mixed) eval_device<float>(*s, rho_d, dF_d, with_dft, acc);
   else eval_device<double>(*s, rho_d, dF_d, with_dft, acc);
   s->last_with_dft = with_dft;
 }
@@ -526,7 +720,8 @@ void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_df
 void parts_from_results(const cdft_system_s& s, const double* res, double mu, double* parts) {
   const bool with_dft = s.last_with_dft;
   const double dV = s.d.dx * s.d.dy * s.d.dz;
-  const double F_hs = s.m > 0 ? res[0] * dV : 0.0;
+  // F_AO = rho_p sum (1 - exp(-psi)) (src/FMT_Species.cpp:712-724) joins the FMT sum before the factor dV
+  const double F_hs = s.m > 0 ? (s.has_ao ? csum({res[0], s.d.ao_reservoir * res[5]}) : res[0]) * dV : 0.0;
   const double F_id = (with_dft && s.d.ideal) ? res[1] : 0.0;
   const double F_mf = (with_dft && s.d.mean_field) ? 0.5 * res[2] : 0.0;
   const double mu_ = with_dft ? mu : 0.0;
@@ -863,10 +1058,15 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
     // uniform weighted densities n_a = rho * sum_R w_a(R) (fillUniform,
     // include/Fundamental_Measures.h:20-31); derivatives collapse onto them.
     const double d = sys->d.hsd;
-    double fex = 0.0, muex = 0.0;
+    double fex = 0.0, muex = 0.0, fao = 0.0, muao = 0.0;
+    auto omega = [](double dd, double* w) {
+      const double v[11] = {M_PI * dd * dd * dd / 6, M_PI * dd * dd, 0, 0, 0, M_PI * dd * dd / 3, 0, 0,
+                            M_PI * dd * dd / 3, 0, M_PI * dd * dd / 3};
+      for (int a = 0; a < 11; ++a) w[a] = v[a];
+    };
     if (sys->m > 0) {
-      const double om[11] = {M_PI * d * d * d / 6, M_PI * d * d, 0, 0, 0, M_PI * d * d / 3, 0, 0, M_PI * d * d / 3, 0,
-                             M_PI * d * d / 3};
+      double om[11];
+      omega(d, om);
       double n[11], out[11];
       for (int a = 0; a < 11; ++a) n[a] = rho * om[a];
       PhiParams pp = make_phi_params(d, sys->d.A, sys->d.B);
@@ -879,10 +1079,24 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
         default: fex = phi_site<M_WBII>(n, pp, out); break;
       }
       for (int a = 0; a < 11; ++a) muex += out[a] * om[a];
+      if (sys->has_ao) {
+        const double dp = 2 * sys->d.ao_Rp, rp = sys->d.ao_reservoir;
+        double omp[11], arg = 0.0, pref = 0.0;
+        omega(dp, omp);
+        switch (sys->d.model) {
+          case CDFT_ROSENFELD: bulk_ao<M_ROSENFELD>(om, omp, rho, pp, dp, &arg, &pref); break;
+          case CDFT_RSLT: bulk_ao<M_RSLT>(om, omp, rho, pp, dp, &arg, &pref); break;
+          case CDFT_ESFMT: bulk_ao<M_ESFMT>(om, omp, rho, pp, dp, &arg, &pref); break;
+          case CDFT_WHITEBEAR_I: bulk_ao<M_WBI>(om, omp, rho, pp, dp, &arg, &pref); break;
+          default: bulk_ao<M_WBII>(om, omp, rho, pp, dp, &arg, &pref); break;
+        }
+        fao = rp * -std::expm1(-arg);
+        muao = rp * pref * std::exp(-arg);
+      }
     }
     const double a = sys->d.mean_field ? sys->a_vdw : 0.0;
-    if (f) *f = csum({rho * std::log(rho) - rho, fex, 0.5 * a * rho * rho});
-    if (mu) *mu = csum({std::log(rho), muex, a * rho});
+    if (f) *f = csum({rho * std::log(rho) - rho, fex, fao, 0.5 * a * rho * rho});
+    if (mu) *mu = csum({std::log(rho), muex, muao, a * rho});
     return CDFT_OK;
   });
 }

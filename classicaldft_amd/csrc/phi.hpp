@@ -99,19 +99,18 @@ __host__ __device__ inline void fcoef(S x, S& f1, S& f1p, S& f2, S& f2p, S& f3, 
   }
 }
 
-// Returns Phi at one site; writes the 11 collapsed derivatives to out[].
-// n[] holds the 11 weighted densities of the site (tensor entries ignored for
-// Rosenfeld / RSLT, whose tensor arrays are never convolved).
-// S = double: Phi and its collapsed first derivatives.  S = Dual with n = n_rho + psi e: the e parts of out[]
-// are the collapsed second derivatives applied to psi (same collapse and parity sign).
+// Derivatives of Phi with respect to the measures: eta, s0, s1, s2, v1[3], v2[3] and the six independent tensor
+// entries (T = xx, xy, xz, yy, yz, zz, each the derivative with respect to one of T_jk, T_kj).
+template <typename S>
+struct PhiDeriv {
+  S eta, s0, s1, s2, v1[3], v2[3], T[6];
+};
+
+// Phi and its derivatives from the fundamental measures (FMT::calculate_Phi and
+// calculate_dPhi_wrt_fundamental_measures, src/FMT.cpp:114-176; model hooks include/FMT.h:139-643).
 template <int MODEL, typename S = double>
-__host__ __device__ inline S phi_site(const S* n, const PhiParams& pp, S* out) {
-  constexpr bool TENSOR = MODEL >= M_ESFMT;
-  const double id = pp.inv_d, id2 = pp.inv_d2;
-  const S eta = n[0];
-  const S s0 = n[1] * id2, s1 = n[1] * id, s2 = n[1];
-  const S v2[3] = {n[2], n[3], n[4]};
-  const S v1[3] = {n[2] * id, n[3] * id, n[4] * id};
+__host__ __device__ inline S phi_meas(const S& eta, const S& s0, const S& s1, const S& s2, const S (&v1)[3],
+                                      const S (&v2)[3], const S* T6, const PhiParams& pp, PhiDeriv<S>& d) {
   const S v1v2 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
   const S v2v2 = v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2];
 
@@ -138,7 +137,7 @@ __host__ __device__ inline S phi_site(const S* n, const PhiParams& pp, S* out) {
     }
   } else {  // esFMT family: FMT.h:396-455
     const double A = pp.A, B = pp.B;
-    const S Txx = n[5], Txy = n[6], Txz = n[7], Tyy = n[8], Tyz = n[9], Tzz = n[10];
+    const S Txx = T6[0], Txy = T6[1], Txz = T6[2], Tyy = T6[3], Tyz = T6[4], Tzz = T6[5];
     // TT = T.T (symmetric)
     const S Qxx = Txx * Txx + Txy * Txy + Txz * Txz;
     const S Qxy = Txx * Txy + Txy * Tyy + Txz * Tyz;
@@ -169,30 +168,57 @@ __host__ __device__ inline S phi_site(const S* n, const PhiParams& pp, S* out) {
 
   const double c1 = 1.0 / kPi, c2 = 1.0 / (2 * kPi);
   const S phi = -c1 * s0 * f1 + c2 * (s1 * s2 - v1v2) * f2 + P3 * f3;
-
-  const S dEta = -c1 * s0 * f1p + c2 * (s1 * s2 - v1v2) * f2p + P3 * f3p;
-  const S dS0 = -c1 * f1;
-  const S dS1 = c2 * s2 * f2;
-  const S dS2t = c2 * s1 * f2 + dS2 * f3;
-  out[0] = dEta;
-  out[1] = dS0 * id2 + dS1 * id + dS2t;
+  d.eta = -c1 * s0 * f1p + c2 * (s1 * s2 - v1v2) * f2p + P3 * f3p;
+  d.s0 = -c1 * f1;
+  d.s1 = c2 * s2 * f2;
+  d.s2 = c2 * s1 * f2 + dS2 * f3;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const S dv1 = -c2 * v2[k] * f2;
-    const S dv2 = -c2 * v1[k] * f2 + dV2[k] * f3;
-    out[2 + k] = -(dv2 + dv1 * id);  // parity sign, Species.h:324
+    d.v1[k] = -c2 * v2[k] * f2;
+    d.v2[k] = -c2 * v1[k] * f2 + dV2[k] * f3;
   }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) d.T[k] = dT[k] * f3;
+  return phi;
+}
+
+// Collapse onto the 11 stored arrays of a species of diameter 1/id (FMT_Species::set_fundamental_measure_derivatives,
+// include/Species.h:310-329): eta; s0/d^2 + s1/d + s2; the vector with the parity sign; tensor off-diagonals twice.
+template <bool TENSOR, typename S>
+__host__ __device__ inline void phi_collapse(const PhiDeriv<S>& d, double id, double id2, S* out) {
+  out[0] = d.eta;
+  out[1] = d.s0 * id2 + d.s1 * id + d.s2;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[2 + k] = -(d.v2[k] + d.v1[k] * id);  // parity sign, Species.h:324
   if constexpr (TENSOR) {
-    out[5] = dT[0] * f3;
-    out[6] = 2 * dT[1] * f3;  // off-diagonals counted twice, Species.h:327
-    out[7] = 2 * dT[2] * f3;
-    out[8] = dT[3] * f3;
-    out[9] = 2 * dT[4] * f3;
-    out[10] = dT[5] * f3;
+    out[5] = d.T[0];
+    out[6] = 2 * d.T[1];  // off-diagonals counted twice, Species.h:327
+    out[7] = 2 * d.T[2];
+    out[8] = d.T[3];
+    out[9] = 2 * d.T[4];
+    out[10] = d.T[5];
   } else {
 #pragma unroll
     for (int a = 5; a < 11; ++a) out[a] = S(0.0);
   }
+}
+
+// Returns Phi at one site; writes the 11 collapsed derivatives to out[].
+// n[] holds the 11 weighted densities of the site (tensor entries ignored for
+// Rosenfeld / RSLT, whose tensor arrays are never convolved).
+// S = double: Phi and its collapsed first derivatives.  S = Dual with n = n_rho + psi e: the e parts of out[]
+// are the collapsed second derivatives applied to psi (same collapse and parity sign).
+template <int MODEL, typename S = double>
+__host__ __device__ inline S phi_site(const S* n, const PhiParams& pp, S* out) {
+  constexpr bool TENSOR = MODEL >= M_ESFMT;
+  const double id = pp.inv_d, id2 = pp.inv_d2;
+  const S eta = n[0];
+  const S s0 = n[1] * id2, s1 = n[1] * id, s2 = n[1];
+  const S v2[3] = {n[2], n[3], n[4]};
+  const S v1[3] = {n[2] * id, n[3] * id, n[4] * id};
+  PhiDeriv<S> d;
+  const S phi = phi_meas<MODEL, S>(eta, s0, s1, s2, v1, v2, n + 5, pp, d);
+  phi_collapse<TENSOR, S>(d, id, id2, out);
   return phi;
 }
 

@@ -312,7 +312,7 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
                                                  long arr_stride, int m, int mf, double* __restrict__ vmf, long nrows,
                                                  int Zp, const cplx<double>* __restrict__ twM,
                                                  const cplx<double>* __restrict__ twN, int RB, PhiParams pp,
-                                                 int* __restrict__ flag, Batch bt) {
+                                                 double inv_dp, double scale, int* __restrict__ flag, Batch bt) {
   B += blockIdx.y * bt.sb;
   B2 += blockIdx.y * bt.sb;
   if (vmf) vmf += blockIdx.y * bt.sr;
@@ -344,18 +344,28 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
     const int rr = s / (2 * M), z = s % (2 * M);
     const long row = row0 + rr;
     if (row >= nrows) continue;
-    Dual n[11];
+    double nv[11], pv[11];
 #pragma unroll
-    for (int b = 0; b < 11; ++b)
-      n[b] = b < m ? Dual(reinterpret_cast<const double*>(lds + (b * RB + rr) * RF::SLOT)[z],
-                          reinterpret_cast<const double*>(lds + ((m + b) * RB + rr) * RF::SLOT)[z])
-                   : Dual(0.0, 0.0);
+    for (int b = 0; b < 11; ++b) {
+      nv[b] = b < m ? reinterpret_cast<const double*>(lds + (b * RB + rr) * RF::SLOT)[z] : 0.0;
+      pv[b] = b < m ? reinterpret_cast<const double*>(lds + ((m + b) * RB + rr) * RF::SLOT)[z] : 0.0;
+    }
+    // measures of rho (diameter 1/inv_d) carrying those of the direction (diameter 1/inv_dp) as e parts
+    const double id = pp.inv_d, id2 = pp.inv_d2, ip2 = inv_dp * inv_dp;
+    const Dual eta(nv[0], pv[0]), s0(nv[1] * id2, pv[1] * ip2), s1(nv[1] * id, pv[1] * inv_dp), s2(nv[1], pv[1]);
+    const Dual v1[3] = {Dual(nv[2] * id, pv[2] * inv_dp), Dual(nv[3] * id, pv[3] * inv_dp),
+                        Dual(nv[4] * id, pv[4] * inv_dp)};
+    const Dual v2[3] = {Dual(nv[2], pv[2]), Dual(nv[3], pv[3]), Dual(nv[4], pv[4])};
+    const Dual T6[6] = {Dual(nv[5], pv[5]), Dual(nv[6], pv[6]), Dual(nv[7], pv[7]),
+                        Dual(nv[8], pv[8]), Dual(nv[9], pv[9]), Dual(nv[10], pv[10])};
+    PhiDeriv<Dual> d;
+    phi_meas<MODEL, Dual>(eta, s0, s1, s2, v1, v2, T6, pp, d);
     Dual out[11];
-    phi_site<MODEL, Dual>(n, pp, out);
-    if (n[0].v > 0.5 && 1 - n[0].v < 0.0) bad = 1;
+    phi_collapse<(MODEL >= M_ESFMT), Dual>(d, id, id2, out);
+    if (nv[0] > 0.5 && 1 - nv[0] < 0.0) bad = 1;
 #pragma unroll
     for (int b = 0; b < 11; ++b)
-      if (b < m) reinterpret_cast<double*>(lds + (b * RB + rr) * RF::SLOT)[z] = out[b].d;
+      if (b < m) reinterpret_cast<double*>(lds + (b * RB + rr) * RF::SLOT)[z] = scale * out[b].d;
     if (mf) vmf[row * 2 * M + z] = reinterpret_cast<const double*>(lds + (2 * m * RB + rr) * RF::SLOT)[z];
   }
   if (bad) atomicOr(flag, 1);
@@ -371,6 +381,114 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
     __syncthreads();
     r2c_from_regs<double, M>(slot, t, twN, brow, ok && a < m, v2);
   }
+}
+
+// Pointwise z pass of an FMT species with Asakura-Oosawa polymer (FP64): as k_z_pointwise2, and the same dPhi
+// collapsed a second time with the polymer diameter 1/inv_dp into the m arrays of B2 (FMT_AO_Species::
+// set_fundamental_measure_derivatives, src/FMT_Species.cpp:663-686).  Sequences a < nconv load array a of B;
+// sequences nconv + a (a < m) only hold the AO rows (their inverse transform runs on zeros so that every thread
+// meets the transform's barrier).
+template <int MODEL, int M>
+__global__ __launch_bounds__(512) void k_z_pw_ao(cplx<double>* __restrict__ B, cplx<double>* __restrict__ B2,
+                                                 long arr_stride, int m, int nconv, double* __restrict__ vmf,
+                                                 long nrows, int Zp, const cplx<double>* __restrict__ twM,
+                                                 const cplx<double>* __restrict__ twN, int RB, PhiParams pp,
+                                                 double inv_dp, double2* __restrict__ partials, int* __restrict__ flag,
+                                                 Batch bt) {
+  B += blockIdx.y * bt.sb;
+  B2 += blockIdx.y * bt.sb;
+  if (vmf) vmf += blockIdx.y * bt.sr;
+  partials += blockIdx.y * bt.sp;
+  flag += blockIdx.y;
+  using RF = RowF<double, M>;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+  const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
+  const int a = seq / RB, r = seq % RB;
+  cplx<double>* slot = lds + seq * RF::SLOT;
+  const long row0 = (long)blockIdx.x * RB;
+  const long rrow = row0 + r < nrows ? row0 + r : nrows - 1;
+  const bool ok = row0 + r < nrows;
+  const bool is_ao = a >= nconv;
+  cplx<double>* brow = is_ao ? B2 + (a - nconv) * arr_stride + rrow * Zp : B + a * arr_stride + rrow * Zp;
+  {
+    cplx<double> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+    if (!is_ao) {
+      c2r_to_regs<double, M>(brow, twN, t, v1);
+    } else {
+#pragma unroll
+      for (int q = 0; q < RF::Q1; ++q)
+#pragma unroll
+        for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = {0.0, 0.0};
+    }
+    reg_fft_seq<double, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+      for (int k2 = 0; k2 < RF::R2; ++k2) slot[t + RF::P * q + RF::R1 * k2] = v2[q][k2];
+  }
+  __syncthreads();
+  CSum F{0.0, 0.0};
+  int bad = 0;
+  const bool has_mf = nconv > m;
+  constexpr bool TENSOR = MODEL >= M_ESFMT;
+  for (int s = threadIdx.x; s < RB * 2 * M; s += blockDim.x) {
+    const int rr = s / (2 * M), z = s % (2 * M);
+    const long row = row0 + rr;
+    if (row >= nrows) continue;
+    double n[11];
+#pragma unroll
+    for (int b = 0; b < 11; ++b) n[b] = b < m ? reinterpret_cast<const double*>(lds + (b * RB + rr) * RF::SLOT)[z] : 0.0;
+    const double id = pp.inv_d, id2 = pp.inv_d2;
+    const double v1[3] = {n[2] * id, n[3] * id, n[4] * id}, v2[3] = {n[2], n[3], n[4]};
+    PhiDeriv<double> d;
+    const double phi = phi_meas<MODEL, double>(n[0], n[1] * id2, n[1] * id, n[1], v1, v2, n + 5, pp, d);
+    if (n[0] > 0.5 && 1 - n[0] < 0.0) bad = 1;
+    F.add(phi);
+    double out[11], outp[11];
+    phi_collapse<TENSOR, double>(d, id, id2, out);
+    phi_collapse<TENSOR, double>(d, inv_dp, inv_dp * inv_dp, outp);
+#pragma unroll
+    for (int b = 0; b < 11; ++b)
+      if (b < m) {
+        reinterpret_cast<double*>(lds + (b * RB + rr) * RF::SLOT)[z] = out[b];
+        reinterpret_cast<double*>(lds + ((nconv + b) * RB + rr) * RF::SLOT)[z] = outp[b];
+      }
+    if (has_mf) vmf[row * 2 * M + z] = reinterpret_cast<const double*>(lds + (m * RB + rr) * RF::SLOT)[z];
+  }
+  if (bad) atomicOr(flag, 1);
+  __syncthreads();
+  {
+    cplx<double> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+#pragma unroll
+    for (int q = 0; q < RF::Q1; ++q)
+#pragma unroll
+      for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = slot[RF::R2 * n1 + RF::P * q + t];
+    __syncthreads();
+    reg_fft_seq<double, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+    __syncthreads();
+    r2c_from_regs<double, M>(slot, t, twN, brow, ok && (a < m || is_ao), v2);
+  }
+  CSum v[1] = {F};
+  block_reduce_store_n<1>(v, partials, blockIdx.x);
+}
+
+// AO free energy density and the field the polymer weights act on (FMT_AO_Species::free_energy_post_process,
+// src/FMT_Species.cpp:690-738): psi <- exp(-psi) in place; partial sums of 1 - exp(-psi) per block.
+// grid = (blocks, members), block = kThreads.
+static __global__ __launch_bounds__(256) void k_ao_exp(double* __restrict__ psi, long n, long stride, double2* __restrict__ partials,
+                                                long sp) {
+  psi += blockIdx.y * stride;
+  partials += blockIdx.y * sp;
+  CSum acc{0.0, 0.0};
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double e = exp(-psi[i]);
+    psi[i] = e;
+    acc.add(1.0 - e);
+  }
+  CSum v[1] = {acc};
+  block_reduce_store_n<1>(v, partials, blockIdx.x);
 }
 
 // Final z pass: dF = [dF_in] + dV ln rho + dV c2r(acc) + v_mf + dV V_ext - dV mu

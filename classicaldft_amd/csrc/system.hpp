@@ -150,6 +150,8 @@ struct cdft_system_s {
   Tables<float> twf;
   DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
   DevBuf A2, B2, vdir;  // Hessian-vector product: spectra of the direction, its host copy (allocated on first use)
+  DevBuf Wao, psi;      // Asakura-Oosawa: polymer weight spectra (m arrays), psi / exp(-psi) (N per member)
+  bool has_ao = false;
   double a_vdw = 0.0;
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;
   // batch of nb same-shape members (weights shared; rho, dF, V_ext, v_mf, mu,

@@ -142,7 +142,7 @@ std::string esFMT::Name() const {
 
 cdft_system FMT::system_for(const Species& s) const {
   const Lattice& L = s.getLattice();
-  if (sys_ && sys_lattice_ == &L && sys_hsd_ == s.getHSD()) return sys_;
+  if (sys_ && sys_lattice_ == &L && sys_hsd_ == s.getHSD() && sys_species_ == &s) return sys_;
   if (sys_) cdft_system_destroy(sys_);
   cdft_system_desc d{};
   d.Nx = (int)L.Nx();
@@ -156,9 +156,14 @@ cdft_system FMT::system_for(const Species& s) const {
   d.B = B_;
   d.hsd = s.getHSD();
   d.precision = precision_;
+  if (auto* fs = dynamic_cast<const FMT_Species*>(&s)) {
+    d.ao_Rp = fs->ao_Rp();
+    d.ao_reservoir = fs->ao_reservoir();
+  }
   check(cdft_system_create(&d, &sys_));
   sys_lattice_ = &L;
   sys_hsd_ = s.getHSD();
+  sys_species_ = &s;
   return sys_;
 }
 
@@ -369,6 +374,10 @@ cdft_system DFT::system() const {
   d.B = fmt_ ? fmt_->B() : 0.0;
   d.hsd = s->getHSD();
   d.precision = precision_;
+  if (auto* fs = dynamic_cast<const FMT_Species*>(s)) {
+    d.ao_Rp = fs->ao_Rp();
+    d.ao_reservoir = fs->ao_reservoir();
+  }
   d.ideal = 1;
   d.mu = mu;
   d.vext = D.has_external_field() ? D.get_external_field().memptr() : nullptr;

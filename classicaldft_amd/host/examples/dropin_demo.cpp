@@ -9,6 +9,7 @@
 //   avdw_whdf_le <a>        lattice vdW parameter of WHDF on the LE kernel, kT = 0.9
 //   hess <sum Hv> <v.Hv>    DFT::matrix_dot_v1 of a seeded direction (16^3, ideal + WB I + LJ/QF)
 //   hess_fmt <sum> <v.Hv>   FMT::add_second_derivative of the same direction
+//   ao <F> <sum dF> <muex>  FMT_AO_Species (Rp = 0.3, reservoir 0.4) with White Bear I on 16^3, BulkMuex(0.5)
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -101,5 +102,15 @@ int main() {
   std::vector<Species*> sp{&s4};
   fmt4.add_second_derivative(vv, d2F, sp);
   std::printf("hess_fmt %.17g %.17g\n", d2F[0].accu(), d2F[0].dotWith(v));
+
+  // Asakura-Oosawa polymer of radius 0.3 at reservoir density 0.4
+  Density d6(0.1, L2);
+  for (long i = 0; i < d6.Ntot(); ++i) d6.set(i, 0.45 * (1.0 + 0.1 * (2.0 * splitmix_u(51, i) - 1.0)));
+  FMT_AO_Species s6(d6, 1.0, 0.3, 0.4, 0.0);
+  WhiteBearI fmt6;
+  std::vector<Species*> a6{&s6};
+  s6.zeroForce();
+  const double F6 = fmt6.calculateFreeEnergyAndDerivatives(a6);
+  std::printf("ao %.17g %.17g %.17g\n", F6, s6.getDF().accu(), fmt6.BulkMuex({0.5}, a6, 0));
   return 0;
 }

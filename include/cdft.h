@@ -98,6 +98,10 @@ typedef struct cdft_system_desc {
   double mf_r_att_min;    /* WCA continuation limit (Potential1::set_WCA_limit), 0 by default */
   int mf_scheme;          /* enum cdft_mf_scheme */
   int mf_ngauss;          /* Gauss-Legendre points per axis of CDFT_MF_GAUSS_E / _F (1..32) */
+  /* Asakura-Oosawa polymer of radius ao_Rp at reservoir density ao_reservoir acting on the FMT species
+     (FMT_AO_Species, include/Species.h:380-461; FP64 systems; ao_Rp = 0: none) */
+  double ao_Rp, ao_reservoir;
+  const double* ao_weights; /* host, 11*N polymer cell weights (diameter 2 Rp), or NULL to generate */
 } cdft_system_desc;
 
 /* A pair potential for the host-side helpers below (fields as in cdft_system_desc). */

@@ -154,9 +154,28 @@ class FMT_Species : public Species {
  public:
   FMT_Species(Density& density, double hsd, double mu = 0, int seq = -1);
   double getHSD() const override { return hsd_; }
+  // Asakura-Oosawa polymer parameters (0 for a plain FMT species)
+  virtual double ao_Rp() const { return 0.0; }
+  virtual double ao_reservoir() const { return 0.0; }
 
  private:
   double hsd_;
+};
+
+// include/Species.h:380-461, src/FMT_Species.cpp:647-738: an FMT species with ideal polymer of radius Rp at
+// reservoir density reservoir_density (FP64).  The AO force counts the tensor off-diagonals twice, as the
+// derivative of the AO free energy requires (the reference stores one of the two entries).
+class FMT_AO_Species : public FMT_Species {
+ public:
+  FMT_AO_Species(Density& density, double hsd, double Rp, double reservoir_density, double mu = 0, int seq = -1)
+      : FMT_Species(density, hsd, mu, seq), Rp_(Rp), reservoir_density_(reservoir_density) {}
+  double ao_Rp() const override { return Rp_; }
+  double ao_reservoir() const override { return reservoir_density_; }
+  double getReservoirDensity() const { return reservoir_density_; }
+  double getHSDP() const { return 2 * Rp_; }
+
+ private:
+  double Rp_, reservoir_density_;
 };
 
 // ------------------------------------------------------------------- FMT ---
@@ -189,6 +208,7 @@ class FMT {
   mutable cdft_system sys_ = nullptr;
   mutable const Lattice* sys_lattice_ = nullptr;
   mutable double sys_hsd_ = -1;
+  mutable const Species* sys_species_ = nullptr;
   cdft_system system_for(const Species& s) const;
 };
 

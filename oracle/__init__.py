@@ -56,6 +56,10 @@ def lib():
         L.orc_mf_kernel2.argtypes = [_i, _i, _i, _d, _d, _d, _i, _d, _d, _d, _d, _i, _d, _i, _i, _p, _p, _d, _p]
         L.orc_mf_kernel2.restype = _d
         L.orc_dft_set_mf.argtypes = [_p, _p, _d]
+        L.orc_fmt_set_ao.argtypes = [_p, _d, _d, _p, _i]
+        L.orc_fmt_set_ao.restype = None
+        L.orc_dft_set_ao.argtypes = [_p, _d, _d, _p, _i]
+        L.orc_dft_set_ao.restype = None
         L.orc_dft_set_mf.restype = None
         L.orc_dft_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _i, _i, _i, _d, _d, _d, _d, _d, _p]
         L.orc_dft_create.restype = _p
@@ -183,6 +187,12 @@ class FMT:
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         self.h = lib().orc_fmt_create(model, A, B, *self.shape, *dxs, hsd, _ptr(w))
 
+    def set_ao(self, Rp, rho_p, ao_weights=None, exact=True):
+        """Asakura-Oosawa polymer (FMT_AO_Species); exact=False keeps the reference's single count of the force's
+        tensor off-diagonals."""
+        self._aow = None if ao_weights is None else np.ascontiguousarray(ao_weights, dtype=np.float64)
+        lib().orc_fmt_set_ao(self.h, float(Rp), float(rho_p), _ptr(self._aow), int(bool(exact)))
+
     def eval(self, rho):
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         dF = np.zeros_like(rho)
@@ -231,6 +241,10 @@ def mf_kernel2(shape, dx, kind=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, 
 
 class DFT:
     """Whole-DFT oracle (ideal + FMT + LJ/QF mean field + V_ext - mu)."""
+
+    def set_ao(self, Rp, rho_p, ao_weights=None, exact=True):
+        self._aow = None if ao_weights is None else np.ascontiguousarray(ao_weights, dtype=np.float64)
+        lib().orc_dft_set_ao(self.h, float(Rp), float(rho_p), _ptr(self._aow), int(bool(exact)))
 
     def set_mean_field(self, w_att, a_vdw):
         """Use another mean-field kernel (e.g. from mf_kernel2)."""

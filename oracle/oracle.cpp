@@ -777,7 +777,16 @@ struct FmtState {
   double dx, dy, dz, hsd;
   Model model;
   std::vector<cd> wk;  // 11 full complex spectra (x 1/N), FMT_Weighted_Density.h:35-39
+  // Asakura-Oosawa polymer (FMT_AO_Species, include/Species.h:380-461): radius Rp, reservoir density rho_p,
+  // polymer weight spectra; ao_exact: count the tensor off-diagonals of the force twice (the derivative of F_AO)
+  // instead of once (the reference stores one of T_jk, T_kj, include/Species.h:406-425).
+  bool ao = false, ao_exact = true;
+  double Rp = 0, rho_p = 0;
+  std::vector<cd> wk_ao;
 };
+
+void weight_spectra(const Grid& g, double dx, double dy, double dz, double hsd, const double* wreal,
+                    std::vector<cd>& wk);
 
 void fmt_setup(FmtState& s, const double* wreal) {
   const Grid& g = s.g;
@@ -793,6 +802,99 @@ void fmt_setup(FmtState& s, const double* wreal) {
     r2c3d(g, wreal + a * N, s.wk.data() + a * Nk);
     for (long k = 0; k < Nk; ++k) s.wk[a * Nk + k] *= 1.0 / (double)N;
   }
+}
+
+// r2c x 1/N of the 11 real-space cell weights of diameter hsd (given, or generated as the reference does).
+void weight_spectra(const Grid& g, double dx, double dy, double dz, double hsd, const double* wreal,
+                    std::vector<cd>& wk) {
+  const long N = g.N(), Nk = g.Nk();
+  std::vector<double> w;
+  if (!wreal) {
+    w.assign(11 * N, 0.0);
+    gen_weights<double, false>(g, dx, dy, dz, hsd, w.data());
+    wreal = w.data();
+  }
+  wk.assign(11 * Nk, cd(0, 0));
+  for (int a = 0; a < 11; ++a) {
+    r2c3d(g, wreal + a * N, wk.data() + a * Nk);
+    for (long k = 0; k < Nk; ++k) wk[a * Nk + k] *= 1.0 / (double)N;
+  }
+}
+
+// The AO polymer's free energy and force (src/FMT_Species.cpp:663-738, src/FMT.cpp:393-422): dPhi collapsed with
+// the polymer diameter, psi = sum_a w^p_a * dPhi^p_a, F_AO = -rho_p sum exp(-psi) + rho_p N, Upsilon_a =
+// w^p_a * exp(-psi), then per site d2Phi(n) . Upsilon (polymer measures) collapsed with the colloid diameter and
+// back-convolved with the colloid weights, x rho_p dV, added to dF.  Returns F_AO (before the factor dV).
+double ao_eval(const FmtState& s, const double* rho, const double* const* np, double* dF) {
+  const Grid& g = s.g;
+  const long N = g.N(), Nk = g.Nk();
+  const bool tensor = s.model.tensor();
+  const int m = tensor ? 11 : 5;
+  const double dV = s.dx * s.dy * s.dz, d = s.hsd, hp = 2 * s.Rp;
+  (void)rho;
+  // dPhi^p per site (FMT_AO_Species::set_fundamental_measure_derivatives)
+  std::vector<double> dphi_p(11 * N, 0.0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (long i = 0; i < N; ++i) {
+    FM meas, D;
+    measures_from_arrays(np, i, d, tensor, meas);
+    s.model.dphi(meas, D);
+    double out[11];
+    collapse(D, hp, tensor, out);
+    for (int a = 0; a < 11; ++a) dphi_p[a * N + i] = out[a];
+  }
+  // psi (free_energy_post_process)
+  std::vector<cd> acc(Nk, cd(0, 0)), tmp(Nk);
+  for (int a = 0; a < m; ++a) {
+    r2c3d(g, dphi_p.data() + a * N, tmp.data());
+    for (long k = 0; k < Nk; ++k) acc[k] += tmp[k] * s.wk_ao[a * Nk + k];
+  }
+  std::vector<double> psi(N);
+  c2r3d(g, acc.data(), psi.data());
+  NSum Fs;
+  for (long i = 0; i < N; ++i) {
+    psi[i] = std::exp(-psi[i]);
+    Fs.add(psi[i]);
+  }
+  const double FAO = -s.rho_p * Fs.sum() + s.rho_p * (double)N;
+  // Upsilon: the polymer weights on exp(-psi)
+  std::vector<cd> ek(Nk);
+  r2c3d(g, psi.data(), ek.data());
+  std::vector<double> ups(11 * N, 0.0);
+  for (int a = 0; a < 11; ++a) {
+    for (long k = 0; k < Nk; ++k) tmp[k] = ek[k] * s.wk_ao[a * Nk + k];
+    c2r3d(g, tmp.data(), ups.data() + a * N);
+  }
+  const double* up[11];
+  for (int a = 0; a < 11; ++a) up[a] = ups.data() + a * N;
+  // d2Phi(n) . Upsilon, collapsed for the colloid (src/FMT.cpp:399-419)
+  std::vector<double> aoarr(11 * N, 0.0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (long i = 0; i < N; ++i) {
+    FM n, u, r;
+    measures_from_arrays(np, i, d, tensor, n);
+    measures_from_arrays(up, i, hp, tensor, u);
+    s.model.d2phi_dot(n, u, r);
+    aoarr[0 * N + i] = r.eta;
+    aoarr[1 * N + i] = r.s0 / (d * d) + r.s1 / d + r.s2;
+    for (int j = 0; j < 3; ++j) aoarr[(2 + j) * N + i] = -r.v1[j] / d - r.v2[j];
+    if (tensor)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) {
+          if (s.ao_exact) aoarr[kTI[j][k] * N + i] += r.T[j][k];
+          else aoarr[kTI[j][k] * N + i] = r.T[j][k];  // setFundamentalMeasures_AO: the last write of TI(j,k) stays
+        }
+  }
+  // computeAOForceContribution: colloid weights, x rho_p dV (all 11 arrays)
+  std::vector<cd> fk(Nk, cd(0, 0));
+  for (int a = 0; a < 11; ++a) {
+    r2c3d(g, aoarr.data() + a * N, tmp.data());
+    for (long k = 0; k < Nk; ++k) fk[k] += tmp[k] * s.wk[a * Nk + k];
+  }
+  std::vector<double> f(N);
+  c2r3d(g, fk.data(), f.data());
+  for (long i = 0; i < N; ++i) dF[i] += f[i] * s.rho_p * dV;
+  return FAO;
 }
 
 // FMT::calculateFreeEnergyAndDerivatives, src/FMT.cpp:363-427 with
@@ -851,7 +953,9 @@ int fmt_eval(const FmtState& s, const double* rho, double* dF, double* Fout, dou
   std::vector<double> out(N);
   c2r3d(g, acc.data(), out.data());
   for (long i = 0; i < N; ++i) dF[i] += out[i] * dV;
-  *Fout = F.sum() * dV;
+  double FAO = 0.0;
+  if (s.ao) FAO = ao_eval(s, rho, np, dF);
+  *Fout = (F.sum() + FAO) * dV;
   return 0;
 }
 
@@ -1314,6 +1418,17 @@ orc_fmt* orc_fmt_create(int kind, double A, double B, int Nx, int Ny, int Nz, do
   fmt_setup(*s, weights);
   return reinterpret_cast<orc_fmt*>(s);
 }
+// Adds an AO polymer (radius Rp, reservoir density rho_p; polymer weights given or generated; exact = 0
+// reproduces the reference's single count of the force's tensor off-diagonals).
+void orc_fmt_set_ao(orc_fmt* h, double Rp, double rho_p, const double* ao_weights, int exact) {
+  auto* s = reinterpret_cast<FmtState*>(h);
+  s->ao = Rp > 0;
+  s->Rp = Rp;
+  s->rho_p = rho_p;
+  s->ao_exact = exact != 0;
+  if (s->ao) weight_spectra(s->g, s->dx, s->dy, s->dz, 2 * Rp, ao_weights, s->wk_ao);
+}
+
 int orc_fmt_run(orc_fmt* h, const double* rho, double* dF, double* F) {
   auto* s = reinterpret_cast<FmtState*>(h);
   std::fill(dF, dF + s->g.N(), 0.0);
@@ -1378,6 +1493,10 @@ orc_dft* orc_dft_create(int kind, double A, double B, int Nx, int Ny, int Nz, do
 void orc_dft_destroy(orc_dft* h) { delete reinterpret_cast<DftState*>(h); }
 
 double orc_dft_avdw(orc_dft* h) { return reinterpret_cast<DftState*>(h)->a_vdw; }
+
+void orc_dft_set_ao(orc_dft* h, double Rp, double rho_p, const double* ao_weights, int exact) {
+  orc_fmt_set_ao(reinterpret_cast<orc_fmt*>(&reinterpret_cast<DftState*>(h)->fmt), Rp, rho_p, ao_weights, exact);
+}
 
 // Replaces the DFT's mean-field kernel (real space, / kT) and a_vdw / kT, e.g. by one of orc_mf_kernel2.
 void orc_dft_set_mf(orc_dft* h, const double* w_att, double a_vdw) {

@@ -115,3 +115,16 @@ def test_other_potentials_through_cpp_api(demo):
     assert rel(demo["mu_twf_bh_gauss_f"][0], np.log(0.5) + muex + a * 0.5) <= 1e-12
     _, a_le = oracle.mf_kernel2((16, 16, 16), 0.1, kind=oracle.POT_WHDF, rcut=2.0, scheme=oracle.MF_LE, kT=0.9)
     assert rel(demo["avdw_whdf_le"][0], a_le) <= 1e-12
+
+
+def test_ao_species_through_cpp_api(demo, cdft):
+    """FMT_AO_Species (include/Species.h:380-461) with White Bear I through the reference-named classes."""
+    shape, dx = (16, 16, 16), 0.1
+    rho = seeded(shape, 0.45, 51, 0.1)
+    o = oracle.FMT(oracle.WBI, shape, dx, 1.0, weights=oracle.fmt_weights(shape, dx, 1.0, mode=oracle.WEIGHTS_ACCURATE))
+    o.set_ao(0.3, 0.4, ao_weights=oracle.fmt_weights(shape, dx, 0.6, mode=oracle.WEIGHTS_ACCURATE))
+    F, dF = o.eval(rho)
+    assert rel(demo["ao"][0], F) <= 1e-11
+    assert abs(demo["ao"][1] - dF.sum()) <= 1e-11 * np.abs(dF).sum()
+    _, mu = cdft.System(shape, dx, 1.0, ao_Rp=0.3, ao_reservoir=0.4).bulk(0.5)
+    assert rel(demo["ao"][2], mu - np.log(0.5)) <= 1e-13
