@@ -40,6 +40,7 @@ class _Desc(ctypes.Structure):
         ("mf_potential", _i), ("mf_alpha", _d), ("mf_bh", _i), ("mf_r_att_min", _d), ("mf_scheme", _i),
         ("mf_ngauss", _i),
         ("ao_Rp", _d), ("ao_reservoir", _d), ("ao_weights", _p),
+        ("fixed_mass", _d), ("boundary", _i),
     ]
 
 
@@ -133,6 +134,7 @@ def lib():
         L.cdft_memcpy_h2d.argtypes = [_p, _p, ctypes.c_ulonglong]
         L.cdft_memcpy_d2h.argtypes = [_p, _p, ctypes.c_ulonglong]
         L.cdft_system_bulk.argtypes = [_p, _d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
+        L.cdft_system_chem_potential.argtypes = [_p, _p]
         L.cdft_fire_default_params.argtypes = [ctypes.POINTER(FireParams)]
         L.cdft_fire_create.argtypes = [_p, _p, _i, ctypes.POINTER(FireParams), ctypes.POINTER(_p)]
         L.cdft_fire_step.argtypes = [_p, ctypes.POINTER(FireReport)]
@@ -175,10 +177,13 @@ class System:
     def __init__(self, shape, dx, hsd=1.0, model=WHITEBEAR_I, precision=FP64, weights=None, ideal=False, mu=0.0,
                  vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0,
                  mus=None, vexts=None, potential=POT_LJ, alpha=0.0, bh=False, r_att_min=0.0, mf_scheme=MF_QF,
-                 ngauss=0, ao_Rp=0.0, ao_reservoir=0.0, ao_weights=None):
+                 ngauss=0, ao_Rp=0.0, ao_reservoir=0.0, ao_weights=None, fixed_mass=0.0, fixed_background=False,
+                 homogeneous_boundary=False):
         """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create).
         potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule;
-        ao_Rp > 0 adds an Asakura-Oosawa polymer of that radius at reservoir density ao_reservoir."""
+        ao_Rp > 0 adds an Asakura-Oosawa polymer of that radius at reservoir density ao_reservoir.
+        fixed_mass / fixed_background / homogeneous_boundary: Species::setFixedMass / setFixedBackground /
+        setHomogeneousBoundary (the full evaluation rescales rho to the mass and constrains dF)."""
         self.shape = tuple(int(n) for n in shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
@@ -199,7 +204,8 @@ class System:
                   float(B), float(hsd), int(precision), arr(weights, 11 * self.N), int(bool(ideal)), float(mu),
                   arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
                   arr(mf_kernel, self.N), int(potential), float(alpha), int(bool(bh)), float(r_att_min),
-                  int(mf_scheme), int(ngauss), float(ao_Rp), float(ao_reservoir), arr(ao_weights, 11 * self.N))
+                  int(mf_scheme), int(ngauss), float(ao_Rp), float(ao_reservoir), arr(ao_weights, 11 * self.N),
+                  float(fixed_mass), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0))
         h = _p()
         if mus is None:
             self.nb = 1
@@ -229,14 +235,16 @@ class System:
     def _ashape(self):
         return self.shape if self.nb == 1 else (self.nb,) + self.shape
 
-    def eval(self, rho, dF=None, accumulate=False):
-        """DFT::calculateFreeEnergyAndDerivatives: returns (parts[5], dF); batches: parts (nb, 5)."""
+    def eval(self, rho, dF=None, accumulate=False, only_fex=False):
+        """DFT::calculateFreeEnergyAndDerivatives(only_fex): returns (parts[5], dF); batches: parts (nb, 5).
+        only_fex leaves out the ideal-gas term (the external field and mu stay, as in the reference)."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         if rho.size != self.nb * self.N:
             raise ValueError(f"expected {self.nb * self.N} densities, got {rho.size}")
         out = np.zeros(self._ashape()) if dF is None else dF
         parts = np.zeros(5 * self.nb)
-        _check(lib().cdft_system_eval(self.h, _ptr(rho), _ptr(out), _ptr(parts), ACCUMULATE if accumulate else 0))
+        flags = (ACCUMULATE if accumulate else 0) | (NO_IDEAL if only_fex else 0)
+        _check(lib().cdft_system_eval(self.h, _ptr(rho), _ptr(out), _ptr(parts), flags))
         return (parts if self.nb == 1 else parts.reshape(self.nb, 5)), out
 
     def fmt_eval(self, rho, dF=None, accumulate=False):
@@ -308,6 +316,13 @@ class System:
         s = _p()
         _check(lib().cdft_system_stream(self.h, ctypes.byref(s)))
         return s.value or 0
+
+    def chem_potential(self):
+        """Species::getChemPotential after the last evaluation: the fixed-mass multiplier (or mu); (nb,) for
+        batches."""
+        mu = np.zeros(self.nb)
+        _check(lib().cdft_system_chem_potential(self.h, _ptr(mu)))
+        return mu[0] if self.nb == 1 else mu
 
     def bulk(self, rho):
         """(beta f, beta mu) at uniform density: DFT::fhelmholtz_times_beta_over_volume, mu_times_beta."""
@@ -446,7 +461,7 @@ class FireBatch:
         vx = None if vexts is None else np.ascontiguousarray(vexts, dtype=np.float64).reshape(-1)
         d = _Desc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A), float(B),
                   float(hsd), int(precision), None, int(bool(ideal)), 0.0, None, int(bool(mean_field)), float(sigma),
-                  float(eps), float(rcut), float(kT), None, 0, 0.0, 0, 0.0, 0, 0, 0.0, 0.0, None)
+                  float(eps), float(rcut), float(kT), None, 0, 0.0, 0, 0.0, 0, 0, 0.0, 0.0, None, 0.0, 0)
         p = FireParams()
         _check(lib().cdft_fire_default_params(ctypes.byref(p)))
         if dt is not None:

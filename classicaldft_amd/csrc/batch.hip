@@ -438,6 +438,8 @@ int cdft_fire_batch_create(const cdft_system_desc* desc, int nb, const double* m
                            const double* rho0, const cdft_fire_params* params, cdft_fire_batch* out) {
   return guarded([&] {
     REQUIRE(desc && out && mu && rho0 && nb >= 1, "bad argument");
+    REQUIRE(!(desc->fixed_mass > 0) && desc->boundary == 0,
+            "batched FIRE runs open (fixed-mu) periodic systems: no fixed mass or boundary constraints");
     auto b = std::make_unique<cdft_fire_batch_s>();
     cdft_system s = nullptr;
     create_system(desc, &s, nb, mu, vext);

@@ -36,6 +36,10 @@ void validate(const cdft_system_desc& d) {
             "Gauss kernel rules need 1..32 points per axis");
     REQUIRE(d.mf_potential != CDFT_POT_WHDF || d.lj_rcut > d.lj_sigma, "WHDF needs rcut > sigma");
   }
+  REQUIRE(d.fixed_mass >= 0, "the fixed mass must be non-negative");
+  REQUIRE(d.boundary >= 0 && d.boundary <= (CDFT_FIXED_BACKGROUND | CDFT_HOMOGENEOUS_BOUNDARY), "unknown boundary flags");
+  // src/Species.cpp:75-76
+  REQUIRE(!(d.fixed_mass > 0 && (d.boundary & CDFT_FIXED_BACKGROUND)), "Cannot have both fixed background and fixed mass");
 }
 
 void validate_ao(const cdft_system_desc& d) {
@@ -215,6 +219,8 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
   s->flag.alloc(sizeof(int) * nb);
   HIPCHK(hipMemset(s->flag.p, 0, sizeof(int) * nb));
   s->results.alloc(sizeof(double) * 8 * nb);
+  s->lagrange.assign(nb, mus ? 0.0 : desc->mu);
+  if (mus) for (int b = 0; b < nb; ++b) s->lagrange[b] = mus[b];
   const double* vx = vexts ? vexts : desc->vext;
   if (vx) {
     s->vext.alloc(sizeof(double) * s->N * nb);
@@ -282,6 +288,54 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
 }
 
 // The full evaluation on device pointers.  with_dft = false: FMT only.
+// mu of the full evaluation: zero with a fixed mass (Species::beginForceCalculation, src/Species.cpp:64-71)
+double eval_mu(const cdft_system_s& s) { return s.d.fixed_mass > 0 ? 0.0 : s.d.mu; }
+
+// ---- species constraints (constraints.hpp) ----
+dim3 cons_grid(const cdft_system_s& s, long n) {
+  return dim3((unsigned)std::max<long>(1, std::min<long>(div_up(n, kThreads), 4096)), s.nb);
+}
+// one compensated sum per member into results[8 b + slot]
+void cons_sum(cdft_system_s& s, const double* a, const double* b, long n, int mode, int slot) {
+  const long nblk = std::min<long>(s.pslots, std::max<long>(1, std::min<long>(div_up(n, kThreads), 1024)));
+  hipLaunchKernelGGL(k_cons_sum, dim3((unsigned)nblk, s.nb), dim3(kThreads), 0, s.stream, a, b, n, s.N, mode, s.g.Nx,
+                     s.g.Ny, s.g.Nz, s.partials.as<double2>(), s.pslots);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_partials_wide, dim3(s.nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+                     s.results.as<double>() + slot, s.pslots, 8L);
+  HIPCHK(hipGetLastError());
+}
+// Species::beginForceCalculation: rho rescaled in place to the fixed mass
+void constraints_pre(cdft_system_s& s, double* rho) {
+  if (!(s.d.fixed_mass > 0)) return;
+  const double dV = s.d.dx * s.d.dy * s.d.dz;
+  cons_sum(s, rho, nullptr, s.N, CONS_SUM, 6);
+  hipLaunchKernelGGL(k_cons_scale, cons_grid(s, s.N), dim3(kThreads), 0, s.stream, rho, s.N, s.N,
+                     s.results.as<double>() + 6, 8L, s.d.fixed_mass, dV);
+  HIPCHK(hipGetLastError());
+}
+// Species::endForceCalculation: fixed background, homogeneous boundary, fixed-mass projection (in that order)
+void constraints_post(cdft_system_s& s, const double* rho, double* dF) {
+  const long nbnd = boundary_count(s.g.Nx, s.g.Ny, s.g.Nz);
+  if (s.d.boundary & CDFT_FIXED_BACKGROUND) {
+    hipLaunchKernelGGL(k_cons_bset, cons_grid(s, nbnd), dim3(kThreads), 0, s.stream, dF, nbnd, s.N, s.g.Nx, s.g.Ny,
+                       s.g.Nz, nullptr, 8L, 0.0);
+    HIPCHK(hipGetLastError());
+  }
+  if (s.d.boundary & CDFT_HOMOGENEOUS_BOUNDARY) {
+    cons_sum(s, dF, nullptr, nbnd, CONS_BSUM, 6);
+    hipLaunchKernelGGL(k_cons_bset, cons_grid(s, nbnd), dim3(kThreads), 0, s.stream, dF, nbnd, s.N, s.g.Nx, s.g.Ny,
+                       s.g.Nz, s.results.as<double>() + 6, 8L, 1.0 / (double)nbnd);
+    HIPCHK(hipGetLastError());
+  }
+  if (s.d.fixed_mass > 0) {
+    cons_sum(s, dF, rho, s.N, CONS_DOT, 7);
+    hipLaunchKernelGGL(k_cons_shift, cons_grid(s, s.N), dim3(kThreads), 0, s.stream, dF, s.N, s.N,
+                       s.results.as<double>() + 7, 8L, s.d.fixed_mass, s.d.dx * s.d.dy * s.d.dz);
+    HIPCHK(hipGetLastError());
+  }
+}
+
 template <typename T>
 void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft, bool accumulate) {
   s.n_marks = 0;
@@ -387,8 +441,8 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
   }
   FinalParams fp{};
   fp.dV = s.d.dx * s.d.dy * s.d.dz;
-  fp.mu = with_dft ? s.d.mu : 0.0;
-  fp.ideal = with_dft ? s.d.ideal : 0;
+  fp.mu = with_dft ? eval_mu(s) : 0.0;
+  fp.ideal = with_dft ? (s.d.ideal && !s.no_ideal) : 0;
   fp.fmt = s.m > 0;
   fp.mf = mf;
   fp.ext = with_dft && s.has_vext;
@@ -401,7 +455,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         constexpr int Mc = decltype(mc)::value;
         constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
         nblk = div_up(s.nrows, RB);
-        const Batch bt{sA, sB, s.N, 4 * s.pslots, with_dft && s.nb > 1 ? s.mus.as<double>() : nullptr};
+        const Batch bt{sA, sB, s.N, 4 * s.pslots, with_dft && s.nb > 1 && !(s.d.fixed_mass > 0) ? s.mus.as<double>() : nullptr};
         hipLaunchKernelGGL((k_z_final2<T, Mc>), dim3(nblk, nb), dim3(RB * P), RB * SLOT * sizeof(cplx<T>), s.stream, A,
                            rho, s.vext.as<double>(), s.vmf.as<double>(), dF, unsigned(s.N * 8), s.nrows, s.g.Zp,
                            tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, fp,
@@ -598,8 +652,8 @@ void eval_device_ao(cdft_system_s& s, const double* rho, double* dF, bool with_d
     fp.dV = s.d.dx * s.d.dy * s.d.dz;
     fp.fmt = 1;
     if (pass == 0) {
-      fp.mu = with_dft ? s.d.mu : 0.0;
-      fp.ideal = with_dft ? s.d.ideal : 0;
+      fp.mu = with_dft ? eval_mu(s) : 0.0;
+      fp.ideal = with_dft ? (s.d.ideal && !s.no_ideal) : 0;
       fp.mf = mf;
       fp.ext = with_dft && s.has_vext;
       fp.accumulate = accumulate;
@@ -611,7 +665,8 @@ void eval_device_ao(cdft_system_s& s, const double* rho, double* dF, bool with_d
       constexpr int Mc = decltype(mc)::value;
       constexpr int RB = row_rb<double, Mc>(32768), P = RowF<double, Mc>::P, SLOT = RowF<double, Mc>::SLOT;
       fblk = div_up(s.nrows, RB);
-      const Batch bt{sA, sB, s.N, 4 * s.pslots, pass == 0 && with_dft && s.nb > 1 ? s.mus.as<double>() : nullptr};
+      const Batch bt{sA, sB, s.N, 4 * s.pslots,
+                     pass == 0 && with_dft && s.nb > 1 && !(s.d.fixed_mass > 0) ? s.mus.as<double>() : nullptr};
       hipLaunchKernelGGL((k_z_final2<double, Mc>), dim3(fblk, nb), dim3(RB * P), RB * SLOT * sizeof(cplx<double>),
                          s.stream, pass == 0 ? A : A2, rho, s.vext.as<double>(), s.vmf.as<double>(), dF,
                          unsigned(s.N * 8), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
@@ -709,11 +764,17 @@ void bulk_ao(const double* om, const double* omp, double rho, const PhiParams& p
   *pref = acc.d;
 }
 
+// One evaluation on device data.  With a fixed mass the full evaluation rescales rho_d in place first
+// (the species density, as Species::beginForceCalculation does).
 void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc) {
+  if (with_dft) constraints_pre(*s, const_cast<double*>(rho_d));
   if (s->has_ao) eval_device_ao(*s, rho_d, dF_d, with_dft, acc);
   else if (s->mixed) eval_device<float>(*s, rho_d, dF_d, with_dft, acc);
   else eval_device<double>(*s, rho_d, dF_d, with_dft, acc);
+  if (with_dft) constraints_post(*s, rho_d, dF_d);
   s->last_with_dft = with_dft;
+  s->last_ideal = with_dft && s->d.ideal && !s->no_ideal;
+  s->last_fixed_mass = with_dft && s->d.fixed_mass > 0;
 }
 
 // F parts of one member from its five device results (sum Phi, sum rho ln rho - rho, sum rho v_mf, sum V rho, sum rho).
@@ -722,7 +783,7 @@ void parts_from_results(const cdft_system_s& s, const double* res, double mu, do
   const double dV = s.d.dx * s.d.dy * s.d.dz;
   // F_AO = rho_p sum (1 - exp(-psi)) (src/FMT_Species.cpp:712-724) joins the FMT sum before the factor dV
   const double F_hs = s.m > 0 ? (s.has_ao ? csum({res[0], s.d.ao_reservoir * res[5]}) : res[0]) * dV : 0.0;
-  const double F_id = (with_dft && s.d.ideal) ? res[1] : 0.0;
+  const double F_id = s.last_ideal ? res[1] : 0.0;
   const double F_mf = (with_dft && s.d.mean_field) ? 0.5 * res[2] : 0.0;
   const double mu_ = with_dft ? mu : 0.0;
   const double F_ext = (with_dft && s.has_vext ? res[3] * dV : 0.0) - res[4] * dV * mu_;
@@ -737,11 +798,12 @@ void parts_from_results(const cdft_system_s& s, const double* res, double mu, do
 // member).  CDFT_ETA_TOO_LARGE if any member's flag is set (flags cleared).
 int fetch_parts(cdft_system s, double* parts) {
   const int nb = s->nb;
-  std::vector<double> res(8 * (size_t)nb), mus(nb, s->d.mu);
+  std::vector<double> res(8 * (size_t)nb), mus(nb, eval_mu(*s));
   std::vector<int> fl(nb, 0);
   HIPCHK(hipMemcpyAsync(res.data(), s->results.p, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipMemcpyAsync(fl.data(), s->flag.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s->stream));
-  if (nb > 1) HIPCHK(hipMemcpyAsync(mus.data(), s->mus.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s->stream));
+  if (nb > 1 && !(s->d.fixed_mass > 0))
+    HIPCHK(hipMemcpyAsync(mus.data(), s->mus.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   bool bad = false;
   for (int b = 0; b < nb; ++b) bad = bad || fl[b];
@@ -751,6 +813,8 @@ int fetch_parts(cdft_system s, double* parts) {
     g_err = "Eta too large in FMT::dPHI";
     return CDFT_ETA_TOO_LARGE;
   }
+  if (s->last_fixed_mass)  // mu_ = sum dF rho / fixed mass (src/Species.cpp:99-110)
+    for (int b = 0; b < nb; ++b) s->lagrange[b] = res[8 * b + 7] / s->d.fixed_mass;
   if (parts)
     for (int b = 0; b < nb; ++b) parts_from_results(*s, res.data() + 8 * b, mus[b], parts + 5 * b);
   return CDFT_OK;
@@ -769,7 +833,9 @@ int run_eval(cdft_system s, const double* rho, double* dF, double* parts, int fl
     rho_d = s->rho.as<double>();
     dF_d = s->dF.as<double>();
   }
+  s->no_ideal = with_dft && (flags & CDFT_NO_IDEAL);
   enqueue_eval(s, rho_d, dF_d, with_dft, acc);
+  s->no_ideal = false;
   if (!dev) HIPCHK(hipMemcpyAsync(dF, s->dF.p, bytes, hipMemcpyDeviceToHost, s->stream));
   return fetch_parts(s, parts);
 }
@@ -925,12 +991,31 @@ int run_hess(cdft_system s, const double* rho, const double* v, double* out, int
     v_d = s->vdir.as<double>();
     out_d = s->dF.as<double>();
   }
+  const bool fixed_bnd = with_dft && (s->d.boundary & CDFT_FIXED_BACKGROUND);
+  if (fixed_bnd) {  // DFT::matrix_dot_v_intern with fixed boundaries, src/DFT.cpp:441-449
+    const long nbnd = boundary_count(s->g.Nx, s->g.Ny, s->g.Nz);
+    cons_sum(*s, v_d, nullptr, nbnd, CONS_BABS, 6);
+    std::vector<double> res(8 * (size_t)s->nb);
+    HIPCHK(hipMemcpyAsync(res.data(), s->results.p, sizeof(double) * 8 * s->nb, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int b = 0; b < s->nb; ++b)
+      REQUIRE(res[8 * b + 6] == 0.0,
+              "Input vector v must have zero boundary entries in DFT::hessian_dot_v when the species has fixed boundaries");
+  }
   hess_device(*s, rho_d, v_d, out_d, with_dft, !(flags & CDFT_NO_IDEAL), acc);
+  if (fixed_bnd) {  // src/DFT.cpp:478-485
+    const long nbnd = boundary_count(s->g.Nx, s->g.Ny, s->g.Nz);
+    hipLaunchKernelGGL(k_cons_bset, cons_grid(*s, nbnd), dim3(kThreads), 0, s->stream, out_d, nbnd, s->N, s->g.Nx,
+                       s->g.Ny, s->g.Nz, nullptr, 8L, 0.0);
+    HIPCHK(hipGetLastError());
+  }
   if (!dev) HIPCHK(hipMemcpyAsync(out, s->dF.p, bytes, hipMemcpyDeviceToHost, s->stream));
   std::vector<double> parts(5 * (size_t)s->nb);
-  const bool last = s->last_with_dft;
+  const bool last = s->last_with_dft, last_fm = s->last_fixed_mass;
+  s->last_fixed_mass = false;
   const int rc = fetch_parts(s, parts.data());  // waits; reports eta > 1
   s->last_with_dft = last;
+  s->last_fixed_mass = last_fm;
   return rc;
 }
 }  // namespace
@@ -951,7 +1036,9 @@ int cdft_system_eval_async(cdft_system sys, const double* rho_d, double* dF_d, i
     REQUIRE(sys && rho_d && dF_d, "null argument");
     const bool fmt_only = flags & CDFT_FMT_ONLY;
     REQUIRE(!fmt_only || sys->m > 0, "system has no FMT functional");
+    sys->no_ideal = !fmt_only && (flags & CDFT_NO_IDEAL);
     enqueue_eval(sys, rho_d, dF_d, !fmt_only, (flags & CDFT_ACCUMULATE) != 0);
+    sys->no_ideal = false;
     return CDFT_OK;
   });
 }
@@ -1097,6 +1184,14 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
     const double a = sys->d.mean_field ? sys->a_vdw : 0.0;
     if (f) *f = csum({rho * std::log(rho) - rho, fex, fao, 0.5 * a * rho * rho});
     if (mu) *mu = csum({std::log(rho), muex, muao, a * rho});
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_chem_potential(cdft_system sys, double* mu) {
+  return guarded([&] {
+    REQUIRE(sys && mu, "null argument");
+    for (int b = 0; b < sys->nb; ++b) mu[b] = sys->lagrange[b];
     return CDFT_OK;
   });
 }

@@ -152,11 +152,11 @@ __global__ void k_copy4(const double* __restrict__ a, const double* __restrict__
 }
 
 // ----------------------------- DDFT kernels --------------------------------
-// r = (g . dF)/dV - Lap_h rho, with max |(g . dF)/dV|:  g_dot_x (DDFT.cpp:320-351),
-// calculate_excess_RHS (:212-220), subtract_ideal_gas (:223-235).
+// r = (g . dF)/dV - Lap_h rho, with max |(g . dF)/dV|:  g_dot_x (DDFT.cpp:320-351; zero on the boundary
+// ix, iy or iz = 0 of a fixed background, :331-337), calculate_excess_RHS (:212-220), subtract_ideal_gas (:223-235).
 __global__ void k_ddft_rhs(const double* __restrict__ rho, const double* __restrict__ dF, double* __restrict__ out,
                            int Nx, int Ny, int Nz, double Dx, double Dy, double Dz, double Lx, double Ly, double Lz,
-                           double invdV, double2* __restrict__ partials) {
+                           double invdV, int fixed_bnd, double2* __restrict__ partials) {
   double mx[1] = {0.0};
   const long N = (long)Nx * Ny * Nz;
   for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < N; p += (long)gridDim.x * kThreads) {
@@ -168,6 +168,7 @@ __global__ void k_ddft_rhs(const double* __restrict__ rho, const double* __restr
     double gx = Dx * ((rho[px] + d0) * (dF[px] - x0) - (d0 + rho[mxp]) * (x0 - dF[mxp])) +
                 Dy * ((rho[py] + d0) * (dF[py] - x0) - (d0 + rho[my]) * (x0 - dF[my])) +
                 Dz * ((rho[pz] + d0) * (dF[pz] - x0) - (d0 + rho[mz]) * (x0 - dF[mz]));
+    if (fixed_bnd && (x == 0 || y == 0 || z == 0)) gx = 0.0;
     gx *= invdV;
     mx[0] = fmax(mx[0], fabs(gx));
     gx += -Lx * (rho[px] + rho[mxp] - 2 * d0) - Ly * (rho[py] + rho[my] - 2 * d0) - Lz * (rho[pz] + rho[mz] - 2 * d0);
@@ -196,13 +197,17 @@ __global__ void k_ddft_update(const cplx<double>* __restrict__ X, const cplx<dou
 }
 
 // d1 <- new; max |d1 - new| and min(new) (apply_diffusion_propogator :268-292 and the
-// negativity test of DDFT::step :160).  Reductions: max dev, max(-new), sum new.
-__global__ void k_ddft_accept(double* __restrict__ d1, const double* __restrict__ nw, long N, double2* __restrict__ partials) {
+// negativity test of DDFT::step :160).  Reductions: max dev, max(-new), sum new.  A fixed
+// background keeps d1 on the boundary and leaves it out of the deviation (:276-287).
+__global__ void k_ddft_accept(double* __restrict__ d1, const double* __restrict__ nw, long N, int fixed_bnd, int Ny,
+                              int Nz, double2* __restrict__ partials) {
   CSum s[1] = {{0, 0}};
   double mx[2] = {0.0, -1e300};
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
-    const double w = nw[i];
-    mx[0] = fmax(mx[0], fabs(d1[i] - w));
+    const long yz = (long)Ny * Nz;
+    const bool keep = fixed_bnd && (i < yz || (i % yz) < Nz || i % Nz == 0);
+    const double w = keep ? d1[i] : nw[i];
+    if (!keep) mx[0] = fmax(mx[0], fabs(d1[i] - w));
     mx[1] = fmax(mx[1], -w);
     s[0].add(w);
     d1[i] = w;
@@ -381,6 +386,7 @@ struct cdft_ddft_s {
   int successes = 0, calls = 0, restarts = 0;
   std::vector<double> Lx, Ly, Lz;
   DevBuf d0, d1, dF, tmp, X, R0, R1, W, lam, ex;
+  DevBuf sp;  // the species density of an evaluation inside the fixed-point loop (fixed mass rescales it)
   Reducer red;
   double parts[5] = {0, 0, 0, 0, 0};
   bool ex_ready = false;
@@ -420,7 +426,7 @@ struct cdft_ddft_s {
     const Geo& g = sys->g;
     hipLaunchKernelGGL(k_ddft_rhs, vgrid(N), dim3(kThreads), 0, sys->stream, rho, dF.as<double>(), tmp.as<double>(),
                        g.Nx, g.Ny, g.Nz, D[0], D[1], D[2], L[0], L[1], L[2], 1.0 / (dx * dy * dz),
-                       red.part.as<double2>());
+                       (sys->d.boundary & CDFT_FIXED_BACKGROUND) ? 1 : 0, red.part.as<double2>());
     HIPCHK(hipGetLastError());
     r2c(tmp.as<double>(), R);
     double r;
@@ -436,7 +442,7 @@ struct cdft_ddft_s {
     HIPCHK(hipGetLastError());
     c2r(W.as<cplx<double>>(), tmp.as<double>());
     hipLaunchKernelGGL(k_ddft_accept, vgrid(N), dim3(kThreads), 0, sys->stream, d1.as<double>(), tmp.as<double>(), N,
-                       red.part.as<double2>());
+                       (sys->d.boundary & CDFT_FIXED_BACKGROUND) ? 1 : 0, g.Ny, g.Nz, red.part.as<double2>());
     HIPCHK(hipGetLastError());
     double r[3];
     red.finish(sys->stream, 1, 2, r);
@@ -462,9 +468,13 @@ struct cdft_ddft_s {
       double dev = 1;
       restart = false;
       for (int i = 0; i < 100 && dev > tol && !restart; ++i) {
-        rc = eval(d1.as<double>());
+        // species->set_density(d1): with a fixed mass the evaluation rescales the species copy, not d1
+        const bool copy_sp = sys->d.fixed_mass > 0;
+        if (copy_sp) copy(d1, sp);
+        const double* cur = copy_sp ? sp.as<double>() : d1.as<double>();
+        rc = eval(cur);
         if (rc) return rc;
-        excess_rhs(d1.as<double>(), R1.as<cplx<double>>());
+        excess_rhs(cur, R1.as<cplx<double>>());
         const double old = dev;
         double mn;
         propagate(dev, mn, sum);
@@ -582,6 +592,7 @@ int cdft_fire_get(cdft_fire f, double* rho, double* dF_raw, double* x, double* v
     hipLaunchKernelGGL(k_rho_from_alias, vgrid(f->N), dim3(kThreads), 0, f->sys->stream, f->x.as<double>(),
                        f->rho.as<double>(), f->N, f->al);
     HIPCHK(hipGetLastError());
+    constraints_pre(*f->sys, f->rho.as<double>());  // with a fixed mass: the density the next evaluation uses
     if (rho) HIPCHK(hipMemcpyAsync(rho, f->rho.p, b, hipMemcpyDeviceToHost, f->sys->stream));
     if (dF_raw) HIPCHK(hipMemcpyAsync(dF_raw, f->raw.p, b, hipMemcpyDeviceToHost, f->sys->stream));
     if (x) HIPCHK(hipMemcpyAsync(x, f->x.p, b, hipMemcpyDeviceToHost, f->sys->stream));
@@ -614,6 +625,7 @@ int cdft_ddft_create(cdft_system sys, const double* rho0, double dt, double dt_m
     const size_t b = sizeof(double) * d->N;
     const size_t cb = sizeof(cplx<double>) * sys->Nkp;
     for (DevBuf* q : {&d->d0, &d->d1, &d->dF, &d->tmp}) q->alloc(b);
+    if (sys->d.fixed_mass > 0) d->sp.alloc(b);
     for (DevBuf* q : {&d->X, &d->R0, &d->R1, &d->W}) {
       q->alloc(cb);
       HIPCHK(hipMemset(q->p, 0, cb));

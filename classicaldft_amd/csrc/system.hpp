@@ -19,6 +19,7 @@
 #include "kernels2.hpp"
 #include "rows2.hpp"
 #include "weights.hpp"
+#include "constraints.hpp"
 
 using namespace cdft;
 
@@ -167,6 +168,9 @@ struct cdft_system_s {
   long slab_bytes = 0;   // cache budget of one x-slab of the y / z / y section (0: no slabs)
   bool has_vext = false;
   bool last_with_dft = true;
+  // species constraints (desc fixed_mass / boundary); no_ideal: the next evaluation skips the ideal gas
+  bool no_ideal = false, last_ideal = true, last_fixed_mass = false;
+  std::vector<double> lagrange;  // fixed-mass multiplier of each member after the last fetched evaluation
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -410,6 +414,8 @@ namespace cdft_impl {
 void hess_device(cdft_system_s& s, const double* rho, const double* v, double* out, bool with_dft, bool ideal,
                  bool accumulate);
 void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc);
+// Species::beginForceCalculation on device data: rho rescaled in place to the fixed mass (no-op without one).
+void constraints_pre(cdft_system_s& s, double* rho);
 // nb > 1: a batch of nb same-shape members with their own mus[nb] and V_ext
 // (nb x N or none); desc->mu / desc->vext are then ignored.
 void create_system(const cdft_system_desc* desc, cdft_system* out, int nb = 1, const double* mus = nullptr,

@@ -115,13 +115,10 @@ void Species::beginForceCalculation() {
   }
 }
 
-double Species::endForceCalculation() {
-  if (fixedMass_ > 0.0) {
-    mu_ = 0.0;
-    for (long p = 0; p < density_.Ntot(); ++p) mu_ += dF_.get(p) * density_.get(p);
-    mu_ /= fixedMass_;
-    for (long p = 0; p < density_.Ntot(); ++p) dF_.set(p, dF_.get(p) - mu_ * density_.dV());
-  }
+double Species::endForceCalculation(double multiplier) {
+  if (fixedBackground_ && fixedMass_ > 0.0)
+    throw std::runtime_error("Cannot have both fixed background and fixed mass .... aborting");
+  if (fixedMass_ > 0.0) mu_ = multiplier;
   return 0;
 }
 
@@ -359,8 +356,12 @@ void DFT::set_precision(int p) {
 cdft_system DFT::system() const {
   Species* s = species_.front();
   const Density& D = s->getDensity();
-  const double mu = s->getChemPotential();
-  if (sys_ && sys_mu_ == mu && sys_field_ == D.field_version()) return sys_;
+  // with a fixed mass mu_ holds the last multiplier and the evaluation runs at mu = 0
+  const double mu = s->is_mass_fixed() ? 0.0 : s->getChemPotential();
+  const double mass = s->is_mass_fixed() ? s->getFixedMass() : 0.0;
+  const int bnd = (s->is_background_fixed() ? CDFT_FIXED_BACKGROUND : 0) |
+                  (s->is_boundary_homogeneous() ? CDFT_HOMOGENEOUS_BOUNDARY : 0);
+  if (sys_ && sys_mu_ == mu && sys_mass_ == mass && sys_bnd_ == bnd && sys_field_ == D.field_version()) return sys_;
   invalidate();
   cdft_system_desc d{};
   d.Nx = (int)D.Nx();
@@ -380,10 +381,14 @@ cdft_system DFT::system() const {
   }
   d.ideal = 1;
   d.mu = mu;
+  d.fixed_mass = mass;
+  d.boundary = bnd;
   d.vext = D.has_external_field() ? D.get_external_field().memptr() : nullptr;
   if (!interactions_.empty()) interactions_.front()->describe(d);
   check(cdft_system_create(&d, &sys_));
   sys_mu_ = mu;
+  sys_mass_ = mass;
+  sys_bnd_ = bnd;
   sys_field_ = D.field_version();
   return sys_;
 }
@@ -395,28 +400,17 @@ double DFT::calculateFreeEnergyAndDerivatives(bool onlyFex) {
   s->beginForceCalculation();
   cdft_system sys = system();
   double parts[5];
-  check(cdft_system_eval(sys, s->getDensity().get_density_pointer(), s->getDF().memptr(), parts, 0));
+  // onlyFex skips the ideal gas only; the external field and mu stay (src/DFT.cpp:381-420)
+  check(cdft_system_eval(sys, s->getDensity().get_density_pointer(), s->getDF().memptr(), parts,
+                         onlyFex ? CDFT_NO_IDEAL : 0));
   F_id_ = parts[1];
   F_hs_ = parts[2];
   F_mf_ = parts[3];
   F_ext_ = parts[4];
-  double F = parts[0];
-  if (onlyFex) {
-    // drop the ideal and external parts: dF_id = dV ln rho, dF_ext = dV (V_ext - mu)
-    const Density& D = s->getDensity();
-    const double dV = D.dV();
-    DFT_Vec& df = s->getDF();
-    const bool fld = D.has_external_field();
-    for (long i = 0; i < D.Ntot(); ++i) {
-      double v = std::log(D.get(i)) * dV - s->getChemPotential() * dV;
-      if (fld) v += D.get_external_field().get(i) * dV;
-      df.set(i, df.get(i) - v);
-    }
-    F = F_hs_ + F_mf_;
-    F_id_ = F_ext_ = 0.0;
-  }
-  s->endForceCalculation();
-  return F;
+  double mu = 0.0;
+  check(cdft_system_chem_potential(sys, &mu));
+  s->endForceCalculation(mu);
+  return parts[0];
 }
 
 void Density::writeDensity(std::string filename) const {
@@ -488,6 +482,17 @@ double DFT::omega_times_beta_over_volume(double density) const {
 }
 
 // ---------------------------------------------------------------- Minimizer
+// The species takes the minimizer's density and, with a fixed mass, the multiplier of its last evaluation.
+void Minimizer::sync_species() {
+  DFT_Vec rho(dft_->get_lattice().Ntot());
+  density(rho);
+  Species* s = dft_->getSpecies(0);
+  s->set_density(rho);
+  double mu = 0.0;
+  check(cdft_system_chem_potential(dft_->system(), &mu));
+  s->endForceCalculation(mu);
+}
+
 // Minimizer::run / resume, src/Minimizer.cpp:36-87 (no wall-clock or snapshot I/O)
 void Minimizer::run(long maxSteps) {
   step_counter_ = 0;
@@ -523,9 +528,7 @@ double fireMinimizer2::step() {
   if (rc < 0) check(rc);
   calls_ = rep_.calls;
   F_ = rep_.F;
-  DFT_Vec rho(dft_->get_lattice().Ntot());
-  density(rho);
-  dft_->getSpecies(0)->set_density(rho);
+  sync_species();
   if (snapshot_every_ > 0 && snapshot_count_ % snapshot_every_ == 0) {
     std::string of = "snapshot_s0.dat";
     dft_->writeDensity(0, of);
@@ -539,9 +542,7 @@ int fireMinimizer2::run_to_raw_tolerance(long maxSteps, double tol) {
   if (rc < 0) check(rc);
   F_ = rep_.F;
   calls_ = rep_.calls;
-  DFT_Vec rho(dft_->get_lattice().Ntot());
-  density(rho);
-  dft_->getSpecies(0)->set_density(rho);
+  sync_species();
   return rep_.steps;
 }
 void fireMinimizer2::density(DFT_Vec& out) const {
@@ -570,9 +571,7 @@ double DDFT::step() {
   check(cdft_ddft_step(ddft_, &rep_));
   F_ = rep_.F;
   calls_ = rep_.calls;
-  DFT_Vec rho(dft_->get_lattice().Ntot());
-  density(rho);
-  dft_->getSpecies(0)->set_density(rho);
+  sync_species();
   return F_;
 }
 void DDFT::density(DFT_Vec& out) const {

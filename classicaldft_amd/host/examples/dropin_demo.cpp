@@ -10,6 +10,9 @@
 //   hess <sum Hv> <v.Hv>    DFT::matrix_dot_v1 of a seeded direction (16^3, ideal + WB I + LJ/QF)
 //   hess_fmt <sum> <v.Hv>   FMT::add_second_derivative of the same direction
 //   ao <F> <sum dF> <muex>  FMT_AO_Species (Rp = 0.3, reservoir 0.4) with White Bear I on 16^3, BulkMuex(0.5)
+//   fixed_mass_eval <F> <mu> <sum dF>   Species::setFixedMass(0.95 N), one evaluation (16^3, WB I + LJ/QF)
+//   fixed_mass_fire <F> <mu> <N>        then 10 fireMinimizer2 steps
+//   fixed_bg <F> <sum dF> <F onlyFex>   Species::setFixedBackground on another 16^3 system
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -112,5 +115,36 @@ int main() {
   s6.zeroForce();
   const double F6 = fmt6.calculateFreeEnergyAndDerivatives(a6);
   std::printf("ao %.17g %.17g %.17g\n", F6, s6.getDF().accu(), fmt6.BulkMuex({0.5}, a6, 0));
+
+  // species constraints: fixed mass (evaluation and FIRE), fixed background (evaluation, onlyFex)
+  Density d7(0.1, L2);
+  for (long i = 0; i < d7.Ntot(); ++i) d7.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(61, i) - 1.0)));
+  FMT_Species s7(d7, 1.0, 0.0);
+  Interaction_Interpolation_QF qf7(&s7, &s7, &lj, 1.0);
+  WhiteBearI fmt7;
+  DFT dft7(&s7);
+  dft7.addHardCoreContribution(&fmt7);
+  dft7.addInteraction(&qf7);
+  s7.setFixedMass(0.95 * d7.getNumberAtoms());
+  const double F7 = dft7.calculateFreeEnergyAndDerivatives(false);
+  std::printf("fixed_mass_eval %.17g %.17g %.17g\n", F7, s7.getChemPotential(), s7.getDF().accu());
+  fireMinimizer2 fire7(&dft7);
+  fire7.set_snapshot_interval(0);
+  for (int i = 0; i < 10; ++i) fire7.step();
+  std::printf("fixed_mass_fire %.17g %.17g %.17g\n", fire7.getF(), s7.getChemPotential(), d7.getNumberAtoms());
+
+  Density d8(0.1, L2);
+  for (long i = 0; i < d8.Ntot(); ++i) d8.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(71, i) - 1.0)));
+  FMT_Species s8(d8, 1.0, 0.3);
+  Interaction_Interpolation_QF qf8(&s8, &s8, &lj, 1.0);
+  WhiteBearI fmt8;
+  DFT dft8(&s8);
+  dft8.addHardCoreContribution(&fmt8);
+  dft8.addInteraction(&qf8);
+  s8.setFixedBackground(true);
+  const double F8 = dft8.calculateFreeEnergyAndDerivatives(false);
+  const double dF8 = s8.getDF().accu();
+  const double F8x = dft8.calculateFreeEnergyAndDerivatives(true);
+  std::printf("fixed_bg %.17g %.17g %.17g\n", F8, dF8, F8x);
   return 0;
 }

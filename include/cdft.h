@@ -70,7 +70,15 @@ enum cdft_flags {
   CDFT_DEVICE_PTRS = 1, /* rho / dF / vext are device pointers */
   CDFT_ACCUMULATE = 2,  /* add into dF instead of overwriting it */
   CDFT_FMT_ONLY = 4,    /* hard-sphere part only (cdft_system_eval_async / _time) */
-  CDFT_NO_IDEAL = 8     /* Hessian-vector product without the ideal-gas term (DFT::full_hessian_ false) */
+  CDFT_NO_IDEAL = 8     /* no ideal-gas term: DFT::calculateFreeEnergyAndDerivatives(onlyFex = true) for
+                           cdft_system_eval; DFT::full_hessian_ false for the Hessian-vector product */
+};
+
+/* Species boundary conditions (include/Species.h:24-25, applied in Species::endForceCalculation,
+   src/Species.cpp:73-98).  Boundary points: ix, iy or iz = 0 (Lattice::is_boundary_point). */
+enum cdft_boundary {
+  CDFT_FIXED_BACKGROUND = 1,     /* dF = 0 on the boundary (setFixedBackground) */
+  CDFT_HOMOGENEOUS_BOUNDARY = 2  /* boundary dF replaced by its mean (setHomogeneousBoundary) */
 };
 
 typedef struct cdft_system_s* cdft_system;
@@ -102,6 +110,13 @@ typedef struct cdft_system_desc {
      (FMT_AO_Species, include/Species.h:380-461; FP64 systems; ao_Rp = 0: none) */
   double ao_Rp, ao_reservoir;
   const double* ao_weights; /* host, 11*N polymer cell weights (diameter 2 Rp), or NULL to generate */
+  /* Species constraints of the full evaluation (cdft_system_eval and the solvers; not cdft_fmt_eval):
+     fixed_mass > 0 (Species::setFixedMass, src/Species.cpp:64-71, 99-110) rescales rho to that mass in place
+     before each evaluation (the staged copy for host pointers), evaluates with mu = 0 and then projects
+     dF -= mu dV with the multiplier mu = sum dF rho / fixed_mass (cdft_system_chem_potential); boundary:
+     enum cdft_boundary flags.  Fixed mass and a fixed background exclude each other, as in the reference. */
+  double fixed_mass;
+  int boundary;
 } cdft_system_desc;
 
 /* A pair potential for the host-side helpers below (fields as in cdft_system_desc). */
@@ -159,6 +174,9 @@ int cdft_system_time(cdft_system sys, const double* rho_d, double* dF_d, int fla
    fhelmholtz_times_beta_over_volume (src/DFT.cpp:58-113; FMT::BulkMuex /
    BulkFex src/FMT.cpp:616-709; Interaction_Base::Mu src/Interaction.cpp:61-72). */
 int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu);
+/* Species::getChemPotential after the last fetched evaluation: the fixed-mass multiplier
+   (src/Species.cpp:99-110) of each member, or the desc mu(s) without a fixed mass. */
+int cdft_system_chem_potential(cdft_system sys, double* mu);
 
 /* Hard-sphere diameter of the WCA-split Lennard-Jones potential (cut and
    shifted at rcut), d = int_0^rmin (1 - exp(-V0(r)/kT)) dr with

@@ -131,6 +131,14 @@ class Species {
   void setChemPotential(double m) { mu_ = m; fixedMass_ = -1; }
   void setFixedMass(double m) { fixedMass_ = m; if (m > 0.0) mu_ = 0.0; }
   bool is_mass_fixed() const { return fixedMass_ > 0; }
+  double getFixedMass() const { return fixedMass_; }
+  // include/Species.h:24-29: forces zeroed on (fixed background) or averaged over (homogeneous boundary) the
+  // boundary points ix, iy or iz = 0; applied on the GPU by the full evaluation
+  void setFixedBackground(bool fixed) { fixedBackground_ = fixed; }
+  void setHomogeneousBoundary(bool homo) { homogeneousBoundary_ = homo; }
+  bool is_background_fixed() const { return fixedBackground_; }
+  bool is_fixed_boundary() const { return fixedBackground_; }
+  bool is_boundary_homogeneous() const { return homogeneousBoundary_; }
   void zeroForce() { dF_.zeros(); }
   void addToForce(const DFT_Vec& f) { dF_.IncrementBy(f); }
   void addToForce(long i, double v) { dF_.IncrementBy(i, v); }
@@ -140,13 +148,16 @@ class Species {
   double get_convergence_monitor() const { return dF_.inf_norm() / density_.dV(); }
   virtual double getHSD() const { return 0.0; }
   void beginForceCalculation();   // src/Species.cpp:64-71
-  double endForceCalculation();   // src/Species.cpp:73-112 (fixed mass)
+  // src/Species.cpp:73-112: the boundary conditions and the fixed-mass projection of dF run on the GPU inside
+  // the evaluation; this records the fixed-mass multiplier it returns as mu (Species::getChemPotential).
+  double endForceCalculation(double multiplier);
 
  protected:
   Density& density_;
   DFT_Vec dF_;
   double mu_;
   double fixedMass_ = -1;
+  bool fixedBackground_ = false, homogeneousBoundary_ = false;
 };
 
 // include/Species.h:109-373: the weights live on the GPU (resident spectra).
@@ -413,7 +424,8 @@ class DFT {
   FMT* fmt_ = nullptr;
   int precision_ = CDFT_FP64;
   mutable cdft_system sys_ = nullptr;
-  mutable double sys_mu_ = NAN;
+  mutable double sys_mu_ = NAN, sys_mass_ = NAN;
+  mutable int sys_bnd_ = -1;
   mutable unsigned sys_field_ = ~0u;
   double F_id_ = 0, F_ext_ = 0, F_hs_ = 0, F_mf_ = 0;
   bool full_hessian_ = true, use_squared_matrix_ = false;
@@ -434,8 +446,10 @@ class Minimizer {
   double getForceTerminationCriterion() const { return forceLimit_; }
   virtual double get_convergence_monitor() const = 0;
   virtual double step() = 0;
+  virtual void density(DFT_Vec& out) const = 0;  // current density (host copy)
 
  protected:
+  void sync_species();
   DFT* dft_;
   double F_ = 0.0;
   int calls_ = 0;
@@ -462,7 +476,7 @@ class fireMinimizer2 : public Minimizer {
   void set_snapshot_interval(int n) { snapshot_every_ = n; }
   // Runs FIRE steps on the GPU until max |dF/drho|/dV <= tol.
   int run_to_raw_tolerance(long maxSteps, double tol);
-  void density(DFT_Vec& out) const;  // current density (host copy)
+  void density(DFT_Vec& out) const override;  // current density (host copy)
 
  private:
   void ensure();
@@ -485,7 +499,7 @@ class DDFT : public Minimizer {
   double get_time() const { return rep_.time; }
   double get_convergence_monitor() const override { return rep_.rhs_max; }
   double get_mass() const { return rep_.mass; }
-  void density(DFT_Vec& out) const;
+  void density(DFT_Vec& out) const override;
 
  private:
   void ensure();

@@ -69,7 +69,11 @@ def lib():
         L.orc_dft_eval.argtypes = [_p, _p, _p, _p]
         L.orc_dft_eval.restype = _i
         L.orc_dft_hess.argtypes = [_p, _p, _p, _p, _i]
-        L.orc_dft_hess.restype = None
+        L.orc_dft_hess.restype = _i
+        L.orc_dft_set_constraints.argtypes = [_p, _d, _i]
+        L.orc_dft_set_constraints.restype = _i
+        L.orc_dft_lagrange.argtypes = [_p]
+        L.orc_dft_lagrange.restype = _d
         L.orc_fire_create.argtypes = [_p, _p, _d, _d]
         L.orc_fire_create.restype = _p
         L.orc_fire_destroy.argtypes = [_p]
@@ -246,6 +250,17 @@ class DFT:
         self._aow = None if ao_weights is None else np.ascontiguousarray(ao_weights, dtype=np.float64)
         lib().orc_dft_set_ao(self.h, float(Rp), float(rho_p), _ptr(self._aow), int(bool(exact)))
 
+    def set_constraints(self, fixed_mass=0.0, fixed_background=False, homogeneous_boundary=False):
+        """Species::setFixedMass / setFixedBackground / setHomogeneousBoundary (include/Species.h:23-25)."""
+        bnd = (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0)
+        if lib().orc_dft_set_constraints(self.h, float(fixed_mass), bnd):
+            raise ValueError("Cannot have both fixed background and fixed mass")
+
+    @property
+    def chem_potential(self):
+        """mu_ after the last evaluation with a fixed mass (the Lagrange multiplier)."""
+        return lib().orc_dft_lagrange(self.h)
+
     def set_mean_field(self, w_att, a_vdw):
         """Use another mean-field kernel (e.g. from mf_kernel2)."""
         w = np.ascontiguousarray(w_att, dtype=np.float64)
@@ -266,12 +281,13 @@ class DFT:
         return lib().orc_dft_avdw(self.h)
 
     def eval(self, rho):
-        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        rho = np.array(rho, dtype=np.float64, order="C", copy=True)
         dF = np.zeros_like(rho)
         parts = np.zeros(5)
         rc = lib().orc_dft_eval(self.h, _ptr(rho), _ptr(dF), _ptr(parts))
         if rc == 1:
             raise EtaTooLarge("eta too large")
+        self.density = rho  # rescaled to the fixed mass when one is set
         return parts, dF
 
     def hess(self, rho, v, full=True):
@@ -279,7 +295,8 @@ class DFT:
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         v = np.ascontiguousarray(v, dtype=np.float64)
         out = np.zeros_like(rho)
-        lib().orc_dft_hess(self.h, _ptr(rho), _ptr(v), _ptr(out), int(bool(full)))
+        if lib().orc_dft_hess(self.h, _ptr(rho), _ptr(v), _ptr(out), int(bool(full))) == 2:
+            raise ValueError("Input vector v must have zero boundary entries when the species has fixed boundaries")
         return out
 
     def __del__(self):

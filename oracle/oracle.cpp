@@ -1243,12 +1243,59 @@ struct DftState {
   double mu = 0.0;
   std::vector<double> vext;  // may be empty
   double parts[5] = {0, 0, 0, 0, 0};  // F, F_id, F_hs, F_mf, F_ext
+  // Species constraints (include/Species.h:24-27): fixed mass (> 0), fixed background (bnd & 1),
+  // homogeneous boundary (bnd & 2); lagrange = mu_ after endForceCalculation with a fixed mass.
+  double fixed_mass = 0.0;
+  int bnd = 0;
+  double lagrange = 0.0;
 };
 
-int dft_eval(DftState& s, const double* rho, double* dF) {
+// Lattice::is_boundary_point (include/Lattice.h:189-194): ix, iy or iz = 0.
+inline bool boundary_point(const Grid& g, long p) {
+  const long iz = p % g.Nz, iy = (p / g.Nz) % g.Ny, ix = p / ((long)g.Nz * g.Ny);
+  return ix == 0 || iy == 0 || iz == 0;
+}
+
+// Species::endForceCalculation (src/Species.cpp:73-112): forces zeroed on a fixed background, replaced by their
+// mean on a homogeneous boundary, then projected on the fixed-mass shell (mu_ = sum dF rho / M, dF -= mu_ dV).
+void end_force_calculation(DftState& s, const double* rho, double* dF) {
+  const Grid& g = s.fmt.g;
+  const long N = g.N();
+  const double dV = s.fmt.dx * s.fmt.dy * s.fmt.dz;
+  if (s.bnd & 1)
+    for (long p = 0; p < N; ++p)
+      if (boundary_point(g, p)) dF[p] = 0.0;
+  if (s.bnd & 2) {
+    double sum = 0.0;
+    long nb = 0;
+    for (long p = 0; p < N; ++p)
+      if (boundary_point(g, p)) { sum += dF[p]; ++nb; }
+    for (long p = 0; p < N; ++p)
+      if (boundary_point(g, p)) dF[p] = sum / nb;
+  }
+  if (s.fixed_mass > 0.0) {
+    double mu = 0.0;
+    for (long p = 0; p < N; ++p) mu += dF[p] * rho[p];
+    mu /= s.fixed_mass;
+    for (long p = 0; p < N; ++p) dF[p] -= mu * dV;
+    s.lagrange = mu;
+  }
+}
+
+// DFT::calculateFreeEnergyAndDerivatives (src/DFT.cpp:360-425) with Species::beginForceCalculation
+// (src/Species.cpp:64-71): a fixed mass rescales rho in place to that mass and sets mu to zero.
+int dft_eval(DftState& s, double* rho, double* dF) {
   const Grid& g = s.fmt.g;
   const long N = g.N(), Nk = g.Nk();
   const double dV = s.fmt.dx * s.fmt.dy * s.fmt.dz;
+  double mu_eff = s.mu;
+  if (s.fixed_mass > 0.0) {
+    double n = 0.0;
+    for (long i = 0; i < N; ++i) n += rho[i];
+    const double f = s.fixed_mass / (n * dV);
+    for (long i = 0; i < N; ++i) rho[i] *= f;
+    mu_eff = 0.0;
+  }
   std::fill(dF, dF + N, 0.0);
   NSum Fid;
   if (s.ideal)
@@ -1282,11 +1329,12 @@ int dft_eval(DftState& s, const double* rho, double* dF) {
     Nat.add(rho[i]);
     if (!s.vext.empty()) fdot += s.vext[i] * rho[i];
   }
-  const double Fext = fdot * dV - Nat.sum() * dV * s.mu;
+  const double Fext = fdot * dV - Nat.sum() * dV * mu_eff;
   for (long i = 0; i < N; ++i) {
     if (!s.vext.empty()) dF[i] += s.vext[i] * dV;
-    dF[i] += -s.mu * dV;
+    dF[i] += -mu_eff * dV;
   }
+  end_force_calculation(s, rho, dF);
   NSum F;
   F.add(Fid.sum());
   F.add(Fhs);
@@ -1508,12 +1556,34 @@ void orc_dft_set_mf(orc_dft* h, const double* w_att, double a_vdw) {
 }
 
 // out = (d2F/drho drho) v; full = 0 leaves out the ideal-gas term (DFT::full_hessian_ false).
-void orc_dft_hess(orc_dft* h, const double* rho, const double* v, double* out, int full) {
-  dft_hess(*reinterpret_cast<DftState*>(h), rho, v, out, full != 0);
+// With a fixed background (DFT::matrix_dot_v_intern, src/DFT.cpp:441-449, 478-485) v must vanish on the boundary
+// (returns 2 otherwise) and the product is zeroed there.
+int orc_dft_hess(orc_dft* h, const double* rho, const double* v, double* out, int full) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  const long N = s->fmt.g.N();
+  if (s->bnd & 1)
+    for (long p = 0; p < N; ++p)
+      if (boundary_point(s->fmt.g, p) && std::fabs(v[p]) > 0.0) return 2;
+  dft_hess(*s, rho, v, out, full != 0);
+  if (s->bnd & 1)
+    for (long p = 0; p < N; ++p)
+      if (boundary_point(s->fmt.g, p)) out[p] = 0.0;
+  return 0;
 }
 
 // parts[5] = F, F_id, F_hs, F_mf, F_ext.  Returns 1 on eta too large.
-int orc_dft_eval(orc_dft* h, const double* rho, double* dF, double* parts) {
+// Species constraints; returns 1 for the combination the reference rejects (src/Species.cpp:75-76).
+int orc_dft_set_constraints(orc_dft* h, double fixed_mass, int bnd) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  if (fixed_mass > 0.0 && (bnd & 1)) return 1;
+  s->fixed_mass = fixed_mass;
+  s->bnd = bnd;
+  return 0;
+}
+double orc_dft_lagrange(orc_dft* h) { return reinterpret_cast<DftState*>(h)->lagrange; }
+
+// rho is rescaled in place when the species has a fixed mass.
+int orc_dft_eval(orc_dft* h, double* rho, double* dF, double* parts) {
   auto* s = reinterpret_cast<DftState*>(h);
   const int rc = dft_eval(*s, rho, dF);
   if (parts) std::memcpy(parts, s->parts, sizeof(s->parts));
@@ -1706,6 +1776,7 @@ struct Ddft {
           double gx = D[0] * ((d[px] + d0) * (dF[px] - x0) - (d0 + d[mx]) * (x0 - dF[mx])) +
                       D[1] * ((d[py] + d0) * (dF[py] - x0) - (d0 + d[my]) * (x0 - dF[my])) +
                       D[2] * ((d[pz] + d0) * (dF[pz] - x0) - (d0 + d[mz]) * (x0 - dF[mz]));
+          if ((dft->bnd & 1) && (ix == 0 || iy == 0 || iz == 0)) gx = 0.0;  // g_dot_x, DDFT.cpp:331-337
           gx *= 1.0 / dV;
           rmax = std::max(rmax, std::fabs(gx));
           r[p] = gx;
@@ -1743,6 +1814,7 @@ struct Ddft {
     c2r3d(g, w.data(), out.data());
     double maxdev = 0.0;
     for (long i = 0; i < N; ++i) {
+      if ((dft->bnd & 1) && boundary_point(g, i)) continue;  // fixed boundary keeps d1 (DDFT.cpp:276-287)
       const double o = out[i] / (double)N;
       maxdev = std::max(maxdev, std::fabs(d1[i] - o));
       d1[i] = o;
@@ -1829,6 +1901,12 @@ int orc_fire_step(orc_fire* h, double* info) {
 void orc_fire_get(orc_fire* h, double* rho, double* x, double* v, double* dF_raw) {
   auto* f = reinterpret_cast<Fire*>(h);
   f->set_rho_from_alias();
+  if (f->dft->fixed_mass > 0.0) {  // the density the next force evaluation uses (Species::beginForceCalculation)
+    double n = 0.0;
+    for (long i = 0; i < f->N; ++i) n += f->rho[i];
+    const double s = f->dft->fixed_mass / (n * f->dV);
+    for (long i = 0; i < f->N; ++i) f->rho[i] *= s;
+  }
   if (rho) std::memcpy(rho, f->rho.data(), sizeof(double) * f->N);
   if (x) std::memcpy(x, f->x.data(), sizeof(double) * f->N);
   if (v) std::memcpy(v, f->v.data(), sizeof(double) * f->N);

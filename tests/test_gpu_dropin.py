@@ -128,3 +128,31 @@ def test_ao_species_through_cpp_api(demo, cdft):
     assert abs(demo["ao"][1] - dF.sum()) <= 1e-11 * np.abs(dF).sum()
     _, mu = cdft.System(shape, dx, 1.0, ao_Rp=0.3, ao_reservoir=0.4).bulk(0.5)
     assert rel(demo["ao"][2], mu - np.log(0.5)) <= 1e-13
+
+
+def test_species_constraints_through_cpp_api(demo):
+    """Species::setFixedMass (evaluation and fireMinimizer2) and setFixedBackground with onlyFex through the
+    reference-named classes (src/Species.cpp:64-112, src/DFT.cpp:360-425)."""
+    shape, dx = (16, 16, 16), 0.1
+    rho = seeded(shape, 0.5, 61, 0.1)
+    M = 0.95 * rho.sum() * dx ** 3
+    o = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, accurate_weights=True)
+    o.set_constraints(fixed_mass=M)
+    p, dF = o.eval(rho)
+    F, mu, dsum = demo["fixed_mass_eval"]
+    assert rel(F, p[0]) <= 1e-11 and rel(mu, o.chem_potential) <= 1e-11
+    assert abs(dsum - dF.sum()) <= 1e-11 * np.abs(dF).sum()
+    f = oracle.Fire(o, o.density)
+    for _ in range(10):
+        _, info = f.step()
+    F, mu, N = demo["fixed_mass_fire"]
+    assert rel(F, info["F"]) <= 1e-10
+    assert rel(mu, o.chem_potential) <= 1e-9 and rel(N, M) <= 1e-12
+    rho8 = seeded(shape, 0.5, 71, 0.1)
+    o8 = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, mu=0.3, accurate_weights=True)
+    o8.set_constraints(fixed_background=True)
+    p8, d8 = o8.eval(rho8)
+    F8, dsum8, F8x = demo["fixed_bg"]
+    assert rel(F8, p8[0]) <= 1e-11 and abs(dsum8 - d8.sum()) <= 1e-11 * np.abs(d8).sum()
+    ox = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, mu=0.3, ideal=False, accurate_weights=True)
+    assert rel(F8x, ox.eval(rho8)[0][0]) <= 1e-11
