@@ -135,6 +135,7 @@ def lib():
         L.cdft_memcpy_d2h.argtypes = [_p, _p, ctypes.c_ulonglong]
         L.cdft_system_bulk.argtypes = [_p, _d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
         L.cdft_system_chem_potential.argtypes = [_p, _p]
+        L.cdft_system_set_fixed_masses.argtypes = [_p, _p]
         L.cdft_fire_default_params.argtypes = [ctypes.POINTER(FireParams)]
         L.cdft_fire_create.argtypes = [_p, _p, _i, ctypes.POINTER(FireParams), ctypes.POINTER(_p)]
         L.cdft_fire_step.argtypes = [_p, ctypes.POINTER(FireReport)]
@@ -183,7 +184,9 @@ class System:
         potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule;
         ao_Rp > 0 adds an Asakura-Oosawa polymer of that radius at reservoir density ao_reservoir.
         fixed_mass / fixed_background / homogeneous_boundary: Species::setFixedMass / setFixedBackground /
-        setHomogeneousBoundary (the full evaluation rescales rho to the mass and constrains dF)."""
+        setHomogeneousBoundary (the full evaluation rescales rho to the mass and constrains dF); batches take one
+        fixed mass per member."""
+        fm = np.atleast_1d(np.asarray(fixed_mass, dtype=np.float64)).reshape(-1)
         self.shape = tuple(int(n) for n in shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
@@ -205,7 +208,7 @@ class System:
                   arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
                   arr(mf_kernel, self.N), int(potential), float(alpha), int(bool(bh)), float(r_att_min),
                   int(mf_scheme), int(ngauss), float(ao_Rp), float(ao_reservoir), arr(ao_weights, 11 * self.N),
-                  float(fixed_mass), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0))
+                  float(fm[0]), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0))
         h = _p()
         if mus is None:
             self.nb = 1
@@ -219,6 +222,10 @@ class System:
         self.h = h
         self.model = model
         self.precision = precision
+        if fm.size > 1:
+            if fm.size != self.nb:
+                raise ValueError(f"expected {self.nb} fixed masses, got {fm.size}")
+            _check(lib().cdft_system_set_fixed_masses(self.h, _ptr(np.ascontiguousarray(fm))))
 
     def close(self):
         if getattr(self, "h", None):

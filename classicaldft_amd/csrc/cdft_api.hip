@@ -221,6 +221,11 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
   s->results.alloc(sizeof(double) * 8 * nb);
   s->lagrange.assign(nb, mus ? 0.0 : desc->mu);
   if (mus) for (int b = 0; b < nb; ++b) s->lagrange[b] = mus[b];
+  if (desc->fixed_mass > 0) {
+    s->masses.assign(nb, desc->fixed_mass);
+    s->masses_d.alloc(sizeof(double) * nb);
+    HIPCHK(hipMemcpy(s->masses_d.p, s->masses.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+  }
   const double* vx = vexts ? vexts : desc->vext;
   if (vx) {
     s->vext.alloc(sizeof(double) * s->N * nb);
@@ -311,7 +316,7 @@ void constraints_pre(cdft_system_s& s, double* rho) {
   const double dV = s.d.dx * s.d.dy * s.d.dz;
   cons_sum(s, rho, nullptr, s.N, CONS_SUM, 6);
   hipLaunchKernelGGL(k_cons_scale, cons_grid(s, s.N), dim3(kThreads), 0, s.stream, rho, s.N, s.N,
-                     s.results.as<double>() + 6, 8L, s.d.fixed_mass, dV);
+                     s.results.as<double>() + 6, 8L, s.masses_d.as<double>(), dV);
   HIPCHK(hipGetLastError());
 }
 // Species::endForceCalculation: fixed background, homogeneous boundary, fixed-mass projection (in that order)
@@ -331,7 +336,7 @@ void constraints_post(cdft_system_s& s, const double* rho, double* dF) {
   if (s.d.fixed_mass > 0) {
     cons_sum(s, dF, rho, s.N, CONS_DOT, 7);
     hipLaunchKernelGGL(k_cons_shift, cons_grid(s, s.N), dim3(kThreads), 0, s.stream, dF, s.N, s.N,
-                       s.results.as<double>() + 7, 8L, s.d.fixed_mass, s.d.dx * s.d.dy * s.d.dz);
+                       s.results.as<double>() + 7, 8L, s.masses_d.as<double>(), s.d.dx * s.d.dy * s.d.dz);
     HIPCHK(hipGetLastError());
   }
 }
@@ -814,7 +819,7 @@ int fetch_parts(cdft_system s, double* parts) {
     return CDFT_ETA_TOO_LARGE;
   }
   if (s->last_fixed_mass)  // mu_ = sum dF rho / fixed mass (src/Species.cpp:99-110)
-    for (int b = 0; b < nb; ++b) s->lagrange[b] = res[8 * b + 7] / s->d.fixed_mass;
+    for (int b = 0; b < nb; ++b) s->lagrange[b] = res[8 * b + 7] / s->masses[b];
   if (parts)
     for (int b = 0; b < nb; ++b) parts_from_results(*s, res.data() + 8 * b, mus[b], parts + 5 * b);
   return CDFT_OK;
@@ -1184,6 +1189,18 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
     const double a = sys->d.mean_field ? sys->a_vdw : 0.0;
     if (f) *f = csum({rho * std::log(rho) - rho, fex, fao, 0.5 * a * rho * rho});
     if (mu) *mu = csum({std::log(rho), muex, muao, a * rho});
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_set_fixed_masses(cdft_system sys, const double* masses) {
+  return guarded([&] {
+    REQUIRE(sys && masses, "null argument");
+    REQUIRE(sys->d.fixed_mass > 0, "the system was created without a fixed mass");
+    for (int b = 0; b < sys->nb; ++b) REQUIRE(masses[b] > 0, "fixed masses must be positive");
+    HIPCHK(hipStreamSynchronize(sys->stream));
+    sys->masses.assign(masses, masses + sys->nb);
+    HIPCHK(hipMemcpy(sys->masses_d.p, masses, sizeof(double) * sys->nb, hipMemcpyHostToDevice));
     return CDFT_OK;
   });
 }

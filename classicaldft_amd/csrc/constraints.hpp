@@ -52,11 +52,12 @@ static __global__ __launch_bounds__(256) void k_cons_sum(const double* __restric
 }
 
 // rho *= mass / (sum rho dV): Density::scale_to(fixedMass_ / getNumberAtoms()), src/Species.cpp:66-69
+// (mass[] per member)
 static __global__ __launch_bounds__(256) void k_cons_scale(double* __restrict__ rho, long n, long stride,
-                                                           const double* __restrict__ res, long so, double mass,
-                                                           double dV) {
+                                                           const double* __restrict__ res, long so,
+                                                           const double* __restrict__ mass, double dV) {
   rho += blockIdx.y * stride;
-  const double f = mass / (res[blockIdx.y * so] * dV);
+  const double f = mass[blockIdx.y] / (res[blockIdx.y * so] * dV);
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) rho[i] *= f;
 }
 
@@ -70,12 +71,12 @@ static __global__ __launch_bounds__(256) void k_cons_bset(double* __restrict__ a
     a[boundary_site(p, Nx, Ny, Nz)] = v;
 }
 
-// dF -= mu dV with mu = (sum dF rho) / mass (src/Species.cpp:99-110)
+// dF -= mu dV with mu = (sum dF rho) / mass (src/Species.cpp:99-110; mass[] per member)
 static __global__ __launch_bounds__(256) void k_cons_shift(double* __restrict__ dF, long n, long stride,
-                                                           const double* __restrict__ res, long so, double mass,
-                                                           double dV) {
+                                                           const double* __restrict__ res, long so,
+                                                           const double* __restrict__ mass, double dV) {
   dF += blockIdx.y * stride;
-  const double mu = res[blockIdx.y * so] / mass;
+  const double mu = res[blockIdx.y * so] / mass[blockIdx.y];
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     dF[i] -= mu * dV;
 }

@@ -171,6 +171,8 @@ struct cdft_system_s {
   // species constraints (desc fixed_mass / boundary); no_ideal: the next evaluation skips the ideal gas
   bool no_ideal = false, last_ideal = true, last_fixed_mass = false;
   std::vector<double> lagrange;  // fixed-mass multiplier of each member after the last fetched evaluation
+  std::vector<double> masses;    // fixed mass of each member (desc fixed_mass unless set per member)
+  DevBuf masses_d;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;

@@ -177,6 +177,9 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu);
 /* Species::getChemPotential after the last fetched evaluation: the fixed-mass multiplier
    (src/Species.cpp:99-110) of each member, or the desc mu(s) without a fixed mass. */
 int cdft_system_chem_potential(cdft_system sys, double* mu);
+/* One fixed mass per batch member (nb values > 0) for a system created with fixed_mass > 0 (the whole-DFT
+   batch with per-system mu, V_ext and fixed mass). */
+int cdft_system_set_fixed_masses(cdft_system sys, const double* masses);
 
 /* Hard-sphere diameter of the WCA-split Lennard-Jones potential (cut and
    shifted at rcut), d = int_0^rmin (1 - exp(-V0(r)/kT)) dr with

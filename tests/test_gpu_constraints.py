@@ -102,6 +102,22 @@ def test_batch_members_with_fixed_mass(cdft):
         assert close(lam[i], s.chem_potential(), 1e-13)
 
 
+def test_batch_members_with_their_own_fixed_mass(cdft):
+    rhos = np.array([density(3), density(4, rho0=0.4)])
+    Ms = [0.9 * rhos[0].sum() * DX ** 3, 1.05 * rhos[1].sum() * DX ** 3]
+    w = oracle.fmt_weights(SHAPE, DX, HSD)
+    b = cdft.System(SHAPE, DX, HSD, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mus=[MU, MU],
+                    vexts=[vext(), vext()], mean_field=True, kT=KT, fixed_mass=Ms)
+    pb, db = b.eval(rhos)
+    lam = b.chem_potential()
+    for i in range(2):
+        o, _ = pair(cdft, fixed_mass=Ms[i])
+        p0, d0 = o.eval(rhos[i])
+        assert np.max(np.abs(pb[i] - p0)) <= TOL * abs(p0[0])
+        assert maxrel(db[i], d0) <= TOL
+        assert close(lam[i], o.chem_potential, TOL)
+
+
 def test_fire_with_fixed_mass_replays_the_oracle(cdft):
     shape = (16, 16, 16)
     rho0 = density(11, shape)
