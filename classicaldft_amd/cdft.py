@@ -141,6 +141,8 @@ def lib():
         L.cdft_fire_step.argtypes = [_p, ctypes.POINTER(FireReport)]
         L.cdft_fire_run.argtypes = [_p, _i, _d, ctypes.POINTER(FireReport)]
         L.cdft_fire_get.argtypes = [_p, _p, _p, _p, _p]
+        L.cdft_fire_set_fixed_direction.argtypes = [_p, _p, _i]
+        L.cdft_fire_reset.argtypes = [_p, _p, _i]
         L.cdft_fire_destroy.argtypes = [_p]
         L.cdft_ddft_create.argtypes = [_p, _p, _d, _d, _d, ctypes.POINTER(_p)]
         L.cdft_ddft_step.argtypes = [_p, ctypes.POINTER(DdftReport)]
@@ -403,6 +405,16 @@ class Fire:
         rho, df, x, v = (np.zeros(shp) for _ in range(4))
         _check(lib().cdft_fire_get(self.h, _ptr(rho), _ptr(df), _ptr(x), _ptr(v)))
         return rho, x, v, df
+
+    def set_fixed_direction(self, e):
+        """Minimizer::set_fixed_direction: each force is reflected, dF -= 2 (e . dF) e (None clears)."""
+        d = None if e is None else np.ascontiguousarray(e, dtype=np.float64)
+        _check(lib().cdft_fire_set_fixed_direction(self.h, _ptr(d), 0))
+
+    def reset(self, rho):
+        """fireMinimizer2::reset from the density rho (dt, alpha and counters kept)."""
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        return lib().cdft_fire_reset(self.h, _ptr(r), 0)
 
     def close(self):
         if getattr(self, "h", None):

@@ -219,6 +219,20 @@ __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, lon
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) b[i] = a[i];
 }
 
+__global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, long N, double2* __restrict__ partials) {
+  CSum s[1] = {{0, 0}};
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) s[0].add(a[i] * b[i]);
+  reduce_store<1, 0>(s, nullptr, partials);
+}
+
+__global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, long N, double a) {
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) y[i] += a * x[i];
+}
+
+__global__ void k_zero(double* __restrict__ y, long N) {
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) y[i] = 0.0;
+}
+
 __global__ void k_sum(const double* __restrict__ a, long N, double2* __restrict__ partials) {
   CSum s[1] = {{0, 0}};
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) s[0].add(a[i]);
@@ -256,6 +270,8 @@ struct cdft_fire_s {
   double dV = 0;
   Alias al{};
   DevBuf x, v, dF, raw, rho, x_rem, v_rem, dF_rem, raw_rem;
+  DevBuf dir;  // Minimizer::fixed_direction_ (allocated when set)
+  bool has_dir = false;
   Reducer red;
   cdft_fire_params p{};
   // reference state (Minimizer.h:45-56, 105-160)
@@ -279,6 +295,16 @@ struct cdft_fire_s {
     if (rc == CDFT_ETA_TOO_LARGE) return rc;
     calls++;
     F = parts[0];
+    if (has_dir) {  // Minimizer.cpp:103-110: dF -= 2 (e . dF) e
+      hipLaunchKernelGGL(k_dot, vgrid(N), dim3(kThreads), 0, sys->stream, dir.as<double>(), raw.as<double>(), N,
+                         red.part.as<double2>());
+      HIPCHK(hipGetLastError());
+      double d;
+      red.finish(sys->stream, 1, 0, &d);
+      hipLaunchKernelGGL(k_axpy, vgrid(N), dim3(kThreads), 0, sys->stream, raw.as<double>(), dir.as<double>(), N,
+                         -2 * d);
+      HIPCHK(hipGetLastError());
+    }
     launch_post();
     double r[4];
     red.finish(sys->stream, 2, 2, r);
@@ -558,6 +584,46 @@ int cdft_fire_create(cdft_system sys, const double* rho0, int flags, const cdft_
     if (rc != CDFT_OK) return rc;
     *out = f.release();
     return (int)CDFT_OK;
+  });
+}
+
+int cdft_fire_set_fixed_direction(cdft_fire f, const double* dir, int flags) {
+  return guarded([&] {
+    REQUIRE(f, "null argument");
+    if (!dir) {
+      f->has_dir = false;
+      return (int)CDFT_OK;
+    }
+    const size_t b = sizeof(double) * f->N;
+    if (!f->dir.p) f->dir.alloc(b);
+    HIPCHK(hipMemcpyAsync(f->dir.p, dir, b, (flags & CDFT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          f->sys->stream));
+    HIPCHK(hipStreamSynchronize(f->sys->stream));
+    f->has_dir = true;
+    return (int)CDFT_OK;
+  });
+}
+
+// fireMinimizer2::reset (Minimizer.cpp:30-34, 149-158): aliases from rho, dt_best = dt, getDF_DX, v = 0; dt, alpha
+// and the step counters are kept.
+int cdft_fire_reset(cdft_fire f, const double* rho, int flags) {
+  return guarded([&] {
+    REQUIRE(f && rho, "null argument");
+    hipStream_t st = f->sys->stream;
+    HIPCHK(hipMemcpyAsync(f->rho.p, rho, sizeof(double) * f->N,
+                          (flags & CDFT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_alias_from_rho, vgrid(f->N), dim3(kThreads), 0, st, f->rho.as<double>(), f->x.as<double>(),
+                       f->N, f->al);
+    hipLaunchKernelGGL(k_rho_from_alias, vgrid(f->N), dim3(kThreads), 0, st, f->x.as<double>(), f->rho.as<double>(),
+                       f->N, f->al);
+    HIPCHK(hipGetLastError());
+    f->dt_best = f->dt;
+    const int rc = f->getDF_DX();
+    hipLaunchKernelGGL(k_zero, vgrid(f->N), dim3(kThreads), 0, st, f->v.as<double>(), f->N);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    f->P_next = 0.0;  // P = -v . dF of the next step, with v = 0
+    return rc;
   });
 }
 

@@ -504,10 +504,15 @@ void Minimizer::resume(long maxSteps) {
   do {
     F_ = step();
     step_counter_++;
+    const Density& D = dft_->getSpecies(0)->getDensity();
+    const double Ntotal = D.getNumberAtoms(), Volume = D.getVolume();
     if (get_convergence_monitor() < forceLimit_) break;
     if (maxSteps > 0 && step_counter_ == maxSteps) break;
+    if (Ntotal / Volume < minDensity_) break;
   } while (true);
 }
+
+void Minimizer::set_fixed_direction(const DFT_Vec& fixed) { fixed_direction_.set(fixed); }
 
 fireMinimizer2::fireMinimizer2(DFT* dft) : Minimizer(dft) {
   cdft_fire_default_params(&params_);
@@ -520,6 +525,21 @@ void fireMinimizer2::ensure() {
   if (fire_) return;
   Species* s = dft_->getSpecies(0);
   check(cdft_fire_create(dft_->system(), s->getDensity().get_density_pointer(), 0, &params_, &fire_));
+  if (fixed_direction_.size() == s->getDensity().Ntot()) {
+    // the constructor's getDF_DX already reflects the force: restart with the direction in place
+    check(cdft_fire_set_fixed_direction(fire_, fixed_direction_.memptr(), 0));
+    check(cdft_fire_reset(fire_, s->getDensity().get_density_pointer(), 0));
+  }
+}
+void fireMinimizer2::reset() {
+  if (!fire_) return ensure();
+  Species* s = dft_->getSpecies(0);
+  check(cdft_fire_set_fixed_direction(
+      fire_, fixed_direction_.size() == s->getDensity().Ntot() ? fixed_direction_.memptr() : nullptr, 0));
+  const int rc = cdft_fire_reset(fire_, s->getDensity().get_density_pointer(), 0);
+  if (rc < 0) check(rc);
+  if (rc == CDFT_ETA_TOO_LARGE) throw Eta_Too_Large_Exception();
+  sync_species();
 }
 double fireMinimizer2::step() {
   ensure();

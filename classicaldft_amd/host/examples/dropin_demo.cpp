@@ -13,6 +13,8 @@
 //   fixed_mass_eval <F> <mu> <sum dF>   Species::setFixedMass(0.95 N), one evaluation (16^3, WB I + LJ/QF)
 //   fixed_mass_fire <F> <mu> <N>        then 10 fireMinimizer2 steps
 //   fixed_bg <F> <sum dF> <F onlyFex>   Species::setFixedBackground on another 16^3 system
+//   fire_dir <F>            5 fireMinimizer2 steps with Minimizer::set_fixed_direction (ideal + WB I, mu = 0.4)
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -146,5 +148,24 @@ int main() {
   const double dF8 = s8.getDF().accu();
   const double F8x = dft8.calculateFreeEnergyAndDerivatives(true);
   std::printf("fixed_bg %.17g %.17g %.17g\n", F8, dF8, F8x);
+
+  Density d9(0.1, L2);
+  for (long i = 0; i < d9.Ntot(); ++i) d9.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(81, i) - 1.0)));
+  FMT_Species s9(d9, 1.0, 0.4);
+  WhiteBearI fmt9;
+  DFT dft9(&s9);
+  dft9.addHardCoreContribution(&fmt9);
+  DFT_Vec e(d9.Ntot());
+  double en = 0.0;
+  for (long i = 0; i < d9.Ntot(); ++i) {
+    e.set(i, 2.0 * splitmix_u(83, i) - 1.0);
+    en += e.get(i) * e.get(i);
+  }
+  for (long i = 0; i < d9.Ntot(); ++i) e.set(i, e.get(i) / std::sqrt(en));
+  fireMinimizer2 fire9(&dft9);
+  fire9.set_snapshot_interval(0);
+  fire9.set_fixed_direction(e);
+  for (int i = 0; i < 5; ++i) fire9.step();
+  std::printf("fire_dir %.17g\n", fire9.getF());
   return 0;
 }

@@ -219,6 +219,12 @@ int cdft_fire_default_params(cdft_fire_params* p);
 /* Constructor: alias from rho0 (host unless flags has CDFT_DEVICE_PTRS), first
    getDF_DX, v = 0.  params may be NULL (reference defaults). */
 int cdft_fire_create(cdft_system sys, const double* rho0, int flags, const cdft_fire_params* params, cdft_fire* out);
+/* Minimizer::set_fixed_direction (include/Minimizer.h:31, used in getDF_DX, src/Minimizer.cpp:103-110): every
+   force is reflected, dF -= 2 (e . dF) e, before the alias conversion; dir = NULL clears it. */
+int cdft_fire_set_fixed_direction(cdft_fire f, const double* dir, int flags);
+/* fireMinimizer2::reset (src/Minimizer.cpp:30-34, 149-158): restart from rho (aliases, one evaluation, v = 0),
+   keeping dt, alpha and the step counters. */
+int cdft_fire_reset(cdft_fire f, const double* rho, int flags);
 /* One fireMinimizer2::step.  Returns CDFT_OK, CDFT_ETA_TOO_LARGE (the step
    back-tracked: x, v/2 and dF restored, dt halved) or CDFT_UPHILL_LIMIT. */
 int cdft_fire_step(cdft_fire f, cdft_fire_report* rep);

@@ -444,6 +444,12 @@ class Minimizer {
   int getCalls() const { return calls_; }
   void setForceTerminationCriterion(double v) { forceLimit_ = v; }
   double getForceTerminationCriterion() const { return forceLimit_; }
+  // include/Minimizer.h:28-31: resume stops when N / V drops below minDensity; a fixed direction e reflects every
+  // force, dF -= 2 (e . dF) e (src/Minimizer.cpp:77-84, 103-110)
+  void setMinDensity(double m) { minDensity_ = m; }
+  void set_fixed_direction(const DFT_Vec& fixed);
+  void setVerbose(bool) {}
+  virtual void reset() = 0;  // restart from the species density (src/Minimizer.cpp:30-34)
   virtual double get_convergence_monitor() const = 0;
   virtual double step() = 0;
   virtual void density(DFT_Vec& out) const = 0;  // current density (host copy)
@@ -455,6 +461,8 @@ class Minimizer {
   int calls_ = 0;
   int step_counter_ = 0;
   double forceLimit_ = 0.1;
+  double minDensity_ = -1;
+  DFT_Vec fixed_direction_;
 };
 
 class fireMinimizer2 : public Minimizer {
@@ -471,6 +479,8 @@ class fireMinimizer2 : public Minimizer {
   void setTimeStepMax(double dt) { params_.dt_max = dt; }
   void setAlphaStart(double a) { params_.alpha_start = a; }
   void setAlphaFac(double a) { params_.f_alf = a; }
+  void setBacktrackFac(double a) { f_back_ = a; }  // stored only: the reference's step does not use it
+  void reset() override;                           // fireMinimizer2::reset, src/Minimizer.cpp:149-158
   // src/Minimizer.cpp:259-268: every 10th step writes snapshot_s0.dat (the density) to the working directory;
   // n = 0 turns the snapshots off.  The step counter is per minimizer (a function-static in the reference).
   void set_snapshot_interval(int n) { snapshot_every_ = n; }
@@ -485,6 +495,7 @@ class fireMinimizer2 : public Minimizer {
   cdft_fire_report rep_{};
   int snapshot_every_ = 10;
   long snapshot_count_ = 0;
+  double f_back_ = 0.8;
 };
 
 class DDFT : public Minimizer {
@@ -492,6 +503,7 @@ class DDFT : public Minimizer {
   explicit DDFT(DFT* dft, bool showGraphics = false);
   ~DDFT() override;
   double step() override;
+  void reset() override {}  // Minimizer::reset only refreshes the FIRE aliases, which DDFT does not use
   void set_tolerence_fixed_point(double e) { tol_ = e; }
   void set_max_time_step(double t);
   void setTimeStep(double dt) { dt_ = dt; }

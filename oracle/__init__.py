@@ -80,6 +80,9 @@ def lib():
         L.orc_fire_step.argtypes = [_p, _p]
         L.orc_fire_step.restype = _i
         L.orc_fire_get.argtypes = [_p, _p, _p, _p, _p]
+        L.orc_fire_set_fixed_direction.argtypes = [_p, _p]
+        L.orc_fire_reset.argtypes = [_p, _p]
+        L.orc_fire_reset.restype = _i
         L.orc_ddft_create.argtypes = [_p, _p, _d, _d, _d]
         L.orc_ddft_create.restype = _p
         L.orc_ddft_destroy.argtypes = [_p]
@@ -322,6 +325,16 @@ class Fire:
         rho, x, v, df = (np.zeros(self.shape) for _ in range(4))
         lib().orc_fire_get(self.h, _ptr(rho), _ptr(x), _ptr(v), _ptr(df))
         return rho, x, v, df
+
+    def set_fixed_direction(self, e):
+        """Minimizer::set_fixed_direction: the force is reflected, dF -= 2 (e . dF) e (None clears)."""
+        self._dir = None if e is None else np.ascontiguousarray(e, dtype=np.float64)
+        lib().orc_fire_set_fixed_direction(self.h, _ptr(self._dir))
+
+    def reset(self, rho):
+        """fireMinimizer2::reset from the density rho."""
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        return lib().orc_fire_reset(self.h, _ptr(r))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -1628,11 +1628,17 @@ struct Fire {
     }
   }
   // Minimizer::getDF_DX, Minimizer.cpp:90-115 (+ DFT.cpp:348-358). Returns 1 on eta too large.
+  std::vector<double> fixed_dir;  // Minimizer::fixed_direction_ (empty: none)
   int getDF_DX() {
     set_rho_from_alias();
     if (dft_eval(*dft, rho.data(), dF.data())) return 1;
     calls++;
     F = dft->parts[0];
+    if (!fixed_dir.empty()) {  // Minimizer.cpp:103-110: dF -= 2 (e . dF) e
+      double d = 0.0;
+      for (long i = 0; i < N; ++i) d += fixed_dir[i] * dF[i];
+      for (long i = 0; i < N; ++i) dF[i] += -2 * d * fixed_dir[i];
+    }
     dF_raw = dF;
     for (long i = 0; i < N; ++i) {  // FMT_Species.cpp:74-93
       const double y = x[i];
@@ -1888,6 +1894,22 @@ orc_fire* orc_fire_create(orc_dft* h, const double* rho0, double dt, double dt_m
   return reinterpret_cast<orc_fire*>(f);
 }
 void orc_fire_destroy(orc_fire* h) { delete reinterpret_cast<Fire*>(h); }
+void orc_fire_set_fixed_direction(orc_fire* h, const double* dir) {
+  auto* f = reinterpret_cast<Fire*>(h);
+  if (dir) f->fixed_dir.assign(dir, dir + f->N);
+  else f->fixed_dir.clear();
+}
+// fireMinimizer2::reset (Minimizer.cpp:149-158): aliases from rho, dt_best = dt, getDF_DX, v = 0 (dt, alpha and the
+// step counters are kept).
+int orc_fire_reset(orc_fire* h, const double* rho) {
+  auto* f = reinterpret_cast<Fire*>(h);
+  f->rho.assign(rho, rho + f->N);
+  f->alias_from_rho();
+  f->dt_best = f->dt;
+  const int rc = f->getDF_DX();
+  std::fill(f->v.begin(), f->v.end(), 0.0);
+  return rc;
+}
 // One fireMinimizer2::step.  info[8] = F, fmax(alias), fmax_raw, dt, alpha, vv, rms_force, calls
 int orc_fire_step(orc_fire* h, double* info) {
   auto* f = reinterpret_cast<Fire*>(h);

@@ -156,3 +156,18 @@ def test_species_constraints_through_cpp_api(demo):
     assert rel(F8, p8[0]) <= 1e-11 and abs(dsum8 - d8.sum()) <= 1e-11 * np.abs(d8).sum()
     ox = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, mu=0.3, ideal=False, accurate_weights=True)
     assert rel(F8x, ox.eval(rho8)[0][0]) <= 1e-11
+
+
+def test_fire_fixed_direction_through_cpp_api(demo):
+    """Minimizer::set_fixed_direction (src/Minimizer.cpp:103-110) through fireMinimizer2."""
+    shape, dx = (16, 16, 16), 0.1
+    rho = seeded(shape, 0.5, 81, 0.1)
+    e = 2 * oracle.splitmix64_uniform(int(np.prod(shape)), 83).reshape(shape) - 1
+    e /= np.sqrt(np.sum(e * e))
+    o = oracle.DFT(shape, dx, 1.0, mu=0.4, accurate_weights=True)
+    f = oracle.Fire(o, rho)
+    f.set_fixed_direction(e)
+    f.reset(rho)
+    for _ in range(5):
+        _, info = f.step()
+    assert rel(demo["fire_dir"][0], info["F"]) <= 1e-10

@@ -120,3 +120,33 @@ def test_ddft_ideal_gas_mode_decay(cdft):
     c_meas = (rho[:, 0, 0] - 0.5) @ np.cos(2 * np.pi * 2 * x / 16) / 8
     c_exp = 0.5 * amp * np.exp(lam * r["time"])
     assert abs(c_meas / c_exp - 1) < 1e-5
+
+
+def test_fire_fixed_direction_and_reset_replay(cdft):
+    """Minimizer::set_fixed_direction and fireMinimizer2::reset (src/Minimizer.cpp:30-34, 103-110, 149-158)."""
+    shape, dx, hsd = (16, 16, 16), 0.1, 1.0
+    rho0 = density(shape, 0.5, 13, 0.1)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    e = np.cos(2 * np.pi * np.arange(16) / 16)[:, None, None] * np.ones(shape)
+    e /= np.linalg.norm(e)
+    o = oracle.DFT(shape, dx, hsd, mf=True, kT=1.0, mu=0.5)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mu=0.5, mean_field=True, kT=1.0)
+    fo, fg = oracle.Fire(o, rho0), cdft.Fire(s, rho0)
+    for f in (fo, fg):
+        f.set_fixed_direction(e)
+        assert f.reset(rho0) == 0
+    for i in range(24):
+        if i == 12:  # restart both from the current density, dt and alpha kept
+            for f in (fo, fg):
+                f.set_fixed_direction(None)
+                assert f.reset(fo.state()[0]) == 0
+        rc0, a = fo.step()
+        rc1, b = fg.step()
+        assert rc0 == rc1 == 0
+        assert close(a["F"], b["F"], 1e-10), (i, a["F"], b["F"])
+        assert close(a["dt"], b["dt"], 1e-12) and close(a["alpha"], b["alpha"], 1e-12)
+    r0, r1 = fo.state()[0], fg.state()[0]
+    assert np.abs(r0 - r1).max() <= 1e-10 * np.abs(r0).max()
+    _, _, _, d1 = fg.state()
+    _, _, _, d0 = fo.state()
+    assert np.abs(d0 - d1).max() <= 1e-9 * np.abs(d0).max()

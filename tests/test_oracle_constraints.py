@@ -135,3 +135,37 @@ def test_ddft_fixed_background_keeps_the_boundary():
     b = boundary_mask()
     assert np.array_equal(r[b], rho[b])
     assert np.max(np.abs(r[~b] - rho[~b])) > 1e-6
+
+
+def test_fire_fixed_direction_reflects_the_force():
+    """Minimizer::getDF_DX with a fixed direction e (src/Minimizer.cpp:103-110): dF' = dF - 2 (e.dF) e is the
+    reflection of the force in the plane orthogonal to e (|e| = 1): the e component changes sign, the rest stays."""
+    rho = density(8)
+    o = make()
+    e = np.random.default_rng(9).standard_normal(SHAPE)
+    e /= np.linalg.norm(e)
+    f = oracle.Fire(o, rho)
+    f.set_fixed_direction(e)
+    f.reset(rho)
+    r, _, v, d1 = f.state()
+    assert np.all(v == 0.0)
+    _, d0 = make().eval(r)
+    assert abs(np.sum(e * d1) + np.sum(e * d0)) <= 1e-12 * np.linalg.norm(d0)
+    perp0 = d0 - np.sum(e * d0) * e
+    perp1 = d1 - np.sum(e * d1) * e
+    assert np.max(np.abs(perp1 - perp0)) <= 1e-12 * np.abs(d0).max()
+    assert abs(np.linalg.norm(d1) - np.linalg.norm(d0)) <= 1e-12 * np.linalg.norm(d0)
+
+
+def test_fire_reset_keeps_dt_and_restarts_from_the_density():
+    rho = density(10)
+    o = make()
+    f = oracle.Fire(o, rho)
+    for _ in range(8):
+        _, info = f.step()
+    r = f.state()[0]
+    assert f.reset(r) == 0
+    r2, _, v, _ = f.state()
+    assert np.all(v == 0.0) and np.max(np.abs(r2 - r)) <= 1e-12 * r.max()
+    _, info2 = f.step()
+    assert info2["dt"] >= info["dt"] * 0.5  # dt is kept (a fresh minimizer would restart at 1e-3)
