@@ -67,6 +67,14 @@ class FireReport(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class OpenDdftReport(ctypes.Structure):
+    _fields_ = [("F", _d), ("dt", _d), ("time", _d), ("rhs_max", _d), ("calls", _i), ("restarts", _i),
+                ("evaluations", _i)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class DdftReport(ctypes.Structure):
     _fields_ = [("F", _d), ("dt", _d), ("time", _d), ("rhs_max", _d), ("mass", _d), ("calls", _i), ("restarts", _i)]
 
@@ -144,6 +152,11 @@ def lib():
         L.cdft_fire_set_fixed_direction.argtypes = [_p, _p, _i]
         L.cdft_fire_reset.argtypes = [_p, _p, _i]
         L.cdft_fire_destroy.argtypes = [_p]
+        L.cdft_ddft_open_create.argtypes = [_p, _p, _d, _d, _d, _d, ctypes.POINTER(_p)]
+        L.cdft_ddft_open_step.argtypes = [_p, ctypes.POINTER(OpenDdftReport)]
+        L.cdft_ddft_open_get_density.argtypes = [_p, _p]
+        L.cdft_ddft_open_destroy.argtypes = [_p]
+        L.cdft_dst1_3d.argtypes = [_i, _i, _i, _p]
         L.cdft_ddft_create.argtypes = [_p, _p, _d, _d, _d, ctypes.POINTER(_p)]
         L.cdft_ddft_step.argtypes = [_p, ctypes.POINTER(DdftReport)]
         L.cdft_ddft_set_max_time_step.argtypes = [_p, _d]
@@ -697,3 +710,44 @@ def fft_c2r(s, Nz, precision=FP64):
     out = np.zeros((Nx, Ny, Nz))
     _check(lib().cdft_fft_c2r(Nx, Ny, Nz, precision, _ptr(s), _ptr(out)))
     return out
+
+
+class OpenDDFT:
+    """Open-boundary DDFT on the device (DDFT_IF_Open, mduran/src/DDFT_IF_Open.cpp:25-463): the density is held
+    at `background` on the planes ix, iy or iz = 0, the interior evolves through 3D DST-I transforms."""
+
+    def __init__(self, system, rho0, background, dt=0.0, dt_max=0.0, tol=0.0):
+        self.system = system
+        r = np.ascontiguousarray(rho0, dtype=np.float64)
+        h = _p()
+        _check(lib().cdft_ddft_open_create(system.h, _ptr(r), float(background), float(dt), float(dt_max), float(tol),
+                                           ctypes.byref(h)))
+        self.h = h
+
+    def step(self):
+        r = OpenDdftReport()
+        _check(lib().cdft_ddft_open_step(self.h, ctypes.byref(r)))
+        return r.as_dict()
+
+    def density(self):
+        rho = np.zeros(self.system.shape)
+        _check(lib().cdft_ddft_open_get_density(self.h, _ptr(rho)))
+        return rho
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_ddft_open_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dst1_3d(a):
+    """3D DST-I (FFTW RODFT00) of a compact array on the GPU."""
+    a = np.array(a, dtype=np.float64, order="C", copy=True)
+    _check(lib().cdft_dst1_3d(*a.shape, _ptr(a)))
+    return a

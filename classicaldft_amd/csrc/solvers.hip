@@ -215,6 +215,26 @@ __global__ void k_ddft_accept(double* __restrict__ d1, const double* __restrict_
   reduce_store<1, 2>(s, mx, partials);
 }
 
+// Open-boundary DDFT (fftDiffusion's unpack, mduran/src/DDFT_IF_Open.cpp:325-339, 353-375): interior sites take
+// S/norm + bg from the compact back-transform, boundary sites (ix, iy or iz = 0) the background.  Reductions:
+// max |d1_old - d1_new| over the interior, max(-d1_new) over all sites.
+__global__ void k_open_accept(double* __restrict__ d1, const double* __restrict__ S, int Nx, int Ny, int Nz,
+                              double norm, double bg, double2* __restrict__ partials) {
+  double mx[2] = {0.0, -1e300};
+  const long N = (long)Nx * Ny * Nz;
+  for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < N; p += (long)gridDim.x * kThreads) {
+    const int iz = (int)(p % Nz), iy = (int)((p / Nz) % Ny), ix = (int)(p / ((long)Nz * Ny));
+    double v = bg;
+    if (ix && iy && iz) {
+      v = S[(((long)ix - 1) * (Ny - 1) + iy - 1) * (Nz - 1) + iz - 1] / norm + bg;
+      mx[0] = fmax(mx[0], fabs(d1[p] - v));
+    }
+    d1[p] = v;
+    mx[1] = fmax(mx[1], -v);
+  }
+  reduce_store<0, 2>(nullptr, mx, partials);
+}
+
 __global__ void k_copy(const double* __restrict__ a, double* __restrict__ b, long N) {
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) b[i] = a[i];
 }
@@ -527,6 +547,133 @@ struct cdft_ddft_s {
   }
 };
 
+// ============================ open-boundary DDFT ============================
+// DDFT_IF_Open (mduran/src/DDFT_IF_Open.cpp:25-463): the density is held at `bg` on the planes ix, iy or
+// iz = 0 and the interior evolves with the integrating factor of the Dirichlet Laplacian (3D DST-I,
+// dst.hpp).  step() = step_string (:94-167) + the time-step bookkeeping of step (:171-184), with the
+// reference's control flow: at most 20 fixed-point iterations, dt halved on restart and restored after.
+struct cdft_ddft_open_s {
+  cdft_system sys = nullptr;
+  long N = 0, ni = 0;
+  int n[3] = {0, 0, 0};  // interior extents Nx-1, Ny-1, Nz-1
+  double dx = 0, dy = 0, dz = 0, bg = 0;
+  double dt = 0, dtMax = 1.0, tol = 1e-4, F = 0, time = 0, rhs_max = 0;
+  int successes = 0, calls = 0, restarts = 0, evals = 0;
+  std::vector<double> lam;  // Lx | Ly | Lz (:58-82)
+  DevBuf d0, d1, dF, tmp, S, R0s, R1s, lamd, ex;
+  Radices rad[3];
+  DevBuf tw[3];
+  int nbl[3] = {1, 1, 1};
+  Reducer red;
+  double parts[5] = {0, 0, 0, 0, 0};
+
+  int eval(const double* rho) {
+    evals++;
+    enqueue_eval(sys, rho, dF.as<double>(), true, false);
+    const int rc = fetch_parts(sys, parts);
+    if (rc == CDFT_OK) F = parts[0];
+    return rc;
+  }
+  void dst3(double* a) {
+    for (int ax = 2; ax >= 0; --ax) {
+      const long nlines = ni / n[ax];
+      hipLaunchKernelGGL(k_dst1, dim3(div_up(nlines, nbl[ax])), dim3(256),
+                         sizeof(cplx<double>) * (size_t)nbl[ax] * rad[ax].L, sys->stream, a, n[0], n[1], n[2], ax,
+                         rad[ax], tw[ax].as<cplx<double>>(), nbl[ax]);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  void pack(const double* x, double val, double* o) {
+    hipLaunchKernelGGL(k_open_pack, vgrid(ni), dim3(kThreads), 0, sys->stream, x, o, sys->g.Nx, sys->g.Ny,
+                       sys->g.Nz, val);
+    HIPCHK(hipGetLastError());
+  }
+  // calcNonlinearTerm (:377-455) of d with the current dF, packed and transformed
+  void rhs_sin(const double* d, DevBuf& out) {
+    const Geo& g = sys->g;
+    hipLaunchKernelGGL(k_ddft_rhs, vgrid(N), dim3(kThreads), 0, sys->stream, d, dF.as<double>(), tmp.as<double>(),
+                       g.Nx, g.Ny, g.Nz, 1 / (2 * dx * dx), 1 / (2 * dy * dy), 1 / (2 * dz * dz), 1 / (dx * dx),
+                       1 / (dy * dy), 1 / (dz * dz), 1.0 / (dx * dy * dz), 0, red.part.as<double2>());
+    HIPCHK(hipGetLastError());
+    pack(tmp.as<double>(), 0.0, out.as<double>());
+    dst3(out.as<double>());
+  }
+  void copy(const DevBuf& a, DevBuf& b, long n_) {
+    hipLaunchKernelGGL(k_copy, vgrid(n_), dim3(kThreads), 0, sys->stream, a.as<double>(), b.as<double>(), n_);
+    HIPCHK(hipGetLastError());
+  }
+  // fftDiffusion (:268-339) from d0 into d1; returns the max deviation, sets the minimum of the new d1
+  double diffuse(double dtl, double& mn) {
+    pack(d0.as<double>(), bg, S.as<double>());
+    dst3(S.as<double>());
+    std::vector<double> e(lam.size());
+    for (size_t i = 0; i < lam.size(); ++i) e[i] = std::exp(lam[i] * dtl);
+    HIPCHK(hipMemcpyAsync(ex.p, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice, sys->stream));
+    hipLaunchKernelGGL(k_open_update, vgrid(ni), dim3(kThreads), 0, sys->stream, S.as<double>(), R0s.as<double>(),
+                       R1s.as<double>(), lamd.as<double>(), ex.as<double>(), n[0], n[1], n[2], dtl);
+    HIPCHK(hipGetLastError());
+    dst3(S.as<double>());
+    const Geo& g = sys->g;
+    hipLaunchKernelGGL(k_open_accept, vgrid(N), dim3(kThreads), 0, sys->stream, d1.as<double>(), S.as<double>(),
+                       g.Nx, g.Ny, g.Nz, 8.0 * g.Nx * g.Ny * g.Nz, bg, red.part.as<double2>());
+    HIPCHK(hipGetLastError());
+    double r[2];
+    red.finish(sys->stream, 0, 2, r);  // synchronises: e stays alive until the copy is done
+    mn = -r[1];
+    return r[0];
+  }
+  int step() {
+    double dtl = dt;
+    int rc = eval(d0.as<double>());
+    if (rc) return rc;
+    rhs_sin(d0.as<double>(), R0s);
+    {
+      double r;
+      red.finish(sys->stream, 0, 1, &r);  // max |g . dF| / dV of k_ddft_rhs
+      rhs_max = r / (dx * dy * dz);
+    }
+    copy(d0, d1, N);
+    copy(R0s, R1s, ni);
+    double deviation = 1;
+    bool restart;
+    int halvings = 0;
+    do {
+      restart = false;
+      double old_error = 0;
+      for (int i = 0; i < 20 && deviation > tol && !restart; ++i) {
+        double mn;
+        deviation = diffuse(dtl, mn);
+        if (mn < 0 || (i > 0 && old_error < deviation)) restart = true;
+        else {
+          rc = eval(d1.as<double>());
+          if (rc) return rc;
+          rhs_sin(d1.as<double>(), R1s);
+        }
+        old_error = deviation;
+      }
+      if (deviation > tol) restart = true;
+      if (restart) {
+        REQUIRE(++halvings <= 60, "open DDFT: the time step could not be reduced enough");
+        dtl /= 2;
+        restarts++;
+        copy(d0, d1, N);
+        copy(R0s, R1s, ni);
+      }
+    } while (restart);
+    copy(d1, d0, N);
+    HIPCHK(hipStreamSynchronize(sys->stream));
+    calls++;
+    time += dtl;
+    if (dtl < dt * 1.01) successes = 0;
+    else successes++;
+    if (successes >= 5 && dt < dtMax) {
+      dt = std::min(2 * dtl, dtMax);
+      successes = 0;
+    }
+    return CDFT_OK;
+  }
+};
+
 namespace {
 void default_fire_params(cdft_fire_params* p) {  // fireMinimizer2 constructor, Minimizer.cpp:117-147
   p->dt = 1e-3;
@@ -758,6 +905,116 @@ int cdft_ddft_get_density(cdft_ddft d, double* rho) {
 int cdft_ddft_destroy(cdft_ddft d) {
   return guarded([&] {
     delete d;
+    return CDFT_OK;
+  });
+}
+
+int cdft_ddft_open_create(cdft_system sys, const double* rho0, double background, double dt, double dt_max, double tol,
+                          cdft_ddft_open* out) {
+  return guarded([&] {
+    REQUIRE(sys && rho0 && out, "null argument");
+    REQUIRE(sys->nb == 1, "the open DDFT runs on single systems");
+    REQUIRE(!(sys->d.fixed_mass > 0) && sys->d.boundary == 0,
+            "the open DDFT fixes the boundary itself: no species constraints");
+    const Geo& g = sys->g;
+    REQUIRE(g.Nx > 1 && g.Ny > 1 && g.Nz > 1, "the open DDFT needs at least two points per axis");
+    for (int L : {2 * g.Nx, 2 * g.Ny, 2 * g.Nz})
+      REQUIRE(smooth23(L) && L <= 1024, "the open DDFT needs Nx, Ny, Nz whose doubles are products of 2 and 3, <= 1024");
+    auto d = std::make_unique<cdft_ddft_open_s>();
+    d->sys = sys;
+    d->N = sys->N;
+    d->n[0] = g.Nx - 1;
+    d->n[1] = g.Ny - 1;
+    d->n[2] = g.Nz - 1;
+    d->ni = (long)d->n[0] * d->n[1] * d->n[2];
+    d->dx = sys->d.dx;
+    d->dy = sys->d.dy;
+    d->dz = sys->d.dz;
+    d->bg = background;
+    d->dt = dt > 0 ? dt : 0.0001 * d->dx * d->dx;
+    d->dtMax = dt_max > 0 ? dt_max : 1.0;
+    d->tol = tol > 0 ? tol : 1e-4;
+    const int Ns[3] = {g.Nx, g.Ny, g.Nz};
+    const double Ds[3] = {1 / (d->dx * d->dx), 1 / (d->dy * d->dy), 1 / (d->dz * d->dz)};
+    for (int ax = 0; ax < 3; ++ax) {
+      for (int i = 0; i < Ns[ax] - 1; ++i) d->lam.push_back(2 * Ds[ax] * (std::cos((M_PI * (i + 1)) / Ns[ax]) - 1));
+      d->rad[ax] = make_radices(2 * Ns[ax]);
+      upload_twiddles<double>(d->tw[ax], 2 * Ns[ax]);
+      d->nbl[ax] = std::max(1, (256 * kDstPts) / (2 * Ns[ax]));
+    }
+    const size_t b = sizeof(double) * d->N, bi = sizeof(double) * d->ni;
+    for (DevBuf* q : {&d->d0, &d->d1, &d->dF, &d->tmp}) q->alloc(b);
+    for (DevBuf* q : {&d->S, &d->R0s, &d->R1s}) q->alloc(bi);
+    d->lamd.alloc(sizeof(double) * d->lam.size());
+    d->ex.alloc(sizeof(double) * d->lam.size());
+    HIPCHK(hipMemcpy(d->lamd.p, d->lam.data(), sizeof(double) * d->lam.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d->d0.p, rho0, b, hipMemcpyHostToDevice));
+    d->red.init();
+    const int rc = d->eval(d->d0.as<double>());
+    if (rc) return rc;
+    *out = d.release();
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_ddft_open_step(cdft_ddft_open d, cdft_ddft_open_report* rep) {
+  return guarded([&] {
+    REQUIRE(d, "null argument");
+    const int rc = d->step();
+    if (rep) {
+      rep->F = d->F;
+      rep->dt = d->dt;
+      rep->time = d->time;
+      rep->rhs_max = d->rhs_max;
+      rep->calls = d->calls;
+      rep->restarts = d->restarts;
+      rep->evaluations = d->evals;
+    }
+    return rc;
+  });
+}
+
+int cdft_ddft_open_get_density(cdft_ddft_open d, double* rho) {
+  return guarded([&] {
+    REQUIRE(d && rho, "null argument");
+    HIPCHK(hipMemcpy(rho, d->d0.p, sizeof(double) * d->N, hipMemcpyDeviceToHost));
+    return CDFT_OK;
+  });
+}
+
+int cdft_ddft_open_destroy(cdft_ddft_open d) {
+  return guarded([&] {
+    delete d;
+    return CDFT_OK;
+  });
+}
+
+// Standalone 3D DST-I of a compact (n0, n1, n2) host array, in place (test and interop helper).
+int cdft_dst1_3d(int n0, int n1, int n2, double* a) {
+  return guarded([&] {
+    REQUIRE(a && n0 > 0 && n1 > 0 && n2 > 0, "bad argument");
+    const int ns[3] = {n0, n1, n2};
+    for (int ax = 0; ax < 3; ++ax)
+      REQUIRE(smooth23(2 * (ns[ax] + 1)) && 2 * (ns[ax] + 1) <= 1024, "DST-I lengths n need 2(n+1) a product of 2 and 3, <= 1024");
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    const long ni = (long)n0 * n1 * n2;
+    DevBuf buf, tw[3];
+    buf.alloc(sizeof(double) * ni);
+    HIPCHK(hipMemcpyAsync(buf.p, a, sizeof(double) * ni, hipMemcpyHostToDevice, st));
+    for (int ax = 2; ax >= 0; --ax) {
+      const int L = 2 * (ns[ax] + 1);
+      const Radices rad = make_radices(L);
+      upload_twiddles<double>(tw[ax], L);
+      const int nbl = std::max(1, (256 * kDstPts) / L);
+      const long nlines = ni / ns[ax];
+      hipLaunchKernelGGL(k_dst1, dim3(div_up(nlines, nbl)), dim3(256), sizeof(cplx<double>) * (size_t)nbl * L, st,
+                         buf.as<double>(), n0, n1, n2, ax, rad, tw[ax].as<cplx<double>>(), nbl);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(a, buf.p, sizeof(double) * ni, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamDestroy(st));
     return CDFT_OK;
   });
 }

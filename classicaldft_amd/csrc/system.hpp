@@ -20,6 +20,7 @@
 #include "rows2.hpp"
 #include "weights.hpp"
 #include "constraints.hpp"
+#include "dst.hpp"
 
 using namespace cdft;
 

@@ -594,6 +594,31 @@ double DDFT::step() {
   sync_species();
   return F_;
 }
+DDFT_IF_Open::DDFT_IF_Open(DFT* dft, double background, bool) : Minimizer(dft), background_(background) {
+  const double dx = dft->get_lattice().getDX();
+  dt_ = 0.0001 * dx * dx;
+}
+DDFT_IF_Open::~DDFT_IF_Open() {
+  if (ddft_) cdft_ddft_open_destroy(ddft_);
+}
+void DDFT_IF_Open::ensure() {
+  if (ddft_) return;
+  check(cdft_ddft_open_create(dft_->system(), dft_->getSpecies(0)->getDensity().get_density_pointer(), background_,
+                              dt_, dtMax_, tol_, &ddft_));
+}
+double DDFT_IF_Open::step() {
+  ensure();
+  check(cdft_ddft_open_step(ddft_, &rep_));
+  F_ = rep_.F;
+  calls_ = rep_.calls;
+  sync_species();
+  return F_;
+}
+void DDFT_IF_Open::density(DFT_Vec& out) const {
+  out.resize(dft_->get_lattice().Ntot());
+  check(cdft_ddft_open_get_density(ddft_, out.memptr()));
+}
+
 void DDFT::density(DFT_Vec& out) const {
   out.resize(dft_->get_lattice().Ntot());
   check(cdft_ddft_get_density(ddft_, out.memptr()));

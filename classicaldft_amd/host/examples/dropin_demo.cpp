@@ -14,6 +14,7 @@
 //   fixed_mass_fire <F> <mu> <N>        then 10 fireMinimizer2 steps
 //   fixed_bg <F> <sum dF> <F onlyFex>   Species::setFixedBackground on another 16^3 system
 //   fire_dir <F>            5 fireMinimizer2 steps with Minimizer::set_fixed_direction (ideal + WB I, mu = 0.4)
+//   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -167,5 +168,18 @@ int main() {
   fire9.set_fixed_direction(e);
   for (int i = 0; i < 5; ++i) fire9.step();
   std::printf("fire_dir %.17g\n", fire9.getF());
+
+  Density d10(0.1, L2);
+  for (long i = 0; i < d10.Ntot(); ++i) d10.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(91, i) - 1.0)));
+  FMT_Species s10(d10, 1.0, 0.0);
+  Interaction_Interpolation_QF qf10(&s10, &s10, &lj, 1.0);
+  WhiteBearI fmt10;
+  DFT dft10(&s10);
+  dft10.addHardCoreContribution(&fmt10);
+  dft10.addInteraction(&qf10);
+  DDFT_IF_Open open10(&dft10, 0.5);
+  open10.step();
+  open10.step();
+  std::printf("open_ddft %.17g %.17g %.17g\n", open10.getF(), d10.get(0), d10.get((8 * 16 + 8) * 16 + 8));
   return 0;
 }

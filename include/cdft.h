@@ -248,6 +248,26 @@ int cdft_ddft_set_max_time_step(cdft_ddft d, double dt_max);
 int cdft_ddft_get_density(cdft_ddft d, double* rho);
 int cdft_ddft_destroy(cdft_ddft d);
 
+/* Open-boundary DDFT: DDFT_IF_Open (mduran/src/DDFT_IF_Open.cpp:25-463, declared in
+   include/Obsolete/Minimizer_old.h:288-325).  The density is held at `background` on the planes ix, iy or
+   iz = 0; the interior evolves with the integrating factor of the Dirichlet Laplacian through 3D DST-I
+   transforms.  step = DDFT_IF_Open::step (step_string: at most 20 fixed-point iterations, dt halved on a
+   restart and restored for the next step).  Nx, Ny, Nz: 2N a product of 2 and 3, <= 1024.  Systems
+   without species constraints; dt = 0 gives 1e-4 dx^2, dt_max and tol <= 0 the defaults 1 and 1e-4. */
+typedef struct cdft_ddft_open_s* cdft_ddft_open;
+typedef struct cdft_ddft_open_report {
+  double F, dt, time;
+  double rhs_max;  /* max |g . dF| / dV^2 of the step's starting density (DDFT::get_convergence_monitor) */
+  int calls, restarts, evaluations;
+} cdft_ddft_open_report;
+int cdft_ddft_open_create(cdft_system sys, const double* rho0, double background, double dt, double dt_max, double tol,
+                          cdft_ddft_open* out);
+int cdft_ddft_open_step(cdft_ddft_open d, cdft_ddft_open_report* rep);
+int cdft_ddft_open_get_density(cdft_ddft_open d, double* rho);
+int cdft_ddft_open_destroy(cdft_ddft_open d);
+/* 3D DST-I (FFTW RODFT00) of a compact (n0, n1, n2) host array, in place, on the GPU. */
+int cdft_dst1_3d(int n0, int n1, int n2, double* a);
+
 /* Device memory plumbing for callers without their own HIP runtime. */
 int cdft_malloc(void** p, unsigned long long bytes);
 int cdft_free(void* p);

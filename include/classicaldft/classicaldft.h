@@ -520,4 +520,28 @@ class DDFT : public Minimizer {
   cdft_ddft_report rep_{};
 };
 
+// DDFT_IF_Open (include/Obsolete/Minimizer_old.h:288-325, mduran/src/DDFT_IF_Open.cpp:25-463): the density is
+// held at `background` on the planes ix, iy or iz = 0 (an open system in contact with a reservoir) and the
+// interior evolves through 3D DST-I transforms on the GPU.
+class DDFT_IF_Open : public Minimizer {
+ public:
+  DDFT_IF_Open(DFT* dft, double background, bool showGraphics = false);
+  ~DDFT_IF_Open() override;
+  double step() override;
+  void reset() override {}
+  void set_tolerence_fixed_point(double e) { tol_ = e; }
+  void set_max_time_step(double t) { dtMax_ = t; }
+  void setTimeStep(double dt) { dt_ = dt; }
+  double getTimeStep() const { return rep_.dt > 0 ? rep_.dt : dt_; }
+  double get_time() const { return rep_.time; }
+  double get_convergence_monitor() const override { return rep_.rhs_max; }
+  void density(DFT_Vec& out) const override;
+
+ private:
+  void ensure();
+  double background_, dt_ = 0.0, dtMax_ = 1.0, tol_ = 1e-4;
+  cdft_ddft_open ddft_ = nullptr;
+  cdft_ddft_open_report rep_{};
+};
+
 #endif  // CLASSICALDFT_AMD_H

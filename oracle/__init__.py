@@ -89,6 +89,13 @@ def lib():
         L.orc_ddft_step.argtypes = [_p, _p]
         L.orc_ddft_step.restype = _i
         L.orc_ddft_get.argtypes = [_p, _p]
+        L.orc_dst1_3d.argtypes = [_i, _i, _i, _p]
+        L.orc_open_create.argtypes = [_p, _p, _d, _d, _d, _d]
+        L.orc_open_create.restype = _p
+        L.orc_open_destroy.argtypes = [_p]
+        L.orc_open_step.argtypes = [_p, _p]
+        L.orc_open_step.restype = _i
+        L.orc_open_get.argtypes = [_p, _p]
         _LIB = L
     return _LIB
 
@@ -381,3 +388,40 @@ def splitmix64_uniform(n, seed):
         z = z ^ (z >> np.uint64(31))
     out[:] = (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
     return out
+
+
+def dst1_3d(a):
+    """3D DST-I (FFTW RODFT00) of a compact array, by direct sums."""
+    a = np.array(a, dtype=np.float64, order="C", copy=True)
+    lib().orc_dst1_3d(*a.shape, _ptr(a))
+    return a
+
+
+class OpenDDFT:
+    """DDFT_IF_Open: DDFT with the density fixed at `background` on the planes ix, iy or iz = 0."""
+
+    def __init__(self, dft, rho0, background, dt=0.0, dt_max=1.0, tol=1e-4):
+        self.dft = dft
+        self.shape = dft.shape
+        r = np.ascontiguousarray(rho0, dtype=np.float64)
+        self.h = lib().orc_open_create(dft.h, _ptr(r), float(background), dt, dt_max, tol)
+
+    def step(self):
+        info = np.zeros(6)
+        rc = lib().orc_open_step(self.h, _ptr(info))
+        if rc == 1:
+            raise EtaTooLarge("eta too large")
+        if rc == 2:
+            raise RuntimeError("open DDFT: the time step could not be reduced enough")
+        return dict(F=info[0], dt=info[1], time=info[2], calls=int(info[3]), restarts=int(info[4]),
+                    evaluations=int(info[5]))
+
+    def density(self):
+        rho = np.zeros(self.shape)
+        lib().orc_open_get(self.h, _ptr(rho))
+        return rho
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_open_destroy(self.h)
+            self.h = None

@@ -1970,3 +1970,202 @@ void orc_ddft_get(orc_ddft* h, double* rho) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Open-boundary DDFT: DDFT_IF_Open (mduran/src/DDFT_IF_Open.cpp:25-463, declared in
+// include/Obsolete/Minimizer_old.h:288-325).  The density is held at `background` on the planes ix, iy or
+// iz = 0; the interior (Nx-1)(Ny-1)(Nz-1) points evolve with the integrating factor of the Dirichlet
+// Laplacian, diagonalised by the 3D DST-I (FFTW RODFT00: Y_k = 2 sum_j X_j sin(pi (j+1)(k+1)/(n+1)),
+// applied twice = 2(n+1) x identity per axis).
+// ---------------------------------------------------------------------------
+namespace {
+// 3D DST-I of a compact (n0, n1, n2) array (z fastest), in place, by direct sums along each axis.
+void dst1_3d(int n0, int n1, int n2, double* a) {
+  const int n[3] = {n0, n1, n2};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int L = n[ax];
+    std::vector<double> S((size_t)L * L), in(L);
+    for (int j = 0; j < L; ++j)
+      for (int k = 0; k < L; ++k) S[(size_t)j * L + k] = 2.0 * std::sin(M_PI * (j + 1.0) * (k + 1.0) / (L + 1.0));
+    const long es = ax == 2 ? 1 : ax == 1 ? n2 : (long)n1 * n2;
+    const long nlines = (long)n0 * n1 * n2 / L;
+    for (long line = 0; line < nlines; ++line) {
+      long base;
+      if (ax == 2) base = line * n2;
+      else if (ax == 1) base = (line / n2) * (long)n1 * n2 + line % n2;
+      else base = line;
+      for (int j = 0; j < L; ++j) in[j] = a[base + j * es];
+      for (int k = 0; k < L; ++k) {
+        double acc = 0.0;
+        for (int j = 0; j < L; ++j) acc += in[j] * S[(size_t)j * L + k];
+        a[base + k * es] = acc;
+      }
+    }
+  }
+}
+
+struct OpenDdft {
+  DftState* dft;
+  Grid g;
+  double dx, dy, dz, bg;
+  double dt, dtMax = 1.0, tol = 1e-4, F = 0.0, time = 0.0;
+  int successes = 0, calls = 0, restarts = 0, evals = 0;
+  std::vector<double> Lx, Ly, Lz;
+  std::vector<double> rho, dF, R0s, R1s;
+  long ni() const { return (long)(g.Nx - 1) * (g.Ny - 1) * (g.Nz - 1); }
+  int eval(std::vector<double>& d) {
+    evals++;
+    std::vector<double> tmp(d);
+    if (dft_eval(*dft, tmp.data(), dF.data())) return 1;
+    F = dft->parts[0];
+    return 0;
+  }
+  // pack_for_sin_transform (:341-351): interior values minus val
+  std::vector<double> pack(const std::vector<double>& x, double val) const {
+    std::vector<double> o(ni());
+    long pos = 0;
+    for (int ix = 1; ix < g.Nx; ++ix)
+      for (int iy = 1; iy < g.Ny; ++iy)
+        for (int iz = 1; iz < g.Nz; ++iz) o[pos++] = x[iz + g.Nz * (iy + (long)g.Ny * ix)] - val;
+    return o;
+  }
+  // calcNonlinearTerm (:377-455) with periodic neighbours, then its DST of the interior
+  std::vector<double> rhs_sin(const std::vector<double>& d) const {
+    const double Dx = 1 / (dx * dx), Dy = 1 / (dy * dy), Dz = 1 / (dz * dz), dV = dx * dy * dz;
+    std::vector<double> r(g.N());
+    auto P = [&](int ix, int iy, int iz) { return pbc(iz, g.Nz) + g.Nz * (pbc(iy, g.Ny) + (long)g.Ny * pbc(ix, g.Nx)); };
+    for (int ix = 0; ix < g.Nx; ++ix)
+      for (int iy = 0; iy < g.Ny; ++iy)
+        for (int iz = 0; iz < g.Nz; ++iz) {
+          const long i0 = P(ix, iy, iz), px = P(ix + 1, iy, iz), mx = P(ix - 1, iy, iz), py = P(ix, iy + 1, iz),
+                     my = P(ix, iy - 1, iz), pz = P(ix, iy, iz + 1), mz = P(ix, iy, iz - 1);
+          const double r0 = d[i0], f0 = dF[i0];
+          double R = Dx * ((d[px] + r0) * (dF[px] - f0) - (r0 + d[mx]) * (f0 - dF[mx])) +
+                     Dy * ((d[py] + r0) * (dF[py] - f0) - (r0 + d[my]) * (f0 - dF[my])) +
+                     Dz * ((d[pz] + r0) * (dF[pz] - f0) - (r0 + d[mz]) * (f0 - dF[mz]));
+          R *= 1.0 / (2 * dV);
+          R -= Dx * (d[px] + d[mx] - 2 * r0) + Dy * (d[py] + d[my] - 2 * r0) + Dz * (d[pz] + d[mz] - 2 * r0);
+          r[i0] = R;
+        }
+    std::vector<double> o = pack(r, 0.0);
+    dst1_3d(g.Nx - 1, g.Ny - 1, g.Nz - 1, o.data());
+    return o;
+  }
+  // fftDiffusion (:268-339): propagates rho (= d0) by dt into d1; returns the max deviation from d1's old
+  // interior values; sets the minimum of the new d1.
+  double diffuse(double dtl, std::vector<double>& d1, double& mn) const {
+    std::vector<double> x = pack(rho, bg);
+    dst1_3d(g.Nx - 1, g.Ny - 1, g.Nz - 1, x.data());
+    long pos = 0;
+    for (int ix = 0; ix < g.Nx - 1; ++ix)
+      for (int iy = 0; iy < g.Ny - 1; ++iy)
+        for (int iz = 0; iz < g.Nz - 1; ++iz, ++pos) {
+          const double Lam = Lx[ix] + Ly[iy] + Lz[iz];
+          const double fac = std::exp(Lx[ix] * dtl) * std::exp(Ly[iy] * dtl) * std::exp(Lz[iz] * dtl);
+          double v = x[pos] * fac;
+          v += ((fac - 1) / Lam) * R0s[pos];
+          v += ((fac - 1 - dtl * Lam) / (Lam * Lam * dtl)) * (R1s[pos] - R0s[pos]);
+          x[pos] = v;
+        }
+    dst1_3d(g.Nx - 1, g.Ny - 1, g.Nz - 1, x.data());
+    const double norm = 8.0 * g.Nx * g.Ny * g.Nz;
+    double maxdev = 0.0;
+    pos = 0;
+    mn = 1e300;
+    for (int ix = 0; ix < g.Nx; ++ix)
+      for (int iy = 0; iy < g.Ny; ++iy)
+        for (int iz = 0; iz < g.Nz; ++iz) {
+          const long p = iz + g.Nz * (iy + (long)g.Ny * ix);
+          double v = bg;
+          if (ix && iy && iz) {
+            v = x[pos++] / norm + bg;
+            maxdev = std::max(maxdev, std::fabs(d1[p] - v));
+          }
+          d1[p] = v;
+          mn = std::min(mn, v);
+        }
+    return maxdev;
+  }
+  // step_string (:94-167) and step (:171-184)
+  int step() {
+    double dtl = dt;
+    if (eval(rho)) return 1;
+    R0s = rhs_sin(rho);
+    std::vector<double> d1 = rho;
+    R1s = R0s;
+    double deviation = 1;
+    bool restart;
+    int halvings = 0;
+    do {
+      restart = false;
+      double old_error = 0;
+      for (int i = 0; i < 20 && deviation > tol && !restart; ++i) {
+        double mn;
+        deviation = diffuse(dtl, d1, mn);
+        if (mn < 0 || (i > 0 && old_error < deviation)) restart = true;
+        else {
+          if (eval(d1)) return 1;
+          R1s = rhs_sin(d1);
+        }
+        old_error = deviation;
+      }
+      if (deviation > tol) restart = true;
+      if (restart) {
+        if (++halvings > 60) return 2;
+        dtl /= 2;
+        restarts++;
+        d1 = rho;
+        R1s = R0s;
+      }
+    } while (restart);
+    rho = d1;
+    calls++;
+    time += dtl;
+    if (dtl < dt * 1.01) successes = 0;
+    else successes++;
+    if (successes >= 5 && dt < dtMax) { dt = std::min(2 * dtl, dtMax); successes = 0; }
+    return 0;
+  }
+};
+}  // namespace
+
+extern "C" {
+struct orc_open;
+void orc_dst1_3d(int n0, int n1, int n2, double* a) { dst1_3d(n0, n1, n2, a); }
+// DDFT_IF_Open(dft, background): dt = 1e-4 dx^2 unless dt > 0; the first step puts the boundary planes at the
+// background.
+orc_open* orc_open_create(orc_dft* h, const double* rho0, double background, double dt, double dtMax, double tol) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  auto* d = new OpenDdft();
+  d->dft = s;
+  d->g = s->fmt.g;
+  d->dx = s->fmt.dx; d->dy = s->fmt.dy; d->dz = s->fmt.dz;
+  d->bg = background;
+  d->dt = dt > 0 ? dt : 0.0001 * d->dx * d->dx;
+  d->dtMax = dtMax;
+  d->tol = tol;
+  d->rho.assign(rho0, rho0 + d->g.N());
+  d->dF.assign(d->g.N(), 0.0);
+  const double Dx = 1 / (d->dx * d->dx), Dy = 1 / (d->dy * d->dy), Dz = 1 / (d->dz * d->dz);
+  for (int i = 0; i < d->g.Nx - 1; ++i) d->Lx.push_back(2 * Dx * (std::cos(M_PI * (i + 1) / d->g.Nx) - 1));  // :58-82
+  for (int i = 0; i < d->g.Ny - 1; ++i) d->Ly.push_back(2 * Dy * (std::cos(M_PI * (i + 1) / d->g.Ny) - 1));
+  for (int i = 0; i < d->g.Nz - 1; ++i) d->Lz.push_back(2 * Dz * (std::cos(M_PI * (i + 1) / d->g.Nz) - 1));
+  d->eval(d->rho);
+  return reinterpret_cast<orc_open*>(d);
+}
+void orc_open_destroy(orc_open* h) { delete reinterpret_cast<OpenDdft*>(h); }
+// info[6] = F, dt, time, calls, restarts, evaluations
+int orc_open_step(orc_open* h, double* info) {
+  auto* d = reinterpret_cast<OpenDdft*>(h);
+  const int rc = d->step();
+  if (info) {
+    info[0] = d->F; info[1] = d->dt; info[2] = d->time; info[3] = d->calls; info[4] = d->restarts;
+    info[5] = d->evals;
+  }
+  return rc;
+}
+void orc_open_get(orc_open* h, double* rho) {
+  auto* d = reinterpret_cast<OpenDdft*>(h);
+  std::memcpy(rho, d->rho.data(), sizeof(double) * d->g.N());
+}
+}  // extern "C"

@@ -171,3 +171,16 @@ def test_fire_fixed_direction_through_cpp_api(demo):
     for _ in range(5):
         _, info = f.step()
     assert rel(demo["fire_dir"][0], info["F"]) <= 1e-10
+
+
+def test_open_ddft_through_cpp_api(demo):
+    """DDFT_IF_Open (mduran/src/DDFT_IF_Open.cpp:25-463) through the reference-named class."""
+    shape, dx = (16, 16, 16), 0.1
+    rho = seeded(shape, 0.5, 91, 0.1)
+    o = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0, accurate_weights=True)
+    d = oracle.OpenDDFT(o, rho, 0.5)
+    d.step()
+    info = d.step()
+    r = d.density()
+    F, r0, r8 = demo["open_ddft"]
+    assert rel(F, info["F"]) <= 1e-10 and r0 == 0.5 and rel(r8, r[8, 8, 8]) <= 1e-10
