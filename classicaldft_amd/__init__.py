@@ -7,8 +7,11 @@ no CPU fallback.
 from .cdft import (  # noqa: F401
     CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT, FP64, MIXED, MODEL_NONE, ROSENFELD, RSLT, ESFMT,
     WHITEBEAR_I, WHITEBEAR_II, DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, CdftError, EtaTooLarge, System, DeviceArray, Fire,
-    DDFT, OpenDDFT, dst1_3d, FireBatch, UNet, fft_r2c, fft_c2r, lj_hsd, potential_hsd, potential_vdw, potential_watt,
+    DDFT, OpenDDFT, dst1_3d, mf_kernel_points, FireBatch, UNet, fft_r2c, fft_c2r, lj_hsd, potential_hsd, potential_vdw, potential_watt,
     POT_LJ, POT_TWF, POT_WHDF, MF_QF, MF_ZERO, MF_LE, MF_QE, MF_LF, MF_GAUSS_E, MF_GAUSS_F,
     device_count, library_path,
 )
-from .formats import read_density, write_density  # noqa: F401,E402
+from .formats import (  # noqa: F401,E402
+    read_density, write_density, read_points_file, read_weight_cache, write_weight_cache, weight_cache_name,
+    potential_identifier,
+)

@@ -157,6 +157,7 @@ def lib():
         L.cdft_ddft_open_get_density.argtypes = [_p, _p]
         L.cdft_ddft_open_destroy.argtypes = [_p]
         L.cdft_dst1_3d.argtypes = [_i, _i, _i, _p]
+        L.cdft_mf_kernel_points.argtypes = [_i, _i, _i, _d, _d, _d, ctypes.POINTER(_PotDesc), _p, _i, _i, _p]
         L.cdft_ddft_create.argtypes = [_p, _p, _d, _d, _d, ctypes.POINTER(_p)]
         L.cdft_ddft_step.argtypes = [_p, ctypes.POINTER(DdftReport)]
         L.cdft_ddft_set_max_time_step.argtypes = [_p, _d]
@@ -751,3 +752,17 @@ def dst1_3d(a):
     a = np.array(a, dtype=np.float64, order="C", copy=True)
     _check(lib().cdft_dst1_3d(*a.shape, _ptr(a)))
     return a
+
+
+def mf_kernel_points(shape, dx, points, potential=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, bh=False,
+                     r_att_min=0.0, nradial=0):
+    """Points-file mean-field kernel on the GPU (Interaction::generateWeights, src/Interaction.cpp:672-860), before
+    the division by kT: pass kernel / kT as System(mf_kernel=...).  points: (n, 4) rows (ux, uy, uz, weight) as
+    formats.read_points_file returns them."""
+    dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 4)
+    pd = _PotDesc(int(potential), float(sigma), float(eps), float(rcut), float(alpha), int(bool(bh)), float(r_att_min))
+    out = np.zeros(tuple(int(n) for n in shape))
+    _check(lib().cdft_mf_kernel_points(int(shape[0]), int(shape[1]), int(shape[2]), dxs[0], dxs[1], dxs[2],
+                                       ctypes.byref(pd), _ptr(pts), len(pts), int(nradial), _ptr(out)))
+    return out

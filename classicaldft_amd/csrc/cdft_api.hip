@@ -1193,6 +1193,43 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
   });
 }
 
+int cdft_mf_kernel_points(int Nx, int Ny, int Nz, double dx, double dy, double dz, const cdft_potential_desc* pot,
+                          const double* points, int npoints, int nradial, double* w_out) {
+  return guarded([&] {
+    REQUIRE(pot && points && w_out && npoints > 0 && Nx > 0 && Ny > 0 && Nz > 0, "bad argument");
+    REQUIRE(dx > 0 && dy > 0 && dz > 0, "lattice spacings must be positive");
+    const Pot v = pot_from(*pot);
+    const int nr = nradial > 0 ? nradial : 512;  // Nr = 128*4, src/Interaction.cpp:769
+    std::vector<double> x(nr), w(nr);
+    gauss_legendre01(nr, x.data(), w.data());
+    for (int k = 0; k < nr; ++k) {  // gsl_integration_glfixed_point(0, Rc, k, ...)
+      x[k] *= v.rcut;
+      w[k] *= v.rcut;
+    }
+    const long N = (long)Nx * Ny * Nz;
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    DevBuf dp, dr, dw, out;
+    dp.alloc(sizeof(double) * 4 * (size_t)npoints);
+    dr.alloc(sizeof(double) * nr);
+    dw.alloc(sizeof(double) * nr);
+    out.alloc(sizeof(double) * N);
+    HIPCHK(hipMemcpyAsync(dp.p, points, sizeof(double) * 4 * (size_t)npoints, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dr.p, x.data(), sizeof(double) * nr, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dw.p, w.data(), sizeof(double) * nr, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(out.p, 0, sizeof(double) * N, st));
+    const long total = (long)npoints * 48 * nr;
+    hipLaunchKernelGGL(k_mf_points, dim3((unsigned)std::min<long>(div_up(total, 256), 65536)), dim3(256), 0, st,
+                       dp.as<double>(), (long)npoints, dr.as<double>(), dw.as<double>(), nr, v, Nx, Ny, Nz, dx, dy, dz,
+                       out.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(w_out, out.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamDestroy(st));
+    return CDFT_OK;
+  });
+}
+
 int cdft_system_set_fixed_masses(cdft_system sys, const double* masses) {
   return guarded([&] {
     REQUIRE(sys && masses, "null argument");

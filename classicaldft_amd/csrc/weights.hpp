@@ -242,4 +242,50 @@ static __global__ void k_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy,
   out[p] = acc / kT;
 }
 
+// Points-file mean-field kernel (Interaction::generateWeights, src/Interaction.cpp:672-860): every direction u of
+// the points list (unit vector, weight), under the 6 axis permutations and 8 sign reflections, and every radial
+// Gauss-Legendre node r on [0, Rc] puts w = weight / 48 * wr r^2 Watt(r) on the 8 corners of the lattice cell
+// holding r u, with trilinear shares (periodic wrap).  One thread per (direction, permutation, reflection,
+// radial node); the corners accumulate with atomic adds, so the last bits of a cell depend on the order.
+// The kernel is returned before the division by kT, as the reference's weight cache stores it.
+static __global__ __launch_bounds__(256) void k_mf_points(const double* __restrict__ pts, long npts,
+                                                          const double* __restrict__ rr, const double* __restrict__ wr,
+                                                          int nr, Pot v, int Nx, int Ny, int Nz, double dx, double dy,
+                                                          double dz, double* __restrict__ out) {
+  const long total = npts * 48 * nr;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(t % nr);
+    const int is = (int)((t / nr) % 8);
+    const int iperm = (int)((t / ((long)nr * 8)) % 6);
+    const long pos = t / ((long)nr * 48);
+    // permutation (ii, jj, kk) of the direction's components, src/Interaction.cpp:780-791
+    const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    const double* u = pts + 4 * pos;
+    const double r = rr[k];
+    const double watt = v.watt2(r * r);
+    // reflections, :802-811: x flips for is in {1,4,5,7}, y for {2,4,6,7}, z for {3,5,6,7}, as bit masks over is
+    // (0xB2, 0xD4, 0xE8) and exact multiplications by -1; the equivalent chain of ifs was compiled for gfx950 with
+    // the is == 3 case of the z flip dropped
+    const double sx = ((0xB2 >> is) & 1) ? -1.0 : 1.0, sy = ((0xD4 >> is) & 1) ? -1.0 : 1.0,
+                 sz = ((0xE8 >> is) & 1) ? -1.0 : 1.0;
+    const double x = sx * (r * u[perm[iperm][0]]), y = sy * (r * u[perm[iperm][1]]), z = sz * (r * u[perm[iperm][2]]);
+    const double ww = u[3] / 48.0;
+    int ix0 = int(x / dx), iy0 = int(y / dy), iz0 = int(z / dz);
+    if (x < 0) ix0 -= 1;
+    if (y < 0) iy0 -= 1;
+    if (z < 0) iz0 -= 1;
+    const double ddx = (x / dx) - ix0, ddy = (y / dy) - iy0, ddz = (z / dz) - iz0;
+    for (int io = 0; io < 2; ++io)
+      for (int jo = 0; jo < 2; ++jo)
+        for (int ko = 0; ko < 2; ++ko) {
+          int nx = (ix0 + io) % Nx, ny = (iy0 + jo) % Ny, nz = (iz0 + ko) % Nz;
+          if (nx < 0) nx += Nx;
+          if (ny < 0) ny += Ny;
+          if (nz < 0) nz += Nz;
+          const double cc = (io == 0 ? 1 - ddx : ddx) * (jo == 0 ? 1 - ddy : ddy) * (ko == 0 ? 1 - ddz : ddz);
+          atomicAdd(out + (nz + (long)Nz * (ny + (long)Ny * nx)), cc * ww * wr[k] * r * r * watt);
+        }
+  }
+}
+
 }  // namespace cdft

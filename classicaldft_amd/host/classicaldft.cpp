@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <sstream>
 #include <functional>
 
@@ -106,7 +107,15 @@ double Density::getNumberAtoms() const {
 bool Density::has_field_() const { return vWall_.inf_norm() > 0.0; }
 
 // ------------------------------------------------------------------ Species
-Species::Species(Density& density, double mu, int) : density_(density), dF_(density.Ntot()), mu_(mu) {}
+int Species::SequenceNumber_ = 0;
+Species::Species(Density& density, double mu, int seq) : density_(density), dF_(density.Ntot()), mu_(mu) {
+  if (seq >= 0) {
+    seq_num_ = seq;
+    SequenceNumber_ = seq + 1;
+  } else {
+    seq_num_ = SequenceNumber_++;
+  }
+}
 
 void Species::beginForceCalculation() {
   if (fixedMass_ > 0.0) {
@@ -288,6 +297,8 @@ Interaction_Base::Interaction_Base(Species* s1, Species* s2, Potential1* v, doub
 }
 
 void Interaction_Base::describe(cdft_system_desc& d) const {
+  ensure_kernel();
+  d.mf_kernel = kernel_.empty() ? nullptr : kernel_.data();
   d.mean_field = 1;
   d.lj_sigma = v_->getSigma();
   d.lj_eps = v_->getEps();
@@ -319,6 +330,97 @@ double Interaction_Base::getVDWParameter() const {
     cdft_system_destroy(s);
   }
   return a_vdw_;
+}
+
+std::string Potential1::getIdentifier() const {
+  std::stringstream ss;
+  const char* name = kind() == CDFT_POT_TWF ? "TWF_" : (kind() == CDFT_POT_WHDF ? "WHDF_" : "LJ_");
+  ss << name << sigma_ << "_" << eps_ << "_";
+  if (kind() == CDFT_POT_TWF) ss << alpha() << "_";
+  ss << rcut_ << r_att_min_ << "_" << bh_;
+  return ss.str();
+}
+
+Interaction::Interaction(Species* s1, Species* s2, Potential1* v, double kT, std::string pointsFile)
+    : Interaction_Base(s1, s2, v, kT), pointsFile_(std::move(pointsFile)) {
+  const Lattice& L = s1->getLattice();
+  std::stringstream ss;
+  ss << "weights_" << s1->getSequenceNumber() << "_" << s2->getSequenceNumber() << "_" << L.Nx() << "_" << L.Ny()
+     << "_" << L.Nz() << "_" << v->getIdentifier() << "_" << ".dat";
+  weightsFile_ = ss.str();
+}
+
+bool Interaction::readWeights() {
+  const Lattice& L = s1_->getLattice();
+  std::ifstream in(weightsFile_.c_str(), std::ios::binary);
+  if (!in.good()) return false;
+  std::string buf;
+  std::getline(in, buf);
+  std::stringstream s1(buf);
+  long nx = 0, ny = 0, nz = 0;
+  double dx = 0;
+  s1 >> nx >> ny >> nz >> dx;
+  bool ok = nx == L.Nx() && ny == L.Ny() && nz == L.Nz() && std::fabs(L.getDX() - dx) <= 1e-8 * (L.getDX() + dx);
+  std::getline(in, buf);
+  std::stringstream s2(buf);
+  std::string identifier;
+  s2 >> identifier;
+  ok = ok && identifier == v_->getIdentifier();
+  std::getline(in, buf);
+  ok = ok && buf == pointsFile_;
+  if (!ok) return false;
+  long n = 0;
+  in >> n;
+  if (n != L.Ntot()) return false;
+  kernel_.assign(n, 0.0);
+  for (long i = 0; i < n; ++i) in >> kernel_[i];
+  if (!in) return false;
+  for (double& w : kernel_) w /= kT_;  // Interaction_Base::initialize: the temperature
+  return true;
+}
+
+void Interaction::generateWeights() {
+  const Lattice& L = s1_->getLattice();
+  std::ifstream in(pointsFile_.c_str());
+  if (!in.good()) throw std::runtime_error("File " + pointsFile_ + " could not be opened");
+  std::vector<double> pts;
+  for (std::string buf; std::getline(in, buf);) {
+    std::stringstream st(buf);
+    std::vector<double> line;
+    for (double d; st >> d;) line.push_back(d);
+    if (line.empty()) continue;
+    if (line.size() < 3) throw std::runtime_error("a point of " + pointsFile_ + " needs three coordinates");
+    line.push_back(4 * M_PI);
+    const double r = std::sqrt(line[0] * line[0] + line[1] * line[1] + line[2] * line[2]);
+    pts.insert(pts.end(), {line[0] / r, line[1] / r, line[2] / r, line[3]});
+  }
+  const long np = (long)pts.size() / 4;
+  for (long i = 0; i < np; ++i) pts[4 * i + 3] /= np;
+  std::vector<double> w(L.Ntot());
+  const cdft_potential_desc pd = v_->desc();
+  check(cdft_mf_kernel_points((int)L.Nx(), (int)L.Ny(), (int)L.Nz(), L.getDX(), L.getDY(), L.getDZ(), &pd, pts.data(),
+                              (int)np, 0, w.data()));
+  {  // the cache: header, identifier, points file, then the kernel (DFT_Vec::save, default stream precision)
+    std::ofstream of(weightsFile_.c_str(), std::ios::binary);
+    of.flags(std::ios::scientific);
+    of.precision(std::numeric_limits<double>::digits10 + 1);
+    of << L.Nx() << " " << L.Ny() << " " << L.Nz() << " " << L.getDX() << std::endl;
+    of << v_->getIdentifier() << std::endl;
+  }
+  std::ofstream of(weightsFile_.c_str(), std::ios::binary | std::ios::app);
+  of << pointsFile_ << std::endl;
+  of << w.size() << std::endl;
+  for (double x : w) of << x << " ";
+  of << std::endl;
+  kernel_.resize(w.size());
+  for (size_t i = 0; i < w.size(); ++i) kernel_[i] = w[i] / kT_;
+}
+
+void Interaction::ensure_kernel() const {
+  if (!kernel_.empty()) return;
+  auto* self = const_cast<Interaction*>(this);
+  self->from_cache_ = self->readWeights();
+  if (!self->from_cache_) self->generateWeights();
 }
 
 // src/Interaction.cpp:61-72

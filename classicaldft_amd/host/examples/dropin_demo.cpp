@@ -15,6 +15,9 @@
 //   fixed_bg <F> <sum dF> <F onlyFex>   Species::setFixedBackground on another 16^3 system
 //   fire_dir <F>            5 fireMinimizer2 steps with Minimizer::set_fixed_direction (ideal + WB I, mu = 0.4)
 //   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
+//   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
+//                           a_vdw of a first object (kernel generated, cache written) and of a second one (cache
+//                           read: 1), F of a 16^3 DFT with the cached kernel
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -181,5 +184,27 @@ int main() {
   open10.step();
   open10.step();
   std::printf("open_ddft %.17g %.17g %.17g\n", open10.getF(), d10.get(0), d10.get((8 * 16 + 8) * 16 + 8));
+
+  {
+    std::FILE* pf = std::fopen("sphere_points.dat", "w");
+    for (int i = 0; i < 60; ++i) {  // Fibonacci sphere
+      const double z = 1 - 2 * (i + 0.5) / 60, phi = M_PI * (1 + std::sqrt(5.0)) * (i + 0.5), r = std::sqrt(1 - z * z);
+      std::fprintf(pf, "%.17g %.17g %.17g\n", r * std::cos(phi), r * std::sin(phi), z);
+    }
+    std::fclose(pf);
+  }
+  Density d11(0.1, L2);
+  for (long i = 0; i < d11.Ntot(); ++i) d11.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(101, i) - 1.0)));
+  FMT_Species s11(d11, 1.0, 0.0, 7);
+  Interaction pts1(&s11, &s11, &lj, 1.0, "sphere_points.dat");
+  const double a1 = pts1.getVDWParameter();
+  Interaction pts2(&s11, &s11, &lj, 1.0, "sphere_points.dat");
+  const double a2 = pts2.getVDWParameter();
+  WhiteBearI fmt11;
+  DFT dft11(&s11);
+  dft11.addHardCoreContribution(&fmt11);
+  dft11.addInteraction(&pts2);
+  const double F11 = dft11.calculateFreeEnergyAndDerivatives(false);
+  std::printf("points %.17g %.17g %d %.17g\n", a1, a2, pts2.weights_from_cache() ? 1 : 0, F11);
   return 0;
 }

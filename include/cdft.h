@@ -177,6 +177,14 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu);
 /* Species::getChemPotential after the last fetched evaluation: the fixed-mass multiplier
    (src/Species.cpp:99-110) of each member, or the desc mu(s) without a fixed mass. */
 int cdft_system_chem_potential(cdft_system sys, double* mu);
+/* Points-file mean-field kernel: Interaction::generateWeights (src/Interaction.cpp:672-860).  points: npoints
+   rows of (ux, uy, uz, weight) with unit directions and the weights already divided by npoints, as the reference
+   reads its points file (:744-764); nradial Gauss-Legendre nodes on [0, rcut] (0: the reference's 512); the 6
+   axis permutations and 8 reflections of every direction.  w_out (host, N values) receives the kernel BEFORE the
+   division by kT (what the reference's weight cache stores); pass w_out / kT as desc mf_kernel.  Atomic
+   accumulation on the GPU: cells agree with a serial sum to rounding, not bit for bit. */
+int cdft_mf_kernel_points(int Nx, int Ny, int Nz, double dx, double dy, double dz, const cdft_potential_desc* pot,
+                          const double* points, int npoints, int nradial, double* w_out);
 /* One fixed mass per batch member (nb values > 0) for a system created with fixed_mass > 0 (the whole-DFT
    batch with per-system mu, V_ext and fixed mass). */
 int cdft_system_set_fixed_masses(cdft_system sys, const double* masses);

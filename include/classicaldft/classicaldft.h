@@ -122,6 +122,7 @@ class Species {
  public:
   Species(Density& density, double mu = 0, int seq = -1);
   virtual ~Species() = default;
+  int getSequenceNumber() const { return seq_num_; }  // include/Species.h:16-19
   const Lattice& getLattice() const { return density_; }
   const Density& getDensity() const { return density_; }
   Density& density() { return density_; }
@@ -158,6 +159,8 @@ class Species {
   double mu_;
   double fixedMass_ = -1;
   bool fixedBackground_ = false, homogeneousBoundary_ = false;
+  int seq_num_ = 0;
+  static int SequenceNumber_;
 };
 
 // include/Species.h:109-373: the weights live on the GPU (resident spectra).
@@ -273,6 +276,8 @@ class Potential1 {
   cdft_potential_desc desc() const;
   virtual int kind() const = 0;
   virtual double alpha() const { return 0.0; }
+  // include/Potential1.h:160-165, 227-232, 297-302 (rcut and the WCA limit printed without a separator)
+  std::string getIdentifier() const;
   bool isBH() const { return bh_; }
   double getWCALimit() const { return r_att_min_; }
 
@@ -339,11 +344,30 @@ class Interaction_Base {
   void describe(cdft_system_desc& d) const;
 
  protected:
+  virtual void ensure_kernel() const {}
   Species *s1_, *s2_;
   Potential1* v_;
   double kT_;
   int scheme_, ngauss_;
   mutable double a_vdw_ = NAN;
+  mutable std::vector<double> kernel_;  // explicit kernel (already / kT) of the points-file class
+};
+
+// include/Interaction.h:97-123, src/Interaction.cpp:566-860: the kernel from a file of spherical integration
+// points (generated on the GPU), cached in weights_<seq1>_<seq2>_<Nx>_<Ny>_<Nz>_<identifier>_.dat in the working
+// directory in the reference's format and read back when the grid, spacing, potential and points file match.
+class Interaction : public Interaction_Base {
+ public:
+  Interaction(Species* s1, Species* s2, Potential1* v, double kT, std::string pointsFile);
+  bool readWeights();       // src/Interaction.cpp:589-656
+  void generateWeights();   // src/Interaction.cpp:672-860, writes the cache
+  bool weights_from_cache() const { return from_cache_; }
+  const std::string& weights_file() const { return weightsFile_; }
+
+ protected:
+  void ensure_kernel() const override;
+  std::string pointsFile_, weightsFile_;
+  bool from_cache_ = false;
 };
 
 class Interaction_Interpolation_QF : public Interaction_Base {  // Interaction.h:416-431

@@ -54,6 +54,8 @@ def lib():
         L.orc_mf_kernel.argtypes = [_i, _i, _i, _d, _d, _d, _d, _d, _d, _d, _p]
         L.orc_mf_kernel.restype = _d
         L.orc_mf_kernel2.argtypes = [_i, _i, _i, _d, _d, _d, _i, _d, _d, _d, _d, _i, _d, _i, _i, _p, _p, _d, _p]
+        L.orc_mf_kernel_points.argtypes = [_i, _i, _i, _d, _d, _d, _i, _d, _d, _d, _d, _i, _d, _p, _i, _p, _p, _i, _p]
+        L.orc_mf_kernel_points.restype = None
         L.orc_mf_kernel2.restype = _d
         L.orc_dft_set_mf.argtypes = [_p, _p, _d]
         L.orc_fmt_set_ao.argtypes = [_p, _d, _d, _p, _i]
@@ -251,6 +253,22 @@ def mf_kernel2(shape, dx, kind=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, 
     a = lib().orc_mf_kernel2(Nx, Ny, Nz, dxs[0], dxs[1], dxs[2], int(kind), sigma, eps, rcut, alpha, int(bool(bh)),
                              r_att_min, int(scheme), int(ngauss), _ptr(gp), _ptr(gw), kT, _ptr(w))
     return w, a
+
+
+def mf_kernel_points(shape, dx, points, kind=POT_LJ, sigma=1.0, eps=1.0, rcut=2.5, alpha=0.0, bh=False,
+                     r_att_min=0.0, nradial=512):
+    """Points-file mean-field kernel (Interaction::generateWeights), before the division by kT; the radial
+    Gauss-Legendre nodes on [0, rcut] come from numpy."""
+    Nx, Ny, Nz = shape
+    dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+    x, w = gauss_legendre01(nradial)
+    rr, wr = np.ascontiguousarray(rcut * x), np.ascontiguousarray(rcut * w)
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.zeros(shape)
+    lib().orc_mf_kernel_points(Nx, Ny, Nz, dxs[0], dxs[1], dxs[2], int(kind), sigma, eps, rcut,
+                               alpha if alpha > 0 else 50.0, int(bool(bh)), r_att_min, _ptr(pts), len(pts), _ptr(rr),
+                               _ptr(wr), nradial, _ptr(out))
+    return out
 
 
 class DFT:

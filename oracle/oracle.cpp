@@ -1504,6 +1504,51 @@ double orc_mf_kernel2(int Nx, int Ny, int Nz, double dx, double dy, double dz, i
   return mf_kernel_general(g, dx, dy, dz, v, scheme, ng, gp, gw, kT, w_att);
 }
 
+// Points-file kernel, Interaction::generateWeights (src/Interaction.cpp:672-860): serial restatement, before the
+// division by kT.  pts: npts x (ux, uy, uz, weight); rr / wr: radial Gauss-Legendre nodes and weights on [0, Rc].
+void orc_mf_kernel_points(int Nx, int Ny, int Nz, double dx, double dy, double dz, int kind, double sigma, double eps,
+                          double rcut, double alpha, int bh, double r_att_min, const double* pts, int npts,
+                          const double* rr, const double* wr, int nr, double* w) {
+  Pot1 v;
+  v.init(kind, sigma, eps, rcut, alpha, bh != 0, r_att_min);
+  std::fill(w, w + (long)Nx * Ny * Nz, 0.0);
+  const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  for (int pos = 0; pos < npts; ++pos)
+    for (int ip = 0; ip < 6; ++ip)
+      for (int is = 0; is < 8; ++is)
+        for (int k = 0; k < nr; ++k) {
+          const double r = rr[k], watt = v.Watt2(r * r);
+          const double* u = pts + 4 * pos;
+          double x = r * u[perm[ip][0]], y = r * u[perm[ip][1]], z = r * u[perm[ip][2]];
+          if (is == 1) { x = -x; }
+          if (is == 2) { y = -y; }
+          if (is == 3) { z = -z; }
+          if (is == 4) { x = -x; y = -y; }
+          if (is == 5) { x = -x; z = -z; }
+          if (is == 6) { y = -y; z = -z; }
+          if (is == 7) { x = -x; y = -y; z = -z; }
+          const double ww = u[3] / 48.0;
+          int ix0 = int(x / dx), iy0 = int(y / dy), iz0 = int(z / dz);
+          if (x < 0) ix0 -= 1;
+          if (y < 0) iy0 -= 1;
+          if (z < 0) iz0 -= 1;
+          const double ddx = (x / dx) - ix0, ddy = (y / dy) - iy0, ddz = (z / dz) - iz0;
+          for (int io = 0; io < 2; ++io)
+            for (int jo = 0; jo < 2; ++jo)
+              for (int ko = 0; ko < 2; ++ko) {
+                int nx = ix0 + io, ny = iy0 + jo, nz = iz0 + ko;
+                while (nx < 0) nx += Nx;
+                while (ny < 0) ny += Ny;
+                while (nz < 0) nz += Nz;
+                while (nx >= Nx) nx -= Nx;
+                while (ny >= Ny) ny -= Ny;
+                while (nz >= Nz) nz -= Nz;
+                const double cc = (io == 0 ? 1 - ddx : ddx) * (jo == 0 ? 1 - ddy : ddy) * (ko == 0 ? 1 - ddz : ddz);
+                w[nz + (long)Nz * (ny + (long)Ny * nx)] += cc * ww * wr[k] * r * r * watt;
+              }
+        }
+}
+
 // ---- stateful DFT / FIRE2 / DDFT objects (opaque handles) ----
 struct orc_dft;
 

@@ -184,3 +184,23 @@ def test_open_ddft_through_cpp_api(demo):
     r = d.density()
     F, r0, r8 = demo["open_ddft"]
     assert rel(F, info["F"]) <= 1e-10 and r0 == 0.5 and rel(r8, r[8, 8, 8]) <= 1e-10
+
+
+def test_points_interaction_through_cpp_api(demo):
+    """Interaction(s1, s2, v, kT, pointsFile) (src/Interaction.cpp:566-860): the kernel generated on the GPU, the
+    reference-format cache written and read back by a second object, and a DFT with it (cache precision: the
+    reference's 6 significant digits)."""
+    from classicaldft_amd import formats
+    shape, dx = (16, 16, 16), 0.1
+    pts = formats.read_points_file(os.path.join(demo["_dir"], "sphere_points.dat"))
+    w = oracle.mf_kernel_points(shape, dx, pts)
+    a1, a2, cached, F = demo["points"]
+    assert rel(a1, w.sum()) <= 1e-12 and cached == 1 and rel(a2, a1) <= 1e-5
+    ident = formats.potential_identifier(0, 1.0, 1.0, 2.5)
+    path = os.path.join(demo["_dir"], formats.weight_cache_name(7, 7, shape, ident))
+    wc = formats.read_weight_cache(path, shape, dx, ident, "sphere_points.dat")
+    assert wc is not None and np.max(np.abs(wc - w)) <= 5e-6 * np.abs(w).max()
+    rho = seeded(shape, 0.5, 101, 0.1)
+    o = oracle.DFT(shape, dx, 1.0, accurate_weights=True)
+    o.set_mean_field(wc, wc.sum())
+    assert rel(F, o.eval(rho)[0][0]) <= 1e-11

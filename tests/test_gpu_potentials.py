@@ -61,3 +61,33 @@ def test_full_dft_with_twf_gauss_e(cdft):
     # bulk mu uses the kernel's a_vdw (src/Interaction.cpp:61-72)
     _, muex = oracle.bulk(oracle.WBI, 0.5, 1.0)
     assert abs(s.bulk(0.5)[1] - (np.log(0.5) + muex + a * 0.5)) <= 1e-12 * abs(s.bulk(0.5)[1])
+
+
+@pytest.mark.parametrize("kind,bh", [(0, False), (1, True), (2, False)])
+def test_points_kernel_matches_oracle(cdft, kind, bh):
+    """Interaction::generateWeights (src/Interaction.cpp:672-860) on the GPU against the serial restatement: the
+    cells agree to rounding (atomic accumulation order), the total exactly up to rounding."""
+    shape, dx = (24, 18, 16), 0.15
+    i = np.arange(40) + 0.5
+    z = 1 - 2 * i / 40
+    phi = np.pi * (1 + 5 ** 0.5) * i
+    r = np.sqrt(1 - z * z)
+    pts = np.stack([r * np.cos(phi), r * np.sin(phi), z, np.full(40, 4 * np.pi / 40)], axis=1)
+    rc = 2.0
+    w0 = oracle.mf_kernel_points(shape, dx, pts, kind=kind, rcut=rc, alpha=50.0, bh=bh, nradial=512)
+    w1 = cdft.mf_kernel_points(shape, dx, pts, potential=kind, rcut=rc, alpha=50.0, bh=bh)
+    m = np.abs(w0).max()
+    assert np.max(np.abs(w1 - w0)) <= 1e-13 * m
+    assert abs(w1.sum() - w0.sum()) <= 1e-12 * np.abs(w0).sum()
+    # a full DFT with the kernel (divided by kT) as the mean field
+    kT = 1.3
+    rho = 0.4 * (1 + 0.1 * (2 * oracle.splitmix64_uniform(int(np.prod(shape)), 5).reshape(shape) - 1))
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I, weights=oracle.fmt_weights(shape, dx, 1.0), ideal=True,
+                    mean_field=True, kT=kT, mf_kernel=w1 / kT)
+    o = oracle.DFT(shape, dx, 1.0, model=oracle.WBI, kT=kT)
+    o.set_mean_field(w1 / kT, w1.sum() / kT)
+    p0, d0 = o.eval(rho)
+    p1, d1 = s.eval(rho)
+    assert abs(p1[0] - p0[0]) <= 1e-11 * abs(p0[0])
+    assert np.max(np.abs(d1 - d0)) <= 1e-11 * np.abs(d0).max()
+    assert abs(s.a_vdw - w1.sum() / kT) <= 1e-12 * abs(w1.sum() / kT)
