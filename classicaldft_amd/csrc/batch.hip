@@ -617,7 +617,7 @@ int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
     static std::mutex mtx;
     const Key key{s->g.Nx, s->g.Ny, s->g.Nz, (int)s->mixed, s->d.model, s->nconv, s->nb};
     auto apply = [&](const Cfg& c) {
-      s->reg_fft = c.reg;
+      s->reg_fft = c.reg || s->wq;  // folded weight spectra are only read by the register x pass
       s->nb_half_cols = c.cols;
       s->nb_half_fwd = c.fwd;
       s->nb_half_bwd = c.bwd;
@@ -635,7 +635,8 @@ int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
     }
     std::vector<Cfg> cands;
     const bool reg_possible = reg_fft_ok<double>(*s, s->g.Nx) || rows_reg_ok<double>(*s);
-    for (int r = reg_possible ? 1 : 0; r >= (s->nb > 1 ? 1 : 0); --r)
+    // batches and folded weight spectra (ColGeo::wq) run the register x pass only
+    for (int r = reg_possible ? 1 : 0; r >= ((s->nb > 1 || s->wq) ? 1 : 0); --r)
       for (int cols = 0; cols < 2; ++cols)
         for (int fwd = 0; fwd < 2; ++fwd)
           for (int bwd = 0; bwd < 2; ++bwd)

@@ -107,6 +107,57 @@ void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights
   }
 }
 
+// Folds W (ColGeo::wq) when every array has the parities of the lattice weights (checked in x and in y), to
+// rounding: half the weight footprint of the two x-pass convolutions.  Only with the register x pass (the LDS kernels read the full
+// layout) and without the AO polymer.
+template <typename T>
+void fold_weight_spectra(cdft_system_s& s) {
+  if (s.nconv == 0 || s.has_ao || !reg_fft_ok<T>(s, s.g.Nx)) return;
+  if (const char* e = std::getenv("CDFT_NO_WFOLD"); e && e[0] == '1') return;
+  const Geo& g = s.g;
+  const int nblk = 64;
+  DevBuf out;
+  out.alloc(sizeof(double) * 2 * nblk * s.nconv);
+  hipLaunchKernelGGL((k_wsym<T>), dim3(nblk, s.nconv), dim3(256), 0, s.stream, s.W.as<T>(), s.Nkp, g.Nx, g.Ny, g.Zh,
+                     g.Zp, kOddX, kOddY, out.as<double>());
+  HIPCHK(hipGetLastError());
+  std::vector<double> h(2 * nblk * s.nconv);
+  HIPCHK(hipMemcpyAsync(h.data(), out.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  const double tol = std::is_same<T, float>::value ? 1e-5 : 1e-11;
+  for (int a = 0; a < s.nconv; ++a) {
+    double dev = 0.0, mx = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+      dev = std::max(dev, h[2 * (a * nblk + b)]);
+      mx = std::max(mx, h[2 * (a * nblk + b) + 1]);
+    }
+    if (dev > tol * mx) return;  // e.g. caller weights without the lattice symmetries: keep the full layout
+  }
+  const long nq = (long)(g.Nx / 2 + 1) * g.Ny * g.Zp;
+  DevBuf Wq;
+  Wq.alloc(sizeof(T) * nq * s.nconv);
+  hipLaunchKernelGGL((k_wfold<T>), dim3((unsigned)std::min<long>(div_up(nq * s.nconv, 256), 65536)), dim3(256), 0,
+                     s.stream, s.W.as<T>(), Wq.as<T>(), s.Nkp, nq, s.nconv);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s.stream));
+  std::swap(s.W.p, Wq.p);  // the full layout is released with Wq
+  std::swap(s.W.bytes, Wq.bytes);
+  s.wq = true;
+  s.wq_stride = unsigned(nq);
+}
+
+// col_geo of the x pass of a convolution with the weight spectra W (folded when W is the system's folded W).
+template <typename T>
+ColGeo conv_geo(const cdft_system_s& s, const void* W) {
+  ColGeo cg = col_geo<T>(s.g, 0);
+  if (s.wq && W == s.W.p) {
+    cg.wq = 1;
+    cg.wstride = s.wq_stride;
+    cg.pxm = kOddX;
+  }
+  return cg;
+}
+
 template <typename T>
 void build_weight_spectra(cdft_system_s& s) {
   const long N = s.N, Nkp = s.Nkp;
@@ -142,6 +193,7 @@ void build_weight_spectra(cdft_system_s& s) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s.stream));
   }
+  fold_weight_spectra<T>(s);
 }
 
 void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const double* mus, const double* vexts) {
@@ -359,7 +411,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
     launch_cols<T>(s, A, 0, 1, 1, true, nb, sA);
     s.mark("y_fwd");
     if (reg_fft_ok<T>(s, s.g.Nx)) {
-      const ColGeo cg = col_geo<T>(s.g, 0);
+      const ColGeo cg = conv_geo<T>(s, s.W.p);
       with_reg_len(s.g.Nx, [&](auto lc) {
         constexpr int Lc = decltype(lc)::value;
         with_nb<T, Lc>(s.nb_half_fwd, [&](auto nbc) {
@@ -372,6 +424,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
       });
     } else {
       REQUIRE(nb == 1, "batched systems need transform lengths the register FFT supports");
+      REQUIRE(!s.wq, "folded weight spectra are read by the register x pass only");
       const int TW = s.TWconv;
       dim3 grid(div_up(s.g.Zh, TW), s.g.Ny);
       hipLaunchKernelGGL((k_conv_fwd<T, 16>), grid, dim3(kThreads), 2 * sizeof(cplx<T>) * s.g.Nx * TW, s.stream, A, B,
@@ -416,7 +469,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
     }
     if (s.m > 0) {
       if (reg_fft_ok<T>(s, s.g.Nx)) {
-        const ColGeo cg = col_geo<T>(s.g, 0);
+        const ColGeo cg = conv_geo<T>(s, s.W.p);
         with_reg_len(s.g.Nx, [&](auto lc) {
           constexpr int Lc = decltype(lc)::value;
           with_nb<T, Lc>(s.nb_half_bwd, [&](auto nbc) {
@@ -429,6 +482,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
           });
         });
       } else {
+        REQUIRE(!s.wq, "folded weight spectra are read by the register x pass only");
         const int TW = s.TWconv;
         dim3 grid(div_up(s.g.Zh, TW), s.g.Ny);
         hipLaunchKernelGGL((k_conv_bwd<T, 16>), grid, dim3(kThreads), 2 * sizeof(cplx<T>) * s.g.Nx * TW, s.stream, B,
@@ -504,7 +558,7 @@ void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<
   const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
   launch_rows_r2c<double, double>(s, in, s.N, A, sA, s.nb);
   launch_cols<double>(s, A, 0, 1, 1, true, s.nb, sA);
-  const ColGeo cg = col_geo<double>(s.g, 0);
+  const ColGeo cg = conv_geo<double>(s, W);
   with_reg_len(s.g.Nx, [&](auto lc) {
     constexpr int Lc = decltype(lc)::value;
     with_nb<double, Lc>(s.nb_half_fwd, [&](auto nbc) {
@@ -548,7 +602,7 @@ void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 
 void conv_bwd_f64(cdft_system_s& s, const cplx<double>* D, cplx<double>* Acc, const double* W, int m) {
   const Tables<double>& tw = tables<double>(s);
   const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
-  const ColGeo cg = col_geo<double>(s.g, 0);
+  const ColGeo cg = conv_geo<double>(s, W);
   with_reg_len(s.g.Nx, [&](auto lc) {
     constexpr int Lc = decltype(lc)::value;
     with_nb<double, Lc>(s.nb_half_bwd, [&](auto nbc) {
@@ -712,7 +766,7 @@ void hess_device(cdft_system_s& s, const double* rho, const double* v, double* o
       default: launch_hess<M_WBII>(s, mf); break;
     }
     launch_cols<double>(s, B, Nkp, s.m, 1, true, s.nb, sB);
-    const ColGeo cg = col_geo<double>(s.g, 0);
+    const ColGeo cg = conv_geo<double>(s, s.W.p);
     with_reg_len(s.g.Nx, [&](auto lc) {
       constexpr int Lc = decltype(lc)::value;
       with_nb<double, Lc>(s.nb_half_bwd, [&](auto nbc) {

@@ -78,6 +78,11 @@ struct ColGeo {
   unsigned cz;     // columns per contiguous group
   unsigned os;     // element stride between groups
   unsigned es;     // element stride along the transform
+  // Folded weight spectra (x passes of the convolutions; wq = 0: the full Nx x Ny x Zp layout).  Every weight is
+  // even or odd in x (pxm: bit a set when array a is odd), so W(-kx, ky, kz) = +-W(kx, ky, kz): only the rows
+  // kx <= Nx/2 are stored, (Nx/2+1) x Ny x Zp values per array, wstride apart.  A column reads its rows kx and
+  // Nx - kx from the same stored row, so the fold also halves the distinct lines each block fetches.
+  unsigned wq, wstride, pxm;
 };
 
 // In-place transform of ncols columns per array.  grid = (ceil(ncols/NB), narr, members), block = NB * P.
@@ -140,6 +145,19 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
       v1[q][n1] = buf_ld<cplx<T>>(ra, vo, R2 * n1 * cg.es * E);
   }
   auto load_w = [&](int a, T (&w)[Q2][R2]) {
+    if (cg.wq) {
+      const auto rw = buf_rsrc(W + (long)a * cg.wstride, cg.wstride * EW);
+#pragma unroll
+      for (int q = 0; q < Q2; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+          const unsigned kx = t + P * q + R1 * k2;
+          const bool fx = kx > (unsigned)L / 2;
+          const T v = buf_ld<T>(rw, (c + (fx ? L - kx : kx) * cg.es) * EW, 0);
+          w[q][k2] = (fx && ((cg.pxm >> a) & 1u)) ? -v : v;
+        }
+      return;
+    }
     const auto rw = buf_rsrc(W + a * arr_stride, cg.bytes / 2);
 #pragma unroll
     for (int q = 0; q < Q2; ++q) {
@@ -205,6 +223,19 @@ __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D
     }
   };
   auto load_w = [&](int a, T (&w)[Q2][R2]) {
+    if (cg.wq) {
+      const auto rw = buf_rsrc(W + (long)a * cg.wstride, cg.wstride * EW);
+#pragma unroll
+      for (int q = 0; q < Q2; ++q)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+          const unsigned kx = t + P * q + R1 * k2;
+          const bool fx = kx > (unsigned)L / 2;
+          const T v = buf_ld<T>(rw, (c + (fx ? L - kx : kx) * cg.es) * EW, 0);
+          w[q][k2] = (fx && ((cg.pxm >> a) & 1u)) ? -v : v;
+        }
+      return;
+    }
     const auto rw = buf_rsrc(W + a * arr_stride, cg.bytes / 2);
 #pragma unroll
     for (int q = 0; q < Q2; ++q) {

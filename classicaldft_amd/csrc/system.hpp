@@ -153,6 +153,8 @@ struct cdft_system_s {
   DevBuf A, B, W, vmf, rho, dF, vext, partials, partials2, flag, results;
   DevBuf A2, B2, vdir;  // Hessian-vector product: spectra of the direction, its host copy (allocated on first use)
   DevBuf Wao, psi;      // Asakura-Oosawa: polymer weight spectra (m arrays), psi / exp(-psi) (N per member)
+  bool wq = false;      // W holds the folded (kx, ky <= N/2) spectra (ColGeo::wq)
+  unsigned wq_stride = 0;
   bool has_ao = false;
   double a_vdw = 0.0;
   int RBrow = 1, RBpw = 1, TWcol = 16, TWconv = 8;

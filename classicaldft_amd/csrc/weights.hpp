@@ -242,6 +242,57 @@ static __global__ void k_mf_kernel(int Nx, int Ny, int Nz, double dx, double dy,
   out[p] = acc / kT;
 }
 
+// Parity of the weight arrays in x (bit a set = odd): 2 v_x, 6 T_xy, 7 T_xz (include/Species.h:351-360 order); eta,
+// s, v_y, v_z, the other tensor entries and the mean-field kernel (stored after the m FMT arrays, index 5 or 11)
+// are even in x.  kOddY is the same for y (used by the check only).
+constexpr unsigned kOddX = (1u << 2) | (1u << 6) | (1u << 7);
+constexpr unsigned kOddY = (1u << 3) | (1u << 6) | (1u << 9);
+
+// Symmetry check of real spectra W (narr arrays of Nx x Ny x Zp): per block and array, max |W(k) - s_x W(-kx)|,
+// max |W(k) - s_y W(-ky)| and max |W(k)| over kz < Zh.  grid = (nblk, narr).
+template <typename T>
+__global__ __launch_bounds__(256) void k_wsym(const T* __restrict__ W, long nkp, int Nx, int Ny, int Zh, int Zp,
+                                              unsigned pxm, unsigned pym, double* __restrict__ out) {
+  const int a = blockIdx.y;
+  W += (long)a * nkp;
+  const long n = (long)Nx * Ny * Zh;
+  double dev = 0.0, mx = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int kz = (int)(i % Zh), ky = (int)((i / Zh) % Ny), kx = (int)(i / ((long)Zh * Ny));
+    const double v = (double)W[((long)kx * Ny + ky) * Zp + kz];
+    const double vx = (double)W[((long)((Nx - kx) % Nx) * Ny + ky) * Zp + kz];
+    const double vy = (double)W[((long)kx * Ny + (Ny - ky) % Ny) * Zp + kz];
+    dev = fmax(dev, fabs(v - (((pxm >> a) & 1u) ? -vx : vx)));
+    dev = fmax(dev, fabs(v - (((pym >> a) & 1u) ? -vy : vy)));
+    mx = fmax(mx, fabs(v));
+  }
+  __shared__ double sd[256], sm[256];
+  sd[threadIdx.x] = dev;
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + off]);
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * ((long)a * gridDim.x + blockIdx.x)] = sd[0];
+    out[2 * ((long)a * gridDim.x + blockIdx.x) + 1] = sm[0];
+  }
+}
+
+// Wq[a][(kx Ny + ky) Zp + kz] = W[a][(kx Ny + ky) Zp + kz] for kx <= Nx/2 (the first rows of each array).
+template <typename T>
+__global__ __launch_bounds__(256) void k_wfold(const T* __restrict__ W, T* __restrict__ Wq, long nkp, long nq, int narr) {
+  const long n = nq * narr;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long a = i / nq, r = i - a * nq;
+    Wq[i] = W[a * nkp + r];
+  }
+}
+
 // Points-file mean-field kernel (Interaction::generateWeights, src/Interaction.cpp:672-860): every direction u of
 // the points list (unit vector, weight), under the 6 axis permutations and 8 sign reflections, and every radial
 // Gauss-Legendre node r on [0, Rc] puts w = weight / 48 * wr r^2 Watt(r) on the 8 corners of the lattice cell
