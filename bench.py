@@ -315,6 +315,7 @@ def main():
     ap.add_argument("--fire-max-steps", type=int, default=20000)
     ap.add_argument("--ddft-steps", type=int, default=0, help="DDFT fcc-seed steps at 256^3 (0: skip)")
     ap.add_argument("--batch", action="store_true", help="batched FIRE on 96 systems of 48^3 / 64^3 / 96^3")
+    ap.add_argument("--no-ao", action="store_true", help="skip the Asakura-Oosawa polymer timing")
     ap.add_argument("--warm-systems", type=int, default=64,
                     help="held-out 64^3 systems for the learned warm start vs uniform start (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -345,9 +346,11 @@ def main():
     print("# mixed timing", flush=True)
     tmx, Fmx, dFmx, stmx = time_system(cdft, cdft.MIXED, rho_d, dF_d, args.steps, args.warmup)
 
-    print("# AO polymer timing", flush=True)
-    ao = time_ao(cdft, rho_d, dF_d, max(3, args.steps // 5))
-    ao["vs_fp64_plain"] = ao["ms_per_step"] / t64
+    ao = None
+    if not args.no_ao:
+        print("# AO polymer timing", flush=True)
+        ao = time_ao(cdft, rho_d, dF_d, max(3, args.steps // 5))
+        ao["vs_fp64_plain"] = ao["ms_per_step"] / t64
 
     m = 11
     B64 = model_bytes(8, m)
