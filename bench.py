@@ -136,7 +136,7 @@ def fire_time_to_tol(cdft, n=128, tol=1e-8, max_steps=20000, precision=0):
     return info
 
 
-def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha=50.0, dt=1e-4):
+def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha=50.0, dt=1e-4, precision=0):
     """DDFT from a liquid with a 4x4x4 fcc seed of Gaussians (alpha/pi)^{3/2} exp(-alpha |r - R|^2) centred in the
     box, dt pinned at 1e-4; ms per step and the relative mass drift."""
     shape = (n, n, n)
@@ -157,17 +157,27 @@ def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha
     sub = np.einsum("si,sj,sk->ijk", gx, gy, gz) * (alpha / np.pi) ** 1.5
     idx = np.where(inside)[0]
     rho[np.ix_(idx, idx, idx)] = np.maximum(sub, 1e-12)
-    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=kT)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=kT, precision=precision)
     d = cdft.DDFT(s, rho, dt=dt, dt_max=dt)
     d.set_max_time_step(dt)
     m0 = rho.sum() * dx ** 3
     t0 = time.perf_counter()
     r = None
-    for _ in range(steps):
-        r = d.step()
+    done, stalled = 0, None
+    for i in range(steps):
+        try:
+            r = d.step()
+        except cdft.cdft.DdftStalled as e:  # the reference's restart loop would not end here
+            stalled = str(e)
+            break
+        done += 1
+        if (i + 1) % 100 == 0:
+            print(f"# ddft step {i + 1}: dt {r['dt']:.3g}, {time.perf_counter() - t0:.1f} s", flush=True)
     sec = time.perf_counter() - t0
-    out = {"grid": list(shape), "dx": dx, "kT": kT, "steps": steps, "seconds": sec, "ms_per_step": 1e3 * sec / steps,
-           "F": r["F"], "evaluations": r["calls"], "mass_rel_drift": abs(r["mass"] - m0) / m0, "dt": r["dt"]}
+    out = {"grid": list(shape), "dx": dx, "kT": kT, "steps": done, "steps_requested": steps, "seconds": sec,
+           "ms_per_step": 1e3 * sec / max(1, done), "F": r["F"] if r else None, "evaluations": r["calls"] if r else 0,
+           "mass_rel_drift": abs(r["mass"] - m0) / m0 if r else None, "dt": r["dt"] if r else None,
+           "stalled": stalled}
     d.close()
     s.close()
     return out
@@ -409,7 +419,8 @@ def main():
         result["batched_fire"] = batch_fire(cdft)
     if args.ddft_steps:
         print("# DDFT fcc seed", flush=True)
-        result["ddft_fcc_seed"] = ddft_fcc_seed(cdft, args.ddft_steps)
+        result["ddft_fcc_seed"] = {"fp64": ddft_fcc_seed(cdft, args.ddft_steps, precision=cdft.FP64),
+                                   "mixed": ddft_fcc_seed(cdft, args.ddft_steps, precision=cdft.MIXED)}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"# cpu baseline ({threads} threads)", flush=True)

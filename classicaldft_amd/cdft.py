@@ -169,12 +169,18 @@ def lib():
     return _LIB
 
 
+class DdftStalled(CdftError):
+    """CDFT_DDFT_STALLED: the DDFT fixed point fails at the smallest time step the reference can set."""
+
+
 def _check(rc):
     if rc == CDFT_OK:
         return
     msg = lib().cdft_last_error().decode(errors="replace")
     if rc == CDFT_ETA_TOO_LARGE:
         raise EtaTooLarge(rc, msg)
+    if rc == 3:
+        raise DdftStalled(rc, msg)
     raise CdftError(rc, msg)
 
 

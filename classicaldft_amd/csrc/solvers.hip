@@ -527,7 +527,12 @@ struct cdft_ddft_s {
         if (mn < 0 || (i > 0 && old < dev)) restart = true;
       }
       if (restart || dev > tol) {
+        const double before = dt;
         change_timestep(dt / 10);
+        if (dt == before) {  // DDFT.cpp:184-186 ignores the change: the same attempt would repeat forever
+          g_err = "DDFT: the fixed-point iteration fails at the smallest time step change_timestep can set";
+          return CDFT_DDFT_STALLED;
+        }
         copy(d0, d1);
         decreased = true;
         restart = true;
@@ -834,7 +839,7 @@ int cdft_ddft_create(cdft_system sys, const double* rho0, double dt, double dt_m
     d->dz = sys->d.dz;
     d->dt = dt > 0 ? dt : 0.0001 * d->dx * d->dx;  // DDFT.cpp:46
     d->dtMax = dt_max > 0 ? dt_max : 1.0;
-    d->tol = tol > 0 ? tol : 1e-4;
+    d->tol = tol != 0 ? tol : 1e-4;  // any other value as DDFT::set_tolerence_fixed_point takes it
     const size_t b = sizeof(double) * d->N;
     const size_t cb = sizeof(cplx<double>) * sys->Nkp;
     for (DevBuf* q : {&d->d0, &d->d1, &d->dF, &d->tmp}) q->alloc(b);
@@ -933,7 +938,7 @@ int cdft_ddft_open_create(cdft_system sys, const double* rho0, double background
     d->bg = background;
     d->dt = dt > 0 ? dt : 0.0001 * d->dx * d->dx;
     d->dtMax = dt_max > 0 ? dt_max : 1.0;
-    d->tol = tol > 0 ? tol : 1e-4;
+    d->tol = tol != 0 ? tol : 1e-4;  // any other value as DDFT::set_tolerence_fixed_point takes it
     const int Ns[3] = {g.Nx, g.Ny, g.Nz};
     const double Ds[3] = {1 / (d->dx * d->dx), 1 / (d->dy * d->dy), 1 / (d->dz * d->dz)};
     for (int ax = 0; ax < 3; ++ax) {

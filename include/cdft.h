@@ -26,6 +26,9 @@ enum cdft_status {
   CDFT_OK = 0,
   CDFT_ETA_TOO_LARGE = 1, /* eta > 1 somewhere: Eta_Too_Large_Exception, include/FMT.h:649-663 */
   CDFT_UPHILL_LIMIT = 2,  /* FIRE2: too many uphill steps, src/Minimizer.cpp:207 */
+  CDFT_DDFT_STALLED = 3,  /* DDFT: the fixed point failed and dt / 10 is a change below the 1e-10 that
+                             DDFT::change_timestep ignores (src/DDFT.cpp:184-186), so the reference's restart
+                             loop would repeat forever; the step is abandoned with the state unchanged */
   CDFT_BAD_ARG = -1,
   CDFT_HIP_ERROR = -2,
   CDFT_NOT_SUPPORTED = -3
@@ -249,7 +252,9 @@ typedef struct cdft_ddft_report {
   int calls, restarts;
 } cdft_ddft_report;
 /* dt <= 0 selects the reference default 1e-4 dx^2 (src/DDFT.cpp:46); dt_max
-   <= 0 selects 1; tol <= 0 selects 1e-4 (include/Minimizer.h:239). */
+   <= 0 selects 1; tol = 0 selects 1e-4 (include/Minimizer.h:239), any other value is used as given
+   (DDFT::set_tolerence_fixed_point).  cdft_ddft_step returns CDFT_DDFT_STALLED where the reference's restart
+   loop would not end. */
 int cdft_ddft_create(cdft_system sys, const double* rho0, double dt, double dt_max, double tol, cdft_ddft* out);
 int cdft_ddft_step(cdft_ddft d, cdft_ddft_report* rep);
 int cdft_ddft_set_max_time_step(cdft_ddft d, double dt_max);
@@ -261,7 +266,7 @@ int cdft_ddft_destroy(cdft_ddft d);
    iz = 0; the interior evolves with the integrating factor of the Dirichlet Laplacian through 3D DST-I
    transforms.  step = DDFT_IF_Open::step (step_string: at most 20 fixed-point iterations, dt halved on a
    restart and restored for the next step).  Nx, Ny, Nz: 2N a product of 2 and 3, <= 1024.  Systems
-   without species constraints; dt = 0 gives 1e-4 dx^2, dt_max and tol <= 0 the defaults 1 and 1e-4. */
+   without species constraints; dt = 0 gives 1e-4 dx^2, dt_max <= 0 and tol = 0 the defaults 1 and 1e-4. */
 typedef struct cdft_ddft_open_s* cdft_ddft_open;
 typedef struct cdft_ddft_open_report {
   double F, dt, time;

@@ -379,6 +379,8 @@ class DDFT:
         rc = lib().orc_ddft_step(self.h, _ptr(info))
         if rc == 1:
             raise EtaTooLarge("eta too large")
+        if rc == 2:
+            raise RuntimeError("DDFT stalled: the fixed point fails at the smallest settable time step")
         return dict(F=info[0], dt=info[1], time=info[2], rhs_max=info[3], calls=int(info[4]), restarts=int(info[5]))
 
     def density(self):

@@ -1895,7 +1895,9 @@ struct Ddft {
         if (mn < 0 || (i > 0 && old < dev)) restart = true;
       }
       if (restart || dev > tol) {
+        const double before = dt;
         change_timestep(dt / 10);
+        if (dt == before) { rho = d0; return 2; }  // the reference's restart loop would not end
         d1 = d0;
         decreased = true;
         restart = true;

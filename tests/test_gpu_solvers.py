@@ -150,3 +150,22 @@ def test_fire_fixed_direction_and_reset_replay(cdft):
     _, _, _, d1 = fg.state()
     _, _, _, d0 = fo.state()
     assert np.abs(d0 - d1).max() <= 1e-9 * np.abs(d0).max()
+
+
+def test_ddft_stalls_instead_of_looping(cdft):
+    """A fixed point that never converges (a negative tolerance) drives dt down by 10 per restart until dt / 10 is
+    a change DDFT::change_timestep ignores (< 1e-10, src/DDFT.cpp:184-186); the reference's restart loop would then
+    repeat forever.  Both sides stop there with the density of the step's start."""
+    shape, dx = (12, 12, 12), 0.1
+    rho0 = density(shape, 0.5, 23, 0.1)
+    w = oracle.fmt_weights(shape, dx, 1.0)
+    o = oracle.DFT(shape, dx, 1.0, mf=True, kT=1.0)
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mean_field=True, kT=1.0)
+    do = oracle.DDFT(o, rho0, dt=1e-4, dt_max=1e-4, tol=-1.0)
+    dg = cdft.DDFT(s, rho0, dt=1e-4, dt_max=1e-4, tol=-1.0)
+    with pytest.raises(RuntimeError):
+        do.step()
+    with pytest.raises(cdft.cdft.DdftStalled):
+        dg.step()
+    assert np.abs(dg.density() - rho0).max() <= 1e-15 * rho0.max()
+    assert np.abs(do.density() - rho0).max() <= 1e-15 * rho0.max()
