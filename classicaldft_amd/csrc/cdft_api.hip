@@ -1261,8 +1261,8 @@ int cdft_mf_kernel_points(int Nx, int Ny, int Nz, double dx, double dy, double d
       w[k] *= v.rcut;
     }
     const long N = (long)Nx * Ny * Nz;
-    hipStream_t st;
-    HIPCHK(hipStreamCreate(&st));
+    OwnedStream owned;
+    hipStream_t st = owned.s;
     DevBuf dp, dr, dw, out;
     dp.alloc(sizeof(double) * 4 * (size_t)npoints);
     dr.alloc(sizeof(double) * nr);
@@ -1279,7 +1279,6 @@ int cdft_mf_kernel_points(int Nx, int Ny, int Nz, double dx, double dy, double d
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(w_out, out.p, sizeof(double) * N, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipStreamDestroy(st));
     return CDFT_OK;
   });
 }

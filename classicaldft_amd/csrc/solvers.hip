@@ -1001,8 +1001,8 @@ int cdft_dst1_3d(int n0, int n1, int n2, double* a) {
     const int ns[3] = {n0, n1, n2};
     for (int ax = 0; ax < 3; ++ax)
       REQUIRE(smooth23(2 * (ns[ax] + 1)) && 2 * (ns[ax] + 1) <= 1024, "DST-I lengths n need 2(n+1) a product of 2 and 3, <= 1024");
-    hipStream_t st;
-    HIPCHK(hipStreamCreate(&st));
+    OwnedStream owned;
+    hipStream_t st = owned.s;
     const long ni = (long)n0 * n1 * n2;
     DevBuf buf, tw[3];
     buf.alloc(sizeof(double) * ni);
@@ -1019,7 +1019,6 @@ int cdft_dst1_3d(int n0, int n1, int n2, double* a) {
     }
     HIPCHK(hipMemcpyAsync(a, buf.p, sizeof(double) * ni, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    HIPCHK(hipStreamDestroy(st));
     return CDFT_OK;
   });
 }

@@ -46,6 +46,17 @@ struct CdftError : std::runtime_error {
     if (!(cond)) throw CdftError(CDFT_BAD_ARG, msg); \
   } while (0)
 
+// A stream created for one standalone call, destroyed on every exit path.
+struct OwnedStream {
+  hipStream_t s = nullptr;
+  OwnedStream() { HIPCHK(hipStreamCreate(&s)); }
+  OwnedStream(const OwnedStream&) = delete;
+  OwnedStream& operator=(const OwnedStream&) = delete;
+  ~OwnedStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
