@@ -40,7 +40,7 @@ class _Desc(ctypes.Structure):
         ("mf_potential", _i), ("mf_alpha", _d), ("mf_bh", _i), ("mf_r_att_min", _d), ("mf_scheme", _i),
         ("mf_ngauss", _i),
         ("ao_Rp", _d), ("ao_reservoir", _d), ("ao_weights", _p),
-        ("fixed_mass", _d), ("boundary", _i),
+        ("fixed_mass", _d), ("boundary", _i), ("mf_scale", _d),
     ]
 
 
@@ -201,7 +201,7 @@ class System:
                  vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0,
                  mus=None, vexts=None, potential=POT_LJ, alpha=0.0, bh=False, r_att_min=0.0, mf_scheme=MF_QF,
                  ngauss=0, ao_Rp=0.0, ao_reservoir=0.0, ao_weights=None, fixed_mass=0.0, fixed_background=False,
-                 homogeneous_boundary=False):
+                 homogeneous_boundary=False, mf_scale=1.0):
         """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create).
         potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule;
         ao_Rp > 0 adds an Asakura-Oosawa polymer of that radius at reservoir density ao_reservoir.
@@ -230,7 +230,7 @@ class System:
                   arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
                   arr(mf_kernel, self.N), int(potential), float(alpha), int(bool(bh)), float(r_att_min),
                   int(mf_scheme), int(ngauss), float(ao_Rp), float(ao_reservoir), arr(ao_weights, 11 * self.N),
-                  float(fm[0]), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0))
+                  float(fm[0]), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0), float(mf_scale))
         h = _p()
         if mus is None:
             self.nb = 1
@@ -500,7 +500,7 @@ class FireBatch:
         vx = None if vexts is None else np.ascontiguousarray(vexts, dtype=np.float64).reshape(-1)
         d = _Desc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A), float(B),
                   float(hsd), int(precision), None, int(bool(ideal)), 0.0, None, int(bool(mean_field)), float(sigma),
-                  float(eps), float(rcut), float(kT), None, 0, 0.0, 0, 0.0, 0, 0, 0.0, 0.0, None, 0.0, 0)
+                  float(eps), float(rcut), float(kT), None, 0, 0.0, 0, 0.0, 0, 0, 0.0, 0.0, None, 0.0, 0, 0.0)
         p = FireParams()
         _check(lib().cdft_fire_default_params(ctypes.byref(p)))
         if dt is not None:

@@ -158,6 +158,8 @@ ColGeo conv_geo(const cdft_system_s& s, const void* W) {
   return cg;
 }
 
+double mf_scale(const cdft_system_desc& d) { return d.mf_scale != 0 ? d.mf_scale : 1.0; }
+
 template <typename T>
 void build_weight_spectra(cdft_system_s& s) {
   const long N = s.N, Nkp = s.Nkp;
@@ -175,7 +177,7 @@ void build_weight_spectra(cdft_system_s& s) {
       launch_mf_kernel(s, w.as<double>());
       HIPCHK(hipGetLastError());
     }
-    // a_vdw = sum_R w_att(R) (src/Interaction.cpp:290-296, 42-46)
+    // a_vdw = sum_R w_att(R) (src/Interaction.cpp:290-296, 42-46), times scale_interaction's factor
     const int nblk = 256;
     hipLaunchKernelGGL(k_sum_real, dim3(nblk), dim3(kThreads), 0, s.stream, w.as<double>(), N,
                        s.partials.as<double2>());
@@ -183,13 +185,15 @@ void build_weight_spectra(cdft_system_s& s) {
                        s.results.as<double>(), 0L, 0L);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(&s.a_vdw, s.results.p, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    s.a_vdw *= mf_scale(s.d);
     launch_rows_r2c<double, double>(s, w.as<double>(), 0, S.as<cplx<double>>(), 0, 1);
     launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 1, true);
     launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 0, true);
     // v = (dV/N) c2r(rho_k w_k): fold dV/N into the stored spectrum
     const double dV = s.d.dx * s.d.dy * s.d.dz;
     hipLaunchKernelGGL((k_extract_weight<T>), dim3(nb), dim3(kThreads), 0, s.stream, S.as<cplx<double>>(),
-                       s.W.as<T>() + s.m * Nkp, Nkp, 0, dV / (double)N);
+                       s.W.as<T>() + s.m * Nkp, Nkp, 0, mf_scale(s.d) * dV / (double)N);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s.stream));
   }
@@ -1352,6 +1356,9 @@ int cdft_system_get_mf_kernel(cdft_system sys, double* w) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(w, buf.p, sizeof(double) * sys->N, hipMemcpyDeviceToHost, sys->stream));
     HIPCHK(hipStreamSynchronize(sys->stream));
+    const double f = mf_scale(sys->d);
+    if (f != 1.0)
+      for (long i = 0; i < sys->N; ++i) w[i] *= f;
     return CDFT_OK;
   });
 }

@@ -310,6 +310,34 @@ void Interaction_Base::describe(cdft_system_desc& d) const {
   d.mf_r_att_min = v_->isBH() ? 0.0 : v_->getWCALimit();
   d.mf_scheme = scheme_;
   d.mf_ngauss = ngauss_;
+  d.mf_scale = scale_;
+}
+
+double Interaction_Base::getW(long s) const {
+  if (w_.empty()) {
+    const Lattice& L = s1_->getLattice();
+    cdft_system_desc d{};
+    d.Nx = (int)L.Nx();
+    d.Ny = (int)L.Ny();
+    d.Nz = (int)L.Nz();
+    d.dx = L.getDX();
+    d.dy = L.getDY();
+    d.dz = L.getDZ();
+    d.model = CDFT_MODEL_NONE;
+    describe(d);
+    if (d.mf_kernel) {
+      w_.assign(d.mf_kernel, d.mf_kernel + L.Ntot());
+      for (double& x : w_) x *= scale_;
+    } else {
+      cdft_system sys = nullptr;
+      check(cdft_system_create(&d, &sys));
+      w_.resize(L.Ntot());
+      const int rc = cdft_system_get_mf_kernel(sys, w_.data());
+      cdft_system_destroy(sys);
+      check(rc);
+    }
+  }
+  return w_.at(s);
 }
 
 double Interaction_Base::getVDWParameter() const {
@@ -463,7 +491,10 @@ cdft_system DFT::system() const {
   const double mass = s->is_mass_fixed() ? s->getFixedMass() : 0.0;
   const int bnd = (s->is_background_fixed() ? CDFT_FIXED_BACKGROUND : 0) |
                   (s->is_boundary_homogeneous() ? CDFT_HOMOGENEOUS_BOUNDARY : 0);
-  if (sys_ && sys_mu_ == mu && sys_mass_ == mass && sys_bnd_ == bnd && sys_field_ == D.field_version()) return sys_;
+  const double mfs = interactions_.empty() ? 1.0 : interactions_.front()->scale();
+  if (sys_ && sys_mu_ == mu && sys_mass_ == mass && sys_bnd_ == bnd && sys_mfs_ == mfs &&
+      sys_field_ == D.field_version())
+    return sys_;
   invalidate();
   cdft_system_desc d{};
   d.Nx = (int)D.Nx();
@@ -491,6 +522,7 @@ cdft_system DFT::system() const {
   sys_mu_ = mu;
   sys_mass_ = mass;
   sys_bnd_ = bnd;
+  sys_mfs_ = mfs;
   sys_field_ = D.field_version();
   return sys_;
 }

@@ -14,6 +14,7 @@
 //   fixed_mass_fire <F> <mu> <N>        then 10 fireMinimizer2 steps
 //   fixed_bg <F> <sum dF> <F onlyFex>   Species::setFixedBackground on another 16^3 system
 //   fire_dir <F>            5 fireMinimizer2 steps with Minimizer::set_fixed_direction (ideal + WB I, mu = 0.4)
+//   scaled_mf <a> <w0> <mu>  Interaction_Base::scale_interaction(0.5) on LJ/QF: a_vdw, getW(0), mu_times_beta(0.5)
 //   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
 //   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
 //                           a_vdw of a first object (kernel generated, cache written) and of a second one (cache
@@ -171,6 +172,17 @@ int main() {
   fire9.set_fixed_direction(e);
   for (int i = 0; i < 5; ++i) fire9.step();
   std::printf("fire_dir %.17g\n", fire9.getF());
+
+  {
+    FMT_Species s12(d2, 1.0, 0.0);
+    Interaction_Interpolation_QF qf12(&s12, &s12, &lj, 1.0);
+    qf12.scale_interaction(0.5);
+    WhiteBearI fmt12;
+    DFT dft12(&s12);
+    dft12.addHardCoreContribution(&fmt12);
+    dft12.addInteraction(&qf12);
+    std::printf("scaled_mf %.17g %.17g %.17g\n", qf12.getVDWParameter(), qf12.getW(0), dft12.mu_times_beta(0.5));
+  }
 
   Density d10(0.1, L2);
   for (long i = 0; i < d10.Ntot(); ++i) d10.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(91, i) - 1.0)));

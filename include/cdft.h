@@ -120,6 +120,9 @@ typedef struct cdft_system_desc {
      enum cdft_boundary flags.  Fixed mass and a fixed background exclude each other, as in the reference. */
   double fixed_mass;
   int boundary;
+  /* factor on the mean-field kernel and a_vdw (Interaction_Base::scale_interaction, src/Interaction.cpp:54-59);
+     0 means 1 */
+  double mf_scale;
 } cdft_system_desc;
 
 /* A pair potential for the host-side helpers below (fields as in cdft_system_desc). */

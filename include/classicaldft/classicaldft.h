@@ -342,6 +342,13 @@ class Interaction_Base {
   double kT() const { return kT_; }
   // the mean-field fields of a system description (potential, split, rule, kT)
   void describe(cdft_system_desc& d) const;
+  // src/Interaction.cpp:54-59: multiplies the kernel and a_vdw (DFTs built afterwards use the scaled kernel)
+  void scale_interaction(double scale_fac) { scale_ *= scale_fac; a_vdw_ = NAN; w_.clear(); }
+  double scale() const { return scale_; }
+  // include/Interaction.h:41: the kernel value (already / kT, scaled) at lattice position s
+  double getW(long s) const;
+  // include/Interaction.h:57
+  std::string get_name() const { return v_ ? v_->getIdentifier() : std::string("none"); }
 
  protected:
   virtual void ensure_kernel() const {}
@@ -351,6 +358,8 @@ class Interaction_Base {
   int scheme_, ngauss_;
   mutable double a_vdw_ = NAN;
   mutable std::vector<double> kernel_;  // explicit kernel (already / kT) of the points-file class
+  mutable std::vector<double> w_;       // host copy for getW
+  double scale_ = 1.0;
 };
 
 // include/Interaction.h:97-123, src/Interaction.cpp:566-860: the kernel from a file of spherical integration
@@ -448,7 +457,7 @@ class DFT {
   FMT* fmt_ = nullptr;
   int precision_ = CDFT_FP64;
   mutable cdft_system sys_ = nullptr;
-  mutable double sys_mu_ = NAN, sys_mass_ = NAN;
+  mutable double sys_mu_ = NAN, sys_mass_ = NAN, sys_mfs_ = NAN;
   mutable int sys_bnd_ = -1;
   mutable unsigned sys_field_ = ~0u;
   double F_id_ = 0, F_ext_ = 0, F_hs_ = 0, F_mf_ = 0;

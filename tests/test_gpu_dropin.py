@@ -204,3 +204,12 @@ def test_points_interaction_through_cpp_api(demo):
     o = oracle.DFT(shape, dx, 1.0, accurate_weights=True)
     o.set_mean_field(wc, wc.sum())
     assert rel(F, o.eval(rho)[0][0]) <= 1e-11
+
+
+def test_scaled_interaction_through_cpp_api(demo):
+    """Interaction_Base::scale_interaction (src/Interaction.cpp:54-59), getW and the bulk mu that follows."""
+    w, a = oracle.mf_kernel((16, 16, 16), 0.1, kT=1.0)
+    a_s, w0, mu = demo["scaled_mf"]
+    assert rel(a_s, 0.5 * a) <= 1e-12 and rel(w0, 0.5 * w[0, 0, 0]) <= 1e-13
+    _, muex = oracle.bulk(oracle.WBI, 0.5, 1.0)
+    assert rel(mu, np.log(0.5) + muex + 0.5 * a * 0.5) <= 1e-12

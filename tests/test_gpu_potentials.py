@@ -91,3 +91,21 @@ def test_points_kernel_matches_oracle(cdft, kind, bh):
     assert abs(p1[0] - p0[0]) <= 1e-11 * abs(p0[0])
     assert np.max(np.abs(d1 - d0)) <= 1e-11 * np.abs(d0).max()
     assert abs(s.a_vdw - w1.sum() / kT) <= 1e-12 * abs(w1.sum() / kT)
+
+
+def test_scaled_mean_field(cdft):
+    """scale_interaction's factor (desc mf_scale) on the kernel spectrum and a_vdw, against the oracle's DFT with the
+    kernel and a_vdw scaled."""
+    shape, dx, kT, f = (16, 12, 24), 0.1, 1.0, 0.7
+    rho = 0.5 * (1 + 0.1 * (2 * oracle.splitmix64_uniform(int(np.prod(shape)), 3).reshape(shape) - 1))
+    w, a = oracle.mf_kernel(shape, dx, kT=kT)
+    o = oracle.DFT(shape, dx, 1.0, model=oracle.WBI, kT=kT, mu=0.2)
+    o.set_mean_field(f * w, f * a)
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I, weights=oracle.fmt_weights(shape, dx, 1.0), ideal=True,
+                    mu=0.2, mean_field=True, kT=kT, mf_scale=f)
+    assert abs(s.a_vdw - f * a) <= 1e-12 * abs(f * a)
+    assert np.max(np.abs(s.device_mf_kernel() - f * w)) <= 1e-13 * np.abs(w).max()
+    p0, d0 = o.eval(rho)
+    p1, d1 = s.eval(rho)
+    assert abs(p1[0] - p0[0]) <= 1e-11 * abs(p0[0]) and abs(p1[3] - p0[3]) <= 1e-11 * abs(p0[0])
+    assert np.max(np.abs(d1 - d0)) <= 1e-11 * np.abs(d0).max()
