@@ -10,7 +10,7 @@ CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT = 0, 1, 2
 CDFT_BAD_ARG, CDFT_HIP_ERROR, CDFT_NOT_SUPPORTED = -1, -2, -3
 MODEL_NONE, ROSENFELD, RSLT, ESFMT, WHITEBEAR_I, WHITEBEAR_II = -1, 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
-DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL = 1, 2, 4, 8
+DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, SQUARED_MATRIX = 1, 2, 4, 8, 16
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -119,6 +119,8 @@ def lib():
         L.cdft_fire_batch_destroy.argtypes = [_p]
         L.cdft_fire_batch_destroy.restype = None
         L.cdft_system_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_system_log_det.argtypes = [_p, _p, _i, _d, _d, ctypes.c_long, _i, _i, _i, ctypes.POINTER(_d),
+                                          ctypes.POINTER(_d)]
         L.cdft_potential_hsd.argtypes = [ctypes.POINTER(_PotDesc), _d, ctypes.POINTER(_d)]
         L.cdft_potential_vdw.argtypes = [ctypes.POINTER(_PotDesc), _d, ctypes.POINTER(_d)]
         L.cdft_potential_watt.argtypes = [ctypes.POINTER(_PotDesc), _i, _p, _p]
@@ -297,6 +299,21 @@ class System:
         _check(lib().cdft_system_hessian_dot(self.h, _ptr(rho), _ptr(v), _ptr(res), fl))
         return res
 
+
+    def log_det(self, rho, order, lam_max, lam_min, seed, num_samples, has_zero_eigenvalue=False, ideal=True,
+                squared=False):
+        """(log det, variance): Log_Det(dft, order, lam_max, lam_min).calculate_log_det(seed, num_samples,
+        has_zero_eigenvalue, variance) (src/Log_Det.cpp:22-166); ideal = full_hessian_, squared =
+        use_squared_matrix_."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        if rho.size != self.N:
+            raise ValueError(f"expected {self.N} densities")
+        ld, var = _d(), _d()
+        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0)
+        _check(lib().cdft_system_log_det(self.h, _ptr(rho), int(order), float(lam_max), float(lam_min), int(seed),
+                                         int(num_samples), int(bool(has_zero_eigenvalue)), fl, ctypes.byref(ld),
+                                         ctypes.byref(var)))
+        return ld.value, var.value
     def fmt_hessian_dot(self, rho, v, out=None, accumulate=False):
         """FMT::add_second_derivative: the hard-sphere part of the Hessian applied to v."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)

@@ -4,6 +4,8 @@
 // each of the m FMT weights and the mean-field kernel), the spectral work
 // arrays, the twiddle tables and a HIP stream.  cdft_system_eval runs the
 // fused pipeline documented in kernels.hpp; no C++ exception crosses the ABI.
+#include <random>
+
 #include "system.hpp"
 
 namespace cdft_impl {
@@ -1091,6 +1093,157 @@ int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, do
   return guarded([&] {
     REQUIRE(sys && sys->m > 0, "system has no FMT functional");
     return run_hess(sys, rho, v, out, flags, false);
+  });
+}
+
+namespace {
+// One term of the Chebyshev recursion of Log_Det::calculate_log_det (src/Log_Det.cpp:122-153):
+// x = (((r + z) scale) cA) + cB w1 - w0 with z = zf * sum (the zero-mode shift lam_mid accu(w1) / Ntot, sum on
+// the device; zf = 0 without it); w2 = x; u = c0 w1 + ck x on the first term (w1 = the probe, no w0), else
+// u += ck x.
+__global__ __launch_bounds__(256) void k_cheb_step(const double* __restrict__ r, const double* __restrict__ w1,
+                                                   const double* __restrict__ w0, double* __restrict__ w2,
+                                                   double* __restrict__ u, long n, const double* __restrict__ sum,
+                                                   double zf, double scale, double cA, double cB, double c0,
+                                                   double ck, int first) {
+  const double z = zf != 0.0 ? zf * sum[0] : 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double x = r[i] + z;
+    x *= scale;
+    x *= cA;
+    x += cB * w1[i];
+    if (w0) x -= w0[i];
+    w2[i] = x;
+    u[i] = first ? c0 * w1[i] + ck * x : u[i] + ck * x;
+  }
+}
+}  // namespace
+
+int cdft_system_log_det(cdft_system sys, const double* rho, int order, double lam_max, double lam_min, long seed,
+                        int num_samples, int has_zero_eigenvalue, int flags, double* log_det, double* variance) {
+  return guarded([&] {
+    cdft_system_s& s = *sys;
+    REQUIRE(sys && rho && log_det && variance, "null argument");
+    REQUIRE(s.nb == 1, "the log-determinant takes one system, not a batch");
+    REQUIRE(order >= 1 && num_samples >= 1, "order and num_samples must be positive");
+    REQUIRE(lam_max > 0 && lam_min > 0 && lam_max > lam_min, "need 0 < lam_min < lam_max");
+    const long N = s.N;
+    const size_t bytes = sizeof(double) * N;
+    const bool bnd = s.d.boundary & CDFT_FIXED_BACKGROUND;
+    const long nbnd = boundary_count(s.g.Nx, s.g.Ny, s.g.Nz);
+    const bool ideal = !(flags & CDFT_NO_IDEAL), squared = flags & CDFT_SQUARED_MATRIX;
+    // Log_Det::Log_Det and get_coefficients (src/Log_Det.cpp:22-62)
+    const double scale = 1.0 / (lam_min + lam_max), a = lam_min * scale, b = lam_max * scale;
+    const double lam_mid = (lam_max + lam_min) / 2;
+    std::vector<double> c(order + 1, 0.0);
+    for (int k = 0; k <= order; ++k) {
+      const double xk = std::cos(M_PI * (k + 0.5) / (order + 1));
+      const double f = std::log(0.5 * (b - a) * xk + 0.5 * (b + a));
+      double tm2 = 1.0, tm1 = xk;
+      for (int j = 0; j <= order; ++j) {
+        const double t = j == 0 ? 1.0 : j == 1 ? xk : 2 * xk * tm1 - tm2;
+        if (j >= 2) {
+          tm2 = tm1;
+          tm1 = t;
+        }
+        c[j] += f * t;
+      }
+    }
+    for (int j = 0; j <= order; ++j) c[j] *= (j == 0 ? 1.0 : 2.0) / (order + 1);
+    const double c1 = order >= 1 ? c[1] : 0.0;
+    // device vectors: probe, three recursion terms, accumulator, product, projected input, sums
+    DevBuf rho_buf, work, sums;
+    work.alloc(7 * bytes);
+    sums.alloc(2 * sizeof(double));
+    double* W = work.as<double>();
+    double *v = W, *w0 = W + N, *w1 = W + 2 * N, *w2 = W + 3 * N, *u = W + 4 * N, *r = W + 5 * N, *t = W + 6 * N;
+    const double* rho_d = rho;
+    if (!(flags & CDFT_DEVICE_PTRS)) {
+      rho_buf.alloc(bytes);
+      HIPCHK(hipMemcpyAsync(rho_buf.p, rho, bytes, hipMemcpyHostToDevice, s.stream));
+      rho_d = rho_buf.as<double>();
+    }
+    const long nblk = std::min<long>(s.pslots, std::max<long>(1, std::min<long>(div_up(N, kThreads), 1024)));
+    auto dsum = [&](const double* x, const double* y, int slot) {  // compensated sum (x) or dot (x, y)
+      hipLaunchKernelGGL(k_cons_sum, dim3((unsigned)nblk, 1), dim3(kThreads), 0, s.stream, x, y, N, N,
+                         y ? CONS_DOT : CONS_SUM, s.g.Nx, s.g.Ny, s.g.Nz, s.partials.as<double2>(), s.pslots);
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+                         sums.as<double>() + slot, s.pslots, 1L);
+      HIPCHK(hipGetLastError());
+    };
+    auto zero_bnd = [&](double* x) {
+      if (!bnd) return;
+      hipLaunchKernelGGL(k_cons_bset, cons_grid(s, nbnd), dim3(kThreads), 0, s.stream, x, nbnd, N, s.g.Nx, s.g.Ny,
+                         s.g.Nz, nullptr, 8L, 0.0);
+      HIPCHK(hipGetLastError());
+    };
+    // Log_Det::matrix_dot_v1 (src/Log_Det.cpp:73-92) through Dynamical_Matrix::matrix_dot_v
+    // (src/Dynamical_Matrix.cpp:34-50) and DFT::matrix_dot_v_intern (src/DFT.cpp:435-488): r = A x
+    auto matvec = [&](const double* x) {
+      HIPCHK(hipMemcpyAsync(t, x, bytes, hipMemcpyDeviceToDevice, s.stream));
+      zero_bnd(t);
+      hess_device(s, rho_d, t, r, true, ideal, false);
+      zero_bnd(r);
+      if (squared) {
+        HIPCHK(hipMemcpyAsync(t, r, bytes, hipMemcpyDeviceToDevice, s.stream));
+        hess_device(s, rho_d, t, r, true, ideal, false);
+        zero_bnd(r);
+      }
+    };
+    const dim3 eg((unsigned)std::max<long>(1, std::min<long>(div_up(N, kThreads), 8192)));
+    const double zf = has_zero_eigenvalue ? lam_mid / (double)N : 0.0;
+    if (seed <= 0) {
+      std::random_device rd;
+      seed = rd();
+    }
+    std::mt19937 rng(seed);
+    std::uniform_int_distribution<> distrib(0, 1);
+    std::vector<double> vh(N);
+    double ld = 0.0, var = 0.0, sd = 0.0;
+    for (int i = 1; i <= num_samples; ++i) {
+      for (long p = 0; p < N; ++p) vh[p] = distrib(rng) > 0 ? 1.0 : -1.0;
+      HIPCHK(hipMemcpyAsync(v, vh.data(), bytes, hipMemcpyHostToDevice, s.stream));
+      zero_bnd(v);
+      matvec(v);
+      if (zf != 0.0) dsum(v, nullptr, 0);
+      hipLaunchKernelGGL(k_cheb_step, eg, dim3(kThreads), 0, s.stream, r, v, nullptr, w1, u, N, sums.as<double>(),
+                         zf, scale, 2.0 / (b - a), -(b + a) / (b - a), c[0], c1, 1);
+      HIPCHK(hipGetLastError());
+      double *p0 = v, *p1 = w1, *p2 = w2, *spare = w0;
+      for (int k = 2; k <= order; ++k) {
+        matvec(p1);
+        if (zf != 0.0) dsum(p1, nullptr, 0);
+        hipLaunchKernelGGL(k_cheb_step, eg, dim3(kThreads), 0, s.stream, r, p1, p0, p2, u, N, sums.as<double>(), zf,
+                           scale, 4.0 / (b - a), -2 * (b + a) / (b - a), 0.0, c[k], 0);
+        HIPCHK(hipGetLastError());
+        // w0 <- w1, w1 <- w2; the probe v stays intact for the final dot product
+        double* freed = p0 == v ? spare : p0;
+        p0 = p1;
+        p1 = p2;
+        p2 = freed;
+      }
+      dsum(v, u, 1);
+      double vu = 0.0;
+      HIPCHK(hipMemcpyAsync(&vu, sums.as<double>() + 1, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+      HIPCHK(hipStreamSynchronize(s.stream));
+      ld += vu;
+      var += vu * vu;
+      const double av = ld / i, av2 = var / i;
+      sd = std::sqrt(std::fabs(av2 - av * av));
+    }
+    double out = ld / num_samples;
+    if (has_zero_eigenvalue) out -= std::log(lam_mid);
+    out -= (double)(N - (bnd ? nbnd : 0)) * std::log(scale);
+    const bool last = s.last_with_dft, last_fm = s.last_fixed_mass;
+    s.last_fixed_mass = false;
+    const int rc = fetch_parts(&s, nullptr);  // reports eta > 1 met by any product
+    s.last_with_dft = last;
+    s.last_fixed_mass = last_fm;
+    if (rc != CDFT_OK) return rc;
+    *log_det = out;
+    *variance = sd;
+    return (int)CDFT_OK;
   });
 }
 

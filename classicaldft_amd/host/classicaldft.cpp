@@ -601,6 +601,16 @@ void DFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param) const {
   result.set(rw[0]);
 }
 
+double Log_Det::calculate_log_det(long seed, int num_samples, bool has_zero_eigenvalue, double& variance) {
+  const Species* s = matrix_.species_.front();
+  double ld = 0.0;
+  const int flags = (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0);
+  check(cdft_system_log_det(matrix_.system(), s->getDensity().get_density_pointer(), order_, lam_max_, lam_min_, seed,
+                            num_samples, has_zero_eigenvalue ? 1 : 0, flags, &ld, &variance));
+  if (verbose_) std::printf("\t samples = %d log_det = %g variance = %g\n", num_samples, ld, variance);
+  return ld;
+}
+
 double DFT::mu_times_beta(double density) const {
   double f, mu;
   check(cdft_system_bulk(system(), density, &f, &mu));

@@ -15,6 +15,7 @@
 //   fixed_bg <F> <sum dF> <F onlyFex>   Species::setFixedBackground on another 16^3 system
 //   fire_dir <F>            5 fireMinimizer2 steps with Minimizer::set_fixed_direction (ideal + WB I, mu = 0.4)
 //   scaled_mf <a> <w0> <mu>  Interaction_Base::scale_interaction(0.5) on LJ/QF: a_vdw, getW(0), mu_times_beta(0.5)
+//   log_det <ld> <var>       Log_Det(dft, 20, 0.05, 5e-4).calculate_log_det(7, 2, false) (16^3, ideal + WB I)
 //   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
 //   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
 //                           a_vdw of a first object (kernel generated, cache written) and of a second one (cache
@@ -106,6 +107,16 @@ int main() {
   for (long i = 0; i < d4.Ntot(); ++i) v.set(i, 2.0 * splitmix_u(43, i) - 1.0);
   dft4.matrix_dot_v1(v, Hv);
   std::printf("hess %.17g %.17g\n", Hv.accu(), Hv.dotWith(v));
+  {
+    FMT_Species s13(d4, 1.0, 0.0);
+    WhiteBearI fmt13;
+    DFT dft13(&s13);
+    dft13.addHardCoreContribution(&fmt13);
+    Log_Det ldet(dft13, 20, 0.05, 5e-4);
+    double var = 0.0;
+    const double ldv = ldet.calculate_log_det(7, 2, false, var);
+    std::printf("log_det %.17g %.17g\n", ldv, var);
+  }
   std::vector<DFT_Vec> vv(1), d2F(1);
   vv[0].set(v);
   d2F[0].zeros(d4.Ntot());

@@ -74,7 +74,8 @@ enum cdft_flags {
   CDFT_ACCUMULATE = 2,  /* add into dF instead of overwriting it */
   CDFT_FMT_ONLY = 4,    /* hard-sphere part only (cdft_system_eval_async / _time) */
   CDFT_NO_IDEAL = 8     /* no ideal-gas term: DFT::calculateFreeEnergyAndDerivatives(onlyFex = true) for
-                           cdft_system_eval; DFT::full_hessian_ false for the Hessian-vector product */
+                           cdft_system_eval; DFT::full_hessian_ false for the Hessian-vector product */,
+  CDFT_SQUARED_MATRIX = 16 /* cdft_system_log_det: apply the Hessian twice (use_squared_matrix_) */
 };
 
 /* Species boundary conditions (include/Species.h:24-25, applied in Species::endForceCalculation,
@@ -163,6 +164,16 @@ int cdft_system_hessian_dot(cdft_system sys, const double* rho, const double* v,
 /* Replaces FMT::add_second_derivative (src/FMT.cpp:437-567): the hard-sphere
    part only; add into out with CDFT_ACCUMULATE as the reference does. */
 int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
+
+/* Replaces Log_Det(matrix, order, lam_max, lam_min).calculate_log_det(seed, num_samples, has_zero_eigenvalue,
+   variance) (include/Log_Det.h:22-28, src/Log_Det.cpp:22-166) with this system's DFT as the matrix: the
+   stochastic Chebyshev estimate of log det of the Hessian at rho (Han, Malioutov, Avron and Shin 2017).  Probes are
+   the reference's (std::mt19937(seed), uniform_int_distribution<>(0, 1) -> +-1, a random seed if seed <= 0); every
+   Hessian-vector product runs on the device, as Log_Det::matrix_dot_v1 applies it (boundary entries of input and
+   output zeroed with a fixed background).  Flags: CDFT_DEVICE_PTRS (rho), CDFT_NO_IDEAL (full_hessian_ false),
+   CDFT_SQUARED_MATRIX.  FP64 single systems; CDFT_ETA_TOO_LARGE if eta > 1. */
+int cdft_system_log_det(cdft_system sys, const double* rho, int order, double lam_max, double lam_min, long seed,
+                        int num_samples, int has_zero_eigenvalue, int flags, double* log_det, double* variance);
 
 /* Asynchronous form on device pointers: enqueues one evaluation on the
    system's stream and returns without synchronising.  cdft_system_fetch

@@ -463,6 +463,24 @@ class DFT {
   double F_id_ = 0, F_ext_ = 0, F_hs_ = 0, F_mf_ = 0;
   bool full_hessian_ = true, use_squared_matrix_ = false;
   void invalidate() const;
+  friend class Log_Det;
+};
+
+// include/Log_Det.h:22-51, src/Log_Det.cpp:22-166: stochastic Chebyshev estimate of log det of the DFT's
+// Hessian (its full_hessian_ and use_squared_matrix_ settings apply), every product on the GPU.
+class Log_Det {
+ public:
+  Log_Det(const DFT& matrix, int order, double lam_max, double lam_min, bool verbose = false)
+      : matrix_(matrix), order_(order), lam_max_(lam_max), lam_min_(lam_min), verbose_(verbose) {}
+  double calculate_log_det(long seed, int num_samples, bool has_zero_eigenvalue, double& variance);
+  void set_verbose(bool b) { verbose_ = b; }
+  void set_debug(bool b) { debug_ = b; }
+
+ private:
+  const DFT& matrix_;
+  int order_;
+  double lam_max_, lam_min_;
+  bool verbose_ = false, debug_ = false;
 };
 
 // ------------------------------------------------------------- minimizer ---

@@ -72,6 +72,10 @@ def lib():
         L.orc_dft_eval.restype = _i
         L.orc_dft_hess.argtypes = [_p, _p, _p, _p, _i]
         L.orc_dft_hess.restype = _i
+        L.orc_cheb_log_coeffs.argtypes = [_i, _d, _d, _p]
+        L.orc_rademacher.argtypes = [ctypes.c_long, ctypes.c_long, _i, _p]
+        L.orc_dft_log_det.argtypes = [_p, _p, _i, _d, _d, ctypes.c_long, _i, _i, _i, _i, _p, _p]
+        L.orc_dft_log_det.restype = _i
         L.orc_dft_set_constraints.argtypes = [_p, _d, _i]
         L.orc_dft_set_constraints.restype = _i
         L.orc_dft_lagrange.argtypes = [_p]
@@ -327,6 +331,16 @@ class DFT:
             raise ValueError("Input vector v must have zero boundary entries when the species has fixed boundaries")
         return out
 
+    def log_det(self, rho, order, lam_max, lam_min, seed, num_samples, has_zero=False, full=True, squared=False):
+        """(log det, variance): Log_Det(matrix, order, lam_max, lam_min).calculate_log_det(seed, num_samples,
+        has_zero_eigenvalue, variance) (src/Log_Det.cpp:22-166) on this DFT's Hessian at rho."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        ld, var = ctypes.c_double(), ctypes.c_double()
+        lib().orc_dft_log_det(self.h, _ptr(rho), int(order), float(lam_max), float(lam_min), int(seed),
+                              int(num_samples), int(bool(has_zero)), int(bool(full)), int(bool(squared)),
+                              ctypes.byref(ld), ctypes.byref(var))
+        return ld.value, var.value
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().orc_dft_destroy(self.h)
@@ -445,3 +459,19 @@ class OpenDDFT:
         if getattr(self, "h", None):
             lib().orc_open_destroy(self.h)
             self.h = None
+
+
+def cheb_log_coeffs(order, lam_max, lam_min):
+    """Log_Det::get_coefficients (src/Log_Det.cpp:36-62) for the scaled interval [a, b] = [lam_min, lam_max] /
+    (lam_min + lam_max)."""
+    c = np.zeros(order + 1)
+    sc = 1.0 / (lam_min + lam_max)
+    lib().orc_cheb_log_coeffs(int(order), lam_min * sc, lam_max * sc, _ptr(c))
+    return c
+
+
+def rademacher(seed, n, nsamples=1):
+    """The +-1 probes of src/Log_Det.cpp:112-120 (std::mt19937(seed), uniform_int_distribution<>(0, 1))."""
+    out = np.zeros(n * nsamples)
+    lib().orc_rademacher(int(seed), int(n), int(nsamples), _ptr(out))
+    return out.reshape(nsamples, n)

@@ -20,11 +20,14 @@
 //                   uniform F_mf/V = a_vdw rho^2 / 2
 //   FIRE2, DDFT  -> invariants (mass conservation, exact diffusion solution)
 //                   and trajectory replay against the GPU path.
+//   log det      -> numpy's MT19937 (standard known answer), numpy's Chebyshev
+//                   interpolant, the estimator evaluated on the dense Hessian.
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <random>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1613,6 +1616,115 @@ int orc_dft_hess(orc_dft* h, const double* rho, const double* v, double* out, in
   if (s->bnd & 1)
     for (long p = 0; p < N; ++p)
       if (boundary_point(s->fmt.g, p)) out[p] = 0.0;
+  return 0;
+}
+
+// Coefficients of the Chebyshev interpolant of log(0.5 (b - a) x + 0.5 (b + a)) at the order + 1 nodes
+// x_k = cos(pi (k + 1/2) / (order + 1)): Log_Det::get_coefficients (src/Log_Det.cpp:36-62).
+void orc_cheb_log_coeffs(int order, double a, double b, double* c) {
+  for (int j = 0; j <= order; ++j) c[j] = 0.0;
+  for (int k = 0; k <= order; ++k) {
+    const double xk = std::cos(M_PI * (k + 0.5) / (order + 1));
+    const double f = std::log(0.5 * (b - a) * xk + 0.5 * (b + a));
+    double t0 = 1.0, t1 = xk;  // T_j(xk) by the three-term recurrence
+    for (int j = 0; j <= order; ++j) {
+      const double t = j == 0 ? 1.0 : j == 1 ? xk : 2 * xk * t1 - t0;
+      if (j >= 2) { t0 = t1; t1 = t; }
+      c[j] += f * t;
+    }
+  }
+  for (int j = 0; j <= order; ++j) c[j] *= (j == 0 ? 1.0 : 2.0) / (order + 1);
+}
+
+// Rademacher probes as src/Log_Det.cpp:112-120 draws them: std::mt19937(seed), then per site
+// uniform_int_distribution<>(0, 1) -> +1 / -1; nsamples vectors of n values, one after another.
+void orc_rademacher(long seed, long n, int nsamples, double* out) {
+  std::mt19937 rng(seed);
+  std::uniform_int_distribution<> distrib(0, 1);
+  for (long i = 0; i < n * (long)nsamples; ++i) out[i] = distrib(rng) > 0 ? 1.0 : -1.0;
+}
+
+// Log_Det::calculate_log_det (src/Log_Det.cpp:96-166) on this DFT's Hessian, applied as
+// Log_Det::matrix_dot_v1 / Dynamical_Matrix::matrix_dot_v (src/Log_Det.cpp:73-92,
+// src/Dynamical_Matrix.cpp:34-50): with a fixed background the input's boundary entries are zeroed on a copy and
+// the product's after it; squared != 0 applies the matrix twice (use_squared_matrix_).  full = full_hessian_.
+// Returns 2 if eta > 1 is met, else 0.
+int orc_dft_log_det(orc_dft* h, const double* rho, int order, double lam_max, double lam_min, long seed,
+                    int num_samples, int has_zero, int full, int squared, double* log_det, double* variance) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  const Grid& g = s->fmt.g;
+  const long N = g.N();
+  const bool bnd = s->bnd & 1;
+  long nbnd = 0;
+  for (long p = 0; p < N; ++p) nbnd += bnd && boundary_point(g, p);
+  const double scale = 1.0 / (lam_min + lam_max), a = lam_min * scale, b = lam_max * scale;
+  const double lam_mid = (lam_max + lam_min) / 2;
+  std::vector<double> c(order + 1);
+  orc_cheb_log_coeffs(order, a, b, c.data());
+  auto zero_bnd = [&](std::vector<double>& x) {
+    if (bnd)
+      for (long p = 0; p < N; ++p)
+        if (boundary_point(g, p)) x[p] = 0.0;
+  };
+  std::vector<double> t(N), r(N);
+  auto matvec = [&](const std::vector<double>& x) {
+    t = x;
+    zero_bnd(t);
+    orc_dft_hess(h, rho, t.data(), r.data(), full);
+    if (squared) {
+      t = r;
+      orc_dft_hess(h, rho, t.data(), r.data(), full);
+    }
+    zero_bnd(r);
+  };
+  auto accu = [&](const std::vector<double>& x) {
+    double sum = 0.0;
+    for (long i = 0; i < N; ++i) sum += x[i];
+    return sum;
+  };
+  std::mt19937 rng(seed);
+  std::uniform_int_distribution<> distrib(0, 1);
+  std::vector<double> v(N), w0(N), w1(N), w2(N), u(N);
+  double ld = 0.0, var = 0.0, sd = 0.0;
+  for (int i = 1; i <= num_samples; ++i) {
+    for (long p = 0; p < N; ++p) v[p] = distrib(rng) > 0 ? 1.0 : -1.0;
+    zero_bnd(v);
+    matvec(v);
+    const double z0 = has_zero ? lam_mid * accu(v) / N : 0.0;
+    for (long p = 0; p < N; ++p) {
+      double x = r[p] + z0;
+      x *= scale;
+      x *= 2.0 / (b - a);
+      w1[p] = x - (b + a) / (b - a) * v[p];
+    }
+    w0 = v;
+    for (long p = 0; p < N; ++p) u[p] = c[0] * w0[p] + c[1] * w1[p];
+    for (int k = 2; k <= order; ++k) {
+      matvec(w1);
+      const double z = has_zero ? lam_mid * accu(w1) / N : 0.0;
+      for (long p = 0; p < N; ++p) {
+        double x = r[p] + z;
+        x *= scale;
+        x *= 4.0 / (b - a);
+        x += -2 * (b + a) / (b - a) * w1[p];
+        w2[p] = x - w0[p];
+        u[p] += c[k] * w2[p];
+      }
+      std::swap(w0, w1);
+      std::swap(w1, w2);
+    }
+    double vu = 0.0;
+    for (long p = 0; p < N; ++p) vu += v[p] * u[p];
+    ld += vu;
+    var += vu * vu;
+    const double av = ld / i, av2 = var / i;
+    sd = std::sqrt(std::fabs(av2 - av * av));
+  }
+  double out = ld / num_samples;
+  if (has_zero) out -= std::log(lam_mid);
+  out -= (double)(N - nbnd) * std::log(scale);
+  *log_det = out;
+  *variance = sd;
   return 0;
 }
 

@@ -213,3 +213,14 @@ def test_scaled_interaction_through_cpp_api(demo):
     assert rel(a_s, 0.5 * a) <= 1e-12 and rel(w0, 0.5 * w[0, 0, 0]) <= 1e-13
     _, muex = oracle.bulk(oracle.WBI, 0.5, 1.0)
     assert rel(mu, np.log(0.5) + muex + 0.5 * a * 0.5) <= 1e-12
+
+
+def test_log_det_through_cpp_api(demo):
+    """Log_Det(dft, 20, 0.05, 5e-4).calculate_log_det(7, 2, false, variance) (src/Log_Det.cpp:96-166)."""
+    shape, dx = (16, 16, 16), 0.1
+    rho = seeded(shape, 0.5, 41, 0.1)
+    o = oracle.DFT(shape, dx, 1.0, accurate_weights=True)
+    ld, var = o.log_det(rho, 20, 0.05, 5e-4, 7, 2)
+    assert rel(demo["log_det"][0], ld) <= 1e-10
+    q = ld + rho.size * np.log(1.0 / (0.05 + 5e-4))  # the mean of v.u over the probes
+    assert abs(demo["log_det"][1] ** 2 - var ** 2) <= 1e-10 * q * q
