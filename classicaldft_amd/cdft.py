@@ -119,6 +119,7 @@ def lib():
         L.cdft_fire_batch_destroy.argtypes = [_p]
         L.cdft_fire_batch_destroy.restype = None
         L.cdft_system_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_system_eigen_objective.argtypes = [_p, _p, _p, _p, _d, _i, _i, ctypes.POINTER(_d), _p]
         L.cdft_system_log_det.argtypes = [_p, _p, _i, _d, _d, ctypes.c_long, _i, _i, _i, ctypes.POINTER(_d),
                                           ctypes.POINTER(_d)]
         L.cdft_potential_hsd.argtypes = [ctypes.POINTER(_PotDesc), _d, ctypes.POINTER(_d)]
@@ -212,6 +213,7 @@ class System:
         fixed mass per member."""
         fm = np.atleast_1d(np.asarray(fixed_mass, dtype=np.float64)).reshape(-1)
         self.shape = tuple(int(n) for n in shape)
+        self.fixed_background = bool(fixed_background)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
         self.dV = dxs[0] * dxs[1] * dxs[2]
@@ -299,6 +301,22 @@ class System:
         _check(lib().cdft_system_hessian_dot(self.h, _ptr(rho), _ptr(v), _ptr(res), fl))
         return res
 
+
+    def eigen_objective(self, rho, x, vshift=None, scale=1.0, change_sign=False, ideal=True, squared=False):
+        """(f, df) of Eigenvalues::calculate_gradients (src/Eigenvalues.cpp:46-70): the scaled Rayleigh quotient
+        of the Hessian at rho (plus vshift (vshift.x)/2) and its gradient; ideal = full_hessian_."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if rho.size != self.N or x.size != self.N:
+            raise ValueError(f"expected {self.N} densities and components")
+        vs = None if vshift is None else np.ascontiguousarray(vshift, dtype=np.float64)
+        df = np.zeros(x.shape)
+        f = _d()
+        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0)
+        _check(lib().cdft_system_eigen_objective(self.h, _ptr(rho), _ptr(x), None if vs is None else _ptr(vs),
+                                                 float(scale), int(bool(change_sign)), fl, ctypes.byref(f),
+                                                 _ptr(df)))
+        return f.value, df
 
     def log_det(self, rho, order, lam_max, lam_min, seed, num_samples, has_zero_eigenvalue=False, ideal=True,
                 squared=False):
@@ -789,3 +807,52 @@ def mf_kernel_points(shape, dx, points, potential=POT_LJ, sigma=1.0, eps=1.0, rc
     _check(lib().cdft_mf_kernel_points(int(shape[0]), int(shape[1]), int(shape[2]), dxs[0], dxs[1], dxs[2],
                                        ctypes.byref(pd), _ptr(pts), len(pts), int(nradial), _ptr(out)))
     return out
+
+
+class Eigenvalues:
+    """Eigenvalues(matrix) of include/Eigenvalues.h:16-59: the smallest (or, with change_sign, the largest)
+    eigenvalue of the Hessian of `system` at `rho` by minimising the scaled Rayleigh quotient of
+    System.eigen_objective (every product on the GPU).  The reference minimises with nlopt's LD_LBFGS, absent here;
+    this uses scipy's L-BFGS-B with the same relative tolerance on f, so the iterates differ and the converged
+    eigenpair agrees."""
+
+    def __init__(self, system, rho, ideal=True, squared=False, verbose=False):
+        self.system, self.rho = system, np.ascontiguousarray(rho, dtype=np.float64)
+        self.ideal, self.squared, self.verbose = ideal, squared, verbose
+        self.scale, self.tol, self.change_sign, self.vshift = 1e6, 1e-6, False, None
+        self.eigen_vec, self.eigen_val, self.num_eval, self.return_code = None, 0.0, 0, 0
+
+    def set_scale(self, d):
+        self.scale = float(d)
+
+    def set_tolerence(self, d):
+        self.tol = float(d)
+
+    def set_change_sign(self, b):
+        self.change_sign = bool(b)
+
+    def set_vshift(self, v):
+        self.vshift = np.ascontiguousarray(v, dtype=np.float64)
+
+    def calculate_gradients(self, x):
+        """(f, df) at x (src/Eigenvalues.cpp:46-70)."""
+        self.num_eval += 1
+        return self.system.eigen_objective(self.rho, x, self.vshift, self.scale, self.change_sign, self.ideal,
+                                           self.squared)
+
+    def calculate_eigenvector(self, x0=None, seed=0, maxiter=10000):
+        """src/Eigenvalues.cpp:72-99: start from x0 (else a seeded normal vector, zero on a fixed boundary)."""
+        from scipy.optimize import minimize
+        if x0 is None:
+            x0 = np.random.default_rng(seed).standard_normal(self.rho.size)
+            if self.system.fixed_background:
+                m = np.zeros(self.system.shape, bool)
+                m[0], m[:, 0], m[:, :, 0] = True, True, True
+                x0[m.ravel()] = 0.0
+        res = minimize(lambda x: self.calculate_gradients(x), np.ravel(x0).astype(np.float64), jac=True,
+                       method="L-BFGS-B", options={"ftol": self.tol, "gtol": 0.0, "maxiter": maxiter,
+                                                   "maxfun": maxiter})
+        self.return_code = int(res.status)
+        self.eigen_val = float(res.fun) / self.scale * (-1.0 if self.change_sign else 1.0)
+        self.eigen_vec = res.x / np.linalg.norm(res.x)
+        return self.eigen_val

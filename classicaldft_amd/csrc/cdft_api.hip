@@ -1247,6 +1247,104 @@ int cdft_system_log_det(cdft_system sys, const double* rho, int order, double la
   });
 }
 
+namespace {
+// r <- (r + vs (0.5 sum[0])) * sgn_scale   (src/Eigenvalues.cpp:49-53; vs null: no shift)
+__global__ __launch_bounds__(256) void k_eig_shift_scale(double* __restrict__ r, const double* __restrict__ vs,
+                                                         const double* __restrict__ sum, double sgn_scale, long n) {
+  const double h = vs ? 0.5 * sum[0] : 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double x = r[i];
+    if (vs) x += h * vs[i];
+    r[i] = x * sgn_scale;
+  }
+}
+// df = (r - f x) (2 / x2), f = sum[1] / x2, x2 = sum[2]   (src/Eigenvalues.cpp:55-59)
+__global__ __launch_bounds__(256) void k_eig_grad(const double* __restrict__ r, const double* __restrict__ x,
+                                                  const double* __restrict__ sum, double* __restrict__ df, long n) {
+  const double x2 = sum[2], f = sum[1] / x2, g = 2 / x2;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    df[i] = (r[i] + (-f) * x[i]) * g;
+}
+}  // namespace
+
+int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double* x, const double* vshift,
+                                double scale, int change_sign, int flags, double* f, double* df) {
+  return guarded([&] {
+    cdft_system_s& s = *sys;
+    REQUIRE(sys && rho && x && f && df, "null argument");
+    REQUIRE(s.nb == 1, "the eigenvalue objective takes one system, not a batch");
+    const long N = s.N;
+    const size_t bytes = sizeof(double) * N;
+    const bool dev = flags & CDFT_DEVICE_PTRS, bnd = s.d.boundary & CDFT_FIXED_BACKGROUND;
+    const bool ideal = !(flags & CDFT_NO_IDEAL), squared = flags & CDFT_SQUARED_MATRIX;
+    const long nbnd = boundary_count(s.g.Nx, s.g.Ny, s.g.Nz);
+    DevBuf work, sums, in;
+    work.alloc(2 * bytes + (dev ? 0 : 3 * bytes));
+    sums.alloc(3 * sizeof(double));
+    double *r = work.as<double>(), *t = r + N;
+    const double *rho_d = rho, *x_d = x, *vs_d = vshift;
+    double* df_d = df;
+    if (!dev) {
+      double* h = t + N;
+      HIPCHK(hipMemcpyAsync(h, rho, bytes, hipMemcpyHostToDevice, s.stream));
+      HIPCHK(hipMemcpyAsync(h + N, x, bytes, hipMemcpyHostToDevice, s.stream));
+      rho_d = h;
+      x_d = h + N;
+      df_d = h + 2 * N;
+      if (vshift) {
+        in.alloc(bytes);
+        HIPCHK(hipMemcpyAsync(in.p, vshift, bytes, hipMemcpyHostToDevice, s.stream));
+        vs_d = in.as<double>();
+      }
+    }
+    const long nblk = std::min<long>(s.pslots, std::max<long>(1, std::min<long>(div_up(N, kThreads), 1024)));
+    auto ddot = [&](const double* a, const double* b, int slot) {
+      hipLaunchKernelGGL(k_cons_sum, dim3((unsigned)nblk, 1), dim3(kThreads), 0, s.stream, a, b, N, N, CONS_DOT,
+                         s.g.Nx, s.g.Ny, s.g.Nz, s.partials.as<double2>(), s.pslots);
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
+                         sums.as<double>() + slot, s.pslots, 1L);
+      HIPCHK(hipGetLastError());
+    };
+    auto zero_bnd = [&](double* v) {
+      if (!bnd) return;
+      hipLaunchKernelGGL(k_cons_bset, cons_grid(s, nbnd), dim3(kThreads), 0, s.stream, v, nbnd, N, s.g.Nx, s.g.Ny,
+                         s.g.Nz, nullptr, 8L, 0.0);
+      HIPCHK(hipGetLastError());
+    };
+    // Dynamical_Matrix::matrix_dot_v1 (src/Dynamical_Matrix.cpp:54-70): r = A x
+    HIPCHK(hipMemcpyAsync(t, x_d, bytes, hipMemcpyDeviceToDevice, s.stream));
+    zero_bnd(t);
+    hess_device(s, rho_d, t, r, true, ideal, false);
+    zero_bnd(r);
+    if (squared) {
+      HIPCHK(hipMemcpyAsync(t, r, bytes, hipMemcpyDeviceToDevice, s.stream));
+      hess_device(s, rho_d, t, r, true, ideal, false);
+      zero_bnd(r);
+    }
+    if (vs_d) ddot(vs_d, x_d, 0);
+    const dim3 eg((unsigned)std::max<long>(1, std::min<long>(div_up(N, kThreads), 8192)));
+    hipLaunchKernelGGL(k_eig_shift_scale, eg, dim3(kThreads), 0, s.stream, r, vs_d, sums.as<double>(),
+                       (change_sign ? -1.0 : 1.0) * scale, N);
+    HIPCHK(hipGetLastError());
+    ddot(x_d, r, 1);
+    ddot(x_d, x_d, 2);
+    hipLaunchKernelGGL(k_eig_grad, eg, dim3(kThreads), 0, s.stream, r, x_d, sums.as<double>(), df_d, N);
+    HIPCHK(hipGetLastError());
+    double sh[3];
+    HIPCHK(hipMemcpyAsync(sh, sums.p, sizeof(sh), hipMemcpyDeviceToHost, s.stream));
+    if (!dev) HIPCHK(hipMemcpyAsync(df, df_d, bytes, hipMemcpyDeviceToHost, s.stream));
+    const bool last = s.last_with_dft, last_fm = s.last_fixed_mass;
+    s.last_fixed_mass = false;
+    const int rc = fetch_parts(&s, nullptr);  // waits; reports eta > 1
+    s.last_with_dft = last;
+    s.last_fixed_mass = last_fm;
+    if (rc != CDFT_OK) return rc;
+    *f = sh[1] / sh[2];
+    return (int)CDFT_OK;
+  });
+}
+
 int cdft_system_eval_async(cdft_system sys, const double* rho_d, double* dF_d, int flags) {
   return guarded([&] {
     REQUIRE(sys && rho_d && dF_d, "null argument");

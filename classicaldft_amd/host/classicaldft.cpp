@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <cstdio>
+#include <random>
 #include <fstream>
 #include <limits>
 #include <sstream>
@@ -599,6 +601,119 @@ void DFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param) const {
   vw[0].set(v);
   matrix_dot_v(vw, rw, param);
   result.set(rw[0]);
+}
+
+void Eigenvalues::set_eigen_vec_(const std::vector<double>& v_in) {
+  if (eigen_vec_.size() != (long)v_in.size()) eigen_vec_.zeros((long)v_in.size());
+  for (long i = 0; i < (long)v_in.size(); ++i) eigen_vec_.set(i, v_in[i]);
+}
+
+double Eigenvalues::calculate_gradients(DFT_Vec& df) {
+  const Species* s = matrix_.species_.front();
+  const long N = s->getDensity().Ntot();
+  if (df.size() != N) df.zeros(N);
+  const int flags = (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0);
+  double f = 0.0;
+  check(cdft_system_eigen_objective(matrix_.system(), s->getDensity().get_density_pointer(), eigen_vec_.memptr(),
+                                    vshift_.size() == N ? vshift_.memptr() : nullptr, scale_, change_sign_ ? 1 : 0,
+                                    flags, &f, df.memptr()));
+  ++num_eval_;
+  if (verbose_) std::printf("\r\tEvaluation %d gives estimated eigenvalue %g   ", num_eval_, (change_sign_ ? -1 : 1) * f / scale_);
+  return f;
+}
+
+void Eigenvalues::calculate_eigenvector() {
+  const Species* sp = matrix_.species_.front();
+  const long N = sp->getDensity().Ntot();
+  if (eigen_vec_.size() != N) {  // src/Eigenvalues.cpp:78-88
+    eigen_vec_.zeros(N);
+    std::mt19937_64 rng(seed_);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (long i = 0; i < N; ++i) eigen_vec_.set(i, nd(rng));
+    if (sp->is_fixed_boundary()) {
+      const Lattice& L = sp->getLattice();
+      for (long i = 0; i < N; ++i) {
+        const long ix = i / (L.Ny() * L.Nz()), iy = (i / L.Nz()) % L.Ny(), iz = i % L.Nz();
+        if (ix == 0 || iy == 0 || iz == 0) eigen_vec_.set(i, 0.0);
+      }
+    }
+  }
+  // limited-memory BFGS (8 pairs) with a backtracking Armijo line search; stops when the relative change of f is
+  // at most tol_ (nlopt's ftol_rel)
+  const int m = 8;
+  std::vector<DFT_Vec> S, Y;
+  std::vector<double> rho_h;
+  DFT_Vec x, g(N), xn, gn(N), d(N);
+  x.set(eigen_vec_);
+  double f = calculate_gradients(g);
+  return_code_ = 5;
+  while (num_eval_ < max_eval_) {
+    d.set(g);
+    d.MultBy(-1.0);
+    std::vector<double> alpha(S.size());
+    for (int i = (int)S.size() - 1; i >= 0; --i) {
+      alpha[i] = rho_h[i] * S[i].dotWith(d);
+      d.IncrementBy_Scaled_Vector(Y[i], -alpha[i]);
+    }
+    if (!S.empty()) d.MultBy(S.back().dotWith(Y.back()) / Y.back().dotWith(Y.back()));
+    for (size_t i = 0; i < S.size(); ++i) {
+      const double beta = rho_h[i] * Y[i].dotWith(d);
+      d.IncrementBy_Scaled_Vector(S[i], alpha[i] - beta);
+    }
+    double gd = g.dotWith(d);
+    if (!(gd < 0)) {  // not a descent direction: restart along -g
+      S.clear();
+      Y.clear();
+      rho_h.clear();
+      d.set(g);
+      d.MultBy(-1.0);
+      gd = g.dotWith(d);
+    }
+    double step = S.empty() ? 1.0 / std::max(1e-300, g.euclidean_norm()) * x.euclidean_norm() : 1.0;
+    double fn = f;
+    bool moved = false;
+    for (int ls = 0; ls < 60 && num_eval_ < max_eval_; ++ls) {
+      xn.set(x);
+      xn.IncrementBy_Scaled_Vector(d, step);
+      eigen_vec_.set(xn);
+      fn = calculate_gradients(gn);
+      if (fn <= f + 1e-4 * step * gd) {
+        moved = true;
+        break;
+      }
+      step *= 0.5;
+    }
+    if (!moved) break;
+    DFT_Vec sv, yv;
+    sv.set(xn);
+    sv.IncrementBy_Scaled_Vector(x, -1.0);
+    yv.set(gn);
+    yv.IncrementBy_Scaled_Vector(g, -1.0);
+    const double sy = sv.dotWith(yv);
+    if (sy > 0) {
+      if ((int)S.size() == m) {
+        S.erase(S.begin());
+        Y.erase(Y.begin());
+        rho_h.erase(rho_h.begin());
+      }
+      S.push_back(sv);
+      Y.push_back(yv);
+      rho_h.push_back(1.0 / sy);
+    }
+    const double df_rel = std::fabs(fn - f);
+    x.set(xn);
+    g.set(gn);
+    f = fn;
+    if (df_rel <= tol_ * std::fabs(f)) {
+      return_code_ = 3;
+      break;
+    }
+  }
+  eigen_vec_.set(x);
+  eigen_val_ = f / scale_;
+  if (change_sign_) eigen_val_ *= -1;
+  eigen_vec_.normalise();
+  if (verbose_) std::printf("\n");
 }
 
 double Log_Det::calculate_log_det(long seed, int num_samples, bool has_zero_eigenvalue, double& variance) {

@@ -16,6 +16,7 @@
 //   fire_dir <F>            5 fireMinimizer2 steps with Minimizer::set_fixed_direction (ideal + WB I, mu = 0.4)
 //   scaled_mf <a> <w0> <mu>  Interaction_Base::scale_interaction(0.5) on LJ/QF: a_vdw, getW(0), mu_times_beta(0.5)
 //   log_det <ld> <var>       Log_Det(dft, 20, 0.05, 5e-4).calculate_log_det(7, 2, false) (16^3, ideal + WB I)
+//   eigen <min> <max> <n1> <n2>  Eigenvalues: smallest and (change_sign) largest Hessian eigenvalue (8^3, ideal + WB I)
 //   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
 //   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
 //                           a_vdw of a first object (kernel generated, cache written) and of a second one (cache
@@ -116,6 +117,23 @@ int main() {
     double var = 0.0;
     const double ldv = ldet.calculate_log_det(7, 2, false, var);
     std::printf("log_det %.17g %.17g\n", ldv, var);
+  }
+  {
+    double L5[3] = {1.6, 1.6, 1.6};
+    Density d5(0.2, L5);
+    for (long i = 0; i < d5.Ntot(); ++i) d5.set(i, 0.4 * (1.0 + 0.1 * (2.0 * splitmix_u(47, i) - 1.0)));
+    FMT_Species s14(d5, 1.0, 0.0);
+    WhiteBearI fmt14;
+    DFT dft14(&s14);
+    dft14.addHardCoreContribution(&fmt14);
+    Eigenvalues lo(dft14), hi(dft14);
+    lo.set_tolerence(1e-13);
+    hi.set_tolerence(1e-13);
+    hi.set_change_sign(true);
+    lo.calculate_eigenvector();
+    hi.calculate_eigenvector();
+    std::printf("eigen %.17g %.17g %d %d\n", lo.get_eigenvalue(), hi.get_eigenvalue(), lo.get_num_eval(),
+                hi.get_num_eval());
   }
   std::vector<DFT_Vec> vv(1), d2F(1);
   vv[0].set(v);

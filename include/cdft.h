@@ -165,6 +165,14 @@ int cdft_system_hessian_dot(cdft_system sys, const double* rho, const double* v,
    part only; add into out with CDFT_ACCUMULATE as the reference does. */
 int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
 
+/* Replaces Eigenvalues::calculate_gradients (include/Eigenvalues.h:44, src/Eigenvalues.cpp:46-70) with this
+   system's DFT as the matrix: r = A x (Dynamical_Matrix::matrix_dot_v1: boundary entries of input and output zeroed
+   with a fixed background; twice with CDFT_SQUARED_MATRIX), r += vshift (vshift . x) / 2 when vshift is not null,
+   r *= (change_sign ? -1 : 1) scale; *f = x.r / x.x (the scaled Rayleigh quotient) and df = 2 (r - f x) / x.x.
+   Flags: CDFT_DEVICE_PTRS (rho, x, vshift, df), CDFT_NO_IDEAL (full_hessian_ false), CDFT_SQUARED_MATRIX. */
+int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double* x, const double* vshift,
+                                double scale, int change_sign, int flags, double* f, double* df);
+
 /* Replaces Log_Det(matrix, order, lam_max, lam_min).calculate_log_det(seed, num_samples, has_zero_eigenvalue,
    variance) (include/Log_Det.h:22-28, src/Log_Det.cpp:22-166) with this system's DFT as the matrix: the
    stochastic Chebyshev estimate of log det of the Hessian at rho (Han, Malioutov, Avron and Shin 2017).  Probes are

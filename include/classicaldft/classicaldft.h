@@ -43,6 +43,7 @@ class DFT_Vec {
   double max() const;
   double* memptr() { return d_.data(); }
   const double* memptr() const { return d_.data(); }
+  void normalise() { MultBy(1.0 / euclidean_norm()); }
 
  private:
   std::vector<double> d_;
@@ -464,6 +465,45 @@ class DFT {
   bool full_hessian_ = true, use_squared_matrix_ = false;
   void invalidate() const;
   friend class Log_Det;
+  friend class Eigenvalues;
+};
+
+// include/Eigenvalues.h:16-59, src/Eigenvalues.cpp: the smallest eigenvalue of the DFT's Hessian (the largest with
+// set_change_sign) by minimising the scaled Rayleigh quotient, whose value and gradient come from the GPU
+// (cdft_system_eigen_objective).  The reference minimises with nlopt's LD_LBFGS (absent here); this uses a
+// limited-memory BFGS of its own with the same relative tolerance on f: the iterates differ, the converged eigenpair
+// agrees.  get_return_code: 3 = converged on the f tolerance, 5 = evaluation limit (nlopt's FTOL_REACHED /
+// MAXEVAL_REACHED).  calculate_eigenvector starts from get_eigen_vec() when it has Ntot entries, else from a
+// seeded normal vector (set_seed) that is zero on a fixed boundary.
+class Eigenvalues {
+ public:
+  Eigenvalues(const DFT& matrix, bool verbose = false) : matrix_(matrix), verbose_(verbose) {}
+  void set_scale(double d) { scale_ = d; }
+  void set_tolerence(double d) { tol_ = d; }
+  void set_change_sign(bool b) { change_sign_ = b; }
+  void set_vshift(DFT_Vec& v) { vshift_ = v; }
+  void set_seed(unsigned long s) { seed_ = s; }
+  void set_max_evaluations(int n) { max_eval_ = n; }
+  double get_scale() const { return scale_; }
+  double get_tolerence() const { return tol_; }
+  double get_eigenvalue() const { return eigen_val_; }
+  bool get_change_sign() const { return change_sign_; }
+  int get_num_eval() const { return num_eval_; }
+  int get_return_code() const { return return_code_; }
+  const DFT_Vec& get_eigen_vec() const { return eigen_vec_; }
+  double get_eigen_val() const { return eigen_val_; }
+  void calculate_eigenvector();
+  void set_eigen_vec_(const std::vector<double>& v_in);
+  double calculate_gradients(DFT_Vec& df);
+  void clear() { eigen_vec_.zeros(1); }
+
+ private:
+  const DFT& matrix_;
+  DFT_Vec vshift_, eigen_vec_;
+  double scale_ = 1e6, tol_ = 1e-6, eigen_val_ = 0;
+  bool change_sign_ = false, verbose_ = false;
+  int num_eval_ = 0, return_code_ = 0, max_eval_ = 100000;
+  unsigned long seed_ = 1;
 };
 
 // include/Log_Det.h:22-51, src/Log_Det.cpp:22-166: stochastic Chebyshev estimate of log det of the DFT's

@@ -76,6 +76,8 @@ def lib():
         L.orc_rademacher.argtypes = [ctypes.c_long, ctypes.c_long, _i, _p]
         L.orc_dft_log_det.argtypes = [_p, _p, _i, _d, _d, ctypes.c_long, _i, _i, _i, _i, _p, _p]
         L.orc_dft_log_det.restype = _i
+        L.orc_dft_eig_objective.argtypes = [_p, _p, _p, _p, _d, _i, _i, _i, _p]
+        L.orc_dft_eig_objective.restype = _d
         L.orc_dft_set_constraints.argtypes = [_p, _d, _i]
         L.orc_dft_set_constraints.restype = _i
         L.orc_dft_lagrange.argtypes = [_p]
@@ -330,6 +332,16 @@ class DFT:
         if lib().orc_dft_hess(self.h, _ptr(rho), _ptr(v), _ptr(out), int(bool(full))) == 2:
             raise ValueError("Input vector v must have zero boundary entries when the species has fixed boundaries")
         return out
+
+    def eig_objective(self, rho, x, vshift=None, scale=1.0, change_sign=False, full=True, squared=False):
+        """(f, df) of Eigenvalues::calculate_gradients (src/Eigenvalues.cpp:46-70) on this DFT's Hessian at rho."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        vs = None if vshift is None else np.ascontiguousarray(vshift, dtype=np.float64)
+        df = np.zeros_like(x)
+        f = lib().orc_dft_eig_objective(self.h, _ptr(rho), _ptr(x), None if vs is None else _ptr(vs), float(scale),
+                                        int(bool(change_sign)), int(bool(full)), int(bool(squared)), _ptr(df))
+        return f, df
 
     def log_det(self, rho, order, lam_max, lam_min, seed, num_samples, has_zero=False, full=True, squared=False):
         """(log det, variance): Log_Det(matrix, order, lam_max, lam_min).calculate_log_det(seed, num_samples,

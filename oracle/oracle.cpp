@@ -1728,6 +1728,46 @@ int orc_dft_log_det(orc_dft* h, const double* rho, int order, double lam_max, do
   return 0;
 }
 
+// Eigenvalues::calculate_gradients (src/Eigenvalues.cpp:46-70) on this DFT's Hessian: r = A x through
+// Dynamical_Matrix::matrix_dot_v1 (boundary entries of input and output zeroed with a fixed background, twice when
+// squared), r += vshift (0.5 vshift.x) if vshift, r *= (change_sign ? -1 : 1) scale; f = x.r / x.x,
+// df = (r - f x) 2 / x.x.  Returns f.
+double orc_dft_eig_objective(orc_dft* h, const double* rho, const double* x, const double* vshift, double scale,
+                             int change_sign, int full, int squared, double* df) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  const Grid& g = s->fmt.g;
+  const long N = g.N();
+  const bool bnd = s->bnd & 1;
+  std::vector<double> t(x, x + N), r(N);
+  auto zero_bnd = [&](std::vector<double>& v) {
+    if (bnd)
+      for (long p = 0; p < N; ++p)
+        if (boundary_point(g, p)) v[p] = 0.0;
+  };
+  zero_bnd(t);
+  orc_dft_hess(h, rho, t.data(), r.data(), full);
+  if (squared) {
+    t = r;
+    orc_dft_hess(h, rho, t.data(), r.data(), full);
+  }
+  zero_bnd(r);
+  if (vshift) {
+    double vx = 0.0;
+    for (long p = 0; p < N; ++p) vx += vshift[p] * x[p];
+    for (long p = 0; p < N; ++p) r[p] += 0.5 * vx * vshift[p];
+  }
+  const double sg = (change_sign ? -1.0 : 1.0) * scale;
+  double x2 = 0.0, xr = 0.0;
+  for (long p = 0; p < N; ++p) {
+    r[p] *= sg;
+    x2 += x[p] * x[p];
+  }
+  for (long p = 0; p < N; ++p) xr += x[p] * r[p];
+  const double f = xr / x2;
+  for (long p = 0; p < N; ++p) df[p] = (r[p] - f * x[p]) * (2.0 / x2);
+  return f;
+}
+
 // parts[5] = F, F_id, F_hs, F_mf, F_ext.  Returns 1 on eta too large.
 // Species constraints; returns 1 for the combination the reference rejects (src/Species.cpp:75-76).
 int orc_dft_set_constraints(orc_dft* h, double fixed_mass, int bnd) {

@@ -224,3 +224,16 @@ def test_log_det_through_cpp_api(demo):
     assert rel(demo["log_det"][0], ld) <= 1e-10
     q = ld + rho.size * np.log(1.0 / (0.05 + 5e-4))  # the mean of v.u over the probes
     assert abs(demo["log_det"][1] ** 2 - var ** 2) <= 1e-10 * q * q
+
+
+def test_eigenvalues_through_cpp_api(demo):
+    """Eigenvalues::calculate_eigenvector (src/Eigenvalues.cpp:72-99): the smallest and, with change_sign, the largest
+    eigenvalue of the 8^3 Hessian against numpy's eigvalsh of the dense matrix built from oracle products."""
+    shape, dx = (8, 8, 8), 0.2
+    rho = seeded(shape, 0.4, 47, 0.1)
+    o = oracle.DFT(shape, dx, 1.0, accurate_weights=True)
+    N = rho.size
+    H = np.stack([o.hess(rho, np.eye(N)[j].reshape(shape)).ravel() for j in range(N)], axis=1)
+    ev = np.linalg.eigvalsh(0.5 * (H + H.T))
+    lo, hi = demo["eigen"][:2]
+    assert rel(lo, ev[0]) <= 1e-6 and rel(hi, ev[-1]) <= 1e-6

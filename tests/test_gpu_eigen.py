@@ -1,0 +1,59 @@
+"""Eigenvalues on the device (include/Eigenvalues.h:16-59, src/Eigenvalues.cpp:46-99): the objective and gradient
+of Eigenvalues::calculate_gradients against the oracle to 1e-11 (periodic and fixed background, vshift, change_sign,
+full_hessian_ false, squared matrix), and the converged smallest / largest eigenvalue against numpy's eigvalsh of
+the dense Hessian.  The oracle's objective is pinned in tests/test_oracle_eigen.py."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def density(shape, seed, rho0=0.4, amp=0.1):
+    u = oracle.splitmix64_uniform(int(np.prod(shape)), seed).reshape(shape)
+    return rho0 * (1 + amp * (2 * u - 1))
+
+
+@pytest.mark.parametrize("bnd,vs,sign,squared,ideal", [(False, False, False, False, True),
+                                                        (False, True, True, False, True),
+                                                        (True, True, False, False, True),
+                                                        (False, False, True, True, False),
+                                                        (True, False, False, True, True)])
+def test_objective_matches_oracle(cdft, bnd, vs, sign, squared, ideal):
+    shape, dx = (16, 12, 24), 0.1
+    rho = density(shape, 0xE16)
+    w = oracle.fmt_weights(shape, dx, 1.0, mode=oracle.WEIGHTS_ACCURATE)
+    o = oracle.DFT(shape, dx, 1.0, model=oracle.WBI, mf=True, kT=2.0, accurate_weights=True)
+    if bnd:
+        o.set_constraints(fixed_background=True)
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mean_field=True, kT=2.0,
+                    fixed_background=bnd)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(shape)
+    v = rng.standard_normal(shape) if vs else None
+    f0, g0 = o.eig_objective(rho, x, v, 1e6, sign, ideal, squared)
+    f1, g1 = s.eigen_objective(rho, x, v, 1e6, sign, ideal, squared)
+    assert abs(f1 - f0) <= 1e-11 * abs(f0)
+    assert np.max(np.abs(g1 - g0)) <= 1e-11 * np.max(np.abs(g0))
+
+
+def test_smallest_and_largest_eigenvalue(cdft):
+    shape, dx = (8, 8, 8), 0.2
+    rho = density(shape, 0xE17)
+    o = oracle.DFT(shape, dx, 1.0, model=oracle.WBI, accurate_weights=True)
+    N = rho.size
+    H = np.stack([o.hess(rho, np.eye(N)[j].reshape(shape)).ravel() for j in range(N)], axis=1)
+    ev = np.linalg.eigvalsh(0.5 * (H + H.T))
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I,
+                    weights=oracle.fmt_weights(shape, dx, 1.0, mode=oracle.WEIGHTS_ACCURATE), ideal=True)
+    lo = cdft.Eigenvalues(s, rho)
+    lo.set_tolerence(1e-13)
+    hi = cdft.Eigenvalues(s, rho)
+    hi.set_tolerence(1e-13)
+    hi.set_change_sign(True)
+    e_lo, e_hi = lo.calculate_eigenvector(), hi.calculate_eigenvector()
+    print("eigen", e_lo, ev[0], e_hi, ev[-1], lo.num_eval, hi.num_eval)
+    assert abs(e_lo - ev[0]) <= 1e-6 * abs(ev[0]) and abs(e_hi - ev[-1]) <= 1e-6 * abs(ev[-1])
+    Hv = s.hessian_dot(rho, lo.eigen_vec.reshape(shape)).ravel()
+    assert np.linalg.norm(Hv - e_lo * lo.eigen_vec) <= 1e-2 * abs(e_lo)
