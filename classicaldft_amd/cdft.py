@@ -10,7 +10,7 @@ CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT = 0, 1, 2
 CDFT_BAD_ARG, CDFT_HIP_ERROR, CDFT_NOT_SUPPORTED = -1, -2, -3
 MODEL_NONE, ROSENFELD, RSLT, ESFMT, WHITEBEAR_I, WHITEBEAR_II = -1, 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
-DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, SQUARED_MATRIX = 1, 2, 4, 8, 16
+DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, SQUARED_MATRIX, DYNAMIC_MATRIX = 1, 2, 4, 8, 16, 32
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -119,6 +119,7 @@ def lib():
         L.cdft_fire_batch_destroy.argtypes = [_p]
         L.cdft_fire_batch_destroy.restype = None
         L.cdft_system_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_system_dynamic_dot.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_system_eigen_objective.argtypes = [_p, _p, _p, _p, _d, _i, _i, ctypes.POINTER(_d), _p]
         L.cdft_system_log_det.argtypes = [_p, _p, _i, _d, _d, ctypes.c_long, _i, _i, _i, ctypes.POINTER(_d),
                                           ctypes.POINTER(_d)]
@@ -302,7 +303,19 @@ class System:
         return res
 
 
-    def eigen_objective(self, rho, x, vshift=None, scale=1.0, change_sign=False, ideal=True, squared=False):
+    def dynamic_dot(self, rho, v, ideal=True, squared=False):
+        """DDFT::matrix_dot_v_intern (src/DDFT.cpp:353-361): g . (H v), g the DDFT mobility operator."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if rho.size != self.N or v.size != self.N:
+            raise ValueError(f"expected {self.N} densities and components")
+        out = np.zeros(rho.shape)
+        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0)
+        _check(lib().cdft_system_dynamic_dot(self.h, _ptr(rho), _ptr(v), _ptr(out), fl))
+        return out
+
+    def eigen_objective(self, rho, x, vshift=None, scale=1.0, change_sign=False, ideal=True, squared=False,
+                        dynamic=False):
         """(f, df) of Eigenvalues::calculate_gradients (src/Eigenvalues.cpp:46-70): the scaled Rayleigh quotient
         of the Hessian at rho (plus vshift (vshift.x)/2) and its gradient; ideal = full_hessian_."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
@@ -312,14 +325,14 @@ class System:
         vs = None if vshift is None else np.ascontiguousarray(vshift, dtype=np.float64)
         df = np.zeros(x.shape)
         f = _d()
-        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0)
+        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0) | (DYNAMIC_MATRIX if dynamic else 0)
         _check(lib().cdft_system_eigen_objective(self.h, _ptr(rho), _ptr(x), None if vs is None else _ptr(vs),
                                                  float(scale), int(bool(change_sign)), fl, ctypes.byref(f),
                                                  _ptr(df)))
         return f.value, df
 
     def log_det(self, rho, order, lam_max, lam_min, seed, num_samples, has_zero_eigenvalue=False, ideal=True,
-                squared=False):
+                squared=False, dynamic=False):
         """(log det, variance): Log_Det(dft, order, lam_max, lam_min).calculate_log_det(seed, num_samples,
         has_zero_eigenvalue, variance) (src/Log_Det.cpp:22-166); ideal = full_hessian_, squared =
         use_squared_matrix_."""
@@ -327,7 +340,7 @@ class System:
         if rho.size != self.N:
             raise ValueError(f"expected {self.N} densities")
         ld, var = _d(), _d()
-        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0)
+        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0) | (DYNAMIC_MATRIX if dynamic else 0)
         _check(lib().cdft_system_log_det(self.h, _ptr(rho), int(order), float(lam_max), float(lam_min), int(seed),
                                          int(num_samples), int(bool(has_zero_eigenvalue)), fl, ctypes.byref(ld),
                                          ctypes.byref(var)))
@@ -816,9 +829,9 @@ class Eigenvalues:
     this uses scipy's L-BFGS-B with the same relative tolerance on f, so the iterates differ and the converged
     eigenpair agrees."""
 
-    def __init__(self, system, rho, ideal=True, squared=False, verbose=False):
+    def __init__(self, system, rho, ideal=True, squared=False, verbose=False, dynamic=False):
         self.system, self.rho = system, np.ascontiguousarray(rho, dtype=np.float64)
-        self.ideal, self.squared, self.verbose = ideal, squared, verbose
+        self.ideal, self.squared, self.verbose, self.dynamic = ideal, squared, verbose, dynamic
         self.scale, self.tol, self.change_sign, self.vshift = 1e6, 1e-6, False, None
         self.eigen_vec, self.eigen_val, self.num_eval, self.return_code = None, 0.0, 0, 0
 
@@ -838,7 +851,7 @@ class Eigenvalues:
         """(f, df) at x (src/Eigenvalues.cpp:46-70)."""
         self.num_eval += 1
         return self.system.eigen_objective(self.rho, x, self.vshift, self.scale, self.change_sign, self.ideal,
-                                           self.squared)
+                                           self.squared, self.dynamic)
 
     def calculate_eigenvector(self, x0=None, seed=0, maxiter=10000):
         """src/Eigenvalues.cpp:72-99: start from x0 (else a seeded normal vector, zero on a fixed boundary)."""

@@ -1097,18 +1097,87 @@ int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, do
 }
 
 namespace {
+// gx = g . x: DDFT::g_dot_x (src/DDFT.cpp:320-351, forward differences with the density averaged over each link,
+// D = 1 / (2 dx^2) per axis, periodic neighbours via get_neighbors :294-312); zero on the boundary ix, iy or
+// iz = 0 of a fixed background.
+__global__ __launch_bounds__(256) void k_gdotx(const double* __restrict__ rho, const double* __restrict__ x,
+                                               double* __restrict__ out, int Nx, int Ny, int Nz, double Dx, double Dy,
+                                               double Dz, int fixed_bnd) {
+  const long N = (long)Nx * Ny * Nz;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
+    const int z = p % Nz, y = (p / Nz) % Ny, ix = p / ((long)Nz * Ny);
+    const long px = ((long)((ix + 1) % Nx) * Ny + y) * Nz + z, mx = ((long)((ix + Nx - 1) % Nx) * Ny + y) * Nz + z;
+    const long py = ((long)ix * Ny + (y + 1) % Ny) * Nz + z, my = ((long)ix * Ny + (y + Ny - 1) % Ny) * Nz + z;
+    const long pz = ((long)ix * Ny + y) * Nz + (z + 1) % Nz, mz = ((long)ix * Ny + y) * Nz + (z + Nz - 1) % Nz;
+    const double x0 = x[p], d0 = rho[p];
+    double g = Dx * ((rho[px] + d0) * (x[px] - x0) - (d0 + rho[mx]) * (x0 - x[mx])) +
+               Dy * ((rho[py] + d0) * (x[py] - x0) - (d0 + rho[my]) * (x0 - x[my])) +
+               Dz * ((rho[pz] + d0) * (x[pz] - x0) - (d0 + rho[mz]) * (x0 - x[mz]));
+    if (fixed_bnd && (ix == 0 || y == 0 || z == 0)) g = 0.0;
+    out[p] = g;
+  }
+}
+__global__ __launch_bounds__(256) void k_negate(double* __restrict__ a, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = -a[i];
+}
+
+// r = A x as Dynamical_Matrix::matrix_dot_v1 applies it (src/Dynamical_Matrix.cpp:34-70): the input's boundary
+// entries zeroed on a copy (fixed background), the product, the output's boundary entries zeroed.  A is the DFT
+// Hessian (DFT::matrix_dot_v_intern, src/DFT.cpp:435-488; CDFT_NO_IDEAL = full_hessian_ false) or, with
+// CDFT_DYNAMIC_MATRIX, the DDFT dynamical matrix g . (H x) (DDFT::matrix_dot_v_intern, src/DDFT.cpp:353-361);
+// CDFT_SQUARED_MATRIX applies it twice, negated for the dynamical matrix (is_dynamic).  t: scratch of N values.
+void matrix_apply(cdft_system_s& s, const double* rho, const double* x, double* r, double* t, int flags) {
+  const long N = s.N;
+  const size_t bytes = sizeof(double) * N;
+  const bool bnd = s.d.boundary & CDFT_FIXED_BACKGROUND, ideal = !(flags & CDFT_NO_IDEAL);
+  const bool dynamic = flags & CDFT_DYNAMIC_MATRIX, squared = flags & CDFT_SQUARED_MATRIX;
+  const long nbnd = boundary_count(s.g.Nx, s.g.Ny, s.g.Nz);
+  const dim3 eg((unsigned)std::max<long>(1, std::min<long>(div_up(N, kThreads), 8192)));
+  auto zero_bnd = [&](double* v) {
+    if (!bnd) return;
+    hipLaunchKernelGGL(k_cons_bset, cons_grid(s, nbnd), dim3(kThreads), 0, s.stream, v, nbnd, N, s.g.Nx, s.g.Ny,
+                       s.g.Nz, nullptr, 8L, 0.0);
+    HIPCHK(hipGetLastError());
+  };
+  auto once = [&]() {  // t -> r
+    hess_device(s, rho, t, r, true, ideal, false);
+    zero_bnd(r);
+    if (dynamic) {
+      HIPCHK(hipMemcpyAsync(t, r, bytes, hipMemcpyDeviceToDevice, s.stream));
+      hipLaunchKernelGGL(k_gdotx, eg, dim3(kThreads), 0, s.stream, rho, t, r, s.g.Nx, s.g.Ny, s.g.Nz,
+                         1.0 / (2 * s.d.dx * s.d.dx), 1.0 / (2 * s.d.dy * s.d.dy), 1.0 / (2 * s.d.dz * s.d.dz),
+                         bnd ? 1 : 0);
+      HIPCHK(hipGetLastError());
+    }
+  };
+  HIPCHK(hipMemcpyAsync(t, x, bytes, hipMemcpyDeviceToDevice, s.stream));
+  zero_bnd(t);
+  once();
+  if (squared) {
+    HIPCHK(hipMemcpyAsync(t, r, bytes, hipMemcpyDeviceToDevice, s.stream));
+    once();
+    if (dynamic) {
+      hipLaunchKernelGGL(k_negate, eg, dim3(kThreads), 0, s.stream, r, N);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  zero_bnd(r);
+}
+}  // namespace
+
+namespace {
 // One term of the Chebyshev recursion of Log_Det::calculate_log_det (src/Log_Det.cpp:122-153):
-// x = (((r + z) scale) cA) + cB w1 - w0 with z = zf * sum (the zero-mode shift lam_mid accu(w1) / Ntot, sum on
+// x = (((rs r + z) scale) cA) + cB w1 - w0 (rs = -1 negates the product of a dynamical matrix) with z = zf * sum (the zero-mode shift lam_mid accu(w1) / Ntot, sum on
 // the device; zf = 0 without it); w2 = x; u = c0 w1 + ck x on the first term (w1 = the probe, no w0), else
 // u += ck x.
 __global__ __launch_bounds__(256) void k_cheb_step(const double* __restrict__ r, const double* __restrict__ w1,
                                                    const double* __restrict__ w0, double* __restrict__ w2,
                                                    double* __restrict__ u, long n, const double* __restrict__ sum,
                                                    double zf, double scale, double cA, double cB, double c0,
-                                                   double ck, int first) {
+                                                   double ck, int first, double rs) {
   const double z = zf != 0.0 ? zf * sum[0] : 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    double x = r[i] + z;
+    double x = rs * r[i] + z;
     x *= scale;
     x *= cA;
     x += cB * w1[i];
@@ -1131,7 +1200,6 @@ int cdft_system_log_det(cdft_system sys, const double* rho, int order, double la
     const size_t bytes = sizeof(double) * N;
     const bool bnd = s.d.boundary & CDFT_FIXED_BACKGROUND;
     const long nbnd = boundary_count(s.g.Nx, s.g.Ny, s.g.Nz);
-    const bool ideal = !(flags & CDFT_NO_IDEAL), squared = flags & CDFT_SQUARED_MATRIX;
     // Log_Det::Log_Det and get_coefficients (src/Log_Det.cpp:22-62)
     const double scale = 1.0 / (lam_min + lam_max), a = lam_min * scale, b = lam_max * scale;
     const double lam_mid = (lam_max + lam_min) / 2;
@@ -1178,19 +1246,10 @@ int cdft_system_log_det(cdft_system sys, const double* rho, int order, double la
                          s.g.Nz, nullptr, 8L, 0.0);
       HIPCHK(hipGetLastError());
     };
-    // Log_Det::matrix_dot_v1 (src/Log_Det.cpp:73-92) through Dynamical_Matrix::matrix_dot_v
-    // (src/Dynamical_Matrix.cpp:34-50) and DFT::matrix_dot_v_intern (src/DFT.cpp:435-488): r = A x
-    auto matvec = [&](const double* x) {
-      HIPCHK(hipMemcpyAsync(t, x, bytes, hipMemcpyDeviceToDevice, s.stream));
-      zero_bnd(t);
-      hess_device(s, rho_d, t, r, true, ideal, false);
-      zero_bnd(r);
-      if (squared) {
-        HIPCHK(hipMemcpyAsync(t, r, bytes, hipMemcpyDeviceToDevice, s.stream));
-        hess_device(s, rho_d, t, r, true, ideal, false);
-        zero_bnd(r);
-      }
-    };
+    // Log_Det::matrix_dot_v1 (src/Log_Det.cpp:73-92): r = A x; a dynamical matrix's product is negated
+    // (src/Log_Det.cpp:124, 139)
+    auto matvec = [&](const double* x) { matrix_apply(s, rho_d, x, r, t, flags); };
+    const double rs = (flags & CDFT_DYNAMIC_MATRIX) ? -1.0 : 1.0;
     const dim3 eg((unsigned)std::max<long>(1, std::min<long>(div_up(N, kThreads), 8192)));
     const double zf = has_zero_eigenvalue ? lam_mid / (double)N : 0.0;
     if (seed <= 0) {
@@ -1208,14 +1267,14 @@ int cdft_system_log_det(cdft_system sys, const double* rho, int order, double la
       matvec(v);
       if (zf != 0.0) dsum(v, nullptr, 0);
       hipLaunchKernelGGL(k_cheb_step, eg, dim3(kThreads), 0, s.stream, r, v, nullptr, w1, u, N, sums.as<double>(),
-                         zf, scale, 2.0 / (b - a), -(b + a) / (b - a), c[0], c1, 1);
+                         zf, scale, 2.0 / (b - a), -(b + a) / (b - a), c[0], c1, 1, rs);
       HIPCHK(hipGetLastError());
       double *p0 = v, *p1 = w1, *p2 = w2, *spare = w0;
       for (int k = 2; k <= order; ++k) {
         matvec(p1);
         if (zf != 0.0) dsum(p1, nullptr, 0);
         hipLaunchKernelGGL(k_cheb_step, eg, dim3(kThreads), 0, s.stream, r, p1, p0, p2, u, N, sums.as<double>(), zf,
-                           scale, 4.0 / (b - a), -2 * (b + a) / (b - a), 0.0, c[k], 0);
+                           scale, 4.0 / (b - a), -2 * (b + a) / (b - a), 0.0, c[k], 0, rs);
         HIPCHK(hipGetLastError());
         // w0 <- w1, w1 <- w2; the probe v stays intact for the final dot product
         double* freed = p0 == v ? spare : p0;
@@ -1267,6 +1326,38 @@ __global__ __launch_bounds__(256) void k_eig_grad(const double* __restrict__ r, 
 }
 }  // namespace
 
+int cdft_system_dynamic_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags) {
+  return guarded([&] {
+    cdft_system_s& s = *sys;
+    REQUIRE(sys && rho && v && out, "null argument");
+    REQUIRE(s.nb == 1, "the dynamical matrix takes one system, not a batch");
+    const long N = s.N;
+    const size_t bytes = sizeof(double) * N;
+    const bool dev = flags & CDFT_DEVICE_PTRS;
+    DevBuf work;
+    work.alloc((dev ? 1 : 4) * bytes);
+    double* t = work.as<double>();
+    const double *rho_d = rho, *v_d = v;
+    double* out_d = out;
+    if (!dev) {
+      double* h = t + N;
+      HIPCHK(hipMemcpyAsync(h, rho, bytes, hipMemcpyHostToDevice, s.stream));
+      HIPCHK(hipMemcpyAsync(h + N, v, bytes, hipMemcpyHostToDevice, s.stream));
+      rho_d = h;
+      v_d = h + N;
+      out_d = h + 2 * N;
+    }
+    matrix_apply(s, rho_d, v_d, out_d, t, (flags & (CDFT_NO_IDEAL | CDFT_SQUARED_MATRIX)) | CDFT_DYNAMIC_MATRIX);
+    if (!dev) HIPCHK(hipMemcpyAsync(out, out_d, bytes, hipMemcpyDeviceToHost, s.stream));
+    const bool last = s.last_with_dft, last_fm = s.last_fixed_mass;
+    s.last_fixed_mass = false;
+    const int rc = fetch_parts(&s, nullptr);  // waits; reports eta > 1
+    s.last_with_dft = last;
+    s.last_fixed_mass = last_fm;
+    return rc;
+  });
+}
+
 int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double* x, const double* vshift,
                                 double scale, int change_sign, int flags, double* f, double* df) {
   return guarded([&] {
@@ -1275,9 +1366,7 @@ int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double
     REQUIRE(s.nb == 1, "the eigenvalue objective takes one system, not a batch");
     const long N = s.N;
     const size_t bytes = sizeof(double) * N;
-    const bool dev = flags & CDFT_DEVICE_PTRS, bnd = s.d.boundary & CDFT_FIXED_BACKGROUND;
-    const bool ideal = !(flags & CDFT_NO_IDEAL), squared = flags & CDFT_SQUARED_MATRIX;
-    const long nbnd = boundary_count(s.g.Nx, s.g.Ny, s.g.Nz);
+    const bool dev = flags & CDFT_DEVICE_PTRS;
     DevBuf work, sums, in;
     work.alloc(2 * bytes + (dev ? 0 : 3 * bytes));
     sums.alloc(3 * sizeof(double));
@@ -1306,22 +1395,7 @@ int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double
                          sums.as<double>() + slot, s.pslots, 1L);
       HIPCHK(hipGetLastError());
     };
-    auto zero_bnd = [&](double* v) {
-      if (!bnd) return;
-      hipLaunchKernelGGL(k_cons_bset, cons_grid(s, nbnd), dim3(kThreads), 0, s.stream, v, nbnd, N, s.g.Nx, s.g.Ny,
-                         s.g.Nz, nullptr, 8L, 0.0);
-      HIPCHK(hipGetLastError());
-    };
-    // Dynamical_Matrix::matrix_dot_v1 (src/Dynamical_Matrix.cpp:54-70): r = A x
-    HIPCHK(hipMemcpyAsync(t, x_d, bytes, hipMemcpyDeviceToDevice, s.stream));
-    zero_bnd(t);
-    hess_device(s, rho_d, t, r, true, ideal, false);
-    zero_bnd(r);
-    if (squared) {
-      HIPCHK(hipMemcpyAsync(t, r, bytes, hipMemcpyDeviceToDevice, s.stream));
-      hess_device(s, rho_d, t, r, true, ideal, false);
-      zero_bnd(r);
-    }
+    matrix_apply(s, rho_d, x_d, r, t, flags);  // Dynamical_Matrix::matrix_dot_v1: r = A x
     if (vs_d) ddot(vs_d, x_d, 0);
     const dim3 eg((unsigned)std::max<long>(1, std::min<long>(div_up(N, kThreads), 8192)));
     hipLaunchKernelGGL(k_eig_shift_scale, eg, dim3(kThreads), 0, s.stream, r, vs_d, sums.as<double>(),

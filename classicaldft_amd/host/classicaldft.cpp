@@ -603,6 +603,20 @@ void DFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param) const {
   result.set(rw[0]);
 }
 
+Log_Det::Log_Det(const DDFT& matrix, int order, double lam_max, double lam_min, bool verbose)
+    : matrix_(matrix.getDFT()), order_(order), lam_max_(lam_max), lam_min_(lam_min), verbose_(verbose),
+      dynamic_(true) {}
+
+Eigenvalues::Eigenvalues(const DDFT& matrix, bool verbose) : matrix_(matrix.getDFT()), verbose_(verbose),
+                                                              dynamic_(true) {}
+
+int Eigenvalues::flags() const {
+  if (dynamic_ && matrix_.use_squared_matrix_)
+    throw std::runtime_error("Eigenvalues: the DFT's use_squared_matrix_ inside a DDFT matrix is not supported");
+  return (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0) |
+         (dynamic_ ? CDFT_DYNAMIC_MATRIX : 0);
+}
+
 void Eigenvalues::set_eigen_vec_(const std::vector<double>& v_in) {
   if (eigen_vec_.size() != (long)v_in.size()) eigen_vec_.zeros((long)v_in.size());
   for (long i = 0; i < (long)v_in.size(); ++i) eigen_vec_.set(i, v_in[i]);
@@ -612,7 +626,7 @@ double Eigenvalues::calculate_gradients(DFT_Vec& df) {
   const Species* s = matrix_.species_.front();
   const long N = s->getDensity().Ntot();
   if (df.size() != N) df.zeros(N);
-  const int flags = (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0);
+  const int flags = this->flags();
   double f = 0.0;
   check(cdft_system_eigen_objective(matrix_.system(), s->getDensity().get_density_pointer(), eigen_vec_.memptr(),
                                     vshift_.size() == N ? vshift_.memptr() : nullptr, scale_, change_sign_ ? 1 : 0,
@@ -719,7 +733,10 @@ void Eigenvalues::calculate_eigenvector() {
 double Log_Det::calculate_log_det(long seed, int num_samples, bool has_zero_eigenvalue, double& variance) {
   const Species* s = matrix_.species_.front();
   double ld = 0.0;
-  const int flags = (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0);
+  if (dynamic_ && matrix_.use_squared_matrix_)
+    throw std::runtime_error("Log_Det: the DFT's use_squared_matrix_ inside a DDFT matrix is not supported");
+  const int flags = (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0) |
+                    (dynamic_ ? CDFT_DYNAMIC_MATRIX : 0);
   check(cdft_system_log_det(matrix_.system(), s->getDensity().get_density_pointer(), order_, lam_max_, lam_min_, seed,
                             num_samples, has_zero_eigenvalue ? 1 : 0, flags, &ld, &variance));
   if (verbose_) std::printf("\t samples = %d log_det = %g variance = %g\n", num_samples, ld, variance);

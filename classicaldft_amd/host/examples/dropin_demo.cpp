@@ -16,6 +16,7 @@
 //   fire_dir <F>            5 fireMinimizer2 steps with Minimizer::set_fixed_direction (ideal + WB I, mu = 0.4)
 //   scaled_mf <a> <w0> <mu>  Interaction_Base::scale_interaction(0.5) on LJ/QF: a_vdw, getW(0), mu_times_beta(0.5)
 //   log_det <ld> <var>       Log_Det(dft, 20, 0.05, 5e-4).calculate_log_det(7, 2, false) (16^3, ideal + WB I)
+//   log_det_ddft <ld> <var>  Log_Det(ddft, 12, 40, 5e-3).calculate_log_det(7, 2, true): the DDFT dynamical matrix
 //   eigen <min> <max> <n1> <n2>  Eigenvalues: smallest and (change_sign) largest Hessian eigenvalue (8^3, ideal + WB I)
 //   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
 //   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
@@ -117,6 +118,10 @@ int main() {
     double var = 0.0;
     const double ldv = ldet.calculate_log_det(7, 2, false, var);
     std::printf("log_det %.17g %.17g\n", ldv, var);
+    DDFT ddft13(&dft13);
+    Log_Det ldd(ddft13, 12, 40.0, 5e-3);
+    const double ldd_v = ldd.calculate_log_det(7, 2, true, var);
+    std::printf("log_det_ddft %.17g %.17g\n", ldd_v, var);
   }
   {
     double L5[3] = {1.6, 1.6, 1.6};

@@ -75,7 +75,10 @@ enum cdft_flags {
   CDFT_FMT_ONLY = 4,    /* hard-sphere part only (cdft_system_eval_async / _time) */
   CDFT_NO_IDEAL = 8     /* no ideal-gas term: DFT::calculateFreeEnergyAndDerivatives(onlyFex = true) for
                            cdft_system_eval; DFT::full_hessian_ false for the Hessian-vector product */,
-  CDFT_SQUARED_MATRIX = 16 /* cdft_system_log_det: apply the Hessian twice (use_squared_matrix_) */
+  CDFT_SQUARED_MATRIX = 16, /* log det / eigenvalue objective / dynamical matrix: apply the matrix twice
+                               (use_squared_matrix_; negated for the dynamical matrix, is_dynamic) */
+  CDFT_DYNAMIC_MATRIX = 32  /* log det / eigenvalue objective: the matrix is the DDFT dynamical matrix
+                               g . (H x) of cdft_system_dynamic_dot instead of the Hessian H */
 };
 
 /* Species boundary conditions (include/Species.h:24-25, applied in Species::endForceCalculation,
@@ -165,11 +168,19 @@ int cdft_system_hessian_dot(cdft_system sys, const double* rho, const double* v,
    part only; add into out with CDFT_ACCUMULATE as the reference does. */
 int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
 
+/* Replaces DDFT::matrix_dot_v_intern (include/Minimizer.h:211-220, src/DDFT.cpp:353-361) as
+   Dynamical_Matrix::matrix_dot_v1 applies it: out = g . (H v), H the Hessian at rho (cdft_system_hessian_dot) and
+   g . x the DDFT mobility operator DDFT::g_dot_x (src/DDFT.cpp:320-351: forward differences, the density averaged
+   over each link, periodic; zero on the boundary with a fixed background).  Flags: CDFT_DEVICE_PTRS, CDFT_NO_IDEAL,
+   CDFT_SQUARED_MATRIX (out = -(gH)(gH) v, is_dynamic).  FP64 single systems. */
+int cdft_system_dynamic_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
+
 /* Replaces Eigenvalues::calculate_gradients (include/Eigenvalues.h:44, src/Eigenvalues.cpp:46-70) with this
    system's DFT as the matrix: r = A x (Dynamical_Matrix::matrix_dot_v1: boundary entries of input and output zeroed
    with a fixed background; twice with CDFT_SQUARED_MATRIX), r += vshift (vshift . x) / 2 when vshift is not null,
    r *= (change_sign ? -1 : 1) scale; *f = x.r / x.x (the scaled Rayleigh quotient) and df = 2 (r - f x) / x.x.
-   Flags: CDFT_DEVICE_PTRS (rho, x, vshift, df), CDFT_NO_IDEAL (full_hessian_ false), CDFT_SQUARED_MATRIX. */
+   Flags: CDFT_DEVICE_PTRS (rho, x, vshift, df), CDFT_NO_IDEAL (full_hessian_ false), CDFT_SQUARED_MATRIX,
+   CDFT_DYNAMIC_MATRIX (the DDFT's matrix: Eigenvalues(ddft)). */
 int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double* x, const double* vshift,
                                 double scale, int change_sign, int flags, double* f, double* df);
 
@@ -179,7 +190,8 @@ int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double
    the reference's (std::mt19937(seed), uniform_int_distribution<>(0, 1) -> +-1, a random seed if seed <= 0); every
    Hessian-vector product runs on the device, as Log_Det::matrix_dot_v1 applies it (boundary entries of input and
    output zeroed with a fixed background).  Flags: CDFT_DEVICE_PTRS (rho), CDFT_NO_IDEAL (full_hessian_ false),
-   CDFT_SQUARED_MATRIX.  FP64 single systems; CDFT_ETA_TOO_LARGE if eta > 1. */
+   CDFT_SQUARED_MATRIX, CDFT_DYNAMIC_MATRIX (Log_Det(ddft): each product negated, is_dynamic).  FP64 single
+   systems; CDFT_ETA_TOO_LARGE if eta > 1. */
 int cdft_system_log_det(cdft_system sys, const double* rho, int order, double lam_max, double lam_min, long seed,
                         int num_samples, int has_zero_eigenvalue, int flags, double* log_det, double* variance);
 

@@ -468,6 +468,8 @@ class DFT {
   friend class Eigenvalues;
 };
 
+class DDFT;
+
 // include/Eigenvalues.h:16-59, src/Eigenvalues.cpp: the smallest eigenvalue of the DFT's Hessian (the largest with
 // set_change_sign) by minimising the scaled Rayleigh quotient, whose value and gradient come from the GPU
 // (cdft_system_eigen_objective).  The reference minimises with nlopt's LD_LBFGS (absent here); this uses a
@@ -478,6 +480,8 @@ class DFT {
 class Eigenvalues {
  public:
   Eigenvalues(const DFT& matrix, bool verbose = false) : matrix_(matrix), verbose_(verbose) {}
+  // the DDFT's dynamical matrix g . H (DDFT::matrix_dot_v_intern, src/DDFT.cpp:353-361) instead of the Hessian
+  explicit Eigenvalues(const DDFT& matrix, bool verbose = false);
   void set_scale(double d) { scale_ = d; }
   void set_tolerence(double d) { tol_ = d; }
   void set_change_sign(bool b) { change_sign_ = b; }
@@ -504,6 +508,8 @@ class Eigenvalues {
   bool change_sign_ = false, verbose_ = false;
   int num_eval_ = 0, return_code_ = 0, max_eval_ = 100000;
   unsigned long seed_ = 1;
+  bool dynamic_ = false;
+  int flags() const;
 };
 
 // include/Log_Det.h:22-51, src/Log_Det.cpp:22-166: stochastic Chebyshev estimate of log det of the DFT's
@@ -512,6 +518,8 @@ class Log_Det {
  public:
   Log_Det(const DFT& matrix, int order, double lam_max, double lam_min, bool verbose = false)
       : matrix_(matrix), order_(order), lam_max_(lam_max), lam_min_(lam_min), verbose_(verbose) {}
+  // the DDFT's dynamical matrix, each product negated (is_dynamic, src/Log_Det.cpp:124, 139)
+  Log_Det(const DDFT& matrix, int order, double lam_max, double lam_min, bool verbose = false);
   double calculate_log_det(long seed, int num_samples, bool has_zero_eigenvalue, double& variance);
   void set_verbose(bool b) { verbose_ = b; }
   void set_debug(bool b) { debug_ = b; }
@@ -520,7 +528,7 @@ class Log_Det {
   const DFT& matrix_;
   int order_;
   double lam_max_, lam_min_;
-  bool verbose_ = false, debug_ = false;
+  bool verbose_ = false, debug_ = false, dynamic_ = false;
 };
 
 // ------------------------------------------------------------- minimizer ---
@@ -528,6 +536,7 @@ class Log_Det {
 class Minimizer {
  public:
   explicit Minimizer(DFT* dft) : dft_(dft) {}
+  const DFT& getDFT() const { return *dft_; }
   virtual ~Minimizer() = default;
   void run(long maxSteps = -1);
   void resume(long maxSteps = -1);

@@ -78,6 +78,8 @@ def lib():
         L.orc_dft_log_det.restype = _i
         L.orc_dft_eig_objective.argtypes = [_p, _p, _p, _p, _d, _i, _i, _i, _p]
         L.orc_dft_eig_objective.restype = _d
+        L.orc_gdotx.argtypes = [_p, _p, _p, _p]
+        L.orc_dft_matrix_apply.argtypes = [_p, _p, _p, _p, _i, _i]
         L.orc_dft_set_constraints.argtypes = [_p, _d, _i]
         L.orc_dft_set_constraints.restype = _i
         L.orc_dft_lagrange.argtypes = [_p]
@@ -333,23 +335,43 @@ class DFT:
             raise ValueError("Input vector v must have zero boundary entries when the species has fixed boundaries")
         return out
 
-    def eig_objective(self, rho, x, vshift=None, scale=1.0, change_sign=False, full=True, squared=False):
+    def gdotx(self, rho, x):
+        """DDFT::g_dot_x (src/DDFT.cpp:320-351) at density rho."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(rho)
+        lib().orc_gdotx(self.h, _ptr(rho), _ptr(x), _ptr(out))
+        return out
+
+    def dynamic_dot(self, rho, v, full=True, squared=False):
+        """DDFT::matrix_dot_v_intern (src/DDFT.cpp:353-361) as Dynamical_Matrix::matrix_dot_v1 applies it."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros_like(rho)
+        lib().orc_dft_matrix_apply(self.h, _ptr(rho), _ptr(v), _ptr(out), int(bool(full)), 2 | int(bool(squared)))
+        return out
+
+    def eig_objective(self, rho, x, vshift=None, scale=1.0, change_sign=False, full=True, squared=False,
+                      dynamic=False):
         """(f, df) of Eigenvalues::calculate_gradients (src/Eigenvalues.cpp:46-70) on this DFT's Hessian at rho."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         x = np.ascontiguousarray(x, dtype=np.float64)
         vs = None if vshift is None else np.ascontiguousarray(vshift, dtype=np.float64)
         df = np.zeros_like(x)
         f = lib().orc_dft_eig_objective(self.h, _ptr(rho), _ptr(x), None if vs is None else _ptr(vs), float(scale),
-                                        int(bool(change_sign)), int(bool(full)), int(bool(squared)), _ptr(df))
+                                        int(bool(change_sign)), int(bool(full)),
+                                        int(bool(squared)) | (2 if dynamic else 0), _ptr(df))
         return f, df
 
-    def log_det(self, rho, order, lam_max, lam_min, seed, num_samples, has_zero=False, full=True, squared=False):
+    def log_det(self, rho, order, lam_max, lam_min, seed, num_samples, has_zero=False, full=True, squared=False,
+                dynamic=False):
         """(log det, variance): Log_Det(matrix, order, lam_max, lam_min).calculate_log_det(seed, num_samples,
         has_zero_eigenvalue, variance) (src/Log_Det.cpp:22-166) on this DFT's Hessian at rho."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         ld, var = ctypes.c_double(), ctypes.c_double()
         lib().orc_dft_log_det(self.h, _ptr(rho), int(order), float(lam_max), float(lam_min), int(seed),
-                              int(num_samples), int(bool(has_zero)), int(bool(full)), int(bool(squared)),
+                              int(num_samples), int(bool(has_zero)), int(bool(full)),
+                              int(bool(squared)) | (2 if dynamic else 0),
                               ctypes.byref(ld), ctypes.byref(var))
         return ld.value, var.value
 

@@ -1619,6 +1619,65 @@ int orc_dft_hess(orc_dft* h, const double* rho, const double* v, double* out, in
   return 0;
 }
 
+// g . x: DDFT::g_dot_x (src/DDFT.cpp:320-351) with get_neighbors (:294-312): forward differences, the density
+// averaged over each link, D = 1 / (2 dx^2) per axis, periodic; zero on the boundary with a fixed background.
+void orc_gdotx(orc_dft* h, const double* rho, const double* x, double* out) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  const Grid& g = s->fmt.g;
+  const double D[3] = {1 / (2 * s->fmt.dx * s->fmt.dx), 1 / (2 * s->fmt.dy * s->fmt.dy),
+                       1 / (2 * s->fmt.dz * s->fmt.dz)};
+  auto P = [&](int ix, int iy, int iz) {
+    ix = (ix + g.Nx) % g.Nx;
+    iy = (iy + g.Ny) % g.Ny;
+    iz = (iz + g.Nz) % g.Nz;
+    return ((long)ix * g.Ny + iy) * g.Nz + iz;
+  };
+  for (int ix = 0; ix < g.Nx; ++ix)
+    for (int iy = 0; iy < g.Ny; ++iy)
+      for (int iz = 0; iz < g.Nz; ++iz) {
+        const long p = P(ix, iy, iz);
+        const long q[6] = {P(ix + 1, iy, iz), P(ix - 1, iy, iz), P(ix, iy + 1, iz),
+                           P(ix, iy - 1, iz), P(ix, iy, iz + 1), P(ix, iy, iz - 1)};
+        double gx = 0.0;
+        for (int a = 0; a < 3; ++a)
+          gx += D[a] * ((rho[q[2 * a]] + rho[p]) * (x[q[2 * a]] - x[p]) - (rho[p] + rho[q[2 * a + 1]]) * (x[p] - x[q[2 * a + 1]]));
+        out[p] = ((s->bnd & 1) && (ix == 0 || iy == 0 || iz == 0)) ? 0.0 : gx;
+      }
+}
+
+// r = A x as Dynamical_Matrix::matrix_dot_v1 applies it (src/Dynamical_Matrix.cpp:34-70): A = the Hessian
+// (DFT::matrix_dot_v_intern) or, mode bit 2, the DDFT dynamical matrix g . (H x) (DDFT::matrix_dot_v_intern,
+// src/DDFT.cpp:353-361); mode bit 1 applies it twice (negated for the dynamical matrix); boundary entries of the
+// input copy and of the result zeroed with a fixed background.
+void orc_dft_matrix_apply(orc_dft* h, const double* rho, const double* x, double* r, int full, int mode) {
+  auto* s = reinterpret_cast<DftState*>(h);
+  const Grid& g = s->fmt.g;
+  const long N = g.N();
+  const bool bnd = s->bnd & 1, dynamic = mode & 2;
+  auto zero_bnd = [&](double* v) {
+    if (bnd)
+      for (long p = 0; p < N; ++p)
+        if (boundary_point(g, p)) v[p] = 0.0;
+  };
+  std::vector<double> t(x, x + N), u(N);
+  zero_bnd(t.data());
+  auto once = [&]() {
+    orc_dft_hess(h, rho, t.data(), r, full);
+    if (dynamic) {
+      std::copy(r, r + N, u.begin());
+      orc_gdotx(h, rho, u.data(), r);
+    }
+  };
+  once();
+  if (mode & 1) {
+    std::copy(r, r + N, t.begin());
+    once();
+    if (dynamic)
+      for (long p = 0; p < N; ++p) r[p] = -r[p];
+  }
+  zero_bnd(r);
+}
+
 // Coefficients of the Chebyshev interpolant of log(0.5 (b - a) x + 0.5 (b + a)) at the order + 1 nodes
 // x_k = cos(pi (k + 1/2) / (order + 1)): Log_Det::get_coefficients (src/Log_Det.cpp:36-62).
 void orc_cheb_log_coeffs(int order, double a, double b, double* c) {
@@ -1647,7 +1706,8 @@ void orc_rademacher(long seed, long n, int nsamples, double* out) {
 // Log_Det::calculate_log_det (src/Log_Det.cpp:96-166) on this DFT's Hessian, applied as
 // Log_Det::matrix_dot_v1 / Dynamical_Matrix::matrix_dot_v (src/Log_Det.cpp:73-92,
 // src/Dynamical_Matrix.cpp:34-50): with a fixed background the input's boundary entries are zeroed on a copy and
-// the product's after it; squared != 0 applies the matrix twice (use_squared_matrix_).  full = full_hessian_.
+// the product's after it; squared: bit 1 applies the matrix twice (use_squared_matrix_), bit 2 takes the DDFT
+// dynamical matrix (orc_dft_matrix_apply).  full = full_hessian_.
 // Returns 2 if eta > 1 is met, else 0.
 int orc_dft_log_det(orc_dft* h, const double* rho, int order, double lam_max, double lam_min, long seed,
                     int num_samples, int has_zero, int full, int squared, double* log_det, double* variance) {
@@ -1666,16 +1726,11 @@ int orc_dft_log_det(orc_dft* h, const double* rho, int order, double lam_max, do
       for (long p = 0; p < N; ++p)
         if (boundary_point(g, p)) x[p] = 0.0;
   };
-  std::vector<double> t(N), r(N);
+  std::vector<double> r(N);
+  const double rs = (squared & 2) ? -1.0 : 1.0;  // a dynamical matrix's products are negated (src/Log_Det.cpp:124, 139)
   auto matvec = [&](const std::vector<double>& x) {
-    t = x;
-    zero_bnd(t);
-    orc_dft_hess(h, rho, t.data(), r.data(), full);
-    if (squared) {
-      t = r;
-      orc_dft_hess(h, rho, t.data(), r.data(), full);
-    }
-    zero_bnd(r);
+    orc_dft_matrix_apply(h, rho, x.data(), r.data(), full, squared);
+    for (long p = 0; p < N; ++p) r[p] *= rs;
   };
   auto accu = [&](const std::vector<double>& x) {
     double sum = 0.0;
@@ -1737,20 +1792,8 @@ double orc_dft_eig_objective(orc_dft* h, const double* rho, const double* x, con
   auto* s = reinterpret_cast<DftState*>(h);
   const Grid& g = s->fmt.g;
   const long N = g.N();
-  const bool bnd = s->bnd & 1;
-  std::vector<double> t(x, x + N), r(N);
-  auto zero_bnd = [&](std::vector<double>& v) {
-    if (bnd)
-      for (long p = 0; p < N; ++p)
-        if (boundary_point(g, p)) v[p] = 0.0;
-  };
-  zero_bnd(t);
-  orc_dft_hess(h, rho, t.data(), r.data(), full);
-  if (squared) {
-    t = r;
-    orc_dft_hess(h, rho, t.data(), r.data(), full);
-  }
-  zero_bnd(r);
+  std::vector<double> r(N);
+  orc_dft_matrix_apply(h, rho, x, r.data(), full, squared);
   if (vshift) {
     double vx = 0.0;
     for (long p = 0; p < N; ++p) vx += vshift[p] * x[p];

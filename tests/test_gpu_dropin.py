@@ -224,6 +224,8 @@ def test_log_det_through_cpp_api(demo):
     assert rel(demo["log_det"][0], ld) <= 1e-10
     q = ld + rho.size * np.log(1.0 / (0.05 + 5e-4))  # the mean of v.u over the probes
     assert abs(demo["log_det"][1] ** 2 - var ** 2) <= 1e-10 * q * q
+    ld2, var2 = o.log_det(rho, 12, 40.0, 5e-3, 7, 2, has_zero=True, dynamic=True)
+    assert rel(demo["log_det_ddft"][0], ld2) <= 1e-10
 
 
 def test_eigenvalues_through_cpp_api(demo):

@@ -15,12 +15,14 @@ def density(shape, seed, rho0=0.4, amp=0.1):
     return rho0 * (1 + amp * (2 * u - 1))
 
 
-@pytest.mark.parametrize("bnd,vs,sign,squared,ideal", [(False, False, False, False, True),
-                                                        (False, True, True, False, True),
-                                                        (True, True, False, False, True),
-                                                        (False, False, True, True, False),
-                                                        (True, False, False, True, True)])
-def test_objective_matches_oracle(cdft, bnd, vs, sign, squared, ideal):
+@pytest.mark.parametrize("bnd,vs,sign,squared,ideal,dyn", [(False, False, False, False, True, False),
+                                                            (False, True, True, False, True, False),
+                                                            (True, True, False, False, True, False),
+                                                            (False, False, True, True, False, False),
+                                                            (True, False, False, True, True, False),
+                                                            (False, True, False, False, True, True),
+                                                            (True, False, True, True, True, True)])
+def test_objective_matches_oracle(cdft, bnd, vs, sign, squared, ideal, dyn):
     shape, dx = (16, 12, 24), 0.1
     rho = density(shape, 0xE16)
     w = oracle.fmt_weights(shape, dx, 1.0, mode=oracle.WEIGHTS_ACCURATE)
@@ -32,8 +34,8 @@ def test_objective_matches_oracle(cdft, bnd, vs, sign, squared, ideal):
     rng = np.random.default_rng(9)
     x = rng.standard_normal(shape)
     v = rng.standard_normal(shape) if vs else None
-    f0, g0 = o.eig_objective(rho, x, v, 1e6, sign, ideal, squared)
-    f1, g1 = s.eigen_objective(rho, x, v, 1e6, sign, ideal, squared)
+    f0, g0 = o.eig_objective(rho, x, v, 1e6, sign, ideal, squared, dyn)
+    f1, g1 = s.eigen_objective(rho, x, v, 1e6, sign, ideal, squared, dyn)
     assert abs(f1 - f0) <= 1e-11 * abs(f0)
     assert np.max(np.abs(g1 - g0)) <= 1e-11 * np.max(np.abs(g0))
 

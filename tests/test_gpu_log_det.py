@@ -73,3 +73,25 @@ def test_log_det_rejects_bad_arguments(cdft):
     for args in ((0, 0.05, 5e-4), (10, 5e-4, 0.05), (10, 0.05, -1.0)):
         with pytest.raises(cdft.CdftError):
             s.log_det(rho, *args, 1, 1)
+
+
+@pytest.mark.parametrize("bnd,squared", [(False, False), (True, False), (False, True)])
+def test_dynamic_matrix_matches_oracle(cdft, bnd, squared):
+    """DDFT::matrix_dot_v_intern (src/DDFT.cpp:353-361) = g . (H v) on the device vs the oracle (pinned in
+    tests/test_oracle_log_det.py), and the log det of the dynamical matrix (products negated, zero mode shifted)."""
+    shape, dx = (16, 12, 24), 0.1
+    rho = density(shape, 0x1E0)
+    w = oracle.fmt_weights(shape, dx, 1.0, mode=oracle.WEIGHTS_ACCURATE)
+    o = oracle.DFT(shape, dx, 1.0, model=oracle.WBI, mf=True, kT=3.0, accurate_weights=True)
+    if bnd:
+        o.set_constraints(fixed_background=True)
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mean_field=True, kT=3.0,
+                    fixed_background=bnd)
+    v = np.random.default_rng(4).standard_normal(shape)
+    y0 = o.dynamic_dot(rho, v, squared=squared)
+    y1 = s.dynamic_dot(rho, v, squared=squared)
+    assert np.max(np.abs(y1 - y0)) <= 1e-11 * np.max(np.abs(y0))
+    lmax, lmin = (1600.0, 1e-4) if squared else (40.0, 5e-3)
+    ld0, _ = o.log_det(rho, 10, lmax, lmin, 21, 2, has_zero=True, squared=squared, dynamic=True)
+    ld1, _ = s.log_det(rho, 10, lmax, lmin, 21, 2, has_zero_eigenvalue=True, squared=squared, dynamic=True)
+    assert rel(ld1, ld0) <= 1e-10

@@ -101,3 +101,61 @@ def test_log_det_estimates_slogdet():
     ns = 200
     ld, var = o.log_det(rho, 40, 1.2 * ev.max(), 0.8 * ev.min(), 11, ns)
     assert abs(ld - exact) <= 4 * var / np.sqrt(ns) + 1e-8 * abs(exact)
+
+
+def _dense(f, N):
+    return np.stack([f(np.eye(N)[j].reshape(SHAPE)).ravel() for j in range(N)], axis=1)
+
+
+def test_mobility_operator():
+    """g . x of DDFT::g_dot_x (src/DDFT.cpp:320-351): symmetric, negative semi-definite, constants in its null space
+    and its columns summing to zero (mass conservation), rho0 times the discrete Laplacian at uniform density."""
+    o, rho = _dft()
+    N = rho.size
+    G = _dense(lambda x: o.gdotx(rho, x), N)
+    assert np.max(np.abs(G - G.T)) <= 1e-12 * np.max(np.abs(G))
+    assert np.max(np.linalg.eigvalsh(G)) <= 1e-10 * np.max(np.abs(G))
+    assert np.max(np.abs(G @ np.ones(N))) <= 1e-12 * np.max(np.abs(G)) and np.max(np.abs(G.sum(0))) <= 1e-10
+    rho0 = np.full(SHAPE, 0.4)
+    k = 2 * np.pi * 2 / (SHAPE[1] * DX)
+    y = np.arange(SHAPE[1]) * DX
+    wave = np.broadcast_to(np.cos(k * y)[None, :, None], SHAPE).copy()
+    lam = 0.4 * 2 * (np.cos(k * DX) - 1) / DX ** 2
+    assert np.max(np.abs(o.gdotx(rho0, wave) - lam * wave)) <= 1e-12 * abs(lam)
+
+
+@pytest.mark.parametrize("bnd,squared", [(False, False), (True, False), (False, True)])
+def test_dynamic_log_det_is_the_dense_estimator(bnd, squared):
+    """Log_Det on the DDFT dynamical matrix (DDFT::matrix_dot_v_intern, src/DDFT.cpp:353-361; is_dynamic negation,
+    src/Dynamical_Matrix.cpp:48 and src/Log_Det.cpp:124, 139), zero mode shifted: the estimator evaluated with the
+    dense, non-symmetric operator -P G P H P (squared: -(P G P H P)^2) by the matrix three-term recurrence."""
+    o, rho = _dft(bnd)
+    op, _ = _dft(False)
+    N = rho.size
+    mask = _boundary_mask() if bnd else np.zeros(N, bool)
+    P = np.diag(np.where(mask, 0.0, 1.0))
+    H = _dense(lambda x: op.hess(rho, x), N)
+    G = _dense(lambda x: op.gdotx(rho, x), N)
+    M = P @ G @ P @ H @ P
+    A = -(M @ M) if squared else M
+    assert np.max(np.abs(_dense(lambda x: o.dynamic_dot(rho, x, squared=squared), N) - A)) <= 1e-10 * np.max(np.abs(A))
+    B = -A  # Log_Det negates the products of a dynamical matrix
+    ev = np.sort(np.linalg.eigvals(B[~mask][:, ~mask]).real)
+    lam_min, lam_max = 0.5 * ev[ev > 1e-9 * ev.max()].min(), 1.2 * ev.max()
+    lam_mid = 0.5 * (lam_min + lam_max)
+    B = B + lam_mid / N * np.ones((N, N))
+    order, seed, ns = 16, 5, 3
+    sc = 1.0 / (lam_min + lam_max)
+    a, b = lam_min * sc, lam_max * sc
+    X = (2.0 / (b - a)) * sc * B - (b + a) / (b - a) * np.eye(N)
+    c = oracle.cheb_log_coeffs(order, lam_max, lam_min)
+    T0, T1 = np.eye(N), X
+    pX = c[0] * T0 + c[1] * T1
+    for k in range(2, order + 1):
+        T0, T1 = T1, 2 * X @ T1 - T0
+        pX += c[k] * T1
+    V = oracle.rademacher(seed, N, ns) * np.where(mask, 0.0, 1.0)
+    q = np.einsum("si,ij,sj->s", V, pX, V)
+    expect = q.mean() - (N - mask.sum()) * np.log(sc) - np.log(lam_mid)
+    ld, _ = o.log_det(rho, order, lam_max, lam_min, seed, ns, has_zero=True, squared=squared, dynamic=True)
+    assert abs(ld - expect) <= 1e-9 * abs(expect)
