@@ -59,3 +59,34 @@ def test_smallest_and_largest_eigenvalue(cdft):
     assert abs(e_lo - ev[0]) <= 1e-6 * abs(ev[0]) and abs(e_hi - ev[-1]) <= 1e-6 * abs(ev[-1])
     Hv = s.hessian_dot(rho, lo.eigen_vec.reshape(shape)).ravel()
     assert np.linalg.norm(Hv - e_lo * lo.eigen_vec) <= 1e-2 * abs(e_lo)
+
+
+@pytest.mark.parametrize("dynamic,bnd,shift", [(False, False, 0.0), (False, True, -0.05), (True, False, 0.0)])
+def test_arnoldi_largest_eigenpairs(cdft, dynamic, bnd, shift):
+    """Arnoldi::determine_largest_eigenpairs (src/Arnoldi.cpp:72-285): the k eigenvalues of A + shift I of largest
+    modulus, minus the shift, against numpy's eigenvalues of the dense operator (Hessian, or g H for the DDFT)."""
+    shape, dx = (8, 8, 8), 0.2
+    rho = density(shape, 0xA70)
+    o = oracle.DFT(shape, dx, 1.0, model=oracle.WBI, accurate_weights=True)
+    N = rho.size
+    mask = np.zeros(shape, bool)
+    if bnd:
+        mask[0], mask[:, 0], mask[:, :, 0] = True, True, True
+    P = np.diag(np.where(mask.ravel(), 0.0, 1.0))
+    H = np.stack([o.hess(rho, np.eye(N)[j].reshape(shape)).ravel() for j in range(N)], axis=1)
+    A = P @ H @ P
+    if dynamic:
+        G = np.stack([o.gdotx(rho, np.eye(N)[j].reshape(shape)).ravel() for j in range(N)], axis=1)
+        A = P @ G @ A
+    act = ~mask.ravel()
+    ev = np.linalg.eigvals(A[act][:, act]).real
+    ref = ev[np.argsort(-np.abs(ev + shift))][:4]
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I,
+                    weights=oracle.fmt_weights(shape, dx, 1.0, mode=oracle.WEIGHTS_ACCURATE), ideal=True,
+                    fixed_background=bnd)
+    ar = cdft.Arnoldi(s, rho, dynamic=dynamic)
+    got = ar.determine_largest_eigenpairs(shift, 4, 16, tol=1e-12)
+    print("arnoldi", got, ref)
+    assert np.allclose(got, ref, rtol=1e-8, atol=1e-12 * np.abs(ref).max())
+    v = ar.get_eigenvector(0)
+    assert np.linalg.norm(ar.matrix_dot_v(v, shift) - (got[0] + shift) * v) <= 1e-6 * abs(got[0] + shift)
