@@ -757,6 +757,134 @@ double DFT::omega_times_beta_over_volume(double density) const {
   return fhelmholtz_times_beta_over_volume(density) - density * mu_times_beta(density);
 }
 
+// Iterates the estimates of the critical point xc = 0.13044 (6/pi) d^-3, Tc = -0.090082 (2 a(Tc)) d^-3 Tc with
+// d = hsd(Tc) (or HSD) until Tc changes by at most 1e-8 (T1 + Tc): DFT::getCriticalPoint (src/DFT_Coex.cpp:39-50).
+void DFT::getCriticalPoint(Potential1& p, double& xc, double& Tc, double HSD) const {
+  for (;;) {
+    const double T1 = Tc;
+    const double d = HSD < 0 ? p.getHSD(Tc) : HSD;
+    xc = 0.13044 * (6.0 / M_PI) / (d * d * d);
+    Tc = -0.090082 * 2 * p.getVDW_Parameter(Tc) / (d * d * d) * Tc;
+    if (!(std::fabs(T1 - Tc) > 1e-8 * (T1 + Tc))) break;
+  }
+}
+
+namespace {
+// Golden-section search of [a, b] for a maximum (sense = 1) or minimum (sense = -1) of f, until b - a <= tol;
+// returns the midpoint (the refinement steps of src/DFT_Coex.cpp:74-98, 113-131).
+template <typename Fn>
+double golden_extremum(Fn f, double a, double b, int sense, double tol) {
+  const double r = (3 - std::sqrt(5.0)) / 2;
+  double u = a + r * (b - a), v = b - r * (b - a);
+  double fu = f(u), fv = f(v);
+  while (b - a > tol) {
+    if (sense * fu > sense * fv) {
+      b = v;
+      v = u;
+      fv = fu;
+      u = a + r * (b - a);
+      fu = f(u);
+    } else {
+      a = u;
+      u = v;
+      fu = fv;
+      v = b - r * (b - a);
+      fv = f(v);
+    }
+  }
+  return (a + b) / 2;
+}
+}  // namespace
+
+// The stationary points of the pressure: its local maximum xs1 (refined to 1e-8) and the following local minimum
+// xs2 (refined to tol), each bracketed by stepping dx: DFT::findSpinodal (src/DFT_Coex.cpp:53-134).
+void DFT::findSpinodal(double xmax, double dx, double& xs1, double& xs2, double tol) const {
+  auto P = [&](double x) { return -omega_times_beta_over_volume(x); };
+  double x = 2 * dx, p = P(x);
+  double dp = p - P(dx);
+  if (dp < 0) throw std::runtime_error("0. Could not get started in DFT::findSpinodal");
+  while (dp > 0 && x < xmax - dx) {
+    const double q = P(x + dx);
+    dp = q - p;
+    p = q;
+    x += dx;
+  }
+  if (x >= xmax - dx) throw std::runtime_error("1. Xmax exceeded in DFT::findSpinodal");
+  double a = x - 2 * dx, b = x;
+  double fs = P(a + dx);
+  if (fs < P(a) || fs < P(b)) throw std::runtime_error("Failed sanity check 1 in DFT::findSpinodal");
+  xs1 = golden_extremum(P, a, b, 1, 1e-8);
+  x = xs1;
+  p = P(x);
+  do {
+    const double q = P(x + dx);
+    dp = q - p;
+    p = q;
+    x += dx;
+  } while (dp < 0 && x < xmax - dx);
+  if (x >= xmax - dx) throw std::runtime_error("2. Xmax exceeded in DFT::findSpinodal");
+  a = x - 2 * dx;
+  b = x;
+  fs = P(a + dx);
+  if (fs > P(a) || fs > P(b)) throw std::runtime_error("Failed sanity check 2 in DFT::findSpinodal");
+  xs2 = golden_extremum(P, a, b, -1, tol);
+}
+
+// Bisection of mu(x) = mu on [xmin, xmax] until the bracketing mu values differ by at most tol:
+// DFT::find_density_from_mu (src/DFT_Coex.cpp:136-157).
+double DFT::find_density_from_mu(double mu, double xmin, double xmax, double tol) const {
+  double mu1 = mu_times_beta(xmin), mu2 = mu_times_beta(xmax);
+  if (mu1 > mu2) {
+    std::swap(xmin, xmax);
+    std::swap(mu1, mu2);
+  }
+  if (mu2 < mu || mu1 > mu) throw std::runtime_error("DFT::find_density_from_mu failed");
+  do {
+    const double x = (xmin + xmax) / 2, m = mu_times_beta(x);
+    if (m > mu) {
+      xmax = x;
+      mu2 = m;
+    } else {
+      xmin = x;
+      mu1 = m;
+    }
+  } while (mu2 - mu1 > tol);
+  return (xmin + xmax) / 2;
+}
+
+// Vapour-liquid coexistence: equal mu by find_density_from_mu on the liquid branch [xs2, xmax - dx], equal pressure
+// by shrinking the vapour density by 1.1 until the pressure difference changes sign, then bisection to tol:
+// DFT::findCoex (src/DFT_Coex.cpp:159-196).
+void DFT::findCoex(double xmax, double dx, double& x1, double& x2, double tol) const {
+  double xs1, xs2;
+  findSpinodal(xmax, dx, xs1, xs2, tol);
+  auto P = [&](double x) { return -omega_times_beta_over_volume(x); };
+  auto liquid = [&](double xv) { return find_density_from_mu(mu_times_beta(xv), xs2, xmax - dx, tol); };
+  double xvap = xs1, xliq = liquid(xvap);
+  double dp1 = P(xvap) - P(xliq), dp2;
+  do {
+    dp2 = dp1;
+    xvap /= 1.1;
+    xliq = liquid(xvap);
+    dp1 = P(xvap) - P(xliq);
+  } while (xvap > 1e-16 && ((dp1 < 0) == (dp2 < 0)));
+  if (xvap < 1e-16) throw std::runtime_error("DFT::findCoex failed 1");
+  double y1 = xvap, y2 = xvap * 1.1;
+  do {
+    const double y = (y1 + y2) / 2;
+    xliq = liquid(y);
+    const double dp = P(y) - P(xliq);
+    if ((dp < 0) == (dp1 < 0)) {
+      y1 = y;
+      dp1 = dp;
+    } else {
+      y2 = y;
+    }
+  } while (std::fabs(y2 - y1) > tol);
+  x1 = (y1 + y2) / 2;
+  x2 = xliq;
+}
+
 // ---------------------------------------------------------------- Minimizer
 // The species takes the minimizer's density and, with a fixed mass, the multiplier of its last evaluation.
 void Minimizer::sync_species() {

@@ -17,6 +17,8 @@
 //   scaled_mf <a> <w0> <mu>  Interaction_Base::scale_interaction(0.5) on LJ/QF: a_vdw, getW(0), mu_times_beta(0.5)
 //   log_det <ld> <var>       Log_Det(dft, 20, 0.05, 5e-4).calculate_log_det(7, 2, false) (16^3, ideal + WB I)
 //   log_det_ddft <ld> <var>  Log_Det(ddft, 12, 40, 5e-3).calculate_log_det(7, 2, true): the DDFT dynamical matrix
+//   coex <xs1> <xs2> <x1> <x2> <x> <xc> <Tc>  findSpinodal / findCoex / find_density_from_mu(mu(0.3)) /
+//                           getCriticalPoint (16^3, ideal + WB I + LJ/QF at kT = 0.8)
 //   eigen <min> <max> <n1> <n2>  Eigenvalues: smallest and (change_sign) largest Hessian eigenvalue (8^3, ideal + WB I)
 //   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
 //   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
@@ -139,6 +141,22 @@ int main() {
     hi.calculate_eigenvector();
     std::printf("eigen %.17g %.17g %d %d\n", lo.get_eigenvalue(), hi.get_eigenvalue(), lo.get_num_eval(),
                 hi.get_num_eval());
+  }
+  {
+    double L6[3] = {1.6, 1.6, 1.6};
+    Density d6(0.1, L6);
+    FMT_Species s15(d6, 1.0, 0.0);
+    Interaction_Interpolation_QF qf15(&s15, &s15, &lj, 0.8);
+    WhiteBearI fmt15;
+    DFT dft15(&s15);
+    dft15.addHardCoreContribution(&fmt15);
+    dft15.addInteraction(&qf15);
+    double xs1 = 0, xs2 = 0, x1 = 0, x2 = 0, xc = 0, Tc = 1.0;
+    dft15.findSpinodal(1.2, 0.01, xs1, xs2, 1e-10);
+    dft15.findCoex(1.2, 0.01, x1, x2, 1e-10);
+    const double xmu = dft15.find_density_from_mu(dft15.mu_times_beta(0.3), 0.2, 0.5, 1e-13);
+    dft15.getCriticalPoint(lj, xc, Tc);
+    std::printf("coex %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", xs1, xs2, x1, x2, xmu, xc, Tc);
   }
   std::vector<DFT_Vec> vv(1), d2F(1);
   vv[0].set(v);

@@ -444,6 +444,12 @@ class DFT {
   double mu_times_beta(double density) const;
   double fhelmholtz_times_beta_over_volume(double density) const;
   double omega_times_beta_over_volume(double density) const;
+  // Bulk phase diagram of this functional (src/DFT_Coex.cpp:39-196), on cdft_system_bulk: pressure p = -omega,
+  // chemical potential mu(x).
+  void getCriticalPoint(Potential1& p, double& xc, double& Tc, double HSD = -1) const;
+  void findSpinodal(double xmax, double dx, double& xs1, double& xs2, double tol) const;
+  double find_density_from_mu(double mu, double xmin, double xmax, double tol) const;
+  void findCoex(double xmax, double dx, double& x1, double& x2, double tol) const;
   double get_f_id() const { return F_id_; }
   double get_f_ext() const { return F_ext_; }
   double get_f_hs() const { return F_hs_; }

@@ -239,3 +239,34 @@ def test_eigenvalues_through_cpp_api(demo):
     ev = np.linalg.eigvalsh(0.5 * (H + H.T))
     lo, hi = demo["eigen"][:2]
     assert rel(lo, ev[0]) <= 1e-6 and rel(hi, ev[-1]) <= 1e-6
+
+
+def test_coexistence_through_cpp_api(demo, cdft):
+    """DFT::findSpinodal / findCoex / find_density_from_mu / getCriticalPoint (src/DFT_Coex.cpp:39-196) through the
+    C++ API, checked by what defines them: the spinodal densities are stationary points of the pressure, the
+    coexisting densities have equal mu and equal pressure, mu(x) = mu(0.3) gives 0.3, and (xc, Tc) is the fixed point
+    of the critical-point estimate."""
+    xs1, xs2, x1, x2, xmu, xc, Tc = demo["coex"]
+    s = cdft.System((16, 16, 16), 0.1, 1.0, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=0.8)
+
+    def mu(x):
+        return s.bulk(x)[1]
+
+    def p(x):
+        f, m = s.bulk(x)
+        return x * m - f
+
+    assert 0 < x1 < xs1 < xs2 < x2 < 1.2
+    assert abs(mu(x1) - mu(x2)) <= 1e-7 * abs(mu(x1))
+    # findCoex stops on |y2 - y1| <= tol and takes the liquid density of the last bisection step: with tol = 1e-10
+    # the pressures agree to about 1e-7 relative
+    assert abs(p(x1) - p(x2)) <= 1e-6 * abs(p(x1))
+    h = 1e-6
+    slope = abs(p(x1 + h) - p(x1 - h)) / (2 * h)
+    for xs in (xs1, xs2):
+        assert abs(p(xs + h) - p(xs - h)) / (2 * h) <= 1e-4 * slope
+    assert abs(xmu - 0.3) <= 1e-9
+    d = cdft.lj_hsd(Tc)
+    # the loop ends when Tc moves by at most 1e-8 relative, xc holds the diameter of the step before
+    assert abs(xc - 0.13044 * (6 / np.pi) / d ** 3) <= 1e-7 * xc
+    assert abs(Tc + 0.090082 * 2 * cdft.potential_vdw(Tc) / d ** 3 * Tc) <= 1e-6 * Tc
