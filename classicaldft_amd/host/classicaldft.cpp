@@ -603,6 +603,14 @@ void DFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param) const {
   result.set(rw[0]);
 }
 
+void DDFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void*) const {
+  const Density& d = dft_->getDensity(0);
+  if (v.size() != d.Ntot()) throw std::runtime_error("DDFT::matrix_dot_v: one direction of Ntot values");
+  if (result.size() != d.Ntot()) result.zeros(d.Ntot());
+  check(cdft_system_dynamic_dot(dft_->system(), d.get_density_pointer(), v.memptr(), result.memptr(),
+                                use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0));
+}
+
 Log_Det::Log_Det(const DDFT& matrix, int order, double lam_max, double lam_min, bool verbose)
     : matrix_(matrix.getDFT()), order_(order), lam_max_(lam_max), lam_min_(lam_min), verbose_(verbose),
       dynamic_(true) {}

@@ -19,6 +19,7 @@
 //   log_det_ddft <ld> <var>  Log_Det(ddft, 12, 40, 5e-3).calculate_log_det(7, 2, true): the DDFT dynamical matrix
 //   coex <xs1> <xs2> <x1> <x2> <x> <xc> <Tc>  findSpinodal / findCoex / find_density_from_mu(mu(0.3)) /
 //                           getCriticalPoint (16^3, ideal + WB I + LJ/QF at kT = 0.8)
+//   ddft_dot <sum> <v.Mv>   DDFT::matrix_dot_v1 of the seeded direction (the dynamical matrix g . H)
 //   eigen <min> <max> <n1> <n2>  Eigenvalues: smallest and (change_sign) largest Hessian eigenvalue (8^3, ideal + WB I)
 //   open_ddft <F> <rho(0,0,0)> <rho(8,8,8)>  2 DDFT_IF_Open steps (16^3, WB I + LJ/QF, background 0.5)
 //   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
@@ -124,6 +125,9 @@ int main() {
     Log_Det ldd(ddft13, 12, 40.0, 5e-3);
     const double ldd_v = ldd.calculate_log_det(7, 2, true, var);
     std::printf("log_det_ddft %.17g %.17g\n", ldd_v, var);
+    DFT_Vec gv;
+    ddft13.matrix_dot_v1(v, gv);
+    std::printf("ddft_dot %.17g %.17g\n", gv.accu(), gv.dotWith(v));
   }
   {
     double L5[3] = {1.6, 1.6, 1.6};

@@ -608,6 +608,11 @@ class DDFT : public Minimizer {
  public:
   explicit DDFT(DFT* dft, bool showGraphics = false);
   ~DDFT() override;
+  // Dynamical_Matrix interface of the DDFT (include/Minimizer.h:211-220, src/DDFT.cpp:353-361 through
+  // src/Dynamical_Matrix.cpp:34-70): result = g . (H v), applied twice and negated with use_squared_matrix_.
+  void matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param = nullptr) const;
+  void set_use_squared_matrix_(bool b) { use_squared_matrix_ = b; }
+  bool is_dynamic() const { return true; }
   double step() override;
   void reset() override {}  // Minimizer::reset only refreshes the FIRE aliases, which DDFT does not use
   void set_tolerence_fixed_point(double e) { tol_ = e; }
@@ -623,6 +628,7 @@ class DDFT : public Minimizer {
   void ensure();
   double dt_ = 0.0, dtMax_ = 1.0, tol_ = 1e-4;
   cdft_ddft ddft_ = nullptr;
+  bool use_squared_matrix_ = false;
   cdft_ddft_report rep_{};
 };
 

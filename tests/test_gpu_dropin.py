@@ -226,6 +226,10 @@ def test_log_det_through_cpp_api(demo):
     assert abs(demo["log_det"][1] ** 2 - var ** 2) <= 1e-10 * q * q
     ld2, var2 = o.log_det(rho, 12, 40.0, 5e-3, 7, 2, has_zero=True, dynamic=True)
     assert rel(demo["log_det_ddft"][0], ld2) <= 1e-10
+    v = 2 * oracle.splitmix64_uniform(int(np.prod(shape)), 43).reshape(shape) - 1
+    gv = o.dynamic_dot(rho, v)
+    assert abs(demo["ddft_dot"][0]) <= 1e-10 * np.abs(gv).sum()  # g conserves mass: the entries sum to zero
+    assert rel(demo["ddft_dot"][1], np.sum(v * gv)) <= 1e-10
 
 
 def test_eigenvalues_through_cpp_api(demo):
