@@ -23,6 +23,10 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 HIP_SOURCES = ("cdft_api.hip", "solvers.hip", "batch.hip", "unet.hip")
+# host-only C++ of libcdft.so, compiled by g++ without floating-point contraction (weights_ref.cpp reproduces the
+# reference's double expressions operation for operation)
+HOST_SOURCES = ("weights_ref.cpp",)
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 
 def hipcc():
@@ -46,6 +50,10 @@ def build_cdft():
     objs = [os.path.join(objdir, f[:-4] + ".o") for f in HIP_SOURCES]
     cmds = [[hipcc(), *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", "-o", o, os.path.join(CSRC, f)]
             for f, o in zip(HIP_SOURCES, objs)]
+    hobjs = [os.path.join(objdir, f[:-4] + ".o") for f in HOST_SOURCES]
+    cmds += [["g++", *HOST_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", "-o", o, os.path.join(CSRC, f)]
+             for f, o in zip(HOST_SOURCES, hobjs)]
+    objs = objs + hobjs
     for c in cmds:
         print(" ".join(c), flush=True)
     procs = [subprocess.Popen(c) for c in cmds]

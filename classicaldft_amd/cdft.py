@@ -10,7 +10,8 @@ CDFT_OK, CDFT_ETA_TOO_LARGE, CDFT_UPHILL_LIMIT = 0, 1, 2
 CDFT_BAD_ARG, CDFT_HIP_ERROR, CDFT_NOT_SUPPORTED = -1, -2, -3
 MODEL_NONE, ROSENFELD, RSLT, ESFMT, WHITEBEAR_I, WHITEBEAR_II = -1, 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
-DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, SQUARED_MATRIX, DYNAMIC_MATRIX = 1, 2, 4, 8, 16, 32
+WEIGHTS_ACCURATE, WEIGHTS_REFERENCE = 0, 1  # cdft_weights_mode
+DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, SQUARED_MATRIX, DYNAMIC_MATRIX, CENTRAL_DIFFERENCES = 1, 2, 4, 8, 16, 32, 64
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -40,7 +41,7 @@ class _Desc(ctypes.Structure):
         ("mf_potential", _i), ("mf_alpha", _d), ("mf_bh", _i), ("mf_r_att_min", _d), ("mf_scheme", _i),
         ("mf_ngauss", _i),
         ("ao_Rp", _d), ("ao_reservoir", _d), ("ao_weights", _p),
-        ("fixed_mass", _d), ("boundary", _i), ("mf_scale", _d),
+        ("fixed_mass", _d), ("boundary", _i), ("mf_scale", _d), ("weights_mode", _i),
     ]
 
 
@@ -165,6 +166,7 @@ def lib():
         L.cdft_ddft_create.argtypes = [_p, _p, _d, _d, _d, ctypes.POINTER(_p)]
         L.cdft_ddft_step.argtypes = [_p, ctypes.POINTER(DdftReport)]
         L.cdft_ddft_set_max_time_step.argtypes = [_p, _d]
+        L.cdft_ddft_set_central_differences.argtypes = [_p, _i]
         L.cdft_ddft_get_density.argtypes = [_p, _p]
         L.cdft_ddft_destroy.argtypes = [_p]
         L.cdft_system_set_timing.argtypes = [_p, _i]
@@ -192,6 +194,17 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(_p)
 
 
+def _out(a, n, shape):
+    """A caller-supplied output array the library writes n doubles into: float64, C-contiguous, n elements."""
+    if a is None:
+        return np.zeros(shape)
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous or not a.flags.writeable:
+        raise ValueError("output array must be a writeable, C-contiguous float64 numpy array")
+    if a.size != n:
+        raise ValueError(f"output array must hold {n} values, got {a.size}")
+    return a
+
+
 def device_count():
     n = _i(0)
     _check(lib().cdft_device_count(ctypes.byref(n)))
@@ -205,13 +218,14 @@ class System:
                  vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0,
                  mus=None, vexts=None, potential=POT_LJ, alpha=0.0, bh=False, r_att_min=0.0, mf_scheme=MF_QF,
                  ngauss=0, ao_Rp=0.0, ao_reservoir=0.0, ao_weights=None, fixed_mass=0.0, fixed_background=False,
-                 homogeneous_boundary=False, mf_scale=1.0):
+                 homogeneous_boundary=False, mf_scale=1.0, weights_mode=WEIGHTS_ACCURATE):
         """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create).
         potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule;
         ao_Rp > 0 adds an Asakura-Oosawa polymer of that radius at reservoir density ao_reservoir.
         fixed_mass / fixed_background / homogeneous_boundary: Species::setFixedMass / setFixedBackground /
         setHomogeneousBoundary (the full evaluation rescales rho to the mass and constrains dF); batches take one
-        fixed mass per member."""
+        fixed mass per member.  weights_mode: how generated cell weights are computed (WEIGHTS_ACCURATE on the
+        device, or WEIGHTS_REFERENCE: the reference's double expressions on the host)."""
         fm = np.atleast_1d(np.asarray(fixed_mass, dtype=np.float64)).reshape(-1)
         self.shape = tuple(int(n) for n in shape)
         self.fixed_background = bool(fixed_background)
@@ -235,7 +249,8 @@ class System:
                   arr(vext, self.N), int(bool(mean_field)), float(sigma), float(eps), float(rcut), float(kT),
                   arr(mf_kernel, self.N), int(potential), float(alpha), int(bool(bh)), float(r_att_min),
                   int(mf_scheme), int(ngauss), float(ao_Rp), float(ao_reservoir), arr(ao_weights, 11 * self.N),
-                  float(fm[0]), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0), float(mf_scale))
+                  float(fm[0]), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0), float(mf_scale),
+                  int(weights_mode))
         h = _p()
         if mus is None:
             self.nb = 1
@@ -275,7 +290,7 @@ class System:
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         if rho.size != self.nb * self.N:
             raise ValueError(f"expected {self.nb * self.N} densities, got {rho.size}")
-        out = np.zeros(self._ashape()) if dF is None else dF
+        out = _out(dF, self.nb * self.N, self._ashape())
         parts = np.zeros(5 * self.nb)
         flags = (ACCUMULATE if accumulate else 0) | (NO_IDEAL if only_fex else 0)
         _check(lib().cdft_system_eval(self.h, _ptr(rho), _ptr(out), _ptr(parts), flags))
@@ -286,7 +301,7 @@ class System:
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         if rho.size != self.nb * self.N:
             raise ValueError(f"expected {self.nb * self.N} densities, got {rho.size}")
-        out = np.zeros(self._ashape()) if dF is None else dF
+        out = _out(dF, self.nb * self.N, self._ashape())
         F = np.zeros(self.nb)
         _check(lib().cdft_fmt_eval(self.h, _ptr(rho), _ptr(out), _ptr(F), ACCUMULATE if accumulate else 0))
         return (F[0] if self.nb == 1 else F), out
@@ -297,20 +312,21 @@ class System:
         v = np.ascontiguousarray(v, dtype=np.float64)
         if rho.size != self.nb * self.N or v.size != rho.size:
             raise ValueError(f"expected {self.nb * self.N} densities and directions")
-        res = np.zeros(self._ashape()) if out is None else out
+        res = _out(out, self.nb * self.N, self._ashape())
         fl = (0 if ideal else NO_IDEAL) | (ACCUMULATE if accumulate else 0)
         _check(lib().cdft_system_hessian_dot(self.h, _ptr(rho), _ptr(v), _ptr(res), fl))
         return res
 
 
-    def dynamic_dot(self, rho, v, ideal=True, squared=False):
-        """DDFT::matrix_dot_v_intern (src/DDFT.cpp:353-361): g . (H v), g the DDFT mobility operator."""
+    def dynamic_dot(self, rho, v, ideal=True, squared=False, central=False):
+        """DDFT::matrix_dot_v_intern (src/DDFT.cpp:353-361): g . (H v), g the DDFT mobility operator (central:
+        the DDFT's central_differences)."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         v = np.ascontiguousarray(v, dtype=np.float64)
         if rho.size != self.N or v.size != self.N:
             raise ValueError(f"expected {self.N} densities and components")
         out = np.zeros(rho.shape)
-        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0)
+        fl = (0 if ideal else NO_IDEAL) | (SQUARED_MATRIX if squared else 0) | (CENTRAL_DIFFERENCES if central else 0)
         _check(lib().cdft_system_dynamic_dot(self.h, _ptr(rho), _ptr(v), _ptr(out), fl))
         return out
 
@@ -345,13 +361,14 @@ class System:
                                          int(num_samples), int(bool(has_zero_eigenvalue)), fl, ctypes.byref(ld),
                                          ctypes.byref(var)))
         return ld.value, var.value
+
     def fmt_hessian_dot(self, rho, v, out=None, accumulate=False):
         """FMT::add_second_derivative: the hard-sphere part of the Hessian applied to v."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         v = np.ascontiguousarray(v, dtype=np.float64)
         if rho.size != self.nb * self.N or v.size != rho.size:
             raise ValueError(f"expected {self.nb * self.N} densities and directions")
-        res = np.zeros(self._ashape()) if out is None else out
+        res = _out(out, self.nb * self.N, self._ashape())
         _check(lib().cdft_fmt_hessian_dot(self.h, _ptr(rho), _ptr(v), _ptr(res), ACCUMULATE if accumulate else 0))
         return res
 
@@ -499,12 +516,15 @@ class Fire:
 class DDFT:
     """Integrating-factor DDFT on the device (src/DDFT.cpp:42-351)."""
 
-    def __init__(self, system, rho0, dt=0.0, dt_max=0.0, tol=0.0):
+    def __init__(self, system, rho0, dt=0.0, dt_max=0.0, tol=0.0, central_differences=False):
+        """central_differences: the third argument of DDFT(dft, showGraphics, central_differences)."""
         self.system = system
         rho0 = np.ascontiguousarray(rho0, dtype=np.float64)
         h = _p()
         _check(lib().cdft_ddft_create(system.h, _ptr(rho0), float(dt), float(dt_max), float(tol), ctypes.byref(h)))
         self.h = h
+        if central_differences:
+            _check(lib().cdft_ddft_set_central_differences(self.h, 1))
 
     def step(self):
         r = DdftReport()

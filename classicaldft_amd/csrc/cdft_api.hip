@@ -38,6 +38,7 @@ void validate(const cdft_system_desc& d) {
             "Gauss kernel rules need 1..32 points per axis");
     REQUIRE(d.mf_potential != CDFT_POT_WHDF || d.lj_rcut > d.lj_sigma, "WHDF needs rcut > sigma");
   }
+  REQUIRE(d.weights_mode == CDFT_WEIGHTS_ACCURATE || d.weights_mode == CDFT_WEIGHTS_REFERENCE, "unknown weights mode");
   REQUIRE(d.fixed_mass >= 0, "the fixed mass must be non-negative");
   REQUIRE(d.boundary >= 0 && d.boundary <= (CDFT_FIXED_BACKGROUND | CDFT_HOMOGENEOUS_BOUNDARY), "unknown boundary flags");
   // src/Species.cpp:75-76
@@ -70,6 +71,32 @@ void launch_mf_kernel(cdft_system_s& s, double* out) {
   HIPCHK(hipGetLastError());
 }
 
+// The 11 real-space cell weights of spheres of diameter hsd (11 * N doubles, device) on the system's stream:
+// the double-double device generator, or with CDFT_WEIGHTS_REFERENCE the reference's expressions on the host.
+void cell_weights(cdft_system_s& s, double hsd, double* wreal) {
+  const long N = s.N;
+  if (s.d.weights_mode == CDFT_WEIGHTS_REFERENCE) {
+    std::vector<double> h(11 * N);
+    reference_weights(s.g.Nx, s.g.Ny, s.g.Nz, s.d.dx, s.d.dy, s.d.dz, hsd, h.data());
+    HIPCHK(hipMemcpyAsync(wreal, h.data(), sizeof(double) * 11 * N, hipMemcpyHostToDevice, s.stream));
+    HIPCHK(hipStreamSynchronize(s.stream));
+    return;
+  }
+  const double R = hsd / 2;
+  const int Sxm = 2 + int(R / s.d.dx), Sym = 2 + int(R / s.d.dy), Szm = 2 + int(R / s.d.dz);
+  REQUIRE(Sxm < 4 * s.g.Nx && Sym < 4 * s.g.Ny && Szm < 4 * s.g.Nz, "sphere is too large for the periodic box");
+  DevBuf cells;
+  const long ncell = (long)(Sxm + 1) * (Sym + 1) * (Szm + 1);
+  cells.alloc(sizeof(double) * 11 * ncell);
+  hipLaunchKernelGGL(k_fmt_cells, dim3(div_up(11 * ncell, 64)), dim3(64), 0, s.stream, Sxm, Sym, Szm, s.d.dx, s.d.dy,
+                     s.d.dz, R, cells.as<double>());
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fmt_gather, dim3(div_up(N, kThreads)), dim3(kThreads), 0, s.stream, s.g.Nx, s.g.Ny, s.g.Nz,
+                     Sxm, Sym, Szm, cells.as<double>(), wreal);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s.stream));
+}
+
 // The m weight spectra (x 1/N) of spheres of diameter hsd into W: from host cell weights or generated on the device.
 template <typename T>
 void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights, T* W) {
@@ -83,20 +110,9 @@ void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights
     if (host_weights) {
       HIPCHK(hipMemcpyAsync(wreal.p, host_weights, sizeof(double) * 11 * N, hipMemcpyHostToDevice, s.stream));
     } else {
-      const double R = hsd / 2;
-      const int Sxm = 2 + int(R / s.d.dx), Sym = 2 + int(R / s.d.dy), Szm = 2 + int(R / s.d.dz);
-      REQUIRE(Sxm < 4 * s.g.Nx && Sym < 4 * s.g.Ny && Szm < 4 * s.g.Nz, "sphere is too large for the periodic box");
-      DevBuf cells;
-      const long ncell = (long)(Sxm + 1) * (Sym + 1) * (Szm + 1);
-      cells.alloc(sizeof(double) * 11 * ncell);
-      hipLaunchKernelGGL(k_fmt_cells, dim3(div_up(ncell, 64)), dim3(64), 0, s.stream, Sxm, Sym, Szm, s.d.dx, s.d.dy,
-                         s.d.dz, R, cells.as<double>());
-      HIPCHK(hipGetLastError());
-      hipLaunchKernelGGL(k_fmt_gather, dim3(div_up(N, kThreads)), dim3(kThreads), 0, s.stream, s.g.Nx, s.g.Ny, s.g.Nz,
-                         Sxm, Sym, Szm, cells.as<double>(), wreal.as<double>());
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipStreamSynchronize(s.stream));
+      cell_weights(s, hsd, wreal.as<double>());
     }
+    HIPCHK(hipStreamSynchronize(s.stream));
     for (int a = 0; a < s.m; ++a) {
       launch_rows_r2c<double, double>(s, wreal.as<double>() + a * N, 0, S.as<cplx<double>>(), 0, 1);
       launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 1, true);
@@ -1102,17 +1118,21 @@ namespace {
 // iz = 0 of a fixed background.
 __global__ __launch_bounds__(256) void k_gdotx(const double* __restrict__ rho, const double* __restrict__ x,
                                                double* __restrict__ out, int Nx, int Ny, int Nz, double Dx, double Dy,
-                                               double Dz, int fixed_bnd) {
+                                               double Dz, int fixed_bnd, int central) {
   const long N = (long)Nx * Ny * Nz;
+  const int st = central ? 2 : 1;  // x's neighbours at the stride, the density's at one site (src/DDFT.cpp:325)
   for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
     const int z = p % Nz, y = (p / Nz) % Ny, ix = p / ((long)Nz * Ny);
     const long px = ((long)((ix + 1) % Nx) * Ny + y) * Nz + z, mx = ((long)((ix + Nx - 1) % Nx) * Ny + y) * Nz + z;
     const long py = ((long)ix * Ny + (y + 1) % Ny) * Nz + z, my = ((long)ix * Ny + (y + Ny - 1) % Ny) * Nz + z;
     const long pz = ((long)ix * Ny + y) * Nz + (z + 1) % Nz, mz = ((long)ix * Ny + y) * Nz + (z + Nz - 1) % Nz;
-    const double x0 = x[p], d0 = rho[p];
-    double g = Dx * ((rho[px] + d0) * (x[px] - x0) - (d0 + rho[mx]) * (x0 - x[mx])) +
-               Dy * ((rho[py] + d0) * (x[py] - x0) - (d0 + rho[my]) * (x0 - x[my])) +
-               Dz * ((rho[pz] + d0) * (x[pz] - x0) - (d0 + rho[mz]) * (x0 - x[mz]));
+    const long qx = ((long)((ix + st) % Nx) * Ny + y) * Nz + z, nx = ((long)((ix + 2 * Nx - st) % Nx) * Ny + y) * Nz + z;
+    const long qy = ((long)ix * Ny + (y + st) % Ny) * Nz + z, ny = ((long)ix * Ny + (y + 2 * Ny - st) % Ny) * Nz + z;
+    const long qz = ((long)ix * Ny + y) * Nz + (z + st) % Nz, nz = ((long)ix * Ny + y) * Nz + (z + 2 * Nz - st) % Nz;
+    const double x0 = x[p], d0 = central ? 0.0 : rho[p];
+    double g = Dx * ((rho[px] + d0) * (x[qx] - x0) - (d0 + rho[mx]) * (x0 - x[nx])) +
+               Dy * ((rho[py] + d0) * (x[qy] - x0) - (d0 + rho[my]) * (x0 - x[ny])) +
+               Dz * ((rho[pz] + d0) * (x[qz] - x0) - (d0 + rho[mz]) * (x0 - x[nz]));
     if (fixed_bnd && (ix == 0 || y == 0 || z == 0)) g = 0.0;
     out[p] = g;
   }
@@ -1131,6 +1151,7 @@ void matrix_apply(cdft_system_s& s, const double* rho, const double* x, double* 
   const size_t bytes = sizeof(double) * N;
   const bool bnd = s.d.boundary & CDFT_FIXED_BACKGROUND, ideal = !(flags & CDFT_NO_IDEAL);
   const bool dynamic = flags & CDFT_DYNAMIC_MATRIX, squared = flags & CDFT_SQUARED_MATRIX;
+  const bool central = flags & CDFT_CENTRAL_DIFFERENCES;
   const long nbnd = boundary_count(s.g.Nx, s.g.Ny, s.g.Nz);
   const dim3 eg((unsigned)std::max<long>(1, std::min<long>(div_up(N, kThreads), 8192)));
   auto zero_bnd = [&](double* v) {
@@ -1144,9 +1165,10 @@ void matrix_apply(cdft_system_s& s, const double* rho, const double* x, double* 
     zero_bnd(r);
     if (dynamic) {
       HIPCHK(hipMemcpyAsync(t, r, bytes, hipMemcpyDeviceToDevice, s.stream));
+      const double st = central ? 2.0 : 1.0;
       hipLaunchKernelGGL(k_gdotx, eg, dim3(kThreads), 0, s.stream, rho, t, r, s.g.Nx, s.g.Ny, s.g.Nz,
-                         1.0 / (2 * s.d.dx * s.d.dx), 1.0 / (2 * s.d.dy * s.d.dy), 1.0 / (2 * s.d.dz * s.d.dz),
-                         bnd ? 1 : 0);
+                         1.0 / (2 * st * s.d.dx * s.d.dx), 1.0 / (2 * st * s.d.dy * s.d.dy),
+                         1.0 / (2 * st * s.d.dz * s.d.dz), bnd ? 1 : 0, central ? 1 : 0);
       HIPCHK(hipGetLastError());
     }
   };
@@ -1191,8 +1213,8 @@ __global__ __launch_bounds__(256) void k_cheb_step(const double* __restrict__ r,
 int cdft_system_log_det(cdft_system sys, const double* rho, int order, double lam_max, double lam_min, long seed,
                         int num_samples, int has_zero_eigenvalue, int flags, double* log_det, double* variance) {
   return guarded([&] {
-    cdft_system_s& s = *sys;
     REQUIRE(sys && rho && log_det && variance, "null argument");
+    cdft_system_s& s = *sys;
     REQUIRE(s.nb == 1, "the log-determinant takes one system, not a batch");
     REQUIRE(order >= 1 && num_samples >= 1, "order and num_samples must be positive");
     REQUIRE(lam_max > 0 && lam_min > 0 && lam_max > lam_min, "need 0 < lam_min < lam_max");
@@ -1328,8 +1350,8 @@ __global__ __launch_bounds__(256) void k_eig_grad(const double* __restrict__ r, 
 
 int cdft_system_dynamic_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags) {
   return guarded([&] {
-    cdft_system_s& s = *sys;
     REQUIRE(sys && rho && v && out, "null argument");
+    cdft_system_s& s = *sys;
     REQUIRE(s.nb == 1, "the dynamical matrix takes one system, not a batch");
     const long N = s.N;
     const size_t bytes = sizeof(double) * N;
@@ -1347,7 +1369,8 @@ int cdft_system_dynamic_dot(cdft_system sys, const double* rho, const double* v,
       v_d = h + N;
       out_d = h + 2 * N;
     }
-    matrix_apply(s, rho_d, v_d, out_d, t, (flags & (CDFT_NO_IDEAL | CDFT_SQUARED_MATRIX)) | CDFT_DYNAMIC_MATRIX);
+    matrix_apply(s, rho_d, v_d, out_d, t,
+                 (flags & (CDFT_NO_IDEAL | CDFT_SQUARED_MATRIX | CDFT_CENTRAL_DIFFERENCES)) | CDFT_DYNAMIC_MATRIX);
     if (!dev) HIPCHK(hipMemcpyAsync(out, out_d, bytes, hipMemcpyDeviceToHost, s.stream));
     const bool last = s.last_with_dft, last_fm = s.last_fixed_mass;
     s.last_fixed_mass = false;
@@ -1361,8 +1384,8 @@ int cdft_system_dynamic_dot(cdft_system sys, const double* rho, const double* v,
 int cdft_system_eigen_objective(cdft_system sys, const double* rho, const double* x, const double* vshift,
                                 double scale, int change_sign, int flags, double* f, double* df) {
   return guarded([&] {
-    cdft_system_s& s = *sys;
     REQUIRE(sys && rho && x && f && df, "null argument");
+    cdft_system_s& s = *sys;
     REQUIRE(s.nb == 1, "the eigenvalue objective takes one system, not a batch");
     const long N = s.N;
     const size_t bytes = sizeof(double) * N;
@@ -1652,19 +1675,9 @@ int cdft_system_get_weights(cdft_system sys, double* w11) {
   return guarded([&] {
     REQUIRE(sys && w11, "null argument");
     REQUIRE(sys->m > 0, "system has no FMT functional");
-    const cdft_system_desc& d = sys->d;
-    const double R = d.hsd / 2;
-    const int Sxm = 2 + int(R / d.dx), Sym = 2 + int(R / d.dy), Szm = 2 + int(R / d.dz);
-    REQUIRE(Sxm < 4 * sys->g.Nx && Sym < 4 * sys->g.Ny && Szm < 4 * sys->g.Nz, "sphere is too large for the box");
-    DevBuf cells, wreal;
-    const long ncell = (long)(Sxm + 1) * (Sym + 1) * (Szm + 1);
-    cells.alloc(sizeof(double) * 11 * ncell);
+    DevBuf wreal;
     wreal.alloc(sizeof(double) * 11 * sys->N);
-    hipLaunchKernelGGL(k_fmt_cells, dim3(div_up(ncell, 64)), dim3(64), 0, sys->stream, Sxm, Sym, Szm, d.dx, d.dy, d.dz,
-                       R, cells.as<double>());
-    hipLaunchKernelGGL(k_fmt_gather, dim3(div_up(sys->N, kThreads)), dim3(kThreads), 0, sys->stream, sys->g.Nx,
-                       sys->g.Ny, sys->g.Nz, Sxm, Sym, Szm, cells.as<double>(), wreal.as<double>());
-    HIPCHK(hipGetLastError());
+    cell_weights(*sys, sys->d.hsd, wreal.as<double>());
     HIPCHK(hipMemcpyAsync(w11, wreal.p, sizeof(double) * 11 * sys->N, hipMemcpyDeviceToHost, sys->stream));
     HIPCHK(hipStreamSynchronize(sys->stream));
     return CDFT_OK;

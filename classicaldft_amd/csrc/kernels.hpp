@@ -40,15 +40,23 @@ __device__ __forceinline__ long sidx(const Geo& g, int x, int y, int kz) { retur
 
 // Compensated (Neumaier) accumulator (include/Summation.h:15-56) and the
 // error-free combination of two of them.
+// Compensated sums need their additions unfused: a caller's multiply contracted into t = s + v would make the
+// recovered error term refer to a different sum.
+#ifndef CDFT_EXACT_FP
+#define CDFT_EXACT_FP _Pragma("clang fp contract(off)")
+#endif
+
 struct CSum {
   double s, c;
   __device__ __forceinline__ void add(double v) {
+    CDFT_EXACT_FP
     const double t = s + v;
     if (fabs(s) >= fabs(v)) c += (s - t) + v;
     else c += (v - t) + s;
     s = t;
   }
   __device__ __forceinline__ void merge(double s2, double c2) {
+    CDFT_EXACT_FP
     const double t = s + s2;
     const double bp = t - s;
     const double e = (s - (t - bp)) + (s2 - bp);  // TwoSum error

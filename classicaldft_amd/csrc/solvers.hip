@@ -154,24 +154,31 @@ __global__ void k_copy4(const double* __restrict__ a, const double* __restrict__
 // ----------------------------- DDFT kernels --------------------------------
 // r = (g . dF)/dV - Lap_h rho, with max |(g . dF)/dV|:  g_dot_x (DDFT.cpp:320-351; zero on the boundary
 // ix, iy or iz = 0 of a fixed background, :331-337), calculate_excess_RHS (:212-220), subtract_ideal_gas (:223-235).
+// With central differences (DDFT(dft, showGraphics, true), :325, 345) dF's neighbours are two sites away, the
+// link densities lose the site's own rho and D = 1/(4 dx^2) (passed in Dx, Dy, Dz); the density's neighbours stay
+// one site away, as in the ideal-gas Laplacian.
 __global__ void k_ddft_rhs(const double* __restrict__ rho, const double* __restrict__ dF, double* __restrict__ out,
                            int Nx, int Ny, int Nz, double Dx, double Dy, double Dz, double Lx, double Ly, double Lz,
-                           double invdV, int fixed_bnd, double2* __restrict__ partials) {
+                           double invdV, int fixed_bnd, int central, double2* __restrict__ partials) {
   double mx[1] = {0.0};
   const long N = (long)Nx * Ny * Nz;
+  const int st = central ? 2 : 1;
   for (long p = (long)blockIdx.x * kThreads + threadIdx.x; p < N; p += (long)gridDim.x * kThreads) {
     const int z = p % Nz, y = (p / Nz) % Ny, x = p / ((long)Nz * Ny);
     const long px = ((long)((x + 1) % Nx) * Ny + y) * Nz + z, mxp = ((long)((x + Nx - 1) % Nx) * Ny + y) * Nz + z;
     const long py = ((long)x * Ny + (y + 1) % Ny) * Nz + z, my = ((long)x * Ny + (y + Ny - 1) % Ny) * Nz + z;
     const long pz = ((long)x * Ny + y) * Nz + (z + 1) % Nz, mz = ((long)x * Ny + y) * Nz + (z + Nz - 1) % Nz;
-    const double x0 = dF[p], d0 = rho[p];
-    double gx = Dx * ((rho[px] + d0) * (dF[px] - x0) - (d0 + rho[mxp]) * (x0 - dF[mxp])) +
-                Dy * ((rho[py] + d0) * (dF[py] - x0) - (d0 + rho[my]) * (x0 - dF[my])) +
-                Dz * ((rho[pz] + d0) * (dF[pz] - x0) - (d0 + rho[mz]) * (x0 - dF[mz]));
+    const long qx = ((long)((x + st) % Nx) * Ny + y) * Nz + z, nxq = ((long)((x + 2 * Nx - st) % Nx) * Ny + y) * Nz + z;
+    const long qy = ((long)x * Ny + (y + st) % Ny) * Nz + z, nyq = ((long)x * Ny + (y + 2 * Ny - st) % Ny) * Nz + z;
+    const long qz = ((long)x * Ny + y) * Nz + (z + st) % Nz, nzq = ((long)x * Ny + y) * Nz + (z + 2 * Nz - st) % Nz;
+    const double x0 = dF[p], r0 = rho[p], d0 = central ? 0.0 : r0;
+    double gx = Dx * ((rho[px] + d0) * (dF[qx] - x0) - (d0 + rho[mxp]) * (x0 - dF[nxq])) +
+                Dy * ((rho[py] + d0) * (dF[qy] - x0) - (d0 + rho[my]) * (x0 - dF[nyq])) +
+                Dz * ((rho[pz] + d0) * (dF[qz] - x0) - (d0 + rho[mz]) * (x0 - dF[nzq]));
     if (fixed_bnd && (x == 0 || y == 0 || z == 0)) gx = 0.0;
     gx *= invdV;
     mx[0] = fmax(mx[0], fabs(gx));
-    gx += -Lx * (rho[px] + rho[mxp] - 2 * d0) - Ly * (rho[py] + rho[my] - 2 * d0) - Lz * (rho[pz] + rho[mz] - 2 * d0);
+    gx += -Lx * (rho[px] + rho[mxp] - 2 * r0) - Ly * (rho[py] + rho[my] - 2 * r0) - Lz * (rho[pz] + rho[mz] - 2 * r0);
     out[p] = gx;
   }
   reduce_store<0, 1>(nullptr, mx, partials);
@@ -427,6 +434,7 @@ struct cdft_fire_s {
 struct cdft_ddft_s {
   cdft_system sys = nullptr;
   long N = 0;
+  bool central = false;  // central differences in g . dF (DDFT(dft, showGraphics, central_differences))
   double dx = 0, dy = 0, dz = 0;
   double dt = 0, time = 0, tol = 1e-4, dtMax = 1.0, RHS_max = 1.0, F = 0, mass = 0;
   int successes = 0, calls = 0, restarts = 0;
@@ -467,12 +475,13 @@ struct cdft_ddft_s {
     ex_ready = true;
   }
   double excess_rhs(const double* rho, cplx<double>* R) {
-    const double D[3] = {1 / (2 * dx * dx), 1 / (2 * dy * dy), 1 / (2 * dz * dz)};
+    const double st = central ? 2.0 : 1.0;  // DDFT::g_dot_x stride, src/DDFT.cpp:325-327
+    const double D[3] = {1 / (2 * st * dx * dx), 1 / (2 * st * dy * dy), 1 / (2 * st * dz * dz)};
     const double L[3] = {1 / (dx * dx), 1 / (dy * dy), 1 / (dz * dz)};
     const Geo& g = sys->g;
     hipLaunchKernelGGL(k_ddft_rhs, vgrid(N), dim3(kThreads), 0, sys->stream, rho, dF.as<double>(), tmp.as<double>(),
                        g.Nx, g.Ny, g.Nz, D[0], D[1], D[2], L[0], L[1], L[2], 1.0 / (dx * dy * dz),
-                       (sys->d.boundary & CDFT_FIXED_BACKGROUND) ? 1 : 0, red.part.as<double2>());
+                       (sys->d.boundary & CDFT_FIXED_BACKGROUND) ? 1 : 0, central ? 1 : 0, red.part.as<double2>());
     HIPCHK(hipGetLastError());
     r2c(tmp.as<double>(), R);
     double r;
@@ -598,7 +607,7 @@ struct cdft_ddft_open_s {
     const Geo& g = sys->g;
     hipLaunchKernelGGL(k_ddft_rhs, vgrid(N), dim3(kThreads), 0, sys->stream, d, dF.as<double>(), tmp.as<double>(),
                        g.Nx, g.Ny, g.Nz, 1 / (2 * dx * dx), 1 / (2 * dy * dy), 1 / (2 * dz * dz), 1 / (dx * dx),
-                       1 / (dy * dy), 1 / (dz * dz), 1.0 / (dx * dy * dz), 0, red.part.as<double2>());
+                       1 / (dy * dy), 1 / (dz * dz), 1.0 / (dx * dy * dz), 0, 0, red.part.as<double2>());
     HIPCHK(hipGetLastError());
     pack(tmp.as<double>(), 0.0, out.as<double>());
     dst3(out.as<double>());
@@ -894,6 +903,14 @@ int cdft_ddft_set_max_time_step(cdft_ddft d, double dt_max) {
   return guarded([&] {
     REQUIRE(d && dt_max > 0, "bad argument");
     d->dtMax = dt_max;
+    return CDFT_OK;
+  });
+}
+
+int cdft_ddft_set_central_differences(cdft_ddft d, int on) {
+  return guarded([&] {
+    REQUIRE(d, "null argument");
+    d->central = on != 0;
     return CDFT_OK;
   });
 }

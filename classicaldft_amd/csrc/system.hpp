@@ -19,6 +19,7 @@
 #include "kernels2.hpp"
 #include "rows2.hpp"
 #include "weights.hpp"
+#include "weights_ref.hpp"
 #include "constraints.hpp"
 #include "dst.hpp"
 

@@ -596,11 +596,25 @@ void DFT::matrix_dot_v(const std::vector<DFT_Vec>& v, std::vector<DFT_Vec>& resu
   }
 }
 
+// Zero the boundary points ix, iy or iz = 0 (Dynamical_Matrix::set_boundary_points_to_zero).
+static void zero_boundary(const Lattice& L, DFT_Vec& v) {
+  const long Nx = L.Nx(), Ny = L.Ny(), Nz = L.Nz();
+  for (long ix = 0; ix < Nx; ++ix)
+    for (long iy = 0; iy < Ny; ++iy)
+      for (long iz = 0; iz < Nz; ++iz)
+        if (ix == 0 || iy == 0 || iz == 0) v.set(iz + Nz * (iy + Ny * ix), 0.0);
+}
+
+// Dynamical_Matrix::matrix_dot_v1 (src/Dynamical_Matrix.cpp:52-70): with a fixed background the boundary entries
+// of the input are zeroed before the product and those of the result after it.
 void DFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param) const {
+  const Species* s = species_.front();
   std::vector<DFT_Vec> vw(1), rw(1);
   vw[0].set(v);
+  if (s->is_fixed_boundary()) zero_boundary(s->getLattice(), vw[0]);
   matrix_dot_v(vw, rw, param);
   result.set(rw[0]);
+  if (s->is_fixed_boundary()) zero_boundary(s->getLattice(), result);
 }
 
 void DDFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void*) const {
@@ -608,21 +622,29 @@ void DDFT::matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void*) const {
   if (v.size() != d.Ntot()) throw std::runtime_error("DDFT::matrix_dot_v: one direction of Ntot values");
   if (result.size() != d.Ntot()) result.zeros(d.Ntot());
   check(cdft_system_dynamic_dot(dft_->system(), d.get_density_pointer(), v.memptr(), result.memptr(),
-                                use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0));
+                                (use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0) |
+                                    (dft_->full_hessian_ ? 0 : CDFT_NO_IDEAL) |
+                                    (central_ ? CDFT_CENTRAL_DIFFERENCES : 0)));
 }
 
 Log_Det::Log_Det(const DDFT& matrix, int order, double lam_max, double lam_min, bool verbose)
     : matrix_(matrix.getDFT()), order_(order), lam_max_(lam_max), lam_min_(lam_min), verbose_(verbose),
-      dynamic_(true) {}
+      dynamic_(true), ddft_(&matrix) {}
 
 Eigenvalues::Eigenvalues(const DDFT& matrix, bool verbose) : matrix_(matrix.getDFT()), verbose_(verbose),
-                                                              dynamic_(true) {}
+                                                              dynamic_(true), ddft_(&matrix) {}
+
+// Flags of the matrix a Log_Det / Eigenvalues acts on: the DFT's Hessian with its own full_hessian_ and
+// use_squared_matrix_, or the DDFT's dynamical matrix with the DDFT's use_squared_matrix_ (Dynamical_Matrix::
+// matrix_dot_v, src/Dynamical_Matrix.cpp:34-50, is the DDFT's; its intern product uses the DFT's full_hessian_).
+static int matrix_flags(bool full_hessian, bool dft_squared, const DDFT* ddft) {
+  const bool squared = ddft ? ddft->get_use_squared_matrix_() : dft_squared;
+  return (full_hessian ? 0 : CDFT_NO_IDEAL) | (squared ? CDFT_SQUARED_MATRIX : 0) | (ddft ? CDFT_DYNAMIC_MATRIX : 0) |
+         (ddft && ddft->get_central_differences() ? CDFT_CENTRAL_DIFFERENCES : 0);
+}
 
 int Eigenvalues::flags() const {
-  if (dynamic_ && matrix_.use_squared_matrix_)
-    throw std::runtime_error("Eigenvalues: the DFT's use_squared_matrix_ inside a DDFT matrix is not supported");
-  return (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0) |
-         (dynamic_ ? CDFT_DYNAMIC_MATRIX : 0);
+  return matrix_flags(matrix_.full_hessian_, matrix_.use_squared_matrix_, dynamic_ ? ddft_ : nullptr);
 }
 
 void Eigenvalues::set_eigen_vec_(const std::vector<double>& v_in) {
@@ -741,10 +763,7 @@ void Eigenvalues::calculate_eigenvector() {
 double Log_Det::calculate_log_det(long seed, int num_samples, bool has_zero_eigenvalue, double& variance) {
   const Species* s = matrix_.species_.front();
   double ld = 0.0;
-  if (dynamic_ && matrix_.use_squared_matrix_)
-    throw std::runtime_error("Log_Det: the DFT's use_squared_matrix_ inside a DDFT matrix is not supported");
-  const int flags = (matrix_.full_hessian_ ? 0 : CDFT_NO_IDEAL) | (matrix_.use_squared_matrix_ ? CDFT_SQUARED_MATRIX : 0) |
-                    (dynamic_ ? CDFT_DYNAMIC_MATRIX : 0);
+  const int flags = matrix_flags(matrix_.full_hessian_, matrix_.use_squared_matrix_, dynamic_ ? ddft_ : nullptr);
   check(cdft_system_log_det(matrix_.system(), s->getDensity().get_density_pointer(), order_, lam_max_, lam_min_, seed,
                             num_samples, has_zero_eigenvalue ? 1 : 0, flags, &ld, &variance));
   if (verbose_) std::printf("\t samples = %d log_det = %g variance = %g\n", num_samples, ld, variance);
@@ -982,7 +1001,7 @@ void fireMinimizer2::density(DFT_Vec& out) const {
   check(cdft_fire_get(fire_, out.memptr(), nullptr, nullptr, nullptr));
 }
 
-DDFT::DDFT(DFT* dft, bool) : Minimizer(dft) {
+DDFT::DDFT(DFT* dft, bool, bool central_differences) : Minimizer(dft), central_(central_differences) {
   const double dx = dft->get_lattice().getDX();
   dt_ = 0.0001 * dx * dx;  // src/DDFT.cpp:46
 }
@@ -993,6 +1012,7 @@ void DDFT::ensure() {
   if (ddft_) return;
   check(cdft_ddft_create(dft_->system(), dft_->getSpecies(0)->getDensity().get_density_pointer(), dt_, dtMax_, tol_,
                          &ddft_));
+  if (central_) check(cdft_ddft_set_central_differences(ddft_, 1));
 }
 void DDFT::set_max_time_step(double t) {
   dtMax_ = t;

@@ -77,8 +77,10 @@ enum cdft_flags {
                            cdft_system_eval; DFT::full_hessian_ false for the Hessian-vector product */,
   CDFT_SQUARED_MATRIX = 16, /* log det / eigenvalue objective / dynamical matrix: apply the matrix twice
                                (use_squared_matrix_; negated for the dynamical matrix, is_dynamic) */
-  CDFT_DYNAMIC_MATRIX = 32  /* log det / eigenvalue objective: the matrix is the DDFT dynamical matrix
+  CDFT_DYNAMIC_MATRIX = 32,  /* log det / eigenvalue objective: the matrix is the DDFT dynamical matrix
                                g . (H x) of cdft_system_dynamic_dot instead of the Hessian H */
+  CDFT_CENTRAL_DIFFERENCES = 64  /* dynamical matrix: g . x with central differences (the DDFT was built with
+                                    central_differences = true, src/DDFT.cpp:325, 345) */
 };
 
 /* Species boundary conditions (include/Species.h:24-25, applied in Species::endForceCalculation,
@@ -86,6 +88,16 @@ enum cdft_flags {
 enum cdft_boundary {
   CDFT_FIXED_BACKGROUND = 1,     /* dF = 0 on the boundary (setFixedBackground) */
   CDFT_HOMOGENEOUS_BOUNDARY = 2  /* boundary dF replaced by its mean (setHomogeneousBoundary) */
+};
+
+/* Generator of the FMT cell weights (FMT_Species::generateWeights, src/FMT_Species.cpp:310-496) */
+enum cdft_weights_mode {
+  CDFT_WEIGHTS_ACCURATE = 0,  /* default: the same integrals in a well-conditioned double-double form on the device,
+                                 correct to the last bit of each cell weight */
+  CDFT_WEIGHTS_REFERENCE = 1  /* the reference's own double expressions (asin forms, R += 1e-15 shift,
+                                 src/FMT_Species.cpp:112-308) on the host, operation for operation: a reference run's
+                                 weights wherever the libm and the floating-point contraction agree, with their
+                                 rounding noise (up to ~1e-6 relative per cell at dx = 0.1, ~1e-4 at dx = 0.04) */
 };
 
 typedef struct cdft_system_s* cdft_system;
@@ -127,6 +139,9 @@ typedef struct cdft_system_desc {
   /* factor on the mean-field kernel and a_vdw (Interaction_Base::scale_interaction, src/Interaction.cpp:54-59);
      0 means 1 */
   double mf_scale;
+  /* how generated FMT (and AO polymer) cell weights are computed when fmt_weights / ao_weights is NULL:
+     enum cdft_weights_mode */
+  int weights_mode;
 } cdft_system_desc;
 
 /* A pair potential for the host-side helpers below (fields as in cdft_system_desc). */
@@ -292,6 +307,10 @@ typedef struct cdft_ddft_report {
 int cdft_ddft_create(cdft_system sys, const double* rho0, double dt, double dt_max, double tol, cdft_ddft* out);
 int cdft_ddft_step(cdft_ddft d, cdft_ddft_report* rep);
 int cdft_ddft_set_max_time_step(cdft_ddft d, double dt_max);
+/* The third argument of DDFT(DFT*, bool showGraphics, bool central_differences) (include/Minimizer.h:192): with
+   on != 0 the excess RHS uses DDFT::g_dot_x with central differences (src/DDFT.cpp:320-351: dF's neighbours two
+   sites away, the link density without the site's own, D = 1/(4 dx^2)).  Off by default, as in the reference. */
+int cdft_ddft_set_central_differences(cdft_ddft d, int on);
 int cdft_ddft_get_density(cdft_ddft d, double* rho);
 int cdft_ddft_destroy(cdft_ddft d);
 

@@ -472,6 +472,7 @@ class DFT {
   void invalidate() const;
   friend class Log_Det;
   friend class Eigenvalues;
+  friend class DDFT;
 };
 
 class DDFT;
@@ -515,11 +516,13 @@ class Eigenvalues {
   int num_eval_ = 0, return_code_ = 0, max_eval_ = 100000;
   unsigned long seed_ = 1;
   bool dynamic_ = false;
+  const DDFT* ddft_ = nullptr;  // the DDFT whose matrix this is (dynamic_), for its use_squared_matrix_
   int flags() const;
 };
 
 // include/Log_Det.h:22-51, src/Log_Det.cpp:22-166: stochastic Chebyshev estimate of log det of the DFT's
-// Hessian (its full_hessian_ and use_squared_matrix_ settings apply), every product on the GPU.
+// Hessian (its full_hessian_ and use_squared_matrix_ settings apply), every product on the GPU.  Built on a DDFT,
+// the matrix is the DDFT's (its use_squared_matrix_, the DFT's full_hessian_).
 class Log_Det {
  public:
   Log_Det(const DFT& matrix, int order, double lam_max, double lam_min, bool verbose = false)
@@ -535,6 +538,7 @@ class Log_Det {
   int order_;
   double lam_max_, lam_min_;
   bool verbose_ = false, debug_ = false, dynamic_ = false;
+  const DDFT* ddft_ = nullptr;  // the DDFT whose matrix this is (dynamic_), for its use_squared_matrix_
 };
 
 // ------------------------------------------------------------- minimizer ---
@@ -606,12 +610,16 @@ class fireMinimizer2 : public Minimizer {
 
 class DDFT : public Minimizer {
  public:
-  explicit DDFT(DFT* dft, bool showGraphics = false);
+  // include/Minimizer.h:192; showGraphics has no effect here, central_differences selects the central-difference
+  // g . x (src/DDFT.cpp:320-351) of the steps and of the dynamical matrix
+  explicit DDFT(DFT* dft, bool showGraphics = true, bool central_differences = false);
   ~DDFT() override;
   // Dynamical_Matrix interface of the DDFT (include/Minimizer.h:211-220, src/DDFT.cpp:353-361 through
   // src/Dynamical_Matrix.cpp:34-70): result = g . (H v), applied twice and negated with use_squared_matrix_.
   void matrix_dot_v1(const DFT_Vec& v, DFT_Vec& result, void* param = nullptr) const;
   void set_use_squared_matrix_(bool b) { use_squared_matrix_ = b; }
+  bool get_use_squared_matrix_() const { return use_squared_matrix_; }
+  bool get_central_differences() const { return central_; }
   bool is_dynamic() const { return true; }
   double step() override;
   void reset() override {}  // Minimizer::reset only refreshes the FIRE aliases, which DDFT does not use
@@ -628,7 +636,7 @@ class DDFT : public Minimizer {
   void ensure();
   double dt_ = 0.0, dtMax_ = 1.0, tol_ = 1e-4;
   cdft_ddft ddft_ = nullptr;
-  bool use_squared_matrix_ = false;
+  bool use_squared_matrix_ = false, central_ = false;
   cdft_ddft_report rep_{};
 };
 

@@ -79,6 +79,7 @@ def lib():
         L.orc_dft_eig_objective.argtypes = [_p, _p, _p, _p, _d, _i, _i, _i, _p]
         L.orc_dft_eig_objective.restype = _d
         L.orc_gdotx.argtypes = [_p, _p, _p, _p]
+        L.orc_dft_set_central.argtypes = [_p, _i]
         L.orc_dft_matrix_apply.argtypes = [_p, _p, _p, _p, _i, _i]
         L.orc_dft_set_constraints.argtypes = [_p, _d, _i]
         L.orc_dft_set_constraints.restype = _i
@@ -334,6 +335,11 @@ class DFT:
         if lib().orc_dft_hess(self.h, _ptr(rho), _ptr(v), _ptr(out), int(bool(full))) == 2:
             raise ValueError("Input vector v must have zero boundary entries when the species has fixed boundaries")
         return out
+
+    def set_central_differences(self, on=True):
+        """DDFT(dft, showGraphics, central_differences): g . x of the DDFT steps and dynamical matrices with
+        central differences (src/DDFT.cpp:325, 345)."""
+        lib().orc_dft_set_central(self.h, int(bool(on)))
 
     def gdotx(self, rho, x):
         """DDFT::g_dot_x (src/DDFT.cpp:320-351) at density rho."""

@@ -31,6 +31,7 @@
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#include <quadmath.h>
 #endif
 
 namespace {
@@ -199,7 +200,7 @@ void c2r3d(const Grid& g, const cd* in, double* out) {
 // 8 Tyy, 9 Tyz, 10 Tzz.
 //
 // Two evaluations of the same integrals are provided:
-//  * "faithful" (T = double): the reference's expressions, including its
+//  * "faithful" (faithful::generate, double): the reference's expressions, including its
 //    asin(p/q) forms, the 1e-15 guards and the R += 1e-15 shift.  Where the
 //    exact argument of an asin is +-1 (a corner coordinate equal to zero),
 //    rounding leaves it at 1 - O(1e-16) and asin's square-root singularity
@@ -212,58 +213,48 @@ void c2r3d(const Grid& g, const cd* in, double* out) {
 //    asin(V/sqrt(R^2-X^2)) = atan2(V, sqrt E),
 //    asin(XV/sqrt((R^2-V^2)(R^2-X^2))) = atan2(XV, R sqrt E), and at the
 //    upper limit X = sqrt(R^2-Va^2-Vb^2) one has E_a = Vb^2, E_b = Va^2
-//    exactly.  In long double this is the oracle's accurate weight set.
+//    exactly.  In IEEE quad (__float128) this is the oracle's accurate weight
+//    set, correct to the last bit of every double cell weight.
 // ---------------------------------------------------------------------------
+// Elementary functions of the well-conditioned weights for long double and
+// for IEEE quad (__float128, libquadmath): the accurate weight set is
+// evaluated in quad, so that it serves as the reference for generators
+// accurate to the last bit of a double.
+inline long double wsqrt(long double a) { return std::sqrt(a); }
+inline long double watan2(long double y, long double x) { return std::atan2(y, x); }
+inline __float128 wsqrt(__float128 a) { return sqrtq(a); }
+inline __float128 watan2(__float128 y, __float128 x) { return atan2q(y, x); }
+inline double wsqrt(double a) { return std::sqrt(a); }
+inline double watan2(double y, double x) { return std::atan2(y, x); }
+template <typename T>
+T wpi() { return T(3.14159265358979323846264338327950288L); }
+template <>
+inline __float128 wpi<__float128>() { return M_PIq; }
+
 template <typename T>
 struct Moments {
   T Iy[5], Iz[5], Jy[5], Jz[5];
 };
 
 template <typename T>
-void I_faithful(T X, T A, T I[5]) {  // FMT_Species.cpp:112-122
-  const T a = std::asin(X / A);
-  const T b = std::sqrt(std::fabs(A * A - X * X));
-  const T A2 = A * A, X2 = X * X;
-  I[0] = T(0.5) * X * b + T(0.5) * A2 * a;
-  I[1] = -(T(1) / 3) * (A2 - X2) * b;
-  I[2] = T(0.125) * X * (2 * X2 - A2) * b + T(0.125) * A2 * A2 * a;
-  I[3] = -(T(1) / 15) * (A2 - X2) * (3 * X2 + 2 * A2) * b;
-  I[4] = (T(1) / 48) * (8 * X2 * X2 - 2 * A2 * X2 - 3 * A2 * A2) * X * b + T(0.0625) * A2 * A2 * A2 * a;
-}
-
-template <typename T>
-T clamp1(T v) { return std::max(T(-1), std::min(T(1), v)); }
-
-template <typename T>
 void J_from(T X, T V, T R, T aV, T aX, T aVX, T b, T J[5]) {  // FMT_Species.cpp:133-137
   const T R2 = R * R, V2 = V * V, X2 = X * X;
-  const T pi = T(3.14159265358979323846264338327950288L);
+  const T pi = wpi<T>();
   J[0] = X * aV + V * aX - R * aVX;
   J[1] = T(-0.5) * V * b + T(0.5) * (X2 - R2) * aV;
   J[2] = -(T(1) / 6) * V * (V2 - 3 * R2) * aX + (T(1) / 3) * X2 * X * aV - (T(1) / 3) * R2 * R * aVX - (T(1) / 6) * V * X * b;
   J[3] = T(0.25) * (X2 * X2 - R2 * R2) * aV + (pi / 8) * R2 * R2 - (V / T(12)) * (5 * R2 + X2 - 2 * V2) * b;
-  J[4] = T(0.025) * X * V * (3 * V2 - 2 * X2 - 7 * R2) * b + T(0.025) * V * (3 * V2 * V2 - 10 * V2 * R2 + 15 * R2 * R2) * aX +
-         T(0.2) * X2 * X2 * X * aV - T(0.2) * R2 * R2 * R * aVX;
-}
-
-template <typename T>
-void J_faithful(T X, T V, T R, T J[5]) {  // FMT_Species.cpp:126-132
-  const T tiny = T(1e-15);
-  const T R2 = R * R, V2 = V * V, X2 = X * X;
-  const T aV = std::asin(clamp1(V / std::max(tiny, std::sqrt(R2 - X2))));
-  const T aX = std::asin(clamp1(X / std::sqrt(R2 - V2)));
-  const T aVX = std::asin(clamp1(X * V / std::max(tiny, std::sqrt((R2 - V2) * (R2 - X2)))));
-  const T b = std::sqrt(std::fabs(R2 - V2 - X2));
-  J_from(X, V, R, aV, aX, aVX, b, J);
+  J[4] = (T(1) / 40) * X * V * (3 * V2 - 2 * X2 - 7 * R2) * b + (T(1) / 40) * V * (3 * V2 * V2 - 10 * V2 * R2 + 15 * R2 * R2) * aX +
+         (T(1) / 5) * X2 * X2 * X * aV - (T(1) / 5) * R2 * R2 * R * aVX;
 }
 
 template <typename T>
 void IJ_wc(T X, T V, T R, T E, T I[5], T J[5]) {
   if (E < 0) E = 0;
-  const T sE = std::sqrt(E);
-  const T aX = std::atan2(X, sE);
-  const T aV = std::atan2(V, sE);
-  const T aVX = std::atan2(X * V, R * sE);
+  const T sE = wsqrt(E);
+  const T aX = watan2(X, sE);
+  const T aV = watan2(V, sE);
+  const T aVX = watan2(X * V, R * sE);
   const T A2 = X * X + E;  // = R^2 - V^2
   const T X2 = X * X;
   I[0] = T(0.5) * X * sE + T(0.5) * A2 * aX;
@@ -274,27 +265,19 @@ void IJ_wc(T X, T V, T R, T E, T I[5], T J[5]) {
   J_from(X, V, R, aV, aX, aVX, sE, J);
 }
 
-// Fill the moments for G(R, X, Vy, Vz, ...).  Ey = R^2-X^2-Vy^2, Ez = R^2-X^2-Vz^2
-// (used only by the well-conditioned path).  The faithful path applies the
-// reference's R += 1e-15 (FMT_Species.cpp:156,203,231,260,288).
+// Fill the moments for G(R, X, Vy, Vz, ...).  Ey = R^2-X^2-Vy^2, Ez = R^2-X^2-Vz^2.
+// (The reference-faithful path, with its R += 1e-15 shift, is faithful::generate below.)
 template <typename T, bool WC>
 void moments(T& R, T X, T Vy, T Vz, T Ey, T Ez, Moments<T>& q) {
-  if (WC) {
-    IJ_wc(X, Vy, R, Ey, q.Iy, q.Jy);
-    IJ_wc(X, Vz, R, Ez, q.Iz, q.Jz);
-  } else {
-    R += T(1e-15);
-    I_faithful(X, std::sqrt(R * R - Vy * Vy), q.Iy);
-    I_faithful(X, std::sqrt(R * R - Vz * Vz), q.Iz);
-    J_faithful(X, Vy, R, q.Jy);
-    J_faithful(X, Vz, R, q.Jz);
-  }
+  static_assert(WC, "only the well-conditioned form is templated");
+  IJ_wc(X, Vy, R, Ey, q.Iy, q.Jy);
+  IJ_wc(X, Vz, R, Ez, q.Iz, q.Jz);
 }
 
 // G_eta, FMT_Species.cpp:148-193
 template <typename T>
 T G_eta(T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
-  const T pi = T(3.14159265358979323846264338327950288L);
+  const T pi = wpi<T>();
   const T R2 = R * R;
   const T A = Ty * Tz * Vy * Vz + T(0.25) * Vy * Vy * Vz * Vz + T(0.125) * R2 * R2 -
               (T(1) / 6) * (Ty * Vy * Vy * Vy + Tz * Vz * Vz * Vz) - T(0.5) * Tz * Vz * Vy * Vy - T(0.5) * Ty * Vy * Vz * Vz +
@@ -304,7 +287,7 @@ T G_eta(T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
   const T C = T(0.5) * Ty * Vy + (T(1) / 3) * (R2 - Vy * Vy);
   const T D = T(0.5) * Tz * Vz + (T(1) / 3) * (R2 - Vz * Vz);
   const T X2 = X * X;
-  T g = Tx * A * X - T(0.5) * A * X2 - (T(1) / 3) * B * Tx * X2 * X + T(0.25) * X2 * X2 * B + T(0.025) * Tx * X2 * X2 * X -
+  T g = Tx * A * X - T(0.5) * A * X2 - (T(1) / 3) * B * Tx * X2 * X + T(0.25) * X2 * X2 * B + (T(1) / 40) * Tx * X2 * X2 * X -
         (T(1) / 48) * X2 * X2 * X2;
   g -= T(0.5) * Ty * Tz * (Tx * R2 * q.Jy[0] - R2 * q.Jy[1] - Tx * q.Jy[2] + q.Jy[3]);
   g -= T(0.5) * Ty * Tz * (Tx * R2 * q.Jz[0] - R2 * q.Jz[1] - Tx * q.Jz[2] + q.Jz[3]);
@@ -317,16 +300,16 @@ T G_eta(T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
 // orders (o, o+1); s -> 0, vx -> 1, txx -> 2 (FMT_Species.cpp:195-278).
 template <typename T>
 T G_moment(int o, T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
-  const T pi = T(3.14159265358979323846264338327950288L);
+  const T pi = wpi<T>();
   const T A = Ty * Vy + Tz * Vz - T(0.5) * Vy * Vy - T(0.5) * Vz * Vz + T(0.5) * R * R + (pi / 2) * Ty * Tz;
   const T X2 = X * X;
   T g;
   if (o == 0)
     g = T(0.125) * R * X2 * X2 - (T(1) / 6) * R * Tx * X2 * X - T(0.5) * R * A * X2 + R * A * Tx * X;
   else if (o == 1)
-    g = T(0.1) * R * X2 * X2 * X - T(0.125) * R * Tx * X2 * X2 - (T(1) / 3) * R * A * X2 * X + T(0.5) * R * A * Tx * X2;
+    g = (T(1) / 10) * R * X2 * X2 * X - T(0.125) * R * Tx * X2 * X2 - (T(1) / 3) * R * A * X2 * X + T(0.5) * R * A * Tx * X2;
   else
-    g = (T(1) / 12) * R * X2 * X2 * X2 - T(0.1) * R * Tx * X2 * X2 * X - T(0.25) * R * A * X2 * X2 + (T(1) / 3) * R * A * Tx * X2 * X;
+    g = (T(1) / 12) * R * X2 * X2 * X2 - (T(1) / 10) * R * Tx * X2 * X2 * X - T(0.25) * R * A * X2 * X2 + (T(1) / 3) * R * A * Tx * X2 * X;
   g -= R * Tz * (Tx * q.Iy[o] - q.Iy[o + 1]);
   g -= R * Ty * (Tx * q.Iz[o] - q.Iz[o + 1]);
   g -= R * Ty * Tz * (Tx * q.Jy[o] - q.Jy[o + 1]);
@@ -337,13 +320,13 @@ T G_moment(int o, T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
 // G_txy, FMT_Species.cpp:280-308
 template <typename T>
 T G_txy(T R, T X, T Vy, T Vz, T Tx, T Ty, T Tz, const Moments<T>& q) {
-  const T pi = T(3.14159265358979323846264338327950288L);
+  const T pi = wpi<T>();
   const T R2 = R * R;
   const T A = (T(1) / 6) * (3 * Ty * R2 + 2 * Vy * Vy * Vy - 3 * Ty * Vy * Vy - 3 * Ty * Vz * Vz + 6 * Ty * Tz * Vz + T(1.5) * pi * R2 * Tz);
   const T B = T(0.25) * pi * Tz + T(0.5) * Ty;
   const T C = -(T(1) / 6) * (2 * R2 - 2 * Vz * Vz + 3 * Tz * Vz);
   const T X2 = X * X;
-  T g = T(-0.5) * R * A * Tx * X2 + (T(1) / 3) * R * A * X2 * X + T(0.25) * R * B * Tx * X2 * X2 - T(0.2) * R * B * X2 * X2 * X;
+  T g = T(-0.5) * R * A * Tx * X2 + (T(1) / 3) * R * A * X2 * X + T(0.25) * R * B * Tx * X2 * X2 - (T(1) / 5) * R * B * X2 * X2 * X;
   g += T(0.5) * R * Tz * (Tx * R2 * q.Jy[1] - R2 * q.Jy[2] - Tx * q.Jy[3] + q.Jy[4]);
   g += T(0.5) * R * Tz * (Tx * R2 * q.Jz[1] - R2 * q.Jz[2] - Tx * q.Jz[3] + q.Jz[4]);
   g += T(0.5) * R * Tz * (2 * Ty - Vy) * (Tx * q.Iy[1] - q.Iy[2]);
@@ -358,7 +341,7 @@ enum { GK_ETA, GK_S, GK_V, GK_TXX, GK_TXY };
 template <typename T, bool WC>
 T G_diff(int k, T R, T Xc, T Va, T Vb, T Tx, T Ta, T Tb) {
   const T h2 = R * R - Va * Va - Vb * Vb;
-  const T h = std::sqrt(h2);
+  const T h = wsqrt(h2);
   auto eval = [&](T X, T Ea, T Eb) {
     T r = R;
     Moments<T> q;
@@ -392,9 +375,12 @@ int gen_weights(const Grid& g, double dx_, double dy_, double dz_, double hsd, d
   const int Sxm = 2 + int(R / dx), Sym = 2 + int(R / dy), Szm = 2 + int(R / dz);
   const long N = g.N();
   const int I2[2] = {-1, 1};
-  for (int Sx = 0; Sx <= Sxm; ++Sx)
-    for (int Sy = 0; Sy <= Sym; ++Sy)
-      for (int Sz = 0; Sz <= Szm; ++Sz) {
+  // the cells in parallel (each writes its own 11 values), then the scatter in the reference's order
+  const long ncell = long(Sxm + 1) * (Sym + 1) * (Szm + 1);
+  std::vector<double> cells(11 * ncell);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long ci = 0; ci < ncell; ++ci) {
+        const int Sz = int(ci % (Szm + 1)), Sy = int((ci / (Szm + 1)) % (Sym + 1)), Sx = int(ci / (long(Szm + 1) * (Sym + 1)));
         auto lo = [](int S) { return T(S == 0 ? 0 : S - 1); };
         const T R2min = dx * dx * lo(Sx) * lo(Sx) + dy * dy * lo(Sy) * lo(Sy) + dz * dz * lo(Sz) * lo(Sz);
         const T R2max = dx * dx * T(Sx + 1) * T(Sx + 1) + dy * dy * T(Sy + 1) * T(Sy + 1) + dz * dz * T(Sz + 1) * T(Sz + 1);
@@ -423,7 +409,7 @@ int gen_weights(const Grid& g, double dx_, double dy_, double dz_, double hsd, d
                     for (int jz = 0; jz < 2; ++jz) {
                       const T Vx = Tx - dx * vx[jx], Vy = Ty - dy * vy[jy], Vz = Tz - dz * vz[jz];
                       const int sgn = ((jx + jy + jz) & 1) ? -1 : 1;
-                      if ((R * R - (Vx * Vx + Vy * Vy + Vz * Vz)) < std::nextafter(T(0), T(1))) continue;
+                      if (!(R * R - (Vx * Vx + Vy * Vy + Vz * Vz) > T(0))) continue;  // below the smallest positive double
                       const T c = T(sgn) / dV;
                       const T cr = c / (R * R);
                       we += c * G_diff<T, WC>(GK_ETA, R, Vx, Vy, Vz, Tx, Ty, Tz);
@@ -442,24 +428,216 @@ int gen_weights(const Grid& g, double dx_, double dy_, double dz_, double hsd, d
             }
           }
         }
-        // octant scatter with vector/tensor parity: FMT_Species.cpp:471-493
-        static const int TIdx[3][3] = {{5, 6, 7}, {6, 8, 9}, {7, 9, 10}};
+        double* cw = cells.data() + 11 * ci;
+        cw[0] = double(we);
+        cw[1] = double(ws);
+        for (int q = 0; q < 3; ++q) cw[2 + q] = double(wv[q]);
+        cw[5] = double(wt[0][0]);
+        cw[6] = double(wt[0][1]);
+        cw[7] = double(wt[0][2]);
+        cw[8] = double(wt[1][1]);
+        cw[9] = double(wt[1][2]);
+        cw[10] = double(wt[2][2]);
+  }
+  // octant scatter with vector/tensor parity: FMT_Species.cpp:471-493
+  for (int Sx = 0; Sx <= Sxm; ++Sx)
+    for (int Sy = 0; Sy <= Sym; ++Sy)
+      for (int Sz = 0; Sz <= Szm; ++Sz) {
+        const double* cw = cells.data() + 11 * ((long(Sx) * (Sym + 1) + Sy) * (Szm + 1) + Sz);
         for (int ox = 0; ox < (Sx == 0 ? 1 : 2); ++ox)
           for (int oy = 0; oy < (Sy == 0 ? 1 : 2); ++oy)
             for (int oz = 0; oz < (Sz == 0 ? 1 : 2); ++oz) {
-              const int sx = 1 - 2 * ox, sy = 1 - 2 * oy, sz = 1 - 2 * oz;
-              const long pos = pbc(Sz * sz, g.Nz) + g.Nz * (pbc(Sy * sy, g.Ny) + g.Ny * pbc(Sx * sx, g.Nx));
-              const int sg[3] = {sx, sy, sz};
-              w[0 * N + pos] += double(we);
-              w[1 * N + pos] += double(ws);
-              for (int a = 0; a < 3; ++a) {
-                w[(2 + a) * N + pos] += double(T(sg[a]) * wv[a]);
-                for (int b = a; b < 3; ++b) w[TIdx[a][b] * N + pos] += double(T(sg[a] * sg[b]) * wt[a][b]);
+              const int sg[3] = {1 - 2 * ox, 1 - 2 * oy, 1 - 2 * oz};
+              const long pos = pbc(Sz * sg[2], g.Nz) + g.Nz * (pbc(Sy * sg[1], g.Ny) + g.Ny * pbc(Sx * sg[0], g.Nx));
+              w[0 * N + pos] += cw[0];
+              w[1 * N + pos] += cw[1];
+              for (int q = 0; q < 3; ++q) w[(2 + q) * N + pos] += sg[q] * cw[2 + q];
+              w[5 * N + pos] += cw[5];
+              w[6 * N + pos] += sg[0] * sg[1] * cw[6];
+              w[7 * N + pos] += sg[0] * sg[2] * cw[7];
+              w[8 * N + pos] += cw[8];
+              w[9 * N + pos] += sg[1] * sg[2] * cw[9];
+              w[10 * N + pos] += cw[10];
+            }
+      }
+  return 0;
+}
+
+// Reference-faithful weights, operation for operation: getI / getJ and the five
+// G functions (src/FMT_Species.cpp:112-308) and the corner loop of
+// generateWeights (:310-496) with every expression in the reference's order of
+// evaluation (this file is built with -ffp-contract=off), so that the result is
+// the reference's on an x86-64 glibc host built without FMA contraction.
+namespace faithful {
+void getI(double X, double A, double I[]) {  // :112-122
+  double a = std::asin(X / A);
+  double b = std::sqrt(std::fabs(A * A - X * X));
+  I[0] = 0.5 * X * b + 0.5 * A * A * a;
+  I[1] = -(1.0 / 3) * (A * A - X * X) * b;
+  I[2] = 0.125 * X * (2 * X * X - A * A) * b + 0.125 * A * A * A * A * a;
+  I[3] = -(1.0 / 15) * (A * A - X * X) * (3 * X * X + 2 * A * A) * b;
+  I[4] = (1.0 / 48) * (8 * X * X * X * X - 2 * A * A * X * X - 3 * A * A * A * A) * X * b + 0.0625 * A * A * A * A * A * A * a;
+}
+void getJ(double X, double V, double R, double J[]) {  // :124-145
+  const double SMALL = 1e-15;
+  double aV = std::asin(std::max(-1.0, std::min(1.0, V / std::max(SMALL, std::sqrt(R * R - X * X)))));
+  double aX = std::asin(std::max(-1.0, std::min(1.0, X / std::sqrt(R * R - V * V))));
+  double aVX = std::asin(std::max(-1.0, std::min(1.0, X * V / std::max(SMALL, std::sqrt((R * R - V * V) * (R * R - X * X))))));
+  double b = std::sqrt(std::fabs(R * R - V * V - X * X));
+  J[0] = X * aV + V * aX - R * aVX;
+  J[1] = -0.5 * V * b + 0.5 * (X * X - R * R) * aV;
+  J[2] = -(1.0 / 6) * V * (V * V - 3 * R * R) * aX + (1.0 / 3) * X * X * X * aV - (1.0 / 3) * R * R * R * aVX - (1.0 / 6) * V * X * b;
+  J[3] = 0.25 * (X * X * X * X - R * R * R * R) * aV + (M_PI / 8) * R * R * R * R - (V / 12.0) * (5 * R * R + X * X - 2 * V * V) * b;
+  J[4] = 0.025 * X * V * (3 * V * V - 2 * X * X - 7 * R * R) * b + 0.025 * V * (3 * V * V * V * V - 10 * V * V * R * R + 15 * R * R * R * R) * aX +
+         0.2 * X * X * X * X * X * aV - 0.2 * R * R * R * R * R * aVX;
+}
+struct IJ {
+  double Iy[5], Iz[5], Jy[5], Jz[5];
+  IJ(double& R, double X, double Vy, double Vz) {
+    R += 1e-15;
+    getI(X, std::sqrt(R * R - Vy * Vy), Iy);
+    getI(X, std::sqrt(R * R - Vz * Vz), Iz);
+    getJ(X, Vy, R, Jy);
+    getJ(X, Vz, R, Jz);
+  }
+};
+double G_eta(double R, double X, double Vy, double Vz, double Tx, double Ty, double Tz) {  // :148-193
+  IJ q(R, X, Vy, Vz);
+  double A = Ty * Tz * Vy * Vz + 0.25 * Vy * Vy * Vz * Vz + 0.125 * R * R * R * R - (1.0 / 6) * (Ty * Vy * Vy * Vy + Tz * Vz * Vz * Vz) -
+             0.5 * Tz * Vz * Vy * Vy - 0.5 * Ty * Vy * Vz * Vz + 0.125 * Vy * Vy * Vy * Vy + 0.125 * Vz * Vz * Vz * Vz +
+             0.5 * R * R * (Ty * Vy + Tz * Vz - 0.5 * Vy * Vy - 0.5 * Vz * Vz + 0.5 * M_PI * Ty * Tz);
+  double B = 0.5 * Ty * Vy + 0.5 * Tz * Vz - 0.25 * Vy * Vy - 0.25 * Vz * Vz + (M_PI / 4) * Ty * Tz + 0.25 * R * R;
+  double C = 0.5 * Ty * Vy + (1.0 / 3) * (R * R - Vy * Vy);
+  double D = 0.5 * Tz * Vz + (1.0 / 3) * (R * R - Vz * Vz);
+  double g = Tx * A * X - 0.5 * A * X * X - (1.0 / 3) * B * Tx * X * X * X + 0.25 * X * X * X * X * B + 0.025 * Tx * X * X * X * X * X -
+             (1.0 / 48) * X * X * X * X * X * X;
+  g -= 0.5 * Ty * Tz * (Tx * R * R * q.Jy[0] - R * R * q.Jy[1] - Tx * q.Jy[2] + q.Jy[3]);
+  g -= 0.5 * Ty * Tz * (Tx * R * R * q.Jz[0] - R * R * q.Jz[1] - Tx * q.Jz[2] + q.Jz[3]);
+  g -= (Tx * Tz * C * q.Iy[0] - Tz * C * q.Iy[1] - (1.0 / 3) * Tx * Tz * q.Iy[2] + (1.0 / 3) * Tz * q.Iy[3]);
+  g -= (Tx * Ty * D * q.Iz[0] - Ty * D * q.Iz[1] - (1.0 / 3) * Tx * Ty * q.Iz[2] + (1.0 / 3) * Ty * q.Iz[3]);
+  return g;
+}
+double G_s(double R, double X, double Vy, double Vz, double Tx, double Ty, double Tz) {  // :195-221
+  IJ q(R, X, Vy, Vz);
+  double A = Ty * Vy + Tz * Vz - 0.5 * Vy * Vy - 0.5 * Vz * Vz + 0.5 * R * R + (M_PI / 2) * Ty * Tz;
+  double g = 0.125 * R * X * X * X * X - (1.0 / 6) * R * Tx * X * X * X - 0.5 * R * A * X * X + R * A * Tx * X;
+  g -= R * Tz * (Tx * q.Iy[0] - q.Iy[1]);
+  g -= R * Ty * (Tx * q.Iz[0] - q.Iz[1]);
+  g -= R * Ty * Tz * (Tx * q.Jy[0] - q.Jy[1]);
+  g -= R * Ty * Tz * (Tx * q.Jz[0] - q.Jz[1]);
+  return g;
+}
+double G_vx(double R, double X, double Vy, double Vz, double Tx, double Ty, double Tz) {  // :223-250
+  IJ q(R, X, Vy, Vz);
+  double A = Ty * Vy + Tz * Vz - 0.5 * Vy * Vy - 0.5 * Vz * Vz + 0.5 * R * R + (M_PI / 2) * Ty * Tz;
+  double g = 0.1 * R * X * X * X * X * X - 0.125 * R * Tx * X * X * X * X - (1.0 / 3) * R * A * X * X * X + 0.5 * R * A * Tx * X * X;
+  g -= R * Tz * (Tx * q.Iy[1] - q.Iy[2]);
+  g -= R * Ty * (Tx * q.Iz[1] - q.Iz[2]);
+  g -= R * Ty * Tz * (Tx * q.Jy[1] - q.Jy[2]);
+  g -= R * Ty * Tz * (Tx * q.Jz[1] - q.Jz[2]);
+  return g;
+}
+double G_txx(double R, double X, double Vy, double Vz, double Tx, double Ty, double Tz) {  // :252-278
+  IJ q(R, X, Vy, Vz);
+  double A = Ty * Vy + Tz * Vz - 0.5 * Vy * Vy - 0.5 * Vz * Vz + 0.5 * R * R + (M_PI / 2) * Ty * Tz;
+  double g = (1.0 / 12) * R * X * X * X * X * X * X - 0.1 * R * Tx * X * X * X * X * X - 0.25 * R * A * X * X * X * X +
+             (1.0 / 3) * R * A * Tx * X * X * X;
+  g -= R * Tz * (Tx * q.Iy[2] - q.Iy[3]);
+  g -= R * Ty * (Tx * q.Iz[2] - q.Iz[3]);
+  g -= R * Ty * Tz * (Tx * q.Jy[2] - q.Jy[3]);
+  g -= R * Ty * Tz * (Tx * q.Jz[2] - q.Jz[3]);
+  return g;
+}
+double G_txy(double R, double X, double Vy, double Vz, double Tx, double Ty, double Tz) {  // :280-308
+  IJ q(R, X, Vy, Vz);
+  double A = (1.0 / 6) * (3 * Ty * R * R + 2 * Vy * Vy * Vy - 3 * Ty * Vy * Vy - 3 * Ty * Vz * Vz + 6 * Ty * Tz * Vz + 1.5 * M_PI * R * R * Tz);
+  double B = 0.25 * M_PI * Tz + 0.5 * Ty;
+  double C = -(1.0 / 6) * (2 * R * R - 2 * Vz * Vz + 3 * Tz * Vz);
+  double g = -0.5 * R * A * Tx * X * X + (1.0 / 3) * R * A * X * X * X + 0.25 * R * B * Tx * X * X * X * X - 0.2 * R * B * X * X * X * X * X;
+  g += 0.5 * R * Tz * (Tx * R * R * q.Jy[1] - R * R * q.Jy[2] - Tx * q.Jy[3] + q.Jy[4]);
+  g += 0.5 * R * Tz * (Tx * R * R * q.Jz[1] - R * R * q.Jz[2] - Tx * q.Jz[3] + q.Jz[4]);
+  g += 0.5 * R * Tz * (2 * Ty - Vy) * (Tx * q.Iy[1] - q.Iy[2]);
+  g += R * (-Tx * C * q.Iz[1] + C * q.Iz[2] - (1.0 / 3) * Tx * q.Iz[3] + (1.0 / 3) * q.Iz[4]);
+  return g;
+}
+// generateWeights, :310-496
+int generate(const Grid& g, double dx, double dy, double dz, double hsd, double* w) {
+  double dV = dx * dy * dz;
+  double R = hsd / 2;
+  if (hsd < std::min(std::min(dx, dy), dz)) return -1;
+  int Sx_max = 2 + int(R / dx), Sy_max = 2 + int(R / dy), Sz_max = 2 + int(R / dz);
+  const long N = g.N();
+  int I[2] = {-1, 1};
+  for (int Sx = 0; Sx <= Sx_max; Sx++)
+    for (int Sy = 0; Sy <= Sy_max; Sy++)
+      for (int Sz = 0; Sz <= Sz_max; Sz++) {
+        double R2_min = dx * dx * (Sx - (Sx == 0 ? 0 : 1)) * (Sx - (Sx == 0 ? 0 : 1)) + dy * dy * (Sy - (Sy == 0 ? 0 : 1)) * (Sy - (Sy == 0 ? 0 : 1)) +
+                        dz * dz * (Sz - (Sz == 0 ? 0 : 1)) * (Sz - (Sz == 0 ? 0 : 1));
+        double R2_max = dx * dx * (Sx + 1) * (Sx + 1) + dy * dy * (Sy + 1) * (Sy + 1) + dz * dz * (Sz + 1) * (Sz + 1);
+        double w_eta = 0.0, w_s = 0.0, w_v[3] = {0.0, 0.0, 0.0}, w_T[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        if (R * R > R2_max) {
+          w_eta = dV;
+        } else if (R * R > R2_min)
+          for (int ax : I) {
+            int ix = ax;
+            double Tx = dx * (Sx + ix);
+            int px = (Tx < 0 ? -1 : 1);
+            if (Tx < 0) { ix = 1; Tx = dx; }
+            for (int ay : I) {
+              int iy = ay;
+              double Ty = dy * (Sy + iy);
+              int py = (Ty < 0 ? -1 : 1);
+              if (Ty < 0) { iy = 1; Ty = dy; }
+              for (int az : I) {
+                int iz = az;
+                double Tz = dz * (Sz + iz);
+                int pz = (Tz < 0 ? -1 : 1);
+                if (Tz < 0) { iz = 1; Tz = dz; }
+                int vx[2] = {0, ix}, vy[2] = {0, iy}, vz[2] = {0, iz};
+                for (int jx = 0; jx < 2; jx++)
+                  for (int jy = 0; jy < 2; jy++)
+                    for (int jz = 0; jz < 2; jz++) {
+                      double Vx = Tx - dx * vx[jx];
+                      double Vy = Ty - dy * vy[jy];
+                      double Vz = Tz - dz * vz[jz];
+                      int sgn = 1;
+                      if (jx == 1) sgn *= -1;
+                      if (jy == 1) sgn *= -1;
+                      if (jz == 1) sgn *= -1;
+                      if ((R * R - (Vx * Vx + Vy * Vy + Vz * Vz)) < std::nextafter(0.0, 1.0)) continue;
+                      w_eta += (1 / dV) * sgn * (G_eta(R, Vx, Vy, Vz, Tx, Ty, Tz) - G_eta(R, std::sqrt(R * R - Vy * Vy - Vz * Vz), Vy, Vz, Tx, Ty, Tz));
+                      w_s += (1 / dV) * sgn * (G_s(R, Vx, Vy, Vz, Tx, Ty, Tz) - G_s(R, std::sqrt(R * R - Vy * Vy - Vz * Vz), Vy, Vz, Tx, Ty, Tz));
+                      w_v[0] += (1 / dV) * px * (1.0 / R) * sgn * (G_vx(R, Vx, Vy, Vz, Tx, Ty, Tz) - G_vx(R, std::sqrt(R * R - Vy * Vy - Vz * Vz), Vy, Vz, Tx, Ty, Tz));
+                      w_v[1] += (1 / dV) * py * (1.0 / R) * sgn * (G_vx(R, Vy, Vx, Vz, Ty, Tx, Tz) - G_vx(R, std::sqrt(R * R - Vx * Vx - Vz * Vz), Vx, Vz, Ty, Tx, Tz));
+                      w_v[2] += (1 / dV) * pz * (1.0 / R) * sgn * (G_vx(R, Vz, Vy, Vx, Tz, Ty, Tx) - G_vx(R, std::sqrt(R * R - Vy * Vy - Vx * Vx), Vy, Vx, Tz, Ty, Tx));
+                      w_T[0][0] += (1 / dV) * (1.0 / (R * R)) * sgn * (G_txx(R, Vx, Vy, Vz, Tx, Ty, Tz) - G_txx(R, std::sqrt(R * R - Vy * Vy - Vz * Vz), Vy, Vz, Tx, Ty, Tz));
+                      w_T[1][1] += (1 / dV) * (1.0 / (R * R)) * sgn * (G_txx(R, Vy, Vx, Vz, Ty, Tx, Tz) - G_txx(R, std::sqrt(R * R - Vx * Vx - Vz * Vz), Vx, Vz, Ty, Tx, Tz));
+                      w_T[2][2] += (1 / dV) * (1.0 / (R * R)) * sgn * (G_txx(R, Vz, Vy, Vx, Tz, Ty, Tx) - G_txx(R, std::sqrt(R * R - Vy * Vy - Vx * Vx), Vy, Vx, Tz, Ty, Tx));
+                      w_T[0][1] += (1 / dV) * px * py * (1.0 / (R * R)) * sgn * (G_txy(R, Vx, Vy, Vz, Tx, Ty, Tz) - G_txy(R, std::sqrt(R * R - Vy * Vy - Vz * Vz), Vy, Vz, Tx, Ty, Tz));
+                      w_T[0][2] += (1 / dV) * px * pz * (1.0 / (R * R)) * sgn * (G_txy(R, Vx, Vz, Vy, Tx, Tz, Ty) - G_txy(R, std::sqrt(R * R - Vy * Vy - Vz * Vz), Vz, Vy, Tx, Tz, Ty));
+                      w_T[1][2] += (1 / dV) * py * pz * (1.0 / (R * R)) * sgn * (G_txy(R, Vy, Vz, Vx, Ty, Tz, Tx) - G_txy(R, std::sqrt(R * R - Vz * Vz - Vx * Vx), Vz, Vx, Ty, Tz, Tx));
+                    }
+              }
+            }
+          }
+        static const int TI[3][3] = {{5, 6, 7}, {6, 8, 9}, {7, 9, 10}};
+        for (int ix = 0; ix < (Sx == 0 ? 1 : 2); ix++)
+          for (int iy = 0; iy < (Sy == 0 ? 1 : 2); iy++)
+            for (int iz = 0; iz < (Sz == 0 ? 1 : 2); iz++) {
+              long pos = pbc((1 - 2 * iz) * Sz, g.Nz) + g.Nz * (pbc((1 - 2 * iy) * Sy, g.Ny) + g.Ny * pbc((1 - 2 * ix) * Sx, g.Nx));
+              w[0 * N + pos] += w_eta;
+              w[1 * N + pos] += w_s;
+              for (int iv = 0; iv < 3; iv++) {
+                w[(2 + iv) * N + pos] += (iv == 0 ? (1 - 2 * ix) : (iv == 1 ? (1 - 2 * iy) : (1 - 2 * iz))) * w_v[iv];
+                for (int it = iv; it < 3; it++)
+                  w[TI[iv][it] * N + pos] += (iv == 0 ? (1 - 2 * ix) : (iv == 1 ? (1 - 2 * iy) : (1 - 2 * iz))) *
+                                             (it == 0 ? (1 - 2 * ix) : (it == 1 ? (1 - 2 * iy) : (1 - 2 * iz))) * w_T[iv][it];
               }
             }
       }
   return 0;
 }
+}  // namespace faithful
 
 // ---------------------------------------------------------------------------
 // Fundamental measures and the Phi models: include/Fundamental_Measures.h:5-242,
@@ -797,7 +975,7 @@ void fmt_setup(FmtState& s, const double* wreal) {
   std::vector<double> w;
   if (!wreal) {
     w.assign(11 * N, 0.0);
-    gen_weights<double, false>(g, s.dx, s.dy, s.dz, s.hsd, w.data());
+    faithful::generate(g, s.dx, s.dy, s.dz, s.hsd, w.data());
     wreal = w.data();
   }
   s.wk.assign(11 * Nk, cd(0, 0));
@@ -814,7 +992,7 @@ void weight_spectra(const Grid& g, double dx, double dy, double dz, double hsd, 
   std::vector<double> w;
   if (!wreal) {
     w.assign(11 * N, 0.0);
-    gen_weights<double, false>(g, dx, dy, dz, hsd, w.data());
+    faithful::generate(g, dx, dy, dz, hsd, w.data());
     wreal = w.data();
   }
   wk.assign(11 * Nk, cd(0, 0));
@@ -1251,7 +1429,35 @@ struct DftState {
   double fixed_mass = 0.0;
   int bnd = 0;
   double lagrange = 0.0;
+  // DDFT(dft, showGraphics, central_differences) (include/Minimizer.h:192): g . x with central differences
+  bool central = false;
 };
+
+// One site of DDFT::g_dot_x (src/DDFT.cpp:320-351): the neighbours of x at distance stride (2 with central
+// differences, :325), those of the density at distance 1 (Density::get_neighbors), d0 = rho at the site or 0 with
+// central differences (:345), D = 1 / (2 stride dx^2) per axis (:327); periodic.  No boundary handling.
+double gdotx_site(const Grid& g, const double* rho, const double* x, int ix, int iy, int iz, const double h[3],
+                  bool central) {
+  const int st = central ? 2 : 1;
+  auto P = [&](int a, int b, int c) {
+    a = ((a % g.Nx) + g.Nx) % g.Nx;
+    b = ((b % g.Ny) + g.Ny) % g.Ny;
+    c = ((c % g.Nz) + g.Nz) % g.Nz;
+    return ((long)a * g.Ny + b) * g.Nz + c;
+  };
+  const long p = P(ix, iy, iz);
+  const double x0 = x[p], d0 = central ? 0.0 : rho[p];
+  const int e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double gx = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double D = 1 / (2 * st * h[a] * h[a]);
+    const double dp = rho[P(ix + e[a][0], iy + e[a][1], iz + e[a][2])], dm = rho[P(ix - e[a][0], iy - e[a][1], iz - e[a][2])];
+    const double xp = x[P(ix + st * e[a][0], iy + st * e[a][1], iz + st * e[a][2])];
+    const double xm = x[P(ix - st * e[a][0], iy - st * e[a][1], iz - st * e[a][2])];
+    gx += D * ((dp + d0) * (xp - x0) - (d0 + dm) * (x0 - xm));
+  }
+  return gx;
+}
 
 // Lattice::is_boundary_point (include/Lattice.h:189-194): ix, iy or iz = 0.
 inline bool boundary_point(const Grid& g, long p) {
@@ -1398,14 +1604,14 @@ void orc_c2r3d(int Nx, int Ny, int Nz, const double* in, double* out) {
   c2r3d(g, reinterpret_cast<const cd*>(in), out);
 }
 
-// mode 0: reference-faithful (double); 1: well-conditioned, long double;
+// mode 0: reference-faithful (double); 1: well-conditioned, IEEE quad;
 // 2: well-conditioned, double.
 int orc_fmt_weights(int Nx, int Ny, int Nz, double dx, double dy, double dz, double hsd, double* w, int mode) {
   Grid g{Nx, Ny, Nz};
   std::fill(w, w + 11 * g.N(), 0.0);
-  if (mode == 1) return gen_weights<long double, true>(g, dx, dy, dz, hsd, w);
+  if (mode == 1) return gen_weights<__float128, true>(g, dx, dy, dz, hsd, w);
   if (mode == 2) return gen_weights<double, true>(g, dx, dy, dz, hsd, w);
-  return gen_weights<double, false>(g, dx, dy, dz, hsd, w);
+  return faithful::generate(g, dx, dy, dz, hsd, w);
 }
 
 // Phi and the 19 derivatives at one set of measures (input in get_as_vector order).
@@ -1569,7 +1775,7 @@ orc_dft* orc_dft_create(int kind, double A, double B, int Nx, int Ny, int Nz, do
   const long N = s->fmt.g.N();
   if (has_fmt == 2) {  // accurate (well-conditioned, long double) weights
     std::vector<double> w(11 * N, 0.0);
-    gen_weights<long double, true>(s->fmt.g, dx, dy, dz, hsd, w.data());
+    gen_weights<__float128, true>(s->fmt.g, dx, dy, dz, hsd, w.data());
     fmt_setup(s->fmt, w.data());
   } else if (s->has_fmt) {
     fmt_setup(s->fmt, nullptr);
@@ -1620,30 +1826,24 @@ int orc_dft_hess(orc_dft* h, const double* rho, const double* v, double* out, in
 }
 
 // g . x: DDFT::g_dot_x (src/DDFT.cpp:320-351) with get_neighbors (:294-312): forward differences, the density
-// averaged over each link, D = 1 / (2 dx^2) per axis, periodic; zero on the boundary with a fixed background.
+// averaged over each link, D = 1 / (2 dx^2) per axis (central differences: gdotx_site), periodic; zero on the
+// boundary with a fixed background.
 void orc_gdotx(orc_dft* h, const double* rho, const double* x, double* out) {
   auto* s = reinterpret_cast<DftState*>(h);
   const Grid& g = s->fmt.g;
-  const double D[3] = {1 / (2 * s->fmt.dx * s->fmt.dx), 1 / (2 * s->fmt.dy * s->fmt.dy),
-                       1 / (2 * s->fmt.dz * s->fmt.dz)};
-  auto P = [&](int ix, int iy, int iz) {
-    ix = (ix + g.Nx) % g.Nx;
-    iy = (iy + g.Ny) % g.Ny;
-    iz = (iz + g.Nz) % g.Nz;
-    return ((long)ix * g.Ny + iy) * g.Nz + iz;
-  };
+  const double hh[3] = {s->fmt.dx, s->fmt.dy, s->fmt.dz};
   for (int ix = 0; ix < g.Nx; ++ix)
     for (int iy = 0; iy < g.Ny; ++iy)
       for (int iz = 0; iz < g.Nz; ++iz) {
-        const long p = P(ix, iy, iz);
-        const long q[6] = {P(ix + 1, iy, iz), P(ix - 1, iy, iz), P(ix, iy + 1, iz),
-                           P(ix, iy - 1, iz), P(ix, iy, iz + 1), P(ix, iy, iz - 1)};
-        double gx = 0.0;
-        for (int a = 0; a < 3; ++a)
-          gx += D[a] * ((rho[q[2 * a]] + rho[p]) * (x[q[2 * a]] - x[p]) - (rho[p] + rho[q[2 * a + 1]]) * (x[p] - x[q[2 * a + 1]]));
+        const long p = ((long)ix * g.Ny + iy) * g.Nz + iz;
+        const double gx = gdotx_site(g, rho, x, ix, iy, iz, hh, s->central);
         out[p] = ((s->bnd & 1) && (ix == 0 || iy == 0 || iz == 0)) ? 0.0 : gx;
       }
 }
+
+// DDFT(dft, showGraphics, central_differences): the DDFT steps and dynamical matrices of this DFT use central
+// differences in g . x.
+void orc_dft_set_central(orc_dft* h, int on) { reinterpret_cast<DftState*>(h)->central = on != 0; }
 
 // r = A x as Dynamical_Matrix::matrix_dot_v1 applies it (src/Dynamical_Matrix.cpp:34-70): A = the Hessian
 // (DFT::matrix_dot_v_intern) or, mode bit 2, the DDFT dynamical matrix g . (H x) (DDFT::matrix_dot_v_intern,
@@ -2007,7 +2207,7 @@ struct Ddft {
   // (subtract_ideal_gas :223-235), then r2c (calculate_excess_RHS :212-220).
   double excess_rhs(const std::vector<double>& d, std::vector<cd>& out) {
     const long N = g.N();
-    const double D[3] = {1 / (2 * dx * dx), 1 / (2 * dy * dy), 1 / (2 * dz * dz)};
+    const double hh[3] = {dx, dy, dz};
     const double L[3] = {1 / (dx * dx), 1 / (dy * dy), 1 / (dz * dz)};
     const double dV = dx * dy * dz;
     std::vector<double> r(N);
@@ -2016,12 +2216,7 @@ struct Ddft {
       for (int iy = 0; iy < g.Ny; ++iy)
         for (int iz = 0; iz < g.Nz; ++iz) {
           const long p = P(ix, iy, iz);
-          const long px = P(ix + 1, iy, iz), mx = P(ix - 1, iy, iz), py = P(ix, iy + 1, iz), my = P(ix, iy - 1, iz),
-                     pz = P(ix, iy, iz + 1), mz = P(ix, iy, iz - 1);
-          const double x0 = dF[p], d0 = d[p];
-          double gx = D[0] * ((d[px] + d0) * (dF[px] - x0) - (d0 + d[mx]) * (x0 - dF[mx])) +
-                      D[1] * ((d[py] + d0) * (dF[py] - x0) - (d0 + d[my]) * (x0 - dF[my])) +
-                      D[2] * ((d[pz] + d0) * (dF[pz] - x0) - (d0 + d[mz]) * (x0 - dF[mz]));
+          double gx = gdotx_site(g, d.data(), dF.data(), ix, iy, iz, hh, dft->central);
           if ((dft->bnd & 1) && (ix == 0 || iy == 0 || iz == 0)) gx = 0.0;  // g_dot_x, DDFT.cpp:331-337
           gx *= 1.0 / dV;
           rmax = std::max(rmax, std::fabs(gx));

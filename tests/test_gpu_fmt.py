@@ -5,8 +5,9 @@ max-norm dF.  The weights are an input of the
 comparison: the reference's double-precision weight generator carries
 roundoff that alone moves F by ~1e-10 at dx = 0.1 (see
 tests/test_oracle_weights.py), so the FP64 parity cases hand both sides the
-same weights, and a separate case checks the device generator against the
-oracle's accurate (long double, well-conditioned) weights.
+same weights, and separate cases check the device generator against the
+oracle's accurate weights (well-conditioned integrals in IEEE quad), and the
+reference-expression mode against the oracle's reference-faithful weights.
 """
 import numpy as np
 import pytest
@@ -86,20 +87,74 @@ def test_long_axes_fp64_parity(cdft, shape):
     assert maxrel(dF1, dF0) <= TOL
 
 
-def test_device_weights_match_accurate_oracle(cdft):
-    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
+@pytest.mark.parametrize("shape,dx", [((32, 32, 32), 0.1), ((64, 64, 64), 0.04), ((48, 36, 64), 0.05)])
+def test_device_weights_match_accurate_oracle(cdft, shape, dx):
+    """The double-double device generator against the quad oracle, cell by cell.  In plain double the
+    cancelling corner sums leave 4e-11 dV (dx = 0.1) .. 1e-8 dV (dx = 0.04); here every cell is the correctly
+    rounded double up to the last bit."""
+    hsd = 1.0
     s = cdft.System(shape, dx, hsd)
     wg = s.device_weights()
     wa = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_ACCURATE)
     dV = dx ** 3
     err = np.abs(wg - wa).max() / dV
-    print("max |w_gpu - w_accurate| / dV =", err)
-    # each cell value is a difference of O(R^6) G terms divided by dV, so
-    # double precision leaves ~1e-11 dV; the long-double oracle is ~1e-15 dV
-    assert err < 1e-10
+    print(shape, dx, "max |w_gpu - w_accurate| / dV =", err)
+    assert err <= 1e-15
     # sum rules of the trilinear-hat partition of unity
-    assert abs(wg[0].sum() / (np.pi * hsd ** 3 / 6) - 1) < 1e-12
-    assert abs(wg[1].sum() / (np.pi * hsd ** 2) - 1) < 1e-12
+    assert abs(wg[0].sum() / (np.pi * hsd ** 3 / 6) - 1) < 1e-13
+    assert abs(wg[1].sum() / (np.pi * hsd ** 2) - 1) < 1e-13
+
+
+@pytest.mark.parametrize("shape,dx", [((32, 32, 32), 0.1), ((48, 48, 48), 0.04)])
+def test_reference_weights_mode_matches_faithful_oracle(cdft, shape, dx):
+    """CDFT_WEIGHTS_REFERENCE: the reference's own double expressions (asin forms, R += 1e-15), evaluated on the
+    host operation for operation, equal the oracle's reference-faithful transcription bit for bit, noise
+    included; with them the drop-in reproduces the oracle's reference-faithful F and dF."""
+    hsd = 1.0
+    s = cdft.System(shape, dx, hsd, weights_mode=cdft.WEIGHTS_REFERENCE)
+    wr = s.device_weights()
+    wf = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_FAITHFUL)
+    assert np.array_equal(wr, wf)
+    wa = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_ACCURATE)
+    assert np.abs(wr - wa).max() > 1e-12 * dx ** 3  # the reference's rounding noise is there
+    rho = seeded_density(shape, 0.6, 0x5EED0001)
+    F0, dF0 = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=wf)
+    F1, dF1 = s.fmt_eval(rho)
+    print(shape, "reference-mode F", rel(F1, F0), "dF", maxrel(dF1, dF0))
+    assert rel(F1, F0) <= TOL
+    assert maxrel(dF1, dF0) <= TOL
+
+
+def test_metric_grid_parity_shared_faithful_weights(cdft):
+    """The metric workload: 256^3, dx = 0.04, White Bear I, rho = 0.6 (1 + 0.05 U[-1,1]), seed 0x5EED0256, both
+    sides on the reference-faithful weights."""
+    shape, dx, hsd = (256, 256, 256), 0.04, 1.0
+    rho = seeded_density(shape, 0.6, 0x5EED0256)
+    w = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_FAITHFUL)
+    F0, dF0 = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=w)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, weights=w)
+    del w
+    F1, dF1 = s.fmt_eval(rho)
+    s.close()
+    print("256^3 shared weights F", F0, F1, rel(F1, F0), "dF", maxrel(dF1, dF0))
+    assert rel(F1, F0) <= TOL
+    assert maxrel(dF1, dF0) <= TOL
+
+
+def test_metric_grid_parity_own_weights(cdft):
+    """The metric workload with the product's own device-generated weights against the oracle on its accurate
+    (quad) weights."""
+    shape, dx, hsd = (256, 256, 256), 0.04, 1.0
+    rho = seeded_density(shape, 0.6, 0x5EED0256)
+    wa = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_ACCURATE)
+    F0, dF0 = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=wa)
+    del wa
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I)
+    F1, dF1 = s.fmt_eval(rho)
+    s.close()
+    print("256^3 own weights F", F0, F1, rel(F1, F0), "dF", maxrel(dF1, dF0))
+    assert rel(F1, F0) <= TOL
+    assert maxrel(dF1, dF0) <= TOL
 
 
 def test_config1_fp64_parity_own_weights(cdft):

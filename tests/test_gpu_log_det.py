@@ -75,6 +75,21 @@ def test_log_det_rejects_bad_arguments(cdft):
             s.log_det(rho, *args, 1, 1)
 
 
+def test_dynamic_matrix_central_differences(cdft):
+    """The dynamical matrix of a DDFT built with central differences (CDFT_CENTRAL_DIFFERENCES) vs the oracle."""
+    shape, dx = (16, 12, 24), 0.1
+    rho = density(shape, 0x1E1)
+    w = oracle.fmt_weights(shape, dx, 1.0, mode=oracle.WEIGHTS_ACCURATE)
+    o = oracle.DFT(shape, dx, 1.0, model=oracle.WBI, mf=True, kT=3.0, accurate_weights=True)
+    o.set_central_differences(True)
+    s = cdft.System(shape, dx, 1.0, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mean_field=True, kT=3.0)
+    v = np.random.default_rng(5).standard_normal(shape)
+    for squared in (False, True):
+        y0 = o.dynamic_dot(rho, v, squared=squared)
+        y1 = s.dynamic_dot(rho, v, squared=squared, central=True)
+        assert np.max(np.abs(y1 - y0)) <= 1e-11 * np.max(np.abs(y0))
+
+
 @pytest.mark.parametrize("bnd,squared", [(False, False), (True, False), (False, True)])
 def test_dynamic_matrix_matches_oracle(cdft, bnd, squared):
     """DDFT::matrix_dot_v_intern (src/DDFT.cpp:353-361) = g . (H v) on the device vs the oracle (pinned in

@@ -105,6 +105,50 @@ def test_ddft_replay_and_mass(cdft):
     assert np.abs(r0 - r1).max() <= 1e-10 * np.abs(r0).max()
 
 
+def test_ddft_central_differences_replay(cdft):
+    """DDFT(dft, showGraphics, central_differences = true) (include/Minimizer.h:192): the excess RHS through the
+    central-difference g . dF (src/DDFT.cpp:320-351); the device replays the oracle step for step."""
+    shape, dx, hsd = (16, 16, 16), 0.1, 1.0
+    rho0 = density(shape, 0.5, 23, 0.1)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    o = oracle.DFT(shape, dx, hsd, mf=True, kT=1.0)
+    o.set_central_differences(True)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, weights=w, ideal=True, mean_field=True, kT=1.0)
+    do = oracle.DDFT(o, rho0)
+    dg = cdft.DDFT(s, rho0, central_differences=True)
+    dp = cdft.DDFT(s, rho0)
+    m0 = rho0.sum() * dx ** 3
+    for i in range(3):
+        a = do.step()
+        b = dg.step()
+        dp.step()
+        assert a["calls"] == b["calls"] and a["restarts"] == b["restarts"]
+        assert close(a["F"], b["F"], 1e-10) and close(a["dt"], b["dt"], 1e-12)
+        assert abs(b["mass"] / m0 - 1) < 1e-12
+    r0, r1 = do.density(), dg.density()
+    assert np.abs(r0 - r1).max() <= 1e-10 * np.abs(r0).max()
+    assert np.abs(dp.density() - r1).max() > 1e-6 * np.abs(r0).max()  # the stride matters
+
+
+def test_ddft_mean_field_linear_response(cdft):
+    """The linear-response pin of tests/test_oracle_ddft.py on the device: a density mode of the ideal gas with a
+    mean-field kernel decays at Lambda_k (1 + rho0 w_k), to second order in dt, and the device replays the oracle."""
+    import test_oracle_ddft as lr
+    w, rate, mode = lr.mode_setup()
+    errs = []
+    for dt in (4e-3, 2e-3):
+        s = cdft.System(lr.SHAPE, lr.DX, 1.0, model=cdft.MODEL_NONE, ideal=True, mean_field=True, mf_kernel=w)
+        d = cdft.DDFT(s, lr.RHO0 * (1 + lr.EPS * mode), dt=dt, dt_max=dt, tol=1e-14)
+        for _ in range(int(round(lr.T / dt))):
+            r = d.step()
+        assert r["restarts"] == 0
+        e = lr.amplitude(d.density(), mode) / (lr.EPS * np.exp(rate * r["time"])) - 1
+        e0, _ = lr.oracle_run(dt, 1e-14)
+        assert abs(e - e0) <= 1e-7  # rounding of rho (~1e-16) on a mode of amplitude 5e-7
+        errs.append(e)
+    assert abs(errs[0]) < 5e-5 and 3.6 < errs[0] / errs[1] < 4.4
+
+
 def test_ddft_ideal_gas_mode_decay(cdft):
     """Ideal gas, one small Fourier mode: rho_k(t) = rho_k(0) exp(Lambda_k t) to O(amplitude)."""
     shape, dx = (16, 16, 16), 0.1

@@ -124,6 +124,28 @@ def test_mobility_operator():
     assert np.max(np.abs(o.gdotx(rho0, wave) - lam * wave)) <= 1e-12 * abs(lam)
 
 
+def test_mobility_operator_central_differences():
+    """g . x with central differences (DDFT(dft, showGraphics, true), src/DDFT.cpp:325, 345): symmetric, negative
+    semi-definite, mass conserving; at uniform density rho0 (x_{i+2} - 2 x_i + x_{i-2}) / (4 dx^2) per axis, whose
+    Fourier eigenvalue is rho0 (cos(2 k dx) - 1) / (2 dx^2); and different from the forward-difference operator."""
+    o, rho = _dft()
+    o.set_central_differences(True)
+    N = rho.size
+    G = _dense(lambda x: o.gdotx(rho, x), N)
+    assert np.max(np.abs(G - G.T)) <= 1e-12 * np.max(np.abs(G))
+    assert np.max(np.linalg.eigvalsh(G)) <= 1e-10 * np.max(np.abs(G))
+    assert np.max(np.abs(G @ np.ones(N))) <= 1e-12 * np.max(np.abs(G)) and np.max(np.abs(G.sum(0))) <= 1e-10
+    rho0 = np.full(SHAPE, 0.4)
+    k = 2 * np.pi * 1 / (SHAPE[1] * DX)
+    y = np.arange(SHAPE[1]) * DX
+    wave = np.broadcast_to(np.cos(k * y)[None, :, None], SHAPE).copy()
+    lam = 0.4 * (np.cos(2 * k * DX) - 1) / (2 * DX ** 2)
+    assert np.max(np.abs(o.gdotx(rho0, wave) - lam * wave)) <= 1e-12 * abs(lam)
+    o.set_central_differences(False)
+    G1 = _dense(lambda x: o.gdotx(rho, x), N)
+    assert np.max(np.abs(G1 - G)) > 0.1 * np.max(np.abs(G))
+
+
 @pytest.mark.parametrize("bnd,squared", [(False, False), (True, False), (False, True)])
 def test_dynamic_log_det_is_the_dense_estimator(bnd, squared):
     """Log_Det on the DDFT dynamical matrix (DDFT::matrix_dot_v_intern, src/DDFT.cpp:353-361; is_dynamic negation,
