@@ -121,6 +121,7 @@ def lib():
         L.cdft_fire_batch_destroy.restype = None
         L.cdft_system_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_system_dynamic_dot.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_system_hessian_dot_noalloc.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_system_eigen_objective.argtypes = [_p, _p, _p, _p, _d, _i, _i, ctypes.POINTER(_d), _p]
         L.cdft_system_log_det.argtypes = [_p, _p, _i, _d, _d, ctypes.c_long, _i, _i, _i, ctypes.POINTER(_d),
                                           ctypes.POINTER(_d)]

@@ -4,6 +4,7 @@
 // each of the m FMT weights and the mean-field kernel), the spectral work
 // arrays, the twiddle tables and a HIP stream.  cdft_system_eval runs the
 // fused pipeline documented in kernels.hpp; no C++ exception crosses the ABI.
+#include <mutex>
 #include <random>
 
 #include "system.hpp"
@@ -11,6 +12,32 @@
 namespace cdft_impl {
 
 thread_local std::string g_err;
+
+// Work memory of the allocation-free entry point (cdft_system_hessian_dot_noalloc): each GPU the library runs on
+// holds one copy of this array until the process ends.  Host code never uses its name as an address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress, taken once per device at system set-up.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Records the address of the current device's scratch (set-up; the device must be current).
+void scratch_setup(int dev) {
+  REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  REQUIRE(p != nullptr, "no device address for the scratch array");
+  g_scratch_ptr[dev] = p;
+}
+void* scratch_of(int dev) {
+  REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  REQUIRE(g_scratch_ptr[dev], "scratch of this device not set up");
+  return g_scratch_ptr[dev];
+}
 __global__ void k_zero_int(int* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
@@ -226,6 +253,7 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
   s->d = *desc;
   s->nb = nb;
   HIPCHK(hipGetDevice(&s->device));
+  scratch_setup(s->device);
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   Geo& g = s->g;
   g.Nx = desc->Nx;
@@ -595,7 +623,9 @@ void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<
 }
 
 template <int MODEL>
-void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 1.0) {
+void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 1.0,
+                 const cplx<double>* B2 = nullptr) {
+  if (!B2) B2 = s.B2.as<cplx<double>>();
   const Tables<double>& tw = tables<double>(s);
   const int M = s.g.Nz / 2;
   PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
@@ -610,7 +640,7 @@ void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 
     if (RB < 1) return;
     const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, 0, nullptr};
     hipLaunchKernelGGL((k_z_hess2<MODEL, Mc>), dim3(div_up(s.nrows, RB), s.nb), dim3(nseq * RB * RF::P),
-                       size_t(RB) * per, s.stream, s.B.as<cplx<double>>(), s.B2.as<cplx<double>>(), s.Nkp, s.m,
+                       size_t(RB) * per, s.stream, s.B.as<cplx<double>>(), B2, s.Nkp, s.m,
                        mf ? 1 : 0, s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
                        tw.n.template as<cplx<double>>(), RB, pp, inv_dp > 0 ? inv_dp : pp.inv_d, scale,
                        s.flag.as<int>(), bt);
@@ -762,9 +792,11 @@ void eval_device_ao(cdft_system_s& s, const double* rho, double* dF, bool with_d
   s.mark("z_final");
 }
 
-// out = [out] + (d2 beta Omega / drho drho) v on FP64 device data (with_dft = false: FMT part only).
+// out = [out] + (d2 beta Omega / drho drho) v on FP64 device data (with_dft = false: FMT part only).  The spectra
+// of the direction v go to A2 (nb spectra) and B2 (nconv nb spectra): the system's buffers (allocated on first use)
+// unless the caller passes its own.
 void hess_device(cdft_system_s& s, const double* rho, const double* v, double* out, bool with_dft, bool ideal,
-                 bool accumulate) {
+                 bool accumulate, cplx<double>* A2, cplx<double>* B2) {
   REQUIRE(!s.mixed, "the Hessian-vector product needs an FP64 system");
   REQUIRE(!s.has_ao, "the Hessian-vector product is not implemented with an AO polymer");
   REQUIRE(rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) && reg_fft_ok<double>(s, s.g.Ny),
@@ -772,20 +804,24 @@ void hess_device(cdft_system_s& s, const double* rho, const double* v, double* o
   const bool mf = with_dft && s.d.mean_field;
   const long Nkp = s.Nkp, sA = Nkp, sB = s.nconv * Nkp;
   const size_t cs = sizeof(cplx<double>);
-  if (s.A2.bytes < cs * Nkp * s.nb) s.A2.alloc(cs * Nkp * s.nb);
-  if (s.B2.bytes < cs * Nkp * s.nconv * s.nb) s.B2.alloc(cs * Nkp * s.nconv * s.nb);
+  if (!A2 || !B2) {
+    if (s.A2.bytes < cs * Nkp * s.nb) s.A2.alloc(cs * Nkp * s.nb);
+    if (s.B2.bytes < cs * Nkp * s.nconv * s.nb) s.B2.alloc(cs * Nkp * s.nconv * s.nb);
+    A2 = s.A2.as<cplx<double>>();
+    B2 = s.B2.as<cplx<double>>();
+  }
   cplx<double>* A = s.A.as<cplx<double>>();
   cplx<double>* B = s.B.as<cplx<double>>();
   const Tables<double>& tw = tables<double>(s);
   if (s.m > 0) forward_weighted(s, rho, A, B, s.m);
-  if (s.m > 0 || mf) forward_weighted(s, v, s.A2.as<cplx<double>>(), s.B2.as<cplx<double>>(), s.m + (mf ? 1 : 0));
+  if (s.m > 0 || mf) forward_weighted(s, v, A2, B2, s.m + (mf ? 1 : 0));
   if (s.m > 0) {
     switch (s.d.model) {
-      case CDFT_ROSENFELD: launch_hess<M_ROSENFELD>(s, mf); break;
-      case CDFT_RSLT: launch_hess<M_RSLT>(s, mf); break;
-      case CDFT_ESFMT: launch_hess<M_ESFMT>(s, mf); break;
-      case CDFT_WHITEBEAR_I: launch_hess<M_WBI>(s, mf); break;
-      default: launch_hess<M_WBII>(s, mf); break;
+      case CDFT_ROSENFELD: launch_hess<M_ROSENFELD>(s, mf, 0.0, 1.0, B2); break;
+      case CDFT_RSLT: launch_hess<M_RSLT>(s, mf, 0.0, 1.0, B2); break;
+      case CDFT_ESFMT: launch_hess<M_ESFMT>(s, mf, 0.0, 1.0, B2); break;
+      case CDFT_WHITEBEAR_I: launch_hess<M_WBI>(s, mf, 0.0, 1.0, B2); break;
+      default: launch_hess<M_WBII>(s, mf, 0.0, 1.0, B2); break;
     }
     launch_cols<double>(s, B, Nkp, s.m, 1, true, s.nb, sB);
     const ColGeo cg = conv_geo<double>(s, s.W.p);
@@ -802,7 +838,7 @@ void hess_device(cdft_system_s& s, const double* rho, const double* v, double* o
     });
     launch_cols<double>(s, A, 0, 1, 1, false, s.nb, sA);
   } else if (mf) {
-    launch_rows_c2r<double, double>(s, s.B2.as<cplx<double>>(), sB, s.vmf.as<double>(), s.N, s.nb, 1.0);
+    launch_rows_c2r<double, double>(s, B2, sB, s.vmf.as<double>(), s.N, s.nb, 1.0);
   }
   FinalParams fp{};
   fp.dV = s.d.dx * s.d.dy * s.d.dz;
@@ -1103,6 +1139,45 @@ int run_hess(cdft_system s, const double* rho, const double* v, double* out, int
 
 int cdft_system_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags) {
   return guarded([&] { return run_hess(sys, rho, v, out, flags, true); });
+}
+
+// No counterpart in the reference: cdft_system_hessian_dot on device pointers for callers that cannot allocate.
+// The spectra of the direction (A2: nb spectra, B2: nconv nb spectra of Nx Ny Zp complex doubles) live in this
+// device's scratch instead of the system's lazily allocated buffers; every element the kernels read there is
+// written earlier in the same call, so nothing needs clearing.  No hipMalloc / hipFree / host staging.
+int cdft_system_hessian_dot_noalloc(cdft_system sys, const double* rho, const double* v, double* out, int flags) {
+  return guarded([&] {
+    REQUIRE(sys && rho && v && out, "null argument");
+    cdft_system_s& s = *sys;
+    REQUIRE(flags & CDFT_DEVICE_PTRS, "cdft_system_hessian_dot_noalloc takes device pointers (CDFT_DEVICE_PTRS)");
+    REQUIRE(!(s.d.boundary & CDFT_FIXED_BACKGROUND),
+            "cdft_system_hessian_dot_noalloc: fixed-background systems use cdft_system_hessian_dot");
+    const size_t cs = sizeof(cplx<double>);
+    const size_t need = cs * (size_t)s.Nkp * (size_t)s.nb * (1 + (size_t)s.nconv);
+    if (need > kScratchBytes) {
+      g_err = "cdft_system_hessian_dot_noalloc: the system needs " + std::to_string(need) +
+              " bytes of work memory, more than the " + std::to_string(kScratchBytes) + " bytes of its scratch";
+      return CDFT_BAD_ARG;
+    }
+    cplx<double>* A2 = static_cast<cplx<double>*>(scratch_of(s.device));
+    cplx<double>* B2 = A2 + (size_t)s.Nkp * s.nb;
+    hess_device(s, rho, v, out, true, !(flags & CDFT_NO_IDEAL), (flags & CDFT_ACCUMULATE) != 0, A2, B2);
+    // eta > 1 anywhere: the flag the kernels raised (read without host containers)
+    int fl[64];
+    for (int b0 = 0; b0 < s.nb; b0 += 64) {
+      const int n = std::min(64, s.nb - b0);
+      HIPCHK(hipMemcpyAsync(fl, s.flag.as<int>() + b0, sizeof(int) * n, hipMemcpyDeviceToHost, s.stream));
+      HIPCHK(hipStreamSynchronize(s.stream));
+      for (int b = 0; b < n; ++b)
+        if (fl[b]) {
+          hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(kThreads), 0, s.stream, s.flag.as<int>(), s.nb);
+          HIPCHK(hipStreamSynchronize(s.stream));
+          g_err = "Eta too large in FMT::dPHI";
+          return CDFT_ETA_TOO_LARGE;
+        }
+    }
+    return CDFT_OK;
+  });
 }
 
 int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags) {

@@ -429,7 +429,7 @@ inline void launch_cols(const cdft_system_s& s, cplx<T>* data, long stride, int 
 namespace cdft_impl {
 // Defined in cdft_api.hip.
 void hess_device(cdft_system_s& s, const double* rho, const double* v, double* out, bool with_dft, bool ideal,
-                 bool accumulate);
+                 bool accumulate, cplx<double>* A2 = nullptr, cplx<double>* B2 = nullptr);
 void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_dft, bool acc);
 // Species::beginForceCalculation on device data: rho rescaled in place to the fixed mass (no-op without one).
 void constraints_pre(cdft_system_s& s, double* rho);

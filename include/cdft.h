@@ -179,6 +179,15 @@ int cdft_fmt_eval(cdft_system sys, const double* rho, double* dF, double* F_hs, 
    CDFT_DEVICE_PTRS, CDFT_ACCUMULATE, CDFT_NO_IDEAL.  CDFT_ETA_TOO_LARGE if eta > 1. */
 int cdft_system_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
 
+/* No counterpart in the reference.  cdft_system_hessian_dot for callers that cannot allocate device memory at the
+   call (code under HIP graph capture, tight serving loops): the same product on device pointers (CDFT_DEVICE_PTRS
+   required; CDFT_NO_IDEAL, CDFT_ACCUMULATE as there), with its work spectra in a fixed 64 MiB array of the library
+   on each GPU instead of system buffers allocated on first use.  Systems whose (nconv + 1) nb Nx Ny Zp complex
+   doubles exceed 64 MiB (e.g. larger than 64^3 with White Bear + mean field) are refused with CDFT_BAD_ARG;
+   fixed-background systems likewise.  Calls that overlap on one GPU (two host threads, or two streams) race on
+   that array; each call synchronises the system's stream before it returns. */
+int cdft_system_hessian_dot_noalloc(cdft_system sys, const double* rho, const double* v, double* out, int flags);
+
 /* Replaces FMT::add_second_derivative (src/FMT.cpp:437-567): the hard-sphere
    part only; add into out with CDFT_ACCUMULATE as the reference does. */
 int cdft_fmt_hessian_dot(cdft_system sys, const double* rho, const double* v, double* out, int flags);
