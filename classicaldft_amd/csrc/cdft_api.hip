@@ -124,9 +124,56 @@ void cell_weights(cdft_system_s& s, double hsd, double* wreal) {
   HIPCHK(hipStreamSynchronize(s.stream));
 }
 
+// Per block: max |w_s - w_Txx - w_Tyy - w_Tzz| and max |w_s| over the real-space cell weights (11 arrays of n).
+__global__ __launch_bounds__(256) void k_trace_dev(const double* __restrict__ w, long n, double* __restrict__ out) {
+  double dev = 0.0, mx = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double ws = w[n + i];
+    dev = fmax(dev, fabs(ws - w[5 * n + i] - w[8 * n + i] - w[10 * n + i]));
+    mx = fmax(mx, fabs(ws));
+  }
+  __shared__ double sd[256], sm[256];
+  sd[threadIdx.x] = dev;
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + off]);
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = sd[0];
+    out[2 * blockIdx.x + 1] = sm[0];
+  }
+}
+
+// Whether the 11 real-space weights satisfy tr w_T = w_s to rounding (1e-13 of max |w_s|): true for the lattice
+// weights of FMT_Species::generateWeights (the tensor integrand's trace is the surface integrand), false for
+// weights carrying the reference generator's rounding noise.
+bool trace_identity_holds(cdft_system_s& s, const double* wreal) {
+  const int nblk = 64;
+  DevBuf out;
+  out.alloc(sizeof(double) * 2 * nblk);
+  hipLaunchKernelGGL(k_trace_dev, dim3(nblk), dim3(256), 0, s.stream, wreal, s.N, out.as<double>());
+  HIPCHK(hipGetLastError());
+  double h[2 * nblk];
+  HIPCHK(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, s.stream));
+  HIPCHK(hipStreamSynchronize(s.stream));
+  double dev = 0.0, mx = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    dev = std::max(dev, h[2 * b]);
+    mx = std::max(mx, h[2 * b + 1]);
+  }
+  return mx > 0.0 && dev <= 1e-13 * mx;
+}
+
 // The m weight spectra (x 1/N) of spheres of diameter hsd into W: from host cell weights or generated on the device.
+// With allow_trace (the colloid weights of a tensor model run by the register kernels), a weight set that satisfies
+// the trace identity is stored without T_zz and s.m, s.nconv, s.trace are updated.
 template <typename T>
-void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights, T* W) {
+void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights, T* W, bool allow_trace = false) {
   const long N = s.N, Nkp = s.Nkp;
   DevBuf S;  // one double-precision spectrum
   S.alloc(sizeof(cplx<double>) * Nkp);
@@ -140,6 +187,11 @@ void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights
       cell_weights(s, hsd, wreal.as<double>());
     }
     HIPCHK(hipStreamSynchronize(s.stream));
+    if (allow_trace && s.m == 11 && trace_identity_holds(s, wreal.as<double>())) {
+      s.m = 10;
+      s.nconv = s.m + (s.d.mean_field ? 1 : 0);
+      s.trace = 1;
+    }
     for (int a = 0; a < s.m; ++a) {
       launch_rows_r2c<double, double>(s, wreal.as<double>() + a * N, 0, S.as<cplx<double>>(), 0, 1);
       launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 1, true);
@@ -211,7 +263,13 @@ void build_weight_spectra(cdft_system_s& s) {
   DevBuf S;  // one double-precision spectrum
   S.alloc(sizeof(cplx<double>) * Nkp);
   const int nb = div_up(Nkp, kThreads);
-  if (s.m > 0) fmt_weight_spectra<T>(s, s.d.hsd, s.d.fmt_weights, s.W.as<T>());
+  const char* nt = std::getenv("CDFT_NO_TRACE");
+  // FP64 only: in the mixed path n_Tzz = n_s - n_Txx - n_Tyy of FP32 spectra carries several times the rounding
+  // of a directly transformed n_Tzz, which raises the gradient noise floor of the mixed FIRE phase
+  const bool allow_trace = !s.mixed && !s.has_ao && !(nt && nt[0] == '1') && rows_reg_ok<T>(s) && reg_fft_ok<T>(s, s.g.Nx) &&
+                           reg_fft_ok<T>(s, s.g.Ny) && rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) &&
+                           reg_fft_ok<double>(s, s.g.Ny);
+  if (s.m > 0) fmt_weight_spectra<T>(s, s.d.hsd, s.d.fmt_weights, s.W.as<T>(), allow_trace);
   if (s.has_ao) fmt_weight_spectra<T>(s, 2 * s.d.ao_Rp, s.d.ao_weights, s.Wao.as<T>());
   if (s.d.mean_field) {
     DevBuf w;
@@ -378,13 +436,14 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
       hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk, s.nb), dim3(nconv * RB * RF::P),
                          size_t(RB) * per, s.stream, s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
                          vmf, nrows, s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, pp,
-                         s.partials.as<double2>() + blk_off, s.flag.as<int>(), bt);
+                         s.partials.as<double2>() + blk_off, s.flag.as<int>(), bt, s.trace);
       HIPCHK(hipGetLastError());
     });
     if (nblk > 0) return nblk;
   }
   REQUIRE(x0 == 0 && x1 == s.g.Nx, "x-slab passes need the register z kernels");
   REQUIRE(s.nb == 1, "batched systems need transform lengths the register FFT supports");
+  REQUIRE(!s.trace, "the trace-reduced weight set needs the register z kernels");
   const int RB = s.RBpw;
   const size_t smem = sizeof(cplx<T>) * nconv * RB * M;
   hipLaunchKernelGGL((k_z_pointwise<T, MODEL, 16>), dim3(div_up(s.nrows, RB)), dim3(kThreads), smem, s.stream,
@@ -643,7 +702,7 @@ void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 
                        size_t(RB) * per, s.stream, s.B.as<cplx<double>>(), B2, s.Nkp, s.m,
                        mf ? 1 : 0, s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
                        tw.n.template as<cplx<double>>(), RB, pp, inv_dp > 0 ? inv_dp : pp.inv_d, scale,
-                       s.flag.as<int>(), bt);
+                       s.flag.as<int>(), bt, s.trace);
     HIPCHK(hipGetLastError());
     done = true;
   });

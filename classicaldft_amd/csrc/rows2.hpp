@@ -227,13 +227,16 @@ __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ i
 // every site, forward z of the m dPhi rows back into B, v_mf = the (m+1)-th
 // array in real space, F partial sums.  Sequence s = a * RB + r.
 // grid = (ceil(nrows/RB), members), block = nconv * RB * P, dynamic LDS = nconv * RB * SLOT elements.
+// trace != 0 (tensor models whose weights satisfy w_Txx + w_Tyy + w_Tzz = w_s): only m = 10 arrays are stored,
+// n_Tzz = n_s - n_Txx - n_Tyy per site, and the T_zz derivative is folded back onto the arrays it was built
+// from (w_Tzz * dPhi_Tzz = w_s * dPhi_Tzz - w_Txx * dPhi_Tzz - w_Tyy * dPhi_Tzz).
 template <typename T, int MODEL, int M>
 __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, int m, int nconv,
                                                       double* __restrict__ vmf, long nrows, int Zp,
                                                       const cplx<T>* __restrict__ twM,
                                                       const cplx<T>* __restrict__ twN, int RB, PhiParams pp,
                                                       double2* __restrict__ partials, int* __restrict__ flag,
-                                                      Batch bt) {
+                                                      Batch bt, int trace) {
   B += blockIdx.y * bt.sb;
   if (vmf) vmf += blockIdx.y * bt.sr;
   partials += blockIdx.y * bt.sp;
@@ -272,8 +275,14 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
 #pragma unroll
     for (int b = 0; b < 11; ++b)
       n[b] = b < m ? double(reinterpret_cast<const T*>(lds + (b * RB + rr) * RF::SLOT)[z]) : 0.0;
+    if (trace) n[10] = n[1] - n[5] - n[8];
     double out[11];
     const double phi = phi_site<MODEL>(n, pp, out);
+    if (trace) {
+      out[1] += out[10];
+      out[5] -= out[10];
+      out[8] -= out[10];
+    }
     if (n[0] > 0.5 && 1 - n[0] < 0.0) bad = 1;
     F.add(phi);
 #pragma unroll
@@ -312,7 +321,8 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
                                                  long arr_stride, int m, int mf, double* __restrict__ vmf, long nrows,
                                                  int Zp, const cplx<double>* __restrict__ twM,
                                                  const cplx<double>* __restrict__ twN, int RB, PhiParams pp,
-                                                 double inv_dp, double scale, int* __restrict__ flag, Batch bt) {
+                                                 double inv_dp, double scale, int* __restrict__ flag, Batch bt,
+                                                 int trace) {
   B += blockIdx.y * bt.sb;
   B2 += blockIdx.y * bt.sb;
   if (vmf) vmf += blockIdx.y * bt.sr;
@@ -350,6 +360,10 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
       nv[b] = b < m ? reinterpret_cast<const double*>(lds + (b * RB + rr) * RF::SLOT)[z] : 0.0;
       pv[b] = b < m ? reinterpret_cast<const double*>(lds + ((m + b) * RB + rr) * RF::SLOT)[z] : 0.0;
     }
+    if (trace) {  // T_zz from the trace identity, as in k_z_pointwise2
+      nv[10] = nv[1] - nv[5] - nv[8];
+      pv[10] = pv[1] - pv[5] - pv[8];
+    }
     // measures of rho (diameter 1/inv_d) carrying those of the direction (diameter 1/inv_dp) as e parts
     const double id = pp.inv_d, id2 = pp.inv_d2, ip2 = inv_dp * inv_dp;
     const Dual eta(nv[0], pv[0]), s0(nv[1] * id2, pv[1] * ip2), s1(nv[1] * id, pv[1] * inv_dp), s2(nv[1], pv[1]);
@@ -362,6 +376,11 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
     phi_meas<MODEL, Dual>(eta, s0, s1, s2, v1, v2, T6, pp, d);
     Dual out[11];
     phi_collapse<(MODEL >= M_ESFMT), Dual>(d, id, id2, out);
+    if (trace) {
+      out[1] = out[1] + out[10];
+      out[5] = out[5] - out[10];
+      out[8] = out[8] - out[10];
+    }
     if (nv[0] > 0.5 && 1 - nv[0] < 0.0) bad = 1;
 #pragma unroll
     for (int b = 0; b < 11; ++b)

@@ -157,6 +157,9 @@ struct cdft_system_s {
   Geo g{};
   long N = 0, Nkp = 0, nrows = 0;
   int m = 0, nconv = 0;
+  // tensor models whose weights satisfy the trace identity w_Txx + w_Tyy + w_Tzz = w_s (checked at set-up): T_zz is
+  // not stored, m = 10, and the pointwise kernels rebuild n_Tzz = n_s - n_Txx - n_Tyy
+  int trace = 0;
   unsigned odd_mask = 0x1Cu;  // arrays 2, 3, 4 (vector weights) have imaginary spectra
   bool mixed = false;
   Radices radX{}, radY{}, radM{};

@@ -292,3 +292,27 @@ def test_weights_without_axis_parity_keep_the_full_spectra(cdft):
         s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, weights=w, precision=prec)
         F1, dF1 = s.fmt_eval(rho)
         assert rel(F1, F0) <= tol and maxrel(dF1, dF0) <= 10 * tol
+
+
+@pytest.mark.parametrize("model", [3, 4, 2])
+def test_trace_reduced_weight_set_matches_full(cdft, model, monkeypatch):
+    """Weights satisfying w_Txx + w_Tyy + w_Tzz = w_s run with 10 stored arrays (n_Tzz rebuilt per site, its
+    derivative folded back); the same system with all 11 arrays (CDFT_NO_TRACE=1) gives the same F, dF and
+    Hessian product to rounding."""
+    shape, dx, hsd = (48, 48, 64), 0.05, 1.0
+    rho = seeded_density(shape, 0.55, 0x5EED0400 + model, amp=0.2)
+    v = np.random.default_rng(model).standard_normal(shape)
+    kw = dict(model=model, A=1.3, B=-0.4, ideal=True, mean_field=True, kT=1.2)
+    res = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("CDFT_NO_TRACE", flag)
+        s = cdft.System(shape, dx, hsd, **kw)
+        parts, dF = s.eval(rho)
+        h = s.hessian_dot(rho, v)
+        res.append((parts[0], dF, h))
+        s.close()
+    (F0, dF0, h0), (F1, dF1, h1) = res
+    print("trace vs full: F", rel(F0, F1), "dF", maxrel(dF0, dF1), "H v", maxrel(h0, h1))
+    assert rel(F0, F1) <= 1e-13
+    assert maxrel(dF0, dF1) <= 1e-13
+    assert maxrel(h0, h1) <= 1e-12
