@@ -1,0 +1,131 @@
+"""The benchmark configurations on the GPU, each against the oracle on a grid the oracle runs in seconds and at its
+full size through properties that do not depend on the size.
+
+* Slit pore (White Bear I + LJ mean field, WCA split, kT = 0.8, walls 10 [exp(-z/0.2) + exp(-(L-z)/0.2)],
+  mu = mu(rho_b = 0.7)): FIRE2 step by step against the oracle at 24 x 24 x 64; at 128^3 FP64 FIRE reaches
+  max |dF/drho| / dV <= 1e-8, and the mixed-precision start refined in FP64 reaches the same minimum.
+* fcc seed in the liquid (rho_l = 0.94, Gaussians alpha = 50 on an fcc lattice a = 1.55, White Bear I + LJ,
+  kT = 0.8, dt = 1e-4): DDFT steps against the oracle with one conventional cell at 48^3, and mass conservation
+  to 1e-12 with the 4 x 4 x 4 seed at 256^3.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+KT = 0.8
+
+
+def close(a, b, tol):
+    return abs(a - b) <= tol * max(abs(a), abs(b), 1e-300)
+
+
+def slit_pore(cdft, shape, dx, rho_b=0.7):
+    hsd = cdft.lj_hsd(KT)
+    Lz = shape[2] * dx
+    z = np.arange(shape[2]) * dx
+    vz = 10.0 * (np.exp(-z / 0.2) + np.exp(-(Lz - z) / 0.2))
+    vext = np.ascontiguousarray(np.broadcast_to(vz, shape))
+    probe = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=KT)
+    mu = probe.bulk(rho_b)[1]
+    probe.close()
+    return hsd, mu, vext, rho_b * np.exp(-vext)
+
+
+def fcc_seed(n, dx, cells, a=1.55, alpha=50.0, rho_l=0.94):
+    """Liquid rho_l with the cube of `cells`^3 fcc cells centred in the box replaced by Gaussians
+    (alpha / pi)^{3/2} exp(-alpha |r - R|^2) on the fcc sites R."""
+    L = n * dx
+    x = np.arange(n) * dx
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cl = np.array([[i, j, k] for i in range(cells) for j in range(cells) for k in range(cells)], dtype=float)
+    sites = ((cl[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a) + (L - cells * a) / 2
+    rho = np.full((n, n, n), rho_l)
+    inside = (x >= (L - cells * a) / 2) & (x < (L + cells * a) / 2)
+    xi = x[inside]
+    g = [np.exp(-alpha * (xi[None, :] - sites[:, k:k + 1]) ** 2) for k in range(3)]
+    sub = np.einsum("si,sj,sk->ijk", g[0], g[1], g[2]) * (alpha / np.pi) ** 1.5
+    idx = np.where(inside)[0]
+    rho[np.ix_(idx, idx, idx)] = np.maximum(sub, 1e-12)
+    return rho
+
+
+def test_slit_pore_fire_replays_the_oracle(cdft):
+    shape, dx = (24, 24, 64), 0.05
+    hsd, mu, vext, rho0 = slit_pore(cdft, shape, dx)
+    o = oracle.DFT(shape, dx, hsd, mf=True, kT=KT, mu=mu, vext=vext, accurate_weights=True)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mu=mu, vext=vext, mean_field=True, kT=KT)
+    fo, fg = oracle.Fire(o, rho0), cdft.Fire(s, rho0)
+    for i in range(30):
+        rc0, a = fo.step()
+        rc1, b = fg.step()
+        assert rc0 == rc1, (i, rc0, rc1)
+        assert close(a["F"], b["F"], 1e-10), (i, a["F"], b["F"])
+        assert close(a["dt"], b["dt"], 1e-12) and close(a["alpha"], b["alpha"], 1e-12)
+        assert close(a["fmax_raw"], b["fmax_raw"], 1e-9)
+    r0, r1 = fo.state()[0], fg.state()[0]
+    assert np.abs(r0 - r1).max() <= 1e-10 * np.abs(r0).max()
+
+
+def test_slit_pore_128_fp64_and_mixed_start_reach_one_minimum(cdft):
+    shape, dx, tol = (128, 128, 128), 0.05, 1e-8
+    hsd, mu, vext, rho0 = slit_pore(cdft, shape, dx)
+    kw = dict(model=cdft.WHITEBEAR_I, ideal=True, mu=mu, vext=vext, mean_field=True, kT=KT)
+    s64 = cdft.System(shape, dx, hsd, **kw)
+    f = cdft.Fire(s64, rho0)
+    rc, rep = f.run(20000, tol)
+    print("FP64: steps", rep["steps"], "F", rep["F"], "fmax", rep["fmax_raw"])
+    assert rep["fmax_raw"] <= tol
+    F64, r64 = rep["F"], f.state()[0]
+    f.close()
+    smx = cdft.System(shape, dx, hsd, precision=cdft.MIXED, **kw)
+    fm = cdft.Fire(smx, rho0)
+    _, repm = fm.run(3000, 1e-5)  # the FP32 spectra's gradient floor is ~1e-6
+    rm = fm.state()[0]
+    fm.close()
+    smx.close()
+    fr = cdft.Fire(s64, rm)
+    _, rep2 = fr.run(20000, tol)
+    print("mixed", repm["steps"], repm["fmax_raw"], "-> FP64", rep2["steps"], rep2["F"], rep2["fmax_raw"])
+    assert rep2["fmax_raw"] <= tol
+    assert close(rep2["F"], F64, 1e-10)
+    assert np.abs(fr.state()[0] - r64).max() <= 1e-6 * r64.max()
+    fr.close()
+    s64.close()
+
+
+def test_fcc_seed_ddft_replays_the_oracle(cdft):
+    n, dx = 48, 0.04
+    rho = fcc_seed(n, dx, 1)
+    hsd = cdft.lj_hsd(KT)
+    o = oracle.DFT((n, n, n), dx, hsd, mf=True, kT=KT, accurate_weights=True)
+    s = cdft.System((n, n, n), dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=KT)
+    do = oracle.DDFT(o, rho, dt=1e-4, dt_max=1e-4)
+    dg = cdft.DDFT(s, rho, dt=1e-4, dt_max=1e-4)
+    m0 = rho.sum() * dx ** 3
+    for i in range(3):
+        a = do.step()
+        b = dg.step()
+        print(i, a, b)
+        assert a["calls"] == b["calls"] and a["restarts"] == b["restarts"]
+        assert close(a["F"], b["F"], 1e-10) and close(a["dt"], b["dt"], 1e-12)
+        assert abs(b["mass"] / m0 - 1) <= 1e-12
+    r0, r1 = do.density(), dg.density()
+    assert np.abs(r0 - r1).max() <= 1e-10 * np.abs(r0).max()
+
+
+def test_fcc_seed_ddft_256_conserves_mass(cdft):
+    n, dx = 256, 0.04
+    rho = fcc_seed(n, dx, 4)
+    s = cdft.System((n, n, n), dx, cdft.lj_hsd(KT), model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=KT)
+    d = cdft.DDFT(s, rho, dt=1e-4, dt_max=1e-4)
+    m0 = rho.sum() * dx ** 3
+    for i in range(4):
+        r = d.step()
+        print(i, r)
+        assert abs(r["mass"] / m0 - 1) <= 1e-12
+    assert r["time"] > 0
+    d.close()
+    s.close()
