@@ -185,9 +185,10 @@ def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha
 
 def batch_fire(cdft, classes=((48, 32), (64, 32), (96, 32)), dx=0.1, kT=0.8, tol=1e-8, ladder_tol=1e-5,
                ladder_rounds=24, check=25, max_rounds=2000, sequential=True):
-    """Batched FIRE2 to max |dF/drho| <= tol on systems of several grid sizes: one device-controlled,
-    hipGraph-captured FIRE loop per grid class (all members in every launch), tiles autotuned per class,
-    mixed precision until ladder_tol (at most ladder_rounds x check steps), then FP64 from a FIRE restart."""
+    """Batched FIRE2 to max |dF/drho| <= tol on systems of several grid sizes: a device-controlled FIRE batch per grid
+    class (all members in every kernel), all classes captured into one hipGraph (one launch per round of `check`
+    steps, the classes concurrent), tiles autotuned per class, mixed precision until ladder_tol (at most
+    ladder_rounds x check steps), then FP64 from a FIRE restart."""
     hsd = cdft.lj_hsd(kT)
     kw = dict(hsd=hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=kT)
     seed = 0x5EED4000
@@ -205,26 +206,23 @@ def batch_fire(cdft, classes=((48, 32), (64, 32), (96, 32)), dx=0.1, kT=0.8, tol
         info["classes"].append({"grid": n, "systems": count, "ms_per_batched_eval_mixed": t_mixed})
     t0 = time.perf_counter()
     phase_t = []
+    group = cdft.FireGroup(batches)  # every class advances in one graph launch per round
     for phase, (prec, ptol) in enumerate(((cdft.MIXED, ladder_tol), (cdft.FP64, tol))):
         if phase == 1:
             for b, fb in enumerate(batches):
                 fb.set_precision(prec)
                 info["classes"][b]["ms_per_batched_eval_fp64"] = fb.autotune(reps=3)
         tp = time.perf_counter()
-        live = list(range(len(batches)))
         last = tp
         for rnd in range(ladder_rounds if phase == 0 else max_rounds):
-            for b in live:
-                batches[b].run(check, ptol)
-            still, worst, nrun = [], 0.0, 0
-            for b in live:
-                reps, st, running = batches[b].status()
+            group.run(check, ptol)
+            worst, nrun = 0.0, 0
+            for fb in batches:
+                reps, st, running = fb.status()
                 if running:
-                    still.append(b)
                     nrun += running
                     worst = max(worst, max(r["fmax_raw"] for r, q in zip(reps, st) if q == 0))
-            live = still
-            if not live:
+            if nrun == 0:
                 break
             if time.perf_counter() - last > 10:
                 last = time.perf_counter()
@@ -233,6 +231,7 @@ def batch_fire(cdft, classes=((48, 32), (64, 32), (96, 32)), dx=0.1, kT=0.8, tol
         phase_t.append(time.perf_counter() - tp)
         print(f"#   phase {phase} done in {phase_t[-1]:.2f} s", flush=True)
     total = time.perf_counter() - t0
+    group.close()
     steps = conv = fails = 0
     for b, fb in enumerate(batches):
         reps, st, _ = fb.status()

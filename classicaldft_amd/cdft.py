@@ -119,6 +119,10 @@ def lib():
         L.cdft_fire_batch_system.argtypes = [_p, ctypes.POINTER(_p)]
         L.cdft_fire_batch_destroy.argtypes = [_p]
         L.cdft_fire_batch_destroy.restype = None
+        L.cdft_fire_group_create.argtypes = [_p, _i, ctypes.POINTER(_p)]
+        L.cdft_fire_group_run.argtypes = [_p, _i, _d]
+        L.cdft_fire_group_destroy.argtypes = [_p]
+        L.cdft_fire_group_destroy.restype = None
         L.cdft_system_hessian_dot.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_system_dynamic_dot.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_system_hessian_dot_noalloc.argtypes = [_p, _p, _p, _p, _i]
@@ -616,6 +620,34 @@ class FireBatch:
     def close(self):
         if getattr(self, "h", None):
             lib().cdft_fire_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FireGroup:
+    """Several FireBatch objects (e.g. one per grid size) stepped together: one hipGraph launch per `run` advances
+    every batch, each on its own stream inside the graph (cdft_fire_group_*).  The batches stay usable on their own
+    (status, density, set_precision); close the group before them."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)
+        arr = (_p * len(self.batches))(*[b.h for b in self.batches])
+        h = _p()
+        _check(lib().cdft_fire_group_create(arr, len(self.batches), ctypes.byref(h)))
+        self.h = h
+
+    def run(self, steps, tol_raw):
+        """Enqueue `steps` steps of every batch as one graph launch (no synchronisation)."""
+        _check(lib().cdft_fire_group_run(self.h, int(steps), float(tol_raw)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_fire_group_destroy(self.h)
             self.h = None
 
     def __del__(self):

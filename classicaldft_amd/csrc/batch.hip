@@ -599,6 +599,113 @@ int cdft_fire_batch_system(cdft_fire_batch b, cdft_system* sys) {
 
 void cdft_fire_batch_destroy(cdft_fire_batch b) { delete b; }
 
+}  // extern "C"
+
+// A group of FIRE batches (typically one per grid size) stepped together: K steps of every batch are captured into
+// one hipGraph, each batch on its own stream forked from the group's stream and joined back, so one graph launch
+// advances systems of different grids concurrently.  The graph is re-captured when the step count, the tolerance
+// or a batch's precision changes.
+struct cdft_fire_group_s {
+  std::vector<cdft_fire_batch> b;
+  std::vector<cdft_system> sys_at_capture;
+  hipStream_t st = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join, pre;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int steps = 0;
+  double tol = -1;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~cdft_fire_group_s() {
+    drop_graph();
+    for (hipEvent_t e : join) (void)hipEventDestroy(e);
+    for (hipEvent_t e : pre) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (done) (void)hipEventDestroy(done);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+extern "C" {
+
+int cdft_fire_group_create(cdft_fire_batch* batches, int n, cdft_fire_group* out) {
+  return guarded([&] {
+    REQUIRE(batches && out && n >= 1, "bad argument");
+    auto g = std::make_unique<cdft_fire_group_s>();
+    for (int i = 0; i < n; ++i) {
+      REQUIRE(batches[i], "null batch");
+      for (int j = 0; j < i; ++j) REQUIRE(batches[j] != batches[i], "a batch appears twice in the group");
+      g->b.push_back(batches[i]);
+    }
+    HIPCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g->done, hipEventDisableTiming));
+    g->join.resize(n);
+    g->pre.resize(n);
+    for (int i = 0; i < n; ++i) {
+      HIPCHK(hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&g->pre[i], hipEventDisableTiming));
+    }
+    *out = g.release();
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_fire_group_run(cdft_fire_group g, int steps, double tol_raw) {
+  return guarded([&] {
+    REQUIRE(g && steps > 0, "bad argument");
+    const int n = (int)g->b.size();
+    bool stale = !g->exec || g->steps != steps || g->tol != tol_raw;
+    for (int i = 0; i < n && !stale; ++i) stale = g->sys_at_capture[i] != g->b[i]->sys;
+    for (cdft_fire_batch b : g->b) b->cp.tol = tol_raw;
+    if (stale) {
+      g->drop_graph();
+      HIPCHK(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+      try {
+        HIPCHK(hipEventRecord(g->fork, g->st));
+        for (int i = 0; i < n; ++i) {
+          hipStream_t bs = g->b[i]->sys->stream;
+          HIPCHK(hipStreamWaitEvent(bs, g->fork, 0));
+          for (int k = 0; k < steps; ++k) g->b[i]->enqueue_step();
+          HIPCHK(hipEventRecord(g->join[i], bs));
+          HIPCHK(hipStreamWaitEvent(g->st, g->join[i], 0));
+        }
+      } catch (...) {
+        hipGraph_t gr = nullptr;
+        (void)hipStreamEndCapture(g->st, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(g->st, &g->graph));
+      HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+      g->steps = steps;
+      g->tol = tol_raw;
+      g->sys_at_capture.clear();
+      for (cdft_fire_batch b : g->b) g->sys_at_capture.push_back(b->sys);
+    }
+    // the launch follows the work already queued on each batch's stream, and that stream follows the launch
+    for (int i = 0; i < n; ++i) {
+      HIPCHK(hipEventRecord(g->pre[i], g->b[i]->sys->stream));
+      HIPCHK(hipStreamWaitEvent(g->st, g->pre[i], 0));
+    }
+    HIPCHK(hipGraphLaunch(g->exec, g->st));
+    HIPCHK(hipEventRecord(g->done, g->st));
+    for (cdft_fire_batch b : g->b) HIPCHK(hipStreamWaitEvent(b->sys->stream, g->done, 0));
+    return (int)CDFT_OK;
+  });
+}
+
+void cdft_fire_group_destroy(cdft_fire_group g) {
+  if (g) (void)hipStreamSynchronize(g->st);
+  delete g;
+}
+
 int cdft_system_autotune(cdft_system s, int reps, double* best_ms) {
   return guarded([&] {
     REQUIRE(s && reps > 0, "bad argument");

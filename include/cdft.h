@@ -415,6 +415,15 @@ int cdft_fire_batch_status(cdft_fire_batch b, cdft_fire_report* reports, int* st
 int cdft_fire_batch_get(cdft_fire_batch b, int member, double* rho);
 int cdft_fire_batch_system(cdft_fire_batch b, cdft_system* sys);
 void cdft_fire_batch_destroy(cdft_fire_batch b);
+/* No reference counterpart (the reference runs one fireMinimizer2 per system, src/Minimizer.cpp:160-271): several
+   batches, e.g. one per grid size, stepped together.  cdft_fire_group_run advances every batch by `steps` FIRE2 steps
+   with one hipGraph launch whose branches (one per batch, each on the batch's stream) run concurrently; members stop
+   at tol_raw as in cdft_fire_batch_run.  The batches stay owned by the caller and usable on their own
+   (status, get, set_precision: the group re-captures after a precision change); destroy the group first. */
+typedef struct cdft_fire_group_s* cdft_fire_group;
+int cdft_fire_group_create(cdft_fire_batch* batches, int n, cdft_fire_group* out);
+int cdft_fire_group_run(cdft_fire_group g, int steps, double tol_raw);
+void cdft_fire_group_destroy(cdft_fire_group g);
 
 /* ------------------------------------------------------- learned warm start ---
    No counterpart in the reference.  A 3-level 3D U-Net on periodic D^3 grids

@@ -129,3 +129,61 @@ def test_fcc_seed_ddft_256_conserves_mass(cdft):
     assert r["time"] > 0
     d.close()
     s.close()
+
+
+def _class_batch(cdft, n, count, seed, precision):
+    from classicaldft_amd.workloads import well_systems
+    hsd = cdft.lj_hsd(KT)
+    kw = dict(hsd=hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=KT)
+    rho_b, vexts = well_systems(n, count, seed, 0.1)
+    probe = cdft.System((n, n, n), 0.1, **kw)
+    mus = np.array([probe.bulk(r)[1] for r in rho_b])
+    probe.close()
+    rho0 = np.repeat(rho_b, n ** 3).reshape(count, n, n, n)
+    return cdft.FireBatch((n, n, n), 0.1, mus, rho0, vexts=vexts, precision=precision, **kw), mus, vexts, rho0
+
+
+def test_batched_uneven_grids_in_one_graph(cdft):
+    """The batched workload's systems (seeds 0x5EED4000 + s, bulk density U[0.3, 0.8], three Gaussian wells) on
+    48^3, 64^3 and 96^3 stepped together by one graph launch per round (mixed precision, then FP64 to 1e-8): every
+    member converges, and each ends where the same batches stepped one by one end (same step count, F to 1e-13)."""
+    classes = [(48, 2, 0x5EED4000), (64, 2, 0x5EED4020), (96, 1, 0x5EED4040)]
+    grouped = [_class_batch(cdft, n, c, s, cdft.MIXED)[0] for n, c, s in classes]
+    alone = [_class_batch(cdft, n, c, s, cdft.MIXED)[0] for n, c, s in classes]
+    group = cdft.FireGroup(grouped)
+    for prec, tol, rounds in ((cdft.MIXED, 1e-5, 24), (cdft.FP64, 1e-8, 400)):
+        for fb in grouped + alone:
+            fb.set_precision(prec)
+        for _ in range(rounds):
+            group.run(25, tol)
+            for fb in alone:
+                fb.run(25, tol)
+            if sum(fb.status()[2] for fb in grouped + alone) == 0:
+                break
+    group.close()
+    for g, a in zip(grouped, alone):
+        rg, sg, _ = g.status()
+        ra, sa, _ = a.status()
+        assert (sg == 1).all() and (sa == 1).all(), (sg, sa)
+        for x, y in zip(rg, ra):
+            assert x["steps"] == y["steps"] and close(x["F"], y["F"], 1e-13), (x, y)
+            assert x["fmax_raw"] <= 1e-8
+    for fb in grouped + alone:
+        fb.close()
+
+
+def test_batched_member_replays_the_oracle(cdft):
+    """A 48^3 member of the batched workload (FP64, device-controlled FIRE2) against the oracle's FIRE2."""
+    fb, mus, vexts, rho0 = _class_batch(cdft, 48, 2, 0x5EED4000, cdft.FP64)
+    hsd = cdft.lj_hsd(KT)
+    o = oracle.DFT((48, 48, 48), 0.1, hsd, mf=True, kT=KT, mu=float(mus[1]), vext=vexts[1], accurate_weights=True)
+    fo = oracle.Fire(o, rho0[1])
+    for i in range(15):
+        rc, a = fo.step()
+        fb.run(1, 0.0)
+        b = fb.status()[0][1]
+        assert close(a["F"], b["F"], 1e-10), (i, a["F"], b["F"])
+        assert close(a["dt"], b["dt"], 1e-12) and close(a["alpha"], b["alpha"], 1e-12)
+    r0 = fo.state()[0]
+    assert np.abs(r0 - fb.density(1)).max() <= 1e-10 * np.abs(r0).max()
+    fb.close()
