@@ -11,6 +11,7 @@ CDFT_BAD_ARG, CDFT_HIP_ERROR, CDFT_NOT_SUPPORTED = -1, -2, -3
 MODEL_NONE, ROSENFELD, RSLT, ESFMT, WHITEBEAR_I, WHITEBEAR_II = -1, 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
 WEIGHTS_ACCURATE, WEIGHTS_REFERENCE = 0, 1  # cdft_weights_mode
+AO_FORCE_EXACT, AO_FORCE_REFERENCE = 0, 1  # cdft_ao_force
 DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, SQUARED_MATRIX, DYNAMIC_MATRIX, CENTRAL_DIFFERENCES = 1, 2, 4, 8, 16, 32, 64
 
 _d = ctypes.c_double
@@ -41,7 +42,7 @@ class _Desc(ctypes.Structure):
         ("mf_potential", _i), ("mf_alpha", _d), ("mf_bh", _i), ("mf_r_att_min", _d), ("mf_scheme", _i),
         ("mf_ngauss", _i),
         ("ao_Rp", _d), ("ao_reservoir", _d), ("ao_weights", _p),
-        ("fixed_mass", _d), ("boundary", _i), ("mf_scale", _d), ("weights_mode", _i),
+        ("fixed_mass", _d), ("boundary", _i), ("mf_scale", _d), ("weights_mode", _i), ("ao_force", _i),
     ]
 
 
@@ -223,14 +224,17 @@ class System:
                  vext=None, mean_field=False, sigma=1.0, eps=1.0, rcut=2.5, kT=1.0, mf_kernel=None, A=1.0, B=0.0,
                  mus=None, vexts=None, potential=POT_LJ, alpha=0.0, bh=False, r_att_min=0.0, mf_scheme=MF_QF,
                  ngauss=0, ao_Rp=0.0, ao_reservoir=0.0, ao_weights=None, fixed_mass=0.0, fixed_background=False,
-                 homogeneous_boundary=False, mf_scale=1.0, weights_mode=WEIGHTS_ACCURATE):
+                 homogeneous_boundary=False, mf_scale=1.0, weights_mode=WEIGHTS_ACCURATE,
+                 ao_force=AO_FORCE_EXACT):
         """mus (and optionally vexts, shape (nb,) + shape) make a batch of nb members (cdft_system_batch_create).
         potential / alpha / bh / r_att_min / mf_scheme / ngauss choose the mean-field potential and kernel rule;
         ao_Rp > 0 adds an Asakura-Oosawa polymer of that radius at reservoir density ao_reservoir.
         fixed_mass / fixed_background / homogeneous_boundary: Species::setFixedMass / setFixedBackground /
         setHomogeneousBoundary (the full evaluation rescales rho to the mass and constrains dF); batches take one
         fixed mass per member.  weights_mode: how generated cell weights are computed (WEIGHTS_ACCURATE on the
-        device, or WEIGHTS_REFERENCE: the reference's double expressions on the host)."""
+        device, or WEIGHTS_REFERENCE: the reference's double expressions on the host).  ao_force: AO_FORCE_EXACT
+        (the derivative of the AO free energy) or AO_FORCE_REFERENCE (each off-diagonal tensor measure counted
+        once, as FMT_AO_Species::setFundamentalMeasures_AO does, include/Species.h:406-425)."""
         fm = np.atleast_1d(np.asarray(fixed_mass, dtype=np.float64)).reshape(-1)
         self.shape = tuple(int(n) for n in shape)
         self.fixed_background = bool(fixed_background)
@@ -255,7 +259,7 @@ class System:
                   arr(mf_kernel, self.N), int(potential), float(alpha), int(bool(bh)), float(r_att_min),
                   int(mf_scheme), int(ngauss), float(ao_Rp), float(ao_reservoir), arr(ao_weights, 11 * self.N),
                   float(fm[0]), (1 if fixed_background else 0) | (2 if homogeneous_boundary else 0), float(mf_scale),
-                  int(weights_mode))
+                  int(weights_mode), int(ao_force))
         h = _p()
         if mus is None:
             self.nb = 1

@@ -66,6 +66,7 @@ void validate(const cdft_system_desc& d) {
     REQUIRE(d.mf_potential != CDFT_POT_WHDF || d.lj_rcut > d.lj_sigma, "WHDF needs rcut > sigma");
   }
   REQUIRE(d.weights_mode == CDFT_WEIGHTS_ACCURATE || d.weights_mode == CDFT_WEIGHTS_REFERENCE, "unknown weights mode");
+  REQUIRE(d.ao_force == CDFT_AO_FORCE_EXACT || d.ao_force == CDFT_AO_FORCE_REFERENCE, "unknown AO force mode");
   REQUIRE(d.fixed_mass >= 0, "the fixed mass must be non-negative");
   REQUIRE(d.boundary >= 0 && d.boundary <= (CDFT_FIXED_BACKGROUND | CDFT_HOMOGENEOUS_BOUNDARY), "unknown boundary flags");
   // src/Species.cpp:75-76
@@ -683,7 +684,7 @@ void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<
 
 template <int MODEL>
 void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 1.0,
-                 const cplx<double>* B2 = nullptr) {
+                 const cplx<double>* B2 = nullptr, int single_offdiag = 0) {
   if (!B2) B2 = s.B2.as<cplx<double>>();
   const Tables<double>& tw = tables<double>(s);
   const int M = s.g.Nz / 2;
@@ -702,7 +703,7 @@ void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 
                        size_t(RB) * per, s.stream, s.B.as<cplx<double>>(), B2, s.Nkp, s.m,
                        mf ? 1 : 0, s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
                        tw.n.template as<cplx<double>>(), RB, pp, inv_dp > 0 ? inv_dp : pp.inv_d, scale,
-                       s.flag.as<int>(), bt, s.trace);
+                       s.flag.as<int>(), bt, s.trace, single_offdiag);
     HIPCHK(hipGetLastError());
     done = true;
   });
@@ -805,12 +806,13 @@ void eval_device_ao(cdft_system_s& s, const double* rho, double* dF, bool with_d
   forward_weighted(s, s.psi.as<double>(), A2, B2, s.m, s.Wao.as<double>());
   forward_weighted(s, rho, A2, B, s.m);
   s.mark("upsilon");
+  const int so = s.d.ao_force == CDFT_AO_FORCE_REFERENCE;
   switch (s.d.model) {
-    case CDFT_ROSENFELD: launch_hess<M_ROSENFELD>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
-    case CDFT_RSLT: launch_hess<M_RSLT>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
-    case CDFT_ESFMT: launch_hess<M_ESFMT>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
-    case CDFT_WHITEBEAR_I: launch_hess<M_WBI>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
-    default: launch_hess<M_WBII>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir); break;
+    case CDFT_ROSENFELD: launch_hess<M_ROSENFELD>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir, nullptr, so); break;
+    case CDFT_RSLT: launch_hess<M_RSLT>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir, nullptr, so); break;
+    case CDFT_ESFMT: launch_hess<M_ESFMT>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir, nullptr, so); break;
+    case CDFT_WHITEBEAR_I: launch_hess<M_WBI>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir, nullptr, so); break;
+    default: launch_hess<M_WBII>(s, false, 1.0 / (2 * s.d.ao_Rp), s.d.ao_reservoir, nullptr, so); break;
   }
   launch_cols<double>(s, B, Nkp, s.m, 1, true, nb, sB);
   conv_bwd_f64(s, B, A2, s.W.as<double>(), s.m);

@@ -322,7 +322,7 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
                                                  int Zp, const cplx<double>* __restrict__ twM,
                                                  const cplx<double>* __restrict__ twN, int RB, PhiParams pp,
                                                  double inv_dp, double scale, int* __restrict__ flag, Batch bt,
-                                                 int trace) {
+                                                 int trace, int single_offdiag) {
   B += blockIdx.y * bt.sb;
   B2 += blockIdx.y * bt.sb;
   if (vmf) vmf += blockIdx.y * bt.sr;
@@ -380,6 +380,11 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
       out[1] = out[1] + out[10];
       out[5] = out[5] - out[10];
       out[8] = out[8] - out[10];
+    }
+    if (single_offdiag) {  // the reference's AO force: one of T_jk, T_kj per off-diagonal array (Species.h:406-425)
+      out[6] = 0.5 * out[6];
+      out[7] = 0.5 * out[7];
+      out[9] = 0.5 * out[9];
     }
     if (nv[0] > 0.5 && 1 - nv[0] < 0.0) bad = 1;
 #pragma unroll

@@ -139,6 +139,13 @@ FMT_Species::FMT_Species(Density& density, double hsd, double mu, int seq) : Spe
     throw std::runtime_error("hsd is less than the lattice spacing ... aborting");
 }
 
+namespace {
+int ao_force_of(const Species* s) {
+  auto* fs = dynamic_cast<const FMT_Species*>(s);
+  return fs ? fs->ao_force() : CDFT_AO_FORCE_EXACT;
+}
+}  // namespace
+
 // ---------------------------------------------------------------------- FMT
 FMT::~FMT() {
   if (sys_) cdft_system_destroy(sys_);
@@ -150,7 +157,8 @@ std::string esFMT::Name() const {
 
 cdft_system FMT::system_for(const Species& s) const {
   const Lattice& L = s.getLattice();
-  if (sys_ && sys_lattice_ == &L && sys_hsd_ == s.getHSD() && sys_species_ == &s) return sys_;
+  if (sys_ && sys_lattice_ == &L && sys_hsd_ == s.getHSD() && sys_species_ == &s && sys_aof_ == ao_force_of(&s))
+    return sys_;
   if (sys_) cdft_system_destroy(sys_);
   cdft_system_desc d{};
   d.Nx = (int)L.Nx();
@@ -167,10 +175,12 @@ cdft_system FMT::system_for(const Species& s) const {
   if (auto* fs = dynamic_cast<const FMT_Species*>(&s)) {
     d.ao_Rp = fs->ao_Rp();
     d.ao_reservoir = fs->ao_reservoir();
+    d.ao_force = fs->ao_force();
   }
   check(cdft_system_create(&d, &sys_));
   sys_lattice_ = &L;
   sys_hsd_ = s.getHSD();
+  sys_aof_ = ao_force_of(&s);
   sys_species_ = &s;
   return sys_;
 }
@@ -495,7 +505,7 @@ cdft_system DFT::system() const {
                   (s->is_boundary_homogeneous() ? CDFT_HOMOGENEOUS_BOUNDARY : 0);
   const double mfs = interactions_.empty() ? 1.0 : interactions_.front()->scale();
   if (sys_ && sys_mu_ == mu && sys_mass_ == mass && sys_bnd_ == bnd && sys_mfs_ == mfs &&
-      sys_field_ == D.field_version())
+      sys_field_ == D.field_version() && sys_aof_ == ao_force_of(s))
     return sys_;
   invalidate();
   cdft_system_desc d{};
@@ -513,6 +523,7 @@ cdft_system DFT::system() const {
   if (auto* fs = dynamic_cast<const FMT_Species*>(s)) {
     d.ao_Rp = fs->ao_Rp();
     d.ao_reservoir = fs->ao_reservoir();
+    d.ao_force = fs->ao_force();
   }
   d.ideal = 1;
   d.mu = mu;
@@ -525,6 +536,7 @@ cdft_system DFT::system() const {
   sys_mass_ = mass;
   sys_bnd_ = bnd;
   sys_mfs_ = mfs;
+  sys_aof_ = ao_force_of(s);
   sys_field_ = D.field_version();
   return sys_;
 }

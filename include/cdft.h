@@ -100,6 +100,8 @@ enum cdft_weights_mode {
                                  rounding noise (up to ~1e-6 relative per cell at dx = 0.1, ~1e-4 at dx = 0.04) */
 };
 
+enum cdft_ao_force { CDFT_AO_FORCE_EXACT = 0, CDFT_AO_FORCE_REFERENCE = 1 };
+
 typedef struct cdft_system_s* cdft_system;
 
 typedef struct cdft_system_desc {
@@ -142,6 +144,11 @@ typedef struct cdft_system_desc {
   /* how generated FMT (and AO polymer) cell weights are computed when fmt_weights / ao_weights is NULL:
      enum cdft_weights_mode */
   int weights_mode;
+  /* AO polymer force, enum cdft_ao_force: CDFT_AO_FORCE_EXACT (0) is the derivative of the AO free energy;
+     CDFT_AO_FORCE_REFERENCE counts each off-diagonal tensor measure once, as
+     FMT_AO_Species::setFundamentalMeasures_AO does (include/Species.h:406-425: TI(j,k) and TI(k,j) name the
+     same array, and the later write replaces the earlier one) */
+  int ao_force;
 } cdft_system_desc;
 
 /* A pair potential for the host-side helpers below (fields as in cdft_system_desc). */

@@ -172,6 +172,7 @@ class FMT_Species : public Species {
   // Asakura-Oosawa polymer parameters (0 for a plain FMT species)
   virtual double ao_Rp() const { return 0.0; }
   virtual double ao_reservoir() const { return 0.0; }
+  virtual int ao_force() const { return CDFT_AO_FORCE_EXACT; }
 
  private:
   double hsd_;
@@ -186,11 +187,16 @@ class FMT_AO_Species : public FMT_Species {
       : FMT_Species(density, hsd, mu, seq), Rp_(Rp), reservoir_density_(reservoir_density) {}
   double ao_Rp() const override { return Rp_; }
   double ao_reservoir() const override { return reservoir_density_; }
+  int ao_force() const override { return reference_force_ ? CDFT_AO_FORCE_REFERENCE : CDFT_AO_FORCE_EXACT; }
+  // true: the force counts each off-diagonal tensor measure once, as setFundamentalMeasures_AO does
+  // (include/Species.h:406-425); false (default): the derivative of the AO free energy.
+  void setReferenceForce(bool on) { reference_force_ = on; }
   double getReservoirDensity() const { return reservoir_density_; }
   double getHSDP() const { return 2 * Rp_; }
 
  private:
   double Rp_, reservoir_density_;
+  bool reference_force_ = false;
 };
 
 // ------------------------------------------------------------------- FMT ---
@@ -223,6 +229,7 @@ class FMT {
   mutable cdft_system sys_ = nullptr;
   mutable const Lattice* sys_lattice_ = nullptr;
   mutable double sys_hsd_ = -1;
+  mutable int sys_aof_ = -1;
   mutable const Species* sys_species_ = nullptr;
   cdft_system system_for(const Species& s) const;
 };
@@ -465,7 +472,7 @@ class DFT {
   int precision_ = CDFT_FP64;
   mutable cdft_system sys_ = nullptr;
   mutable double sys_mu_ = NAN, sys_mass_ = NAN, sys_mfs_ = NAN;
-  mutable int sys_bnd_ = -1;
+  mutable int sys_bnd_ = -1, sys_aof_ = -1;
   mutable unsigned sys_field_ = ~0u;
   double F_id_ = 0, F_ext_ = 0, F_hs_ = 0, F_mf_ = 0;
   bool full_hessian_ = true, use_squared_matrix_ = false;

@@ -39,6 +39,32 @@ def test_fmt_with_ao_matches_oracle(cdft, model, name):
     assert maxrel(d1, d0) <= TOL, maxrel(d1, d0)
 
 
+@pytest.mark.parametrize("model,name", [(3, "WHITEBEAR_I"), (2, "ESFMT"), (0, "ROSENFELD")])
+def test_ao_reference_force_matches_oracle_single_count(cdft, model, name):
+    """ao_force=AO_FORCE_REFERENCE against the oracle's exact=False (setFundamentalMeasures_AO keeps one of
+    T_jk, T_kj); the free energy is the same in both modes, the force differs where the tensor measures do."""
+    rho = density(0x5EED8200 + model)
+    w = oracle.fmt_weights(SHAPE, DX, HSD)
+    wp = oracle.fmt_weights(SHAPE, DX, 2 * RP)
+    A, B = (1.0, -0.5) if model == 2 else (1.0, 0.0)
+    o = oracle.FMT(model, SHAPE, DX, HSD, weights=w, A=A, B=B)
+    o.set_ao(RP, RHO_P, ao_weights=wp, exact=False)
+    F0, d0 = o.eval(rho)
+    s = cdft.System(SHAPE, DX, HSD, model=getattr(cdft, name), A=A, B=B, weights=w, ao_Rp=RP, ao_reservoir=RHO_P,
+                    ao_weights=wp, ao_force=cdft.AO_FORCE_REFERENCE)
+    F1, d1 = s.fmt_eval(rho)
+    assert abs(F1 - F0) <= TOL * abs(F0), (F1, F0)
+    assert maxrel(d1, d0) <= TOL, maxrel(d1, d0)
+    e = cdft.System(SHAPE, DX, HSD, model=getattr(cdft, name), A=A, B=B, weights=w, ao_Rp=RP, ao_reservoir=RHO_P,
+                    ao_weights=wp)
+    F2, d2 = e.fmt_eval(rho)
+    assert F2 == pytest.approx(F1, rel=1e-14)
+    if model != 0:  # Rosenfeld has no tensor measures: both modes agree
+        assert maxrel(d2, d1) > 1e3 * TOL
+    else:
+        assert maxrel(d2, d1) <= 1e-14
+
+
 def test_full_dft_with_ao(cdft):
     kT, mu = 1.0, -0.4
     rho = density(0x5EED8100, rho0=0.4)
