@@ -136,6 +136,27 @@ def fire_time_to_tol(cdft, n=128, tol=1e-8, max_steps=20000, precision=0):
     return info
 
 
+def fire_ladder_time_to_tol(cdft, n=128, tol=1e-8, ladder_tol=1e-5, max_steps=20000):
+    """The same slit pore by the precision ladder: mixed-precision FIRE2 until max |dF/drho| / dV <= ladder_tol
+    (the FP32 spectra's gradient floor is ~1e-6), then FP64 FIRE2 from that density to tol; wall time of both."""
+    s_mx, rho0, info = slit_pore(cdft, n=n, precision=cdft.MIXED)
+    s64, _, _ = slit_pore(cdft, n=n, precision=cdft.FP64)
+    t0 = time.perf_counter()
+    fm = cdft.Fire(s_mx, rho0)
+    _, repm = fm.run(max_steps, ladder_tol)
+    rm = fm.state()[0]
+    t1 = time.perf_counter()
+    f64 = cdft.Fire(s64, rm)
+    rc, rep = f64.run(max_steps, tol)
+    t2 = time.perf_counter()
+    info.update({"seconds": t2 - t0, "seconds_mixed": t1 - t0, "seconds_fp64": t2 - t1, "ladder_tol": ladder_tol,
+                 "steps_mixed": repm["steps"], "steps_fp64": rep["steps"], "F": rep["F"], "fmax_raw": rep["fmax_raw"],
+                 "converged": bool(rep["fmax_raw"] <= tol), "status": rc})
+    for o in (fm, f64, s_mx, s64):
+        o.close()
+    return info
+
+
 def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha=50.0, dt=1e-4, precision=0):
     """DDFT from a liquid with a 4x4x4 fcc seed of Gaussians (alpha/pi)^{3/2} exp(-alpha |r - R|^2) centred in the
     box, dt pinned at 1e-4; ms per step and the relative mass drift."""
@@ -267,17 +288,112 @@ def batch_fire(cdft, classes=((48, 32), (64, 32), (96, 32)), dx=0.1, kT=0.8, tol
     return info
 
 
-def cpu_baseline(rho, threads):
+def host_info():
+    """The host CPU the baseline ran on: logical CPUs, lscpu's model name and core layout."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and keys[k.strip()] not in info:
+                info[keys[k.strip()]] = v.strip()
+    except Exception as e:  # lscpu missing: fall back to /proc/cpuinfo
+        info["lscpu_error"] = str(e)
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    try:
+        info["physical_cores"] = int(info["sockets"]) * int(info["cores_per_socket"])
+    except (KeyError, ValueError):
+        pass
+    return info
+
+
+def cpu_fmt(rho, shape, dx, threads, fft_threads, reps):
+    """Median seconds of `reps` oracle FMT F+dF evaluations after one warm-up (weight set-up excluded)."""
+    import oracle
+    oracle.set_threads(threads, fft_threads)
+    h = oracle.FMT(oracle.WBI, shape, dx, HSD, weights=oracle.fmt_weights(shape, dx, HSD,
+                                                                          mode=oracle.WEIGHTS_ACCURATE))
+    F, dF = h.eval(rho)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        F, dF = h.eval(rho)
+        ts.append(time.perf_counter() - t0)
+    del h
+    return float(np.median(ts)), ts, F, dF
+
+
+def cpu_fire_estimate(cdft, info, threads, steps_gpu, steps=3):
+    """Config 2 on the CPU: seconds per FIRE2 step of the oracle on the same slit pore (White Bear I + LJ QF
+    mean field, walls, mu of rho_b = 0.7), times the GPU's step count to 1e-8: a labelled estimate."""
     import oracle
     oracle.set_threads(threads, threads)
-    t0 = time.time()
-    h = oracle.FMT(oracle.WBI, SHAPE, DX, HSD, weights=oracle.fmt_weights(SHAPE, DX, HSD,
-                                                                          mode=oracle.WEIGHTS_ACCURATE))
-    setup = time.time() - t0
-    t0 = time.time()
-    F, dF = h.eval(rho)
-    dt = time.time() - t0
-    return dt, setup, F, dF
+    shape = tuple(info["grid"])
+    n, dx, kT = shape[0], info["dx"], info["kT"]
+    L = n * dx
+    z = np.arange(n) * dx
+    vz = 10.0 * (np.exp(-z / 0.2) + np.exp(-(L - z) / 0.2))
+    vext = np.ascontiguousarray(np.broadcast_to(vz, shape))
+    t0 = time.perf_counter()
+    o = oracle.DFT(shape, dx, info["hsd"], mf=True, kT=kT, mu=info["mu"], vext=vext, accurate_weights=True)
+    f = oracle.Fire(o, info["rho_bulk"] * np.exp(-vext))
+    setup = time.perf_counter() - t0
+    ts = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        f.step()
+        ts.append(time.perf_counter() - t1)
+    per = float(np.median(ts))
+    return {"seconds_per_step": per, "steps_timed": steps, "setup_seconds": setup, "gpu_steps_to_tol": steps_gpu,
+            "estimate_seconds_to_tol": per * steps_gpu, "threads": threads,
+            "label": "estimate: CPU oracle seconds per FIRE2 step x the GPU run's step count to 1e-8"}
+
+
+def cpu_legs(cdft, rho, threads, fire_info):
+    """CPU baselines of the FP64 oracle: 256^3 with single-threaded FFTs (reference-faithful: the reference's FFTW
+    plans run on one thread, include/DFT_Factory.h:86-90) and with threaded FFTs, the 32^3 config-1 case, and the
+    config-2 FIRE estimate."""
+    out = {"host": host_info(), "threads": threads,
+           "fft_backend": "repo-written mixed-radix CPU FFT" + (" (fftw3.h present but unused)"
+                                                                 if os.path.exists("/usr/include/fftw3.h") else
+                                                                 " (FFTW3 is not installed)")}
+    t_ii, ts_ii, F, dF = cpu_fmt(rho, SHAPE, DX, threads, threads, 3)
+    out["fmt_256_threaded_fft"] = {"seconds": t_ii, "runs": ts_ii, "evals_per_s": 1.0 / t_ii,
+                                   "pointwise_threads": threads, "fft_threads": threads}
+    t_i, ts_i, _, _ = cpu_fmt(rho, SHAPE, DX, threads, 1, 1)
+    out["fmt_256_reference_faithful"] = {"seconds": t_i, "runs": ts_i, "evals_per_s": 1.0 / t_i,
+                                         "pointwise_threads": threads, "fft_threads": 1}
+    shape1 = (32, 32, 32)
+    u = splitmix64_uniform(32 ** 3, 0x5EED0001).reshape(shape1)
+    rho1 = 0.6 * (1.0 + 0.05 * (2.0 * u - 1.0))
+    for key, ft in (("fmt_32_reference_faithful", 1), ("fmt_32_threaded_fft", threads)):
+        t, ts, _, _ = cpu_fmt(rho1, shape1, 0.1, threads, ft, 5)
+        out[key] = {"seconds": t, "evals_per_s": 1.0 / t, "pointwise_threads": threads, "fft_threads": ft}
+    if fire_info:
+        out["config2_fire"] = cpu_fire_estimate(cdft, fire_info, threads, fire_info["steps"])
+    return out, F, dF
+
+
+def sources_sha16():
+    """Digest of the HIP sources the library is built from (ties a measured profile to the kernels it measured)."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "classicaldft_amd", "csrc")
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".hpp", ".cpp")):
+            with open(os.path.join(d, f), "rb") as fh:
+                h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
 
 
 def warm_start_leg(cdft, count, check_unet=True):
@@ -371,11 +487,19 @@ def main():
     if dom:
         dominant = {"kernel": dom, "ms": st64[dom], "bytes": sb.get(dom),
                     "achieved_GBs": (sb[dom] / (st64[dom] * 1e-3) / 1e9) if dom in sb else None}
-    traffic = None
+    # measured HBM bytes per evaluation (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated): used only when the
+    # profile was measured on the kernels built from the current sources
+    traffic, traffic_note = None, "no traffic profile"
     tpath = os.path.join(ROOT, "profiles", "traffic_fmt_256.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_eval_fp64")
+            tj = json.load(f)
+        if tj.get("sources_sha16") == sources_sha16():
+            traffic = tj.get("hbm_bytes_per_eval_fp64")
+            traffic_note = f"profiles/traffic_fmt_256.json, measured on these sources ({tj['sources_sha16']})"
+        else:
+            traffic_note = (f"profiles/traffic_fmt_256.json was measured on other kernel sources "
+                            f"({tj.get('sources_sha16')}, current {sources_sha16()}): not used")
 
     result = {
         "metric": "FMT F+dF evals/s @256^3 (FP64 vs mixed); FIRE time-to-tol, warm vs cold",
@@ -394,7 +518,8 @@ def main():
                    "model": "WhiteBearI", "density": "0.6*(1+0.05*U[-1,1]) splitmix64 seed 0x5EED0256",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
-                     "traffic": traffic, "scope": "one F+dF evaluation (all kernels), algorithmic bytes B(8,11)",
+                     "traffic": traffic, "traffic_source": traffic_note,
+                     "scope": "one F+dF evaluation (all kernels), algorithmic bytes B(8,11)",
                      "algorithmic_bytes": B64, "dominant_kernel": dominant},
         "mixed": {"evals_per_s": 1000.0 / tmx, "ms_per_step": tmx, "speedup_vs_fp64": t64 / tmx,
                   "F_rel_err_vs_fp64": abs(Fmx - F64) / abs(F64),
@@ -408,6 +533,8 @@ def main():
     if args.fire_grid:
         print("# FIRE time to tolerance", flush=True)
         result["fire_time_to_tol"] = {"slit_pore": fire_time_to_tol(cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
+                                      "slit_pore_mixed_then_fp64": fire_ladder_time_to_tol(
+                                          cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
                                       "start": "rho_bulk exp(-V_ext)", "tol": 1e-8,
                                       "criterion": "max_i |dF/drho_i| / dV (raw gradient)"}
     if args.warm_systems:
@@ -423,14 +550,15 @@ def main():
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"# cpu baseline ({threads} threads)", flush=True)
-        dt, setup, Fc, dFc = cpu_baseline(rho, threads)
+        slit = result.get("fire_time_to_tol", {}).get("slit_pore")
+        legs, Fc, dFc = cpu_legs(cdft, rho, threads, slit)
+        ref = legs["fmt_256_reference_faithful"]
         result["cpu_baseline"] = {
-            "value": 1.0 / dt, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "one 256^3 White Bear I F+dF evaluation of the same density by the FP64 C++ oracle "
-                      "(OpenMP pointwise loops and FFT rows, repo-written CPU FFT; weight set-up excluded, "
-                      f"{setup:.1f} s)",
-            "seconds": dt,
-            "gpu_speedup": (1.0 / dt and (1000.0 / t64) / (1.0 / dt)),
+            "value": ref["evals_per_s"], "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": "one 256^3 White Bear I F+dF evaluation of the same density by the FP64 C++ oracle after a "
+                      "warm-up, OpenMP pointwise loops on all the threads used and single-threaded FFTs as the "
+                      "reference runs them; weight set-up excluded",
+            "seconds": ref["seconds"], "gpu_speedup": (1000.0 / t64) / ref["evals_per_s"], **legs,
         }
         result["parity_256"] = {"F_rel": abs(F64 - Fc) / abs(Fc),
                                 "dF_maxrel": float(np.abs(dF64 - dFc).max() / np.abs(dFc).max()),

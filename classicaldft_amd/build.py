@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-HIP_SOURCES = ("cdft_api.hip", "solvers.hip", "batch.hip", "unet.hip")
+HIP_SOURCES = ("cdft_api.hip", "pointwise.hip", "solvers.hip", "batch.hip", "unet.hip")
 # host-only C++ of libcdft.so, compiled by g++ without floating-point contraction (weights_ref.cpp reproduces the
 # reference's double expressions operation for operation)
 HOST_SOURCES = ("weights_ref.cpp",)

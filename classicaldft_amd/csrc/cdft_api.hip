@@ -8,6 +8,12 @@
 #include <random>
 
 #include "system.hpp"
+#include "pw3.hpp"
+
+namespace cdft {
+template <typename T>
+long launch_pw3(const PwArgs& a, int M);  // pointwise.hip
+}
 
 namespace cdft_impl {
 
@@ -347,6 +353,7 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
     if (n > 2) s->nb_half_bwd = e[2] == '1';
   }
   if (const char* e = std::getenv("CDFT_PW_ROWS")) s->pw_rows = std::atoi(e);
+  if (const char* e = std::getenv("CDFT_PW3")) s->pw3 = e[0] != '0';
   // y / z / y section in x-slabs of ~192 MB (all arrays of all members): each slab stays in the 256 MB
   // last-level cache between its three kernels (measured optimum at 256^3, FP64 and mixed)
   s->slab_bytes = 192L << 20;
@@ -420,8 +427,16 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
   const int M = s.g.Nz / 2;
   if (x1 < 0) x1 = s.g.Nx;
   const long nrows = (long)(x1 - x0) * s.g.Ny;
-  PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
-  if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
+  const PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B, MODEL == M_WBI || MODEL == M_WBII);
+  if (rows_reg_ok<T>(s) && s.pw3 && s.pw_rows <= 0) {
+    const PwArgs pa{s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
+                    s.vmf.p ? s.vmf.as<double>() + (long)x0 * s.g.Ny * s.g.Nz : nullptr, nrows, s.g.Zp,
+                    tw.m.p, tw.n.p, pp, s.partials.as<double2>() + blk_off, s.flag.as<int>(),
+                    Batch{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr}, s.trace, MODEL, s.nb, s.stream};
+    const long nblk = launch_pw3<T>(pa, M);
+    REQUIRE(nblk != -2, "launch of the pointwise kernel failed");
+    if (nblk > 0) return nblk;
+  }
   if (rows_reg_ok<T>(s)) {
     long nblk = -1;
     with_reg_len(M, [&](auto mc) {
@@ -688,8 +703,7 @@ void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 
   if (!B2) B2 = s.B2.as<cplx<double>>();
   const Tables<double>& tw = tables<double>(s);
   const int M = s.g.Nz / 2;
-  PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
-  if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
+  const PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B, MODEL == M_WBI || MODEL == M_WBII);
   bool done = false;
   with_reg_len(M, [&](auto mc) {
     constexpr int Mc = decltype(mc)::value;
@@ -731,8 +745,7 @@ template <int MODEL>
 long launch_pw_ao(cdft_system_s& s, int nconv) {
   const Tables<double>& tw = tables<double>(s);
   const int M = s.g.Nz / 2;
-  PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B);
-  if (MODEL == M_WBI || MODEL == M_WBII) { pp.A = 1.0; pp.B = -1.0; }
+  const PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B, MODEL == M_WBI || MODEL == M_WBII);
   long nblk = -1;
   with_reg_len(M, [&](auto mc) {
     constexpr int Mc = decltype(mc)::value;
@@ -1703,8 +1716,8 @@ int cdft_system_bulk(cdft_system sys, double rho, double* f, double* mu) {
       omega(d, om);
       double n[11], out[11];
       for (int a = 0; a < 11; ++a) n[a] = rho * om[a];
-      PhiParams pp = make_phi_params(d, sys->d.A, sys->d.B);
-      if (sys->d.model == CDFT_WHITEBEAR_I || sys->d.model == CDFT_WHITEBEAR_II) { pp.A = 1.0; pp.B = -1.0; }
+      const PhiParams pp = make_phi_params(d, sys->d.A, sys->d.B,
+                                           sys->d.model == CDFT_WHITEBEAR_I || sys->d.model == CDFT_WHITEBEAR_II);
       switch (sys->d.model) {
         case CDFT_ROSENFELD: fex = phi_site<M_ROSENFELD>(n, pp, out); break;
         case CDFT_RSLT: fex = phi_site<M_RSLT>(n, pp, out); break;

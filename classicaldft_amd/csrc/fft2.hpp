@@ -110,11 +110,11 @@ constexpr bool reg_fft_supported() {
 // lds: this sequence's exchange slot, element (k1, n2) at lds[(k1 (R2 + PAD) + n2) ls]
 // (PAD = 1 breaks the power-of-two stride of the pass-2 reads when ls = 1);
 // tw: W_L^j table (j < L).  The caller must __syncthreads() before reusing lds.
+// The two halves of reg_fft_seq; the caller synchronises the block between them.
 template <typename T, int L, int R1, int R2, bool FWD, int PAD = 0>
-__device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1],
-                                            cplx<T> (&v2)[R1 / cgcd(R1, R2)][R2], int t, cplx<T>* lds, int ls,
-                                            const cplx<T>* __restrict__ tw) {
-  constexpr int P = cgcd(R1, R2), Q1 = R2 / P, Q2 = R1 / P, S = R2 + PAD;
+__device__ __forceinline__ void reg_fft_pass1(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1], int t, cplx<T>* lds, int ls,
+                                              const cplx<T>* __restrict__ tw) {
+  constexpr int P = cgcd(R1, R2), Q1 = R2 / P, S = R2 + PAD;
 #pragma unroll
   for (int q = 0; q < Q1; ++q) {
     const int n2 = t + P * q;
@@ -126,7 +126,12 @@ __device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1]
       lds[(k1 * S + n2) * ls] = x;
     }
   }
-  __syncthreads();
+}
+
+template <typename T, int L, int R1, int R2, bool FWD, int PAD = 0>
+__device__ __forceinline__ void reg_fft_pass2(cplx<T> (&v2)[R1 / cgcd(R1, R2)][R2], int t, const cplx<T>* lds,
+                                              int ls) {
+  constexpr int P = cgcd(R1, R2), Q2 = R1 / P, S = R2 + PAD;
 #pragma unroll
   for (int q = 0; q < Q2; ++q) {
     const int k1 = t + P * q;
@@ -134,6 +139,15 @@ __device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1]
     for (int n2 = 0; n2 < R2; ++n2) v2[q][n2] = lds[(k1 * S + n2) * ls];
     dft_small<T, R2, FWD>(v2[q]);
   }
+}
+
+template <typename T, int L, int R1, int R2, bool FWD, int PAD = 0>
+__device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1],
+                                            cplx<T> (&v2)[R1 / cgcd(R1, R2)][R2], int t, cplx<T>* lds, int ls,
+                                            const cplx<T>* __restrict__ tw) {
+  reg_fft_pass1<T, L, R1, R2, FWD, PAD>(v1, t, lds, ls, tw);
+  __syncthreads();
+  reg_fft_pass2<T, L, R1, R2, FWD, PAD>(v2, t, lds, ls);
 }
 
 }  // namespace cdft

@@ -16,17 +16,24 @@ namespace cdft {
 
 enum { M_ROSENFELD = 0, M_RSLT = 1, M_ESFMT = 2, M_WBI = 3, M_WBII = 4 };
 
+constexpr double kPi = 3.14159265358979323846;
+
 struct PhiParams {
   double hsd;
   double A, B;  // esFMT coefficients (WB I/II: 1, -1)
   double inv_d, inv_d2;  // 1/hsd, 1/hsd^2
+  double ca, cb, ea, eb;  // A/(24 pi), B/(24 pi), A/(8 pi), B/(4 pi): site-independent factors of the tensor terms
 };
 
-inline PhiParams make_phi_params(double hsd, double A, double B) {
-  return PhiParams{hsd, A, B, 1.0 / hsd, 1.0 / (hsd * hsd)};
+// WB I and WB II are esFMT(1, -1) whatever A and B the caller holds (FMT.h:505-643).
+inline PhiParams make_phi_params(double hsd, double A, double B, bool whitebear = false) {
+  if (whitebear) {
+    A = 1.0;
+    B = -1.0;
+  }
+  return PhiParams{hsd, A, B, 1.0 / hsd, 1.0 / (hsd * hsd), A / (24 * kPi), B / (24 * kPi), A / (8 * kPi),
+                   B / (4 * kPi)};
 }
-
-constexpr double kPi = 3.14159265358979323846;
 
 // Forward-mode dual number v + d e (e^2 = 0): evaluating the derivative code on
 // n + psi e gives, in the e part, the second derivative of Phi applied to psi
@@ -61,14 +68,18 @@ __host__ __device__ inline Dual flog(const Dual& x) { return Dual(log(x.v), x.d 
 // eta-dependent cofactors f1, f2, f3 and first derivatives.  Written with the
 // two reciprocals 1/(1-x) and 1/x instead of one division per term (a double
 // division is ~10 instructions on the GPU); same formulas as the reference.
+// Both reciprocals are correctly rounded, as the reference's divisions are:
+// f3 and f3' cancel terms of order 1/x^2 and 1/x^3 at small x, so a reciprocal
+// one ulp off would move them far beyond the rounding of the reference.
 template <int MODEL, typename S = double>
 __host__ __device__ inline void fcoef(S x, S& f1, S& f1p, S& f2, S& f2p, S& f3, S& f3p) {
   const S om = 1.0 - x;
+  const bool series = x < 1e-12;
   const S f = 1.0 / om;
   f1 = flog(om);  // FMT.h:153-156
   f1p = -f;
   if constexpr (MODEL == M_WBII) {  // FMT.h:588-632
-    if (x < 1e-12) {
+    if (series) {
       f2 = 1 + x * (1 + x * ((10.0 / 9) + (7.0 / 6) * x));
       f2p = 1 + x * ((20.0 / 9) + 3.5 * x);
       f3 = 1.5 + (7.0 / 3) * x + 3.25 * x * x + 4.2 * x * x * x;
@@ -84,7 +95,7 @@ __host__ __device__ inline void fcoef(S x, S& f1, S& f1p, S& f2, S& f2p, S& f3, 
     f2 = f;  // FMT.h:168-183
     f2p = f * f;
     if constexpr (MODEL == M_RSLT || MODEL == M_WBI) {  // FMT.h:267-281, 536-550
-      if (x < 1e-12) {
+      if (series) {
         f3 = 1.5 + (8.0 / 3) * x + 3.75 * x * x + 4.8 * x * x * x;
         f3p = (8.0 / 3) + 7.5 * x + 14.4 * x * x;
       } else {
@@ -136,7 +147,6 @@ __host__ __device__ inline S phi_meas(const S& eta, const S& s0, const S& s1, co
       for (int k = 0; k < 3; ++k) dV2[k] = -(1.0 / (6 * kPi)) * s2 * v2[k] * q * q;
     }
   } else {  // esFMT family: FMT.h:396-455
-    const double A = pp.A, B = pp.B;
     const S Txx = T6[0], Txy = T6[1], Txz = T6[2], Tyy = T6[3], Tyz = T6[4], Tzz = T6[5];
     // TT = T.T (symmetric)
     const S Qxx = Txx * Txx + Txy * Txy + Txz * Txz;
@@ -151,13 +161,13 @@ __host__ __device__ inline S phi_meas(const S& eta, const S& s0, const S& s1, co
     const S vTv = v2[0] * vT0 + v2[1] * vT1 + v2[2] * vT2;
     const S T2 = Qxx + Qyy + Qzz;
     const S T3 = Qxx * Txx + Qyy * Tyy + Qzz * Tzz + 2 * (Qxy * Txy + Qxz * Txz + Qyz * Tyz);
-    const double ca = A / (24 * kPi), cb = B / (24 * kPi);
+    const double ca = pp.ca, cb = pp.cb;
     P3 = ca * (s2 * s2 * s2 - 3 * s2 * v2v2 + 3 * vTv - T3) + cb * (s2 * s2 * s2 - 3 * s2 * T2 + 2 * T3);
     dS2 = ca * (3 * s2 * s2 - 3 * v2v2) + cb * (3 * s2 * s2 - 3 * T2);
     dV2[0] = ca * (-6 * s2 * v2[0] + 6 * vT0);
     dV2[1] = ca * (-6 * s2 * v2[1] + 6 * vT1);
     dV2[2] = ca * (-6 * s2 * v2[2] + 6 * vT2);
-    const double ea = A / (8 * kPi), eb = B / (4 * kPi);
+    const double ea = pp.ea, eb = pp.eb;
     dT[0] = ea * (v2[0] * v2[0] - Qxx) + eb * (-s2 * Txx + Qxx);
     dT[1] = ea * (v2[0] * v2[1] - Qxy) + eb * (-s2 * Txy + Qxy);
     dT[2] = ea * (v2[0] * v2[2] - Qxz) + eb * (-s2 * Txz + Qxz);

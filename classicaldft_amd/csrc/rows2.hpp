@@ -31,13 +31,13 @@ struct RowF {
   static constexpr int SLOT = slot_stride(R1 * (R2 + 1), int(sizeof(cplx<T>)));
 };
 
-// Deterministic block reduction of NQ compensated sums for blockDim.x <= kThreads:
+// Deterministic block reduction of NQ compensated sums for blockDim.x <= 1024:
 // a fixed xor butterfly inside each wavefront (lanes beyond the block's
 // thread count contribute zero), then thread 0 merges the wavefront results
 // in order and writes partials[blk*NQ+q].
 template <int NQ>
 __device__ void block_reduce_store_n(CSum* v, double2* __restrict__ partials, long blk) {
-  __shared__ double ws[NQ][kThreads / 64][2];
+  __shared__ double ws[NQ][16][2];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int nact = min(64, (int)blockDim.x - wid * 64);
 #pragma unroll
@@ -143,6 +143,38 @@ __device__ __forceinline__ void c2r_to_regs(const cplx<T>* __restrict__ src, con
 // X_k = (Z_k + Z*_{M-k})/2 - i W_N^k (Z_k - Z*_{M-k})/2,
 // X_0 = Re Z_0 + Im Z_0, X_M = Re Z_0 - Im Z_0.  The caller has synchronised so
 // that slot is free; every thread of the block must call this.
+template <typename T, int M>
+__device__ __forceinline__ void r2c_regs_to_slot(cplx<T>* slot, int t,
+                                                 cplx<T> (&v)[RowF<T, M>::Q2][RowF<T, M>::R2]) {
+  using RF = RowF<T, M>;
+#pragma unroll
+  for (int q = 0; q < RF::Q2; ++q)
+#pragma unroll
+    for (int k2 = 0; k2 < RF::R2; ++k2) slot[t + RF::P * q + RF::R1 * k2] = v[q][k2];
+}
+
+// Second half of r2c_from_regs (after the block has synchronised): the half spectrum from the slot to dst.
+template <typename T, int M>
+__device__ __forceinline__ void r2c_slot_to_row(const cplx<T>* slot, int t, const cplx<T>* __restrict__ twN,
+                                                cplx<T>* __restrict__ dst) {
+  using RF = RowF<T, M>;
+  cplx<T>* d = dst + t;
+  const cplx<T>* tw = twN + t;
+#pragma unroll
+  for (int i = 0; i < (M + RF::P) / RF::P; ++i) {
+    const int k = t + RF::P * i;
+    if (k > M) break;
+    cplx<T> X;
+    if (k == 0 || k == M) {
+      const cplx<T> z0 = slot[0];
+      X = {k == 0 ? z0.x + z0.y : z0.x - z0.y, T(0)};
+    } else {
+      r2c_post(slot[k], slot[M - k], tw[RF::P * i], X);
+    }
+    d[RF::P * i] = X;
+  }
+}
+
 template <typename T, int M>
 __device__ __forceinline__ void r2c_from_regs(cplx<T>* slot, int t, const cplx<T>* __restrict__ twN,
                                               cplx<T>* __restrict__ dst, bool store,

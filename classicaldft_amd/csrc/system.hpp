@@ -183,6 +183,7 @@ struct cdft_system_s {
   // forward convolution, backward convolution (tuned per precision at create)
   bool nb_half_cols = false, nb_half_fwd = false, nb_half_bwd = false;
   int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
+  bool pw3 = true;       // specialised pointwise kernel (pw3.hpp) where an instance covers the system
   long slab_bytes = 0;   // cache budget of one x-slab of the y / z / y section (0: no slabs)
   bool has_vext = false;
   bool last_with_dft = true;
