@@ -156,6 +156,34 @@ __global__ __launch_bounds__(256) void k_trace_dev(const double* __restrict__ w,
   }
 }
 
+// out = the k = 0 value of a weight spectrum times `scale` (zero for an odd weight), in FP64.
+__global__ void k_spec0(const cplx<double>* __restrict__ S, double* __restrict__ out, int odd, double scale) {
+  if (threadIdx.x == 0) *out = odd ? 0.0 : S[0].x * scale;
+}
+
+// Mean split of the mixed path, one thread per member: from ms[0] = sum rho, the means of the weighted densities
+// nbar_a = sum rho w_a(0) (w0 carries the spectra's 1/N; the mean-field convolution last), dPhi_a(nbar), and the
+// FMT force of the means dV N sum_a dPhi_a(nbar) w_a(0).  A uniform density is exactly its mean, so the FP32 arrays
+// carry only the deviations from it.
+template <int MODEL>
+__global__ void k_msplit(double* __restrict__ ms, const double* __restrict__ w0, int m, int nconv, PhiParams pp,
+                         double dVN, int nb) {
+  const int mb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mb >= nb) return;
+  double* q = ms + (long)mb * kMS;
+  const double S = q[0];
+  double n[11], out[11];
+  for (int a = 0; a < 11; ++a) n[a] = a < m ? S * w0[a] : 0.0;
+  for (int a = 0; a < nconv; ++a) q[kMSn + a] = S * w0[a];
+  phi_site<MODEL>(n, pp, out);
+  CSum K{0.0, 0.0};
+  for (int a = 0; a < m; ++a) {
+    q[kMSc + a] = out[a];
+    K.add(out[a] * w0[a]);
+  }
+  q[kMSk] = (K.s + K.c) * dVN;
+}
+
 // Whether the 11 real-space weights satisfy tr w_T = w_s to rounding (1e-13 of max |w_s|): true for the lattice
 // weights of FMT_Species::generateWeights (the tensor integrand's trace is the surface integrand), false for
 // weights carrying the reference generator's rounding noise.
@@ -180,7 +208,8 @@ bool trace_identity_holds(cdft_system_s& s, const double* wreal) {
 // With allow_trace (the colloid weights of a tensor model run by the register kernels), a weight set that satisfies
 // the trace identity is stored without T_zz and s.m, s.nconv, s.trace are updated.
 template <typename T>
-void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights, T* W, bool allow_trace = false) {
+void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights, T* W, bool allow_trace = false,
+                        double* w0 = nullptr) {
   const long N = s.N, Nkp = s.Nkp;
   DevBuf S;  // one double-precision spectrum
   S.alloc(sizeof(cplx<double>) * Nkp);
@@ -205,6 +234,9 @@ void fmt_weight_spectra(cdft_system_s& s, double hsd, const double* host_weights
       launch_cols<double>(s, S.as<cplx<double>>(), 0, 1, 0, true);
       hipLaunchKernelGGL((k_extract_weight<T>), dim3(nb), dim3(kThreads), 0, s.stream, S.as<cplx<double>>(),
                          W + a * Nkp, Nkp, (int)((s.odd_mask >> a) & 1u), 1.0 / (double)N);
+      if (w0)
+        hipLaunchKernelGGL(k_spec0, dim3(1), dim3(64), 0, s.stream, S.as<cplx<double>>(), w0 + a,
+                           (int)((s.odd_mask >> a) & 1u), 1.0 / (double)N);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s.stream));
@@ -276,7 +308,9 @@ void build_weight_spectra(cdft_system_s& s) {
   const bool allow_trace = !s.mixed && !s.has_ao && !(nt && nt[0] == '1') && rows_reg_ok<T>(s) && reg_fft_ok<T>(s, s.g.Nx) &&
                            reg_fft_ok<T>(s, s.g.Ny) && rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) &&
                            reg_fft_ok<double>(s, s.g.Ny);
-  if (s.m > 0) fmt_weight_spectra<T>(s, s.d.hsd, s.d.fmt_weights, s.W.as<T>(), allow_trace);
+  if (s.m > 0)
+    fmt_weight_spectra<T>(s, s.d.hsd, s.d.fmt_weights, s.W.as<T>(), allow_trace,
+                          s.mean_split ? s.w0.as<double>() : nullptr);
   if (s.has_ao) fmt_weight_spectra<T>(s, 2 * s.d.ao_Rp, s.d.ao_weights, s.Wao.as<T>());
   if (s.d.mean_field) {
     DevBuf w;
@@ -304,6 +338,9 @@ void build_weight_spectra(cdft_system_s& s) {
     const double dV = s.d.dx * s.d.dy * s.d.dz;
     hipLaunchKernelGGL((k_extract_weight<T>), dim3(nb), dim3(kThreads), 0, s.stream, S.as<cplx<double>>(),
                        s.W.as<T>() + s.m * Nkp, Nkp, 0, mf_scale(s.d) * dV / (double)N);
+    if (s.mean_split)
+      hipLaunchKernelGGL(k_spec0, dim3(1), dim3(64), 0, s.stream, S.as<cplx<double>>(), s.w0.as<double>() + s.m, 0,
+                         mf_scale(s.d) * dV / (double)N);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s.stream));
   }
@@ -406,6 +443,14 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
     HIPCHK(hipMemcpy(s->mus.p, mus, sizeof(double) * nb, hipMemcpyHostToDevice));
     if (nb == 1) s->d.mu = mus[0];
   }
+  // mean split in the mixed path: needs the register kernels (x convolution, pointwise and final z passes)
+  const char* nms = std::getenv("CDFT_NO_MEAN_SPLIT");
+  s->mean_split = s->mixed && s->m > 0 && !(nms && nms[0] == '1') && rows_reg_ok<float>(*s) &&
+                  reg_fft_ok<float>(*s, g.Nx);
+  if (s->mean_split) {
+    s->msplit.alloc(sizeof(double) * kMS * nb);
+    s->w0.alloc(sizeof(double) * 12);
+  }
   if (s->mixed) build_weight_spectra<float>(*s);
   else build_weight_spectra<double>(*s);
   // host pointers of the descriptor are not retained
@@ -424,6 +469,7 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
 template <typename T, int MODEL>
 long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long blk_off = 0) {
   const Tables<T>& tw = tables<T>(s);
+  const double* ms = std::is_same<T, float>::value && s.mean_split ? s.msplit.as<double>() : nullptr;
   const int M = s.g.Nz / 2;
   if (x1 < 0) x1 = s.g.Nx;
   const long nrows = (long)(x1 - x0) * s.g.Ny;
@@ -432,7 +478,8 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
     const PwArgs pa{s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
                     s.vmf.p ? s.vmf.as<double>() + (long)x0 * s.g.Ny * s.g.Nz : nullptr, nrows, s.g.Zp,
                     tw.m.p, tw.n.p, pp, s.partials.as<double2>() + blk_off, s.flag.as<int>(),
-                    Batch{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr}, s.trace, MODEL, s.nb, s.stream};
+                    Batch{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr}, s.trace, MODEL, s.nb, s.stream,
+                    ms};
     const long nblk = launch_pw3<T>(pa, M);
     REQUIRE(nblk != -2, "launch of the pointwise kernel failed");
     if (nblk > 0) return nblk;
@@ -452,7 +499,7 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
       hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk, s.nb), dim3(nconv * RB * RF::P),
                          size_t(RB) * per, s.stream, s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
                          vmf, nrows, s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, pp,
-                         s.partials.as<double2>() + blk_off, s.flag.as<int>(), bt, s.trace);
+                         s.partials.as<double2>() + blk_off, s.flag.as<int>(), bt, s.trace, ms);
       HIPCHK(hipGetLastError());
     });
     if (nblk > 0) return nblk;
@@ -530,9 +577,35 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
   const Tables<T>& tw = tables<T>(s);
   const int nb = s.nb;
   const long sA = Nkp, sB = s.nconv * Nkp;  // member strides of A and of the weight-array block B
+  const bool msplit = std::is_same<T, float>::value && s.mean_split;
+  long zblk = 0;
   if (nconv > 0) {
-    launch_rows_r2c<T, double>(s, rho, s.N, A, sA, nb);
+    // the z pass of rho; in the mixed path it also sums rho per member in FP64 (the mean split)
+    launch_rows_r2c<T, double>(s, rho, s.N, A, sA, nb, msplit ? s.partials.as<double2>() : nullptr, s.pslots,
+                               &zblk);
     s.mark("z_r2c");
+  }
+  if (nconv > 0 && msplit) {
+    // the means of the weighted densities and dPhi at them (k_msplit)
+    hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), zblk, 1,
+                       s.msplit.as<double>(), s.pslots, (long)kMS);
+    const PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B,
+                                         s.d.model == CDFT_WHITEBEAR_I || s.d.model == CDFT_WHITEBEAR_II);
+    const double dVN = s.d.dx * s.d.dy * s.d.dz * (double)s.N;
+    const dim3 g1(div_up(nb, 64)), b1(64);
+    double* msp = s.msplit.as<double>();
+    const double* w0 = s.w0.as<double>();
+    switch (s.d.model) {
+      case CDFT_ROSENFELD: hipLaunchKernelGGL(k_msplit<M_ROSENFELD>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
+      case CDFT_RSLT: hipLaunchKernelGGL(k_msplit<M_RSLT>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
+      case CDFT_ESFMT: hipLaunchKernelGGL(k_msplit<M_ESFMT>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
+      case CDFT_WHITEBEAR_I: hipLaunchKernelGGL(k_msplit<M_WBI>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
+      default: hipLaunchKernelGGL(k_msplit<M_WBII>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
+    }
+    HIPCHK(hipGetLastError());
+    s.mark("mean_split");
+  }
+  if (nconv > 0) {
     launch_cols<T>(s, A, 0, 1, 1, true, nb, sA);
     s.mark("y_fwd");
     if (reg_fft_ok<T>(s, s.g.Nx)) {
@@ -543,7 +616,8 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
           constexpr int NB = decltype(nbc)::value;
           const Batch bt{sA, sB, s.N, 0, nullptr};
           hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB), nb), dim3(NB * F2<Lc>::P), 0,
-                             s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>(), bt);
+                             s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>(), bt,
+                             (int)msplit);
           HIPCHK(hipGetLastError());
         });
       });
@@ -631,6 +705,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
   fp.mf = mf;
   fp.ext = with_dft && s.has_vext;
   fp.accumulate = accumulate;
+  fp.ms = msplit && s.m > 0 ? s.msplit.as<double>() : nullptr;
   {
     const int M = s.g.Nz / 2;
     long nblk = 0;
@@ -690,7 +765,7 @@ void forward_weighted(cdft_system_s& s, const double* in, cplx<double>* A, cplx<
       constexpr int NB = decltype(nbc)::value;
       const Batch bt{sA, sB, s.N, 0, nullptr};
       hipLaunchKernelGGL((k_conv_fwd2<double, Lc, NB>), dim3(div_up(cg.ncols, NB), s.nb), dim3(NB * F2<Lc>::P), 0,
-                         s.stream, A, B, Nkp, W, nconv, s.odd_mask, cg, tw.x.template as<cplx<double>>(), bt);
+                         s.stream, A, B, Nkp, W, nconv, s.odd_mask, cg, tw.x.template as<cplx<double>>(), bt, 0);
       HIPCHK(hipGetLastError());
     });
   });

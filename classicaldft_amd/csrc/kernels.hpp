@@ -519,10 +519,17 @@ __global__ __launch_bounds__(kThreads) void k_z_pointwise(cplx<T>* __restrict__ 
 // Final z pass: dF = [dF_in] + dV ln rho + dV c2r(acc) + v_mf + dV V_ext - dV mu,
 // in the order of src/DFT.cpp:377-425.  partials per block:
 // {sum (rho ln rho - rho) dV, sum rho v_mf, sum V_ext rho, sum rho}.
+// Per-member block of the mixed path's mean split (kMS doubles): [0] sum rho, [1 + a] the mean of weighted density
+// a (a < nconv, the mean-field convolution last), [13 + a] dPhi_a at those means (a < m), [25] the FMT force of the
+// means, dV N sum_a dPhi_a(nbar) w_a(k = 0).
+constexpr int kMS = 32;
+constexpr int kMSn = 1, kMSc = 13, kMSk = 25;
+
 struct FinalParams {
   double dV, mu;
   int ideal, fmt, mf, ext, accumulate;
   int hess;  // Hessian-vector mode of k_z_final2 (the V_ext pointer carries v)
+  const double* ms;  // mean split (mixed path): the constant [25] of the member joins the FMT force
 };
 
 template <typename T, int MAXPTS>

@@ -120,11 +120,12 @@ __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long 
 
 // x pass of rho_k, then per weight a: n_a(k) = rho_k w_a(k) and its inverse x
 // pass, straight from registers.  grid = (ceil(ncols/NB), members), block = NB * P.
+// zero_mean: rho_k(0) is dropped (the mean of every n_a is added back in FP64 by the pointwise kernel).
 template <typename T, int L, int NB>
 __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A, cplx<T>* __restrict__ B,
                                                    long arr_stride, const T* __restrict__ W, int nconv,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw,
-                                                   Batch bt) {
+                                                   Batch bt, int zero_mean) {
   A += blockIdx.y * bt.sa;
   B += blockIdx.y * bt.sb;
   using F = F2<L>;
@@ -167,6 +168,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
     }
   };
   reg_fft_seq<T, L, R1, R2, true>(v1, rk, t, buf + b, NB, tw);
+  if (zero_mean && c0 == 0 && t == 0) rk[0][0] = {T(0), T(0)};  // column (ky, kz) = 0, row kx = 0
   for (int a = 0; a < nconv; ++a) {
     const auto rb = buf_rsrc(B + a * arr_stride, cg.bytes);
     const bool odd = (odd_mask >> a) & 1u;

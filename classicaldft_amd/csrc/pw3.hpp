@@ -36,7 +36,8 @@ struct Pw3Geo {
 };
 
 // NA stored arrays (5 without tensor, 10 with T_zz from the trace identity, 11 otherwise), MF: one more sequence
-// holding the mean-field convolution, written to vmf in real space.  The row transforms run on the first NTF
+// holding the mean-field convolution, written to vmf in real space.  ms: the mean split of the mixed path (as in
+// k_z_pointwise2), null otherwise.  The row transforms run on the first NTF
 // threads (P per sequence, sequence s = a RB + r for array a of row r); the per-site Phi stage spreads the sites
 // of the block's rows over all NT threads (whole wavefronts).
 // grid = (ceil(nrows / RB), members), block = NT, dynamic LDS = lds_bytes.
@@ -44,13 +45,14 @@ template <typename T, int MODEL, int M, int NA, bool MF>
 __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3(
     cplx<T>* __restrict__ B, long arr_stride, double* __restrict__ vmf, long nrows, int Zp,
     const cplx<T>* __restrict__ twM_g, const cplx<T>* __restrict__ twN_g, PhiParams pp,
-    double2* __restrict__ partials, int* __restrict__ flag, Batch bt) {
+    double2* __restrict__ partials, int* __restrict__ flag, Batch bt, const double* __restrict__ ms) {
   constexpr int NS = NA + (MF ? 1 : 0);
   using G = Pw3Geo<T, M, NS>;
   using RF = RowF<T, M>;
   constexpr int RB = G::RB, NT = G::NT, SLOT = RF::SLOT;
   constexpr bool TRACE = MODEL >= M_ESFMT && NA == 10;
   B += blockIdx.y * bt.sb;
+  if (ms) ms += blockIdx.y * kMS;
   if (MF) vmf += blockIdx.y * bt.sr;
   partials += blockIdx.y * bt.sp;
   flag += blockIdx.y;
@@ -91,6 +93,12 @@ __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3
   int bad = 0;
   constexpr int NSITE = RB * 2 * M, NIT = (NSITE + NT - 1) / NT;
   constexpr int AST = RB * SLOT * 2;  // T elements between the slots of consecutive arrays of one row
+  // the mean split's constants of this member, read once (uniform across the block)
+  double nbar[NS], cbar[NA];
+#pragma unroll
+  for (int b = 0; b < NS; ++b) nbar[b] = ms ? ms[kMSn + b] : 0.0;
+#pragma unroll
+  for (int b = 0; b < NA; ++b) cbar[b] = ms ? ms[kMSc + b] : 0.0;
 #pragma unroll 1
   for (int it = 0; it < NIT; ++it) {
     const int s = threadIdx.x + it * NT;
@@ -101,7 +109,7 @@ __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3
     T* base = reinterpret_cast<T*>(lds + rr * SLOT) + z;
     double n[11];
 #pragma unroll
-    for (int b = 0; b < 11; ++b) n[b] = b < NA ? double(base[b * AST]) : 0.0;
+    for (int b = 0; b < 11; ++b) n[b] = b < NA ? double(base[b * AST]) + nbar[b < NA ? b : 0] : 0.0;
     if (TRACE) n[10] = n[1] - n[5] - n[8];
     double out[11];
     const double phi = phi_site<MODEL>(n, pp, out);
@@ -113,8 +121,8 @@ __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3
     if (n[0] > 0.5 && 1 - n[0] < 0.0) bad = 1;
     F.add(phi);
 #pragma unroll
-    for (int b = 0; b < NA; ++b) base[b * AST] = T(out[b]);
-    if (MF) vmf[row * 2 * M + z] = double(base[NA * AST]);
+    for (int b = 0; b < NA; ++b) base[b * AST] = T(out[b] - cbar[b]);
+    if (MF) vmf[row * 2 * M + z] = double(base[NA * AST]) + nbar[NS - 1];
   }
   if (bad) atomicOr(flag, 1);
   __syncthreads();
@@ -158,6 +166,7 @@ struct PwArgs {
   int model;
   int members;
   hipStream_t stream;
+  const double* ms;
 };
 
 }  // namespace cdft

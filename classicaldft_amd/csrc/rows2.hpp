@@ -204,11 +204,13 @@ __device__ __forceinline__ void r2c_from_regs(cplx<T>* slot, int t, const cplx<T
 }
 
 // Real rows -> half spectra.  grid = (ceil(nrows/RB), narr), block = RB * P,
-// dynamic LDS = RB * SLOT elements.
+// dynamic LDS = RB * SLOT elements.  psum (optional): per-block compensated FP64 sums of the input rows of array
+// (member) blockIdx.y at psum[blockIdx.y * psp + block].
 template <typename T, typename TI, int M>
 __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, long in_arr_stride,
                                                    cplx<T>* __restrict__ out, long out_arr_stride, long nrows, int Zp, const cplx<T>* __restrict__ twM,
-                                                   const cplx<T>* __restrict__ twN, int RB) {
+                                                   const cplx<T>* __restrict__ twN, int RB,
+                                                   double2* __restrict__ psum = nullptr, long psp = 0) {
   using RF = RowF<T, M>;
   extern __shared__ unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
@@ -218,16 +220,22 @@ __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, lo
   const long row = ok ? row0 : nrows - 1;
   const cplx<TI>* src = reinterpret_cast<const cplx<TI>*>(in + blockIdx.y * in_arr_stride + row * 2 * M) + t;
   cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
+  CSum acc[1] = {{0.0, 0.0}};
 #pragma unroll
   for (int q = 0; q < RF::Q1; ++q)
 #pragma unroll
     for (int n1 = 0; n1 < RF::R1; ++n1) {
       const cplx<TI> x = src[RF::R2 * n1 + RF::P * q];
       v1[q][n1] = {T(x.x), T(x.y)};
+      if (psum && ok) {
+        acc[0].add(double(x.x));
+        acc[0].add(double(x.y));
+      }
     }
   reg_fft_seq<T, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
   __syncthreads();
   r2c_from_regs<T, M>(slot, t, twN, out + blockIdx.y * out_arr_stride + row * Zp, ok, v2);
+  if (psum) block_reduce_store_n<1>(acc, psum + blockIdx.y * psp, row_tile<T>());
 }
 
 // Half spectra -> real rows (unnormalised, times `scale`).
@@ -262,14 +270,17 @@ __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ i
 // trace != 0 (tensor models whose weights satisfy w_Txx + w_Tyy + w_Tzz = w_s): only m = 10 arrays are stored,
 // n_Tzz = n_s - n_Txx - n_Tyy per site, and the T_zz derivative is folded back onto the arrays it was built
 // from (w_Tzz * dPhi_Tzz = w_s * dPhi_Tzz - w_Txx * dPhi_Tzz - w_Tyy * dPhi_Tzz).
+// ms (mixed path, null otherwise): the FP32 arrays hold the weighted densities without their means, which are
+// added back per site in FP64 (ms[kMSn + a]); dPhi_a(nbar) (ms[kMSc + a]) is subtracted before the FP32 store.
 template <typename T, int MODEL, int M>
 __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, int m, int nconv,
                                                       double* __restrict__ vmf, long nrows, int Zp,
                                                       const cplx<T>* __restrict__ twM,
                                                       const cplx<T>* __restrict__ twN, int RB, PhiParams pp,
                                                       double2* __restrict__ partials, int* __restrict__ flag,
-                                                      Batch bt, int trace) {
+                                                      Batch bt, int trace, const double* __restrict__ ms) {
   B += blockIdx.y * bt.sb;
+  if (ms) ms += blockIdx.y * kMS;
   if (vmf) vmf += blockIdx.y * bt.sr;
   partials += blockIdx.y * bt.sp;
   flag += blockIdx.y;
@@ -306,7 +317,8 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
     double n[11];
 #pragma unroll
     for (int b = 0; b < 11; ++b)
-      n[b] = b < m ? double(reinterpret_cast<const T*>(lds + (b * RB + rr) * RF::SLOT)[z]) : 0.0;
+      n[b] = b < m ? double(reinterpret_cast<const T*>(lds + (b * RB + rr) * RF::SLOT)[z]) + (ms ? ms[kMSn + b] : 0.0)
+                   : 0.0;
     if (trace) n[10] = n[1] - n[5] - n[8];
     double out[11];
     const double phi = phi_site<MODEL>(n, pp, out);
@@ -319,8 +331,10 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
     F.add(phi);
 #pragma unroll
     for (int b = 0; b < 11; ++b)
-      if (b < m) reinterpret_cast<T*>(lds + (b * RB + rr) * RF::SLOT)[z] = T(out[b]);
-    if (has_mf) vmf[row * 2 * M + z] = double(reinterpret_cast<const T*>(lds + (m * RB + rr) * RF::SLOT)[z]);
+      if (b < m) reinterpret_cast<T*>(lds + (b * RB + rr) * RF::SLOT)[z] = T(out[b] - (ms ? ms[kMSc + b] : 0.0));
+    if (has_mf)
+      vmf[row * 2 * M + z] = double(reinterpret_cast<const T*>(lds + (m * RB + rr) * RF::SLOT)[z]) +
+                             (ms ? ms[kMSn + m] : 0.0);
   }
   if (bad) atomicOr(flag, 1);
   __syncthreads();
@@ -585,6 +599,7 @@ __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Ac
       for (int k2 = 0; k2 < RF::R2; ++k2) y[q][k2] = {T(0), T(0)};
   }
   CSum qs[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  const double fmt_mean = fp.ms ? fp.ms[mb * kMS + kMSk] : 0.0;
   if (ok) {
     const auto rr = buf_rsrc(rho, real_bytes);
     const auto rf = buf_rsrc(dF, real_bytes);
@@ -613,7 +628,7 @@ __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Ac
             qs[0].add((d0 * l - d0) * fp.dV);
             d += l * fp.dV;
           }
-          if (fp.fmt) d += double(h ? y[q][k2].y : y[q][k2].x) * fp.dV;
+          if (fp.fmt) d += double(h ? y[q][k2].y : y[q][k2].x) * fp.dV + fmt_mean;
           if (fp.mf) {
             d += vm;
             qs[1].add(d0 * vm);

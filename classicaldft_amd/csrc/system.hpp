@@ -184,6 +184,10 @@ struct cdft_system_s {
   bool nb_half_cols = false, nb_half_fwd = false, nb_half_bwd = false;
   int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
   bool pw3 = true;       // specialised pointwise kernel (pw3.hpp) where an instance covers the system
+  // mixed path: the means of the weighted densities and of dPhi are carried in FP64 and the FP32 arrays hold the
+  // deviations from them (msplit: kMS doubles per member; w0: the FP64 weight spectra at k = 0, nconv values)
+  bool mean_split = false;
+  DevBuf msplit, w0;
   long slab_bytes = 0;   // cache budget of one x-slab of the y / z / y section (0: no slabs)
   bool has_vext = false;
   bool last_with_dft = true;
@@ -258,20 +262,25 @@ inline bool rows_reg_ok(const cdft_system_s& s) {
 }
 
 // ------------------------------- transforms --------------------------------
+// psum (register kernels only): per-block FP64 sums of each input array at psum + array * psp (returns the block
+// count through nblk_out).
 template <typename T, typename TI>
-inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride, cplx<T>* out, long out_stride, int narr) {
+inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride, cplx<T>* out, long out_stride, int narr,
+                            double2* psum = nullptr, long psp = 0, long* nblk_out = nullptr) {
   const Tables<T>& tw = tables<T>(s);
   if (rows_reg_ok<T>(s)) {
     with_reg_len(s.g.Nz / 2, [&](auto mc) {
       constexpr int Mc = decltype(mc)::value;
       constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
+      if (nblk_out) *nblk_out = div_up(s.nrows, RB);
       hipLaunchKernelGGL((k_rows_r2c2<T, TI, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
                          RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride, out, out_stride, s.nrows, s.g.Zp,
-                         tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB);
+                         tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, psum, psp);
     });
     HIPCHK(hipGetLastError());
     return;
   }
+  REQUIRE(!psum, "row sums need the register z kernels");
   const int M = s.g.Nz / 2;
   const int RB = s.RBrow;
   dim3 grid(div_up(s.nrows, RB), narr);

@@ -231,15 +231,47 @@ def test_accumulate_flag(cdft):
     assert np.abs(d2 - 3.0 - d1).max() < 1e-14 * np.abs(d1).max() + 1e-15
 
 
-def test_mixed_precision_error(cdft):
-    shape, dx, hsd = (32, 32, 32), 0.1, 1.0
-    rho = seeded_density(shape, 0.6, 0x5EED0001)
-    w = oracle.fmt_weights(shape, dx, hsd)
-    F0, dF0 = cdft.System(shape, dx, hsd, weights=w).fmt_eval(rho)
-    F1, dF1 = cdft.System(shape, dx, hsd, weights=w, precision=cdft.MIXED).fmt_eval(rho)
-    print("mixed F rel err", rel(F1, F0), "dF", maxrel(dF1, dF0))
-    assert rel(F1, F0) < 1e-6
-    assert maxrel(dF1, dF0) < 1e-5
+# Mixed precision against the FP64 oracle: the north-star bar |F error| <= 1e-7 relative; max-norm dF within 1e-6.
+MIXED_F_TOL, MIXED_DF_TOL = 1e-7, 1e-6
+
+
+@pytest.mark.parametrize("n,dx,seed", [(32, 0.1, 0x5EED0001), (256, 0.04, 0x5EED0256)])
+def test_mixed_precision_error_vs_oracle(cdft, n, dx, seed):
+    """Config 1 (32^3) and the metric workload (256^3) in the mixed path (FP32 transforms and convolutions, FP64
+    Phi, the weighted densities' and dPhi's means in FP64) against the FP64 oracle on accurate weights."""
+    shape, hsd = (n, n, n), 1.0
+    rho = seeded_density(shape, 0.6, seed)
+    oracle.set_threads(16, 16)
+    try:
+        wa = oracle.fmt_weights(shape, dx, hsd, mode=oracle.WEIGHTS_ACCURATE)
+        F0, dF0 = oracle.fmt_eval(oracle.WBI, rho, dx, hsd, weights=wa)
+    finally:
+        oracle.set_threads(1, 1)
+    del wa
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, precision=cdft.MIXED)
+    F1, dF1 = s.fmt_eval(rho)
+    s.close()
+    print(f"{n}^3 mixed vs oracle: F rel {rel(F1, F0):.3e}, dF maxrel {maxrel(dF1, dF0):.3e}")
+    assert rel(F1, F0) <= MIXED_F_TOL
+    assert maxrel(dF1, dF0) <= MIXED_DF_TOL
+
+
+def test_mixed_full_dft_vs_oracle(cdft):
+    """The whole DFT (ideal gas, White Bear I, LJ mean field, V_ext, mu) in the mixed path against the oracle: the
+    mean-field convolution's mean and the FMT force's mean travel in FP64 too."""
+    shape, dx, hsd, kT = (32, 32, 32), 0.1, cdft.lj_hsd(0.8), 0.8
+    rho = seeded_density(shape, 0.6, 0x5EED0007, amp=0.2)
+    rng = np.random.default_rng(7)
+    vext = rng.uniform(0.0, 1.0, shape)
+    o = oracle.DFT(shape, dx, hsd, mf=True, kT=kT, mu=-1.5, vext=vext, accurate_weights=True)
+    p0, dF0 = o.eval(rho)
+    s = cdft.System(shape, dx, hsd, model=cdft.WHITEBEAR_I, ideal=True, mean_field=True, kT=kT, mu=-1.5, vext=vext,
+                    precision=cdft.MIXED)
+    p1, dF1 = s.eval(rho)
+    s.close()
+    print("full DFT mixed vs oracle: F rel", rel(p1[0], p0[0]), "dF", maxrel(dF1, dF0))
+    assert rel(p1[0], p0[0]) <= MIXED_F_TOL
+    assert maxrel(dF1, dF0) <= MIXED_DF_TOL
 
 
 @pytest.mark.parametrize("precision", [0, 1])
