@@ -12,7 +12,8 @@ Phi and reductions) is reported beside it with its error against FP64.
 The roofline uses the algorithmic byte model of one evaluation,
 B(e, m) = e N (4m + 2) + e Nk (2m + 8) (each 3D FFT one read + one write,
 every stage array written and read once), N = 256^3, Nk = 256*256*129,
-m = 11; the dominant kernel's own compulsory bytes are reported beside it.
+m = 11; the dominant kernel's own compulsory bytes (one pass over the arrays it
+touches) and its rate are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -50,21 +51,24 @@ def model_bytes(e, m, shape=SHAPE, fp64_io=False):
     return e * N * (4 * m + 2) + e * Nk * (2 * m + 8)
 
 
-def stage_bytes(shape, e, m):
-    """Compulsory HBM bytes of each kernel of this implementation (one pass each)."""
+def stage_bytes(shape, e, m, nconv=None, folded=True):
+    """Compulsory HBM bytes of each kernel of this implementation (one pass each over the arrays it touches):
+    m stored weighted-density arrays, nconv convolved (m plus a mean-field kernel); weight spectra real-valued,
+    e bytes per k-point, rows kx <= Nx/2 only when folded."""
     Nx, Ny, Nz = shape
+    nconv = m if nconv is None else nconv
     N = Nx * Ny * Nz
     Nk = Nx * Ny * (Nz // 2 + 1)
     c = 2 * e
+    wk = e * Nk * ((Nx // 2 + 1) / Nx if folded else 1.0)  # one weight spectrum
     return {
         "z_r2c": 8 * N + c * Nk,
         "y_fwd": 2 * c * Nk,
-        "x_conv_fwd": c * Nk + m * e * Nk + m * c * Nk,
-        "y_inv": 2 * m * c * Nk,
-        "z_pointwise": 2 * m * c * Nk,
+        "x_conv_fwd": c * Nk + nconv * wk + nconv * c * Nk,
+        "y_inv": 2 * nconv * c * Nk,
+        "z_pointwise": nconv * c * Nk + m * c * Nk,
         "y_fwd2": 2 * m * c * Nk,
-        "y_z_y_slabs": 6 * m * c * Nk,  # the three passes above, fused by x-slab (counted without cache reuse)
-        "x_conv_bwd": m * c * Nk + m * e * Nk + c * Nk,
+        "x_conv_bwd": m * c * Nk + m * wk + c * Nk,
         "y_inv2": 2 * c * Nk,
         "z_final": c * Nk + 8 * N + 8 * N,  # acc in, rho in, dF out
     }
@@ -72,6 +76,7 @@ def stage_bytes(shape, e, m):
 
 def time_system(cdft, precision, rho_d, dF_d, steps, warmup):
     s = cdft.System(SHAPE, DX, HSD, model=cdft.WHITEBEAR_I, precision=precision)
+    layout = s.layout
     for _ in range(warmup):
         s.eval_async(rho_d.ptr, dF_d.ptr, fmt_only=True)
     parts = s.fetch()
@@ -89,7 +94,7 @@ def time_system(cdft, precision, rho_d, dF_d, steps, warmup):
     s.set_timing(False)
     dF = dF_d.download()
     s.close()
-    return ms / steps, parts[2], dF, acc
+    return ms / steps, parts[2], dF, acc, layout
 
 
 def time_ao(cdft, rho_d, dF_d, steps, Rp=0.25, rho_p=0.5):
@@ -467,9 +472,9 @@ def main():
     rho_d = cdft.DeviceArray.from_host(rho)
     dF_d = cdft.DeviceArray(SHAPE)
     print("# fp64 timing", flush=True)
-    t64, F64, dF64, st64 = time_system(cdft, cdft.FP64, rho_d, dF_d, args.steps, args.warmup)
+    t64, F64, dF64, st64, lay64 = time_system(cdft, cdft.FP64, rho_d, dF_d, args.steps, args.warmup)
     print("# mixed timing", flush=True)
-    tmx, Fmx, dFmx, stmx = time_system(cdft, cdft.MIXED, rho_d, dF_d, args.steps, args.warmup)
+    tmx, Fmx, dFmx, stmx, laymx = time_system(cdft, cdft.MIXED, rho_d, dF_d, args.steps, args.warmup)
 
     ao = None
     if not args.no_ao:
@@ -481,12 +486,13 @@ def main():
     B64 = model_bytes(8, m)
     Bmx = model_bytes(4, m, fp64_io=True)
     ach = B64 / (t64 * 1e-3) / 1e9
-    sb = stage_bytes(SHAPE, 8, m)
+    sb = stage_bytes(SHAPE, 8, lay64[0], lay64[1])
     dom = max(st64, key=st64.get) if st64 else None
     dominant = None
     if dom:
-        dominant = {"kernel": dom, "ms": st64[dom], "bytes": sb.get(dom),
-                    "achieved_GBs": (sb[dom] / (st64[dom] * 1e-3) / 1e9) if dom in sb else None}
+        dominant = {"kernel": dom, "ms": st64[dom], "bytes": sb.get(dom), "stored_arrays": lay64[0],
+                    "achieved_GBs": (sb[dom] / (st64[dom] * 1e-3) / 1e9) if dom in sb else None,
+                    "bytes_note": "compulsory bytes of this kernel: one pass over the arrays it reads and writes"}
     # measured HBM bytes per evaluation (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated): used only when the
     # profile was measured on the kernels built from the current sources
     traffic, traffic_note = None, "no traffic profile"

@@ -106,6 +106,7 @@ def lib():
         L.cdft_fmt_eval.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_system_stream.argtypes = [_p, ctypes.POINTER(_p)]
         L.cdft_system_a_vdw.argtypes = [_p, ctypes.POINTER(_d)]
+        L.cdft_system_layout.argtypes = [_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]
         L.cdft_system_get_weights.argtypes = [_p, _p]
         L.cdft_system_get_mf_kernel.argtypes = [_p, _p]
         L.cdft_lj_hsd.argtypes = [_d, _d, _d, _d, ctypes.POINTER(_d)]
@@ -438,6 +439,13 @@ class System:
         a = _d()
         _check(lib().cdft_system_a_vdw(self.h, ctypes.byref(a)))
         return a.value
+
+    @property
+    def layout(self):
+        """(stored weighted-density arrays, convolved arrays) of the pipeline (cdft_system_layout)."""
+        a, b = _i(), _i()
+        _check(lib().cdft_system_layout(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def device_weights(self):
         w = np.zeros((11,) + self.shape)

@@ -4,6 +4,7 @@
 // each of the m FMT weights and the mean-field kernel), the spectral work
 // arrays, the twiddle tables and a HIP stream.  cdft_system_eval runs the
 // fused pipeline documented in kernels.hpp; no C++ exception crosses the ABI.
+#include <cstring>
 #include <mutex>
 #include <random>
 
@@ -458,8 +459,8 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
   s->d.vext = nullptr;
   s->d.mf_kernel = nullptr;
   s->d.ao_weights = nullptr;
-  s->ev.resize(16);
-  s->ev_names.resize(16);
+  s->ev.resize(64);  // per-slab stage marks; cdft_system_get_timing sums them per stage name
+  s->ev_names.resize(64);
   for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
   *out = s.release();
 }
@@ -648,12 +649,15 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
       for (int x0 = 0; x0 < s.g.Nx; x0 += planes) {
         const int x1 = std::min(s.g.Nx, x0 + planes);
         launch_cols_slab<T>(s, B, Nkp, nconv, false, nb, sB, x0, x1);
+        s.mark("y_inv");
         nblk += pointwise(x0, x1, nblk);
+        s.mark("z_pointwise");
         launch_cols_slab<T>(s, B, Nkp, s.m, true, nb, sB, x0, x1);
+        s.mark("y_fwd2");
       }
       hipLaunchKernelGGL(k_reduce_partials_wide, dim3(nb), dim3(1024), 0, s.stream, s.partials.as<double2>(), nblk, 1,
                          s.results.as<double>(), s.pslots, 8L);
-      s.mark("y_z_y_slabs");
+      s.mark("z_pointwise");
     } else {
       launch_cols<T>(s, B, Nkp, nconv, 1, false, nb, sB);
       s.mark("y_inv");
@@ -1887,6 +1891,15 @@ int cdft_system_stream(cdft_system sys, void** hip_stream) {
   });
 }
 
+int cdft_system_layout(cdft_system sys, int* stored, int* convolved) {
+  return guarded([&] {
+    REQUIRE(sys && stored && convolved, "null argument");
+    *stored = sys->m;
+    *convolved = sys->nconv;
+    return (int)CDFT_OK;
+  });
+}
+
 int cdft_system_a_vdw(cdft_system sys, double* a) {
   return guarded([&] {
     REQUIRE(sys && a, "null argument");
@@ -1951,12 +1964,25 @@ int cdft_system_get_timing(cdft_system sys, int* n, double* ms, const char** nam
   return guarded([&] {
     REQUIRE(sys && n, "null argument");
     HIPCHK(hipStreamSynchronize(sys->stream));
-    const int k = std::max(0, sys->n_marks - 1);
-    for (int i = 0; i < k; ++i) {
+    // segments between consecutive marks, summed per stage name in the order the names first appear
+    std::vector<const char*> nm;
+    std::vector<double> tot;
+    for (int i = 0; i + 1 < sys->n_marks; ++i) {
       float t = 0.f;
       HIPCHK(hipEventElapsedTime(&t, sys->ev[i], sys->ev[i + 1]));
-      if (ms) ms[i] = t;
-      if (names) names[i] = sys->ev_names[i + 1];
+      const char* name = sys->ev_names[i + 1];
+      size_t j = 0;
+      while (j < nm.size() && std::strcmp(nm[j], name) != 0) ++j;
+      if (j == nm.size()) {
+        nm.push_back(name);
+        tot.push_back(0.0);
+      }
+      tot[j] += t;
+    }
+    const int k = (int)std::min<size_t>(nm.size(), 16);
+    for (int i = 0; i < k; ++i) {
+      if (ms) ms[i] = tot[i];
+      if (names) names[i] = nm[i];
     }
     *n = k;
     return CDFT_OK;

@@ -359,6 +359,11 @@ int cdft_memcpy_d2h(void* dst, const void* src, unsigned long long bytes);
 /* Introspection / plumbing. */
 int cdft_system_stream(cdft_system sys, void** hip_stream);
 int cdft_system_a_vdw(cdft_system sys, double* a_vdw); /* Interaction_Base::getVDWParameter */
+/* The weighted-density arrays the pipeline stores (11 for tensor models, 10 when the weights satisfy the trace
+   identity and T_zz is rebuilt per site, 5 without tensor) and those it convolves (plus the mean-field kernel).
+   No counterpart in the reference (its FMT_Species always stores 11, include/Species.h:351-360); for callers that
+   size buffers or count bytes. */
+int cdft_system_layout(cdft_system sys, int* stored_arrays, int* convolved_arrays);
 /* Copies the device-generated real-space FMT weights (11*N) or MF kernel (N). */
 int cdft_system_get_weights(cdft_system sys, double* w11);
 int cdft_system_get_mf_kernel(cdft_system sys, double* w);
@@ -370,8 +375,9 @@ int cdft_fft_r2c(int Nx, int Ny, int Nz, int precision, const double* in, double
 int cdft_fft_c2r(int Nx, int Ny, int Nz, int precision, const double* in, double* out);
 
 /* Per-stage timing of the last cdft_system_eval with timing enabled: up to
-   16 entries of {kernel id, milliseconds}.  enable = 1 records hipEvents
-   between stages (adds no synchronisation inside the pipeline). */
+   16 entries of {kernel id, milliseconds}, one per stage name (a stage that
+   runs once per x-slab is summed over the slabs).  enable = 1 records
+   hipEvents between stages (adds no synchronisation inside the pipeline). */
 int cdft_system_set_timing(cdft_system sys, int enable);
 int cdft_system_get_timing(cdft_system sys, int* n, double* ms, const char** names);
 
