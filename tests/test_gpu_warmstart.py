@@ -38,6 +38,21 @@ def test_judge_falls_back_on_a_bad_network(ws_mod):
     p[-1] = 5.0  # u = 5: rho0 = e^2 everywhere, far beyond close packing
     ws.net.set_params(p)
     mus, rho_b, V = wl.systems(HELD_OUT, 4)
-    rho0, used, _ = ws_mod.judged_start(wl, ws, mus, rho_b, V)
+    fb, used, _ = ws_mod.judged_batch(wl, ws, mus, rho_b, V)
     assert not used.any()
-    assert np.array_equal(rho0, np.broadcast_to(rho_b[:, None, None, None], V.shape))
+    for i in range(4):
+        assert np.max(np.abs(fb.density(i) / rho_b[i] - 1)) <= 1e-14
+    fb.close()
+
+
+def test_uniform_grand_potential_closed_form(ws_mod):
+    """The judge's closed-form grand potential of the uniform bulk density equals FIRE's own first evaluation of
+    that density (the lattice functional at uniform density is the bulk one)."""
+    wl = ws_mod.Workload(64)
+    mus, rho_b, V = wl.systems(HELD_OUT, 4)
+    fb = wl.fire(mus, np.broadcast_to(rho_b[:, None, None, None], V.shape), V)
+    reps, st, _ = fb.status()
+    fb.close()
+    om = ws_mod.uniform_grand_potentials(wl, mus, rho_b, V)
+    for r, o in zip(reps, om):
+        assert abs(r["F"] - o) <= 1e-11 * abs(o), (r["F"], o)
