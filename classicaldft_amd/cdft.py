@@ -118,6 +118,7 @@ def lib():
         L.cdft_fire_batch_set_precision.argtypes = [_p, _i]
         L.cdft_fire_batch_status.argtypes = [_p, _p, _p, ctypes.POINTER(_i)]
         L.cdft_fire_batch_get.argtypes = [_p, _i, _p]
+        L.cdft_fire_batch_state.argtypes = [_p, _i, _p, _p, _p, _p]
         L.cdft_fire_batch_system.argtypes = [_p, ctypes.POINTER(_p)]
         L.cdft_fire_batch_destroy.argtypes = [_p]
         L.cdft_fire_batch_destroy.restype = None
@@ -628,6 +629,12 @@ class FireBatch:
         out = np.zeros(self.shape)
         _check(lib().cdft_fire_batch_get(self.h, int(member), _ptr(out)))
         return out
+
+    def state(self, member):
+        """(rho, x, v, dF_raw) of one member, as Fire.state (cdft_fire_batch_state)."""
+        rho, df, x, v = (np.zeros(self.shape) for _ in range(4))
+        _check(lib().cdft_fire_batch_state(self.h, int(member), _ptr(rho), _ptr(df), _ptr(x), _ptr(v)))
+        return rho, x, v, df
 
     def close(self):
         if getattr(self, "h", None):

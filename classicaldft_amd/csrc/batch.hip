@@ -589,6 +589,20 @@ int cdft_fire_batch_get(cdft_fire_batch b, int member, double* rho) {
   });
 }
 
+int cdft_fire_batch_state(cdft_fire_batch b, int member, double* rho, double* dF_raw, double* x, double* v) {
+  return guarded([&] {
+    REQUIRE(b && member >= 0 && member < b->nb, "bad argument");
+    const long o = member * b->N;
+    const size_t bytes = sizeof(double) * b->N;
+    if (rho) HIPCHK(hipMemcpyAsync(rho, b->rho.as<double>() + o, bytes, hipMemcpyDeviceToHost, b->sys->stream));
+    if (dF_raw) HIPCHK(hipMemcpyAsync(dF_raw, b->raw.as<double>() + o, bytes, hipMemcpyDeviceToHost, b->sys->stream));
+    if (x) HIPCHK(hipMemcpyAsync(x, b->x.as<double>() + o, bytes, hipMemcpyDeviceToHost, b->sys->stream));
+    if (v) HIPCHK(hipMemcpyAsync(v, b->v.as<double>() + o, bytes, hipMemcpyDeviceToHost, b->sys->stream));
+    HIPCHK(hipStreamSynchronize(b->sys->stream));
+    return (int)CDFT_OK;
+  });
+}
+
 int cdft_fire_batch_system(cdft_fire_batch b, cdft_system* sys) {
   return guarded([&] {
     REQUIRE(b && sys, "null argument");

@@ -426,6 +426,9 @@ int cdft_fire_batch_run(cdft_fire_batch b, int steps, double tol_raw, int use_gr
 int cdft_fire_batch_set_precision(cdft_fire_batch b, int precision);
 int cdft_fire_batch_status(cdft_fire_batch b, cdft_fire_report* reports, int* status, int* running);
 int cdft_fire_batch_get(cdft_fire_batch b, int member, double* rho);
+/* The FIRE2 state of one member, as cdft_fire_get (any pointer may be null): density, raw gradient dF/drho of the
+   last evaluation, alias coordinates and velocities (fireMinimizer2's x_ and v_, include/Minimizer.h:105-160). */
+int cdft_fire_batch_state(cdft_fire_batch b, int member, double* rho, double* dF_raw, double* x, double* v);
 int cdft_fire_batch_system(cdft_fire_batch b, cdft_system* sys);
 void cdft_fire_batch_destroy(cdft_fire_batch b);
 /* No reference counterpart (the reference runs one fireMinimizer2 per system, src/Minimizer.cpp:160-271): several

@@ -80,6 +80,12 @@ def test_fire_batch_matches_host_fire(cdft, graph):
         assert abs(r["fmax_raw"] - info["fmax_raw"]) <= 1e-8 * info["fmax_raw"]
         assert abs(r["dt"] - info["dt"]) <= 1e-12 * info["dt"]
         assert np.max(np.abs(fb.density(b) - rho_h)) <= 1e-10 * np.max(rho_h)
+        # the whole FIRE state of the member (cdft_fire_batch_state) against the host-controlled run's
+        rho_s, x_s, v_s, df_s = fb.state(b)
+        rho_h, x_h, v_h, df_h = f.state()
+        assert np.array_equal(rho_s, fb.density(b))
+        for got, want in ((df_s, df_h), (x_s, x_h), (v_s, v_h)):
+            assert np.max(np.abs(got - want)) <= 1e-9 * np.max(np.abs(want)) + 1e-300
 
 
 def test_fire_batch_backtracks_like_host_fire(cdft):
