@@ -166,17 +166,25 @@ __global__ void k_spec0(const cplx<double>* __restrict__ S, double* __restrict__
 // nbar_a = sum rho w_a(0) (w0 carries the spectra's 1/N; the mean-field convolution last), dPhi_a(nbar), and the
 // FMT force of the means dV N sum_a dPhi_a(nbar) w_a(0).  A uniform density is exactly its mean, so the FP32 arrays
 // carry only the deviations from it.
+// trace: the weight set without T_zz (n_Tzz = n_s - n_Txx - n_Tyy, its dPhi folded into s, Txx and Tyy as in the
+// pointwise kernels; sum_a dPhi_a w_a(0) is unchanged because w_Tzz(0) = w_s(0) - w_Txx(0) - w_Tyy(0)).
 template <int MODEL>
 __global__ void k_msplit(double* __restrict__ ms, const double* __restrict__ w0, int m, int nconv, PhiParams pp,
-                         double dVN, int nb) {
+                         double dVN, int nb, int trace) {
   const int mb = blockIdx.x * blockDim.x + threadIdx.x;
   if (mb >= nb) return;
   double* q = ms + (long)mb * kMS;
   const double S = q[0];
   double n[11], out[11];
   for (int a = 0; a < 11; ++a) n[a] = a < m ? S * w0[a] : 0.0;
+  if (trace) n[10] = n[1] - n[5] - n[8];
   for (int a = 0; a < nconv; ++a) q[kMSn + a] = S * w0[a];
   phi_site<MODEL>(n, pp, out);
+  if (trace) {
+    out[1] += out[10];
+    out[5] -= out[10];
+    out[8] -= out[10];
+  }
   CSum K{0.0, 0.0};
   for (int a = 0; a < m; ++a) {
     q[kMSc + a] = out[a];
@@ -304,9 +312,7 @@ void build_weight_spectra(cdft_system_s& s) {
   S.alloc(sizeof(cplx<double>) * Nkp);
   const int nb = div_up(Nkp, kThreads);
   const char* nt = std::getenv("CDFT_NO_TRACE");
-  // FP64 only: in the mixed path n_Tzz = n_s - n_Txx - n_Tyy of FP32 spectra carries several times the rounding
-  // of a directly transformed n_Tzz, which raises the gradient noise floor of the mixed FIRE phase
-  const bool allow_trace = !s.mixed && !s.has_ao && !(nt && nt[0] == '1') && rows_reg_ok<T>(s) && reg_fft_ok<T>(s, s.g.Nx) &&
+  const bool allow_trace = !s.has_ao && !(nt && nt[0] == '1') && rows_reg_ok<T>(s) && reg_fft_ok<T>(s, s.g.Nx) &&
                            reg_fft_ok<T>(s, s.g.Ny) && rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) &&
                            reg_fft_ok<double>(s, s.g.Ny);
   if (s.m > 0)
@@ -597,11 +603,11 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
     double* msp = s.msplit.as<double>();
     const double* w0 = s.w0.as<double>();
     switch (s.d.model) {
-      case CDFT_ROSENFELD: hipLaunchKernelGGL(k_msplit<M_ROSENFELD>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
-      case CDFT_RSLT: hipLaunchKernelGGL(k_msplit<M_RSLT>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
-      case CDFT_ESFMT: hipLaunchKernelGGL(k_msplit<M_ESFMT>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
-      case CDFT_WHITEBEAR_I: hipLaunchKernelGGL(k_msplit<M_WBI>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
-      default: hipLaunchKernelGGL(k_msplit<M_WBII>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb); break;
+      case CDFT_ROSENFELD: hipLaunchKernelGGL(k_msplit<M_ROSENFELD>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb, s.trace); break;
+      case CDFT_RSLT: hipLaunchKernelGGL(k_msplit<M_RSLT>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb, s.trace); break;
+      case CDFT_ESFMT: hipLaunchKernelGGL(k_msplit<M_ESFMT>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb, s.trace); break;
+      case CDFT_WHITEBEAR_I: hipLaunchKernelGGL(k_msplit<M_WBI>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb, s.trace); break;
+      default: hipLaunchKernelGGL(k_msplit<M_WBII>, g1, b1, 0, s.stream, msp, w0, s.m, nconv, pp, dVN, nb, s.trace); break;
     }
     HIPCHK(hipGetLastError());
     s.mark("mean_split");

@@ -29,8 +29,7 @@ long by_arrays(const PwArgs& a) {
   const bool mf = a.nconv > a.m;
   if constexpr (MODEL >= M_ESFMT) {
     if (a.m == 11 && !a.trace) return mf ? go<T, MODEL, M, 11, true>(a) : go<T, MODEL, M, 11, false>(a);
-    if constexpr (std::is_same<T, double>::value)
-      if (a.m == 10 && a.trace) return mf ? go<T, MODEL, M, 10, true>(a) : go<T, MODEL, M, 10, false>(a);
+    if (a.m == 10 && a.trace) return mf ? go<T, MODEL, M, 10, true>(a) : go<T, MODEL, M, 10, false>(a);
   } else {
     if (a.m == 5) return mf ? go<T, MODEL, M, 5, true>(a) : go<T, MODEL, M, 5, false>(a);
   }

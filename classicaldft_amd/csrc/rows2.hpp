@@ -154,24 +154,35 @@ __device__ __forceinline__ void r2c_regs_to_slot(cplx<T>* slot, int t,
 }
 
 // Second half of r2c_from_regs (after the block has synchronised): the half spectrum from the slot to dst.
+// Each thread takes the pairs (k, M - k), k <= M/2: with s = Z_k + Z*_{M-k}, d = Z_k - Z*_{M-k} and W^{M-k} =
+// -conj(W^k), X_{M-k} = (s.x - (W^k d).y)/2 - i (s.y + (W^k d).x)/2 reuses the loads and the product of X_k.
 template <typename T, int M>
 __device__ __forceinline__ void r2c_slot_to_row(const cplx<T>* slot, int t, const cplx<T>* __restrict__ twN,
                                                 cplx<T>* __restrict__ dst) {
   using RF = RowF<T, M>;
-  cplx<T>* d = dst + t;
-  const cplx<T>* tw = twN + t;
+  constexpr int NI = (M / 2 + RF::P) / RF::P;
+  // element M - k = M - t - P i sits at P (NI - 1 - i) past these bases (M - t - P (NI - 1) > 0 for every length),
+  // so the mirrored loads and stores take constant offsets
+  const cplx<T>* rs = slot + (M - t - RF::P * (NI - 1));
+  cplx<T>* rd = dst + (M - t - RF::P * (NI - 1));
 #pragma unroll
-  for (int i = 0; i < (M + RF::P) / RF::P; ++i) {
+  for (int i = 0; i < NI; ++i) {
     const int k = t + RF::P * i;
-    if (k > M) break;
-    cplx<T> X;
-    if (k == 0 || k == M) {
-      const cplx<T> z0 = slot[0];
-      X = {k == 0 ? z0.x + z0.y : z0.x - z0.y, T(0)};
-    } else {
-      r2c_post(slot[k], slot[M - k], tw[RF::P * i], X);
+    if (k > M / 2) break;
+    const bool k0 = i == 0 && t == 0;  // Z_0 holds X_0 and X_M
+    const cplx<T> zk = slot[k];
+    const cplx<T> zm = k0 ? zk : rs[RF::P * (NI - 1 - i)];
+    const cplx<T> s = {zk.x + zm.x, zk.y - zm.y};  // Z_k + conj(Z_{M-k})
+    const cplx<T> d = {zk.x - zm.x, zk.y + zm.y};  // Z_k - conj(Z_{M-k})
+    const cplx<T> wd = cmul(twN[k], d);
+    cplx<T> xk = {T(0.5) * (s.x + wd.y), T(0.5) * (s.y - wd.x)};
+    cplx<T> xm = {T(0.5) * (s.x - wd.y), T(-0.5) * (s.y + wd.x)};
+    if (k0) {
+      xk = {zk.x + zk.y, T(0)};
+      xm = {zk.x - zk.y, T(0)};
     }
-    d[RF::P * i] = X;
+    dst[k] = xk;
+    if (2 * k != M) rd[RF::P * (NI - 1 - i)] = xm;
   }
 }
 
@@ -179,28 +190,9 @@ template <typename T, int M>
 __device__ __forceinline__ void r2c_from_regs(cplx<T>* slot, int t, const cplx<T>* __restrict__ twN,
                                               cplx<T>* __restrict__ dst, bool store,
                                               cplx<T> (&v)[RowF<T, M>::Q2][RowF<T, M>::R2]) {
-  using RF = RowF<T, M>;
-#pragma unroll
-  for (int q = 0; q < RF::Q2; ++q)
-#pragma unroll
-    for (int k2 = 0; k2 < RF::R2; ++k2) slot[t + RF::P * q + RF::R1 * k2] = v[q][k2];
+  r2c_regs_to_slot<T, M>(slot, t, v);
   __syncthreads();
-  if (!store) return;
-  cplx<T>* d = dst + t;
-  const cplx<T>* tw = twN + t;
-#pragma unroll
-  for (int i = 0; i < (M + RF::P) / RF::P; ++i) {
-    const int k = t + RF::P * i;
-    if (k > M) break;
-    cplx<T> X;
-    if (k == 0 || k == M) {
-      const cplx<T> z0 = slot[0];
-      X = {k == 0 ? z0.x + z0.y : z0.x - z0.y, T(0)};
-    } else {
-      r2c_post(slot[k], slot[M - k], tw[RF::P * i], X);
-    }
-    d[RF::P * i] = X;
-  }
+  if (store) r2c_slot_to_row<T, M>(slot, t, twN, dst);
 }
 
 // Real rows -> half spectra.  grid = (ceil(nrows/RB), narr), block = RB * P,
