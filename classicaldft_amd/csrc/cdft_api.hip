@@ -484,7 +484,7 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
   if (rows_reg_ok<T>(s) && s.pw3 && s.pw_rows <= 0) {
     const PwArgs pa{s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
                     s.vmf.p ? s.vmf.as<double>() + (long)x0 * s.g.Ny * s.g.Nz : nullptr, nrows, s.g.Zp,
-                    tw.m.p, tw.n.p, pp, s.partials.as<double2>() + blk_off, s.flag.as<int>(),
+                    tw.mp.p, tw.n.p, pp, s.partials.as<double2>() + blk_off, s.flag.as<int>(),
                     Batch{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr}, s.trace, MODEL, s.nb, s.stream,
                     ms};
     const long nblk = launch_pw3<T>(pa, M);
@@ -505,7 +505,7 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
       double* vmf = s.vmf.p ? s.vmf.as<double>() + (long)x0 * s.g.Ny * s.g.Nz : nullptr;
       hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk, s.nb), dim3(nconv * RB * RF::P),
                          size_t(RB) * per, s.stream, s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
-                         vmf, nrows, s.g.Zp, tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, pp,
+                         vmf, nrows, s.g.Zp, tw.mp.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, pp,
                          s.partials.as<double2>() + blk_off, s.flag.as<int>(), bt, s.trace, ms);
       HIPCHK(hipGetLastError());
     });
@@ -800,7 +800,7 @@ void launch_hess(cdft_system_s& s, bool mf, double inv_dp = 0.0, double scale = 
     const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, 0, nullptr};
     hipLaunchKernelGGL((k_z_hess2<MODEL, Mc>), dim3(div_up(s.nrows, RB), s.nb), dim3(nseq * RB * RF::P),
                        size_t(RB) * per, s.stream, s.B.as<cplx<double>>(), B2, s.Nkp, s.m,
-                       mf ? 1 : 0, s.vmf.as<double>(), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
+                       mf ? 1 : 0, s.vmf.as<double>(), s.nrows, s.g.Zp, tw.mp.template as<cplx<double>>(),
                        tw.n.template as<cplx<double>>(), RB, pp, inv_dp > 0 ? inv_dp : pp.inv_d, scale,
                        s.flag.as<int>(), bt, s.trace, single_offdiag);
     HIPCHK(hipGetLastError());
@@ -843,7 +843,7 @@ long launch_pw_ao(cdft_system_s& s, int nconv) {
     const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr};
     hipLaunchKernelGGL((k_z_pw_ao<MODEL, Mc>), dim3(nblk, s.nb), dim3(nseq * RB * RF::P), size_t(RB) * per, s.stream,
                        s.B.as<cplx<double>>(), s.B2.as<cplx<double>>(), s.Nkp, s.m, nconv, s.vmf.as<double>(), s.nrows,
-                       s.g.Zp, tw.m.template as<cplx<double>>(), tw.n.template as<cplx<double>>(), RB, pp,
+                       s.g.Zp, tw.mp.template as<cplx<double>>(), tw.n.template as<cplx<double>>(), RB, pp,
                        1.0 / (2 * s.d.ao_Rp), s.partials.as<double2>(), s.flag.as<int>(), bt);
     HIPCHK(hipGetLastError());
   });

@@ -86,11 +86,15 @@ __device__ __forceinline__ void dft_small(cplx<T>* v) {
 
 constexpr int cgcd(int a, int b) { return b == 0 ? a : cgcd(b, a % b); }
 
-// Factorisations L = R1 * R2 used by the register FFT.
+// Factorisations L = R1 * R2 used by the register FFT (f2_r1 also serves the host's twiddle tables).
+constexpr int f2_r1(int L) {
+  return L == 256 ? 16 : L == 128 ? 16 : L == 96 ? 12 : L == 64 ? 8 : L == 48 ? 12 : L == 32 ? 8 : L == 24 ? 8
+         : L == 16 ? 4 : L == 12 ? 4 : L == 8 ? 4 : L == 6 ? 3 : L == 4 ? 2 : L;
+}
+
 template <int L>
 struct F2 {
-  static constexpr int R1 = L == 256 ? 16 : L == 128 ? 16 : L == 96 ? 12 : L == 64 ? 8 : L == 48 ? 12 : L == 32 ? 8
-                          : L == 24 ? 8 : L == 16 ? 4 : L == 12 ? 4 : L == 8 ? 4 : L == 6 ? 3 : L == 4 ? 2 : L;
+  static constexpr int R1 = f2_r1(L);
   static constexpr int R2 = L / R1;
   static constexpr int P = cgcd(R1, R2);
   static constexpr int Q1 = R2 / P;  // pass-1 sequences (values of n2) per thread
@@ -128,6 +132,30 @@ __device__ __forceinline__ void reg_fft_pass1(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R
   }
 }
 
+// reg_fft_pass1 with its twiddles in pass order: twp[(q R1 + k1) P + t] = W_L^{(t + P q) k1} (the host's
+// upload_pass_twiddles), so the P lanes of a sequence read consecutive entries at constant offsets (no LDS bank
+// conflicts when the table sits in LDS) instead of the strided W_L^{n2 k1}.  Same values, same results.
+template <typename T, int L, int R1, int R2, bool FWD, int PAD = 0>
+__device__ __forceinline__ void reg_fft_pass1p(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1], int t, cplx<T>* lds, int ls,
+                                               const cplx<T>* __restrict__ twp) {
+  constexpr int P = cgcd(R1, R2), Q1 = R2 / P, S = R2 + PAD;
+  const cplx<T>* tt = twp + t;
+#pragma unroll
+  for (int q = 0; q < Q1; ++q) {
+    const int n2 = t + P * q;
+    dft_small<T, R1, FWD>(v1[q]);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      cplx<T> x = v1[q][k1];
+      if (k1) {
+        const cplx<T> w = tt[(q * R1 + k1) * P];
+        x = cmul(x, FWD ? w : cconj(w));
+      }
+      lds[(k1 * S + n2) * ls] = x;
+    }
+  }
+}
+
 template <typename T, int L, int R1, int R2, bool FWD, int PAD = 0>
 __device__ __forceinline__ void reg_fft_pass2(cplx<T> (&v2)[R1 / cgcd(R1, R2)][R2], int t, const cplx<T>* lds,
                                               int ls) {
@@ -146,6 +174,16 @@ __device__ __forceinline__ void reg_fft_seq(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1]
                                             cplx<T> (&v2)[R1 / cgcd(R1, R2)][R2], int t, cplx<T>* lds, int ls,
                                             const cplx<T>* __restrict__ tw) {
   reg_fft_pass1<T, L, R1, R2, FWD, PAD>(v1, t, lds, ls, tw);
+  __syncthreads();
+  reg_fft_pass2<T, L, R1, R2, FWD, PAD>(v2, t, lds, ls);
+}
+
+// reg_fft_seq with the pass-ordered pass-1 table of reg_fft_pass1p.
+template <typename T, int L, int R1, int R2, bool FWD, int PAD = 0>
+__device__ __forceinline__ void reg_fft_seqp(cplx<T> (&v1)[R2 / cgcd(R1, R2)][R1],
+                                             cplx<T> (&v2)[R1 / cgcd(R1, R2)][R2], int t, cplx<T>* lds, int ls,
+                                             const cplx<T>* __restrict__ twp) {
+  reg_fft_pass1p<T, L, R1, R2, FWD, PAD>(v1, t, lds, ls, twp);
   __syncthreads();
   reg_fft_pass2<T, L, R1, R2, FWD, PAD>(v2, t, lds, ls);
 }

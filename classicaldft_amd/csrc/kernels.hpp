@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_r2c(const TI* __restrict__ in
                                                        cplx<T>* __restrict__ out, long out_arr_stride, Geo g,
                                                        Radices radM, const cplx<T>* __restrict__ twM,
                                                        const cplx<T>* __restrict__ twN, int RB) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   const int M = g.Nz / 2;
   const long nrows = (long)g.Nx * g.Ny;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const cplx<T>* __restrict
                                                        TO* __restrict__ out, long out_arr_stride, Geo g, Radices radM,
                                                        const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
                                                        int RB, double scale) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   const int M = g.Nz / 2;
   const long nrows = (long)g.Nx * g.Ny;
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_rows_c2r(const cplx<T>* __restrict
 template <typename T, int MAXPTS, bool FWD>
 __global__ __launch_bounds__(kThreads) void k_cols(cplx<T>* __restrict__ data, long arr_stride, Geo g, int axis,
                                                    Radices rad, const cplx<T>* __restrict__ tw, int TW) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   const int kz0 = blockIdx.x * TW;
   const int o = blockIdx.y;
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_fwd(const cplx<T>* __restrict
                                                        long arr_stride, const T* __restrict__ W, int nconv,
                                                        unsigned odd_mask, Geo g, Radices radX,
                                                        const cplx<T>* __restrict__ twX, int TW) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int L = radX.L;
   cplx<T>* bufR = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* bufW = bufR + L * TW;
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(kThreads) void k_conv_bwd(const cplx<T>* __restrict
                                                        long arr_stride, const T* __restrict__ W, int m,
                                                        unsigned odd_mask, Geo g, Radices radX,
                                                        const cplx<T>* __restrict__ twX, int TW) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int L = radX.L;
   cplx<T>* bufA = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* bufW = bufA + L * TW;
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(kThreads) void k_z_pointwise(cplx<T>* __restrict__ 
                                                           const cplx<T>* __restrict__ twM,
                                                           const cplx<T>* __restrict__ twN, int RB, PhiParams pp,
                                                           double2* __restrict__ partials, int* __restrict__ flag) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   const int M = g.Nz / 2;
   const long nrows = (long)g.Nx * g.Ny;
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(kThreads) void k_z_final(const cplx<T>* __restrict_
                                                       double* __restrict__ dF, Geo g, Radices radM,
                                                       const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
                                                       int RB, FinalParams fp, double2* __restrict__ partials) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<T>* buf = reinterpret_cast<cplx<T>*>(smem_raw);
   const int M = g.Nz / 2;
   const long nrows = (long)g.Nx * g.Ny;

@@ -17,7 +17,7 @@ long go(const PwArgs& a) {
     const long nblk = (a.nrows + G::RB - 1) / G::RB;
     hipLaunchKernelGGL((k_z_pw3<T, MODEL, M, NA, MF>), dim3((unsigned)nblk, a.members), dim3(G::NT), G::lds_bytes,
                        a.stream, static_cast<cplx<T>*>(a.B), a.arr_stride, a.vmf, a.nrows, a.Zp,
-                       static_cast<const cplx<T>*>(a.twM), static_cast<const cplx<T>*>(a.twN), a.pp, a.partials,
+                       static_cast<const cplx<T>*>(a.twP), static_cast<const cplx<T>*>(a.twN), a.pp, a.partials,
                        a.flag, a.bt, a.ms);
     if (hipGetLastError() != hipSuccess) return -2;
     return nblk;

@@ -6,6 +6,8 @@
 // Compared with k_z_pointwise2:
 // * the two twiddle tables of the row transforms are copied to LDS once per
 //   block, so the transforms issue no vector-memory loads besides the rows;
+//   the pass-1 table is in pass order (reg_fft_pass1p), read without bank
+//   conflicts at constant offsets;
 // * the per-site loads and stores of the n_alpha / dPhi_alpha values use
 //   compile-time LDS offsets (no per-array address arithmetic or guards);
 // * rows per block are chosen for the precision (three blocks of at most
@@ -20,7 +22,7 @@ template <typename T, int M, int NS>
 struct Pw3Geo {
   using RF = RowF<T, M>;
   static constexpr int P = RF::P, SLOT = RF::SLOT;
-  static constexpr int TW = 2 * M + 1;  // twM (M entries) + twN (entries 0..M)
+  static constexpr int TW = 2 * M + 1;  // twP (M entries) + twN (entries 0..M)
   static constexpr int per_row = NS * SLOT * int(sizeof(cplx<T>));
   static constexpr int tw_bytes = TW * int(sizeof(cplx<T>));
   static constexpr int rows() {
@@ -44,7 +46,7 @@ struct Pw3Geo {
 template <typename T, int MODEL, int M, int NA, bool MF>
 __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3(
     cplx<T>* __restrict__ B, long arr_stride, double* __restrict__ vmf, long nrows, int Zp,
-    const cplx<T>* __restrict__ twM_g, const cplx<T>* __restrict__ twN_g, PhiParams pp,
+    const cplx<T>* __restrict__ twP_g, const cplx<T>* __restrict__ twN_g, PhiParams pp,
     double2* __restrict__ partials, int* __restrict__ flag, Batch bt, const double* __restrict__ ms) {
   constexpr int NS = NA + (MF ? 1 : 0);
   using G = Pw3Geo<T, M, NS>;
@@ -56,11 +58,11 @@ __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3
   if (MF) vmf += blockIdx.y * bt.sr;
   partials += blockIdx.y * bt.sp;
   flag += blockIdx.y;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* twM = lds + NS * RB * SLOT;
-  cplx<T>* twN = twM + M;
-  for (int i = threadIdx.x; i < G::TW; i += NT) twM[i] = i < M ? twM_g[i] : twN_g[i - M];
+  cplx<T>* twP = lds + NS * RB * SLOT;
+  cplx<T>* twN = twP + M;
+  for (int i = threadIdx.x; i < G::TW; i += NT) twP[i] = i < M ? twP_g[i] : twN_g[i - M];
   const bool act = threadIdx.x < G::NTF;  // a transform thread
   const int seq = act ? threadIdx.x / RF::P : 0, t = threadIdx.x % RF::P;
   const int a = seq / RB, r = seq % RB;
@@ -75,7 +77,7 @@ __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3
     __syncthreads();  // twiddle tables in place
     if (act) {
       c2r_to_regs<T, M>(brow, twN, t, v1);
-      reg_fft_pass1<T, M, RF::R1, RF::R2, false, 1>(v1, t, slot, 1, twM);
+      reg_fft_pass1p<T, M, RF::R1, RF::R2, false, 1>(v1, t, slot, 1, twP);
     }
     __syncthreads();
     if (act) reg_fft_pass2<T, M, RF::R1, RF::R2, false, 1>(v2, t, slot, 1);
@@ -136,7 +138,7 @@ __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3
         for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = slot[RF::R2 * n1 + RF::P * q + t];
     }
     __syncthreads();
-    if (act) reg_fft_pass1<T, M, RF::R1, RF::R2, true, 1>(v1, t, slot, 1, twM);
+    if (act) reg_fft_pass1p<T, M, RF::R1, RF::R2, true, 1>(v1, t, slot, 1, twP);
     __syncthreads();
     if (act) reg_fft_pass2<T, M, RF::R1, RF::R2, true, 1>(v2, t, slot, 1);
     __syncthreads();
@@ -156,7 +158,7 @@ struct PwArgs {
   double* vmf;
   long nrows;
   int Zp;
-  const void* twM;
+  const void* twP;  // pass-1 twiddles of length M in pass order (upload_pass_twiddles)
   const void* twN;
   PhiParams pp;
   double2* partials;

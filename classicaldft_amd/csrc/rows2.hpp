@@ -8,19 +8,25 @@
 // straight from HBM into the pass-1 registers; the r2c post-processing pairs
 // Z_k with Z_{M-k} through the sequence's LDS slot.  Each sequence owns a slot
 // of R1 (R2 + 1) elements: the padded exchange of the transform, or M values in
-// natural order.
+// natural order.  twP: the length-M pass-1 twiddles in pass order
+// (upload_pass_twiddles), twN: W_Nz^j for j <= M.
 #pragma once
 #include "kernels2.hpp"
 #include "phi.hpp"
 
 namespace cdft {
 
-// Slot stride: the padded exchange R1 (R2 + 1), rounded up so that
-// consecutive slots start 64 B apart modulo the 256 B bank window (the
-// sequences of one wavefront then spread over the LDS banks).
-constexpr int slot_stride(int base, int esz) {
-  for (int p = 0; p < 32; ++p)
-    if ((base + p) * esz % 256 == 64 || (base + p) * esz % 256 == 192) return base + p;
+// Slot stride: the padded exchange R1 (R2 + 1), rounded up so that consecutive slots start an odd multiple of
+// u = P esz bytes apart modulo the LDS's 256 B bank window (64 banks of 4 B).  An LDS access serves the lanes of
+// 256 B per cycle: the 256 / u sequences of those lanes, each touching P consecutive elements (or, in the padded
+// pass-2 reads, P elements R2 + 1 apart), then fill the window without sharing a bank.
+constexpr int slot_stride(int base, int esz, int P) {
+  const int u = P * esz;
+  if (u >= 256) return base;
+  for (int p = 0; p < 256; ++p) {
+    const int r = (base + p) * esz % 256;
+    if (r % u == 0 && (r / u) % 2 == 1) return base + p;
+  }
   return base;
 }
 
@@ -28,7 +34,7 @@ template <typename T, int M>
 struct RowF {
   using F = F2<M>;
   static constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
-  static constexpr int SLOT = slot_stride(R1 * (R2 + 1), int(sizeof(cplx<T>)));
+  static constexpr int SLOT = slot_stride(R1 * (R2 + 1), int(sizeof(cplx<T>)), P);
 };
 
 // Deterministic block reduction of NQ compensated sums for blockDim.x <= 1024:
@@ -200,11 +206,11 @@ __device__ __forceinline__ void r2c_from_regs(cplx<T>* slot, int t, const cplx<T
 // (member) blockIdx.y at psum[blockIdx.y * psp + block].
 template <typename T, typename TI, int M>
 __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, long in_arr_stride,
-                                                   cplx<T>* __restrict__ out, long out_arr_stride, long nrows, int Zp, const cplx<T>* __restrict__ twM,
+                                                   cplx<T>* __restrict__ out, long out_arr_stride, long nrows, int Zp, const cplx<T>* __restrict__ twP,
                                                    const cplx<T>* __restrict__ twN, int RB,
                                                    double2* __restrict__ psum = nullptr, long psp = 0) {
   using RF = RowF<T, M>;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
   const long row0 = (long)row_tile<T>() * RB + seq;
@@ -224,7 +230,7 @@ __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, lo
         acc[0].add(double(x.y));
       }
     }
-  reg_fft_seq<T, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+  reg_fft_seqp<T, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twP);
   __syncthreads();
   r2c_from_regs<T, M>(slot, t, twN, out + blockIdx.y * out_arr_stride + row * Zp, ok, v2);
   if (psum) block_reduce_store_n<1>(acc, psum + blockIdx.y * psp, row_tile<T>());
@@ -234,10 +240,10 @@ __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, lo
 template <typename T, typename TO, int M>
 __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ in, long in_arr_stride,
                                                    TO* __restrict__ out, long out_arr_stride, long nrows, int Zp,
-                                                   const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
+                                                   const cplx<T>* __restrict__ twP, const cplx<T>* __restrict__ twN,
                                                    int RB, double scale) {
   using RF = RowF<T, M>;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
   const long row0 = (long)row_tile<T>() * RB + seq;
@@ -245,7 +251,7 @@ __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ i
   const long row = ok ? row0 : nrows - 1;
   cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
   c2r_to_regs<T, M>(in + blockIdx.y * in_arr_stride + row * Zp, twN, t, v1);
-  reg_fft_seq<T, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+  reg_fft_seqp<T, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twP);
   if (!ok) return;
   cplx<TO>* dst = reinterpret_cast<cplx<TO>*>(out + blockIdx.y * out_arr_stride + row * 2 * M) + t;
 #pragma unroll
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(256) void k_rows_c2r2(const cplx<T>* __restrict__ i
 template <typename T, int MODEL, int M>
 __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B, long arr_stride, int m, int nconv,
                                                       double* __restrict__ vmf, long nrows, int Zp,
-                                                      const cplx<T>* __restrict__ twM,
+                                                      const cplx<T>* __restrict__ twP,
                                                       const cplx<T>* __restrict__ twN, int RB, PhiParams pp,
                                                       double2* __restrict__ partials, int* __restrict__ flag,
                                                       Batch bt, int trace, const double* __restrict__ ms) {
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
   partials += blockIdx.y * bt.sp;
   flag += blockIdx.y;
   using RF = RowF<T, M>;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   const int a = seq / RB, r = seq % RB;
@@ -290,7 +296,7 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
   {
     cplx<T> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
     c2r_to_regs<T, M>(brow, twN, t, v1);
-    reg_fft_seq<T, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+    reg_fft_seqp<T, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twP);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RF::Q2; ++q)
@@ -338,7 +344,7 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
 #pragma unroll
       for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = slot[RF::R2 * n1 + RF::P * q + t];
     __syncthreads();
-    reg_fft_seq<T, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+    reg_fft_seqp<T, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twP);
     __syncthreads();
     r2c_from_regs<T, M>(slot, t, twN, brow, ok && a < m, v2);
   }
@@ -357,7 +363,7 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
 template <int MODEL, int M>
 __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, const cplx<double>* __restrict__ B2,
                                                  long arr_stride, int m, int mf, double* __restrict__ vmf, long nrows,
-                                                 int Zp, const cplx<double>* __restrict__ twM,
+                                                 int Zp, const cplx<double>* __restrict__ twP,
                                                  const cplx<double>* __restrict__ twN, int RB, PhiParams pp,
                                                  double inv_dp, double scale, int* __restrict__ flag, Batch bt,
                                                  int trace, int single_offdiag) {
@@ -366,7 +372,7 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
   if (vmf) vmf += blockIdx.y * bt.sr;
   flag += blockIdx.y;
   using RF = RowF<double, M>;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   const int a = seq / RB, r = seq % RB;
@@ -379,7 +385,7 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
   {
     cplx<double> v1[RF::Q1][RF::R1], v2[RF::Q2][RF::R2];
     c2r_to_regs<double, M>(src, twN, t, v1);
-    reg_fft_seq<double, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+    reg_fft_seqp<double, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twP);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RF::Q2; ++q)
@@ -439,7 +445,7 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
 #pragma unroll
       for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = slot[RF::R2 * n1 + RF::P * q + t];
     __syncthreads();
-    reg_fft_seq<double, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+    reg_fft_seqp<double, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twP);
     __syncthreads();
     r2c_from_regs<double, M>(slot, t, twN, brow, ok && a < m, v2);
   }
@@ -453,7 +459,7 @@ __global__ __launch_bounds__(512) void k_z_hess2(cplx<double>* __restrict__ B, c
 template <int MODEL, int M>
 __global__ __launch_bounds__(512) void k_z_pw_ao(cplx<double>* __restrict__ B, cplx<double>* __restrict__ B2,
                                                  long arr_stride, int m, int nconv, double* __restrict__ vmf,
-                                                 long nrows, int Zp, const cplx<double>* __restrict__ twM,
+                                                 long nrows, int Zp, const cplx<double>* __restrict__ twP,
                                                  const cplx<double>* __restrict__ twN, int RB, PhiParams pp,
                                                  double inv_dp, double2* __restrict__ partials, int* __restrict__ flag,
                                                  Batch bt) {
@@ -463,7 +469,7 @@ __global__ __launch_bounds__(512) void k_z_pw_ao(cplx<double>* __restrict__ B, c
   partials += blockIdx.y * bt.sp;
   flag += blockIdx.y;
   using RF = RowF<double, M>;
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
   const int a = seq / RB, r = seq % RB;
@@ -483,7 +489,7 @@ __global__ __launch_bounds__(512) void k_z_pw_ao(cplx<double>* __restrict__ B, c
 #pragma unroll
         for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = {0.0, 0.0};
     }
-    reg_fft_seq<double, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twM);
+    reg_fft_seqp<double, M, RF::R1, RF::R2, false, 1>(v1, v2, t, slot, 1, twP);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RF::Q2; ++q)
@@ -528,7 +534,7 @@ __global__ __launch_bounds__(512) void k_z_pw_ao(cplx<double>* __restrict__ B, c
 #pragma unroll
       for (int n1 = 0; n1 < RF::R1; ++n1) v1[q][n1] = slot[RF::R2 * n1 + RF::P * q + t];
     __syncthreads();
-    reg_fft_seq<double, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twM);
+    reg_fft_seqp<double, M, RF::R1, RF::R2, true, 1>(v1, v2, t, slot, 1, twP);
     __syncthreads();
     r2c_from_regs<double, M>(slot, t, twN, brow, ok && (a < m || is_ao), v2);
   }
@@ -573,9 +579,11 @@ __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Ac
   partials += mb * bt.sp;
   if (bt.mu) fp.mu = bt.mu[mb];
   using RF = RowF<T, M>;
-  extern __shared__ unsigned char smem_raw[];
+  // 8-byte-aligned view of the dynamic LDS and the plain twiddle table: 16-byte exchange accesses and the
+  // pass-ordered table measured slower in this kernel (64-thread blocks) than in the other row kernels
+  extern __shared__ unsigned char smem_fin[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
-  cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_raw) + seq * RF::SLOT;
+  cplx<T>* slot = reinterpret_cast<cplx<T>*>(smem_fin) + seq * RF::SLOT;
   const long row0 = (long)row_tile<T>() * RB + seq;
   const bool ok = row0 < nrows;
   const long row = ok ? row0 : nrows - 1;

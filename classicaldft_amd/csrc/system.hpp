@@ -113,14 +113,36 @@ inline void upload_twiddles(DevBuf& buf, int L) {
   HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cplx<T>) * L, hipMemcpyHostToDevice));
 }
 
+template <typename Fn>
+inline bool with_reg_len(int L, Fn&& f);
+
+// The pass-1 twiddles of the register FFT of length L in pass order (reg_fft_pass1p): entry (q R1 + k1) P + t is
+// W_L^{(t + P q) k1} with L = R1 R2, P = gcd(R1, R2), q < R2 / P; the same values as the plain table's entries.
+template <typename T>
+inline void upload_pass_twiddles(DevBuf& buf, int L) {
+  const int R1 = f2_r1(L), R2 = L / R1, P = cgcd(R1, R2), Q1 = R2 / P;
+  std::vector<cplx<T>> h(L);
+  const long double tp = 6.283185307179586476925286766559005768L;
+  for (int q = 0; q < Q1; ++q)
+    for (int k1 = 0; k1 < R1; ++k1)
+      for (int t = 0; t < P; ++t) {
+        const long double a = -tp * (long double)((t + P * q) * k1) / (long double)L;
+        h[(q * R1 + k1) * P + t] = {T(cosl(a)), T(sinl(a))};
+      }
+  buf.alloc(sizeof(cplx<T>) * L);
+  HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cplx<T>) * L, hipMemcpyHostToDevice));
+}
+
 template <typename T>
 struct Tables {
   DevBuf x, y, m, n;  // Nx, Ny, Nz/2, Nz
+  DevBuf mp;          // Nz/2 in pass order (register z rows only)
   void build(const Geo& g) {
     upload_twiddles<T>(x, g.Nx);
     upload_twiddles<T>(y, g.Ny);
     upload_twiddles<T>(m, std::max(1, g.Nz / 2));
     upload_twiddles<T>(n, g.Nz);
+    if (g.Nz % 2 == 0 && with_reg_len(g.Nz / 2, [](auto) {})) upload_pass_twiddles<T>(mp, g.Nz / 2);
   }
 };
 
@@ -275,7 +297,7 @@ inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride
       if (nblk_out) *nblk_out = div_up(s.nrows, RB);
       hipLaunchKernelGGL((k_rows_r2c2<T, TI, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
                          RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride, out, out_stride, s.nrows, s.g.Zp,
-                         tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, psum, psp);
+                         tw.mp.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, psum, psp);
     });
     HIPCHK(hipGetLastError());
     return;
@@ -299,7 +321,7 @@ inline void launch_rows_c2r(const cdft_system_s& s, const cplx<T>* in, long in_s
       constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
       hipLaunchKernelGGL((k_rows_c2r2<T, TO, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
                          RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride, out, out_stride, s.nrows, s.g.Zp,
-                         tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, scale);
+                         tw.mp.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, scale);
     });
     HIPCHK(hipGetLastError());
     return;
