@@ -46,6 +46,11 @@ class _Desc(ctypes.Structure):
     ]
 
 
+class _MixDesc(ctypes.Structure):
+    _fields_ = [("Nx", _i), ("Ny", _i), ("Nz", _i), ("dx", _d), ("dy", _d), ("dz", _d), ("model", _i), ("A", _d),
+                ("B", _d), ("nspecies", _i), ("hsd", _p), ("fmt_weights", _p), ("weights_mode", _i)]
+
+
 class _PotDesc(ctypes.Structure):
     _fields_ = [("kind", _i), ("sigma", _d), ("eps", _d), ("rcut", _d), ("alpha", _d), ("bh", _i),
                 ("r_att_min", _d)]
@@ -104,6 +109,10 @@ def lib():
         L.cdft_system_destroy.argtypes = [_p]
         L.cdft_system_eval.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_fmt_eval.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_mixture_create.argtypes = [ctypes.POINTER(_MixDesc), ctypes.POINTER(_p)]
+        L.cdft_mixture_destroy.argtypes = [_p]
+        L.cdft_mixture_species.argtypes = [_p, ctypes.POINTER(_i)]
+        L.cdft_mixture_fmt_eval.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_system_stream.argtypes = [_p, ctypes.POINTER(_p)]
         L.cdft_system_a_vdw.argtypes = [_p, ctypes.POINTER(_d)]
         L.cdft_system_layout.argtypes = [_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]
@@ -674,6 +683,85 @@ class FireGroup:
             self.close()
         except Exception:
             pass
+
+
+class Mixture:
+    """Hard spheres of several diameters on one lattice (cdft_mixture_*): the FMT of a mixture,
+    FMT::calculateFreeEnergyAndDerivatives over several FMT species (src/FMT.cpp:363-427).  FP64."""
+
+    def __init__(self, shape, dx, hsd, model=WHITEBEAR_I, weights=None, A=1.0, B=0.0,
+                 weights_mode=WEIGHTS_ACCURATE):
+        """hsd: one diameter per species; weights: None, or one entry per species (None or 11*N cell weights)."""
+        self.shape = tuple(int(n) for n in shape)
+        dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+        self.dx = dxs
+        self.N = int(np.prod(self.shape))
+        self._hsd = np.ascontiguousarray(np.atleast_1d(hsd), dtype=np.float64).reshape(-1)
+        self.nspecies = int(self._hsd.size)
+        if self.nspecies < 1:
+            raise ValueError("a mixture needs at least one species")
+        wptr = None
+        self._w = []
+        if weights is not None:
+            if len(weights) != self.nspecies:
+                raise ValueError(f"expected {self.nspecies} weight sets")
+            ptrs = (_p * self.nspecies)()
+            for i, w in enumerate(weights):
+                if w is None:
+                    ptrs[i] = None
+                    continue
+                a = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+                if a.size != 11 * self.N:
+                    raise ValueError(f"expected {11 * self.N} cell weights per species")
+                self._w.append(a)
+                ptrs[i] = a.ctypes.data
+            self._wptrs = ptrs
+            wptr = ctypes.cast(ptrs, _p)
+        d = _MixDesc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A),
+                     float(B), self.nspecies, self._hsd.ctypes.data, wptr, int(weights_mode))
+        h = _p()
+        _check(lib().cdft_mixture_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self._w = []
+        self._wptrs = None
+        self.model = model
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cdft_mixture_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fmt_eval(self, rho, dF=None, accumulate=False):
+        """Returns (F_hs, dF) for rho of shape (nspecies,) + shape; dF has rho's shape (written, or added to)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        if rho.size != self.nspecies * self.N:
+            raise ValueError(f"expected {self.nspecies} x {self.N} densities, got {rho.size}")
+        rho = rho.reshape((self.nspecies,) + self.shape)
+        out = _out(dF, self.nspecies * self.N, (self.nspecies,) + self.shape)
+        out = out.reshape((self.nspecies,) + self.shape)
+        rp = (_p * self.nspecies)(*[rho[i].ctypes.data for i in range(self.nspecies)])
+        fp = (_p * self.nspecies)(*[out[i].ctypes.data for i in range(self.nspecies)])
+        F = _d(0.0)
+        _check(lib().cdft_mixture_fmt_eval(self.h, ctypes.cast(rp, _p), ctypes.cast(fp, _p), ctypes.byref(F),
+                                           ACCUMULATE if accumulate else 0))
+        return F.value, out
+
+    def fmt_eval_device(self, rho_ptrs, dF_ptrs, accumulate=False):
+        """Device pointers (one per species): returns F_hs; dF written or added to on the device."""
+        if len(rho_ptrs) != self.nspecies or len(dF_ptrs) != self.nspecies:
+            raise ValueError(f"expected {self.nspecies} density and force pointers")
+        rp = (_p * self.nspecies)(*rho_ptrs)
+        fp = (_p * self.nspecies)(*dF_ptrs)
+        F = _d(0.0)
+        _check(lib().cdft_mixture_fmt_eval(self.h, ctypes.cast(rp, _p), ctypes.cast(fp, _p), ctypes.byref(F),
+                                           DEVICE_PTRS | (ACCUMULATE if accumulate else 0)))
+        return F.value
 
 
 class DeviceArray:

@@ -10,6 +10,7 @@
 
 #include "system.hpp"
 #include "pw3.hpp"
+#include "mixture.hpp"
 
 namespace cdft {
 template <typename T>
@@ -312,7 +313,7 @@ void build_weight_spectra(cdft_system_s& s) {
   S.alloc(sizeof(cplx<double>) * Nkp);
   const int nb = div_up(Nkp, kThreads);
   const char* nt = std::getenv("CDFT_NO_TRACE");
-  const bool allow_trace = !s.has_ao && !(nt && nt[0] == '1') && rows_reg_ok<T>(s) && reg_fft_ok<T>(s, s.g.Nx) &&
+  const bool allow_trace = s.trace_ok && !s.has_ao && !(nt && nt[0] == '1') && rows_reg_ok<T>(s) && reg_fft_ok<T>(s, s.g.Nx) &&
                            reg_fft_ok<T>(s, s.g.Ny) && rows_reg_ok<double>(s) && reg_fft_ok<double>(s, s.g.Nx) &&
                            reg_fft_ok<double>(s, s.g.Ny);
   if (s.m > 0)
@@ -354,13 +355,15 @@ void build_weight_spectra(cdft_system_s& s) {
   fold_weight_spectra<T>(s);
 }
 
-void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const double* mus, const double* vexts) {
+void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const double* mus, const double* vexts,
+                   bool allow_trace) {
   REQUIRE(desc && out, "null argument");
   REQUIRE(nb >= 1, "member count must be positive");
   validate(*desc);
   auto s = std::make_unique<cdft_system_s>();
   s->d = *desc;
   s->nb = nb;
+  s->trace_ok = allow_trace;
   HIPCHK(hipGetDevice(&s->device));
   scratch_setup(s->device);
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -1174,6 +1177,159 @@ int standalone_fft(int Nx, int Ny, int Nz, int precision, const double* in, doub
   return CDFT_OK;
 }
 
+// ------------------------------------------------------------------ mixtures
+// The FMT of several hard-sphere species (mixture.hpp).  Each species is an FP64 FMT-only member system holding its
+// weight spectra and transform buffers; all members share the first member's stream.
+template <int MODEL>
+long launch_pw_mix(cdft_system_s& s, cplx<double>* Mx, int nm, double2* partials, int* flag) {
+  const Tables<double>& tw = tables<double>(s);
+  const PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B, MODEL == M_WBI || MODEL == M_WBII);
+  long nblk = -1;
+  with_reg_len(s.g.Nz / 2, [&](auto mc) {
+    constexpr int Mc = decltype(mc)::value;
+    using RF = RowF<double, Mc>;
+    const int per = nm * RF::SLOT * int(sizeof(cplx<double>));
+    const int RB = std::min(kThreads / (nm * RF::P), (65536 - 4096) / per);
+    if (RB < 1) return;
+    nblk = div_up(s.nrows, RB);
+    hipLaunchKernelGGL((k_z_pw_mix<MODEL, Mc>), dim3(nblk), dim3(nm * RB * RF::P), size_t(RB) * per, s.stream, Mx,
+                       s.Nkp, nm, s.nrows, s.g.Zp, tw.mp.template as<cplx<double>>(),
+                       tw.n.template as<cplx<double>>(), RB, pp, partials, flag);
+    HIPCHK(hipGetLastError());
+  });
+  REQUIRE(nblk > 0, "the mixture's z rows do not fit the pointwise kernel");
+  return nblk;
+}
+
+}  // namespace cdft_impl
+
+struct cdft_mixture_s {
+  std::vector<cdft_system> sp;  // one member system per species
+  std::vector<double> hsd;
+  int nm = 0;                   // measure arrays (16 with the tensor, 10 without)
+  DevBuf Mx, partials, flag, results;
+  ~cdft_mixture_s() {
+    for (size_t i = 1; i < sp.size(); ++i)
+      if (sp[i]) {
+        sp[i]->stream = nullptr;  // the first member's stream, destroyed with it
+        delete sp[i];
+      }
+    if (!sp.empty()) delete sp[0];
+  }
+};
+
+namespace cdft_impl {
+
+void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
+  REQUIRE(d && out, "null argument");
+  REQUIRE(d->nspecies >= 1 && d->hsd, "a mixture needs at least one species and its diameters");
+  REQUIRE(d->model >= CDFT_ROSENFELD && d->model <= CDFT_WHITEBEAR_II, "a mixture needs an FMT model");
+  auto x = std::make_unique<cdft_mixture_s>();
+  for (int i = 0; i < d->nspecies; ++i) {
+    cdft_system_desc sd{};
+    sd.Nx = d->Nx;
+    sd.Ny = d->Ny;
+    sd.Nz = d->Nz;
+    sd.dx = d->dx;
+    sd.dy = d->dy;
+    sd.dz = d->dz;
+    sd.model = d->model;
+    sd.A = d->A;
+    sd.B = d->B;
+    sd.hsd = d->hsd[i];
+    sd.precision = CDFT_FP64;
+    sd.fmt_weights = d->fmt_weights ? d->fmt_weights[i] : nullptr;
+    sd.weights_mode = d->weights_mode;
+    cdft_system s = nullptr;
+    create_system(&sd, &s, 1, nullptr, nullptr, false);
+    x->sp.push_back(s);
+    x->hsd.push_back(d->hsd[i]);
+    if (i > 0) {  // one stream for the whole mixture
+      HIPCHK(hipStreamSynchronize(s->stream));
+      HIPCHK(hipStreamDestroy(s->stream));
+      s->stream = x->sp[0]->stream;
+    }
+  }
+  cdft_system_s& s0 = *x->sp[0];
+  REQUIRE(rows_reg_ok<double>(s0) && reg_fft_ok<double>(s0, s0.g.Nx) && reg_fft_ok<double>(s0, s0.g.Ny),
+          "mixtures need Nx, Ny and Nz/2 in {4,6,8,12,16,24,32,48,64,96,128,256}");
+  x->nm = s0.m == 11 ? kMixMeas : 10;
+  x->Mx.alloc(sizeof(cplx<double>) * s0.Nkp * x->nm);
+  x->partials.alloc(sizeof(double2) * s0.pslots);
+  x->flag.alloc(sizeof(int));
+  HIPCHK(hipMemset(x->flag.p, 0, sizeof(int)));
+  x->results.alloc(sizeof(double) * 8);
+  *out = x.release();
+}
+
+// F_hs (x dV) and dF_s of the mixture on device data (dF written, or added to with acc).
+int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF, bool acc, double* F_hs) {
+  cdft_system_s& s0 = *x.sp[0];
+  const long Nkp = s0.Nkp;
+  const int m = s0.m, S = (int)x.sp.size();
+  const int tensor = m == 11;
+  cplx<double>* Mx = x.Mx.as<cplx<double>>();
+  const dim3 eg(std::min(div_up(Nkp, 256), 8192)), eb(256);
+  // 1. each species' weighted densities (z spectral, x and y back in real space), summed into the measures
+  for (int i = 0; i < S; ++i) {
+    cdft_system_s& s = *x.sp[i];
+    forward_weighted(s, rho[i], s.A.as<cplx<double>>(), s.B.as<cplx<double>>(), m);
+    hipLaunchKernelGGL(k_mix_accum, eg, eb, 0, s0.stream, s.B.as<cplx<double>>(), Mx, Nkp, Nkp, x.hsd[i], tensor,
+                       (int)(i == 0));
+    HIPCHK(hipGetLastError());
+  }
+  // 2. Phi and its measure derivatives per site (z inverse and forward in the kernel)
+  long nblk = 0;
+  switch (s0.d.model) {
+    case CDFT_ROSENFELD: nblk = launch_pw_mix<M_ROSENFELD>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
+    case CDFT_RSLT: nblk = launch_pw_mix<M_RSLT>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
+    case CDFT_ESFMT: nblk = launch_pw_mix<M_ESFMT>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
+    case CDFT_WHITEBEAR_I: nblk = launch_pw_mix<M_WBI>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
+    default: nblk = launch_pw_mix<M_WBII>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
+  }
+  hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s0.stream, x.partials.as<double2>(), nblk, 1,
+                     x.results.as<double>(), 0L, 8L);
+  HIPCHK(hipGetLastError());
+  // 3. y forward of the measure derivatives, then per species: collapse, x back-convolution, y and z inverse
+  launch_cols<double>(s0, Mx, Nkp, x.nm, 1, true, 1, 0);
+  FinalParams fp{};
+  fp.dV = s0.d.dx * s0.d.dy * s0.d.dz;
+  fp.fmt = 1;
+  fp.accumulate = acc;
+  for (int i = 0; i < S; ++i) {
+    cdft_system_s& s = *x.sp[i];
+    cplx<double>* A = s.A.as<cplx<double>>();
+    cplx<double>* B = s.B.as<cplx<double>>();
+    hipLaunchKernelGGL(k_mix_collapse, eg, eb, 0, s0.stream, Mx, B, Nkp, Nkp, x.hsd[i], tensor);
+    HIPCHK(hipGetLastError());
+    conv_bwd_f64(s, B, A, s.W.as<double>(), m);
+    launch_cols<double>(s, A, 0, 1, 1, false, 1, Nkp);
+    const Tables<double>& tw = tables<double>(s);
+    with_reg_len(s.g.Nz / 2, [&](auto mc) {
+      constexpr int Mc = decltype(mc)::value;
+      constexpr int RB = row_rb<double, Mc>(32768), P = RowF<double, Mc>::P, SLOT = RowF<double, Mc>::SLOT;
+      const Batch bt{Nkp, s.nconv * Nkp, s.N, 4 * s.pslots, nullptr};
+      hipLaunchKernelGGL((k_z_final2<double, Mc>), dim3(div_up(s.nrows, RB), 1), dim3(RB * P),
+                         RB * SLOT * sizeof(cplx<double>), s0.stream, A, rho[i], nullptr, nullptr, dF[i],
+                         unsigned(s.N * 8), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
+                         tw.n.template as<cplx<double>>(), RB, fp, s.partials2.as<double2>(), bt);
+    });
+    HIPCHK(hipGetLastError());
+  }
+  double res = 0.0;
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&res, x.results.p, sizeof(double), hipMemcpyDeviceToHost, s0.stream));
+  HIPCHK(hipMemcpyAsync(&bad, x.flag.p, sizeof(int), hipMemcpyDeviceToHost, s0.stream));
+  HIPCHK(hipStreamSynchronize(s0.stream));
+  if (bad) {
+    HIPCHK(hipMemset(x.flag.p, 0, sizeof(int)));
+    g_err = "Eta too large in FMT::dPHI";
+    return CDFT_ETA_TOO_LARGE;
+  }
+  if (F_hs) *F_hs = res * fp.dV;
+  return CDFT_OK;
+}
+
 }  // namespace cdft_impl
 
 using namespace cdft_impl;
@@ -1902,6 +2058,59 @@ int cdft_system_layout(cdft_system sys, int* stored, int* convolved) {
     REQUIRE(sys && stored && convolved, "null argument");
     *stored = sys->m;
     *convolved = sys->nconv;
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_mixture_create(const cdft_mixture_desc* desc, cdft_mixture* out) {
+  return guarded([&] {
+    create_mixture(desc, out);
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_mixture_destroy(cdft_mixture mix) {
+  return guarded([&] {
+    if (mix) {
+      (void)hipStreamSynchronize(mix->sp[0]->stream);
+      delete mix;
+    }
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_mixture_species(cdft_mixture mix, int* nspecies) {
+  return guarded([&] {
+    REQUIRE(mix && nspecies, "null argument");
+    *nspecies = (int)mix->sp.size();
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_mixture_fmt_eval(cdft_mixture mix, const double* const* rho, double* const* dF, double* F_hs, int flags) {
+  return guarded([&] {
+    REQUIRE(mix && rho && dF, "null argument");
+    const int S = (int)mix->sp.size();
+    for (int i = 0; i < S; ++i) REQUIRE(rho[i] && dF[i], "null species array");
+    const bool acc = flags & CDFT_ACCUMULATE;
+    if (flags & CDFT_DEVICE_PTRS) return mixture_eval(*mix, rho, dF, acc, F_hs);
+    // host data: staged in each member's density and force buffers
+    cdft_system_s& s0 = *mix->sp[0];
+    const size_t bytes = sizeof(double) * s0.N;
+    std::vector<const double*> rd(S);
+    std::vector<double*> fd(S);
+    for (int i = 0; i < S; ++i) {
+      cdft_system_s& s = *mix->sp[i];
+      HIPCHK(hipMemcpyAsync(s.rho.p, rho[i], bytes, hipMemcpyHostToDevice, s0.stream));
+      if (acc) HIPCHK(hipMemcpyAsync(s.dF.p, dF[i], bytes, hipMemcpyHostToDevice, s0.stream));
+      rd[i] = s.rho.as<double>();
+      fd[i] = s.dF.as<double>();
+    }
+    const int rc = mixture_eval(*mix, rd.data(), fd.data(), acc, F_hs);
+    if (rc != CDFT_OK) return rc;
+    for (int i = 0; i < S; ++i)
+      HIPCHK(hipMemcpyAsync(dF[i], mix->sp[i]->dF.p, bytes, hipMemcpyDeviceToHost, s0.stream));
+    HIPCHK(hipStreamSynchronize(s0.stream));
     return (int)CDFT_OK;
   });
 }

@@ -206,6 +206,7 @@ struct cdft_system_s {
   bool nb_half_cols = false, nb_half_fwd = false, nb_half_bwd = false;
   int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
   bool pw3 = true;       // specialised pointwise kernel (pw3.hpp) where an instance covers the system
+  bool trace_ok = true;  // the trace-reduced weight set may be used (create_system's allow_trace)
   // mixed path: the means of the weighted densities and of dPhi are carried in FP64 and the FP32 arrays hold the
   // deviations from them (msplit: kMS doubles per member; w0: the FP64 weight spectra at k = 0, nconv values)
   bool mean_split = false;
@@ -470,8 +471,9 @@ void enqueue_eval(cdft_system s, const double* rho_d, double* dF_d, bool with_df
 void constraints_pre(cdft_system_s& s, double* rho);
 // nb > 1: a batch of nb same-shape members with their own mus[nb] and V_ext
 // (nb x N or none); desc->mu / desc->vext are then ignored.
+// allow_trace = false keeps all 11 tensor-model arrays (the members of a mixture, which sum them per measure).
 void create_system(const cdft_system_desc* desc, cdft_system* out, int nb = 1, const double* mus = nullptr,
-                   const double* vexts = nullptr);
+                   const double* vexts = nullptr, bool allow_trace = true);
 int fetch_parts(cdft_system s, double* parts);
 double csum(std::initializer_list<double> v);
 template <typename F>

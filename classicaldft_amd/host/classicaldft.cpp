@@ -149,6 +149,7 @@ int ao_force_of(const Species* s) {
 // ---------------------------------------------------------------------- FMT
 FMT::~FMT() {
   if (sys_) cdft_system_destroy(sys_);
+  if (mix_) cdft_mixture_destroy(mix_);
 }
 
 std::string esFMT::Name() const {
@@ -185,18 +186,61 @@ cdft_system FMT::system_for(const Species& s) const {
   return sys_;
 }
 
-double FMT::calculateFreeEnergyAndDerivatives(std::vector<Species*>& allSpecies) {
-  double F = 0.0;
-  int n = 0;
-  for (Species* s : allSpecies) {
-    auto* fs = dynamic_cast<FMT_Species*>(s);
-    if (!fs) continue;
-    if (++n > 1) throw std::runtime_error("FMT on the GPU is implemented for a single species");
-    cdft_system sys = system_for(*fs);
-    double Fs = 0.0;
-    check(cdft_fmt_eval(sys, fs->getDensity().get_density_pointer(), fs->getDF().memptr(), &Fs, CDFT_ACCUMULATE));
-    F += Fs;
+cdft_mixture FMT::mixture_for(const std::vector<FMT_Species*>& species) const {
+  std::vector<const FMT_Species*> key(species.begin(), species.end());
+  std::vector<double> hsd;
+  for (const FMT_Species* fs : species) {
+    if (fs->ao_Rp() > 0) throw std::runtime_error("an AO polymer is implemented for a single FMT species");
+    const Lattice &L = fs->getLattice(), &L0 = species.front()->getLattice();
+    if (L.Nx() != L0.Nx() || L.Ny() != L0.Ny() || L.Nz() != L0.Nz() || L.getDX() != L0.getDX() ||
+        L.getDY() != L0.getDY() || L.getDZ() != L0.getDZ())
+      throw std::runtime_error("the species of a mixture must share one lattice");
+    hsd.push_back(fs->getHSD());
   }
+  if (mix_ && key == mix_species_ && hsd == mix_hsd_) return mix_;
+  if (mix_) cdft_mixture_destroy(mix_);
+  mix_ = nullptr;
+  const Lattice& L = species.front()->getLattice();
+  cdft_mixture_desc d{};
+  d.Nx = (int)L.Nx();
+  d.Ny = (int)L.Ny();
+  d.Nz = (int)L.Nz();
+  d.dx = L.getDX();
+  d.dy = L.getDY();
+  d.dz = L.getDZ();
+  d.model = model_;
+  d.A = A_;
+  d.B = B_;
+  d.nspecies = (int)hsd.size();
+  d.hsd = hsd.data();
+  check(cdft_mixture_create(&d, &mix_));
+  mix_species_ = key;
+  mix_hsd_ = hsd;
+  return mix_;
+}
+
+// src/FMT.cpp:363-427: one FMT species runs the single-species pipeline, several the mixture's (the measures sum
+// over species, FMT::getWeightedDensities src/FMT.cpp:78-111).  Both add dF_hs to each species' force.
+double FMT::calculateFreeEnergyAndDerivatives(std::vector<Species*>& allSpecies) {
+  std::vector<FMT_Species*> fmt;
+  for (Species* s : allSpecies)
+    if (auto* fs = dynamic_cast<FMT_Species*>(s)) fmt.push_back(fs);
+  if (fmt.empty()) return 0.0;
+  double F = 0.0;
+  if (fmt.size() == 1) {
+    cdft_system sys = system_for(*fmt[0]);
+    check(cdft_fmt_eval(sys, fmt[0]->getDensity().get_density_pointer(), fmt[0]->getDF().memptr(), &F,
+                        CDFT_ACCUMULATE));
+    return F;
+  }
+  cdft_mixture mix = mixture_for(fmt);
+  std::vector<const double*> rho;
+  std::vector<double*> dF;
+  for (FMT_Species* fs : fmt) {
+    rho.push_back(fs->getDensity().get_density_pointer());
+    dF.push_back(fs->getDF().memptr());
+  }
+  check(cdft_mixture_fmt_eval(mix, rho.data(), dF.data(), &F, CDFT_ACCUMULATE));
   return F;
 }
 

@@ -364,6 +364,38 @@ int cdft_system_a_vdw(cdft_system sys, double* a_vdw); /* Interaction_Base::getV
    No counterpart in the reference (its FMT_Species always stores 11, include/Species.h:351-360); for callers that
    size buffers or count bytes. */
 int cdft_system_layout(cdft_system sys, int* stored_arrays, int* convolved_arrays);
+/* ---- Hard-sphere mixtures ----
+   Several FMT species of different diameters on one lattice: FMT::calculateFreeEnergyAndDerivatives with more
+   than one FMT_Species in allSpecies (src/FMT.cpp:363-427).  The fundamental measures sum every species' weighted
+   densities scaled by its diameter (FMT::getWeightedDensities, src/FMT.cpp:78-111), Phi is taken once per site,
+   and each species collapses dPhi with its own diameter (FMT_Species::set_fundamental_measure_derivatives,
+   include/Species.h:310-329) before its back-convolution.  FP64; Nx, Ny and Nz/2 must be register-FFT lengths
+   (4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 256). */
+typedef struct cdft_mixture_s* cdft_mixture;
+
+typedef struct cdft_mixture_desc {
+  int Nx, Ny, Nz;
+  double dx, dy, dz;
+  int model;                         /* enum cdft_model (an FMT model) */
+  double A, B;                       /* esFMT coefficients */
+  int nspecies;                      /* >= 1 */
+  const double* hsd;                 /* host, nspecies hard-sphere diameters (FMT_Species::getHSD) */
+  const double* const* fmt_weights;  /* host, NULL, or nspecies pointers to 11*N cell weights (NULL entries are
+                                        generated on the device, as for a single species) */
+  int weights_mode;                  /* enum cdft_weights_mode */
+} cdft_mixture_desc;
+
+/* Replaces the construction of the species' FMT_Species weights (src/FMT_Species.cpp:95-108, 310-496) for a
+   mixture. */
+int cdft_mixture_create(const cdft_mixture_desc* desc, cdft_mixture* out);
+int cdft_mixture_destroy(cdft_mixture mix);
+int cdft_mixture_species(cdft_mixture mix, int* nspecies);
+/* Replaces FMT::calculateFreeEnergyAndDerivatives(vector<Species*>&) over several FMT_Species
+   (src/FMT.cpp:363-427): rho[s], dF[s] are nspecies arrays of N values (host, or device with CDFT_DEVICE_PTRS);
+   *F_hs receives the hard-sphere free energy (x dV); dF[s] receives dF_hs/drho_s (x dV), written or, with
+   CDFT_ACCUMULATE, added to (Species::addToForce).  CDFT_ETA_TOO_LARGE as cdft_fmt_eval. */
+int cdft_mixture_fmt_eval(cdft_mixture mix, const double* const* rho, double* const* dF, double* F_hs, int flags);
+
 /* Copies the device-generated real-space FMT weights (11*N) or MF kernel (N). */
 int cdft_system_get_weights(cdft_system sys, double* w11);
 int cdft_system_get_mf_kernel(cdft_system sys, double* w);

@@ -232,6 +232,11 @@ class FMT {
   mutable int sys_aof_ = -1;
   mutable const Species* sys_species_ = nullptr;
   cdft_system system_for(const Species& s) const;
+  // several FMT species: a hard-sphere mixture (cdft_mixture), rebuilt when the species or their diameters change
+  mutable cdft_mixture mix_ = nullptr;
+  mutable std::vector<const FMT_Species*> mix_species_;
+  mutable std::vector<double> mix_hsd_;
+  cdft_mixture mixture_for(const std::vector<FMT_Species*>& species) const;
 };
 
 class Rosenfeld : public FMT {

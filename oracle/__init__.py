@@ -46,6 +46,8 @@ def lib():
         L.orc_fcoef.argtypes = [_i, _d, _p]
         L.orc_fmt_eval.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _p, _p, _p, _p, _p]
         L.orc_fmt_eval.restype = _i
+        L.orc_fmt_eval_mix.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _i, _p, _p, _p, _p, _p]
+        L.orc_fmt_eval_mix.restype = _i
         L.orc_fmt_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _d, _p]
         L.orc_fmt_create.restype = _p
         L.orc_fmt_run.argtypes = [_p, _p, _p, _p]
@@ -201,6 +203,27 @@ def fmt_eval(model, rho, dx, hsd, weights=None, A=1.0, B=0.0, want_phi=False):
     if rc != 0:
         raise ValueError("bad arguments")
     return (F.value, dF, ph) if want_phi else (F.value, dF)
+
+
+def fmt_eval_mix(model, rho, dx, hsd, weights=None, A=1.0, B=0.0):
+    """F and dF/drho_s of a hard-sphere mixture: rho (S, Nx, Ny, Nz), hsd (S,), weights (S, 11, Nx, Ny, Nz) or None.
+    Returns (F, dF) with dF of rho's shape."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    S, Nx, Ny, Nz = rho.shape
+    hsd = np.ascontiguousarray(hsd, dtype=np.float64)
+    if hsd.shape != (S,):
+        raise ValueError("one diameter per species")
+    dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+    dF = np.zeros_like(rho)
+    F = _d()
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    rc = lib().orc_fmt_eval_mix(model, A, B, Nx, Ny, Nz, dxs[0], dxs[1], dxs[2], S, _ptr(hsd), _ptr(w), _ptr(rho),
+                                _ptr(dF), ctypes.byref(F))
+    if rc == 1:
+        raise EtaTooLarge("eta too large")
+    if rc != 0:
+        raise ValueError("bad arguments")
+    return F.value, dF
 
 
 class FMT:

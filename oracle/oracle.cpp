@@ -1140,6 +1140,87 @@ int fmt_eval(const FmtState& s, const double* rho, double* dF, double* Fout, dou
   return 0;
 }
 
+// Hard-sphere mixture: F and dF/drho_s of S FMT species on one lattice, FMT::calculateFreeEnergyAndDerivatives
+// over several species (src/FMT.cpp:363-427).  The measures at a site sum the species' weighted densities with
+// each species' diameter (FMT::getWeightedDensities, src/FMT.cpp:78-111: s0 += A1/d^2, s1 += A1/d, s2 += A1,
+// v1 += A_v/d, v2 += A_v), Phi and its derivatives are taken once per site, and each species collapses them with
+// its own diameter (FMT_Species::set_fundamental_measure_derivatives, include/Species.h:310-329) before its own
+// back-convolution (FMT_Species::Accumulate_dPhi, include/Species.h:172-178).  wk: S x 11 weight spectra (x 1/N);
+// dF (S x N) is accumulated into.
+int fmt_eval_mix(const Grid& g, const Model& model, double dV, int S, const double* hsd,
+                 const std::vector<std::vector<cd>>& wk, const double* rho, double* dF, double* Fout) {
+  const long N = g.N(), Nk = g.Nk();
+  const bool tensor = model.tensor();
+  const int m = tensor ? 11 : 5;
+  std::vector<cd> rk(Nk), tmp(Nk);
+  std::vector<double> n((size_t)S * 11 * N, 0.0);
+  for (int sp = 0; sp < S; ++sp) {
+    r2c3d(g, rho + sp * N, rk.data());
+    for (int a = 0; a < m; ++a) {
+      for (long k = 0; k < Nk; ++k) tmp[k] = rk[k] * wk[sp][a * Nk + k];
+      c2r3d(g, tmp.data(), n.data() + ((size_t)sp * 11 + a) * N);
+    }
+  }
+  std::vector<double> dphi((size_t)S * 11 * N, 0.0);
+  int bad = 0;
+  std::vector<NSum> part(g_threads);
+#pragma omp parallel num_threads(g_threads)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    NSum loc;
+    FM meas, D;
+    double out[11];
+#pragma omp for schedule(static)
+    for (long i = 0; i < N; ++i) {
+      meas = FM();
+      for (int sp = 0; sp < S; ++sp) {  // FMT.cpp:86-108
+        const double* ns = n.data() + (size_t)sp * 11 * N;
+        const double d = hsd[sp];
+        meas.eta += ns[i];
+        meas.s0 += ns[N + i] / (d * d);
+        meas.s1 += ns[N + i] / d;
+        meas.s2 += ns[N + i];
+        for (int j = 0; j < 3; ++j) {
+          meas.v1[j] += ns[(2 + j) * N + i] / d;
+          meas.v2[j] += ns[(2 + j) * N + i];
+          for (int k = 0; k < 3; ++k) meas.T[j][k] += tensor ? ns[kTI[j][k] * N + i] : 0.0;
+        }
+      }
+      meas.derived();
+      const double phi = model.phi(meas);
+      if (meas.eta > 0.5 && 1 - meas.eta < 0.0) {  // FMT.cpp:289-290
+#pragma omp atomic write
+        bad = 1;
+      }
+      model.dphi(meas, D);
+      for (int sp = 0; sp < S; ++sp) {
+        collapse(D, hsd[sp], tensor, out);
+        for (int a = 0; a < 11; ++a) dphi[((size_t)sp * 11 + a) * N + i] = out[a];
+      }
+      loc.add(phi);
+    }
+    part[t] = loc;
+  }
+  if (bad) return 1;
+  NSum F;
+  for (auto& p : part) { F.s += p.s; F.c += p.c; }
+  std::vector<double> out(N);
+  for (int sp = 0; sp < S; ++sp) {
+    std::vector<cd> acc(Nk, cd(0, 0));
+    for (int a = 0; a < m; ++a) {
+      r2c3d(g, dphi.data() + ((size_t)sp * 11 + a) * N, tmp.data());
+      for (long k = 0; k < Nk; ++k) acc[k] += tmp[k] * wk[sp][a * Nk + k];
+    }
+    c2r3d(g, acc.data(), out.data());
+    for (long i = 0; i < N; ++i) dF[sp * N + i] += out[i] * dV;
+  }
+  *Fout = F.sum() * dV;
+  return 0;
+}
+
 // Hessian of the FMT part applied to v, ACCUMULATED into out: FMT::add_second_derivative, src/FMT.cpp:437-567.
 // psi_b = conv(w_b, v) as measures; Lambda = d2Phi . psi per site; out += dV sum_a conv(conj(w_a), Lambda_a),
 // with the scale factors of the measures (s0: 1/d^2, s1: 1/d, v1: 1/d) and all nine tensor entries.
@@ -1662,6 +1743,21 @@ int orc_fmt_eval(int kind, double A, double B, int Nx, int Ny, int Nz, double dx
   fmt_setup(s, weights);
   std::fill(dF, dF + s.g.N(), 0.0);
   return fmt_eval(s, rho, dF, F, phi_out);
+}
+
+// Mixture form (fmt_eval_mix): S species of diameters hsd[s]; weights: S x 11 x N real-space cell weights or null
+// (generated as the reference does); rho, dF: S x N (dF overwritten).
+int orc_fmt_eval_mix(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, int S,
+                     const double* hsd, const double* weights, const double* rho, double* dF, double* F) {
+  const Grid g{Nx, Ny, Nz};
+  const Model model = make_model(kind, A, B);
+  std::vector<std::vector<cd>> wk(S);
+  for (int sp = 0; sp < S; ++sp) {
+    if (!weights && hsd[sp] < std::min(std::min(dx, dy), dz)) return -1;
+    weight_spectra(g, dx, dy, dz, hsd[sp], weights ? weights + (size_t)sp * 11 * g.N() : nullptr, wk[sp]);
+  }
+  std::fill(dF, dF + (size_t)S * g.N(), 0.0);
+  return fmt_eval_mix(g, model, dx * dy * dz, S, hsd, wk, rho, dF, F);
 }
 
 // Handle form: weight spectra built once (create), evaluations timed alone (run).

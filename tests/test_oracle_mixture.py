@@ -1,0 +1,99 @@
+"""Pins of the oracle's hard-sphere mixture (orc_fmt_eval_mix, the restatement of FMT::calculateFreeEnergyAndDerivatives
+over several species, src/FMT.cpp:78-111, 363-427, with the per-species collapse of include/Species.h:310-329)
+against what the mathematics fixes: identical species reduce to one, a uniform mixture gives the textbook bulk
+free-energy densities of the models (scaled-particle variables n0..n3) and their chemical potentials, and dF_s is
+the derivative of F with respect to each species' density."""
+import numpy as np
+import pytest
+
+import oracle
+
+
+def bulk_phi(model, rhos, ds):
+    """Bulk free-energy density of a hard-sphere mixture in the scaled-particle variables: Rosenfeld (Percus-Yevick
+    compressibility), White Bear I and RSLT (Boublik-Mansoori-Carnahan-Starling-Leland), White Bear II
+    (Hansen-Goos and Roth 2006).  Complex-valued so that complex-step derivatives give the chemical potentials."""
+    rhos = np.asarray(rhos, dtype=complex)
+    R = np.asarray(ds, dtype=float) / 2
+    n0 = rhos.sum()
+    n1 = (rhos * R).sum()
+    n2 = (4 * np.pi * R ** 2 * rhos).sum()
+    n3 = (4 * np.pi / 3 * R ** 3 * rhos).sum()
+    L = np.log(1 - n3)
+    if model == oracle.ROSENFELD:
+        return -n0 * L + n1 * n2 / (1 - n3) + n2 ** 3 / (24 * np.pi * (1 - n3) ** 2)
+    if model in (oracle.WBI, oracle.RSLT):
+        return -n0 * L + n1 * n2 / (1 - n3) + n2 ** 3 * (n3 + (1 - n3) ** 2 * L) / (36 * np.pi * n3 ** 2 * (1 - n3) ** 2)
+    p2 = (2 * n3 - n3 ** 2 + 2 * (1 - n3) * L) / n3
+    p3 = (2 * n3 - 3 * n3 ** 2 + 2 * n3 ** 3 + 2 * (1 - n3) ** 2 * L) / n3 ** 2
+    return -n0 * L + n1 * n2 * (1 + p2 / 3) / (1 - n3) + n2 ** 3 * (1 - p3 / 3) / (24 * np.pi * (1 - n3) ** 2)
+
+
+def seeded(shape, mean, seed, amp):
+    rng = np.random.default_rng(seed)
+    return mean * (1 + amp * rng.uniform(-1, 1, shape))
+
+
+@pytest.mark.parametrize("model,A,B", [(oracle.WBI, 1, -1), (oracle.ROSENFELD, 1, 0), (oracle.ESFMT, 1.3, -0.4)])
+def test_identical_species_reduce_to_one(model, A, B):
+    shape, dx, hsd = (12, 10, 14), 0.1, 0.6
+    rho = seeded(shape, 0.4, 5, 0.1)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    F1, dF1 = oracle.fmt_eval(model, rho, dx, hsd, weights=w, A=A, B=B)
+    Fm, dFm = oracle.fmt_eval_mix(model, np.stack([0.25 * rho, 0.75 * rho]), dx, [hsd, hsd],
+                                  weights=np.stack([w, w]), A=A, B=B)
+    assert abs(Fm - F1) <= 1e-14 * abs(F1)
+    for s in range(2):
+        assert np.abs(dFm[s] - dF1).max() <= 1e-13 * np.abs(dF1).max()
+
+
+@pytest.mark.parametrize("model", [oracle.ROSENFELD, oracle.RSLT, oracle.WBI, oracle.WBII])
+def test_uniform_mixture_is_the_bulk_mixture(model):
+    shape, dx = (20, 20, 20), 0.1
+    ds = [1.0, 0.6, 0.8]
+    rhos = [0.3, 0.5, 0.2]
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    rho = np.stack([np.full(shape, r) for r in rhos])
+    F, dF = oracle.fmt_eval_mix(model, rho, dx, ds, weights=w)
+    V = np.prod(shape) * dx ** 3
+    f = bulk_phi(model, rhos, ds).real
+    assert abs(F / V - f) <= 1e-12 * abs(f)
+    h = 1e-30
+    for s in range(3):
+        r = np.array(rhos, dtype=complex)
+        r[s] += 1j * h
+        mu = bulk_phi(model, r, ds).imag / h
+        assert np.abs(dF[s] / dx ** 3 - mu).max() <= 1e-12 * abs(mu)
+
+
+def test_a_second_diameter_changes_the_answer():
+    """A mixture of diameters 1.0 and 0.6 is not the one-diameter system: the bulk check above would also pass for a
+    restatement that used one diameter for all species only if the formula did; this one pins that it does not."""
+    shape, dx = (20, 20, 20), 0.1
+    w1, w2 = (oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in (1.0, 0.6))
+    rho = np.stack([np.full(shape, 0.3), np.full(shape, 0.5)])
+    F, _ = oracle.fmt_eval_mix(oracle.WBI, rho, dx, [1.0, 0.6], weights=np.stack([w1, w2]))
+    V = np.prod(shape) * dx ** 3
+    assert abs(F / V - bulk_phi(oracle.WBI, [0.3, 0.5], [1.0, 0.6]).real) <= 1e-12 * abs(F / V)
+    assert abs(F / V - bulk_phi(oracle.WBI, [0.3, 0.5], [1.0, 1.0]).real) > 1e-2 * abs(F / V)
+
+
+@pytest.mark.parametrize("model,A,B", [(oracle.WBI, 1, -1), (oracle.ESFMT, 1.3, -0.4), (oracle.RSLT, 1, 0)])
+def test_mixture_central_differences(model, A, B):
+    shape, dx = (10, 8, 12), 0.1
+    ds = [0.6, 0.45]
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    rho = np.stack([seeded(shape, 0.35, 11, 0.2), seeded(shape, 0.5, 12, 0.2)])
+    _, dF = oracle.fmt_eval_mix(model, rho, dx, ds, weights=w, A=A, B=B)
+    rng = np.random.default_rng(3)
+    h = 1e-5
+    for s in range(2):
+        for _ in range(3):
+            i = tuple(int(rng.integers(0, n)) for n in shape)
+            rp, rm = rho.copy(), rho.copy()
+            rp[(s,) + i] += h
+            rm[(s,) + i] -= h
+            Fp, _ = oracle.fmt_eval_mix(model, rp, dx, ds, weights=w, A=A, B=B)
+            Fm, _ = oracle.fmt_eval_mix(model, rm, dx, ds, weights=w, A=A, B=B)
+            fd = (Fp - Fm) / (2 * h)
+            assert abs(fd - dF[(s,) + i]) <= 1e-6 * abs(dF[(s,) + i]) + 1e-12
