@@ -25,6 +25,7 @@
 //   points <a1> <a2> <cached> <F>  points-file Interaction (60 Fibonacci directions written to sphere_points.dat):
 //                           a_vdw of a first object (kernel generated, cache written) and of a second one (cache
 //                           read: 1), F of a 16^3 DFT with the cached kernel
+//   mix <F> <sum dF_a> <sum dF_b>  FMT over two FMT_Species (d = 1 and 0.6, White Bear I, 16 x 16 x 24, dx = 0.1)
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -274,5 +275,20 @@ int main() {
   dft11.addInteraction(&pts2);
   const double F11 = dft11.calculateFreeEnergyAndDerivatives(false);
   std::printf("points %.17g %.17g %d %.17g\n", a1, a2, pts2.weights_from_cache() ? 1 : 0, F11);
+
+  // a hard-sphere mixture: FMT::calculateFreeEnergyAndDerivatives over two FMT_Species of diameters 1 and 0.6
+  double Lm[3] = {1.6, 1.6, 2.4};
+  Density dma(0.1, Lm), dmb(0.1, Lm);
+  for (long i = 0; i < dma.Ntot(); ++i) {
+    dma.set(i, 0.3 * (1.0 + 0.1 * (2.0 * splitmix_u(61, i) - 1.0)));
+    dmb.set(i, 0.5 * (1.0 + 0.1 * (2.0 * splitmix_u(62, i) - 1.0)));
+  }
+  FMT_Species sma(dma, 1.0, 0.0), smb(dmb, 0.6, 0.0);
+  WhiteBearI fmtm;
+  std::vector<Species*> both{&sma, &smb};
+  sma.zeroForce();
+  smb.zeroForce();
+  const double Fm = fmtm.calculateFreeEnergyAndDerivatives(both);
+  std::printf("mix %.17g %.17g %.17g\n", Fm, sma.getDF().accu(), smb.getDF().accu());
   return 0;
 }

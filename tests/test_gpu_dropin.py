@@ -274,3 +274,16 @@ def test_coexistence_through_cpp_api(demo, cdft):
     # the loop ends when Tc moves by at most 1e-8 relative, xc holds the diameter of the step before
     assert abs(xc - 0.13044 * (6 / np.pi) / d ** 3) <= 1e-7 * xc
     assert abs(Tc + 0.090082 * 2 * cdft.potential_vdw(Tc) / d ** 3 * Tc) <= 1e-6 * Tc
+
+
+def test_mixture_through_cpp_api(demo):
+    """FMT::calculateFreeEnergyAndDerivatives with two FMT_Species (diameters 1 and 0.6) runs the device mixture;
+    F and each species' force sum against the oracle's mixture on accurate weights."""
+    shape, dx, ds = (16, 16, 24), 0.1, [1.0, 0.6]
+    rho = np.stack([seeded(shape, 0.3, 61, 0.1), seeded(shape, 0.5, 62, 0.1)])
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    F, dF = oracle.fmt_eval_mix(oracle.WBI, rho, dx, ds, weights=w)
+    got = demo["mix"]
+    assert rel(got[0], F) <= 1e-11
+    for s in range(2):
+        assert abs(got[1 + s] - dF[s].sum()) <= 1e-11 * np.abs(dF[s]).sum()
