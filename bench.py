@@ -338,6 +338,32 @@ def cpu_fmt(rho, shape, dx, threads, fft_threads, reps):
     return float(np.median(ts)), ts, F, dF
 
 
+def streaming_probe(nbytes=1 << 30, reps=10):
+    """Achievable HBM streaming rates of this GPU, beside the published 8 TB/s peak: a device-to-device copy (read
+    + write), a fill (write only) and a sum (read only) of nbytes through torch, timed with events on its stream."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    out = {"bytes": 8 * n, "tool": "torch copy_ / fill_ / sum, 1 GiB float64, 10 timed after 3 untimed"}
+    for name, fn, moved in (("copy", lambda: b.copy_(a), 16 * n), ("write", lambda: b.fill_(1.5), 8 * n),
+                            ("read", lambda: a.sum(), 8 * n)):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name + "_GBs"] = moved / (ms * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_fire_estimate(cdft, info, threads, steps_gpu, steps=3):
     """Config 2 on the CPU: seconds per FIRE2 step of the oracle on the same slit pore (White Bear I + LJ QF
     mean field, walls, mu of rho_b = 0.7), times the GPU's step count to 1e-8: a labelled estimate."""
@@ -507,6 +533,8 @@ def main():
             traffic_note = (f"profiles/traffic_fmt_256.json was measured on other kernel sources "
                             f"({tj.get('sources_sha16')}, current {sources_sha16()}): not used")
 
+    print("# streaming probe", flush=True)
+    probe = streaming_probe()
     result = {
         "metric": "FMT F+dF evals/s @256^3 (FP64 vs mixed); FIRE time-to-tol, warm vs cold",
         "value": 1000.0 / t64,
@@ -526,7 +554,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
                      "traffic": traffic, "traffic_source": traffic_note,
                      "scope": "one F+dF evaluation (all kernels), algorithmic bytes B(8,11)",
-                     "algorithmic_bytes": B64, "dominant_kernel": dominant},
+                     "algorithmic_bytes": B64, "dominant_kernel": dominant,
+                     "streaming_probe": probe,
+                     "frac_of_copy_rate": ach / probe["copy_GBs"],
+                     "traffic_rate_GBs": (traffic / (t64 * 1e-3) / 1e9) if traffic else None},
         "mixed": {"evals_per_s": 1000.0 / tmx, "ms_per_step": tmx, "speedup_vs_fp64": t64 / tmx,
                   "F_rel_err_vs_fp64": abs(Fmx - F64) / abs(F64),
                   "dF_maxrel_err_vs_fp64": float(np.abs(dFmx - dF64).max() / np.abs(dF64).max()),
