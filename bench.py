@@ -338,30 +338,37 @@ def cpu_fmt(rho, shape, dx, threads, fft_threads, reps):
     return float(np.median(ts)), ts, F, dF
 
 
-def streaming_probe(nbytes=1 << 30, reps=10):
+_PROBE = r"""
+import json, torch
+n = (1 << 30) // 8
+a = torch.ones(n, dtype=torch.float64, device="cuda")
+b = torch.empty_like(a)
+out = {"bytes": 8 * n, "tool": "torch copy_ / fill_ / sum, 1 GiB float64, 10 timed after 3 untimed"}
+for name, fn, moved in (("copy", lambda: b.copy_(a), 16 * n), ("write", lambda: b.fill_(1.5), 8 * n),
+                        ("read", lambda: a.sum(), 8 * n)):
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    out[name + "_GBs"] = moved / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
+print(json.dumps(out))
+"""
+
+
+def streaming_probe():
     """Achievable HBM streaming rates of this GPU, beside the published 8 TB/s peak: a device-to-device copy (read
-    + write), a fill (write only) and a sum (read only) of nbytes through torch, timed with events on its stream."""
-    import torch
-    n = nbytes // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    out = {"bytes": 8 * n, "tool": "torch copy_ / fill_ / sum, 1 GiB float64, 10 timed after 3 untimed"}
-    for name, fn, moved in (("copy", lambda: b.copy_(a), 16 * n), ("write", lambda: b.fill_(1.5), 8 * n),
-                            ("read", lambda: a.sum(), 8 * n)):
-        for _ in range(3):
-            fn()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            fn()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        out[name + "_GBs"] = moved / (ms * 1e-3) / 1e9
-    del a, b
-    torch.cuda.empty_cache()
-    return out
+    + write), a fill (write only) and a sum (read only) of 1 GiB through torch, timed with events on its stream.  A
+    child process runs it: torch brings its own HIP runtime, which cannot share a process with the library's."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", _PROBE], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": (r.stderr or "").strip().splitlines()[-1:]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def cpu_fire_estimate(cdft, info, threads, steps_gpu, steps=3):
@@ -556,7 +563,7 @@ def main():
                      "scope": "one F+dF evaluation (all kernels), algorithmic bytes B(8,11)",
                      "algorithmic_bytes": B64, "dominant_kernel": dominant,
                      "streaming_probe": probe,
-                     "frac_of_copy_rate": ach / probe["copy_GBs"],
+                     "frac_of_copy_rate": (ach / probe["copy_GBs"]) if "copy_GBs" in probe else None,
                      "traffic_rate_GBs": (traffic / (t64 * 1e-3) / 1e9) if traffic else None},
         "mixed": {"evals_per_s": 1000.0 / tmx, "ms_per_step": tmx, "speedup_vs_fp64": t64 / tmx,
                   "F_rel_err_vs_fp64": abs(Fmx - F64) / abs(F64),

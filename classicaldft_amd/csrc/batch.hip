@@ -401,10 +401,17 @@ struct cdft_fire_batch_s {
     HIPCHK(hipGetLastError());
   }
 
-  // One fireMinimizer2::step of every active member, enqueued on the system's stream.
+  // One fireMinimizer2::step of every active member, enqueued on the system's stream.  The evaluation kernels skip
+  // the members k_fb_begin leaves inactive (FireCtl::active, read on the device by each block).
   void enqueue_step() {
     hipStream_t st = sys->stream;
     FireCtl* c = ctl.as<FireCtl>();
+    static_assert(sizeof(FireCtl) % sizeof(int) == 0, "FireCtl is read as an int array");
+    sys->skip = Skip{&c->active, long(sizeof(FireCtl) / sizeof(int))};
+    struct Reset {
+      cdft_system_s* s;
+      ~Reset() { s->skip = Skip{}; }
+    } reset{sys};
     hipLaunchKernelGGL(k_fb_begin, dim3(div_up(nb, 64)), dim3(64), 0, st, c, nb, cp);
     hipLaunchKernelGGL(k_fb_save, vgrid(), dim3(kThreads), 0, st, x.as<double>(), v.as<double>(), dF.as<double>(),
                        raw.as<double>(), xr.as<double>(), vr.as<double>(), dFr.as<double>(), rawr.as<double>(), c, N);

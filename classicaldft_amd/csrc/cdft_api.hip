@@ -488,7 +488,7 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
     const PwArgs pa{s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
                     s.vmf.p ? s.vmf.as<double>() + (long)x0 * s.g.Ny * s.g.Nz : nullptr, nrows, s.g.Zp,
                     tw.mp.p, tw.n.p, pp, s.partials.as<double2>() + blk_off, s.flag.as<int>(),
-                    Batch{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr}, s.trace, MODEL, s.nb, s.stream,
+                    Batch{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr, s.skip}, s.trace, MODEL, s.nb, s.stream,
                     ms};
     const long nblk = launch_pw3<T>(pa, M);
     REQUIRE(nblk != -2, "launch of the pointwise kernel failed");
@@ -504,7 +504,7 @@ long launch_pointwise(cdft_system_s& s, int nconv, int x0 = 0, int x1 = -1, long
       if (s.pw_rows > 0) RB = std::min(RB, s.pw_rows);
       if (RB < 1) return;
       nblk = div_up(nrows, RB);
-      const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr};
+      const Batch bt{s.Nkp, s.nconv * s.Nkp, s.N, s.pslots, nullptr, s.skip};
       double* vmf = s.vmf.p ? s.vmf.as<double>() + (long)x0 * s.g.Ny * s.g.Nz : nullptr;
       hipLaunchKernelGGL((k_z_pointwise2<T, MODEL, Mc>), dim3(nblk, s.nb), dim3(nconv * RB * RF::P),
                          size_t(RB) * per, s.stream, s.B.as<cplx<T>>() + (long)x0 * s.g.Ny * s.g.Zp, s.Nkp, s.m, nconv,
@@ -624,7 +624,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         constexpr int Lc = decltype(lc)::value;
         with_nb<T, Lc>(s.nb_half_fwd, [&](auto nbc) {
           constexpr int NB = decltype(nbc)::value;
-          const Batch bt{sA, sB, s.N, 0, nullptr};
+          const Batch bt{sA, sB, s.N, 0, nullptr, s.skip};
           hipLaunchKernelGGL((k_conv_fwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB), nb), dim3(NB * F2<Lc>::P), 0,
                              s.stream, A, B, Nkp, s.W.as<T>(), nconv, s.odd_mask, cg, tw.x.template as<cplx<T>>(), bt,
                              (int)msplit);
@@ -686,7 +686,7 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
           constexpr int Lc = decltype(lc)::value;
           with_nb<T, Lc>(s.nb_half_bwd, [&](auto nbc) {
             constexpr int NB = decltype(nbc)::value;
-            const Batch bt{sA, sB, s.N, 0, nullptr};
+            const Batch bt{sA, sB, s.N, 0, nullptr, s.skip};
             hipLaunchKernelGGL((k_conv_bwd2<T, Lc, NB>), dim3(div_up(cg.ncols, NB), nb), dim3(NB * F2<Lc>::P), 0,
                                s.stream, B, A, Nkp, s.W.as<T>(), s.m, s.odd_mask, cg, tw.x.template as<cplx<T>>(),
                                bt);
@@ -727,7 +727,8 @@ void eval_device(cdft_system_s& s, const double* rho, double* dF, bool with_dft,
         constexpr int Mc = decltype(mc)::value;
         constexpr int RB = row_rb<T, Mc>(32768), P = RowF<T, Mc>::P, SLOT = RowF<T, Mc>::SLOT;
         nblk = div_up(s.nrows, RB);
-        const Batch bt{sA, sB, s.N, 4 * s.pslots, with_dft && s.nb > 1 && !(s.d.fixed_mass > 0) ? s.mus.as<double>() : nullptr};
+        const Batch bt{sA, sB, s.N, 4 * s.pslots,
+                       with_dft && s.nb > 1 && !(s.d.fixed_mass > 0) ? s.mus.as<double>() : nullptr, s.skip};
         hipLaunchKernelGGL((k_z_final2<T, Mc>), dim3(nblk, nb), dim3(RB * P), RB * SLOT * sizeof(cplx<T>), s.stream, A,
                            rho, s.vext.as<double>(), s.vmf.as<double>(), dF, unsigned(s.N * 8), s.nrows, s.g.Zp,
                            tw.m.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, fp,

@@ -47,12 +47,21 @@ __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned voff, 
   }
 }
 
+// Members of a batch that a FIRE batch has stopped (converged, or unable to go on): every block of such a member
+// returns at once, so a step costs what its running members need.  flag null: every member runs.
+struct Skip {
+  const int* flag = nullptr;  // member m runs when flag[m * stride] != 0
+  long stride = 0;
+  __device__ __forceinline__ bool off(unsigned m) const { return flag && !flag[m * stride]; }
+};
+
 // Member strides of a batch of same-shape systems (member = blockIdx.y of the
 // kernels below): one complex array, the block of per-weight complex arrays, a
 // real array, the partial-sum slots; per-member chemical potentials.
 struct Batch {
   long sa, sb, sr, sp;
   const double* mu;  // null: use the scalar of the call
+  Skip sk{};
 };
 
 // Logical tile of this block such that consecutive tiles run on one XCD.  The hardware hands the blocks of a
@@ -83,12 +92,14 @@ struct ColGeo {
   // kx <= Nx/2 are stored, (Nx/2+1) x Ny x Zp values per array, wstride apart.  A column reads its rows kx and
   // Nx - kx from the same stored row, so the fold also halves the distinct lines each block fetches.
   unsigned wq, wstride, pxm;
+  Skip sk;  // stopped batch members (member = blockIdx.z of k_cols2)
 };
 
 // In-place transform of ncols columns per array.  grid = (ceil(ncols/NB), narr, members), block = NB * P.
 template <typename T, int L, int NB, bool FWD>
 __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long arr_stride, ColGeo cg,
                                                const cplx<T>* __restrict__ tw, long mstride) {
+  if (cg.sk.off(blockIdx.z)) return;
   data += blockIdx.z * mstride;  // batch member
   using F = F2<L>;
   constexpr int R1 = F::R1, R2 = F::R2, P = F::P, Q1 = F::Q1, Q2 = F::Q2;
@@ -126,6 +137,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
                                                    long arr_stride, const T* __restrict__ W, int nconv,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw,
                                                    Batch bt, int zero_mean) {
+  if (bt.sk.off(blockIdx.y)) return;
   A += blockIdx.y * bt.sa;
   B += blockIdx.y * bt.sb;
   using F = F2<L>;
@@ -205,6 +217,7 @@ __global__ __launch_bounds__(256) void k_conv_bwd2(const cplx<T>* __restrict__ D
                                                    long arr_stride, const T* __restrict__ W, int m,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw,
                                                    Batch bt) {
+  if (bt.sk.off(blockIdx.y)) return;
   D += blockIdx.y * bt.sb;
   Acc += blockIdx.y * bt.sa;
   using F = F2<L>;

@@ -53,6 +53,7 @@ __global__ __launch_bounds__((Pw3Geo<T, M, NA + (MF ? 1 : 0)>::NT)) void k_z_pw3
   using RF = RowF<T, M>;
   constexpr int RB = G::RB, NT = G::NT, SLOT = RF::SLOT;
   constexpr bool TRACE = MODEL >= M_ESFMT && NA == 10;
+  if (bt.sk.off(blockIdx.y)) return;
   B += blockIdx.y * bt.sb;
   if (ms) ms += blockIdx.y * kMS;
   if (MF) vmf += blockIdx.y * bt.sr;

@@ -208,7 +208,9 @@ template <typename T, typename TI, int M>
 __global__ __launch_bounds__(256) void k_rows_r2c2(const TI* __restrict__ in, long in_arr_stride,
                                                    cplx<T>* __restrict__ out, long out_arr_stride, long nrows, int Zp, const cplx<T>* __restrict__ twP,
                                                    const cplx<T>* __restrict__ twN, int RB,
-                                                   double2* __restrict__ psum = nullptr, long psp = 0) {
+                                                   double2* __restrict__ psum = nullptr, long psp = 0,
+                                                   Skip sk = Skip{}) {
+  if (sk.off(blockIdx.y)) return;
   using RF = RowF<T, M>;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   const int seq = threadIdx.x / RF::P, t = threadIdx.x % RF::P;
@@ -277,6 +279,7 @@ __global__ __launch_bounds__(256, 3) void k_z_pointwise2(cplx<T>* __restrict__ B
                                                       const cplx<T>* __restrict__ twN, int RB, PhiParams pp,
                                                       double2* __restrict__ partials, int* __restrict__ flag,
                                                       Batch bt, int trace, const double* __restrict__ ms) {
+  if (bt.sk.off(blockIdx.y)) return;
   B += blockIdx.y * bt.sb;
   if (ms) ms += blockIdx.y * kMS;
   if (vmf) vmf += blockIdx.y * bt.sr;
@@ -570,6 +573,7 @@ __global__ __launch_bounds__(256) void k_z_final2(const cplx<T>* __restrict__ Ac
                                                   double* __restrict__ dF, unsigned real_bytes, long nrows, int Zp,
                                                   const cplx<T>* __restrict__ twM, const cplx<T>* __restrict__ twN,
                                                   int RB, FinalParams fp, double2* __restrict__ partials, Batch bt) {
+  if (bt.sk.off(blockIdx.y)) return;
   const long mb = blockIdx.y;  // batch member
   Acc += mb * bt.sa;
   rho += mb * bt.sr;

@@ -207,6 +207,7 @@ struct cdft_system_s {
   int pw_rows = 0;       // rows per block of the register pointwise kernel (0: automatic)
   bool pw3 = true;       // specialised pointwise kernel (pw3.hpp) where an instance covers the system
   bool trace_ok = true;  // the trace-reduced weight set may be used (create_system's allow_trace)
+  Skip skip;             // stopped members of a FIRE batch, set while its steps are enqueued (kernels2.hpp)
   // mixed path: the means of the weighted densities and of dPhi are carried in FP64 and the FP32 arrays hold the
   // deviations from them (msplit: kMS doubles per member; w0: the FP64 weight spectra at k = 0, nconv values)
   bool mean_split = false;
@@ -298,7 +299,7 @@ inline void launch_rows_r2c(const cdft_system_s& s, const TI* in, long in_stride
       if (nblk_out) *nblk_out = div_up(s.nrows, RB);
       hipLaunchKernelGGL((k_rows_r2c2<T, TI, Mc>), dim3(div_up(s.nrows, RB), narr), dim3(RB * P),
                          RB * SLOT * sizeof(cplx<T>), s.stream, in, in_stride, out, out_stride, s.nrows, s.g.Zp,
-                         tw.mp.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, psum, psp);
+                         tw.mp.template as<cplx<T>>(), tw.n.template as<cplx<T>>(), RB, psum, psp, s.skip);
     });
     HIPCHK(hipGetLastError());
     return;
@@ -387,7 +388,8 @@ inline bool launch_cols_reg(const cdft_system_s& s, cplx<T>* data, long stride, 
   if (!reg_fft_ok<T>(s, L)) return false;
   const Tables<T>& tw = tables<T>(s);
   const cplx<T>* t = (axis == 0 ? tw.x : tw.y).template as<cplx<T>>();
-  const ColGeo cg = col_geo<T>(s.g, axis);
+  ColGeo cg = col_geo<T>(s.g, axis);
+  cg.sk = s.skip;
   return with_reg_len(L, [&](auto lc) {
     constexpr int Lc = decltype(lc)::value;
     with_nb<T, Lc>(s.nb_half_cols, [&](auto nbc) {
@@ -410,6 +412,7 @@ inline void launch_cols_slab(const cdft_system_s& s, cplx<T>* data, long stride,
   const Tables<T>& tw = tables<T>(s);
   const cplx<T>* t = tw.y.template as<cplx<T>>();
   ColGeo cg = col_geo<T>(s.g, 1);
+  cg.sk = s.skip;
   cg.ncols = unsigned((x1 - x0) * s.g.Zp);
   cg.bytes = unsigned((long)(x1 - x0) * s.g.Ny * s.g.Zp * (long)sizeof(cplx<T>));
   cplx<T>* base = data + (long)x0 * s.g.Ny * s.g.Zp;
