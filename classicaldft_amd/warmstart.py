@@ -167,11 +167,14 @@ class WarmStart:
         dV = C.DeviceArray.from_host(V, np.float32)
         dmu = C.DeviceArray.from_host(np.asarray(mus, np.float32), np.float32)
         out = np.empty_like(V)
+        bufs = {}  # one output buffer per batch size, reused
         for b0 in range(0, n, batch):
             nb = min(batch, n - b0)
-            o = C.DeviceArray((nb,) + V.shape[1:], np.float32)
-            self.net.forward(dV, dmu, np.arange(b0, b0 + nb), o)
-            out[b0:b0 + nb] = o.download()
+            if nb not in bufs:
+                bufs[nb] = C.DeviceArray((nb,) + V.shape[1:], np.float32)
+            self.net.forward(dV, dmu, np.arange(b0, b0 + nb), bufs[nb])
+            out[b0:b0 + nb] = bufs[nb].download()
+        for o in bufs.values():
             o.free()
         dV.free()
         dmu.free()
@@ -179,10 +182,14 @@ class WarmStart:
 
     def predict(self, V, mus, rho_b):
         """Warm-start densities rho0 = rho_b exp(u - beta V_ext) (float64); rho_b: bulk densities at mu
-        (``bulk_density``)."""
-        u = self.predict_u(V, mus).astype(np.float64)
-        rb = np.asarray(rho_b, np.float64)[:, None, None, None]
-        return np.minimum(rb * np.exp(u - V), 7.0)  # far above any fluid density
+        (``bulk_density``).  Formed in place in float32 (a starting point for FIRE; its rounding is far below the
+        network's error) and widened once."""
+        u = self.predict_u(V, mus)
+        np.subtract(u, V, out=u, casting="unsafe")
+        np.exp(u, out=u)
+        u *= np.asarray(rho_b, np.float32)[:, None, None, None]
+        np.minimum(u, np.float32(7.0), out=u)  # far above any fluid density
+        return u.astype(np.float64)
 
 
 def train(net, V, mus, U, train_idx, val_idx, epochs=24, batch=8, lr=2e-3, lr_min=2e-5, seed=7, budget_s=None):
