@@ -1,7 +1,7 @@
 """GPU parity of the FMT / DFT hot path against the CPU oracle (through the C ABI).
 
 Tolerances: the FP64 path must match the oracle to 1e-11 relative in F and in
-max-norm dF (BASELINE north star).  The weights are an input of the
+max-norm dF.  The weights are an input of the
 comparison: the reference's double-precision weight generator carries
 roundoff that alone moves F by ~1e-10 at dx = 0.1 (see
 tests/test_oracle_weights.py), so the FP64 parity cases hand both sides the
@@ -231,7 +231,7 @@ def test_accumulate_flag(cdft):
     assert np.abs(d2 - 3.0 - d1).max() < 1e-14 * np.abs(d1).max() + 1e-15
 
 
-# Mixed precision against the FP64 oracle: the north-star bar |F error| <= 1e-7 relative; max-norm dF within 1e-6.
+# Mixed precision against the FP64 oracle: |F error| <= 1e-7 relative; max-norm dF within 1e-6.
 MIXED_F_TOL, MIXED_DF_TOL = 1e-7, 1e-6
 
 
