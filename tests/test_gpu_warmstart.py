@@ -1,7 +1,7 @@
 """The trained warm start against the uniform start it replaces (held-out systems, FP64 FIRE to 1e-8).
 
 FIRE refines both starts and must reach the same minimum; the warm start must
-need fewer steps.  A network that predicts nonsense must be caught by the judge
+need fewer steps and, with the network and the judgement counted, no more than 1.25x the wall time.  A network that predicts nonsense must be caught by the judge
 (grand potential worse than the uniform density's) and give exactly the
 uniform-start run.
 """
@@ -28,6 +28,9 @@ def test_trained_warm_start_beats_uniform_start(ws_mod):
     assert r["same_minimum"] == 8
     assert r["warm"]["judged_uniform"] <= 1
     assert r["warm"]["mean_steps"] < r["cold"]["mean_steps"], r
+    # wall time, network and judgement included: the step saving is about what they cost (a batch stops evaluating
+    # a member once it has converged), so the warm run must not be slower than the cold one beyond that overhead
+    assert r["warm"]["seconds"] <= 1.25 * r["cold"]["seconds"], r
 
 
 def test_judge_falls_back_on_a_bad_network(ws_mod):
