@@ -2,7 +2,7 @@
 //
 // Built together with oracle.cpp under AddressSanitizer and UndefinedBehaviorSanitizer by
 // tests/test_sanitizers.py (g++ -fsanitize=address,undefined); it walks every major entry point of the oracle on
-// small grids (3D FFTs, weight generation, the five FMT models, the whole DFT with mean field, external field and
+// small grids (3D FFTs, weight generation, the five FMT models, a two-species mixture, the whole DFT with mean field, external field and
 // constraints, the Hessian, AO, FIRE2, DDFT, open DDFT, log-det and eigenvalue objectives) so that an out-of-bounds
 // access, a use after free or undefined arithmetic anywhere on those paths stops the run.  Prints one line per
 // stage and "sanitize: ok" at the end.
@@ -22,6 +22,8 @@ void orc_c2r3d(int Nx, int Ny, int Nz, const double* in, double* out);
 int orc_fmt_weights(int Nx, int Ny, int Nz, double dx, double dy, double dz, double hsd, double* w, int mode);
 int orc_fmt_eval(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, double hsd,
                  const double* weights, const double* rho, double* dF, double* F, double* phi_out);
+int orc_fmt_eval_mix(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, int S,
+                     const double* hsd, const double* weights, const double* rho, double* dF, double* F);
 orc_dft* orc_dft_create(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz,
                         double hsd, int has_fmt, int ideal, int has_mf, double sigma, double eps, double rcut, double kT,
                         double mu, const double* vext);
@@ -94,6 +96,20 @@ int main() {
     expect(std::isfinite(F), "fmt F finite");
   }
   std::printf("fmt: five models\n");
+
+  {  // a two-species mixture (weights generated per species)
+    const double hsd2[2] = {0.6, 0.45};
+    std::vector<double> r2(2 * (size_t)N), d2(2 * (size_t)N);
+    for (int i = 0; i < N; ++i) {
+      r2[i] = 0.6 * rho[i];
+      r2[N + i] = 0.4 * rho[i];
+    }
+    double F2 = 0.0;
+    expect(orc_fmt_eval_mix(3, 1.0, -1.0, Nx, Ny, Nz, dx, dx, dx, 2, hsd2, nullptr, r2.data(), d2.data(), &F2) == 0,
+           "mixture eval");
+    expect(std::isfinite(F2), "mixture F finite");
+    std::printf("fmt: two-species mixture\n");
+  }
 
   std::vector<double> vext(N);
   for (int i = 0; i < N; ++i) vext[i] = 0.3 * std::sin(0.1 * i);
