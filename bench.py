@@ -162,6 +162,40 @@ def fire_ladder_time_to_tol(cdft, n=128, tol=1e-8, ladder_tol=1e-5, max_steps=20
     return info
 
 
+def fire_graph_time_to_tol(cdft, n=128, tol=1e-8, ladder_tol=None, rounds=25, max_steps=20000):
+    """The same slit pore by FIRE2 with device-side control (a FireBatch of one system): `rounds` steps per hipGraph
+    launch, the host checking the status between launches; with ladder_tol, mixed precision to ladder_tol first,
+    then FP64 to tol.  Wall time of the whole run (batch creation and its first evaluation included)."""
+    s, rho0, info = slit_pore(cdft, n=n)
+    s.close()
+    z = np.arange(n) * info["dx"]
+    vz = 10.0 * (np.exp(-z / 0.2) + np.exp(-(n * info["dx"] - z) / 0.2))  # the walls of slit_pore
+    vext = np.ascontiguousarray(np.broadcast_to(vz, (n, n, n)))
+    t0 = time.perf_counter()
+    fb = cdft.FireBatch(tuple(info["grid"]), info["dx"], [info["mu"]], rho0[None], vexts=vext[None],
+                        hsd=info["hsd"], precision=cdft.MIXED if ladder_tol else cdft.FP64, ideal=True,
+                        mean_field=True, kT=info["kT"])
+    phases = ([(cdft.MIXED, ladder_tol)] if ladder_tol else []) + [(cdft.FP64, tol)]
+    split = {}
+    for prec, t in phases:
+        fb.set_precision(prec)
+        while True:
+            fb.run(rounds, t)
+            reps, st, running = fb.status()
+            if running == 0 or reps[0]["steps"] >= max_steps:
+                break
+        split["mixed" if prec == cdft.MIXED else "fp64"] = (time.perf_counter() - t0, reps[0]["steps"])
+    dt = time.perf_counter() - t0
+    rep = reps[0]
+    fb.close()
+    info.update({"seconds": dt, "steps": rep["steps"], "F": rep["F"], "fmax_raw": rep["fmax_raw"],
+                 "converged": bool(rep["fmax_raw"] <= tol), "status": int(st[0]), "steps_per_launch": rounds,
+                 "control": "device (FireBatch of one system, hipGraph of the steps)"})
+    if ladder_tol:
+        info.update({"ladder_tol": ladder_tol, "seconds_mixed": split["mixed"][0], "steps_mixed": split["mixed"][1]})
+    return info
+
+
 def ddft_fcc_seed(cdft, steps, n=256, dx=0.04, kT=0.8, rho_l=0.94, a=1.55, alpha=50.0, dt=1e-4, precision=0):
     """DDFT from a liquid with a 4x4x4 fcc seed of Gaussians (alpha/pi)^{3/2} exp(-alpha |r - R|^2) centred in the
     box, dt pinned at 1e-4; ms per step and the relative mass drift."""
@@ -579,6 +613,10 @@ def main():
         result["fire_time_to_tol"] = {"slit_pore": fire_time_to_tol(cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
                                       "slit_pore_mixed_then_fp64": fire_ladder_time_to_tol(
                                           cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
+                                      "slit_pore_device_control": fire_graph_time_to_tol(
+                                          cdft, n=args.fire_grid, max_steps=args.fire_max_steps),
+                                      "slit_pore_device_control_mixed_then_fp64": fire_graph_time_to_tol(
+                                          cdft, n=args.fire_grid, ladder_tol=1e-5, max_steps=args.fire_max_steps),
                                       "start": "rho_bulk exp(-V_ext)", "tol": 1e-8,
                                       "criterion": "max_i |dF/drho_i| / dV (raw gradient)"}
     if args.warm_systems:
