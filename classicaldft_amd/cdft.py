@@ -48,7 +48,8 @@ class _Desc(ctypes.Structure):
 
 class _MixDesc(ctypes.Structure):
     _fields_ = [("Nx", _i), ("Ny", _i), ("Nz", _i), ("dx", _d), ("dy", _d), ("dz", _d), ("model", _i), ("A", _d),
-                ("B", _d), ("nspecies", _i), ("hsd", _p), ("fmt_weights", _p), ("weights_mode", _i)]
+                ("B", _d), ("nspecies", _i), ("hsd", _p), ("fmt_weights", _p), ("weights_mode", _i), ("ideal", _i),
+                ("mu", _p), ("vext", _p)]
 
 
 class _PotDesc(ctypes.Structure):
@@ -113,6 +114,8 @@ def lib():
         L.cdft_mixture_destroy.argtypes = [_p]
         L.cdft_mixture_species.argtypes = [_p, ctypes.POINTER(_i)]
         L.cdft_mixture_fmt_eval.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_mixture_eval.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_mixture_fire_create.argtypes = [_p, _p, _i, ctypes.POINTER(FireParams), ctypes.POINTER(_p)]
         L.cdft_system_stream.argtypes = [_p, ctypes.POINTER(_p)]
         L.cdft_system_a_vdw.argtypes = [_p, ctypes.POINTER(_d)]
         L.cdft_system_layout.argtypes = [_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]
@@ -494,7 +497,16 @@ class Fire:
             setattr(p, k, v)
         rho0 = np.ascontiguousarray(rho0, dtype=np.float64)
         h = _p()
-        _check(lib().cdft_fire_create(system.h, _ptr(rho0), 0, ctypes.byref(p), ctypes.byref(h)))
+        if isinstance(system, Mixture):  # one FIRE2 state over all species (rho0: (nspecies,) + shape)
+            S = system.nspecies
+            if rho0.size != S * system.N:
+                raise ValueError(f"expected {S} x {system.N} densities")
+            rho0 = rho0.reshape((S,) + system.shape)
+            rp = (_p * S)(*[rho0[i].ctypes.data for i in range(S)])
+            _check(lib().cdft_mixture_fire_create(system.h, ctypes.cast(rp, _p), 0, ctypes.byref(p),
+                                                  ctypes.byref(h)))
+        else:
+            _check(lib().cdft_fire_create(system.h, _ptr(rho0), 0, ctypes.byref(p), ctypes.byref(h)))
         self.h = h
 
     def step(self):
@@ -514,6 +526,8 @@ class Fire:
 
     def state(self):
         shp = self.system.shape
+        if isinstance(self.system, Mixture):
+            shp = (self.system.nspecies,) + shp
         rho, df, x, v = (np.zeros(shp) for _ in range(4))
         _check(lib().cdft_fire_get(self.h, _ptr(rho), _ptr(df), _ptr(x), _ptr(v)))
         return rho, x, v, df
@@ -690,8 +704,10 @@ class Mixture:
     FMT::calculateFreeEnergyAndDerivatives over several FMT species (src/FMT.cpp:363-427).  FP64."""
 
     def __init__(self, shape, dx, hsd, model=WHITEBEAR_I, weights=None, A=1.0, B=0.0,
-                 weights_mode=WEIGHTS_ACCURATE):
-        """hsd: one diameter per species; weights: None, or one entry per species (None or 11*N cell weights)."""
+                 weights_mode=WEIGHTS_ACCURATE, ideal=True, mu=None, vext=None):
+        """hsd: one diameter per species; weights: None, or one entry per species (None or 11*N cell weights).
+        ideal, mu (one beta*mu per species) and vext (None, or one entry per species: None or N values of
+        beta*V_ext) are the rest of the DFT that eval and FIRE over the mixture use."""
         self.shape = tuple(int(n) for n in shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
@@ -717,13 +733,39 @@ class Mixture:
                 ptrs[i] = a.ctypes.data
             self._wptrs = ptrs
             wptr = ctypes.cast(ptrs, _p)
+        mptr = None
+        if mu is not None:
+            self._mu = np.ascontiguousarray(np.atleast_1d(mu), dtype=np.float64).reshape(-1)
+            if self._mu.size != self.nspecies:
+                raise ValueError(f"expected {self.nspecies} chemical potentials")
+            mptr = self._mu.ctypes.data
+        vptr = None
+        self._v = []
+        if vext is not None:
+            if len(vext) != self.nspecies:
+                raise ValueError(f"expected {self.nspecies} external fields")
+            vp = (_p * self.nspecies)()
+            for i, v in enumerate(vext):
+                if v is None:
+                    vp[i] = None
+                    continue
+                a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+                if a.size != self.N:
+                    raise ValueError(f"expected {self.N} external-field values per species")
+                self._v.append(a)
+                vp[i] = a.ctypes.data
+            self._vptrs = vp
+            vptr = ctypes.cast(vp, _p)
         d = _MixDesc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A),
-                     float(B), self.nspecies, self._hsd.ctypes.data, wptr, int(weights_mode))
+                     float(B), self.nspecies, self._hsd.ctypes.data, wptr, int(weights_mode), int(bool(ideal)), mptr,
+                     vptr)
         h = _p()
         _check(lib().cdft_mixture_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
         self._w = []
         self._wptrs = None
+        self._v = []
+        self._vptrs = None
         self.model = model
 
     def close(self):
@@ -751,6 +793,21 @@ class Mixture:
         _check(lib().cdft_mixture_fmt_eval(self.h, ctypes.cast(rp, _p), ctypes.cast(fp, _p), ctypes.byref(F),
                                            ACCUMULATE if accumulate else 0))
         return F.value, out
+
+    def eval(self, rho, dF=None, accumulate=False, only_fex=False):
+        """DFT::calculateFreeEnergyAndDerivatives over the species: returns (parts[5], dF), parts = {F, F_id, F_hs,
+        0, F_ext}; only_fex leaves out the ideal-gas terms."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        if rho.size != self.nspecies * self.N:
+            raise ValueError(f"expected {self.nspecies} x {self.N} densities, got {rho.size}")
+        rho = rho.reshape((self.nspecies,) + self.shape)
+        out = _out(dF, self.nspecies * self.N, (self.nspecies,) + self.shape).reshape((self.nspecies,) + self.shape)
+        rp = (_p * self.nspecies)(*[rho[i].ctypes.data for i in range(self.nspecies)])
+        fp = (_p * self.nspecies)(*[out[i].ctypes.data for i in range(self.nspecies)])
+        parts = np.zeros(5)
+        flags = (ACCUMULATE if accumulate else 0) | (NO_IDEAL if only_fex else 0)
+        _check(lib().cdft_mixture_eval(self.h, ctypes.cast(rp, _p), ctypes.cast(fp, _p), _ptr(parts), flags))
+        return parts, out
 
     def fmt_eval_device(self, rho_ptrs, dF_ptrs, accumulate=False):
         """Device pointers (one per species): returns F_hs; dF written or added to on the device."""

@@ -1241,6 +1241,9 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
     sd.precision = CDFT_FP64;
     sd.fmt_weights = d->fmt_weights ? d->fmt_weights[i] : nullptr;
     sd.weights_mode = d->weights_mode;
+    sd.ideal = d->ideal;
+    sd.mu = d->mu ? d->mu[i] : 0.0;
+    sd.vext = d->vext ? d->vext[i] : nullptr;
     cdft_system s = nullptr;
     create_system(&sd, &s, 1, nullptr, nullptr, false);
     x->sp.push_back(s);
@@ -1263,8 +1266,14 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
   *out = x.release();
 }
 
-// F_hs (x dV) and dF_s of the mixture on device data (dF written, or added to with acc).
-int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF, bool acc, double* F_hs) {
+// The mixture on device data: dF_s written, or added to with acc; parts (may be null) = {F, F_id, F_hs, 0, F_ext}
+// summed over species.  with_dft adds each species' ideal gas (unless no_ideal), external field and -mu N
+// (src/DFT.cpp:377-425) to the hard-sphere term.
+int mixture_species(const cdft_mixture_s& x) { return (int)x.sp.size(); }
+cdft_system mixture_member(const cdft_mixture_s& x, int i) { return x.sp[i]; }
+
+int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF, bool acc, double* parts,
+                 bool with_dft, bool no_ideal) {
   cdft_system_s& s0 = *x.sp[0];
   const long Nkp = s0.Nkp;
   const int m = s0.m, S = (int)x.sp.size();
@@ -1293,12 +1302,16 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
   HIPCHK(hipGetLastError());
   // 3. y forward of the measure derivatives, then per species: collapse, x back-convolution, y and z inverse
   launch_cols<double>(s0, Mx, Nkp, x.nm, 1, true, 1, 0);
-  FinalParams fp{};
-  fp.dV = s0.d.dx * s0.d.dy * s0.d.dz;
-  fp.fmt = 1;
-  fp.accumulate = acc;
+  const double dV = s0.d.dx * s0.d.dy * s0.d.dz;
   for (int i = 0; i < S; ++i) {
     cdft_system_s& s = *x.sp[i];
+    FinalParams fp{};
+    fp.dV = dV;
+    fp.fmt = 1;
+    fp.accumulate = acc;
+    fp.mu = with_dft ? s.d.mu : 0.0;
+    fp.ideal = with_dft && s.d.ideal && !no_ideal;
+    fp.ext = with_dft && s.has_vext;
     cplx<double>* A = s.A.as<cplx<double>>();
     cplx<double>* B = s.B.as<cplx<double>>();
     hipLaunchKernelGGL(k_mix_collapse, eg, eb, 0, s0.stream, Mx, B, Nkp, Nkp, x.hsd[i], tensor);
@@ -1310,24 +1323,48 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
       constexpr int Mc = decltype(mc)::value;
       constexpr int RB = row_rb<double, Mc>(32768), P = RowF<double, Mc>::P, SLOT = RowF<double, Mc>::SLOT;
       const Batch bt{Nkp, s.nconv * Nkp, s.N, 4 * s.pslots, nullptr};
-      hipLaunchKernelGGL((k_z_final2<double, Mc>), dim3(div_up(s.nrows, RB), 1), dim3(RB * P),
-                         RB * SLOT * sizeof(cplx<double>), s0.stream, A, rho[i], nullptr, nullptr, dF[i],
-                         unsigned(s.N * 8), s.nrows, s.g.Zp, tw.m.template as<cplx<double>>(),
-                         tw.n.template as<cplx<double>>(), RB, fp, s.partials2.as<double2>(), bt);
+      const long fblk = div_up(s.nrows, RB);
+      hipLaunchKernelGGL((k_z_final2<double, Mc>), dim3(fblk, 1), dim3(RB * P), RB * SLOT * sizeof(cplx<double>),
+                         s0.stream, A, rho[i], s.vext.as<double>(), nullptr, dF[i], unsigned(s.N * 8), s.nrows,
+                         s.g.Zp, tw.m.template as<cplx<double>>(), tw.n.template as<cplx<double>>(), RB, fp,
+                         s.partials2.as<double2>(), bt);
+      HIPCHK(hipGetLastError());
+      // this species' ideal-gas, external-field and particle sums into its results[1..4]
+      hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s0.stream, s.partials2.as<double2>(), fblk,
+                         4, s.results.as<double>() + 1, 4 * s.pslots, 8L);
     });
     HIPCHK(hipGetLastError());
   }
   double res = 0.0;
   int bad = 0;
+  std::vector<double> sres(8 * (size_t)S);
   HIPCHK(hipMemcpyAsync(&res, x.results.p, sizeof(double), hipMemcpyDeviceToHost, s0.stream));
   HIPCHK(hipMemcpyAsync(&bad, x.flag.p, sizeof(int), hipMemcpyDeviceToHost, s0.stream));
+  if (parts)
+    for (int i = 0; i < S; ++i)
+      HIPCHK(hipMemcpyAsync(sres.data() + 8 * i, x.sp[i]->results.p, sizeof(double) * 8, hipMemcpyDeviceToHost,
+                            s0.stream));
   HIPCHK(hipStreamSynchronize(s0.stream));
   if (bad) {
     HIPCHK(hipMemset(x.flag.p, 0, sizeof(int)));
     g_err = "Eta too large in FMT::dPHI";
     return CDFT_ETA_TOO_LARGE;
   }
-  if (F_hs) *F_hs = res * fp.dV;
+  if (parts) {
+    const double F_hs = res * dV;
+    double F_id = 0.0, F_ext = 0.0;  // species by species, as DFT::calculateFreeEnergyAndDerivatives adds them
+    for (int i = 0; i < S; ++i) {
+      const cdft_system_s& s = *x.sp[i];
+      const double* r = sres.data() + 8 * i;
+      if (with_dft && s.d.ideal && !no_ideal) F_id += r[1];
+      if (with_dft) F_ext += csum({s.has_vext ? r[3] * dV : 0.0, -r[4] * dV * s.d.mu});
+    }
+    parts[0] = csum({F_id, F_hs, F_ext});
+    parts[1] = F_id;
+    parts[2] = F_hs;
+    parts[3] = 0.0;
+    parts[4] = F_ext;
+  }
   return CDFT_OK;
 }
 
@@ -2094,7 +2131,12 @@ int cdft_mixture_fmt_eval(cdft_mixture mix, const double* const* rho, double* co
     const int S = (int)mix->sp.size();
     for (int i = 0; i < S; ++i) REQUIRE(rho[i] && dF[i], "null species array");
     const bool acc = flags & CDFT_ACCUMULATE;
-    if (flags & CDFT_DEVICE_PTRS) return mixture_eval(*mix, rho, dF, acc, F_hs);
+    double parts[5];
+    if (flags & CDFT_DEVICE_PTRS) {
+      const int rc = mixture_eval(*mix, rho, dF, acc, parts);
+      if (rc == CDFT_OK && F_hs) *F_hs = parts[2];
+      return rc;
+    }
     // host data: staged in each member's density and force buffers
     cdft_system_s& s0 = *mix->sp[0];
     const size_t bytes = sizeof(double) * s0.N;
@@ -2107,7 +2149,35 @@ int cdft_mixture_fmt_eval(cdft_mixture mix, const double* const* rho, double* co
       rd[i] = s.rho.as<double>();
       fd[i] = s.dF.as<double>();
     }
-    const int rc = mixture_eval(*mix, rd.data(), fd.data(), acc, F_hs);
+    const int rc = mixture_eval(*mix, rd.data(), fd.data(), acc, parts);
+    if (rc != CDFT_OK) return rc;
+    if (F_hs) *F_hs = parts[2];
+    for (int i = 0; i < S; ++i)
+      HIPCHK(hipMemcpyAsync(dF[i], mix->sp[i]->dF.p, bytes, hipMemcpyDeviceToHost, s0.stream));
+    HIPCHK(hipStreamSynchronize(s0.stream));
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_mixture_eval(cdft_mixture mix, const double* const* rho, double* const* dF, double* parts, int flags) {
+  return guarded([&] {
+    REQUIRE(mix && rho && dF, "null argument");
+    const int S = (int)mix->sp.size();
+    for (int i = 0; i < S; ++i) REQUIRE(rho[i] && dF[i], "null species array");
+    const bool acc = flags & CDFT_ACCUMULATE, no_ideal = flags & CDFT_NO_IDEAL;
+    if (flags & CDFT_DEVICE_PTRS) return mixture_eval(*mix, rho, dF, acc, parts, true, no_ideal);
+    cdft_system_s& s0 = *mix->sp[0];
+    const size_t bytes = sizeof(double) * s0.N;
+    std::vector<const double*> rd(S);
+    std::vector<double*> fd(S);
+    for (int i = 0; i < S; ++i) {
+      cdft_system_s& s = *mix->sp[i];
+      HIPCHK(hipMemcpyAsync(s.rho.p, rho[i], bytes, hipMemcpyHostToDevice, s0.stream));
+      if (acc) HIPCHK(hipMemcpyAsync(s.dF.p, dF[i], bytes, hipMemcpyHostToDevice, s0.stream));
+      rd[i] = s.rho.as<double>();
+      fd[i] = s.dF.as<double>();
+    }
+    const int rc = mixture_eval(*mix, rd.data(), fd.data(), acc, parts, true, no_ideal);
     if (rc != CDFT_OK) return rc;
     for (int i = 0; i < S; ++i)
       HIPCHK(hipMemcpyAsync(dF[i], mix->sp[i]->dF.p, bytes, hipMemcpyDeviceToHost, s0.stream));

@@ -79,23 +79,42 @@ __global__ void k_finish(const double2* __restrict__ partials, int nblk, int NS,
 __global__ void k_zero_flag(int* p) { *p = 0; }
 
 // ----------------------------- FIRE2 kernels -------------------------------
-__global__ void k_alias_from_rho(const double* __restrict__ rho, double* __restrict__ x, long N, Alias al) {
-  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) x[i] = al.x_of(rho[i]);
+// The alias maps of the species whose variables a FIRE run concatenates (one species: one segment): element i
+// belongs to species i / nseg, each with its own c (FMT_Species, src/FMT_Species.cpp:29-93).
+constexpr int kMaxAliasSeg = 8;
+struct AliasSet {
+  Alias a[kMaxAliasSeg];
+  long nseg;  // elements per species
+  int ns;     // species
+  // a chain of selects on constant indices: no division and no dynamic indexing of the argument block
+  __device__ __forceinline__ Alias of(long i) const {
+    Alias r = a[0];
+#pragma unroll
+    for (int s = 1; s < kMaxAliasSeg; ++s)
+      if (s < ns && i >= s * nseg) r = a[s];
+    return r;
+  }
+};
+
+__global__ void k_alias_from_rho(const double* __restrict__ rho, double* __restrict__ x, long N, AliasSet al) {
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads)
+    x[i] = al.of(i).x_of(rho[i]);
 }
 
-__global__ void k_rho_from_alias(const double* __restrict__ x, double* __restrict__ rho, long N, Alias al) {
-  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) rho[i] = al.rho(x[i]);
+__global__ void k_rho_from_alias(const double* __restrict__ x, double* __restrict__ rho, long N, AliasSet al) {
+  for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads)
+    rho[i] = al.of(i).rho(x[i]);
 }
 
 // dF -> alias derivative (in place) and the reductions that follow a
 // getDF_DX: sum dF_a^2, sum v.dF_a, max|dF_a|, max|dF_raw|.
 __global__ void k_fire_post(const double* __restrict__ x, const double* __restrict__ v, const double* __restrict__ raw,
-                            double* __restrict__ dFa, long N, Alias al, double2* __restrict__ partials) {
+                            double* __restrict__ dFa, long N, AliasSet al, double2* __restrict__ partials) {
   CSum s[2] = {{0, 0}, {0, 0}};
   double mx[2] = {0.0, 0.0};
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
     const double r = raw[i];
-    const double a = r * al.dfac(x[i]);
+    const double a = r * al.of(i).dfac(x[i]);
     dFa[i] = a;
     s[0].add(a * a);
     s[1].add(v[i] * a);
@@ -121,14 +140,14 @@ __global__ void k_fire_kick(double* __restrict__ v, const double* __restrict__ d
 
 // v <- (1-a) v + mix dF; x <- x + dt v; rho <- alias(x) (Minimizer.cpp:295-300, FMT_Species.cpp:29-48).
 __global__ void k_fire_mix(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ dF,
-                           double* __restrict__ rho, long N, double alpha, double mix, double dt, Alias al) {
+                           double* __restrict__ rho, long N, double alpha, double mix, double dt, AliasSet al) {
   for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < N; i += (long)gridDim.x * kThreads) {
     double nv = v[i] * (1 - alpha);
     nv += dF[i] * mix;
     v[i] = nv;
     const double nx = x[i] + nv * dt;
     x[i] = nx;
-    rho[i] = al.rho(nx);
+    rho[i] = al.of(i).rho(nx);
   }
 }
 
@@ -292,10 +311,11 @@ using namespace cdft_impl;
 
 // =============================== FIRE2 =====================================
 struct cdft_fire_s {
-  cdft_system sys = nullptr;
-  long N = 0;
+  cdft_system sys = nullptr;    // the evaluating system (a mixture's first member, for its stream)
+  cdft_mixture mix = nullptr;   // several species: x, v, dF, rho hold them one after the other
+  long N = 0;                   // elements of the concatenated variables (species x lattice sites)
   double dV = 0;
-  Alias al{};
+  AliasSet al{};
   DevBuf x, v, dF, raw, rho, x_rem, v_rem, dF_rem, raw_rem;
   DevBuf dir;  // Minimizer::fixed_direction_ (allocated when set)
   bool has_dir = false;
@@ -317,8 +337,20 @@ struct cdft_fire_s {
   }
   // Minimizer::getDF_DX (Minimizer.cpp:90-115) from the current rho buffer.
   int getDF_DX() {
-    enqueue_eval(sys, rho.as<double>(), raw.as<double>(), true, false);
-    int rc = fetch_parts(sys, parts);
+    int rc;
+    if (mix) {
+      const int S = mixture_species(*mix);
+      std::vector<const double*> r(S);
+      std::vector<double*> d(S);
+      for (int i = 0; i < S; ++i) {
+        r[i] = rho.as<double>() + i * al.nseg;
+        d[i] = raw.as<double>() + i * al.nseg;
+      }
+      rc = mixture_eval(*mix, r.data(), d.data(), false, parts, true, false);
+    } else {
+      enqueue_eval(sys, rho.as<double>(), raw.as<double>(), true, false);
+      rc = fetch_parts(sys, parts);
+    }
     if (rc == CDFT_ETA_TOO_LARGE) return rc;
     calls++;
     F = parts[0];
@@ -727,9 +759,11 @@ int cdft_fire_create(cdft_system sys, const double* rho0, int flags, const cdft_
     f->dt_max = f->p.dt_max;
     f->dt_best = f->dt;
     f->alpha = f->p.alpha_start;
-    f->al.fmt = sys->m > 0;
+    f->al.nseg = f->N;
+    f->al.ns = 1;
+    f->al.a[0].fmt = sys->m > 0;
     const double d = sys->d.hsd;
-    f->al.c = (1.0 - 1e-18 * (4 * M_PI * d * d * d / 3)) / f->dV;
+    f->al.a[0].c = (1.0 - 1e-18 * (4 * M_PI * d * d * d / 3)) / f->dV;
     const size_t b = sizeof(double) * f->N;
     for (DevBuf* q : {&f->x, &f->v, &f->dF, &f->raw, &f->rho, &f->x_rem, &f->v_rem, &f->dF_rem, &f->raw_rem}) q->alloc(b);
     f->red.init();
@@ -739,6 +773,51 @@ int cdft_fire_create(cdft_system sys, const double* rho0, int flags, const cdft_
     hipLaunchKernelGGL(k_alias_from_rho, vgrid(f->N), dim3(kThreads), 0, sys->stream, f->rho.as<double>(),
                        f->x.as<double>(), f->N, f->al);
     hipLaunchKernelGGL(k_rho_from_alias, vgrid(f->N), dim3(kThreads), 0, sys->stream, f->x.as<double>(),
+                       f->rho.as<double>(), f->N, f->al);
+    HIPCHK(hipGetLastError());
+    const int rc = f->getDF_DX();
+    if (rc != CDFT_OK) return rc;
+    *out = f.release();
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_mixture_fire_create(cdft_mixture mix, const double* const* rho0, int flags, const cdft_fire_params* params,
+                             cdft_fire* out) {
+  return guarded([&] {
+    REQUIRE(mix && rho0 && out, "null argument");
+    const int S = mixture_species(*mix);
+    REQUIRE(S <= kMaxAliasSeg, "the mixture's FIRE takes at most 8 species");
+    for (int i = 0; i < S; ++i) REQUIRE(rho0[i], "null species density");
+    cdft_system s0 = mixture_member(*mix, 0);
+    auto f = std::make_unique<cdft_fire_s>();
+    f->sys = s0;
+    f->mix = mix;
+    f->N = s0->N * S;
+    f->dV = s0->d.dx * s0->d.dy * s0->d.dz;
+    if (params) f->p = *params;
+    else default_fire_params(&f->p);
+    f->dt = f->p.dt;
+    f->dt_max = f->p.dt_max;
+    f->dt_best = f->dt;
+    f->alpha = f->p.alpha_start;
+    f->al.nseg = s0->N;
+    f->al.ns = S;
+    for (int i = 0; i < S; ++i) {  // FMT_Species aliases, each with its own diameter
+      const double d = mixture_member(*mix, i)->d.hsd;
+      f->al.a[i].fmt = 1;
+      f->al.a[i].c = (1.0 - 1e-18 * (4 * M_PI * d * d * d / 3)) / f->dV;
+    }
+    const size_t b = sizeof(double) * f->N;
+    for (DevBuf* q : {&f->x, &f->v, &f->dF, &f->raw, &f->rho, &f->x_rem, &f->v_rem, &f->dF_rem, &f->raw_rem}) q->alloc(b);
+    f->red.init();
+    for (int i = 0; i < S; ++i)
+      HIPCHK(hipMemcpyAsync(f->rho.as<double>() + i * s0->N, rho0[i], sizeof(double) * s0->N,
+                            (flags & CDFT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s0->stream));
+    HIPCHK(hipMemsetAsync(f->v.p, 0, b, s0->stream));
+    hipLaunchKernelGGL(k_alias_from_rho, vgrid(f->N), dim3(kThreads), 0, s0->stream, f->rho.as<double>(),
+                       f->x.as<double>(), f->N, f->al);
+    hipLaunchKernelGGL(k_rho_from_alias, vgrid(f->N), dim3(kThreads), 0, s0->stream, f->x.as<double>(),
                        f->rho.as<double>(), f->N, f->al);
     HIPCHK(hipGetLastError());
     const int rc = f->getDF_DX();
@@ -819,7 +898,7 @@ int cdft_fire_get(cdft_fire f, double* rho, double* dF_raw, double* x, double* v
     hipLaunchKernelGGL(k_rho_from_alias, vgrid(f->N), dim3(kThreads), 0, f->sys->stream, f->x.as<double>(),
                        f->rho.as<double>(), f->N, f->al);
     HIPCHK(hipGetLastError());
-    constraints_pre(*f->sys, f->rho.as<double>());  // with a fixed mass: the density the next evaluation uses
+    if (!f->mix) constraints_pre(*f->sys, f->rho.as<double>());  // with a fixed mass: the density the next evaluation uses
     if (rho) HIPCHK(hipMemcpyAsync(rho, f->rho.p, b, hipMemcpyDeviceToHost, f->sys->stream));
     if (dF_raw) HIPCHK(hipMemcpyAsync(dF_raw, f->raw.p, b, hipMemcpyDeviceToHost, f->sys->stream));
     if (x) HIPCHK(hipMemcpyAsync(x, f->x.p, b, hipMemcpyDeviceToHost, f->sys->stream));

@@ -477,6 +477,13 @@ void constraints_pre(cdft_system_s& s, double* rho);
 // allow_trace = false keeps all 11 tensor-model arrays (the members of a mixture, which sum them per measure).
 void create_system(const cdft_system_desc* desc, cdft_system* out, int nb = 1, const double* mus = nullptr,
                    const double* vexts = nullptr, bool allow_trace = true);
+// Hard-sphere mixtures (cdft_api.hip): the evaluation on device data (dF written, or added to with acc; parts =
+// {F, F_id, F_hs, 0, F_ext}; with_dft adds the ideal gas unless no_ideal, V_ext and -mu N per species), the number
+// of species and member system i (whose stream the whole mixture uses).
+int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF, bool acc, double* parts,
+                 bool with_dft = false, bool no_ideal = false);
+int mixture_species(const cdft_mixture_s& x);
+cdft_system mixture_member(const cdft_mixture_s& x, int i);
 int fetch_parts(cdft_system s, double* parts);
 double csum(std::initializer_list<double> v);
 template <typename F>

@@ -383,6 +383,12 @@ typedef struct cdft_mixture_desc {
   const double* const* fmt_weights;  /* host, NULL, or nspecies pointers to 11*N cell weights (NULL entries are
                                         generated on the device, as for a single species) */
   int weights_mode;                  /* enum cdft_weights_mode */
+  /* the rest of DFT::calculateFreeEnergyAndDerivatives per species (src/DFT.cpp:377-425, src/Species.cpp:51-62),
+     used by cdft_mixture_eval and the mixture's FIRE: */
+  int ideal;                         /* include each species' ideal-gas term */
+  const double* mu;                  /* host, NULL or nspecies values of beta*mu */
+  const double* const* vext;         /* host, NULL or nspecies pointers (NULL entries: no field) to N values of
+                                        beta*V_ext */
 } cdft_mixture_desc;
 
 /* Replaces the construction of the species' FMT_Species weights (src/FMT_Species.cpp:95-108, 310-496) for a
@@ -395,6 +401,17 @@ int cdft_mixture_species(cdft_mixture mix, int* nspecies);
    *F_hs receives the hard-sphere free energy (x dV); dF[s] receives dF_hs/drho_s (x dV), written or, with
    CDFT_ACCUMULATE, added to (Species::addToForce).  CDFT_ETA_TOO_LARGE as cdft_fmt_eval. */
 int cdft_mixture_fmt_eval(cdft_mixture mix, const double* const* rho, double* const* dF, double* F_hs, int flags);
+/* Replaces DFT::calculateFreeEnergyAndDerivatives(onlyFex) with several species (src/DFT.cpp:360-425): per species
+   the ideal gas (unless CDFT_NO_IDEAL), dV sum V_ext rho - mu N, and the mixture's hard-sphere term.  parts (host,
+   may be NULL) receives {F, F_id, F_hs, F_mf = 0, F_ext} summed over species.  The reference's mean field between
+   species is not implemented there (src/Interaction.cpp:110) and is not here. */
+int cdft_mixture_eval(cdft_mixture mix, const double* const* rho, double* const* dF, double* parts, int flags);
+/* fireMinimizer2 over all species of the mixture (src/Minimizer.cpp:160-327: one FIRE2 state, P, norms and the
+   mixing over the species' alias variables together), on cdft_mixture_eval.  The returned handle is driven by
+   cdft_fire_step / _run / _destroy; cdft_fire_get and cdft_fire_reset take nspecies x N values (species after
+   species).  rho0: nspecies pointers to N values (host, or device with CDFT_DEVICE_PTRS). */
+int cdft_mixture_fire_create(cdft_mixture mix, const double* const* rho0, int flags, const cdft_fire_params* params,
+                             cdft_fire* out);
 
 /* Copies the device-generated real-space FMT weights (11*N) or MF kernel (N). */
 int cdft_system_get_weights(cdft_system sys, double* w11);

@@ -89,6 +89,13 @@ def lib():
         L.orc_dft_lagrange.restype = _d
         L.orc_fire_create.argtypes = [_p, _p, _d, _d]
         L.orc_fire_create.restype = _p
+        L.orc_mix_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _i, _p, _p, _i, _p, _p]
+        L.orc_mix_create.restype = _p
+        L.orc_mix_destroy.argtypes = [_p]
+        L.orc_mix_eval.argtypes = [_p, _p, _p, _p]
+        L.orc_mix_eval.restype = _i
+        L.orc_mix_fire_create.argtypes = [_p, _p, _d, _d]
+        L.orc_mix_fire_create.restype = _p
         L.orc_fire_destroy.argtypes = [_p]
         L.orc_fire_step.argtypes = [_p, _p]
         L.orc_fire_step.restype = _i
@@ -410,12 +417,49 @@ class DFT:
             self.h = None
 
 
+class MixDFT:
+    """Whole-DFT oracle of a hard-sphere mixture: per species ideal gas, V_ext and -mu N, and the mixture FMT."""
+
+    def __init__(self, shape, dx, hsd, model=WBI, A=1.0, B=0.0, weights=None, ideal=True, mu=None, vext=None):
+        self.species_shape = tuple(shape)
+        self._hsd = np.ascontiguousarray(np.atleast_1d(hsd), dtype=np.float64).reshape(-1)
+        self.S = int(self._hsd.size)
+        self.shape = (self.S,) + self.species_shape
+        dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
+        self.dV = dxs[0] * dxs[1] * dxs[2]
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        self._mu = None if mu is None else np.ascontiguousarray(np.atleast_1d(mu), dtype=np.float64)
+        self._vext = None
+        if vext is not None:  # per species: None (no field) or N values
+            self._vext = np.stack([np.zeros(self.species_shape) if v is None else np.asarray(v, dtype=np.float64)
+                                   .reshape(self.species_shape) for v in vext])
+        self.h = lib().orc_mix_create(model, A, B, *self.species_shape, *dxs, self.S, _ptr(self._hsd), _ptr(w),
+                                      int(bool(ideal)), _ptr(self._mu), _ptr(self._vext))
+
+    def eval(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(self.shape)
+        dF = np.zeros(self.shape)
+        parts = np.zeros(5)
+        if lib().orc_mix_eval(self.h, _ptr(rho), _ptr(dF), _ptr(parts)) == 1:
+            raise EtaTooLarge("eta too large")
+        return parts, dF
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mix_destroy(self.h)
+            self.h = None
+
+
 class Fire:
     def __init__(self, dft, rho0, dt=1e-3, dt_max=1e-2):
+        """dft: a DFT, or a MixDFT (one FIRE2 state over all species; rho0 of shape (S,) + lattice)."""
         self.dft = dft
         self.shape = dft.shape
         r = np.ascontiguousarray(rho0, dtype=np.float64)
-        self.h = lib().orc_fire_create(dft.h, _ptr(r), dt, dt_max)
+        if isinstance(dft, MixDFT):
+            self.h = lib().orc_mix_fire_create(dft.h, _ptr(r.reshape(self.shape)), dt, dt_max)
+        else:
+            self.h = lib().orc_fire_create(dft.h, _ptr(r), dt, dt_max)
 
     def step(self):
         info = np.zeros(8)

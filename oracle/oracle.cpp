@@ -1638,6 +1638,43 @@ int dft_eval(DftState& s, double* rho, double* dF) {
   return 0;
 }
 
+// DFT::calculateFreeEnergyAndDerivatives with several hard-sphere species (src/DFT.cpp:360-425): each species'
+// ideal gas, V_ext and -mu N (src/Species.cpp:51-62; dft_eval on an FMT-less state per species) and the mixture's
+// hard-sphere term (fmt_eval_mix).  No mean field: the reference's between-species interaction is unfinished
+// (src/Interaction.cpp:110 throws).
+struct MixDft {
+  Grid g;
+  Model model;
+  double dx = 0, dy = 0, dz = 0;
+  int S = 0;
+  std::vector<double> hsd;
+  std::vector<std::vector<cd>> wk;
+  std::vector<DftState> sp;
+  double parts[5] = {0, 0, 0, 0, 0};  // F, F_id, F_hs, 0, F_ext
+};
+
+int mix_eval(MixDft& m, double* rho, double* dF) {
+  const long N = m.g.N();
+  NSum Fid, Fext;
+  for (int s = 0; s < m.S; ++s) {  // writes dF_s
+    dft_eval(m.sp[s], rho + s * N, dF + s * N);
+    Fid.add(m.sp[s].parts[1]);
+    Fext.add(m.sp[s].parts[4]);
+  }
+  double Fhs = 0.0;
+  if (fmt_eval_mix(m.g, m.model, m.dx * m.dy * m.dz, m.S, m.hsd.data(), m.wk, rho, dF, &Fhs)) return 1;
+  NSum F;
+  F.add(Fid.sum());
+  F.add(Fhs);
+  F.add(Fext.sum());
+  m.parts[0] = F.sum();
+  m.parts[1] = Fid.sum();
+  m.parts[2] = Fhs;
+  m.parts[3] = 0.0;
+  m.parts[4] = Fext.sum();
+  return 0;
+}
+
 // (d2F / drho_i drho_j) v_j: DFT::matrix_dot_v_intern, src/DFT.cpp:435-488 (periodic, no fixed boundary):
 // ideal gas dV v / rho when full (full_hessian_), FMT::add_second_derivative, and the mean field
 // Interaction_Base::add_second_derivative (src/Interaction.cpp:119-154): (dV/N) c2r(v_k w_att,k).
@@ -2136,9 +2173,14 @@ int orc_dft_eval(orc_dft* h, double* rho, double* dF, double* parts) {
 // ---------------------------------------------------------------------------
 namespace {
 struct Fire {
-  DftState* dft;
+  DftState* dft = nullptr;
+  MixDft* mix = nullptr;  // several species: x, v, rho, dF hold them one after the other (Minimizer.cpp:160-327
+                          // sums P, the norms and the maximum over species)
   long N;
   double dV, hsd, c;
+  std::vector<double> cseg;  // per species alias constant (mixtures), segments of nseg sites
+  long nseg = 0;
+  double cat(long i) const { return cseg.empty() ? c : cseg[i / nseg]; }
   std::vector<double> x, v, rho, dF, dF_raw;
   double dt = 1e-3, dt_max = 1e-2, dt_min = 0.0, dt_best = 0.0;
   int N_delay = 5, N_P_positive = 0, N_P_negative = 0, N_P_negative_max = 20, backtracks = 0;
@@ -2154,12 +2196,12 @@ struct Fire {
   void set_rho_from_alias() {  // FMT_Species.cpp:29-48
     for (long i = 0; i < N; ++i) {
       const double y = x[i];
-      rho[i] = 1e-18 + c * y * y / (1 + y * y);
+      rho[i] = 1e-18 + cat(i) * y * y / (1 + y * y);
     }
   }
   void alias_from_rho() {  // FMT_Species.cpp:51-72
     for (long i = 0; i < N; ++i) {
-      const double z = (rho[i] - 1e-18) / c;
+      const double z = (rho[i] - 1e-18) / cat(i);
       x[i] = std::sqrt(std::fabs(z / (1 - z)));
     }
   }
@@ -2167,9 +2209,9 @@ struct Fire {
   std::vector<double> fixed_dir;  // Minimizer::fixed_direction_ (empty: none)
   int getDF_DX() {
     set_rho_from_alias();
-    if (dft_eval(*dft, rho.data(), dF.data())) return 1;
+    if (mix ? mix_eval(*mix, rho.data(), dF.data()) : dft_eval(*dft, rho.data(), dF.data())) return 1;
     calls++;
-    F = dft->parts[0];
+    F = mix ? mix->parts[0] : dft->parts[0];
     if (!fixed_dir.empty()) {  // Minimizer.cpp:103-110: dF -= 2 (e . dF) e
       double d = 0.0;
       for (long i = 0; i < N; ++i) d += fixed_dir[i] * dF[i];
@@ -2178,7 +2220,7 @@ struct Fire {
     dF_raw = dF;
     for (long i = 0; i < N; ++i) {  // FMT_Species.cpp:74-93
       const double y = x[i];
-      dF[i] = dF[i] * (2 * c * y / ((1 + y * y) * (1 + y * y)));
+      dF[i] = dF[i] * (2 * cat(i) * y / ((1 + y * y) * (1 + y * y)));
     }
     return 0;
   }
@@ -2453,10 +2495,74 @@ int orc_fire_step(orc_fire* h, double* info) {
   }
   return rc;
 }
+// A hard-sphere mixture's DFT (MixDft): S species of diameters hsd on one lattice, weights S x 11 x N or null
+// (generated), per species the ideal gas (ideal), beta*mu (mu[S], may be null) and beta*V_ext (vext S x N, may be
+// null).
+struct orc_mix;
+orc_mix* orc_mix_create(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, int S,
+                        const double* hsd, const double* weights, int ideal, const double* mu, const double* vext) {
+  auto* m = new MixDft();
+  m->g = Grid{Nx, Ny, Nz};
+  m->model = make_model(kind, A, B);
+  m->dx = dx; m->dy = dy; m->dz = dz;
+  m->S = S;
+  m->hsd.assign(hsd, hsd + S);
+  m->wk.resize(S);
+  const long N = m->g.N();
+  for (int sp = 0; sp < S; ++sp)
+    weight_spectra(m->g, dx, dy, dz, hsd[sp], weights ? weights + (size_t)sp * 11 * N : nullptr, m->wk[sp]);
+  m->sp.resize(S);
+  for (int sp = 0; sp < S; ++sp) {
+    DftState& d = m->sp[sp];
+    d.fmt.g = m->g;
+    d.fmt.dx = dx; d.fmt.dy = dy; d.fmt.dz = dz; d.fmt.hsd = hsd[sp];
+    d.fmt.model = m->model;
+    d.has_fmt = false;
+    d.ideal = ideal != 0;
+    d.mu = mu ? mu[sp] : 0.0;
+    if (vext) d.vext.assign(vext + (size_t)sp * N, vext + (size_t)(sp + 1) * N);
+  }
+  return reinterpret_cast<orc_mix*>(m);
+}
+void orc_mix_destroy(orc_mix* h) { delete reinterpret_cast<MixDft*>(h); }
+// rho, dF: S x N; parts[5] = F, F_id, F_hs, 0, F_ext.  1 on eta too large.
+int orc_mix_eval(orc_mix* h, const double* rho, double* dF, double* parts) {
+  auto* m = reinterpret_cast<MixDft*>(h);
+  std::vector<double> r(rho, rho + (size_t)m->S * m->g.N());
+  const int rc = mix_eval(*m, r.data(), dF);
+  if (parts) std::memcpy(parts, m->parts, sizeof(m->parts));
+  return rc;
+}
+// FIRE2 over all species of a mixture (one state, concatenated variables; each species its own alias constant).
+orc_fire* orc_mix_fire_create(orc_mix* h, const double* rho0, double dt, double dt_max) {
+  auto* m = reinterpret_cast<MixDft*>(h);
+  auto* f = new Fire();
+  f->mix = m;
+  f->nseg = m->g.N();
+  f->N = f->nseg * m->S;
+  f->dV = m->dx * m->dy * m->dz;
+  f->hsd = m->hsd[0];
+  for (int sp = 0; sp < m->S; ++sp) {
+    const double d = m->hsd[sp];
+    f->cseg.push_back((1.0 - 1e-18 * (4 * M_PI * d * d * d / 3)) / f->dV);
+  }
+  f->c = f->cseg[0];
+  f->rho.assign(rho0, rho0 + f->N);
+  f->x.assign(f->N, 0.0);
+  f->v.assign(f->N, 0.0);
+  f->dF.assign(f->N, 0.0);
+  f->dt = dt;
+  f->dt_max = dt_max;
+  f->dt_best = dt;
+  f->alpha = f->alpha_start;
+  f->alias_from_rho();
+  f->last_status = f->getDF_DX();
+  return reinterpret_cast<orc_fire*>(f);
+}
 void orc_fire_get(orc_fire* h, double* rho, double* x, double* v, double* dF_raw) {
   auto* f = reinterpret_cast<Fire*>(h);
   f->set_rho_from_alias();
-  if (f->dft->fixed_mass > 0.0) {  // the density the next force evaluation uses (Species::beginForceCalculation)
+  if (f->dft && f->dft->fixed_mass > 0.0) {  // the density the next force evaluation uses (Species::beginForceCalculation)
     double n = 0.0;
     for (long i = 0; i < f->N; ++i) n += f->rho[i];
     const double s = f->dft->fixed_mass / (n * f->dV);

@@ -100,3 +100,82 @@ def test_eta_too_large_is_reported(cdft):
     F, _ = mix.fmt_eval(ok)  # the flag was cleared
     assert np.isfinite(F)
     mix.close()
+
+
+def close(a, b, tol):
+    return abs(a - b) <= tol * max(abs(a), abs(b), 1e-300)
+
+
+def test_mixture_dft_eval_matches_the_oracle(cdft):
+    """cdft_mixture_eval (DFT::calculateFreeEnergyAndDerivatives over several species, src/DFT.cpp:360-425): per
+    species ideal gas, V_ext and -mu N beside the mixture's FMT; a field on one species only."""
+    shape, dx, ds = (16, 24, 32), 0.1, [1.0, 0.6]
+    w = np.stack([oracle.fmt_weights(shape, dx, d) for d in ds])
+    rho = np.stack([seeded(shape, 0.3, 0x5EED7031), seeded(shape, 0.45, 0x5EED7032)])
+    v1 = 0.3 * (2 * oracle.splitmix64_uniform(int(np.prod(shape)), 0x5EED7033).reshape(shape) - 1)
+    mu = [0.4, -0.9]
+    o = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=mu, vext=[None, v1])
+    p0, dF0 = o.eval(rho)
+    mix = cdft.Mixture(shape, dx, ds, model=cdft.WHITEBEAR_I, weights=list(w), mu=mu, vext=[None, v1])
+    p1, dF1 = mix.eval(rho)
+    print("mixture DFT parts", p0, p1)
+    for k in (0, 1, 2, 4):
+        assert close(p1[k], p0[k], TOL), (k, p1[k], p0[k])
+    for s in range(2):
+        assert maxrel(dF1[s], dF0[s]) <= TOL
+    # only_fex: the ideal-gas terms leave, field and mu stay
+    p2, dF2 = mix.eval(rho, only_fex=True)
+    oi = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, ideal=False, mu=mu, vext=[None, v1])
+    p3, dF3 = oi.eval(rho)
+    assert p2[1] == 0.0 and close(p2[0], p3[0], TOL)
+    for s in range(2):
+        assert maxrel(dF2[s], dF3[s]) <= TOL
+    mix.close()
+
+
+def test_mixture_fire_replays_the_oracle(cdft):
+    """cdft_mixture_fire_create: one FIRE2 state over both species (src/Minimizer.cpp:160-327), step for step
+    against the oracle's FIRE over the mixture."""
+    shape, dx, ds = (16, 16, 16), 0.1, [1.0, 0.6]
+    w = np.stack([oracle.fmt_weights(shape, dx, d) for d in ds])
+    rho0 = np.stack([seeded(shape, 0.3, 0x5EED7041), seeded(shape, 0.4, 0x5EED7042)])
+    v0 = 0.2 * (2 * oracle.splitmix64_uniform(int(np.prod(shape)), 0x5EED7043).reshape(shape) - 1)
+    mu = [0.5, 0.2]
+    o = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=mu, vext=[v0, None])
+    mix = cdft.Mixture(shape, dx, ds, model=cdft.WHITEBEAR_I, weights=list(w), mu=mu, vext=[v0, None])
+    fo = oracle.Fire(o, rho0)
+    fg = cdft.Fire(mix, rho0)
+    F0 = o.eval(rho0)[0][0]
+    for i in range(40):
+        rc0, a = fo.step()
+        rc1, b = fg.step()
+        assert rc0 == rc1, (i, rc0, rc1)
+        assert close(a["F"], b["F"], 1e-10), (i, a["F"], b["F"])
+        assert close(a["dt"], b["dt"], 1e-12) and close(a["alpha"], b["alpha"], 1e-12)
+        assert close(a["fmax"], b["fmax_alias"], 1e-9) and close(a["fmax_raw"], b["fmax_raw"], 1e-9)
+    r0 = fo.state()[0]
+    r1 = fg.state()[0]
+    assert r1.shape == (2,) + shape
+    assert np.abs(r0 - r1).max() <= 1e-10 * np.abs(r0).max()
+    assert b["F"] < F0
+    fg.close()
+    mix.close()
+
+
+def test_mixture_fire_reaches_the_bulk_mixture(cdft):
+    """With mu_s = ln rho*_s + mu_hs_s(rho*) from the bulk White Bear closed form, FIRE over the mixture from other
+    uniform densities ends at rho*."""
+    from test_oracle_mixture import bulk_mu_hs
+    shape, dx, ds = (16, 16, 16), 0.1, [1.0, 0.6]
+    target = [0.25, 0.4]
+    mhs = bulk_mu_hs(oracle.WBI, target, ds)
+    mu = [np.log(target[s]) + mhs[s] for s in range(2)]
+    mix = cdft.Mixture(shape, dx, ds, model=cdft.WHITEBEAR_I, mu=mu)
+    f = cdft.Fire(mix, np.stack([np.full(shape, 0.2), np.full(shape, 0.45)]))
+    rc, rep = f.run(5000, 1e-10)
+    assert rc != 2 and rep["fmax_raw"] <= 1e-10
+    rho = f.state()[0]
+    for s in range(2):
+        assert np.abs(rho[s] / target[s] - 1).max() < 1e-7
+    f.close()
+    mix.close()

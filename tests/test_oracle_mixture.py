@@ -97,3 +97,80 @@ def test_mixture_central_differences(model, A, B):
             Fm, _ = oracle.fmt_eval_mix(model, rm, dx, ds, weights=w, A=A, B=B)
             fd = (Fp - Fm) / (2 * h)
             assert abs(fd - dF[(s,) + i]) <= 1e-6 * abs(dF[(s,) + i]) + 1e-12
+
+
+def bulk_mu_hs(model, rhos, ds):
+    """Excess chemical potentials d(phi)/d(rho_s) of the bulk mixture, by complex steps."""
+    h = 1e-30
+    out = []
+    for s in range(len(rhos)):
+        r = np.array(rhos, dtype=complex)
+        r[s] += 1j * h
+        out.append(bulk_phi(model, r, ds).imag / h)
+    return np.array(out)
+
+
+def test_mixture_dft_adds_each_species_ideal_gas_field_and_mu():
+    """MixDFT (the mixture's DFT::calculateFreeEnergyAndDerivatives, src/DFT.cpp:360-425): uniform densities give
+    F / V = phi_hs + sum_s rho_s (ln rho_s - 1) + sum_s rho_s (V_s - mu_s) and dF_s / dV = mu_hs_s + ln rho_s
+    + V_s - mu_s; a field on one species and a mu on the other show which species each term belongs to."""
+    shape, dx, ds = (16, 16, 16), 0.1, [1.0, 0.6]
+    rhos = [0.3, 0.5]
+    mu = [0.7, -1.2]
+    V1 = 0.35
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    m = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=mu, vext=[None, np.full(shape, V1)])
+    parts, dF = m.eval(np.stack([np.full(shape, r) for r in rhos]))
+    V = np.prod(shape) * dx ** 3
+    ideal = sum(r * (np.log(r) - 1) for r in rhos)
+    ext = rhos[1] * V1 - rhos[0] * mu[0] - rhos[1] * mu[1]
+    phi = bulk_phi(oracle.WBI, rhos, ds).real
+    assert abs(parts[1] / V - ideal) <= 1e-12 * abs(ideal)
+    assert abs(parts[2] / V - phi) <= 1e-12 * abs(phi)
+    assert abs(parts[4] / V - ext) <= 1e-12 * abs(ext)
+    assert abs(parts[0] / V - (ideal + phi + ext)) <= 1e-12 * abs(ideal + phi + ext)
+    mhs = bulk_mu_hs(oracle.WBI, rhos, ds)
+    want = [mhs[0] + np.log(rhos[0]) - mu[0], mhs[1] + np.log(rhos[1]) + V1 - mu[1]]
+    for s in range(2):
+        assert np.abs(dF[s] / dx ** 3 - want[s]).max() <= 1e-11 * max(1.0, abs(want[s]))
+
+
+def test_mixture_fire_finds_the_bulk_coexisting_densities():
+    """FIRE2 over two species (src/Minimizer.cpp:160-327, one state over both species' aliases) from uniform
+    densities away from equilibrium: with mu_s = ln rho*_s + mu_hs_s(rho*) (the bulk closed form) the minimum of
+    the grand potential is the uniform mixture rho*, which the solver must reach."""
+    shape, dx, ds = (8, 8, 8), 0.1, [1.0, 0.6]
+    target = [0.25, 0.4]
+    mhs = bulk_mu_hs(oracle.WBI, target, ds)
+    mu = [np.log(target[s]) + mhs[s] for s in range(2)]
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    m = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=mu)
+    f = oracle.Fire(m, np.stack([np.full(shape, 0.2), np.full(shape, 0.45)]), dt=1e-3, dt_max=1e-2)
+    for _ in range(3000):
+        rc, info = f.step()
+        assert rc != 2
+        if info["fmax_raw"] < 1e-10:
+            break
+    assert info["fmax_raw"] < 1e-10
+    rho = f.state()[0]
+    for s in range(2):
+        assert np.abs(rho[s] / target[s] - 1).max() < 1e-8
+
+
+def test_one_species_mixture_fire_is_the_single_species_fire():
+    """With one species the mixture's FIRE is fireMinimizer2 on that species' DFT, step for step."""
+    shape, dx, hsd = (8, 8, 8), 0.1, 0.8
+    rng = np.random.default_rng(9)
+    rho0 = 0.4 * (1 + 0.1 * rng.uniform(-1, 1, shape))
+    vext = 0.2 * rng.uniform(-1, 1, shape)
+    o = oracle.DFT(shape, dx, hsd, mu=0.3, vext=vext)
+    w = oracle.fmt_weights(shape, dx, hsd)
+    m = oracle.MixDFT(shape, dx, [hsd], model=oracle.WBI, weights=w[None], mu=[0.3], vext=[vext])
+    f1, f2 = oracle.Fire(o, rho0), oracle.Fire(m, rho0[None])
+    for _ in range(30):
+        rc1, a = f1.step()
+        rc2, b = f2.step()
+        assert rc1 == rc2
+        assert abs(a["F"] - b["F"]) <= 1e-12 * abs(a["F"])
+        assert a["dt"] == b["dt"] and a["alpha"] == b["alpha"]
+    assert np.abs(f1.state()[0] - f2.state()[0][0]).max() <= 1e-12 * rho0.max()
