@@ -519,9 +519,11 @@ DFT::~DFT() { invalidate(); }
 void DFT::invalidate() const {
   if (sys_) cdft_system_destroy(sys_);
   sys_ = nullptr;
+  if (mix_) cdft_mixture_destroy(mix_);
+  mix_ = nullptr;
+  mix_key_.clear();
 }
 void DFT::addSpecies(Species* s) {
-  if (!species_.empty()) throw std::runtime_error("the GPU DFT is implemented for a single species");
   species_.push_back(s);
   invalidate();
 }
@@ -539,7 +541,56 @@ void DFT::set_precision(int p) {
   invalidate();
 }
 
+cdft_mixture DFT::mixture() const {
+  const Lattice& L0 = species_.front()->getLattice();
+  std::vector<double> key, hsd, mu;
+  std::vector<const double*> vext;
+  for (const Species* s : species_) {
+    const auto* fs = dynamic_cast<const FMT_Species*>(s);
+    if (!fs || fs->ao_Rp() > 0)
+      throw std::runtime_error("a DFT of several species takes hard-sphere FMT species (no AO polymer)");
+    if (s->is_mass_fixed() || s->is_background_fixed() || s->is_boundary_homogeneous())
+      throw std::runtime_error("a DFT of several species takes no fixed mass and no fixed boundary");
+    const Lattice& L = s->getLattice();
+    if (L.Nx() != L0.Nx() || L.Ny() != L0.Ny() || L.Nz() != L0.Nz() || L.getDX() != L0.getDX() ||
+        L.getDY() != L0.getDY() || L.getDZ() != L0.getDZ())
+      throw std::runtime_error("the species of a mixture must share one lattice");
+    const Density& D = s->getDensity();
+    hsd.push_back(fs->getHSD());
+    mu.push_back(s->getChemPotential());
+    vext.push_back(D.has_external_field() ? D.get_external_field().memptr() : nullptr);
+    key.insert(key.end(), {hsd.back(), mu.back(), (double)D.field_version(), (double)(size_t)s});
+  }
+  if (!fmt_) throw std::runtime_error("a DFT of several species needs a hard-core (FMT) contribution");
+  if (!interactions_.empty())
+    throw std::runtime_error("mean-field interactions are implemented for a single species");
+  key.push_back(fmt_->model_id());
+  if (mix_ && key == mix_key_) return mix_;
+  if (mix_) cdft_mixture_destroy(mix_);
+  mix_ = nullptr;
+  cdft_mixture_desc d{};
+  d.Nx = (int)L0.Nx();
+  d.Ny = (int)L0.Ny();
+  d.Nz = (int)L0.Nz();
+  d.dx = L0.getDX();
+  d.dy = L0.getDY();
+  d.dz = L0.getDZ();
+  d.model = fmt_->model_id();
+  d.A = fmt_->A();
+  d.B = fmt_->B();
+  d.nspecies = (int)species_.size();
+  d.hsd = hsd.data();
+  d.ideal = 1;
+  d.mu = mu.data();
+  d.vext = vext.data();
+  check(cdft_mixture_create(&d, &mix_));
+  mix_key_ = key;
+  return mix_;
+}
+
 cdft_system DFT::system() const {
+  if (species_.size() != 1)
+    throw std::runtime_error("this operation is implemented for a single species (several: DFT::mixture)");
   Species* s = species_.front();
   const Density& D = s->getDensity();
   // with a fixed mass mu_ holds the last multiplier and the evaluation runs at mu = 0
@@ -587,11 +638,27 @@ cdft_system DFT::system() const {
 
 // src/DFT.cpp:360-425
 double DFT::calculateFreeEnergyAndDerivatives(bool onlyFex) {
+  double parts[5];
+  if (species_.size() > 1) {  // a hard-sphere mixture: every species' force written by the device
+    cdft_mixture m = mixture();
+    std::vector<const double*> r;
+    std::vector<double*> f;
+    for (Species* s : species_) {
+      s->zeroForce();
+      r.push_back(s->getDensity().get_density_pointer());
+      f.push_back(s->getDF().memptr());
+    }
+    check(cdft_mixture_eval(m, r.data(), f.data(), parts, onlyFex ? CDFT_NO_IDEAL : 0));
+    F_id_ = parts[1];
+    F_hs_ = parts[2];
+    F_mf_ = 0.0;
+    F_ext_ = parts[4];
+    return parts[0];
+  }
   Species* s = species_.front();
   s->zeroForce();
   s->beginForceCalculation();
   cdft_system sys = system();
-  double parts[5];
   // onlyFex skips the ideal gas only; the external field and mu stay (src/DFT.cpp:381-420)
   check(cdft_system_eval(sys, s->getDensity().get_density_pointer(), s->getDF().memptr(), parts,
                          onlyFex ? CDFT_NO_IDEAL : 0));
@@ -971,8 +1038,17 @@ void DFT::findCoex(double xmax, double dx, double& x1, double& x2, double tol) c
 // ---------------------------------------------------------------- Minimizer
 // The species takes the minimizer's density and, with a fixed mass, the multiplier of its last evaluation.
 void Minimizer::sync_species() {
-  DFT_Vec rho(dft_->get_lattice().Ntot());
+  const int S = dft_->getNumberOfSpecies();
+  const long N = dft_->get_lattice().Ntot();
+  DFT_Vec rho(N * S);
   density(rho);
+  if (S > 1) {  // species after species
+    for (int i = 0; i < S; ++i) {
+      Species* s = dft_->getSpecies(i);
+      for (long j = 0; j < N; ++j) s->set_density(j, rho.get(i * N + j));
+    }
+    return;
+  }
   Species* s = dft_->getSpecies(0);
   s->set_density(rho);
   double mu = 0.0;
@@ -1008,22 +1084,38 @@ fireMinimizer2::fireMinimizer2(DFT* dft) : Minimizer(dft) {
 fireMinimizer2::~fireMinimizer2() {
   if (fire_) cdft_fire_destroy(fire_);
 }
+// The species' densities one after the other (FIRE over several species runs on the concatenated variables).
+static DFT_Vec all_densities(const DFT& dft) {
+  const int S = dft.getNumberOfSpecies();
+  const long N = dft.get_lattice().Ntot();
+  DFT_Vec r(N * S);
+  for (int i = 0; i < S; ++i) std::memcpy(r.memptr() + i * N, dft.getDensity(i).get_density_pointer(), sizeof(double) * N);
+  return r;
+}
+
 void fireMinimizer2::ensure() {
   if (fire_) return;
-  Species* s = dft_->getSpecies(0);
-  check(cdft_fire_create(dft_->system(), s->getDensity().get_density_pointer(), 0, &params_, &fire_));
-  if (fixed_direction_.size() == s->getDensity().Ntot()) {
+  const int S = dft_->getNumberOfSpecies();
+  const long Nall = dft_->get_lattice().Ntot() * S;
+  if (S > 1) {
+    std::vector<const double*> r;
+    for (int i = 0; i < S; ++i) r.push_back(dft_->getDensity(i).get_density_pointer());
+    check(cdft_mixture_fire_create(dft_->mixture(), r.data(), 0, &params_, &fire_));
+  } else {
+    check(cdft_fire_create(dft_->system(), dft_->getDensity(0).get_density_pointer(), 0, &params_, &fire_));
+  }
+  if (fixed_direction_.size() == Nall) {
     // the constructor's getDF_DX already reflects the force: restart with the direction in place
     check(cdft_fire_set_fixed_direction(fire_, fixed_direction_.memptr(), 0));
-    check(cdft_fire_reset(fire_, s->getDensity().get_density_pointer(), 0));
+    check(cdft_fire_reset(fire_, all_densities(*dft_).memptr(), 0));
   }
 }
 void fireMinimizer2::reset() {
   if (!fire_) return ensure();
-  Species* s = dft_->getSpecies(0);
-  check(cdft_fire_set_fixed_direction(
-      fire_, fixed_direction_.size() == s->getDensity().Ntot() ? fixed_direction_.memptr() : nullptr, 0));
-  const int rc = cdft_fire_reset(fire_, s->getDensity().get_density_pointer(), 0);
+  const long Nall = dft_->get_lattice().Ntot() * dft_->getNumberOfSpecies();
+  check(cdft_fire_set_fixed_direction(fire_, fixed_direction_.size() == Nall ? fixed_direction_.memptr() : nullptr,
+                                      0));
+  const int rc = cdft_fire_reset(fire_, all_densities(*dft_).memptr(), 0);
   if (rc < 0) check(rc);
   if (rc == CDFT_ETA_TOO_LARGE) throw Eta_Too_Large_Exception();
   sync_species();
@@ -1053,7 +1145,7 @@ int fireMinimizer2::run_to_raw_tolerance(long maxSteps, double tol) {
   return rep_.steps;
 }
 void fireMinimizer2::density(DFT_Vec& out) const {
-  out.resize(dft_->get_lattice().Ntot());
+  out.resize(dft_->get_lattice().Ntot() * dft_->getNumberOfSpecies());
   check(cdft_fire_get(fire_, out.memptr(), nullptr, nullptr, nullptr));
 }
 

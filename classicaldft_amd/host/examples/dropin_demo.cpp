@@ -26,6 +26,8 @@
 //                           a_vdw of a first object (kernel generated, cache written) and of a second one (cache
 //                           read: 1), F of a 16^3 DFT with the cached kernel
 //   mix <F> <sum dF_a> <sum dF_b>  FMT over two FMT_Species (d = 1 and 0.6, White Bear I, 16 x 16 x 24, dx = 0.1)
+//   mix_dft <F> <F_id> <sum dF_a> <sum dF_b>  a DFT of two FMT_Species (mu 0.5 / 0.2, a field on the first)
+//   mix_fire <F> <fmax_raw> <N_a> <N_b>      after 20 fireMinimizer2 steps over both species
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -290,5 +292,28 @@ int main() {
   smb.zeroForce();
   const double Fm = fmtm.calculateFreeEnergyAndDerivatives(both);
   std::printf("mix %.17g %.17g %.17g\n", Fm, sma.getDF().accu(), smb.getDF().accu());
+
+  // the same two species in a DFT (mu 0.5 and 0.2, a field on the first): one evaluation, then 20 fireMinimizer2
+  // steps over both species
+  Density dda(0.1, Lm), ddb(0.1, Lm);
+  DFT_Vec field(dda.Ntot());
+  for (long i = 0; i < dda.Ntot(); ++i) {
+    dda.set(i, 0.3 * (1.0 + 0.1 * (2.0 * splitmix_u(63, i) - 1.0)));
+    ddb.set(i, 0.4 * (1.0 + 0.1 * (2.0 * splitmix_u(64, i) - 1.0)));
+    field.set(i, 0.2 * (2.0 * splitmix_u(65, i) - 1.0));
+  }
+  dda.set_external_field(field);
+  FMT_Species sda(dda, 1.0, 0.5), sdb(ddb, 0.6, 0.2);
+  WhiteBearI fmtd;
+  DFT dftm(&sda);
+  dftm.addSpecies(&sdb);
+  dftm.addHardCoreContribution(&fmtd);
+  const double Fd = dftm.calculateFreeEnergyAndDerivatives(false);
+  std::printf("mix_dft %.17g %.17g %.17g %.17g\n", Fd, dftm.get_f_id(), sda.getDF().accu(), sdb.getDF().accu());
+  fireMinimizer2 firem(&dftm);
+  firem.set_snapshot_interval(0);
+  for (int i = 0; i < 20; ++i) firem.step();
+  std::printf("mix_fire %.17g %.17g %.17g %.17g\n", firem.getF(), firem.get_raw_force_max(), dda.getNumberAtoms(),
+              ddb.getNumberAtoms());
   return 0;
 }

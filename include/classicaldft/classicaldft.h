@@ -466,8 +466,13 @@ class DFT {
   double get_f_ext() const { return F_ext_; }
   double get_f_hs() const { return F_hs_; }
   double get_f_mf() const { return F_mf_; }
-  // The GPU system holding this functional (built on first use).
+  // The GPU system holding this functional (built on first use; one species).
   cdft_system system() const;
+  // Several species: the hard-sphere mixture holding this functional (cdft_mixture, built on first use): each
+  // species' ideal gas, external field and mu beside the mixture's FMT (src/DFT.cpp:360-425).  FMT species only,
+  // with no mean-field interaction (the reference's between-species term throws, src/Interaction.cpp:110), no
+  // fixed mass and no fixed boundary.
+  cdft_mixture mixture() const;
   void set_precision(int p);
 
  private:
@@ -479,6 +484,8 @@ class DFT {
   mutable double sys_mu_ = NAN, sys_mass_ = NAN, sys_mfs_ = NAN;
   mutable int sys_bnd_ = -1, sys_aof_ = -1;
   mutable unsigned sys_field_ = ~0u;
+  mutable cdft_mixture mix_ = nullptr;
+  mutable std::vector<double> mix_key_;  // diameters, mu and field versions the mixture was built with
   double F_id_ = 0, F_ext_ = 0, F_hs_ = 0, F_mf_ = 0;
   bool full_hessian_ = true, use_squared_matrix_ = false;
   void invalidate() const;
@@ -608,7 +615,8 @@ class fireMinimizer2 : public Minimizer {
   void set_snapshot_interval(int n) { snapshot_every_ = n; }
   // Runs FIRE steps on the GPU until max |dF/drho|/dV <= tol.
   int run_to_raw_tolerance(long maxSteps, double tol);
-  void density(DFT_Vec& out) const override;  // current density (host copy)
+  // current density (host copy); with several species they follow one another, species 0 first
+  void density(DFT_Vec& out) const override;
 
  private:
   void ensure();

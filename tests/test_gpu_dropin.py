@@ -287,3 +287,27 @@ def test_mixture_through_cpp_api(demo):
     assert rel(got[0], F) <= 1e-11
     for s in range(2):
         assert abs(got[1 + s] - dF[s].sum()) <= 1e-11 * np.abs(dF[s]).sum()
+
+
+def test_mixture_dft_and_fire_through_cpp_api(demo):
+    """DFT with two FMT_Species (DFT::calculateFreeEnergyAndDerivatives over several species, src/DFT.cpp:360-425)
+    and fireMinimizer2 over both (src/Minimizer.cpp:160-327) against the oracle's mixture DFT and FIRE."""
+    shape, dx, ds = (16, 16, 24), 0.1, [1.0, 0.6]
+    rho = np.stack([seeded(shape, 0.3, 63, 0.1), seeded(shape, 0.4, 64, 0.1)])
+    field = 0.2 * (2 * oracle.splitmix64_uniform(int(np.prod(shape)), 65).reshape(shape) - 1)
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    o = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=[0.5, 0.2], vext=[field, None])
+    parts, dF = o.eval(rho)
+    got = demo["mix_dft"]
+    assert rel(got[0], parts[0]) <= 1e-11 and rel(got[1], parts[1]) <= 1e-12
+    for s in range(2):
+        assert abs(got[2 + s] - dF[s].sum()) <= 1e-11 * np.abs(dF[s]).sum()
+    f = oracle.Fire(o, rho)
+    for _ in range(20):
+        rc, info = f.step()
+    got = demo["mix_fire"]
+    assert rel(got[0], info["F"]) <= 1e-10
+    assert rel(got[1], info["fmax_raw"]) <= 1e-8
+    r = f.state()[0]
+    for s in range(2):
+        assert rel(got[2 + s], r[s].sum() * dx ** 3) <= 1e-11
