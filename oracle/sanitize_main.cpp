@@ -2,7 +2,7 @@
 //
 // Built together with oracle.cpp under AddressSanitizer and UndefinedBehaviorSanitizer by
 // tests/test_sanitizers.py (g++ -fsanitize=address,undefined); it walks every major entry point of the oracle on
-// small grids (3D FFTs, weight generation, the five FMT models, a two-species mixture, the whole DFT with mean field, external field and
+// small grids (3D FFTs, weight generation, the five FMT models, a two-species mixture with its DFT and FIRE, the whole DFT with mean field, external field and
 // constraints, the Hessian, AO, FIRE2, DDFT, open DDFT, log-det and eigenvalue objectives) so that an out-of-bounds
 // access, a use after free or undefined arithmetic anywhere on those paths stops the run.  Prints one line per
 // stage and "sanitize: ok" at the end.
@@ -36,6 +36,12 @@ int orc_dft_log_det(orc_dft* h, const double* rho, int order, double lam_max, do
                     int num_samples, int has_zero, int full, int squared, double* log_det, double* variance);
 double orc_dft_eig_objective(orc_dft* h, const double* rho, const double* x, const double* vshift, double scale,
                              int change_sign, int full, int squared, double* df);
+struct orc_mix;
+orc_mix* orc_mix_create(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, int S,
+                        const double* hsd, const double* weights, int ideal, const double* mu, const double* vext);
+void orc_mix_destroy(orc_mix* h);
+int orc_mix_eval(orc_mix* h, const double* rho, double* dF, double* parts);
+orc_fire* orc_mix_fire_create(orc_mix* h, const double* rho0, double dt, double dt_max);
 orc_fire* orc_fire_create(orc_dft* h, const double* rho0, double dt, double dt_max);
 int orc_fire_step(orc_fire* h, double* info);
 void orc_fire_destroy(orc_fire* h);
@@ -109,6 +115,20 @@ int main() {
            "mixture eval");
     expect(std::isfinite(F2), "mixture F finite");
     std::printf("fmt: two-species mixture\n");
+    // the mixture's DFT (ideal gas, mu, a field on the second species) and five FIRE2 steps over both species
+    const double mu2[2] = {0.3, -0.2};
+    std::vector<double> v2(2 * (size_t)N, 0.0), parts(5);
+    for (int i = 0; i < N; ++i) v2[N + i] = 0.2 * std::cos(0.3 * i);
+    orc_mix* mx = orc_mix_create(3, 1.0, -1.0, Nx, Ny, Nz, dx, dx, dx, 2, hsd2, nullptr, 1, mu2, v2.data());
+    expect(orc_mix_eval(mx, r2.data(), d2.data(), parts.data()) == 0, "mixture DFT eval");
+    expect(std::isfinite(parts[0]), "mixture DFT F finite");
+    orc_fire* fm = orc_mix_fire_create(mx, r2.data(), 1e-3, 1e-2);
+    double info[8];
+    for (int k = 0; k < 5; ++k) expect(orc_fire_step(fm, info) != 2, "mixture fire step");
+    expect(std::isfinite(info[0]), "mixture fire F finite");
+    orc_fire_destroy(fm);
+    orc_mix_destroy(mx);
+    std::printf("fmt: two-species mixture DFT and FIRE\n");
   }
 
   std::vector<double> vext(N);
