@@ -117,6 +117,9 @@ def lib():
         L.cdft_mixture_eval.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_mixture_fire_create.argtypes = [_p, _p, _i, ctypes.POINTER(FireParams), ctypes.POINTER(_p)]
         L.cdft_system_stream.argtypes = [_p, ctypes.POINTER(_p)]
+        L.cdft_system_set_stream.argtypes = [_p, _p]
+        L.cdft_set_device.argtypes = [_i]
+        L.cdft_get_device.argtypes = [ctypes.POINTER(_i)]
         L.cdft_system_a_vdw.argtypes = [_p, ctypes.POINTER(_d)]
         L.cdft_system_layout.argtypes = [_p, ctypes.POINTER(_i), ctypes.POINTER(_i)]
         L.cdft_system_get_weights.argtypes = [_p, _p]
@@ -228,6 +231,17 @@ def _out(a, n, shape):
 def device_count():
     n = _i(0)
     _check(lib().cdft_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def set_device(device):
+    """cdft_set_device: the device on which this thread creates its systems."""
+    _check(lib().cdft_set_device(int(device)))
+
+
+def get_device():
+    n = _i(0)
+    _check(lib().cdft_get_device(ctypes.byref(n)))
     return n.value
 
 
@@ -433,6 +447,10 @@ class System:
         s = _p()
         _check(lib().cdft_system_stream(self.h, ctypes.byref(s)))
         return s.value or 0
+
+    def set_stream(self, hip_stream):
+        """cdft_system_set_stream: later work runs on hip_stream (an address; 0 / None: the system's own)."""
+        _check(lib().cdft_system_set_stream(self.h, _p(hip_stream or None)))
 
     def chem_potential(self):
         """Species::getChemPotential after the last evaluation: the fixed-mass multiplier (or mu); (nb,) for

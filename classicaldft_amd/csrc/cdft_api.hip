@@ -366,7 +366,8 @@ void create_system(const cdft_system_desc* desc, cdft_system* out, int nb, const
   s->trace_ok = allow_trace;
   HIPCHK(hipGetDevice(&s->device));
   scratch_setup(s->device);
-  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&s->own_stream, hipStreamNonBlocking));
+  s->stream = s->own_stream;
   Geo& g = s->g;
   g.Nx = desc->Nx;
   g.Ny = desc->Ny;
@@ -1209,12 +1210,8 @@ struct cdft_mixture_s {
   std::vector<double> hsd;
   int nm = 0;                   // measure arrays (16 with the tensor, 10 without)
   DevBuf Mx, partials, flag, results;
-  ~cdft_mixture_s() {
-    for (size_t i = 1; i < sp.size(); ++i)
-      if (sp[i]) {
-        sp[i]->stream = nullptr;  // the first member's stream, destroyed with it
-        delete sp[i];
-      }
+  ~cdft_mixture_s() {  // the later members run on the first member's stream, destroyed with it
+    for (size_t i = 1; i < sp.size(); ++i) delete sp[i];
     if (!sp.empty()) delete sp[0];
   }
 };
@@ -1249,8 +1246,9 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
     x->sp.push_back(s);
     x->hsd.push_back(d->hsd[i]);
     if (i > 0) {  // one stream for the whole mixture
-      HIPCHK(hipStreamSynchronize(s->stream));
-      HIPCHK(hipStreamDestroy(s->stream));
+      HIPCHK(hipStreamSynchronize(s->own_stream));
+      HIPCHK(hipStreamDestroy(s->own_stream));
+      s->own_stream = nullptr;
       s->stream = x->sp[0]->stream;
     }
   }
@@ -2087,6 +2085,34 @@ int cdft_system_stream(cdft_system sys, void** hip_stream) {
   return guarded([&] {
     REQUIRE(sys && hip_stream, "null argument");
     *hip_stream = (void*)sys->stream;
+    return CDFT_OK;
+  });
+}
+
+int cdft_system_set_stream(cdft_system sys, void* hip_stream) {
+  return guarded([&] {
+    REQUIRE(sys, "null argument");
+    REQUIRE(sys->own_stream, "the members of a mixture run on the mixture's stream");
+    HIPCHK(hipStreamSynchronize(sys->stream));  // the work already enqueued ends before the switch
+    sys->stream = hip_stream ? (hipStream_t)hip_stream : sys->own_stream;
+    return CDFT_OK;
+  });
+}
+
+int cdft_set_device(int device) {
+  return guarded([&] {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    REQUIRE(device >= 0 && device < n, "no such device");
+    HIPCHK(hipSetDevice(device));
+    return CDFT_OK;
+  });
+}
+
+int cdft_get_device(int* device) {
+  return guarded([&] {
+    REQUIRE(device, "null argument");
+    HIPCHK(hipGetDevice(device));
     return CDFT_OK;
   });
 }

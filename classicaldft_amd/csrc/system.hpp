@@ -175,7 +175,8 @@ using namespace cdft_impl;
 struct cdft_system_s {
   cdft_system_desc d{};
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where the system's work runs: own_stream, or one the caller set
+  hipStream_t own_stream = nullptr;  // created with the system (null for a mixture's later members)
   Geo g{};
   long N = 0, Nkp = 0, nrows = 0;
   int m = 0, nconv = 0;
@@ -228,7 +229,7 @@ struct cdft_system_s {
 
   ~cdft_system_s() {
     for (auto e : ev) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
   void mark(const char* name) {

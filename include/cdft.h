@@ -161,6 +161,12 @@ typedef struct cdft_potential_desc {
 
 const char* cdft_last_error(void);
 int cdft_device_count(int* n);
+/* The device on which the calling thread creates systems, mixtures and the other handles (each handle then stays
+   on its device; the calling thread's device is the HIP current device).  No counterpart in the reference, which
+   runs on the CPU; the device half of a context (device, stream) for callers without their own HIP runtime.
+   CDFT_BAD_ARG for a device that does not exist. */
+int cdft_set_device(int device);
+int cdft_get_device(int* device);
 
 /* Replaces the FMT_Species + FMT + Interaction + DFT construction
    (src/FMT_Species.cpp:95-108 weights; src/Interaction.cpp:32-52 kernel). */
@@ -358,6 +364,11 @@ int cdft_memcpy_d2h(void* dst, const void* src, unsigned long long bytes);
 
 /* Introspection / plumbing. */
 int cdft_system_stream(cdft_system sys, void** hip_stream);
+/* The stream half of that context: the system's later work (evaluations, solvers, graphs they launch) runs on
+   hip_stream (a hipStream_t of the system's device, kept alive by the caller while the system uses it), so that it
+   orders with the caller's own work; NULL returns to the system's own stream.  The work already enqueued finishes
+   first.  No counterpart in the reference.  Not for the member systems of a mixture. */
+int cdft_system_set_stream(cdft_system sys, void* hip_stream);
 int cdft_system_a_vdw(cdft_system sys, double* a_vdw); /* Interaction_Base::getVDWParameter */
 /* The weighted-density arrays the pipeline stores (11 for tensor models, 10 when the weights satisfy the trace
    identity and T_zz is rebuilt per site, 5 without tensor) and those it convolves (plus the mean-field kernel).
