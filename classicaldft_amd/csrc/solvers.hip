@@ -458,7 +458,7 @@ struct cdft_fire_s {
     r->rms_force = rms_force;
     r->steps = steps;
     r->calls = calls;
-    r->monitor = std::fabs(vv / (N * dV));
+    r->monitor = std::fabs(vv / (al.nseg * dV));  // |vv| / the lattice volume (Minimizer.h:105), any species count
   }
 };
 

@@ -153,6 +153,8 @@ def test_mixture_fire_replays_the_oracle(cdft):
         assert close(a["F"], b["F"], 1e-10), (i, a["F"], b["F"])
         assert close(a["dt"], b["dt"], 1e-12) and close(a["alpha"], b["alpha"], 1e-12)
         assert close(a["fmax"], b["fmax_alias"], 1e-9) and close(a["fmax_raw"], b["fmax_raw"], 1e-9)
+        # fireMinimizer2::get_convergence_monitor = |vv| / V, the lattice volume (include/Minimizer.h:105)
+        assert close(b["monitor"], abs(a["vv"]) / (np.prod(shape) * dx ** 3), 1e-9)
     r0 = fo.state()[0]
     r1 = fg.state()[0]
     assert r1.shape == (2,) + shape
