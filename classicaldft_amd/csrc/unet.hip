@@ -30,61 +30,93 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int wrap(int i, int D) { return i < 0 ? i + D : (i >= D ? i - D : i); }
 
-// Y[p][co] = act(bias[co] + sum_{tap,ci} W[co][tap][ci] X[p + off(tap)][ci]).  One wavefront per 16
-// consecutive positions, all COUT channels; blocks of 4 wavefronts.
-template <int CIN, int COUT, bool RELU>
+// Y[p][co] = act(bias[co] + sum_{tap,ci} W[co][tap][ci] X[p + off(tap)][ci]).  One wavefront per MT tiles of 16
+// consecutive positions, all COUT channels; blocks of 4 wavefronts.  Each weight fragment a wavefront loads feeds
+// the MT tiles' MFMAs and each tap is decoded once for them; every output keeps the same K order (the result does
+// not depend on MT).
+template <int CIN, int COUT, bool RELU, int MT>
 __global__ __launch_bounds__(256) void k_conv3(const bf16* __restrict__ X, const bf16* __restrict__ W,
                                                const float* __restrict__ bias, bf16* __restrict__ Y, int D, long ntiles) {
   static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
   constexpr int K = 27 * CIN, NCB = COUT / 16;
-  const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (tile >= ntiles) return;
+  const long tile0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * MT;
+  if (tile0 >= ntiles) return;
   const int l = threadIdx.x & 63, r = l & 15, g = l >> 4;
   const long P = (long)D * D * D;
-  const long p0 = tile * 16;
-  // row r of the tile: position p0 + r (a tile may span rows and samples on coarse grids)
-  const long b = (p0 + r) / P;
-  const int q = (int)(p0 + r - b * P);
-  const int x = q / (D * D), y = (q / D) % D, z = q % D;
-  const bf16* Xb = X + b * P * CIN;
-  f32x4 acc[NCB];
+  // row r of tile mt: position (tile0 + mt) 16 + r (a tile may span rows and samples on coarse grids)
+  const bf16* Xb[MT];
+  int xs[MT], ys[MT], zs[MT];
+  bool live[MT];
 #pragma unroll
-  for (int cb = 0; cb < NCB; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int mt = 0; mt < MT; ++mt) {
+    live[mt] = tile0 + mt < ntiles;
+    const long p = (live[mt] ? tile0 + mt : tile0) * 16 + r;
+    const long b = p / P;
+    const int q = (int)(p - b * P);
+    xs[mt] = q / (D * D);
+    ys[mt] = (q / D) % D;
+    zs[mt] = q % D;
+    Xb[mt] = X + b * P * CIN;
+  }
+  f32x4 acc[MT][NCB];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) acc[mt][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < K; k0 += 32) {
     const int k = k0 + 8 * g;
-    bf16x8 a;
-    if (k < K) {
-      const int tap = k / CIN, c = k - tap * CIN;
-      const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-      const long src = ((long)wrap(x + dx, D) * D + wrap(y + dy, D)) * D + wrap(z + dz, D);
-      a = *reinterpret_cast<const bf16x8*>(Xb + src * CIN + c);
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) a[j] = (bf16)0.f;
-    }
+    const bool kin = k < K;
+    const int tap = kin ? k / CIN : 0, c = k - tap * CIN;
+    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+    bf16x8 w[NCB];
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
-      bf16x8 w;
-      if (k < K) {
-        w = *reinterpret_cast<const bf16x8*>(W + (long)(cb * 16 + r) * K + k);
+      if (kin) {
+        w[cb] = *reinterpret_cast<const bf16x8*>(W + (long)(cb * 16 + r) * K + k);
       } else {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) w[j] = (bf16)0.f;
+        for (int j = 0; j < 8; ++j) w[cb][j] = (bf16)0.f;
       }
-      acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, acc[cb], 0, 0, 0);
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      bf16x8 a;
+      if (kin) {
+        const long src =
+            ((long)wrap(xs[mt] + dx, D) * D + wrap(ys[mt] + dy, D)) * D + wrap(zs[mt] + dz, D);
+        a = *reinterpret_cast<const bf16x8*>(Xb[mt] + src * CIN + c);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] = (bf16)0.f;
+      }
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb) acc[mt][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w[cb], acc[mt][cb], 0, 0, 0);
     }
   }
 #pragma unroll
-  for (int cb = 0; cb < NCB; ++cb) {
-    const int co = cb * 16 + r;
-    const float bv = bias ? bias[co] : 0.f;
+  for (int mt = 0; mt < MT; ++mt) {
+    if (!live[mt]) continue;
+    const long p0 = (tile0 + mt) * 16;
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-      float v = acc[cb][rr] + bv;
-      if (RELU) v = fmaxf(v, 0.f);
-      Y[(p0 + 4 * g + rr) * COUT + co] = (bf16)v;
+    for (int cb = 0; cb < NCB; ++cb) {
+      const int co = cb * 16 + r;
+      const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        float v = acc[mt][cb][rr] + bv;
+        if (RELU) v = fmaxf(v, 0.f);
+        Y[(p0 + 4 * g + rr) * COUT + co] = (bf16)v;
+      }
     }
   }
+}
+
+// Tiles per wavefront of k_conv3: as many (4, 2, 1) as keep at least 1024 blocks of 4 wavefronts in the launch.
+template <typename F>
+void with_conv_mt(long ntiles, F&& f) {
+  if (ntiles >= 4L * 4 * 1024) f(std::integral_constant<int, 4>{});
+  else if (ntiles >= 2L * 4 * 1024) f(std::integral_constant<int, 2>{});
+  else f(std::integral_constant<int, 1>{});
 }
 
 // Partial weight gradients: part[chunk][co][tap][ci] = sum over the chunk's positions of
@@ -452,12 +484,16 @@ struct cdft_unet_s {
     const long ntiles = npos(kLayers[li].level) / 16;
     dispatch_layer(li, [&](auto ci, auto co) {
       constexpr int CI = decltype(ci)::value, CO = decltype(co)::value;
-      if (relu)
-        hipLaunchKernelGGL((k_conv3<CI, CO, true>), dim3((ntiles + 3) / 4), dim3(256), 0, stream, X, wf[li].as<bf16>(),
-                           w.as<float>() + boff[li], Y, D_, ntiles);
-      else
-        hipLaunchKernelGGL((k_conv3<CI, CO, false>), dim3((ntiles + 3) / 4), dim3(256), 0, stream, X,
-                           wf[li].as<bf16>(), w.as<float>() + boff[li], Y, D_, ntiles);
+      with_conv_mt(ntiles, [&](auto mtc) {
+        constexpr int MT = decltype(mtc)::value;
+        const dim3 grid((ntiles + 4 * MT - 1) / (4 * MT));
+        if (relu)
+          hipLaunchKernelGGL((k_conv3<CI, CO, true, MT>), grid, dim3(256), 0, stream, X, wf[li].as<bf16>(),
+                             w.as<float>() + boff[li], Y, D_, ntiles);
+        else
+          hipLaunchKernelGGL((k_conv3<CI, CO, false, MT>), grid, dim3(256), 0, stream, X, wf[li].as<bf16>(),
+                             w.as<float>() + boff[li], Y, D_, ntiles);
+      });
     });
     HIPCHK(hipGetLastError());
   }
@@ -469,8 +505,11 @@ struct cdft_unet_s {
     dispatch_layer(li, [&](auto ci, auto co) {
       constexpr int CI = decltype(ci)::value, CO = decltype(co)::value;
       if constexpr (CI % 16 == 0)
-        hipLaunchKernelGGL((k_conv3<CO, CI, false>), dim3((ntiles + 3) / 4), dim3(256), 0, stream, dZ,
-                           wt[li].as<bf16>(), nullptr, dX, D_, ntiles);
+        with_conv_mt(ntiles, [&](auto mtc) {
+          constexpr int MT = decltype(mtc)::value;
+          hipLaunchKernelGGL((k_conv3<CO, CI, false, MT>), dim3((ntiles + 4 * MT - 1) / (4 * MT)), dim3(256), 0,
+                             stream, dZ, wt[li].as<bf16>(), nullptr, dX, D_, ntiles);
+        });
     });
     HIPCHK(hipGetLastError());
   }
