@@ -30,3 +30,21 @@ def test_output_array_rejects_read_only():
     a.flags.writeable = False
     with pytest.raises(ValueError):
         C._out(a, 120, (120,))
+
+
+@pytest.mark.parametrize("kw", [
+    {"mu": [0.1]},                                   # one chemical potential for two species
+    {"vext": [None]},                                # one field entry for two species
+    {"vext": [np.zeros(10), None]},                  # a field of the wrong size
+    {"weights": [None]},                             # one weight set for two species
+    {"weights": [np.zeros(11 * 512 - 1), None]},     # a weight set of the wrong size
+])
+def test_mixture_descriptor_checks(kw):
+    """cdft.Mixture validates its per-species arrays before the library is asked for anything."""
+    with pytest.raises(ValueError):
+        C.Mixture((8, 8, 8), 0.1, [1.0, 0.6], **kw)
+
+
+def test_mixture_needs_a_species():
+    with pytest.raises(ValueError):
+        C.Mixture((8, 8, 8), 0.1, [])
