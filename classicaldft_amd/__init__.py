@@ -10,7 +10,7 @@ from .cdft import (  # noqa: F401
     WHITEBEAR_I, WHITEBEAR_II, DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, CdftError, EtaTooLarge, System, Mixture, DeviceArray, Fire,
     DDFT, OpenDDFT, Eigenvalues, Arnoldi, SQUARED_MATRIX, DYNAMIC_MATRIX, dst1_3d, mf_kernel_points, FireBatch, FireGroup, UNet, fft_r2c, fft_c2r, lj_hsd, potential_hsd, potential_vdw, potential_watt,
     POT_LJ, POT_TWF, POT_WHDF, MF_QF, MF_ZERO, MF_LE, MF_QE, MF_LF, MF_GAUSS_E, MF_GAUSS_F,
-    device_count, set_device, get_device, library_path,
+    FIXED_BACKGROUND, HOMOGENEOUS_BOUNDARY, device_count, set_device, get_device, library_path,
 )
 from .formats import (  # noqa: F401,E402
     read_density, write_density, read_points_file, read_weight_cache, write_weight_cache, weight_cache_name,

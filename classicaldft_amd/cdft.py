@@ -13,6 +13,7 @@ FP64, MIXED = 0, 1
 WEIGHTS_ACCURATE, WEIGHTS_REFERENCE = 0, 1  # cdft_weights_mode
 AO_FORCE_EXACT, AO_FORCE_REFERENCE = 0, 1  # cdft_ao_force
 DEVICE_PTRS, ACCUMULATE, FMT_ONLY, NO_IDEAL, SQUARED_MATRIX, DYNAMIC_MATRIX, CENTRAL_DIFFERENCES = 1, 2, 4, 8, 16, 32, 64
+FIXED_BACKGROUND, HOMOGENEOUS_BOUNDARY = 1, 2  # enum cdft_boundary
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -49,7 +50,7 @@ class _Desc(ctypes.Structure):
 class _MixDesc(ctypes.Structure):
     _fields_ = [("Nx", _i), ("Ny", _i), ("Nz", _i), ("dx", _d), ("dy", _d), ("dz", _d), ("model", _i), ("A", _d),
                 ("B", _d), ("nspecies", _i), ("hsd", _p), ("fmt_weights", _p), ("weights_mode", _i), ("ideal", _i),
-                ("mu", _p), ("vext", _p)]
+                ("mu", _p), ("vext", _p), ("fixed_mass", _p), ("boundary", _p)]
 
 
 class _PotDesc(ctypes.Structure):
@@ -115,6 +116,7 @@ def lib():
         L.cdft_mixture_species.argtypes = [_p, ctypes.POINTER(_i)]
         L.cdft_mixture_fmt_eval.argtypes = [_p, _p, _p, _p, _i]
         L.cdft_mixture_eval.argtypes = [_p, _p, _p, _p, _i]
+        L.cdft_mixture_chem_potential.argtypes = [_p, _p]
         L.cdft_mixture_fire_create.argtypes = [_p, _p, _i, ctypes.POINTER(FireParams), ctypes.POINTER(_p)]
         L.cdft_system_stream.argtypes = [_p, ctypes.POINTER(_p)]
         L.cdft_system_set_stream.argtypes = [_p, _p]
@@ -722,10 +724,11 @@ class Mixture:
     FMT::calculateFreeEnergyAndDerivatives over several FMT species (src/FMT.cpp:363-427).  FP64."""
 
     def __init__(self, shape, dx, hsd, model=WHITEBEAR_I, weights=None, A=1.0, B=0.0,
-                 weights_mode=WEIGHTS_ACCURATE, ideal=True, mu=None, vext=None):
+                 weights_mode=WEIGHTS_ACCURATE, ideal=True, mu=None, vext=None, fixed_mass=None, boundary=None):
         """hsd: one diameter per species; weights: None, or one entry per species (None or 11*N cell weights).
-        ideal, mu (one beta*mu per species) and vext (None, or one entry per species: None or N values of
-        beta*V_ext) are the rest of the DFT that eval and FIRE over the mixture use."""
+        ideal, mu (one beta*mu per species), vext (None, or one entry per species: None or N values of
+        beta*V_ext), fixed_mass (None or one value per species, 0: free) and boundary (None or one flag set per
+        species) are the rest of the DFT that eval and FIRE over the mixture use."""
         self.shape = tuple(int(n) for n in shape)
         dxs = (dx, dx, dx) if np.isscalar(dx) else tuple(dx)
         self.dx = dxs
@@ -774,9 +777,20 @@ class Mixture:
                 vp[i] = a.ctypes.data
             self._vptrs = vp
             vptr = ctypes.cast(vp, _p)
+        fptr = bptr = None
+        if fixed_mass is not None:
+            self._fm = np.ascontiguousarray(np.atleast_1d(fixed_mass), dtype=np.float64).reshape(-1)
+            if self._fm.size != self.nspecies:
+                raise ValueError(f"expected {self.nspecies} fixed masses")
+            fptr = self._fm.ctypes.data
+        if boundary is not None:
+            self._bnd = np.ascontiguousarray(np.atleast_1d(boundary), dtype=np.int32).reshape(-1)
+            if self._bnd.size != self.nspecies:
+                raise ValueError(f"expected {self.nspecies} boundary flag sets")
+            bptr = self._bnd.ctypes.data
         d = _MixDesc(self.shape[0], self.shape[1], self.shape[2], dxs[0], dxs[1], dxs[2], int(model), float(A),
                      float(B), self.nspecies, self._hsd.ctypes.data, wptr, int(weights_mode), int(bool(ideal)), mptr,
-                     vptr)
+                     vptr, fptr, bptr)
         h = _p()
         _check(lib().cdft_mixture_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
@@ -826,6 +840,12 @@ class Mixture:
         flags = (ACCUMULATE if accumulate else 0) | (NO_IDEAL if only_fex else 0)
         _check(lib().cdft_mixture_eval(self.h, ctypes.cast(rp, _p), ctypes.cast(fp, _p), _ptr(parts), flags))
         return parts, out
+
+    def chem_potential(self):
+        """cdft_mixture_chem_potential: each species' fixed-mass multiplier after the last evaluation, or its mu."""
+        mu = np.zeros(self.nspecies)
+        _check(lib().cdft_mixture_chem_potential(self.h, _ptr(mu)))
+        return mu
 
     def fmt_eval_device(self, rho_ptrs, dF_ptrs, accumulate=False):
         """Device pointers (one per species): returns F_hs; dF written or added to on the device."""

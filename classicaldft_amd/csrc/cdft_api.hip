@@ -1241,6 +1241,8 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
     sd.ideal = d->ideal;
     sd.mu = d->mu ? d->mu[i] : 0.0;
     sd.vext = d->vext ? d->vext[i] : nullptr;
+    sd.fixed_mass = d->fixed_mass ? d->fixed_mass[i] : 0.0;
+    sd.boundary = d->boundary ? d->boundary[i] : 0;
     cdft_system s = nullptr;
     create_system(&sd, &s, 1, nullptr, nullptr, false);
     x->sp.push_back(s);
@@ -1278,9 +1280,11 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
   const int tensor = m == 11;
   cplx<double>* Mx = x.Mx.as<cplx<double>>();
   const dim3 eg(std::min(div_up(Nkp, 256), 8192)), eb(256);
-  // 1. each species' weighted densities (z spectral, x and y back in real space), summed into the measures
+  // 1. each species' weighted densities (z spectral, x and y back in real space), summed into the measures; with
+  // the DFT terms a fixed-mass species is first rescaled to its mass (Species::beginForceCalculation)
   for (int i = 0; i < S; ++i) {
     cdft_system_s& s = *x.sp[i];
+    if (with_dft) constraints_pre(s, const_cast<double*>(rho[i]));
     forward_weighted(s, rho[i], s.A.as<cplx<double>>(), s.B.as<cplx<double>>(), m);
     hipLaunchKernelGGL(k_mix_accum, eg, eb, 0, s0.stream, s.B.as<cplx<double>>(), Mx, Nkp, Nkp, x.hsd[i], tensor,
                        (int)(i == 0));
@@ -1307,7 +1311,7 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
     fp.dV = dV;
     fp.fmt = 1;
     fp.accumulate = acc;
-    fp.mu = with_dft ? s.d.mu : 0.0;
+    fp.mu = with_dft ? eval_mu(s) : 0.0;
     fp.ideal = with_dft && s.d.ideal && !no_ideal;
     fp.ext = with_dft && s.has_vext;
     cplx<double>* A = s.A.as<cplx<double>>();
@@ -1332,6 +1336,9 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
                          4, s.results.as<double>() + 1, 4 * s.pslots, 8L);
     });
     HIPCHK(hipGetLastError());
+    // Species::endForceCalculation on this species' whole force (boundaries, fixed-mass projection)
+    if (with_dft) constraints_post(s, rho[i], dF[i]);
+    s.last_fixed_mass = with_dft && s.d.fixed_mass > 0;
   }
   double res = 0.0;
   int bad = 0;
@@ -1348,6 +1355,13 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
     g_err = "Eta too large in FMT::dPHI";
     return CDFT_ETA_TOO_LARGE;
   }
+  if (!parts)  // the fixed-mass multipliers need the sums
+    for (int i = 0; i < S && with_dft; ++i)
+      if (x.sp[i]->last_fixed_mass) {
+        HIPCHK(hipMemcpy(sres.data() + 8 * i, x.sp[i]->results.p, sizeof(double) * 8, hipMemcpyDeviceToHost));
+      }
+  for (int i = 0; i < S; ++i)  // mu_ = sum dF rho / fixed mass (src/Species.cpp:99-110)
+    if (x.sp[i]->last_fixed_mass) x.sp[i]->lagrange[0] = sres[8 * i + 7] / x.sp[i]->masses[0];
   if (parts) {
     const double F_hs = res * dV;
     double F_id = 0.0, F_ext = 0.0;  // species by species, as DFT::calculateFreeEnergyAndDerivatives adds them
@@ -1355,7 +1369,7 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
       const cdft_system_s& s = *x.sp[i];
       const double* r = sres.data() + 8 * i;
       if (with_dft && s.d.ideal && !no_ideal) F_id += r[1];
-      if (with_dft) F_ext += csum({s.has_vext ? r[3] * dV : 0.0, -r[4] * dV * s.d.mu});
+      if (with_dft) F_ext += csum({s.has_vext ? r[3] * dV : 0.0, -r[4] * dV * eval_mu(s)});
     }
     parts[0] = csum({F_id, F_hs, F_ext});
     parts[1] = F_id;
@@ -2138,6 +2152,17 @@ int cdft_mixture_destroy(cdft_mixture mix) {
     if (mix) {
       (void)hipStreamSynchronize(mix->sp[0]->stream);
       delete mix;
+    }
+    return (int)CDFT_OK;
+  });
+}
+
+int cdft_mixture_chem_potential(cdft_mixture mix, double* mu) {
+  return guarded([&] {
+    REQUIRE(mix && mu, "null argument");
+    for (size_t i = 0; i < mix->sp.size(); ++i) {
+      const cdft_system_s& s = *mix->sp[i];
+      mu[i] = s.last_fixed_mass ? s.lagrange[0] : s.d.mu;
     }
     return (int)CDFT_OK;
   });

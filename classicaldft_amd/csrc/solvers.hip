@@ -898,7 +898,11 @@ int cdft_fire_get(cdft_fire f, double* rho, double* dF_raw, double* x, double* v
     hipLaunchKernelGGL(k_rho_from_alias, vgrid(f->N), dim3(kThreads), 0, f->sys->stream, f->x.as<double>(),
                        f->rho.as<double>(), f->N, f->al);
     HIPCHK(hipGetLastError());
-    if (!f->mix) constraints_pre(*f->sys, f->rho.as<double>());  // with a fixed mass: the density the next evaluation uses
+    // with a fixed mass: the density the next evaluation uses
+    if (!f->mix) constraints_pre(*f->sys, f->rho.as<double>());
+    else
+      for (int i = 0; i < mixture_species(*f->mix); ++i)
+        constraints_pre(*mixture_member(*f->mix, i), f->rho.as<double>() + i * f->al.nseg);
     if (rho) HIPCHK(hipMemcpyAsync(rho, f->rho.p, b, hipMemcpyDeviceToHost, f->sys->stream));
     if (dF_raw) HIPCHK(hipMemcpyAsync(dF_raw, f->raw.p, b, hipMemcpyDeviceToHost, f->sys->stream));
     if (x) HIPCHK(hipMemcpyAsync(x, f->x.p, b, hipMemcpyDeviceToHost, f->sys->stream));

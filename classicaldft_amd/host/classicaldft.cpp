@@ -543,23 +543,27 @@ void DFT::set_precision(int p) {
 
 cdft_mixture DFT::mixture() const {
   const Lattice& L0 = species_.front()->getLattice();
-  std::vector<double> key, hsd, mu;
+  std::vector<double> key, hsd, mu, mass;
+  std::vector<int> bnd;
   std::vector<const double*> vext;
   for (const Species* s : species_) {
     const auto* fs = dynamic_cast<const FMT_Species*>(s);
     if (!fs || fs->ao_Rp() > 0)
       throw std::runtime_error("a DFT of several species takes hard-sphere FMT species (no AO polymer)");
-    if (s->is_mass_fixed() || s->is_background_fixed() || s->is_boundary_homogeneous())
-      throw std::runtime_error("a DFT of several species takes no fixed mass and no fixed boundary");
     const Lattice& L = s->getLattice();
     if (L.Nx() != L0.Nx() || L.Ny() != L0.Ny() || L.Nz() != L0.Nz() || L.getDX() != L0.getDX() ||
         L.getDY() != L0.getDY() || L.getDZ() != L0.getDZ())
       throw std::runtime_error("the species of a mixture must share one lattice");
     const Density& D = s->getDensity();
     hsd.push_back(fs->getHSD());
-    mu.push_back(s->getChemPotential());
+    // with a fixed mass mu_ holds the last multiplier and the evaluation runs at mu = 0
+    mu.push_back(s->is_mass_fixed() ? 0.0 : s->getChemPotential());
+    mass.push_back(s->is_mass_fixed() ? s->getFixedMass() : 0.0);
+    bnd.push_back((s->is_background_fixed() ? CDFT_FIXED_BACKGROUND : 0) |
+                  (s->is_boundary_homogeneous() ? CDFT_HOMOGENEOUS_BOUNDARY : 0));
     vext.push_back(D.has_external_field() ? D.get_external_field().memptr() : nullptr);
-    key.insert(key.end(), {hsd.back(), mu.back(), (double)D.field_version(), (double)(size_t)s});
+    key.insert(key.end(), {hsd.back(), mu.back(), mass.back(), (double)bnd.back(), (double)D.field_version(),
+                           (double)(size_t)s});
   }
   if (!fmt_) throw std::runtime_error("a DFT of several species needs a hard-core (FMT) contribution");
   if (!interactions_.empty())
@@ -583,6 +587,8 @@ cdft_mixture DFT::mixture() const {
   d.ideal = 1;
   d.mu = mu.data();
   d.vext = vext.data();
+  d.fixed_mass = mass.data();
+  d.boundary = bnd.data();
   check(cdft_mixture_create(&d, &mix_));
   mix_key_ = key;
   return mix_;
@@ -645,10 +651,14 @@ double DFT::calculateFreeEnergyAndDerivatives(bool onlyFex) {
     std::vector<double*> f;
     for (Species* s : species_) {
       s->zeroForce();
+      s->beginForceCalculation();
       r.push_back(s->getDensity().get_density_pointer());
       f.push_back(s->getDF().memptr());
     }
     check(cdft_mixture_eval(m, r.data(), f.data(), parts, onlyFex ? CDFT_NO_IDEAL : 0));
+    std::vector<double> mu(species_.size());
+    check(cdft_mixture_chem_potential(m, mu.data()));
+    for (size_t i = 0; i < species_.size(); ++i) species_[i]->endForceCalculation(mu[i]);
     F_id_ = parts[1];
     F_hs_ = parts[2];
     F_mf_ = 0.0;
@@ -1042,10 +1052,13 @@ void Minimizer::sync_species() {
   const long N = dft_->get_lattice().Ntot();
   DFT_Vec rho(N * S);
   density(rho);
-  if (S > 1) {  // species after species
+  if (S > 1) {  // species after species, with the fixed-mass multipliers of the last evaluation
+    std::vector<double> mu(S);
+    check(cdft_mixture_chem_potential(dft_->mixture(), mu.data()));
     for (int i = 0; i < S; ++i) {
       Species* s = dft_->getSpecies(i);
       for (long j = 0; j < N; ++j) s->set_density(j, rho.get(i * N + j));
+      s->endForceCalculation(mu[i]);
     }
     return;
   }

@@ -28,6 +28,8 @@
 //   mix <F> <sum dF_a> <sum dF_b>  FMT over two FMT_Species (d = 1 and 0.6, White Bear I, 16 x 16 x 24, dx = 0.1)
 //   mix_dft <F> <F_id> <sum dF_a> <sum dF_b>  a DFT of two FMT_Species (mu 0.5 / 0.2, a field on the first)
 //   mix_fire <F> <fmax_raw> <N_a> <N_b>      after 20 fireMinimizer2 steps over both species
+//   mix_mass <F> <mu_b> <N_b>                the second species with a fixed mass (0.42 V); then
+//   mix_mass_fire <F> <mu_b> <N_b>           after 10 fireMinimizer2 steps
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -315,5 +317,24 @@ int main() {
   for (int i = 0; i < 20; ++i) firem.step();
   std::printf("mix_fire %.17g %.17g %.17g %.17g\n", firem.getF(), firem.get_raw_force_max(), dda.getNumberAtoms(),
               ddb.getNumberAtoms());
+
+  // the second species with a fixed mass (Species::setFixedMass): one evaluation and 10 fireMinimizer2 steps
+  Density dfa(0.1, Lm), dfb(0.1, Lm);
+  for (long i = 0; i < dfa.Ntot(); ++i) {
+    dfa.set(i, 0.3 * (1.0 + 0.1 * (2.0 * splitmix_u(66, i) - 1.0)));
+    dfb.set(i, 0.4 * (1.0 + 0.1 * (2.0 * splitmix_u(67, i) - 1.0)));
+  }
+  FMT_Species sfa(dfa, 1.0, 0.3), sfb(dfb, 0.6, 0.5);
+  sfb.setFixedMass(0.42 * dfb.getVolume());
+  WhiteBearI fmtf;
+  DFT dftf(&sfa);
+  dftf.addSpecies(&sfb);
+  dftf.addHardCoreContribution(&fmtf);
+  const double Ff = dftf.calculateFreeEnergyAndDerivatives(false);
+  std::printf("mix_mass %.17g %.17g %.17g\n", Ff, sfb.getChemPotential(), dfb.getNumberAtoms());
+  fireMinimizer2 firef(&dftf);
+  firef.set_snapshot_interval(0);
+  for (int i = 0; i < 10; ++i) firef.step();
+  std::printf("mix_mass_fire %.17g %.17g %.17g\n", firef.getF(), sfb.getChemPotential(), dfb.getNumberAtoms());
   return 0;
 }

@@ -400,6 +400,11 @@ typedef struct cdft_mixture_desc {
   const double* mu;                  /* host, NULL or nspecies values of beta*mu */
   const double* const* vext;         /* host, NULL or nspecies pointers (NULL entries: no field) to N values of
                                         beta*V_ext */
+  /* Species constraints per species (include/Species.h:23-35, src/Species.cpp:64-112), as the desc fields of one
+     system: fixed_mass[s] > 0 rescales rho_s to that mass before each evaluation and projects its force with the
+     multiplier that cdft_mixture_chem_potential returns; boundary[s]: enum cdft_boundary flags. */
+  const double* fixed_mass;          /* host, NULL or nspecies values (0: free) */
+  const int* boundary;               /* host, NULL or nspecies flag sets */
 } cdft_mixture_desc;
 
 /* Replaces the construction of the species' FMT_Species weights (src/FMT_Species.cpp:95-108, 310-496) for a
@@ -417,6 +422,9 @@ int cdft_mixture_fmt_eval(cdft_mixture mix, const double* const* rho, double* co
    may be NULL) receives {F, F_id, F_hs, F_mf = 0, F_ext} summed over species.  The reference's mean field between
    species is not implemented there (src/Interaction.cpp:110) and is not here. */
 int cdft_mixture_eval(cdft_mixture mix, const double* const* rho, double* const* dF, double* parts, int flags);
+/* Species::getChemPotential of every species after the last cdft_mixture_eval (or FIRE step): the fixed-mass
+   multiplier of a species with a fixed mass, its mu otherwise (nspecies values). */
+int cdft_mixture_chem_potential(cdft_mixture mix, double* mu);
 /* fireMinimizer2 over all species of the mixture (src/Minimizer.cpp:160-327: one FIRE2 state, P, norms and the
    mixing over the species' alias variables together), on cdft_mixture_eval.  The returned handle is driven by
    cdft_fire_step / _run / _destroy; cdft_fire_get and cdft_fire_reset take nspecies x N values (species after

@@ -469,9 +469,9 @@ class DFT {
   // The GPU system holding this functional (built on first use; one species).
   cdft_system system() const;
   // Several species: the hard-sphere mixture holding this functional (cdft_mixture, built on first use): each
-  // species' ideal gas, external field and mu beside the mixture's FMT (src/DFT.cpp:360-425).  FMT species only,
-  // with no mean-field interaction (the reference's between-species term throws, src/Interaction.cpp:110), no
-  // fixed mass and no fixed boundary.
+  // species' ideal gas, external field, mu and constraints (fixed mass, fixed boundary) beside the mixture's FMT
+  // (src/DFT.cpp:360-425).  FMT species only, with no mean-field interaction (the reference's between-species term
+  // throws, src/Interaction.cpp:110).
   cdft_mixture mixture() const;
   void set_precision(int p);
 

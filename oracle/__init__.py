@@ -89,7 +89,8 @@ def lib():
         L.orc_dft_lagrange.restype = _d
         L.orc_fire_create.argtypes = [_p, _p, _d, _d]
         L.orc_fire_create.restype = _p
-        L.orc_mix_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _i, _p, _p, _i, _p, _p]
+        L.orc_mix_create.argtypes = [_i, _d, _d, _i, _i, _i, _d, _d, _d, _i, _p, _p, _i, _p, _p, _p, _p]
+        L.orc_mix_chem_potential.argtypes = [_p, _p]
         L.orc_mix_create.restype = _p
         L.orc_mix_destroy.argtypes = [_p]
         L.orc_mix_eval.argtypes = [_p, _p, _p, _p]
@@ -420,7 +421,10 @@ class DFT:
 class MixDFT:
     """Whole-DFT oracle of a hard-sphere mixture: per species ideal gas, V_ext and -mu N, and the mixture FMT."""
 
-    def __init__(self, shape, dx, hsd, model=WBI, A=1.0, B=0.0, weights=None, ideal=True, mu=None, vext=None):
+    def __init__(self, shape, dx, hsd, model=WBI, A=1.0, B=0.0, weights=None, ideal=True, mu=None, vext=None,
+                 fixed_mass=None, boundary=None):
+        """fixed_mass: None or one value per species (> 0: Species::setFixedMass); boundary: None or one flag set
+        per species (1 fixed background, 2 homogeneous boundary)."""
         self.species_shape = tuple(shape)
         self._hsd = np.ascontiguousarray(np.atleast_1d(hsd), dtype=np.float64).reshape(-1)
         self.S = int(self._hsd.size)
@@ -433,8 +437,18 @@ class MixDFT:
         if vext is not None:  # per species: None (no field) or N values
             self._vext = np.stack([np.zeros(self.species_shape) if v is None else np.asarray(v, dtype=np.float64)
                                    .reshape(self.species_shape) for v in vext])
+        self._fm = None if fixed_mass is None else np.ascontiguousarray(fixed_mass, dtype=np.float64)
+        self._bnd = None if boundary is None else np.ascontiguousarray(boundary, dtype=np.int32)
         self.h = lib().orc_mix_create(model, A, B, *self.species_shape, *dxs, self.S, _ptr(self._hsd), _ptr(w),
-                                      int(bool(ideal)), _ptr(self._mu), _ptr(self._vext))
+                                      int(bool(ideal)), _ptr(self._mu), _ptr(self._vext), _ptr(self._fm),
+                                      _ptr(self._bnd))
+
+    @property
+    def chem_potential(self):
+        """Each species' mu_ after the last evaluation (the fixed-mass multiplier, or its mu)."""
+        out = np.zeros(self.S)
+        lib().orc_mix_chem_potential(self.h, _ptr(out))
+        return out
 
     def eval(self, rho):
         rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(self.shape)

@@ -1574,7 +1574,7 @@ void end_force_calculation(DftState& s, const double* rho, double* dF) {
 
 // DFT::calculateFreeEnergyAndDerivatives (src/DFT.cpp:360-425) with Species::beginForceCalculation
 // (src/Species.cpp:64-71): a fixed mass rescales rho in place to that mass and sets mu to zero.
-int dft_eval(DftState& s, double* rho, double* dF) {
+int dft_eval(DftState& s, double* rho, double* dF, bool end_forces = true) {
   const Grid& g = s.fmt.g;
   const long N = g.N(), Nk = g.Nk();
   const double dV = s.fmt.dx * s.fmt.dy * s.fmt.dz;
@@ -1624,7 +1624,7 @@ int dft_eval(DftState& s, double* rho, double* dF) {
     if (!s.vext.empty()) dF[i] += s.vext[i] * dV;
     dF[i] += -mu_eff * dV;
   }
-  end_force_calculation(s, rho, dF);
+  if (end_forces) end_force_calculation(s, rho, dF);
   NSum F;
   F.add(Fid.sum());
   F.add(Fhs);
@@ -1656,13 +1656,15 @@ struct MixDft {
 int mix_eval(MixDft& m, double* rho, double* dF) {
   const long N = m.g.N();
   NSum Fid, Fext;
-  for (int s = 0; s < m.S; ++s) {  // writes dF_s
-    dft_eval(m.sp[s], rho + s * N, dF + s * N);
+  for (int s = 0; s < m.S; ++s) {  // a fixed mass rescales rho_s in place; writes dF_s
+    dft_eval(m.sp[s], rho + s * N, dF + s * N, false);
     Fid.add(m.sp[s].parts[1]);
     Fext.add(m.sp[s].parts[4]);
   }
   double Fhs = 0.0;
   if (fmt_eval_mix(m.g, m.model, m.dx * m.dy * m.dz, m.S, m.hsd.data(), m.wk, rho, dF, &Fhs)) return 1;
+  // Species::endForceCalculation of every species on its whole force (src/DFT.cpp:423-424)
+  for (int s = 0; s < m.S; ++s) end_force_calculation(m.sp[s], rho + s * N, dF + s * N);
   NSum F;
   F.add(Fid.sum());
   F.add(Fhs);
@@ -2500,7 +2502,8 @@ int orc_fire_step(orc_fire* h, double* info) {
 // null).
 struct orc_mix;
 orc_mix* orc_mix_create(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, int S,
-                        const double* hsd, const double* weights, int ideal, const double* mu, const double* vext) {
+                        const double* hsd, const double* weights, int ideal, const double* mu, const double* vext,
+                        const double* fixed_mass, const int* bnd) {
   auto* m = new MixDft();
   m->g = Grid{Nx, Ny, Nz};
   m->model = make_model(kind, A, B);
@@ -2521,8 +2524,15 @@ orc_mix* orc_mix_create(int kind, double A, double B, int Nx, int Ny, int Nz, do
     d.ideal = ideal != 0;
     d.mu = mu ? mu[sp] : 0.0;
     if (vext) d.vext.assign(vext + (size_t)sp * N, vext + (size_t)(sp + 1) * N);
+    d.fixed_mass = fixed_mass ? fixed_mass[sp] : 0.0;
+    d.bnd = bnd ? bnd[sp] : 0;
   }
   return reinterpret_cast<orc_mix*>(m);
+}
+// Species::getChemPotential of each species after the last evaluation (the fixed-mass multiplier, or mu).
+void orc_mix_chem_potential(orc_mix* h, double* mu) {
+  auto* m = reinterpret_cast<MixDft*>(h);
+  for (int sp = 0; sp < m->S; ++sp) mu[sp] = m->sp[sp].fixed_mass > 0.0 ? m->sp[sp].lagrange : m->sp[sp].mu;
 }
 void orc_mix_destroy(orc_mix* h) { delete reinterpret_cast<MixDft*>(h); }
 // rho, dF: S x N; parts[5] = F, F_id, F_hs, 0, F_ext.  1 on eta too large.
@@ -2562,6 +2572,15 @@ orc_fire* orc_mix_fire_create(orc_mix* h, const double* rho0, double dt, double 
 void orc_fire_get(orc_fire* h, double* rho, double* x, double* v, double* dF_raw) {
   auto* f = reinterpret_cast<Fire*>(h);
   f->set_rho_from_alias();
+  if (f->mix)  // each species with a fixed mass: the density its next evaluation uses
+    for (int sp = 0; sp < f->mix->S; ++sp)
+      if (f->mix->sp[sp].fixed_mass > 0.0) {
+        double* r = f->rho.data() + sp * f->nseg;
+        double n = 0.0;
+        for (long i = 0; i < f->nseg; ++i) n += r[i];
+        const double sc = f->mix->sp[sp].fixed_mass / (n * f->dV);
+        for (long i = 0; i < f->nseg; ++i) r[i] *= sc;
+      }
   if (f->dft && f->dft->fixed_mass > 0.0) {  // the density the next force evaluation uses (Species::beginForceCalculation)
     double n = 0.0;
     for (long i = 0; i < f->N; ++i) n += f->rho[i];

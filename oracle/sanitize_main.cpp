@@ -38,7 +38,8 @@ double orc_dft_eig_objective(orc_dft* h, const double* rho, const double* x, con
                              int change_sign, int full, int squared, double* df);
 struct orc_mix;
 orc_mix* orc_mix_create(int kind, double A, double B, int Nx, int Ny, int Nz, double dx, double dy, double dz, int S,
-                        const double* hsd, const double* weights, int ideal, const double* mu, const double* vext);
+                        const double* hsd, const double* weights, int ideal, const double* mu, const double* vext,
+                        const double* fixed_mass, const int* bnd);
 void orc_mix_destroy(orc_mix* h);
 int orc_mix_eval(orc_mix* h, const double* rho, double* dF, double* parts);
 orc_fire* orc_mix_fire_create(orc_mix* h, const double* rho0, double dt, double dt_max);
@@ -119,7 +120,8 @@ int main() {
     const double mu2[2] = {0.3, -0.2};
     std::vector<double> v2(2 * (size_t)N, 0.0), parts(5);
     for (int i = 0; i < N; ++i) v2[N + i] = 0.2 * std::cos(0.3 * i);
-    orc_mix* mx = orc_mix_create(3, 1.0, -1.0, Nx, Ny, Nz, dx, dx, dx, 2, hsd2, nullptr, 1, mu2, v2.data());
+    const double fm2[2] = {0.0, 0.4 * Nx * Ny * Nz * dx * dx * dx * 0.5};  // the second species' mass fixed
+    orc_mix* mx = orc_mix_create(3, 1.0, -1.0, Nx, Ny, Nz, dx, dx, dx, 2, hsd2, nullptr, 1, mu2, v2.data(), fm2, nullptr);
     expect(orc_mix_eval(mx, r2.data(), d2.data(), parts.data()) == 0, "mixture DFT eval");
     expect(std::isfinite(parts[0]), "mixture DFT F finite");
     orc_fire* fm = orc_mix_fire_create(mx, r2.data(), 1e-3, 1e-2);

@@ -311,3 +311,25 @@ def test_mixture_dft_and_fire_through_cpp_api(demo):
     r = f.state()[0]
     for s in range(2):
         assert rel(got[2 + s], r[s].sum() * dx ** 3) <= 1e-11
+
+
+def test_mixture_fixed_mass_through_cpp_api(demo):
+    """Species::setFixedMass on one species of a two-species DFT: F, the multiplier recorded as that species'
+    chemical potential and its mass, after one evaluation and after 10 fireMinimizer2 steps, against the oracle."""
+    shape, dx, ds = (16, 16, 24), 0.1, [1.0, 0.6]
+    V = np.prod(shape) * dx ** 3
+    rho = np.stack([seeded(shape, 0.3, 66, 0.1), seeded(shape, 0.4, 67, 0.1)])
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    o = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=[0.3, 0.5], fixed_mass=[0.0, 0.42 * V])
+    parts, _ = o.eval(rho)
+    got = demo["mix_mass"]
+    assert rel(got[0], parts[0]) <= 1e-11
+    assert rel(got[1], o.chem_potential[1]) <= 1e-10
+    assert rel(got[2], 0.42 * V) <= 1e-12
+    f = oracle.Fire(o, rho)
+    for _ in range(10):
+        rc, info = f.step()
+    got = demo["mix_mass_fire"]
+    assert rel(got[0], info["F"]) <= 1e-10
+    assert rel(got[1], o.chem_potential[1]) <= 1e-8
+    assert rel(got[2], 0.42 * V) <= 1e-12

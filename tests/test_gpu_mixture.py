@@ -181,3 +181,37 @@ def test_mixture_fire_reaches_the_bulk_mixture(cdft):
         assert np.abs(rho[s] / target[s] - 1).max() < 1e-7
     f.close()
     mix.close()
+
+
+def test_mixture_species_constraints_match_the_oracle(cdft):
+    """Fixed mass on one species and a fixed background on the other (Species::setFixedMass /
+    setFixedBackground per species of a mixture): evaluation, multiplier, and a FIRE replay with the constraints."""
+    shape, dx, ds = (16, 16, 24), 0.1, [1.0, 0.6]
+    V = np.prod(shape) * dx ** 3
+    w = np.stack([oracle.fmt_weights(shape, dx, d) for d in ds])
+    rho = np.stack([seeded(shape, 0.3, 0x5EED7051), seeded(shape, 0.4, 0x5EED7052)])
+    kw = dict(mu=[0.3, 0.5], fixed_mass=[0.0, 0.42 * V], boundary=[cdft.FIXED_BACKGROUND, 0])
+    o = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, **kw)
+    mix = cdft.Mixture(shape, dx, ds, model=cdft.WHITEBEAR_I, weights=list(w), **kw)
+    p0, dF0 = o.eval(rho)
+    p1, dF1 = mix.eval(rho)
+    for k in (0, 1, 2, 4):
+        assert close(p1[k], p0[k], TOL), (k, p1[k], p0[k])
+    for s in range(2):
+        assert maxrel(dF1[s], dF0[s]) <= TOL
+    mu0, mu1 = o.chem_potential, mix.chem_potential()
+    assert mu1[0] == 0.3 and close(mu1[1], mu0[1], TOL)
+    fo = oracle.Fire(o, rho)
+    fg = cdft.Fire(mix, rho)
+    for i in range(20):
+        rc0, a = fo.step()
+        rc1, b = fg.step()
+        assert rc0 == rc1, (i, rc0, rc1)
+        assert close(a["F"], b["F"], 1e-10), (i, a["F"], b["F"])
+        assert close(a["dt"], b["dt"], 1e-12)
+    r0, r1 = fo.state()[0], fg.state()[0]
+    assert np.abs(r0 - r1).max() <= 1e-10 * np.abs(r0).max()
+    assert abs(r1[1].sum() * dx ** 3 / (0.42 * V) - 1) <= 1e-12
+    assert close(mix.chem_potential()[1], o.chem_potential[1], 1e-9)
+    fg.close()
+    mix.close()

@@ -38,6 +38,8 @@ def test_output_array_rejects_read_only():
     {"vext": [np.zeros(10), None]},                  # a field of the wrong size
     {"weights": [None]},                             # one weight set for two species
     {"weights": [np.zeros(11 * 512 - 1), None]},     # a weight set of the wrong size
+    {"fixed_mass": [1.0]},                           # one mass for two species
+    {"boundary": [0, 0, 1]},                         # three boundary flag sets
 ])
 def test_mixture_descriptor_checks(kw):
     """cdft.Mixture validates its per-species arrays before the library is asked for anything."""

@@ -174,3 +174,41 @@ def test_one_species_mixture_fire_is_the_single_species_fire():
         assert abs(a["F"] - b["F"]) <= 1e-12 * abs(a["F"])
         assert a["dt"] == b["dt"] and a["alpha"] == b["alpha"]
     assert np.abs(f1.state()[0] - f2.state()[0][0]).max() <= 1e-12 * rho0.max()
+
+
+def test_mixture_species_constraints():
+    """Species::setFixedMass / setFixedBackground per species of a mixture (include/Species.h:23-35,
+    src/Species.cpp:64-112): the fixed-mass species is rescaled to its mass, its force projected so that
+    sum dF rho = 0 with the multiplier reported as its chemical potential; the fixed-background species has zero
+    force on the boundary planes; a one-species mixture with a fixed mass is the single-species DFT with it."""
+    shape, dx, ds = (8, 10, 12), 0.1, [1.0, 0.6]
+    V = np.prod(shape) * dx ** 3
+    rho = np.stack([seeded(shape, 0.3, 21, 0.2), seeded(shape, 0.45, 22, 0.2)])
+    w = np.stack([oracle.fmt_weights(shape, dx, d) for d in ds])
+    mass = 0.5 * V
+    m = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=[0.2, 0.7], fixed_mass=[mass, 0.0],
+                      boundary=[0, 1])
+    parts, dF = m.eval(rho)
+    mu = m.chem_potential
+    free = oracle.MixDFT(shape, dx, ds, model=oracle.WBI, weights=w, mu=[0.0, 0.7])
+    r0 = rho.copy()
+    r0[0] *= mass / (rho[0].sum() * dx ** 3)
+    p0, d0 = free.eval(r0)  # unconstrained force at the rescaled density, mu_0 = 0
+    lam = (d0[0] * r0[0]).sum() / mass
+    assert abs(mu[0] - lam) <= 1e-12 * abs(lam) and mu[1] == 0.7
+    assert np.abs(dF[0] - (d0[0] - lam * dx ** 3)).max() <= 1e-12 * np.abs(d0[0]).max()
+    assert abs((dF[0] * r0[0]).sum()) <= 1e-10 * np.abs(d0[0] * r0[0]).sum()
+    bnd = np.zeros(shape, dtype=bool)
+    bnd[0, :, :] = bnd[:, 0, :] = bnd[:, :, 0] = True
+    assert np.all(dF[1][bnd] == 0.0)
+    assert np.abs(dF[1][~bnd] - d0[1][~bnd]).max() <= 1e-12 * np.abs(d0[1]).max()
+    assert abs(parts[0] - p0[0]) <= 1e-12 * abs(p0[0])
+    # one species with a fixed mass: the single-species DFT
+    o = oracle.DFT(shape, dx, ds[0], mu=0.2)
+    o.set_constraints(fixed_mass=mass)
+    p1, d1 = o.eval(rho[0])
+    m1 = oracle.MixDFT(shape, dx, ds[:1], model=oracle.WBI, weights=w[:1], mu=[0.2], fixed_mass=[mass])
+    p2, d2 = m1.eval(rho[:1])
+    assert abs(p2[0] - p1[0]) <= 1e-12 * abs(p1[0])
+    assert np.abs(d2[0] - d1).max() <= 1e-12 * np.abs(d1).max()
+    assert abs(m1.chem_potential[0] - o.chem_potential) <= 1e-12 * abs(o.chem_potential)
