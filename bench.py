@@ -108,6 +108,27 @@ def time_ao(cdft, rho_d, dF_d, steps, Rp=0.25, rho_p=0.5):
     return {"Rp": Rp, "reservoir_density": rho_p, "ms_per_step": ms / steps, "evals_per_s": 1e3 * steps / ms}
 
 
+def time_mixture(cdft, rho_d, steps, d2=0.6, frac=0.5):
+    """A binary hard-sphere mixture on the 256^3 lattice (FMT over two FMT_Species, diameters HSD and d2): the
+    workload density split between the species, F+dF of both per evaluation, device pointers, FP64.  Wall clock
+    around `steps` evaluations (each returns after its stream has finished)."""
+    import time
+    mix = cdft.Mixture(SHAPE, DX, [HSD, d2], model=cdft.WHITEBEAR_I)
+    rho = rho_d.download()
+    r = [cdft.DeviceArray.from_host(frac * rho), cdft.DeviceArray.from_host((1 - frac) * rho)]
+    f = [cdft.DeviceArray(SHAPE), cdft.DeviceArray(SHAPE)]
+    rp, fp = [a.ptr for a in r], [a.ptr for a in f]
+    for _ in range(2):
+        mix.fmt_eval_device(rp, fp)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        F = mix.fmt_eval_device(rp, fp)
+    sec = time.perf_counter() - t0
+    mix.close()
+    return {"species": 2, "hsd": [HSD, d2], "split": [frac, 1 - frac], "ms_per_step": 1e3 * sec / steps,
+            "evals_per_s": steps / sec, "F": F, "timing": "wall clock, synchronised evaluations"}
+
+
 def slit_pore(cdft, n=128, dx=0.05, kT=0.8, rho_b=0.7, precision=0):
     """Slit pore: White Bear I + LJ (sigma = eps = 1, rc = 2.5, WCA, QF kernel) at kT, hard walls
     V_ext(z) = 10 [exp(-z/0.2) + exp(-(L-z)/0.2)], mu = mu_times_beta(rho_b), start rho_b exp(-V_ext)."""
@@ -513,6 +534,7 @@ def main():
     ap.add_argument("--ddft-steps", type=int, default=0, help="DDFT fcc-seed steps at 256^3 (0: skip)")
     ap.add_argument("--batch", action="store_true", help="batched FIRE on 96 systems of 48^3 / 64^3 / 96^3")
     ap.add_argument("--no-ao", action="store_true", help="skip the Asakura-Oosawa polymer timing")
+    ap.add_argument("--no-mix", action="store_true", help="skip the binary-mixture timing")
     ap.add_argument("--warm-systems", type=int, default=64,
                     help="held-out 64^3 systems for the learned warm start vs uniform start (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -548,6 +570,12 @@ def main():
         print("# AO polymer timing", flush=True)
         ao = time_ao(cdft, rho_d, dF_d, max(3, args.steps // 5))
         ao["vs_fp64_plain"] = ao["ms_per_step"] / t64
+
+    mixture = None
+    if not args.no_mix:
+        print("# binary mixture timing", flush=True)
+        mixture = time_mixture(cdft, rho_d, max(3, args.steps // 5))
+        mixture["vs_fp64_one_species"] = mixture["ms_per_step"] / t64
 
     m = 11
     B64 = model_bytes(8, m)
@@ -606,6 +634,7 @@ def main():
         "stages_ms_fp64": st64,
         "stages_ms_mixed": stmx,
         "ao_polymer_fp64": ao,
+        "mixture_fp64": mixture,
         "F_fp64": F64,
     }
     if args.fire_grid:
