@@ -1210,6 +1210,10 @@ struct cdft_mixture_s {
   std::vector<double> hsd;
   int nm = 0;                   // measure arrays (16 with the tensor, 10 without)
   DevBuf Mx, partials, flag, results;
+  // per species: the nm forward then the nm backward measure weight spectra (mix_fwd_coef / mix_bwd_coef times the
+  // species' weight spectra, in its weight layout), wlen values each
+  std::vector<std::unique_ptr<DevBuf>> wm;
+  std::vector<long> wlen;
   ~cdft_mixture_s() {  // the later members run on the first member's stream, destroyed with it
     for (size_t i = 1; i < sp.size(); ++i) delete sp[i];
     if (!sp.empty()) delete sp[0];
@@ -1258,6 +1262,23 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
   REQUIRE(rows_reg_ok<double>(s0) && reg_fft_ok<double>(s0, s0.g.Nx) && reg_fft_ok<double>(s0, s0.g.Ny),
           "mixtures need Nx, Ny and Nz/2 in {4,6,8,12,16,24,32,48,64,96,128,256}");
   x->nm = s0.m == 11 ? kMixMeas : 10;
+  for (int i = 0; i < d->nspecies; ++i) {
+    cdft_system_s& s = *x->sp[i];
+    const long len = s.wq ? (long)s.wq_stride : s.Nkp;
+    auto w = std::make_unique<DevBuf>();
+    w->alloc(sizeof(double) * 2 * x->nm * len);
+    for (int b = 0; b < x->nm; ++b) {
+      const double* src = s.W.as<double>() + mix_src(b) * len;
+      hipLaunchKernelGGL(k_mix_scale, dim3(std::min(div_up(len, 256), 4096)), dim3(256), 0, s0.stream, src,
+                         w->as<double>() + b * len, len, mix_fwd_coef(b, d->hsd[i]));
+      hipLaunchKernelGGL(k_mix_scale, dim3(std::min(div_up(len, 256), 4096)), dim3(256), 0, s0.stream, src,
+                         w->as<double>() + (x->nm + b) * len, len, mix_bwd_coef(b, d->hsd[i]));
+    }
+    HIPCHK(hipGetLastError());
+    x->wm.push_back(std::move(w));
+    x->wlen.push_back(len);
+  }
+  HIPCHK(hipStreamSynchronize(s0.stream));
   x->Mx.alloc(sizeof(cplx<double>) * s0.Nkp * x->nm);
   x->partials.alloc(sizeof(double2) * s0.pslots);
   x->flag.alloc(sizeof(int));
@@ -1269,6 +1290,60 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
 // The mixture on device data: dF_s written, or added to with acc; parts (may be null) = {F, F_id, F_hs, 0, F_ext}
 // summed over species.  with_dft adds each species' ideal gas (unless no_ideal), external field and -mu N
 // (src/DFT.cpp:377-425) to the hard-sphere term.
+// The x-pass geometry of species i's measure weight spectra (its weight layout, measure parities).
+ColGeo mix_geo(const cdft_mixture_s& x, int i) {
+  const cdft_system_s& s = *x.sp[i];
+  ColGeo cg = col_geo<double>(s.g, 0);
+  if (s.wq) {
+    cg.wq = 1;
+    cg.wstride = (unsigned)x.wlen[i];
+    cg.pxm = kMixOddX;
+  }
+  return cg;
+}
+
+// Species i's share of the measures: its density spectrum (A, after the z and y passes) times the forward measure
+// weight spectra, inverse x, written to Mx (first species) or added to it.
+void mix_conv_fwd(cdft_mixture_s& x, int i, const cplx<double>* A, cplx<double>* Mx) {
+  cdft_system_s& s = *x.sp[i];
+  const Tables<double>& tw = tables<double>(s);
+  const ColGeo cg = mix_geo(x, i);
+  const double* W = x.wm[i]->as<double>();
+  with_reg_len(s.g.Nx, [&](auto lc) {
+    constexpr int Lc = decltype(lc)::value;
+    with_nb<double, Lc>(s.nb_half_fwd, [&](auto nbc) {
+      constexpr int NB = decltype(nbc)::value;
+      const Batch bt{s.Nkp, x.nm * s.Nkp, s.N, 0, nullptr};
+      if (i == 0)
+        hipLaunchKernelGGL((k_conv_fwd2<double, Lc, NB, false>), dim3(div_up(cg.ncols, NB), 1), dim3(NB * F2<Lc>::P),
+                           0, s.stream, A, Mx, s.Nkp, W, x.nm, kMixOdd, cg, tw.x.template as<cplx<double>>(), bt, 0);
+      else
+        hipLaunchKernelGGL((k_conv_fwd2<double, Lc, NB, true>), dim3(div_up(cg.ncols, NB), 1), dim3(NB * F2<Lc>::P),
+                           0, s.stream, A, Mx, s.Nkp, W, x.nm, kMixOdd, cg, tw.x.template as<cplx<double>>(), bt, 0);
+      HIPCHK(hipGetLastError());
+    });
+  });
+}
+
+// Species i's force spectrum (x real): the sum over measures of the derivative spectra dM times its backward
+// measure weight spectra, into Acc.
+void mix_conv_bwd(cdft_mixture_s& x, int i, const cplx<double>* dM, cplx<double>* Acc) {
+  cdft_system_s& s = *x.sp[i];
+  const Tables<double>& tw = tables<double>(s);
+  const ColGeo cg = mix_geo(x, i);
+  const double* W = x.wm[i]->as<double>() + (long)x.nm * x.wlen[i];
+  with_reg_len(s.g.Nx, [&](auto lc) {
+    constexpr int Lc = decltype(lc)::value;
+    with_nb<double, Lc>(s.nb_half_bwd, [&](auto nbc) {
+      constexpr int NB = decltype(nbc)::value;
+      const Batch bt{s.Nkp, x.nm * s.Nkp, s.N, 0, nullptr};
+      hipLaunchKernelGGL((k_conv_bwd2<double, Lc, NB>), dim3(div_up(cg.ncols, NB), 1), dim3(NB * F2<Lc>::P), 0,
+                         s.stream, dM, Acc, s.Nkp, W, x.nm, kMixOdd, cg, tw.x.template as<cplx<double>>(), bt);
+      HIPCHK(hipGetLastError());
+    });
+  });
+}
+
 int mixture_species(const cdft_mixture_s& x) { return (int)x.sp.size(); }
 cdft_system mixture_member(const cdft_mixture_s& x, int i) { return x.sp[i]; }
 
@@ -1276,20 +1351,20 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
                  bool with_dft, bool no_ideal) {
   cdft_system_s& s0 = *x.sp[0];
   const long Nkp = s0.Nkp;
-  const int m = s0.m, S = (int)x.sp.size();
-  const int tensor = m == 11;
+  const int S = (int)x.sp.size();
   cplx<double>* Mx = x.Mx.as<cplx<double>>();
-  const dim3 eg(std::min(div_up(Nkp, 256), 8192)), eb(256);
-  // 1. each species' weighted densities (z spectral, x and y back in real space), summed into the measures; with
-  // the DFT terms a fixed-mass species is first rescaled to its mass (Species::beginForceCalculation)
+  // 1. each species' density spectrum (z and y passes) times its forward measure weights, inverse x, summed into the
+  // measures, which then take one inverse y pass; with the DFT terms a fixed-mass species is first rescaled to its
+  // mass (Species::beginForceCalculation)
   for (int i = 0; i < S; ++i) {
     cdft_system_s& s = *x.sp[i];
     if (with_dft) constraints_pre(s, const_cast<double*>(rho[i]));
-    forward_weighted(s, rho[i], s.A.as<cplx<double>>(), s.B.as<cplx<double>>(), m);
-    hipLaunchKernelGGL(k_mix_accum, eg, eb, 0, s0.stream, s.B.as<cplx<double>>(), Mx, Nkp, Nkp, x.hsd[i], tensor,
-                       (int)(i == 0));
-    HIPCHK(hipGetLastError());
+    cplx<double>* A = s.A.as<cplx<double>>();
+    launch_rows_r2c<double, double>(s, rho[i], s.N, A, Nkp, 1);
+    launch_cols<double>(s, A, 0, 1, 1, true, 1, Nkp);
+    mix_conv_fwd(x, i, A, Mx);  // the measures summed over species in the (x, ky, kz) domain
   }
+  launch_cols<double>(s0, Mx, Nkp, x.nm, 1, false, 1, 0);
   // 2. Phi and its measure derivatives per site (z inverse and forward in the kernel)
   long nblk = 0;
   switch (s0.d.model) {
@@ -1302,7 +1377,8 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
   hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s0.stream, x.partials.as<double2>(), nblk, 1,
                      x.results.as<double>(), 0L, 8L);
   HIPCHK(hipGetLastError());
-  // 3. y forward of the measure derivatives, then per species: collapse, x back-convolution, y and z inverse
+  // 3. y forward of the measure derivatives, then per species: x back-convolution with its backward measure weights
+  // (the collapse onto its arrays folded in), y and z inverse
   launch_cols<double>(s0, Mx, Nkp, x.nm, 1, true, 1, 0);
   const double dV = s0.d.dx * s0.d.dy * s0.d.dz;
   for (int i = 0; i < S; ++i) {
@@ -1315,10 +1391,7 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
     fp.ideal = with_dft && s.d.ideal && !no_ideal;
     fp.ext = with_dft && s.has_vext;
     cplx<double>* A = s.A.as<cplx<double>>();
-    cplx<double>* B = s.B.as<cplx<double>>();
-    hipLaunchKernelGGL(k_mix_collapse, eg, eb, 0, s0.stream, Mx, B, Nkp, Nkp, x.hsd[i], tensor);
-    HIPCHK(hipGetLastError());
-    conv_bwd_f64(s, B, A, s.W.as<double>(), m);
+    mix_conv_bwd(x, i, Mx, A);  // the collapse onto this species folded into its measure weights
     launch_cols<double>(s, A, 0, 1, 1, false, 1, Nkp);
     const Tables<double>& tw = tables<double>(s);
     with_reg_len(s.g.Nz / 2, [&](auto mc) {

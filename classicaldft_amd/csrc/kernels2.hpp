@@ -132,7 +132,8 @@ __global__ __launch_bounds__(256) void k_cols2(cplx<T>* __restrict__ data, long 
 // x pass of rho_k, then per weight a: n_a(k) = rho_k w_a(k) and its inverse x
 // pass, straight from registers.  grid = (ceil(ncols/NB), members), block = NB * P.
 // zero_mean: rho_k(0) is dropped (the mean of every n_a is added back in FP64 by the pointwise kernel).
-template <typename T, int L, int NB>
+// ACC: the products are added to B instead of written (a mixture's measures summed over its species).
+template <typename T, int L, int NB, bool ACC = false>
 __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A, cplx<T>* __restrict__ B,
                                                    long arr_stride, const T* __restrict__ W, int nconv,
                                                    unsigned odd_mask, ColGeo cg, const cplx<T>* __restrict__ tw,
@@ -203,7 +204,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd2(const cplx<T>* __restrict__ A
       for (int q = 0; q < Q1; ++q) {
         const unsigned vo = (c + (t + P * q) * cg.es) * E;
 #pragma unroll
-        for (int k2 = 0; k2 < R1; ++k2) buf_st(rb, vo, R2 * k2 * cg.es * E, u2[q][k2]);
+        for (int k2 = 0; k2 < R1; ++k2) {
+          cplx<T> o = u2[q][k2];
+          if (ACC) {
+            const cplx<T> prev = buf_ld<cplx<T>>(rb, vo, R2 * k2 * cg.es * E);
+            o.x += prev.x;
+            o.y += prev.y;
+          }
+          buf_st(rb, vo, R2 * k2 * cg.es * E, o);
+        }
       }
     }
   }

@@ -9,9 +9,10 @@
 //
 // Unlike one species, the measures s0, s1, s2 and v1, v2 are no longer multiples of one array, so the mixture
 // carries them separately: kMixMeas = 16 measure arrays for the tensor models (eta, s0, s1, s2, v1[3], v2[3] and
-// the six independent tensor entries xx, xy, xz, yy, yz, zz), 10 without the tensor.  Each species' weighted
-// densities come from its own forward convolution (x and y transformed back, z spectral); because the measure
-// sums and the collapse are linear with real coefficients they are formed element by element in that space.
+// the six independent tensor entries xx, xy, xz, yy, yz, zz), 10 without the tensor.  Because the measure sums and
+// the collapse are linear with real coefficients, they are folded into per-species measure weight spectra: each
+// species' x-convolution writes (or adds to) the measures directly, and each species' backward x-convolution reads
+// the measure derivatives directly.
 #pragma once
 #include "rows2.hpp"
 
@@ -19,56 +20,39 @@ namespace cdft {
 
 constexpr int kMixMeas = 16;
 
-// Mx (nm arrays, astride apart) = (first) or += the measure contributions of one species' weighted-density arrays
-// Bs (11 or 5 arrays, astride apart; FMT_Species index order, include/Species.h:351-360), for elements [0, n).
-__global__ __launch_bounds__(256) void k_mix_accum(const cplx<double>* __restrict__ Bs, cplx<double>* __restrict__ Mx,
-                                                   long astride, long n, double d, int tensor, int first) {
-  const long step = (long)gridDim.x * blockDim.x;
-  const double id = 1.0 / d, id2 = id * id;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-    auto put = [&](int a, double sc, cplx<double> v) {
-      cplx<double> o = first ? cplx<double>{0.0, 0.0} : Mx[a * astride + i];
-      o.x += sc * v.x;
-      o.y += sc * v.y;
-      Mx[a * astride + i] = o;
-    };
-    put(0, 1.0, Bs[i]);  // eta
-    const cplx<double> s = Bs[astride + i];
-    put(1, id2, s);  // s0 = A1 / d^2
-    put(2, id, s);   // s1 = A1 / d
-    put(3, 1.0, s);  // s2 = A1
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const cplx<double> v = Bs[(2 + j) * astride + i];
-      put(4 + j, id, v);   // v1 = A_v / d
-      put(7 + j, 1.0, v);  // v2 = A_v
-    }
-    if (tensor)
-#pragma unroll
-      for (int k = 0; k < 6; ++k) put(10 + k, 1.0, Bs[(5 + k) * astride + i]);
+// The measures as linear maps of one species' weight spectra (FMT::getWeightedDensities, src/FMT.cpp:78-111, and the
+// collapse of FMT_Species::set_fundamental_measure_derivatives, include/Species.h:310-329): measure b is
+// fwd_coef(b) times the weighted density of array mix_src(b), and contributes bwd_coef(b) times its derivative to
+// the force through that array's weight.  Both maps are real and per site, so they commute with the transforms: a
+// species' measures are its density spectrum times the weight spectra scaled by fwd_coef (one x-convolution per
+// species, summed over species), and its force is the sum over measures of the derivative spectra times the weight
+// spectra scaled by bwd_coef (one backward x-convolution per species).
+__host__ __device__ constexpr int mix_src(int b) {
+  return b == 0 ? 0 : b < 4 ? 1 : b < 7 ? b - 2 : b < 10 ? b - 5 : b - 5;
+}
+inline double mix_fwd_coef(int b, double d) {
+  const double id = 1.0 / d;
+  switch (b) {
+    case 1: return id * id;                // s0 = A1 / d^2
+    case 2: case 4: case 5: case 6: return id;  // s1 = A1 / d, v1 = A_v / d
+    default: return 1.0;                   // eta, s2, v2, T
   }
 }
+inline double mix_bwd_coef(int b, double d) {
+  const double id = 1.0 / d;
+  if (b >= 4 && b < 7) return -id;   // -(v1 / d): the parity sign of the odd vector weights
+  if (b >= 7 && b < 10) return -1.0;  // -v2
+  return mix_fwd_coef(b, d);
+}
+// Measures whose weight is purely imaginary (the vectors) and those odd in x (v1_x, v2_x, T_xy, T_xz).
+constexpr unsigned kMixOdd = 0x3F0u;
+constexpr unsigned kMixOddX = (1u << 4) | (1u << 7) | (1u << 11) | (1u << 12);
 
-// Bs (the species' 11 or 5 arrays) = the measure derivatives dM (nm arrays) collapsed with the species' diameter d
-// (include/Species.h:310-329): eta; s0/d^2 + s1/d + s2; -(v2 + v1/d) (the parity sign of the odd vector weights);
-// the tensor entries as stored by the pointwise kernel (off-diagonals already counted twice).
-__global__ __launch_bounds__(256) void k_mix_collapse(const cplx<double>* __restrict__ dM, cplx<double>* __restrict__ Bs,
-                                                      long astride, long n, double d, int tensor) {
-  const long step = (long)gridDim.x * blockDim.x;
-  const double id = 1.0 / d, id2 = id * id;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
-    Bs[i] = dM[i];
-    const cplx<double> a = dM[astride + i], b = dM[2 * astride + i], c = dM[3 * astride + i];
-    Bs[astride + i] = {a.x * id2 + b.x * id + c.x, a.y * id2 + b.y * id + c.y};
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const cplx<double> v1 = dM[(4 + j) * astride + i], v2 = dM[(7 + j) * astride + i];
-      Bs[(2 + j) * astride + i] = {-(v2.x + v1.x * id), -(v2.y + v1.y * id)};
-    }
-    if (tensor)
-#pragma unroll
-      for (int k = 0; k < 6; ++k) Bs[(5 + k) * astride + i] = dM[(10 + k) * astride + i];
-  }
+// dst[i] = c * src[i], i < n.
+__global__ __launch_bounds__(256) void k_mix_scale(const double* __restrict__ src, double* __restrict__ dst, long n,
+                                                   double c) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = c * src[i];
 }
 
 // Pointwise z pass of the mixture (FP64): inverse z of the nm measure rows, Phi and its derivatives with respect to
