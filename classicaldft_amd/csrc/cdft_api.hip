@@ -1183,9 +1183,11 @@ int standalone_fft(int Nx, int Ny, int Nz, int precision, const double* in, doub
 // The FMT of several hard-sphere species (mixture.hpp).  Each species is an FP64 FMT-only member system holding its
 // weight spectra and transform buffers; all members share the first member's stream.
 template <int MODEL>
-long launch_pw_mix(cdft_system_s& s, cplx<double>* Mx, int nm, double2* partials, int* flag) {
+long launch_pw_mix(cdft_system_s& s, cplx<double>* Mx, int nm, double2* partials, int* flag, int x0, int x1) {
   const Tables<double>& tw = tables<double>(s);
   const PhiParams pp = make_phi_params(s.d.hsd, s.d.A, s.d.B, MODEL == M_WBI || MODEL == M_WBII);
+  const long nrows = (long)(x1 - x0) * s.g.Ny;  // the rows of the x-planes [x0, x1)
+  cplx<double>* base = Mx + (long)x0 * s.g.Ny * s.g.Zp;
   long nblk = -1;
   with_reg_len(s.g.Nz / 2, [&](auto mc) {
     constexpr int Mc = decltype(mc)::value;
@@ -1193,9 +1195,9 @@ long launch_pw_mix(cdft_system_s& s, cplx<double>* Mx, int nm, double2* partials
     const int per = nm * RF::SLOT * int(sizeof(cplx<double>));
     const int RB = std::min(kThreads / (nm * RF::P), (65536 - 4096) / per);
     if (RB < 1) return;
-    nblk = div_up(s.nrows, RB);
-    hipLaunchKernelGGL((k_z_pw_mix<MODEL, Mc>), dim3(nblk), dim3(nm * RB * RF::P), size_t(RB) * per, s.stream, Mx,
-                       s.Nkp, nm, s.nrows, s.g.Zp, tw.mp.template as<cplx<double>>(),
+    nblk = div_up(nrows, RB);
+    hipLaunchKernelGGL((k_z_pw_mix<MODEL, Mc>), dim3(nblk), dim3(nm * RB * RF::P), size_t(RB) * per, s.stream, base,
+                       s.Nkp, nm, nrows, s.g.Zp, tw.mp.template as<cplx<double>>(),
                        tw.n.template as<cplx<double>>(), RB, pp, partials, flag);
     HIPCHK(hipGetLastError());
   });
@@ -1364,22 +1366,37 @@ int mixture_eval(cdft_mixture_s& x, const double* const* rho, double* const* dF,
     launch_cols<double>(s, A, 0, 1, 1, true, 1, Nkp);
     mix_conv_fwd(x, i, A, Mx);  // the measures summed over species in the (x, ky, kz) domain
   }
-  launch_cols<double>(s0, Mx, Nkp, x.nm, 1, false, 1, 0);
-  // 2. Phi and its measure derivatives per site (z inverse and forward in the kernel)
+  // 2. inverse y of the measures, Phi and its measure derivatives per site (z inverse and forward in the kernel),
+  // forward y of the derivatives: x-slab by x-slab when the slab of all nm arrays fits the cache budget, so that a
+  // slab stays in the last-level cache between the three kernels
+  const int Nx = s0.g.Nx;
+  const long per_plane = (long)x.nm * s0.g.Ny * s0.g.Zp * (long)sizeof(cplx<double>);
+  const long fit = s0.slab_bytes > 0 ? s0.slab_bytes / per_plane : 0;
+  const int planes = fit >= 1 && fit < Nx ? (int)fit : Nx;
+  auto pw = [&](int x0, int x1, double2* part) -> long {
+    switch (s0.d.model) {
+      case CDFT_ROSENFELD: return launch_pw_mix<M_ROSENFELD>(s0, Mx, x.nm, part, x.flag.as<int>(), x0, x1);
+      case CDFT_RSLT: return launch_pw_mix<M_RSLT>(s0, Mx, x.nm, part, x.flag.as<int>(), x0, x1);
+      case CDFT_ESFMT: return launch_pw_mix<M_ESFMT>(s0, Mx, x.nm, part, x.flag.as<int>(), x0, x1);
+      case CDFT_WHITEBEAR_I: return launch_pw_mix<M_WBI>(s0, Mx, x.nm, part, x.flag.as<int>(), x0, x1);
+      default: return launch_pw_mix<M_WBII>(s0, Mx, x.nm, part, x.flag.as<int>(), x0, x1);
+    }
+  };
   long nblk = 0;
-  switch (s0.d.model) {
-    case CDFT_ROSENFELD: nblk = launch_pw_mix<M_ROSENFELD>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
-    case CDFT_RSLT: nblk = launch_pw_mix<M_RSLT>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
-    case CDFT_ESFMT: nblk = launch_pw_mix<M_ESFMT>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
-    case CDFT_WHITEBEAR_I: nblk = launch_pw_mix<M_WBI>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
-    default: nblk = launch_pw_mix<M_WBII>(s0, Mx, x.nm, x.partials.as<double2>(), x.flag.as<int>()); break;
+  for (int x0 = 0; x0 < Nx; x0 += planes) {
+    const int x1 = std::min(Nx, x0 + planes);
+    if (planes < Nx) launch_cols_slab<double>(s0, Mx, Nkp, x.nm, false, 1, 0, x0, x1);
+    else launch_cols<double>(s0, Mx, Nkp, x.nm, 1, false, 1, 0);
+    nblk += pw(x0, x1, x.partials.as<double2>() + nblk);
+    if (planes < Nx) launch_cols_slab<double>(s0, Mx, Nkp, x.nm, true, 1, 0, x0, x1);
+    else launch_cols<double>(s0, Mx, Nkp, x.nm, 1, true, 1, 0);
   }
+  REQUIRE(nblk <= s0.pslots, "partial-sum slots of the mixture's pointwise pass");
   hipLaunchKernelGGL(k_reduce_partials_wide, dim3(1), dim3(1024), 0, s0.stream, x.partials.as<double2>(), nblk, 1,
                      x.results.as<double>(), 0L, 8L);
   HIPCHK(hipGetLastError());
-  // 3. y forward of the measure derivatives, then per species: x back-convolution with its backward measure weights
-  // (the collapse onto its arrays folded in), y and z inverse
-  launch_cols<double>(s0, Mx, Nkp, x.nm, 1, true, 1, 0);
+  // 3. per species: x back-convolution of the measure derivatives with its backward measure weights (the collapse
+  // onto its arrays folded in), y and z inverse
   const double dV = s0.d.dx * s0.d.dy * s0.d.dz;
   for (int i = 0; i < S; ++i) {
     cdft_system_s& s = *x.sp[i];

@@ -215,3 +215,24 @@ def test_mixture_species_constraints_match_the_oracle(cdft):
     assert close(mix.chem_potential()[1], o.chem_potential[1], 1e-9)
     fg.close()
     mix.close()
+
+
+def test_mixture_x_slab_section_matches(cdft, monkeypatch):
+    """The measures' y / z / y section run x-slab by x-slab (a 1 MB cache budget forces 7 slabs at this size) gives
+    the forces of the whole-array section bit for bit, F to rounding, and the oracle's mixture to 1e-11."""
+    shape, dx, ds = (32, 24, 48), 0.05, [1.0, 0.6, 0.8]
+    rho = np.stack([seeded(shape, 0.25, 0x5EED7061), seeded(shape, 0.4, 0x5EED7062), seeded(shape, 0.2, 0x5EED7063)])
+    whole = cdft.Mixture(shape, dx, ds, model=cdft.WHITEBEAR_II)
+    F0, dF0 = whole.fmt_eval(rho)
+    whole.close()
+    monkeypatch.setenv("CDFT_SLAB_MB", "1")
+    slabs = cdft.Mixture(shape, dx, ds, model=cdft.WHITEBEAR_II)
+    F1, dF1 = slabs.fmt_eval(rho)
+    slabs.close()
+    assert np.array_equal(dF1, dF0)
+    assert abs(F1 - F0) <= 1e-14 * abs(F0)
+    w = np.stack([oracle.fmt_weights(shape, dx, d, mode=oracle.WEIGHTS_ACCURATE) for d in ds])
+    F2, dF2 = oracle.fmt_eval_mix(oracle.WBII, rho, dx, ds, weights=w)
+    assert rel(F1, F2) <= TOL
+    for s in range(3):
+        assert maxrel(dF1[s], dF2[s]) <= TOL
