@@ -1250,7 +1250,7 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
     sd.fixed_mass = d->fixed_mass ? d->fixed_mass[i] : 0.0;
     sd.boundary = d->boundary ? d->boundary[i] : 0;
     cdft_system s = nullptr;
-    create_system(&sd, &s, 1, nullptr, nullptr, false);
+    create_system(&sd, &s, 1, nullptr, nullptr, true);
     x->sp.push_back(s);
     x->hsd.push_back(d->hsd[i]);
     if (i > 0) {  // one stream for the whole mixture
@@ -1263,18 +1263,30 @@ void create_mixture(const cdft_mixture_desc* d, cdft_mixture* out) {
   cdft_system_s& s0 = *x->sp[0];
   REQUIRE(rows_reg_ok<double>(s0) && reg_fft_ok<double>(s0, s0.g.Nx) && reg_fft_ok<double>(s0, s0.g.Ny),
           "mixtures need Nx, Ny and Nz/2 in {4,6,8,12,16,24,32,48,64,96,128,256}");
-  x->nm = s0.m == 11 ? kMixMeas : 10;
+  // tensor models: 16 measures, or 15 when every species' weights satisfy the trace identity (stored without T_zz)
+  bool all_trace = true;
+  for (cdft_system m : x->sp) all_trace = all_trace && m->trace;
+  x->nm = s0.m >= 10 ? (all_trace ? kMixMeas - 1 : kMixMeas) : 10;
   for (int i = 0; i < d->nspecies; ++i) {
     cdft_system_s& s = *x->sp[i];
     const long len = s.wq ? (long)s.wq_stride : s.Nkp;
     auto w = std::make_unique<DevBuf>();
     w->alloc(sizeof(double) * 2 * x->nm * len);
+    const dim3 g(std::min(div_up(len, 256), 4096));
+    const double* W = s.W.as<double>();
     for (int b = 0; b < x->nm; ++b) {
-      const double* src = s.W.as<double>() + mix_src(b) * len;
-      hipLaunchKernelGGL(k_mix_scale, dim3(std::min(div_up(len, 256), 4096)), dim3(256), 0, s0.stream, src,
-                         w->as<double>() + b * len, len, mix_fwd_coef(b, d->hsd[i]));
-      hipLaunchKernelGGL(k_mix_scale, dim3(std::min(div_up(len, 256), 4096)), dim3(256), 0, s0.stream, src,
-                         w->as<double>() + (x->nm + b) * len, len, mix_bwd_coef(b, d->hsd[i]));
+      double* fw = w->as<double>() + b * len;
+      double* bw = w->as<double>() + (x->nm + b) * len;
+      if (b == 15 && s.trace) {  // T_zz measure of a species stored without its T_zz weight
+        hipLaunchKernelGGL(k_mix_tzz, g, dim3(256), 0, s0.stream, W + len, W + 5 * len, W + 8 * len, fw, len,
+                           mix_fwd_coef(b, d->hsd[i]));
+        hipLaunchKernelGGL(k_mix_tzz, g, dim3(256), 0, s0.stream, W + len, W + 5 * len, W + 8 * len, bw, len,
+                           mix_bwd_coef(b, d->hsd[i]));
+        continue;
+      }
+      const double* src = W + mix_src(b) * len;
+      hipLaunchKernelGGL(k_mix_scale, g, dim3(256), 0, s0.stream, src, fw, len, mix_fwd_coef(b, d->hsd[i]));
+      hipLaunchKernelGGL(k_mix_scale, g, dim3(256), 0, s0.stream, src, bw, len, mix_bwd_coef(b, d->hsd[i]));
     }
     HIPCHK(hipGetLastError());
     x->wm.push_back(std::move(w));

@@ -48,6 +48,15 @@ inline double mix_bwd_coef(int b, double d) {
 constexpr unsigned kMixOdd = 0x3F0u;
 constexpr unsigned kMixOddX = (1u << 4) | (1u << 7) | (1u << 11) | (1u << 12);
 
+// dst[i] = c * (s[i] - xx[i] - yy[i]), i < n: the T_zz weight of a species stored without it (the trace identity
+// w_Txx + w_Tyy + w_Tzz = w_s), for a mixture that carries the T_zz measure.
+__global__ __launch_bounds__(256) void k_mix_tzz(const double* __restrict__ ws, const double* __restrict__ xx,
+                                                 const double* __restrict__ yy, double* __restrict__ dst, long n,
+                                                 double c) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = c * (ws[i] - xx[i] - yy[i]);
+}
+
 // dst[i] = c * src[i], i < n.
 __global__ __launch_bounds__(256) void k_mix_scale(const double* __restrict__ src, double* __restrict__ dst, long n,
                                                    double c) {
@@ -58,7 +67,8 @@ __global__ __launch_bounds__(256) void k_mix_scale(const double* __restrict__ sr
 // Pointwise z pass of the mixture (FP64): inverse z of the nm measure rows, Phi and its derivatives with respect to
 // the measures per site (FMT::calculate_Phi / calculate_dPhi_wrt_fundamental_measures, src/FMT.cpp:114-176), the
 // derivative rows transformed back in place; tensor off-diagonal derivatives are stored twice (the T_jk and T_kj
-// terms, include/Species.h:327).  Sequence a * RB + r is measure a of row r; grid = ceil(nrows / RB),
+// terms, include/Species.h:327).  nm = 15 (every species' weights satisfy the trace identity): the T_zz measure is
+// s2 - T_xx - T_yy per site and its derivative is folded back onto those three.  Sequence a * RB + r is measure a of row r; grid = ceil(nrows / RB),
 // block = nm RB P, dynamic LDS = nm RB SLOT elements.  F partial sums at partials[block]; flag set when
 // eta > 1 somewhere (src/FMT.cpp:289-290).
 template <int MODEL, int M>
@@ -96,11 +106,17 @@ __global__ __launch_bounds__(256) void k_z_pw_mix(cplx<double>* __restrict__ Mx,
 #pragma unroll
     for (int b = 0; b < kMixMeas; ++b)
       n[b] = b < nm ? reinterpret_cast<const double*>(lds + (b * RB + rr) * RF::SLOT)[z] : 0.0;
+    if (nm == 15) n[15] = n[3] - n[10] - n[13];
     const double v1[3] = {n[4], n[5], n[6]}, v2[3] = {n[7], n[8], n[9]};
     PhiDeriv<double> d;
     const double phi = phi_meas<MODEL>(n[0], n[1], n[2], n[3], v1, v2, n + 10, pp, d);
     double out[kMixMeas] = {d.eta, d.s0, d.s1, d.s2, d.v1[0], d.v1[1], d.v1[2], d.v2[0], d.v2[1], d.v2[2],
                             d.T[0], 2 * d.T[1], 2 * d.T[2], d.T[3], 2 * d.T[4], d.T[5]};
+    if (nm == 15) {
+      out[3] += out[15];
+      out[10] -= out[15];
+      out[13] -= out[15];
+    }
     if (n[0] > 0.5 && 1 - n[0] < 0.0) bad = 1;
     F.add(phi);
 #pragma unroll

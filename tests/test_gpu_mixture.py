@@ -236,3 +236,20 @@ def test_mixture_x_slab_section_matches(cdft, monkeypatch):
     assert rel(F1, F2) <= TOL
     for s in range(3):
         assert maxrel(dF1[s], dF2[s]) <= TOL
+
+
+def test_mixture_with_and_without_the_trace_identity(cdft):
+    """Weights that satisfy w_Txx + w_Tyy + w_Tzz = w_s (device-generated) for one species and weights that do
+    not (the reference generator's, given by the caller) for the other: the mixture carries the T_zz measure and
+    builds the first species' T_zz weight from its other weights; 1e-11 against the oracle."""
+    shape, dx, ds = (24, 16, 32), 0.1, [1.0, 0.6]
+    wf = oracle.fmt_weights(shape, dx, ds[1])  # the reference generator's rounding
+    wa = oracle.fmt_weights(shape, dx, ds[0], mode=oracle.WEIGHTS_ACCURATE)
+    rho = np.stack([seeded(shape, 0.3, 0x5EED7071), seeded(shape, 0.45, 0x5EED7072)])
+    F0, dF0 = oracle.fmt_eval_mix(oracle.ESFMT, rho, dx, ds, weights=np.stack([wa, wf]), A=1.3, B=-0.4)
+    mix = cdft.Mixture(shape, dx, ds, model=cdft.ESFMT, weights=[None, wf], A=1.3, B=-0.4)
+    F1, dF1 = mix.fmt_eval(rho)
+    mix.close()
+    assert rel(F1, F0) <= TOL
+    for s in range(2):
+        assert maxrel(dF1[s], dF0[s]) <= TOL
