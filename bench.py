@@ -535,6 +535,8 @@ def main():
     ap.add_argument("--batch", action="store_true", help="batched FIRE on 96 systems of 48^3 / 64^3 / 96^3")
     ap.add_argument("--no-ao", action="store_true", help="skip the Asakura-Oosawa polymer timing")
     ap.add_argument("--no-mix", action="store_true", help="skip the binary-mixture timing")
+    ap.add_argument("--mixture-only", type=int, default=0,
+                    help="time only this many binary-mixture evaluations and print their JSON (for profiling)")
     ap.add_argument("--warm-systems", type=int, default=64,
                     help="held-out 64^3 systems for the learned warm start vs uniform start (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -560,6 +562,9 @@ def main():
     rho = workload_density()
     rho_d = cdft.DeviceArray.from_host(rho)
     dF_d = cdft.DeviceArray(SHAPE)
+    if args.mixture_only:
+        print(json.dumps({"mixture_fp64": time_mixture(cdft, rho_d, args.mixture_only)}), flush=True)
+        return
     print("# fp64 timing", flush=True)
     t64, F64, dF64, st64, lay64 = time_system(cdft, cdft.FP64, rho_d, dF_d, args.steps, args.warmup)
     print("# mixed timing", flush=True)
